@@ -1,0 +1,168 @@
+/*
+ * zog_hip.h -- C-ABI of the MI355X (gfx950) backend for Zog's `:soa`
+ * ArrayGraph resource: unweighted Brandes betweenness and PageRank.
+ *
+ * The entry points are what the Zigler NIF bodies in lib/zog/resource_graph.ex
+ * call in place of the Zig CPU code.  Plain pointers and sizes only; the
+ * caller owns every host buffer.  Status codes are 0 on success and negative
+ * on error (zg_strerror names them).
+ *
+ * Threading: every call on one handle takes that handle's lock, so concurrent
+ * BEAM dirty-scheduler calls on one resource are safe (they serialise).  Calls
+ * on different handles run concurrently, each on its handle's own HIP stream.
+ */
+#ifndef ZOG_HIP_H
+#define ZOG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define ZG_OK 0
+#define ZG_ERR_INVALID_ARG (-1)
+#define ZG_ERR_NODE_NOT_FOUND (-2)   /* ArrayGraph.Error.NodeNotFound, array_graph.zig:77-80 */
+#define ZG_ERR_OOM (-3)              /* host allocation failed (OutOfMemory) */
+#define ZG_ERR_GPU_OOM (-4)          /* hipMalloc failed */
+#define ZG_ERR_HIP (-5)              /* any other HIP runtime error */
+#define ZG_ERR_NO_DEVICE (-6)        /* no usable gfx950 device */
+#define ZG_ERR_TOO_LARGE (-7)        /* more than 2^32-1 arcs or nodes */
+#define ZG_ERR_INVALID_HANDLE (-8)   /* NULL or destroyed handle */
+
+typedef struct zg_graph zg_graph;
+
+/* Per-run information for Brandes (the reference returns none). */
+typedef struct {
+    uint64_t n_sources;       /* sources processed */
+    uint64_t arcs_traversed;  /* sum over sources of arcs scanned by the forward
+                                 sweep (arcs whose tail is reachable) */
+    uint64_t nodes_reached;   /* sum over sources of reached vertices */
+    uint32_t max_levels;      /* deepest BFS over all sources, in levels */
+    uint32_t batches;         /* launches of the batched multi-source sweep */
+    double kernel_ms;         /* device time of the sweep (HIP events) */
+} zg_bc_info;
+
+/* Per-run information for PageRank. */
+typedef struct {
+    uint64_t iterations;      /* power iterations performed */
+    double last_l1;           /* L1 change of the last iteration, reduced in FP64 */
+    double residual_l1;       /* ||G(r) - r||_1 of the returned vector,
+                                 evaluated in FP64 from the stored FP32 ranks */
+    double error_bound_l1;    /* residual_l1 / (1 - damping): bound on the
+                                 L1 distance of the returned vector to the
+                                 exact fixed point */
+    double kernel_ms;         /* device time of the iterations (HIP events) */
+    int32_t precision;        /* 32 (FP32 Kahan) or 64 (FP64 iterations) */
+    int32_t reserved;
+} zg_pr_info;
+
+/* PageRank precision modes. */
+#define ZG_PR_FP32 32     /* FP32 values, Kahan row sums, FP64 reductions */
+#define ZG_PR_FP64 64     /* FP64 throughout (same data layout) */
+
+/* Replaces `new/5` + `buildGraph` (lib/zog/resource_graph.ex:188-200,
+ * :120-129) for the `:soa` backend: node ids 0..n_cap-1, arcs from[i]->to[i]
+ * in array order (parallel arcs and self-loops kept, weights not needed by
+ * either path).  live_mask (nullable = all live) marks tombstoned ids the way
+ * ArrayGraph.removeNode leaves them (array_graph.zig:133-157).  Builds the
+ * successor CSR (reverse insertion order, array_graph.zig:94-104,166-180) and
+ * the predecessor CSR (ascending source id, duplicates kept,
+ * centrality.zig:881-899) once and keeps both resident in HBM.  An id >=
+ * n_cap returns ZG_ERR_NODE_NOT_FOUND (addEdge, array_graph.zig:89). */
+int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
+                    const uint32_t *to, uint64_t m, zg_graph **out);
+
+/* Replaces the resource destructor (resource_graph.ex:109-118); the NIF's
+ * `nif_destroy` (:202-204) only releases the creation reference, so the
+ * dtor is where this is called.  NULL is a no-op. */
+void zg_graph_destroy(zg_graph *g);
+
+/* Node capacity V (nodeCapacity, array_graph.zig:206-208), live node count
+ * (nodeCount, :196-198) and live arc count; any out pointer may be NULL. */
+int zg_graph_info(const zg_graph *g, uint32_t *n_cap, uint32_t *n_live, uint64_t *n_arcs);
+
+/* Replaces zog.centrality.betweennessUnweighted on an ArrayGraph
+ * (priv/zog/src/centrality.zig:694-798) as called by
+ * `nif_betweenness_unweighted` (resource_graph.ex:206-215) followed by
+ * extractScores (:166-174): unnormalised directed-sum BC over all live
+ * sources into out_bc[n_cap] (0.0 for non-live ids).  sources == NULL means
+ * every live node (the reference); otherwise only the listed sources are
+ * summed (an extension for sampled BC; a non-live source is
+ * ZG_ERR_NODE_NOT_FOUND).  The x0.5 undirected scaling stays with the
+ * caller, as in maybe_scale_undirected (resource_graph.ex:2633-2637).
+ * info may be NULL. */
+int zg_bc_unweighted(zg_graph *g, const uint32_t *sources, uint64_t n_sources, double *out_bc,
+                     zg_bc_info *info);
+
+/* Same computation with device-resident source list and output (both in HBM,
+ * d_sources may be NULL for all live nodes) on the caller's HIP stream
+ * (hipStream_t passed as void*, NULL = the handle's stream); returns after
+ * the stream has drained.  For callers that keep results on the GPU. */
+int zg_bc_unweighted_device(zg_graph *g, const uint32_t *d_sources, uint64_t n_sources,
+                            double *d_out_bc, void *stream, zg_bc_info *info);
+
+/* Replaces zog.centrality.pagerank on an ArrayGraph
+ * (priv/zog/src/centrality.zig:871-965, options :16-20) as called by the
+ * `pagerank/4` NIF (resource_graph.ex:250-264) followed by extractScores:
+ * out[n_cap] (0.0 for non-live ids).  Same iteration, dangling rule, strict
+ * `l1 < tolerance` stop after the swap, n=0 and n=1 special cases.
+ * precision is ZG_PR_FP32 or ZG_PR_FP64.  info may be NULL. */
+int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                int precision, double *out, zg_pr_info *info);
+
+/* Same computation with the output written to device memory d_out[n_cap]
+ * (FP64) on the caller's stream (NULL = the handle's stream). */
+int zg_pagerank_device(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                       int precision, double *d_out, void *stream, zg_pr_info *info);
+
+/* Per-source BFS export for parity checks (no reference counterpart; the
+ * reference keeps these workspaces private, centrality.zig:697-703): for each
+ * of n_sources sources, depth_out[i*n_cap + v] (0xFFFFFFFF = unreached) and
+ * sigma_out[i*n_cap + v] = stored path count (FP64, exact integer below 2^53)
+ * with shift_out[i*max_levels + L] = the power-of-two shift k_L applied to
+ * level L+1's stored counts (true count at level L = stored * 2^(k_0+..+k_{L-1})).
+ * levels_out[i] = number of BFS levels.  shift_out may be NULL; max_levels
+ * bounds its row length (ZG_ERR_INVALID_ARG if a BFS is deeper). */
+int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sources, uint32_t *depth_out,
+                double *sigma_out, int32_t *shift_out, uint32_t max_levels, uint32_t *levels_out);
+
+/* Sets the number of 64-source groups swept per batch (1..16, default 4 =
+ * 256 sources per launch).  Returns the previous value, or <0. */
+int zg_set_batch_groups(zg_graph *g, int groups);
+
+/* Kernel-level profile of the most recent Brandes / PageRank call (no
+ * reference counterpart; for the benchmark's roofline figures).  While
+ * profiling is on, HIP events are recorded on the launch stream around every
+ * kernel launch, and the per-kernel device time is summed.  PageRank counts
+ * only launches that ran an iteration (not the no-op launches queued past
+ * convergence). */
+#define ZG_PROF_BC_FORWARD 0
+#define ZG_PROF_BC_FINALIZE 1
+#define ZG_PROF_BC_BACKWARD 2
+#define ZG_PROF_BC_HUB 3
+#define ZG_PROF_BC_SETUP 4       /* per-batch resets, sources, final reduction */
+#define ZG_PROF_PR_ITER 5
+#define ZG_PROF_SLOTS 8
+typedef struct {
+    double ms[ZG_PROF_SLOTS];
+    uint64_t launches[ZG_PROF_SLOTS];
+} zg_profile;
+
+/* Turns per-kernel profiling on (1) or off (0) for this handle. */
+int zg_set_profiling(zg_graph *g, int on);
+
+/* Copies the profile of the most recent call(s) since profiling was turned
+ * on or last read, then clears it. */
+int zg_get_profile(zg_graph *g, zg_profile *out);
+
+/* Human-readable name of a status code. */
+const char *zg_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZOG_HIP_H */

@@ -1,0 +1,541 @@
+/*
+ * CPU oracle for Zog's `:soa` Brandes betweenness and PageRank.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP path is
+ * compared against and the CPU baseline that bench.py times beside it.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it;
+ * the product library never links or calls it.
+ *
+ * It restates, in plain C, the algorithms of the reference's Zig code:
+ *   - ArrayGraph storage and successor order: priv/zog/src/models/array_graph.zig
+ *     addNode :66-75, addEdge :88-108 (prepend to the per-node linked list),
+ *     SuccessorIterator.next :166-180 (skip deleted edges and deleted
+ *     targets; parallel arcs all yielded), nodeIds :268-284, hasNode :210-213.
+ *   - buildGraph in lib/zog/resource_graph.ex:120-129 (all nodes first, then
+ *     addEdge in array order; an out-of-range id is NodeNotFound).
+ *   - betweennessUnweighted ArrayGraph branch, priv/zog/src/centrality.zig:694-798.
+ *   - pagerank ArrayGraph branch, priv/zog/src/centrality.zig:871-965 with
+ *     PageRankOptions :16-20.
+ *   - CentralityResult.get default 0.0 (centrality.zig:31-33) and
+ *     extractScores (resource_graph.ex:166-174): dense f64[V], 0.0 for
+ *     non-live ids.
+ *
+ * Extensions the reference does not have (each documented where defined):
+ *   - a source subset for Brandes (sampled/batched sources),
+ *   - a "scaled sigma" mode that stores path counts as FP64 with a per-level
+ *     power-of-two shift, so counts beyond 2^64 (and beyond the FP64 range)
+ *     stay usable; it reduces to the reference's formula whenever no shift
+ *     is taken,
+ *   - per-source export of BFS depths and path counts,
+ *   - an N-thread source-chunked BC (the pattern of centrality.zig:216-256).
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ZO_NONE 0xFFFFFFFFu
+
+enum {
+    ZO_OK = 0,
+    ZO_ERR_NODE_NOT_FOUND = -2,
+    ZO_ERR_OOM = -3,
+    ZO_ERR_ARG = -4,
+};
+
+/* ArrayGraph(void, f64) restated: node SoA (first_edge, is_deleted) and edge
+ * SoA (to, next_edge, is_deleted).  Edge data (the f64 weight) is unused by
+ * both paths and is not stored. */
+typedef struct {
+    uint32_t n_cap;
+    uint64_t m;
+    uint32_t *first_edge;   /* ZO_NONE = empty list */
+    uint8_t *node_deleted;
+    uint32_t *to;
+    uint32_t *next_edge;    /* ZO_NONE = end of list */
+    uint8_t *edge_deleted;
+} zo_graph;
+
+void zo_graph_destroy(zo_graph *g)
+{
+    if (!g) return;
+    free(g->first_edge);
+    free(g->node_deleted);
+    free(g->to);
+    free(g->next_edge);
+    free(g->edge_deleted);
+    free(g);
+}
+
+static int zo_has_node(const zo_graph *g, uint32_t id)
+{
+    return id < g->n_cap && !g->node_deleted[id];
+}
+
+/* buildGraph (resource_graph.ex:120-129) + addEdge (array_graph.zig:88-108).
+ * live_mask (nullable) marks tombstoned nodes; the reference reaches that
+ * state through removeNode (array_graph.zig:133-157), which also tombstones
+ * every arc touching the node, so arcs at a dead endpoint are kept but marked
+ * deleted here. */
+int zo_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
+                    const uint32_t *to, uint64_t m, zo_graph **out)
+{
+    *out = NULL;
+    zo_graph *g = (zo_graph *)calloc(1, sizeof(zo_graph));
+    if (!g) return ZO_ERR_OOM;
+    g->n_cap = n_cap;
+    g->m = m;
+    g->first_edge = (uint32_t *)malloc(sizeof(uint32_t) * (n_cap ? n_cap : 1));
+    g->node_deleted = (uint8_t *)calloc(n_cap ? n_cap : 1, 1);
+    g->to = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
+    g->next_edge = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
+    g->edge_deleted = (uint8_t *)calloc(m ? m : 1, 1);
+    if (!g->first_edge || !g->node_deleted || !g->to || !g->next_edge || !g->edge_deleted) {
+        zo_graph_destroy(g);
+        return ZO_ERR_OOM;
+    }
+    for (uint32_t i = 0; i < n_cap; i++) g->first_edge[i] = ZO_NONE;
+    for (uint64_t e = 0; e < m; e++) {
+        uint32_t f = from[e], t = to[e];
+        if (f >= n_cap || t >= n_cap) {
+            zo_graph_destroy(g);
+            return ZO_ERR_NODE_NOT_FOUND;
+        }
+        g->to[e] = t;
+        g->next_edge[e] = g->first_edge[f];
+        g->first_edge[f] = (uint32_t)e;
+    }
+    if (live_mask) {
+        for (uint32_t i = 0; i < n_cap; i++) g->node_deleted[i] = live_mask[i] ? 0 : 1;
+        for (uint64_t e = 0; e < m; e++)
+            if (g->node_deleted[from[e]] || g->node_deleted[to[e]]) g->edge_deleted[e] = 1;
+    }
+    *out = g;
+    return ZO_OK;
+}
+
+/* SuccessorIterator.next (array_graph.zig:166-180): advance `*cur` to the next
+ * live arc; returns its target or ZO_NONE at the end of the list. */
+static inline uint32_t zo_next_succ(const zo_graph *g, uint32_t *cur)
+{
+    while (*cur != ZO_NONE) {
+        uint32_t e = *cur;
+        *cur = g->next_edge[e];
+        if (!g->edge_deleted[e]) {
+            uint32_t w = g->to[e];
+            if (!g->node_deleted[w]) return w;
+        }
+    }
+    return ZO_NONE;
+}
+
+uint32_t zo_node_count(const zo_graph *g)
+{
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < g->n_cap; i++) n += g->node_deleted[i] ? 0 : 1;
+    return n;
+}
+
+/* Successor list of v in the reference's iteration order (reverse insertion),
+ * written to out (capacity cap).  Returns the full count. */
+uint64_t zo_successors(const zo_graph *g, uint32_t v, uint32_t *out, uint64_t cap)
+{
+    uint64_t k = 0;
+    uint32_t cur = g->first_edge[v];
+    uint32_t w;
+    while ((w = zo_next_succ(g, &cur)) != ZO_NONE) {
+        if (k < cap) out[k] = w;
+        k++;
+    }
+    return k;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Brandes, unweighted (centrality.zig:694-798)                              */
+/* ------------------------------------------------------------------------- */
+
+/* Path-count modes.  ZO_SIGMA_U64 is the reference (`sigma: []usize`,
+ * centrality.zig:701, incremented at :763, converted to f64 at :775-779);
+ * an overflow wraps as in the ReleaseFast build and is reported.
+ * ZO_SIGMA_SCALED keeps sigma as FP64 with a per-(source, level) exponent
+ * shift k_L: values at level L+1 are stored as 2^-k_L * sum(level-L values),
+ * k_L = e_L if e_L > ZO_SHIFT_T else 0, where e_L is the largest binary
+ * exponent (ilogb) among level-L values.  The true count is the stored value
+ * times 2^(k_0 + ... + k_{L-1}).  In the backward sweep sigma_u/sigma_w for an
+ * arc u(level L)->w(level L+1) is therefore (s'_u / s'_w) * 2^-k_L.  While no
+ * value exceeds 2^ZO_SHIFT_T every k is 0 and the arithmetic is exactly the
+ * reference's (on FP64 counts). */
+#define ZO_SIGMA_U64 0
+#define ZO_SIGMA_SCALED 1
+#define ZO_SHIFT_T 512
+
+typedef struct {
+    /* per-source workspaces (centrality.zig:697-726) */
+    uint32_t *dist;
+    uint64_t *sigma_u;
+    double *sigma_f;
+    double *delta;
+    uint32_t *queue;    /* BFS queue; queue order is the stack order */
+    /* preds[w] as a singly linked list in append order over an arena */
+    uint32_t *pred_head, *pred_tail, *pred_node, *pred_next;
+    uint64_t pred_cap;
+    int32_t *shift;     /* k_L per level (scaled mode) */
+} zo_ws;
+
+static void zo_ws_free(zo_ws *w)
+{
+    free(w->dist); free(w->sigma_u); free(w->sigma_f); free(w->delta); free(w->queue);
+    free(w->pred_head); free(w->pred_tail); free(w->pred_node); free(w->pred_next);
+    free(w->shift);
+}
+
+static int zo_ws_init(zo_ws *w, const zo_graph *g)
+{
+    memset(w, 0, sizeof(*w));
+    size_t V = g->n_cap ? g->n_cap : 1;
+    w->dist = malloc(sizeof(uint32_t) * V);
+    w->sigma_u = malloc(sizeof(uint64_t) * V);
+    w->sigma_f = malloc(sizeof(double) * V);
+    w->delta = malloc(sizeof(double) * V);
+    w->queue = malloc(sizeof(uint32_t) * V);
+    w->pred_head = malloc(sizeof(uint32_t) * V);
+    w->pred_tail = malloc(sizeof(uint32_t) * V);
+    w->pred_cap = g->m ? g->m : 1;   /* at most one pred entry per arc */
+    w->pred_node = malloc(sizeof(uint32_t) * w->pred_cap);
+    w->pred_next = malloc(sizeof(uint32_t) * w->pred_cap);
+    w->shift = malloc(sizeof(int32_t) * (V + 1));
+    if (!w->dist || !w->sigma_u || !w->sigma_f || !w->delta || !w->queue || !w->pred_head ||
+        !w->pred_tail || !w->pred_node || !w->pred_next || !w->shift) {
+        zo_ws_free(w);
+        return ZO_ERR_OOM;
+    }
+    return ZO_OK;
+}
+
+typedef struct {
+    uint64_t arcs_scanned;  /* m_s: arcs yielded by successors() in the forward sweep */
+    uint32_t reached;       /* n_s: vertices dequeued */
+    uint32_t levels;        /* eccentricity + 1 */
+    int overflow;           /* u64 path count wrapped */
+} zo_src_stats;
+
+/* One source of the ArrayGraph branch: forward BFS (:745-767) then the
+ * dependency accumulation (:769-787), adding delta into bc.  Returns stats. */
+static zo_src_stats zo_brandes_one(const zo_graph *g, zo_ws *w, uint32_t s, int mode, double *bc)
+{
+    zo_src_stats st = {0, 0, 0, 0};
+    uint32_t V = g->n_cap;
+    /* workspace reset (:730-738) */
+    for (uint32_t i = 0; i < V; i++) {
+        w->dist[i] = ZO_NONE;
+        w->sigma_u[i] = 0;
+        w->sigma_f[i] = 0.0;
+        w->delta[i] = 0.0;
+        w->pred_head[i] = ZO_NONE;
+        w->pred_tail[i] = ZO_NONE;
+    }
+    uint64_t npred = 0;
+    uint32_t qlen = 0, head = 0;
+    w->dist[s] = 0;
+    w->sigma_u[s] = 1;
+    w->sigma_f[s] = 1.0;
+    w->queue[qlen++] = s;
+
+    uint32_t cur_level = ZO_NONE;
+    int32_t k_cur = 0;           /* shift applied to values of level cur_level+1 */
+
+    while (head < qlen) {
+        uint32_t v = w->queue[head];
+        uint32_t d_v = w->dist[v];
+        if (mode == ZO_SIGMA_SCALED && d_v != cur_level) {
+            /* entering level d_v: its sigma values are complete (all of level
+             * d_v - 1 has been dequeued); find the shift for level d_v + 1 */
+            cur_level = d_v;
+            int emax = -100000;
+            for (uint32_t i = head; i < qlen && w->dist[w->queue[i]] == d_v; i++) {
+                int e = ilogb(w->sigma_f[w->queue[i]]);
+                if (e > emax) emax = e;
+            }
+            k_cur = emax > ZO_SHIFT_T ? emax : 0;
+            w->shift[d_v] = k_cur;
+        }
+        head++;
+        uint32_t cur = g->first_edge[v];
+        uint32_t t;
+        while ((t = zo_next_succ(g, &cur)) != ZO_NONE) {
+            st.arcs_scanned++;
+            if (w->dist[t] == ZO_NONE) {
+                w->dist[t] = d_v + 1;
+                w->queue[qlen++] = t;
+            }
+            if (w->dist[t] == d_v + 1) {
+                uint64_t sum;
+                if (__builtin_add_overflow(w->sigma_u[t], w->sigma_u[v], &sum)) st.overflow = 1;
+                w->sigma_u[t] = sum;
+                w->sigma_f[t] += ldexp(w->sigma_f[v], -k_cur);
+                /* preds[t].append(v) */
+                uint32_t p = (uint32_t)npred++;
+                w->pred_node[p] = v;
+                w->pred_next[p] = ZO_NONE;
+                if (w->pred_tail[t] == ZO_NONE) w->pred_head[t] = p;
+                else w->pred_next[w->pred_tail[t]] = p;
+                w->pred_tail[t] = p;
+            }
+        }
+    }
+    st.reached = qlen;
+    st.levels = qlen ? w->dist[w->queue[qlen - 1]] + 1 : 0;
+
+    /* dependency accumulation: stack = queue order, popped in reverse */
+    for (uint32_t i = qlen; i > 0; i--) {
+        uint32_t v = w->queue[i - 1];
+        double delta_v = w->delta[v];
+        if (mode == ZO_SIGMA_U64) {
+            double sigma_v = (double)w->sigma_u[v];
+            for (uint32_t p = w->pred_head[v]; p != ZO_NONE; p = w->pred_next[p]) {
+                uint32_t u = w->pred_node[p];
+                double c = ((double)w->sigma_u[u] / sigma_v) * (1.0 + delta_v);
+                w->delta[u] += c;
+            }
+        } else {
+            double sigma_v = w->sigma_f[v];
+            for (uint32_t p = w->pred_head[v]; p != ZO_NONE; p = w->pred_next[p]) {
+                uint32_t u = w->pred_node[p];
+                /* u sits one level above v: ratio carries 2^-k_{dist(u)} */
+                double ratio = ldexp(w->sigma_f[u] / sigma_v, -w->shift[w->dist[u]]);
+                w->delta[u] += ratio * (1.0 + delta_v);
+            }
+        }
+        if (v != s) bc[v] += delta_v;
+    }
+    return st;
+}
+
+typedef struct {
+    const zo_graph *g;
+    const uint32_t *sources;
+    uint64_t begin, end;
+    int mode;
+    double *bc;            /* private accumulator, length n_cap */
+    uint64_t *arcs_per_source; /* nullable, indexed by source position */
+    int overflow;
+    int err;
+} zo_bc_job;
+
+static void *zo_bc_worker(void *arg)
+{
+    zo_bc_job *j = (zo_bc_job *)arg;
+    zo_ws ws;
+    if (zo_ws_init(&ws, j->g) != ZO_OK) {
+        j->err = ZO_ERR_OOM;
+        return NULL;
+    }
+    for (uint64_t i = j->begin; i < j->end; i++) {
+        zo_src_stats st = zo_brandes_one(j->g, &ws, j->sources[i], j->mode, j->bc);
+        if (j->arcs_per_source) j->arcs_per_source[i] = st.arcs_scanned;
+        j->overflow |= st.overflow;
+    }
+    zo_ws_free(&ws);
+    return NULL;
+}
+
+/* Betweenness over a list of sources (NULL = every live node in id order,
+ * centrality.zig:728-729), unnormalised, into out_bc[n_cap] (0.0 for non-live
+ * ids).  n_threads > 1 splits the source list into contiguous chunks, each
+ * with its own workspace and accumulator; the chunks are added in chunk
+ * order, so the result is deterministic for a given thread count.
+ * Returns 0, or 1 if a u64 path count wrapped (mode ZO_SIGMA_U64), or <0. */
+int zo_bc_unweighted(const zo_graph *g, const uint32_t *sources, uint64_t n_sources, int mode,
+                     int n_threads, double *out_bc, uint64_t *arcs_per_source)
+{
+    uint32_t V = g->n_cap;
+    uint32_t *all = NULL;
+    if (!sources) {
+        all = malloc(sizeof(uint32_t) * (V ? V : 1));
+        if (!all) return ZO_ERR_OOM;
+        n_sources = 0;
+        for (uint32_t i = 0; i < V; i++)
+            if (!g->node_deleted[i]) all[n_sources++] = i;
+        sources = all;
+    } else {
+        for (uint64_t i = 0; i < n_sources; i++)
+            if (!zo_has_node(g, sources[i])) {
+                free(all);
+                return ZO_ERR_NODE_NOT_FOUND;
+            }
+    }
+    for (uint32_t i = 0; i < V; i++) out_bc[i] = 0.0;
+    if (n_threads < 1) n_threads = 1;
+    if ((uint64_t)n_threads > n_sources) n_threads = n_sources ? (int)n_sources : 1;
+
+    zo_bc_job *jobs = calloc((size_t)n_threads, sizeof(zo_bc_job));
+    pthread_t *th = calloc((size_t)n_threads, sizeof(pthread_t));
+    int rc = ZO_OK;
+    if (!jobs || !th) rc = ZO_ERR_OOM;
+    uint64_t chunk = n_threads ? (n_sources + n_threads - 1) / n_threads : 0;
+    for (int t = 0; t < n_threads && rc == ZO_OK; t++) {
+        jobs[t].g = g;
+        jobs[t].sources = sources;
+        jobs[t].begin = (uint64_t)t * chunk;
+        jobs[t].end = jobs[t].begin + chunk > n_sources ? n_sources : jobs[t].begin + chunk;
+        if (jobs[t].begin > jobs[t].end) jobs[t].begin = jobs[t].end;
+        jobs[t].mode = mode;
+        jobs[t].arcs_per_source = arcs_per_source;
+        jobs[t].bc = (n_threads == 1) ? out_bc : calloc(V ? V : 1, sizeof(double));
+        if (!jobs[t].bc) rc = ZO_ERR_OOM;
+    }
+    if (rc == ZO_OK) {
+        if (n_threads == 1) {
+            zo_bc_worker(&jobs[0]);
+        } else {
+            for (int t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, zo_bc_worker, &jobs[t]);
+            for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
+            for (int t = 0; t < n_threads; t++)
+                for (uint32_t i = 0; i < V; i++) out_bc[i] += jobs[t].bc[i];
+        }
+        for (int t = 0; t < n_threads; t++) {
+            if (jobs[t].err) rc = jobs[t].err;
+            if (jobs[t].overflow && rc == ZO_OK) rc = 1;
+        }
+    }
+    if (jobs && n_threads > 1)
+        for (int t = 0; t < n_threads; t++) free(jobs[t].bc);
+    free(jobs);
+    free(th);
+    free(all);
+    return rc;
+}
+
+/* Per-source BFS export: depth (ZO_NONE if unreached), path counts as u64
+ * (reference type, wraps) and as scaled FP64 with the per-level shifts
+ * (shift_out[L] for L < levels; nullable), plus stats. */
+int zo_bfs_paths(const zo_graph *g, uint32_t s, uint32_t *depth_out, uint64_t *sigma_u_out,
+                 double *sigma_f_out, int32_t *shift_out, uint64_t *arcs_scanned,
+                 uint32_t *levels, int *overflow)
+{
+    if (!zo_has_node(g, s)) return ZO_ERR_NODE_NOT_FOUND;
+    zo_ws ws;
+    if (zo_ws_init(&ws, g) != ZO_OK) return ZO_ERR_OOM;
+    double *scratch = calloc(g->n_cap ? g->n_cap : 1, sizeof(double));
+    if (!scratch) {
+        zo_ws_free(&ws);
+        return ZO_ERR_OOM;
+    }
+    zo_src_stats st = zo_brandes_one(g, &ws, s, ZO_SIGMA_SCALED, scratch);
+    for (uint32_t i = 0; i < g->n_cap; i++) {
+        if (depth_out) depth_out[i] = ws.dist[i];
+        if (sigma_u_out) sigma_u_out[i] = ws.sigma_u[i];
+        if (sigma_f_out) sigma_f_out[i] = ws.sigma_f[i];
+    }
+    if (shift_out)
+        for (uint32_t L = 0; L < st.levels; L++) shift_out[L] = ws.shift[L];
+    if (arcs_scanned) *arcs_scanned = st.arcs_scanned;
+    if (levels) *levels = st.levels;
+    if (overflow) *overflow = st.overflow;
+    free(scratch);
+    zo_ws_free(&ws);
+    return ZO_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* PageRank (centrality.zig:871-965)                                          */
+/* ------------------------------------------------------------------------- */
+
+/* Returns 0 and fills out[n_cap] (0.0 for non-live), *iters_out = iterations
+ * performed, *l1_out = L1 change of the last iteration (0 if none). */
+int zo_pagerank(const zo_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                double *out, uint64_t *iters_out, double *l1_out)
+{
+    uint32_t V = g->n_cap;
+    for (uint32_t i = 0; i < V; i++) out[i] = 0.0;
+    if (iters_out) *iters_out = 0;
+    if (l1_out) *l1_out = 0.0;
+    /* collectNodes (utils.zig:42-49): live ids ascending */
+    uint32_t *nodes = malloc(sizeof(uint32_t) * (V ? V : 1));
+    if (!nodes) return ZO_ERR_OOM;
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < V; i++)
+        if (!g->node_deleted[i]) nodes[n++] = i;
+    if (n == 0) {
+        free(nodes);
+        return ZO_OK;
+    }
+    if (n == 1) {
+        out[nodes[0]] = 1.0;
+        free(nodes);
+        return ZO_OK;
+    }
+    /* in_neighbors (:881-899): for each live `from` in id order, for each
+     * successor, append `from` to in[to].  Stored as CSR with a counting pass
+     * so each list keeps the append order. */
+    uint64_t *in_ptr = calloc((size_t)V + 1, sizeof(uint64_t));
+    uint64_t *out_deg = calloc(V, sizeof(uint64_t));
+    if (!in_ptr || !out_deg) {
+        free(nodes); free(in_ptr); free(out_deg);
+        return ZO_ERR_OOM;
+    }
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t v = nodes[k];
+        uint32_t cur = g->first_edge[v], t;
+        while ((t = zo_next_succ(g, &cur)) != ZO_NONE) {
+            in_ptr[t + 1]++;
+            out_deg[v]++;   /* out_degrees (:901-910) */
+        }
+    }
+    for (uint32_t i = 0; i < V; i++) in_ptr[i + 1] += in_ptr[i];
+    uint64_t E = in_ptr[V];
+    uint32_t *in_col = malloc(sizeof(uint32_t) * (E ? E : 1));
+    uint64_t *fill = malloc(sizeof(uint64_t) * V);
+    double *ranks = calloc(V, sizeof(double));
+    double *new_ranks = calloc(V, sizeof(double));
+    if (!in_col || !fill || !ranks || !new_ranks) {
+        free(nodes); free(in_ptr); free(out_deg); free(in_col); free(fill); free(ranks); free(new_ranks);
+        return ZO_ERR_OOM;
+    }
+    memcpy(fill, in_ptr, sizeof(uint64_t) * V);
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t v = nodes[k];
+        uint32_t cur = g->first_edge[v], t;
+        while ((t = zo_next_succ(g, &cur)) != ZO_NONE) in_col[fill[t]++] = v;
+    }
+    double n_f = (double)n;
+    double initial_rank = 1.0 / n_f;
+    for (uint32_t k = 0; k < n; k++) ranks[nodes[k]] = initial_rank;
+
+    uint64_t it = 0;
+    double l1_norm = 0.0;
+    while (it < max_iterations) {
+        double sink_sum = 0.0;
+        for (uint32_t k = 0; k < n; k++) {
+            uint32_t v = nodes[k];
+            if (out_deg[v] == 0) sink_sum += ranks[v] / n_f;
+        }
+        for (uint32_t k = 0; k < n; k++) {
+            uint32_t v = nodes[k];
+            double rank_sum = 0.0;
+            for (uint64_t e = in_ptr[v]; e < in_ptr[v + 1]; e++) {
+                uint32_t u = in_col[e];
+                uint64_t du = out_deg[u];
+                if (du > 0) rank_sum += ranks[u] / (double)du;
+            }
+            new_ranks[v] = (1.0 - damping) / n_f + damping * (sink_sum + rank_sum);
+        }
+        l1_norm = 0.0;
+        for (uint32_t k = 0; k < n; k++) {
+            uint32_t v = nodes[k];
+            l1_norm += fabs(new_ranks[v] - ranks[v]);
+        }
+        double *tmp = ranks;
+        ranks = new_ranks;
+        new_ranks = tmp;
+        it++;
+        if (l1_norm < tolerance) break;
+    }
+    for (uint32_t k = 0; k < n; k++) out[nodes[k]] = ranks[nodes[k]];
+    if (iters_out) *iters_out = it;
+    if (l1_out) *l1_out = l1_norm;
+    free(nodes); free(in_ptr); free(out_deg); free(in_col); free(fill); free(ranks); free(new_ranks);
+    return ZO_OK;
+}

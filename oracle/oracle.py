@@ -1,0 +1,144 @@
+"""ctypes front-end of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+bench.py's ``cpu_baseline`` leg, always as the checker or the timed CPU
+baseline, never as the product path.
+
+The oracle restates priv/zog/src/centrality.zig:694-798 (unweighted Brandes,
+ArrayGraph branch) and :871-965 (PageRank, ArrayGraph branch) over a
+restatement of priv/zog/src/models/array_graph.zig (linked-list successors in
+reverse insertion order, tombstones).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+SIGMA_U64 = 0
+SIGMA_SCALED = 1
+NONE_U32 = 0xFFFFFFFF
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        L.zo_graph_create.argtypes = [ctypes.c_uint32, P, P, P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.zo_graph_create.restype = ctypes.c_int
+        L.zo_graph_destroy.argtypes = [P]
+        L.zo_graph_destroy.restype = None
+        L.zo_node_count.argtypes = [P]
+        L.zo_node_count.restype = ctypes.c_uint32
+        L.zo_successors.argtypes = [P, ctypes.c_uint32, P, ctypes.c_uint64]
+        L.zo_successors.restype = ctypes.c_uint64
+        L.zo_bc_unweighted.argtypes = [P, P, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, P, P]
+        L.zo_bc_unweighted.restype = ctypes.c_int
+        L.zo_bfs_paths.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P]
+        L.zo_bfs_paths.restype = ctypes.c_int
+        L.zo_pagerank.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P, P, P]
+        L.zo_pagerank.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Graph:
+    """ArrayGraph(void, f64) restated; built like resource_graph.ex:120-129."""
+
+    def __init__(self, n_cap, src, dst, live_mask=None):
+        self.n_cap = int(n_cap)
+        self.src = np.ascontiguousarray(src, dtype=np.uint32)
+        self.dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        self.live = None if live_mask is None else np.ascontiguousarray(live_mask, dtype=np.uint8)
+        h = ctypes.c_void_p()
+        rc = lib().zo_graph_create(self.n_cap, _ptr(self.live), _ptr(self.src), _ptr(self.dst),
+                                   len(self.src), ctypes.byref(h))
+        if rc != 0:
+            raise OracleError(f"zo_graph_create failed: {rc}")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().zo_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def node_count(self) -> int:
+        return int(lib().zo_node_count(self._h))
+
+    def successors(self, v: int) -> np.ndarray:
+        k = int(lib().zo_successors(self._h, v, None, 0))
+        out = np.empty(k, dtype=np.uint32)
+        lib().zo_successors(self._h, v, _ptr(out), k)
+        return out
+
+    def betweenness(self, sources=None, mode=SIGMA_U64, threads=1, return_arcs=False):
+        """Unnormalised directed-sum BC, f64[n_cap]; returns (bc, overflowed[, arcs])."""
+        out = np.zeros(self.n_cap, dtype=np.float64)
+        src = None if sources is None else np.ascontiguousarray(sources, dtype=np.uint32)
+        nsrc = 0 if src is None else len(src)
+        arcs = None
+        if return_arcs:
+            arcs = np.zeros(nsrc if src is not None else self.node_count(), dtype=np.uint64)
+        rc = lib().zo_bc_unweighted(self._h, _ptr(src), nsrc, mode, threads, _ptr(out), _ptr(arcs))
+        if rc < 0:
+            raise OracleError(f"zo_bc_unweighted failed: {rc}")
+        if return_arcs:
+            return out, rc == 1, arcs
+        return out, rc == 1
+
+    def bfs_paths(self, s: int):
+        """(depth u32[V] (NONE_U32 = unreached), sigma u64[V], sigma_scaled f64[V],
+        shifts i32[levels], arcs_scanned, overflowed)."""
+        V = self.n_cap
+        depth = np.empty(V, dtype=np.uint32)
+        su = np.empty(V, dtype=np.uint64)
+        sf = np.empty(V, dtype=np.float64)
+        shift = np.zeros(V + 1, dtype=np.int32)
+        arcs = ctypes.c_uint64()
+        levels = ctypes.c_uint32()
+        ovf = ctypes.c_int()
+        rc = lib().zo_bfs_paths(self._h, s, _ptr(depth), _ptr(su), _ptr(sf), _ptr(shift),
+                                ctypes.byref(arcs), ctypes.byref(levels), ctypes.byref(ovf))
+        if rc != 0:
+            raise OracleError(f"zo_bfs_paths failed: {rc}")
+        return depth, su, sf, shift[: levels.value].copy(), int(arcs.value), bool(ovf.value)
+
+    def pagerank(self, damping=0.85, max_iterations=100, tolerance=1e-4):
+        """FP64 PageRank, f64[n_cap]; returns (ranks, iterations, last_l1)."""
+        out = np.zeros(self.n_cap, dtype=np.float64)
+        it = ctypes.c_uint64()
+        l1 = ctypes.c_double()
+        rc = lib().zo_pagerank(self._h, damping, max_iterations, tolerance, _ptr(out),
+                               ctypes.byref(it), ctypes.byref(l1))
+        if rc != 0:
+            raise OracleError(f"zo_pagerank failed: {rc}")
+        return out, int(it.value), float(l1.value)

@@ -1,0 +1,127 @@
+"""CPU-side checks of the C-ABI library and the host mirror (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from zog_amd import _lib
+from zog_amd import resource_graph as rg
+from zog_amd.soa import SoA
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "zog_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?(zg_\w+)\s*\(", text, re.M)))
+
+
+def test_header_matches_binding_list():
+    assert declared_symbols() == sorted(_lib.EXPORTED)
+
+
+def test_library_loads_and_exports_every_symbol():
+    L = _lib.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+        assert ctypes.cast(getattr(L, name), ctypes.c_void_p).value
+
+
+def test_strerror_names_codes():
+    for code in range(-8, 1):
+        s = _lib.strerror(code)
+        assert s and "unknown" not in s
+    assert "unknown" in _lib.strerror(-99)
+
+
+def test_null_handle_is_rejected():
+    L = _lib.lib()
+    assert L.zg_graph_info(None, None, None, None) == _lib.ZG_ERR_INVALID_HANDLE
+    assert L.zg_bc_unweighted(None, None, 0, None, None) == _lib.ZG_ERR_INVALID_HANDLE
+    assert L.zg_pagerank(None, 0.85, 10, 1e-4, 32, None, None) == _lib.ZG_ERR_INVALID_HANDLE
+    assert L.zg_set_batch_groups(None, 4) == _lib.ZG_ERR_INVALID_HANDLE
+    L.zg_graph_destroy(None)
+    assert L.zg_graph_create(3, None, None, None, 0, None) == _lib.ZG_ERR_INVALID_ARG
+
+
+def test_errors_raise():
+    with pytest.raises(_lib.NodeNotFoundError):
+        _lib.check(_lib.ZG_ERR_NODE_NOT_FOUND, "x")
+    with pytest.raises(_lib.ZogError):
+        _lib.check(_lib.ZG_ERR_GPU_OOM, "x")
+    _lib.check(_lib.ZG_OK)
+
+
+# --- host mirror of Zog.SoA ---------------------------------------------------
+
+def test_soa_edge_arrays_order():
+    # soa_test.exs:45-55
+    b = SoA.directed().add_edge("A", "B", 1.0).add_edge("B", "C", 2.0)
+    f, t, w = b.to_edge_arrays()
+    assert f.tolist() == [0, 1] and t.tolist() == [1, 2] and w.tolist() == [1.0, 2.0]
+
+
+def test_soa_undirected_both_arcs():
+    b = SoA.undirected().add_edge("A", "B", 5.0)
+    f, t, w = b.to_edge_arrays()
+    assert list(zip(f.tolist(), t.tolist())) == [(0, 1), (1, 0)]
+    assert b.edge_count() == 2 and b.node_count() == 2
+    assert b.all_labels() == ["A", "B"]
+
+
+def test_map_scores_and_scaling():
+    b = SoA.undirected().add_edge("x", "y").add_edge("y", "z")
+    assert rg._map_scores(b, [1.0, 2.0, 3.0]) == {"x": 1.0, "y": 2.0, "z": 3.0}
+    assert rg._maybe_scale_undirected(b, np.array([2.0, 4.0])).tolist() == [1.0, 2.0]
+    d = SoA.directed().add_edge("x", "y")
+    assert rg._maybe_scale_undirected(d, np.array([2.0])).tolist() == [2.0]
+    i = SoA.from_arrays("directed", 3, [0, 1], [1, 2])
+    assert rg._map_scores(i, [0.5, 0.25, 0.25]) == {0: 0.5, 1: 0.25, 2: 0.25}
+
+
+def test_edgelist_parse_integer_labels(tmp_path):
+    # resource_graph_test.exs:545-571 input; placeholder ids up to max id
+    p = tmp_path / "e.txt"
+    p.write_text("# comment\nsource target\n0 1 1.5\n1 2 2.0\n\n2 0 1.0\nx 3\n5,4\n")
+    edges, next_id, labels = rg._parse_edgelist(str(p), True)
+    assert next_id == 6
+    assert [(u, v) for u, v, _ in edges] == [(0, 1), (1, 2), (2, 0), (5, 4)]
+    assert edges[0][2] == 1.5
+
+
+def test_edgelist_parse_labels_first_seen(tmp_path):
+    p = tmp_path / "e.txt"
+    p.write_text("b a\na c\tw\nc,b,2.5\n")
+    edges, next_id, labels = rg._parse_edgelist(str(p), False)
+    assert labels == ["b", "a", "c"] and next_id == 3
+    assert [(u, v, w) for u, v, w in edges] == [(0, 1, 1.0), (1, 2, 1.0), (2, 0, 2.5)]
+
+
+def test_undirected_dedupe_rule():
+    # addParsedEdges (resource_graph.ex:953-988): sort by (min, max), drop
+    # repeats of an unordered pair, both arcs unless a self-loop
+    edges = [(2, 1, 1.0), (0, 1, 1.0), (1, 2, 1.0), (3, 3, 1.0), (1, 0, 1.0)]
+    src, dst = rg._expand_edges(edges, directed=False)
+    assert list(zip(src, dst)) == [(0, 1), (1, 0), (2, 1), (1, 2), (3, 3)]
+    src, dst = rg._expand_edges(edges, directed=True)
+    assert list(zip(src, dst)) == [(2, 1), (0, 1), (1, 2), (3, 3), (1, 0)]
+
+
+def test_generators_deterministic():
+    from zog_amd import generators as G
+    a = G.erdos_renyi_directed(500, 0.02, 11)
+    b = G.erdos_renyi_directed(500, 0.02, 11)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert np.all(a[1] != a[2])
+    keys = a[1].astype(np.int64) * 500 + a[2]
+    assert len(np.unique(keys)) == len(keys)
+    n, s, d = G.rmat(10, 5000, 3)
+    assert n == 1024 and len(s) == 5000
+    assert len(np.unique(s.astype(np.int64) * n + d)) == 5000
+    n, s, d = G.grid_2d(4, 5, seed=1, delete=3)
+    assert n == 20 and len(s) == 2 * (4 * 4 + 3 * 5 - 3)
+    n, s, d = G.barabasi_albert(50, 4, 5)
+    assert len(s) == 2 * (6 + 46 * 4)

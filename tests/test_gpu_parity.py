@@ -1,0 +1,295 @@
+"""Parity of the gfx950 path (through the C ABI) with the CPU oracle.
+
+Bars: BFS depths and path counts bit-exact; BC within 1e-9 relative (per
+vertex, with an absolute floor of 1e-9 times the largest score); PageRank
+L1 <= 1e-6 against the FP64 oracle at the same iteration count.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import graphs
+from zog_amd import generators as gen
+from zog_amd import resource_graph as rg
+from zog_amd.soa import SoA
+
+pytestmark = pytest.mark.gpu
+
+BC_RTOL = 1e-9
+PR_L1 = 1e-6
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make(n, src, dst, live=None, groups=None):
+    g = rg.new(SoA.from_arrays("directed", n, src, dst), live_mask=live)
+    if groups:
+        rg.set_batch_groups(g, groups)
+    return g
+
+
+def assert_bc_close(got, ref):
+    scale = max(1.0, float(np.abs(ref).max()) if len(ref) else 1.0)
+    err = np.abs(got - ref)
+    ok = err <= BC_RTOL * np.maximum(np.abs(ref), 1e-9 * scale)
+    assert ok.all(), f"max err {err.max()} at {int(np.argmax(err))}"
+
+
+def check_bc(n, src, dst, sources=None, live=None, groups=None, mode=0):
+    from oracle import oracle as O
+    say("bc case", n, len(src), None if sources is None else len(sources), groups)
+    og = O.Graph(n, src, dst, live)
+    ref, ovf = og.betweenness(sources=sources, mode=mode)
+    g = make(n, src, dst, live, groups)
+    info = {}
+    got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
+    assert_bc_close(got, ref)
+    rg.destroy(g)
+    return got, ref, info
+
+
+def check_pr(n, src, dst, live=None, iters=50, tol=0.0, damping=0.85, precision=32):
+    from oracle import oracle as O
+    say("pr case", n, len(src), iters, tol, precision)
+    og = O.Graph(n, src, dst, live)
+    ref, it_ref, _ = og.pagerank(damping=damping, max_iterations=iters, tolerance=tol)
+    g = make(n, src, dst, live)
+    info = {}
+    got = rg.pagerank_array(g, damping, iters, tol, precision=precision, info=info)
+    rg.destroy(g)
+    return got, ref, it_ref, info
+
+
+# --- reference cases through the Zog.ResourceGraph mirror ------------------
+
+def test_reference_cases_resource_api():
+    # resource_graph_test.exs:41-54 path graph (undirected)
+    g = rg.new(SoA.undirected().add_edge("A", "B", 1.0).add_edge("B", "C", 1.0))
+    s = rg.betweenness_unweighted(g)
+    assert s["B"] > s["A"] and s["B"] > s["C"] and s["B"] == 1.0
+    rg.destroy(g)
+    # centrality_test.exs:26-39 star: centre 3.0, leaves exactly 0.0
+    g = rg.new(SoA.undirected().add_edge("center", "a").add_edge("center", "b").add_edge("center", "c"))
+    s = rg.betweenness_unweighted(g)
+    assert s == {"center": 3.0, "a": 0.0, "b": 0.0, "c": 0.0}
+    p = rg.pagerank(g)
+    assert all(p["center"] > p[k] for k in "abc")
+    rg.destroy(g)
+    # resource_graph_test.exs:459-541: several algorithms on one resource, raw lists
+    b = SoA.undirected().add_edge("A", "B").add_edge("B", "C").add_edge("C", "A").add_edge("B", "D")
+    g = rg.new(b)
+    assert len(rg.betweenness_unweighted(g)) == 4 and len(rg.pagerank(g)) == 4
+    raw = rg.pagerank(g, raw=True)
+    assert isinstance(raw, list) and len(raw) == 4 and all(isinstance(x, float) for x in raw)
+    assert isinstance(rg.betweenness_unweighted(g, raw=True), list)
+    assert rg.betweenness_centrality(g) == rg.betweenness_unweighted(g)
+    rg.destroy(g)
+    with pytest.raises(rg._lib.ZogError):
+        rg.pagerank(g)
+    # centrality_test.exs:12-24, 138-162 (directed)
+    g = rg.new(SoA.directed().add_edge("A", "B").add_edge("B", "C"))
+    s = rg.betweenness_unweighted(g)
+    assert s == {"A": 0.0, "B": 1.0, "C": 0.0}
+    assert rg.pagerank(g, damping=0.99)["A"] > 0.0
+    rg.destroy(g)
+    g = rg.new(SoA.directed().add_edge("A", "B").add_edge("C", "B"))
+    p = rg.pagerank(g)
+    assert p["B"] > p["A"] and p["B"] > p["C"]
+    rg.destroy(g)
+
+
+def test_empty_and_single_node():
+    g = rg.new(SoA.directed())
+    assert rg.betweenness_unweighted(g) == {} and rg.pagerank(g) == {}
+    rg.destroy(g)
+    g = rg.new(SoA.directed().add_node("A"))
+    assert rg.betweenness_unweighted(g) == {"A": 0.0}
+    assert rg.pagerank(g) == {"A": 1.0}
+    rg.destroy(g)
+
+
+def test_integer_labels_edgelist(tmp_path):
+    # resource_graph_test.exs:545-571
+    p = tmp_path / "e.txt"
+    p.write_text("0 1 1.5\n1 2 2.0\n2 0 1.0\n")
+    g = rg.read_edgelist(str(p), integer_labels=True)
+    assert g.builder.integer_labels and g.builder.next_id == 3
+    pr = rg.pagerank(g)
+    assert set(pr) == {0, 1, 2}
+    assert len(rg.pagerank(g, raw=True)) == 3
+    rg.destroy(g)
+    # sparse ids: placeholders are live, isolated and dangling
+    p.write_text("0 5\n5 2\n")
+    g = rg.read_edgelist(str(p), integer_labels=True, directed=False)
+    from oracle import oracle as O
+    og = O.Graph(6, *graphs.arcs([(0, 5), (5, 0), (2, 5), (5, 2)]))
+    assert np.allclose(rg.betweenness_unweighted(g, raw=True), 0.5 * og.betweenness()[0], rtol=1e-12)
+    ref, _, _ = og.pagerank()
+    assert np.abs(np.array(rg.pagerank(g, raw=True)) - ref).sum() < PR_L1
+    rg.destroy(g)
+
+
+def test_node_not_found():
+    with pytest.raises(rg._lib.NodeNotFoundError):
+        make(3, np.array([0, 5], np.uint32), np.array([1, 1], np.uint32))
+    g = make(3, *graphs.arcs([(0, 1)]))
+    with pytest.raises(rg._lib.NodeNotFoundError):
+        rg.betweenness_unweighted_array(g, sources=[7])
+    rg.destroy(g)
+
+
+# --- Brandes parity -----------------------------------------------------------
+
+def test_wiki_vote_fixture():
+    n, src, dst = graphs.wiki_vote()
+    check_bc(n, src, dst)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_multigraphs(seed):
+    rng = np.random.default_rng(500 + seed)
+    n = int(rng.integers(2, 300))
+    m = int(rng.integers(0, 6 * n))
+    n, src, dst = graphs.random_multigraph(rng, n, m)
+    check_bc(n, src, dst, groups=int(rng.choice([1, 2, 4])))
+
+
+def test_tombstoned_nodes():
+    rng = np.random.default_rng(9)
+    n, src, dst = graphs.random_multigraph(rng, 200, 900)
+    live = (rng.random(n) > 0.2).astype(np.uint8)
+    check_bc(n, src, dst, live=live)
+    got, ref, _, _ = check_pr(n, src, dst, live=live)
+    assert np.abs(got - ref).sum() <= PR_L1
+    assert np.all(got[live == 0] == 0.0)
+
+
+def test_er_parity_config():
+    n, src, dst = gen.erdos_renyi_directed(10_000, 8.0 / 9_999, gen.SEED_BASE + 1)
+    got, ref, info = check_bc(n, src, dst)
+    assert info["n_sources"] == n
+
+
+def test_hubs_split_chunks():
+    # in- and out-hubs far above the 256-arc chunk, parallel arcs on the hubs
+    rng = np.random.default_rng(21)
+    n = 3000
+    hub_in = np.full(2000, 7, np.uint32)
+    hub_in_src = rng.integers(0, n, 2000).astype(np.uint32)
+    hub_out_dst = rng.integers(0, n, 1500).astype(np.uint32)
+    hub_out = np.full(1500, 11, np.uint32)
+    _, s2, d2 = graphs.random_multigraph(rng, n, 9000)
+    src = np.concatenate([hub_in_src, hub_out, s2, hub_out[:300]])
+    dst = np.concatenate([hub_in, hub_out_dst, d2, hub_out_dst[:300]])
+    check_bc(n, src, dst, sources=np.arange(0, n, 3, dtype=np.uint32), groups=4)
+
+
+def test_rmat_small_sources_subset():
+    n, src, dst = gen.rmat(14, 1 << 18, gen.SEED_BASE + 77)
+    sources = gen.seeded_sources(n, src, dst, 300, 5)
+    check_bc(n, src, dst, sources=sources, groups=4)
+
+
+def test_batch_widths_agree():
+    n, src, dst = gen.rmat(11, 1 << 14, 3)
+    sources = np.arange(n, dtype=np.uint32)
+    outs = []
+    for gr in (1, 3, 16):
+        g = make(n, src, dst, groups=gr)
+        outs.append(rg.betweenness_unweighted_array(g, sources=sources))
+        rg.destroy(g)
+    for o in outs[1:]:
+        assert_bc_close(o, outs[0])
+
+
+def test_depth_sigma_bit_exact():
+    from oracle import oracle as O
+    n, src, dst = gen.rmat(12, 1 << 15, 99)
+    sources = gen.seeded_sources(n, src, dst, 70, 1)
+    og = O.Graph(n, src, dst)
+    g = make(n, src, dst)
+    depth, sigma, shift, levels = rg.bc_paths(g, sources, max_levels=64)
+    for i, s in enumerate(sources.tolist()):
+        d_ref, su, sf, sh, arcs, ovf = og.bfs_paths(s)
+        assert not ovf
+        assert np.array_equal(depth[i], d_ref)
+        reached = d_ref != O.NONE_U32
+        assert np.array_equal(sigma[i][reached], su[reached].astype(np.float64))
+        assert np.all(sigma[i][reached] == su[reached])
+        assert levels[i] == len(sh)
+    rg.destroy(g)
+
+
+def test_grid_overflowing_path_counts():
+    # sigma exceeds u64 (and the FP64 range at the far corner): compare with
+    # the oracle's scaled-sigma mode and check the shifted counts
+    from oracle import oracle as O
+    n, src, dst = gen.grid_2d(300, 300)
+    sources = np.array([0, 299, 150 * 300 + 150, 89999], np.uint32)
+    check_bc(n, src, dst, sources=sources, mode=O.SIGMA_SCALED)
+    og = O.Graph(n, src, dst)
+    g = make(n, src, dst)
+    depth, sigma, shift, levels = rg.bc_paths(g, sources[:1], max_levels=700)
+    d_ref, _, sf_ref, sh_ref, _, _ = og.bfs_paths(0)
+    assert np.array_equal(depth[0], d_ref)
+    assert np.array_equal(shift[0][: len(sh_ref)], sh_ref)
+    assert np.allclose(sigma[0], sf_ref, rtol=1e-11)
+    rg.destroy(g)
+
+
+def test_dependency_checksum_rmat():
+    # sum_v BC(v) over a source set = sum over sources of sum_t (dist(s,t) - 1)
+    n, src, dst = gen.rmat(13, 1 << 17, 42)
+    sources = gen.seeded_sources(n, src, dst, 200, 2)
+    g = make(n, src, dst)
+    bc = rg.betweenness_unweighted_array(g, sources=sources)
+    depth, _, _, _ = rg.bc_paths(g, sources, max_levels=64)
+    reached = depth != 0xFFFFFFFF
+    checksum = float(np.where(reached & (depth > 0), depth.astype(np.float64) - 1.0, 0.0).sum())
+    assert abs(bc.sum() - checksum) <= 1e-9 * checksum
+    rg.destroy(g)
+
+
+# --- PageRank parity ------------------------------------------------------------
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_pagerank_fixed_iterations(precision):
+    n, src, dst = gen.erdos_renyi_directed(10_000, 8.0 / 9_999, gen.SEED_BASE + 1)
+    got, ref, it_ref, info = check_pr(n, src, dst, iters=60, tol=0.0, precision=precision)
+    assert info["iterations"] == it_ref == 60
+    l1 = np.abs(got - ref).sum()
+    assert l1 <= (PR_L1 if precision == 32 else 1e-12), l1
+
+
+def test_pagerank_tolerance_stop_and_residual():
+    n, src, dst = gen.rmat(14, 1 << 18, gen.SEED_BASE + 7)
+    got, ref, it_ref, info = check_pr(n, src, dst, iters=100, tol=1e-6)
+    assert info["iterations"] == it_ref
+    assert np.abs(got - ref).sum() <= PR_L1
+    assert info["last_l1"] < 1e-6
+    # a-posteriori bound from the FP64 residual of the returned vector
+    assert info["error_bound_l1"] < 1e-4
+    assert abs(got.sum() - 1.0) < 1e-5
+
+
+def test_pagerank_hub_rows_and_wiki_vote():
+    n, src, dst = graphs.wiki_vote()
+    got, ref, it_ref, info = check_pr(n, src, dst, iters=100, tol=1e-4)
+    assert info["iterations"] == it_ref and np.abs(got - ref).sum() <= PR_L1
+    # rows far beyond one segment (4096 predecessors)
+    rng = np.random.default_rng(4)
+    src = np.concatenate([rng.integers(0, 20000, 30000), rng.integers(0, 20000, 40000)]).astype(np.uint32)
+    dst = np.concatenate([np.full(30000, 5), rng.integers(0, 20000, 40000)]).astype(np.uint32)
+    got, ref, it_ref, info = check_pr(20000, src, dst, iters=40, tol=0.0)
+    assert np.abs(got - ref).sum() <= PR_L1
+
+
+def test_pagerank_max_iterations_zero_and_damping():
+    n, src, dst = gen.erdos_renyi_directed(500, 0.01, 3)
+    got, ref, it_ref, info = check_pr(n, src, dst, iters=0)
+    assert info["iterations"] == 0 and np.allclose(got, 1.0 / n, rtol=1e-7)
+    got, ref, it_ref, info = check_pr(n, src, dst, iters=100, tol=0.0, damping=0.99, precision=64)
+    assert np.abs(got - ref).sum() <= 1e-12

@@ -1,0 +1,142 @@
+"""ctypes binding of libzog_hip.so (the C-ABI declared in include/zog_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C zog_amd/csrc``).  There is no fallback: if the shared library is
+missing, importing the product path raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzog_hip.so")
+
+ZG_OK = 0
+ZG_ERR_INVALID_ARG = -1
+ZG_ERR_NODE_NOT_FOUND = -2
+ZG_ERR_OOM = -3
+ZG_ERR_GPU_OOM = -4
+ZG_ERR_HIP = -5
+ZG_ERR_NO_DEVICE = -6
+ZG_ERR_TOO_LARGE = -7
+ZG_ERR_INVALID_HANDLE = -8
+
+ZG_PR_FP32 = 32
+ZG_PR_FP64 = 64
+ZG_PR_SKIP_RESIDUAL = 0x100
+
+# Every symbol include/zog_hip.h declares.
+EXPORTED = (
+    "zg_graph_create",
+    "zg_graph_destroy",
+    "zg_graph_info",
+    "zg_bc_unweighted",
+    "zg_bc_unweighted_device",
+    "zg_pagerank",
+    "zg_pagerank_device",
+    "zg_bc_paths",
+    "zg_set_batch_groups",
+    "zg_set_profiling",
+    "zg_get_profile",
+    "zg_strerror",
+)
+
+PROF_SLOTS = ("bc_forward", "bc_finalize", "bc_backward", "bc_hub", "bc_setup", "pr_iter")
+
+
+class ZgBcInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_sources", ctypes.c_uint64),
+        ("arcs_traversed", ctypes.c_uint64),
+        ("nodes_reached", ctypes.c_uint64),
+        ("max_levels", ctypes.c_uint32),
+        ("batches", ctypes.c_uint32),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+
+class ZgPrInfo(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint64),
+        ("last_l1", ctypes.c_double),
+        ("residual_l1", ctypes.c_double),
+        ("error_bound_l1", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class ZgProfile(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 8), ("launches", ctypes.c_uint64 * 8)]
+
+
+class ZogError(RuntimeError):
+    """A non-zero status from the C-ABI (the NIF would raise it to Elixir)."""
+
+    def __init__(self, code: int, where: str = ""):
+        self.code = code
+        msg = strerror(code)
+        super().__init__(f"{where}: {msg} ({code})" if where else f"{msg} ({code})")
+
+
+class NodeNotFoundError(ZogError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libzog_hip.so; raises OSError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    L.zg_graph_create.argtypes = [ctypes.c_uint32, P, P, P, ctypes.c_uint64, ctypes.POINTER(P)]
+    L.zg_graph_create.restype = ctypes.c_int
+    L.zg_graph_destroy.argtypes = [P]
+    L.zg_graph_destroy.restype = None
+    L.zg_graph_info.argtypes = [P, P, P, P]
+    L.zg_graph_info.restype = ctypes.c_int
+    L.zg_bc_unweighted.argtypes = [P, P, ctypes.c_uint64, P, ctypes.POINTER(ZgBcInfo)]
+    L.zg_bc_unweighted.restype = ctypes.c_int
+    L.zg_bc_unweighted_device.argtypes = [P, P, ctypes.c_uint64, P, P, ctypes.POINTER(ZgBcInfo)]
+    L.zg_bc_unweighted_device.restype = ctypes.c_int
+    L.zg_pagerank.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int, P,
+                              ctypes.POINTER(ZgPrInfo)]
+    L.zg_pagerank.restype = ctypes.c_int
+    L.zg_pagerank_device.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                     P, P, ctypes.POINTER(ZgPrInfo)]
+    L.zg_pagerank_device.restype = ctypes.c_int
+    L.zg_bc_paths.argtypes = [P, P, ctypes.c_uint64, P, P, P, ctypes.c_uint32, P]
+    L.zg_bc_paths.restype = ctypes.c_int
+    L.zg_set_batch_groups.argtypes = [P, ctypes.c_int]
+    L.zg_set_batch_groups.restype = ctypes.c_int
+    L.zg_set_profiling.argtypes = [P, ctypes.c_int]
+    L.zg_set_profiling.restype = ctypes.c_int
+    L.zg_get_profile.argtypes = [P, ctypes.POINTER(ZgProfile)]
+    L.zg_get_profile.restype = ctypes.c_int
+    L.zg_strerror.argtypes = [ctypes.c_int]
+    L.zg_strerror.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def strerror(code: int) -> str:
+    try:
+        return lib().zg_strerror(code).decode()
+    except OSError:
+        return f"status {code}"
+
+
+def check(code: int, where: str = "") -> None:
+    if code == ZG_OK:
+        return
+    if code == ZG_ERR_NODE_NOT_FOUND:
+        raise NodeNotFoundError(code, where)
+    raise ZogError(code, where)

@@ -1,0 +1,774 @@
+// Unweighted Brandes betweenness as a batched, level-synchronous sweep.
+//
+// Reference: priv/zog/src/centrality.zig:694-798 (ArrayGraph branch).  Per
+// source s: BFS where an arc v->w with dist[w] == dist[v] + 1 adds sigma[v]
+// to sigma[w] (once per parallel arc; self-loops never qualify), then in
+// reverse BFS order delta[u] += (sigma[u]/sigma[w]) * (1 + delta[w]) over the
+// predecessors u of w, and bc[w] += delta[w] for w != s.
+//
+// gfx950 design.  Sources are processed 64 at a time as a *group*: lane l of
+// every wavefront owns source slot l, and per-vertex state is stored as a
+// 64-wide row, so one wavefront reads one vertex's state for all 64 sources
+// with a single coalesced 512-byte access.  A batch sweeps up to
+// ZG_MAX_GROUPS groups in the same launches (blockIdx.y = group).
+//
+// Forward (level d -> d+1), pull over predecessor chunks: for a candidate
+// vertex v with unvisited lanes U, every predecessor u whose frontier mask
+// cur[u] meets U contributes sigma[u][l] for the lanes l in cur[u] & U.  The
+// lanes load 64 predecessors and their masks at once; a ballot picks the ones
+// that contribute, and each contributes one coalesced row read.  New lanes
+// are published in the next-frontier mask; a finalize kernel folds them into
+// the visited mask, appends the vertex's successor chunks (with the lane mask)
+// to the level-(d+1) work list using a wave-aggregated prefix sum, and
+// compacts the candidate list.
+//
+// Backward (level d, from the deepest level up): for an entry (v, lanes at
+// depth d), delta_l(v) = sigma_l(v) * sum_{w in succ(v)} c_l(w), where
+// c_l(w) = (1 + delta_l(w)) / sigma_l(w) is published when w's level is done
+// and is still 0 (cleared at discovery) for lanes where w is not at d+1: an
+// arc v->w out of a reached v has depth(w) <= depth(v) + 1, so c_l(w) != 0
+// exactly when w is a BFS child of v for source l.  No depth compare, no
+// atomics for ordinary vertices, and the per-lane bc contributions are summed
+// across the wavefront before one read-modify-write per (vertex, level).
+//
+// Path counts are FP64 (exact integers below 2^53, equal to the reference's
+// u64 counts there) with a per-(source, level) power-of-two shift k_L taken
+// only when a level's largest exponent passes ZG_SHIFT_T, so counts on
+// high-diameter graphs never overflow; every k is 0 until then.
+#include "common.h"
+
+struct GroupRange {
+    uint32_t off[ZG_MAX_GROUPS];
+    uint32_t cnt[ZG_MAX_GROUPS];
+};
+
+struct BcDev {
+    uint32_t V;
+    uint32_t G;                 // groups in this batch (gridDim.y)
+    uint32_t n_ich;
+    const uint32_t *in_ptr, *in_col, *out_ptr, *out_col;
+    const uint32_t *ich_v, *ich_e0, *ich_e1;
+    const uint32_t *och_first, *och_v, *och_e0, *och_e1;
+    const uint32_t *ohub_idx;
+    const uint8_t *live;
+    double *sig, *cc, *hubacc, *bcg;
+    uint32_t *dep;
+    uint64_t *vis, *fr;         // fr: [3][Gcap][V]
+    uint32_t *cand;             // [2][Gcap][n_ich]
+    uint32_t *flist;
+    uint64_t *fmask;
+    uint32_t *hlist;
+    uint64_t *hmask;
+    int *emax;                  // [L][Gcap][64]
+    const uint32_t *src;        // [Gcap][64]
+    zg_bc_ws::Counters *cnt;
+    size_t fcap, hcap;
+    uint32_t Gcap, n_ohub;
+};
+
+#define ROW(g, v) (((size_t)(g) * a.V + (v)) * 64)
+#define MASK(g, v) ((size_t)(g) * a.V + (v))
+#define EMAX(L, g, l) ((((size_t)(L)) * a.Gcap + (g)) * 64 + (l))
+
+__device__ __forceinline__ uint64_t *fr_buf(const BcDev &a, int k)
+{
+    return a.fr + (size_t)k * a.Gcap * a.V;
+}
+
+// Visited mask of every vertex starts as the group's unused lanes.
+__global__ void bc_init_vis_kernel(BcDev a, GroupRange valid_lanes)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t g = blockIdx.y;
+    if (v >= a.V) return;
+    uint32_t nl = valid_lanes.cnt[g];
+    uint64_t used = nl >= 64 ? ~0ull : ((1ull << nl) - 1);
+    a.vis[MASK(g, v)] = ~used;
+}
+
+// Vertices with several predecessor chunks accumulate sigma with atomics:
+// clear their rows first.
+__global__ void bc_zero_hub_rows_kernel(BcDev a, const uint32_t *rows)
+{
+    uint32_t v = rows[blockIdx.x];
+    a.sig[ROW(blockIdx.y, v) + threadIdx.x] = 0.0;
+}
+
+// Level 0: every lane's source gets depth 0, sigma 1 and joins frontier 0.
+__global__ void bc_source_kernel(BcDev a)
+{
+    const uint32_t g = blockIdx.y, l = threadIdx.x;
+    if (l == 0) a.cnt[g].ncand[0] = a.n_ich;
+    uint32_t s = a.src[g * 64 + l];
+    unsigned long long arcs = 0, reached = 0;
+    if (s != ZG_NONE) {
+        if (s >= a.V || !a.live[s]) {
+            atomicOr(&a.cnt[g].err, ZG_BC_ERR_SOURCE);
+        } else {
+            const uint64_t bit = 1ull << l;
+            atomicOr((unsigned long long *)&a.vis[MASK(g, s)], (unsigned long long)bit);
+            atomicOr((unsigned long long *)&fr_buf(a, 0)[MASK(g, s)], (unsigned long long)bit);
+            a.sig[ROW(g, s) + l] = 1.0;
+            a.cc[ROW(g, s) + l] = 0.0;
+            if (a.dep) a.dep[ROW(g, s) + l] = 0;
+            uint32_t c0 = a.och_first[s], k = a.och_first[s + 1] - c0;
+            uint32_t pos = atomicAdd(&a.cnt[g].fcount, k);
+            for (uint32_t q = 0; q < k; q++) {
+                if (pos + q < a.fcap) {
+                    a.flist[g * a.fcap + pos + q] = c0 + q;
+                    a.fmask[g * a.fcap + pos + q] = bit;
+                } else {
+                    atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+                }
+            }
+            arcs = a.out_ptr[s + 1] - a.out_ptr[s];
+            reached = 1;
+        }
+    }
+    arcs = zg_wave_sum_u64(arcs);
+    reached = zg_wave_sum_u64(reached);
+    if (l == 0) {
+        atomicAdd(&a.cnt[g].arcs, arcs);
+        atomicAdd(&a.cnt[g].reached, reached);
+    }
+}
+
+// Forward level d: discover level d+1 by pulling over predecessor chunks.
+__global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d)
+{
+    const uint32_t g = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[g].ncand[(d + 1) & 1] = 0;
+    const uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t nc = a.cnt[g].ncand[d & 1];
+    if (wid >= nc) return;
+    const uint32_t ch = d == 0 ? wid : a.cand[((size_t)(d & 1) * a.Gcap + g) * a.n_ich + wid];
+    const uint32_t v = a.ich_v[ch], e0 = a.ich_e0[ch], e1 = a.ich_e1[ch];
+    const uint64_t unvis = ~a.vis[MASK(g, v)];
+    if (unvis == 0) return;
+    const uint64_t *cur = fr_buf(a, d % 3) + (size_t)g * a.V;
+    const double *sg = a.sig + ROW(g, 0);
+    double acc = 0.0;
+    uint64_t m = 0;
+    for (uint32_t base = e0; base < e1; base += 64) {
+        const uint32_t e = base + lane;
+        uint32_t u = 0;
+        uint64_t f = 0;
+        if (e < e1) {
+            u = a.in_col[e];
+            f = cur[u] & unvis;
+        }
+        uint64_t act = __ballot(f != 0);
+        while (act) {
+            const int j = __builtin_ctzll(act);
+            act &= act - 1;
+            const uint32_t uj = zg_readlane32(u, j);
+            const uint64_t fj = zg_readlane64(f, j);
+            m |= fj;
+            if ((fj >> lane) & 1) acc += sg[(size_t)uj * 64 + lane];
+        }
+    }
+    if (m == 0) return;
+    const bool mine = (m >> lane) & 1;
+    const bool multi = (a.in_ptr[v + 1] - a.in_ptr[v]) > ZG_CHUNK;
+    const size_t r = ROW(g, v) + lane;
+    if (mine) {
+        const int k = a.emax[EMAX(d, g, lane)];
+        const double val = ldexp(acc, -k);
+        if (multi) {
+            atomicAdd(&a.sig[r], val);
+        } else {
+            a.sig[r] = val;
+            const int ex = ilogb(val);
+            if (ex > ZG_SHIFT_T) atomicMax(&a.emax[EMAX(d + 1, g, lane)], ex);
+        }
+        a.cc[r] = 0.0;
+        if (a.dep) a.dep[r] = (uint32_t)(d + 1);
+    }
+    if (lane == 0) {
+        uint64_t *nxt = fr_buf(a, (d + 1) % 3) + MASK(g, v);
+        if (multi) atomicOr((unsigned long long *)nxt, (unsigned long long)m);
+        else *nxt = m;
+    }
+}
+
+// Finalize level d: fold level d+1 into the visited masks, append its work
+// lists, compact the candidates, clear the frontier buffer of level d-1.
+__global__ __launch_bounds__(256) void bc_finalize_kernel(BcDev a, int d, GroupRange stale)
+{
+    const uint32_t g = blockIdx.y;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    if (i < stale.cnt[g]) {
+        uint32_t oc = a.flist[g * a.fcap + stale.off[g] + i];
+        fr_buf(a, (d + 2) % 3)[MASK(g, a.och_v[oc])] = 0;
+    }
+    const uint32_t nc = a.cnt[g].ncand[d & 1];
+    uint32_t v = 0, ch = 0, kf = 0, kh = 0, kc = 0;
+    uint64_t nm = 0;
+    bool disc = false;
+    if (i < nc) {
+        ch = d == 0 ? i : a.cand[((size_t)(d & 1) * a.Gcap + g) * a.n_ich + i];
+        v = a.ich_v[ch];
+        nm = fr_buf(a, (d + 1) % 3)[MASK(g, v)];
+        const uint64_t vo = a.vis[MASK(g, v)];
+        disc = nm != 0 && a.ich_e0[ch] == a.in_ptr[v];
+        if (disc) {
+            a.vis[MASK(g, v)] = vo | nm;
+            kf = a.och_first[v + 1] - a.och_first[v];
+            kh = kf > 1 ? 1 : 0;
+        }
+        kc = (vo | nm) != ~0ull ? 1 : 0;
+    }
+    // wave-aggregated appends
+    const uint32_t sf = zg_wave_incl_scan(kf, lane);
+    const uint32_t sh = zg_wave_incl_scan(kh, lane);
+    const uint32_t sc = zg_wave_incl_scan(kc, lane);
+    const uint32_t tf = zg_readlane32(sf, 63), th = zg_readlane32(sh, 63), tc = zg_readlane32(sc, 63);
+    uint32_t bf = 0, bh = 0, bcand = 0;
+    if (lane == 63) {
+        if (tf) bf = atomicAdd(&a.cnt[g].fcount, tf);
+        if (th) bh = atomicAdd(&a.cnt[g].hcount, th);
+        if (tc) bcand = atomicAdd(&a.cnt[g].ncand[(d + 1) & 1], tc);
+    }
+    bf = zg_readlane32(bf, 63);
+    bh = zg_readlane32(bh, 63);
+    bcand = zg_readlane32(bcand, 63);
+    if (kf) {
+        const uint32_t pos = bf + sf - kf, c0 = a.och_first[v];
+        for (uint32_t q = 0; q < kf; q++) {
+            if (pos + q < a.fcap) {
+                a.flist[g * a.fcap + pos + q] = c0 + q;
+                a.fmask[g * a.fcap + pos + q] = nm;
+            } else {
+                atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+            }
+        }
+    }
+    if (kh) {
+        const uint32_t pos = bh + sh - 1;
+        if (pos < a.hcap) {
+            a.hlist[g * a.hcap + pos] = v;
+            a.hmask[g * a.hcap + pos] = nm;
+        } else {
+            atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+        }
+    }
+    if (kc) a.cand[((size_t)((d + 1) & 1) * a.Gcap + g) * a.n_ich + bcand + sc - 1] = ch;
+    // per-source work statistics
+    unsigned long long arcs = 0, reached = 0;
+    if (disc) {
+        const unsigned long long pc = __popcll(nm);
+        arcs = pc * (a.out_ptr[v + 1] - a.out_ptr[v]);
+        reached = pc;
+        // vertices summed by atomics: their level-(d+1) exponent is known only now
+        if (a.in_ptr[v + 1] - a.in_ptr[v] > ZG_CHUNK) {
+            uint64_t mm = nm;
+            while (mm) {
+                const int l = __builtin_ctzll(mm);
+                mm &= mm - 1;
+                const int ex = ilogb(a.sig[ROW(g, v) + l]);
+                if (ex > ZG_SHIFT_T) atomicMax(&a.emax[EMAX(d + 1, g, l)], ex);
+            }
+        }
+    }
+    arcs = zg_wave_sum_u64(arcs);
+    reached = zg_wave_sum_u64(reached);
+    if (lane == 0 && (arcs | reached)) {
+        atomicAdd(&a.cnt[g].arcs, arcs);
+        atomicAdd(&a.cnt[g].reached, reached);
+    }
+}
+
+// delta / c / bc for lanes `active` of vertex v at level d, given S = sum of
+// c over v's successors.
+__device__ __forceinline__ void bc_finish(const BcDev &a, uint32_t g, uint32_t v, int lane, bool active,
+                                          double S, int d)
+{
+    double contrib = 0.0;
+    if (active) {
+        const size_t r = ROW(g, v) + lane;
+        const double sv = a.sig[r];
+        const int k = a.emax[EMAX(d, g, lane)];
+        const double delta = sv * ldexp(S, -k);
+        a.cc[r] = (1.0 + delta) / sv;
+        if (a.src[g * 64 + lane] != v) contrib = delta;
+    }
+    contrib = zg_wave_sum(contrib);
+    if (lane == 0) a.bcg[(size_t)g * a.V + v] += contrib;
+}
+
+// Backward level d over the level's successor-chunk entries.
+__global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupRange R)
+{
+    const uint32_t g = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= R.cnt[g]) return;
+    const size_t idx = g * a.fcap + R.off[g] + wid;
+    const uint32_t oc = a.flist[idx];
+    const uint64_t mask = a.fmask[idx];
+    const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
+    const bool active = (mask >> lane) & 1;
+    const double *cg = a.cc + ROW(g, 0) + lane;
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
+    for (uint32_t base = e0; base < e1; base += 64) {
+        const uint32_t e = base + lane;
+        const uint32_t w = e < e1 ? a.out_col[e] : 0;
+        const int n = (int)min(64u, e1 - base);
+        int j = 0;
+        for (; j + 4 <= n; j += 4) {
+            const uint32_t w0 = zg_readlane32(w, j), w1 = zg_readlane32(w, j + 1);
+            const uint32_t w2 = zg_readlane32(w, j + 2), w3 = zg_readlane32(w, j + 3);
+            if (active) {
+                S0 += cg[(size_t)w0 * 64];
+                S1 += cg[(size_t)w1 * 64];
+                S2 += cg[(size_t)w2 * 64];
+                S3 += cg[(size_t)w3 * 64];
+            }
+        }
+        for (; j < n; j++) {
+            const uint32_t wj = zg_readlane32(w, j);
+            if (active) S0 += cg[(size_t)wj * 64];
+        }
+    }
+    const double S = (S0 + S1) + (S2 + S3);
+    const uint32_t hub = a.ohub_idx[v];
+    if (hub != ZG_NONE) {
+        if (active) atomicAdd(&a.hubacc[((size_t)g * a.n_ohub + hub) * 64 + lane], S);
+        return;
+    }
+    bc_finish(a, g, v, lane, active, S, d);
+}
+
+// Backward level d for vertices with several successor chunks.
+__global__ __launch_bounds__(256) void bc_hub_kernel(BcDev a, int d, GroupRange R)
+{
+    const uint32_t g = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= R.cnt[g]) return;
+    const size_t idx = g * a.hcap + R.off[g] + wid;
+    const uint32_t v = a.hlist[idx];
+    const uint64_t mask = a.hmask[idx];
+    const bool active = (mask >> lane) & 1;
+    double S = 0.0;
+    if (active) {
+        double *p = &a.hubacc[((size_t)g * a.n_ohub + a.ohub_idx[v]) * 64 + lane];
+        S = *p;
+        *p = 0.0;
+    }
+    bc_finish(a, g, v, lane, active, S, d);
+}
+
+// out[v] = sum over groups of the group accumulators (fixed order).
+__global__ void bc_reduce_kernel(uint32_t V, uint32_t G, const double *bcg, const uint8_t *live, double *out)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double s = 0.0;
+    for (uint32_t g = 0; g < G; g++) s += bcg[(size_t)g * V + v];
+    out[v] = live[v] ? s : 0.0;
+}
+
+// Depth / sigma export of one group: [lane][v] planes.
+__global__ void bc_export_kernel(uint32_t V, uint32_t g, const uint32_t *dep, const double *sig,
+                                 uint32_t *dep_out, double *sig_out)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t l = blockIdx.y;
+    if (v >= V) return;
+    size_t r = ((size_t)g * V + v) * 64 + l;
+    uint32_t dd = dep[r];
+    dep_out[(size_t)l * V + v] = dd;
+    sig_out[(size_t)l * V + v] = dd == ZG_NONE ? 0.0 : sig[r];
+}
+
+int zg_bc_ws_release(zg_graph *g)
+{
+    zg_bc_ws &w = g->bc;
+    zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep);
+    zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.cand); zg_dfree(w.flist); zg_dfree(w.fmask);
+    zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.src); zg_dfree(w.cnt);
+    if (w.h_cnt) (void)hipHostFree(w.h_cnt);
+    w.h_cnt = nullptr;
+    w.groups = 0;
+    w.lcap = 0;
+    w.with_depth = false;
+    return ZG_OK;
+}
+
+static int bc_ws_ensure(zg_graph *g, bool with_depth)
+{
+    zg_bc_ws &w = g->bc;
+    if (w.groups == g->groups && (w.with_depth || !with_depth)) return ZG_OK;
+    zg_bc_ws_release(g);
+    const size_t G = g->groups, V = g->V;
+    w.fcap = 64 * (size_t)g->n_och + 64 * (size_t)g->max_och_per_v;
+    w.hcap = 64 * (size_t)g->n_ohub + 64;
+    w.lcap = 256;
+    int rc = ZG_OK;
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, G * V * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, G * V * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.hubacc, G * (size_t)g->n_ohub * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.bcg, G * V);
+    if (rc == ZG_OK && with_depth) rc = zg_dmalloc(&w.dep, G * V * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cand, 2 * G * (size_t)g->n_ich);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.flist, G * w.fcap);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fmask, G * w.fcap);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.hlist, G * w.hcap);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.hmask, G * w.hcap);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.emax, (size_t)w.lcap * G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.src, G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cnt, G);
+    if (rc == ZG_OK) {
+        hipError_t e = hipHostMalloc((void **)&w.h_cnt, G * sizeof(zg_bc_ws::Counters), hipHostMallocDefault);
+        if (e != hipSuccess) rc = zg_hip_code(e);
+    }
+    if (rc == ZG_OK) {
+        hipError_t e = hipMemset(w.hubacc, 0, sizeof(double) * (G * (size_t)g->n_ohub * 64 ? G * (size_t)g->n_ohub * 64 : 1));
+        if (e == hipSuccess) e = hipMemset(w.emax, 0, sizeof(int) * (size_t)w.lcap * G * 64);
+        if (e != hipSuccess) rc = zg_hip_code(e);
+    }
+    if (rc != ZG_OK) {
+        zg_bc_ws_release(g);
+        return rc;
+    }
+    w.groups = (int)G;
+    w.with_depth = with_depth;
+    w.last_levels = w.lcap;
+    return ZG_OK;
+}
+
+// Grows the per-level shift table to hold level `need` (copies the prefix).
+static int bc_grow_levels(zg_graph *g, uint32_t need, hipStream_t s)
+{
+    zg_bc_ws &w = g->bc;
+    if (need < w.lcap) return ZG_OK;
+    uint32_t nl = w.lcap;
+    while (nl <= need) nl *= 2;
+    int *ne = nullptr;
+    const size_t row = (size_t)w.groups * 64;
+    ZG_TRY_RC(zg_dmalloc(&ne, (size_t)nl * row));
+    ZG_TRY(hipMemsetAsync(ne, 0, sizeof(int) * (size_t)nl * row, s));
+    ZG_TRY(hipMemcpyAsync(ne, w.emax, sizeof(int) * (size_t)w.lcap * row, hipMemcpyDeviceToDevice, s));
+    ZG_TRY(hipStreamSynchronize(s));
+    zg_dfree(w.emax);
+    w.emax = ne;
+    w.lcap = nl;
+    return ZG_OK;
+}
+
+static BcDev bc_dev(zg_graph *g, uint32_t groups_used)
+{
+    zg_bc_ws &w = g->bc;
+    BcDev a;
+    a.V = g->V;
+    a.G = groups_used;
+    a.Gcap = (uint32_t)w.groups;
+    a.n_ich = g->n_ich;
+    a.in_ptr = g->in_ptr; a.in_col = g->in_col; a.out_ptr = g->out_ptr; a.out_col = g->out_col;
+    a.ich_v = g->ich_v; a.ich_e0 = g->ich_e0; a.ich_e1 = g->ich_e1;
+    a.och_first = g->och_first; a.och_v = g->och_v; a.och_e0 = g->och_e0; a.och_e1 = g->och_e1;
+    a.ohub_idx = g->ohub_idx;
+    a.live = g->live;
+    a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep;
+    a.vis = w.vis; a.fr = w.fr; a.cand = w.cand;
+    a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
+    a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
+    a.fcap = w.fcap; a.hcap = w.hcap;
+    a.n_ohub = g->n_ohub;
+    return a;
+}
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Per-batch callback used by the depth/sigma export.
+typedef int (*bc_batch_hook)(zg_graph *g, hipStream_t s, uint64_t first, uint32_t n_in_batch,
+                             uint32_t groups_used, void *user);
+
+// Runs the batched sweep over d_sources[0..n) accumulating into the group
+// accumulators; the caller reduces them.
+static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStream_t s, zg_bc_info *inf,
+                    bc_batch_hook hook, void *user)
+{
+    zg_bc_ws &w = g->bc;
+    const uint32_t V = g->V;
+    const uint32_t Gcap = (uint32_t)w.groups;
+    ZG_TRY(hipMemsetAsync(w.bcg, 0, sizeof(double) * (size_t)Gcap * V, s));
+    std::vector<std::vector<uint32_t>> foff(Gcap), hoff(Gcap);
+    for (uint64_t b0 = 0; b0 < n; b0 += (uint64_t)Gcap * 64) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)Gcap * 64, n - b0);
+        const uint32_t G = cdiv(nb, 64);
+        BcDev a = bc_dev(g, G);
+        GroupRange valid;
+        memset(&valid, 0, sizeof(valid));
+        for (uint32_t q = 0; q < G; q++) valid.cnt[q] = std::min<uint32_t>(64, nb - 64 * q);
+        // batch reset
+        ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * Gcap * 64, s));
+        ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
+        ZG_TRY(hipMemsetAsync(w.fr, 0, sizeof(uint64_t) * 3 * (size_t)Gcap * V, s));
+        ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
+        ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
+        if (w.dep) ZG_TRY(hipMemsetAsync(w.dep, 0xFF, sizeof(uint32_t) * (size_t)Gcap * V * 64, s));
+        if (V) ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_init_vis_kernel, dim3(cdiv(V, 256), G), dim3(256), 0, s, a, valid);
+        if (g->n_inhub) ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_hub_rows_kernel, dim3(g->n_inhub, G), dim3(64), 0, s, a, g->inhub);
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_source_kernel, dim3(1, G), dim3(64), 0, s, a);
+        ZG_TRY(hipGetLastError());
+        // |F_0| per group = sum of successor-chunk counts of its sources:
+        // read back once with the level-0 counters below.
+        for (uint32_t q = 0; q < G; q++) {
+            foff[q].assign(1, 0);
+            hoff[q].assign(1, 0);
+        }
+        ZG_TRY(hipMemcpyAsync(w.h_cnt, w.cnt, sizeof(zg_bc_ws::Counters) * G, hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+        std::vector<uint32_t> ncand(G);
+        for (uint32_t q = 0; q < G; q++) {
+            if (w.h_cnt[q].err & ZG_BC_ERR_SOURCE) return ZG_ERR_NODE_NOT_FOUND;
+            foff[q].push_back(w.h_cnt[q].fcount);  // end of level 0
+            hoff[q].push_back(w.h_cnt[q].hcount);
+            ncand[q] = g->n_ich;
+        }
+        int d = 0;
+        for (;;) {
+            ZG_TRY_RC(bc_grow_levels(g, (uint32_t)d + 2, s));
+            a.emax = w.emax;
+            uint32_t mc = 0;
+            for (uint32_t q = 0; q < G; q++) mc = std::max(mc, ncand[q]);
+            if (mc) ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(cdiv(mc, 4), G), dim3(256), 0, s, a, d);
+            else ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(1, G), dim3(256), 0, s, a, d);
+            GroupRange stale;
+            memset(&stale, 0, sizeof(stale));
+            uint32_t ms = 0;
+            if (d >= 1)
+                for (uint32_t q = 0; q < G; q++) {
+                    stale.off[q] = foff[q][d - 1];
+                    stale.cnt[q] = foff[q][d] - foff[q][d - 1];
+                    ms = std::max(ms, stale.cnt[q]);
+                }
+            uint32_t mf = std::max(mc, ms);
+            if (mf) ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_kernel, dim3(cdiv(mf, 256), G), dim3(256), 0, s, a, d, stale);
+            ZG_TRY(hipGetLastError());
+            ZG_TRY(hipMemcpyAsync(w.h_cnt, w.cnt, sizeof(zg_bc_ws::Counters) * G, hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+            bool any = false;
+            for (uint32_t q = 0; q < G; q++) {
+                if (w.h_cnt[q].err) return ZG_ERR_HIP;  // work-list capacity exceeded
+                uint32_t fc = w.h_cnt[q].fcount;
+                if (fc > foff[q].back()) any = true;
+                foff[q].push_back(fc);
+                hoff[q].push_back(w.h_cnt[q].hcount);
+                ncand[q] = w.h_cnt[q].ncand[(d + 1) & 1];
+            }
+            if (!any) break;
+            d++;
+        }
+        const int maxd = d;  // deepest non-empty level
+        for (uint32_t q = 0; q < G; q++) {
+            inf->arcs_traversed += w.h_cnt[q].arcs;
+            inf->nodes_reached += w.h_cnt[q].reached;
+        }
+        inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)maxd + 1);
+        w.last_levels = (uint32_t)maxd + 2;
+        for (int L = maxd; L >= 1; L--) {
+            GroupRange R, H;
+            memset(&R, 0, sizeof(R));
+            memset(&H, 0, sizeof(H));
+            uint32_t mr = 0, mh = 0;
+            for (uint32_t q = 0; q < G; q++) {
+                R.off[q] = foff[q][L];
+                R.cnt[q] = foff[q][L + 1] - foff[q][L];
+                H.off[q] = hoff[q][L];
+                H.cnt[q] = hoff[q][L + 1] - hoff[q][L];
+                mr = std::max(mr, R.cnt[q]);
+                mh = std::max(mh, H.cnt[q]);
+            }
+            if (mr) ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, bc_backward_kernel, dim3(cdiv(mr, 4), G), dim3(256), 0, s, a, L, R);
+            if (mh) ZG_PROF_LAUNCH(ZG_PROF_BC_HUB, bc_hub_kernel, dim3(cdiv(mh, 4), G), dim3(256), 0, s, a, L, H);
+        }
+        ZG_TRY(hipGetLastError());
+        inf->batches++;
+        if (hook) ZG_TRY_RC(hook(g, s, b0, nb, G, user));
+    }
+    inf->n_sources = n;
+    return ZG_OK;
+}
+
+static int bc_all_live_sources(zg_graph *g, std::vector<uint32_t> &out)
+{
+    out.clear();
+    out.reserve(g->n_live);
+    for (uint32_t v = 0; v < g->V; v++)
+        if (g->h_live[v]) out.push_back(v);
+    return ZG_OK;
+}
+
+static int zg_bc_device_impl(zg_graph *g, const uint32_t *d_sources, uint64_t n_sources, double *d_out,
+                             hipStream_t s, zg_bc_info *info)
+{
+    zg_bc_info inf;
+    memset(&inf, 0, sizeof(inf));
+    const uint32_t V = g->V;
+    uint32_t *d_tmp = nullptr;
+    if (!d_sources) {
+        std::vector<uint32_t> all;
+        bc_all_live_sources(g, all);
+        n_sources = all.size();
+        ZG_TRY_RC(zg_dmalloc(&d_tmp, all.size()));
+        if (!all.empty())
+            ZG_TRY(hipMemcpyAsync(d_tmp, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice, s));
+        ZG_TRY(hipStreamSynchronize(s));
+        d_sources = d_tmp;
+    }
+    int rc = bc_ws_ensure(g, false);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e0, s));
+    if (rc == ZG_OK) rc = bc_sweep(g, d_sources, n_sources, s, &inf, nullptr, nullptr);
+    if (rc == ZG_OK && V) {
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reduce_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V,
+                       (uint32_t)g->bc.groups, g->bc.bcg, g->live, d_out);
+        rc = zg_hip_code(hipGetLastError());
+    }
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e1, s));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventSynchronize(e1));
+    if (rc == ZG_OK) {
+        float ms = 0.f;
+        rc = zg_hip_code(hipEventElapsedTime(&ms, e0, e1));
+        inf.kernel_ms = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    if (rc == ZG_OK) rc = zg_prof_collect(g, s);
+    zg_dfree(d_tmp);
+    if (rc == ZG_OK && info) *info = inf;
+    return rc;
+}
+
+extern "C" int zg_bc_unweighted_device(zg_graph *g, const uint32_t *d_sources, uint64_t n_sources,
+                                       double *d_out_bc, void *stream, zg_bc_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!d_out_bc && g->V) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    return zg_bc_device_impl(g, d_sources, n_sources, d_out_bc, s, info);
+}
+
+extern "C" int zg_bc_unweighted(zg_graph *g, const uint32_t *sources, uint64_t n_sources, double *out_bc,
+                                zg_bc_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!out_bc && g->V) return ZG_ERR_INVALID_ARG;
+    if (sources && n_sources) {
+        for (uint64_t i = 0; i < n_sources; i++)
+            if (sources[i] >= g->V || !g->h_live[sources[i]]) return ZG_ERR_NODE_NOT_FOUND;
+    }
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    uint32_t *d_src = nullptr;
+    double *d_out = nullptr;
+    int rc = zg_dmalloc(&d_out, g->V);
+    if (rc == ZG_OK && sources) {
+        rc = zg_dmalloc(&d_src, n_sources);
+        if (rc == ZG_OK && n_sources)
+            rc = zg_hip_code(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, s));
+    }
+    if (rc == ZG_OK) rc = zg_bc_device_impl(g, sources ? d_src : nullptr, sources ? n_sources : 0, d_out, s, info);
+    if (rc == ZG_OK && g->V)
+        rc = zg_hip_code(hipMemcpy(out_bc, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
+    zg_dfree(d_src);
+    zg_dfree(d_out);
+    return rc;
+}
+
+struct PathsCtx {
+    uint32_t *depth_out;
+    double *sigma_out;
+    int32_t *shift_out;
+    uint32_t max_levels;
+    uint32_t *levels_out;
+    uint32_t *d_dep;
+    double *d_sig;
+    std::vector<int> h_emax;
+};
+
+static int bc_paths_hook(zg_graph *g, hipStream_t s, uint64_t first, uint32_t nb, uint32_t G, void *user)
+{
+    PathsCtx *c = (PathsCtx *)user;
+    const uint32_t V = g->V;
+    zg_bc_ws &w = g->bc;
+    const uint32_t lv = w.last_levels;  // levels of this batch + 1
+    c->h_emax.resize((size_t)lv * w.groups * 64);
+    ZG_TRY(hipMemcpyAsync(c->h_emax.data(), w.emax, sizeof(int) * c->h_emax.size(), hipMemcpyDeviceToHost, s));
+    for (uint32_t q = 0; q < G; q++) {
+        const uint32_t nl = std::min<uint32_t>(64, nb - 64 * q);
+        if (V) {
+            hipLaunchKernelGGL(bc_export_kernel, dim3(cdiv(V, 256), 64), dim3(256), 0, s, V, q, w.dep, w.sig,
+                               c->d_dep, c->d_sig);
+            ZG_TRY(hipGetLastError());
+        }
+        const uint64_t i0 = first + 64ull * q;
+        ZG_TRY(hipMemcpyAsync(c->depth_out + i0 * V, c->d_dep, sizeof(uint32_t) * (size_t)nl * V,
+                              hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipMemcpyAsync(c->sigma_out + i0 * V, c->d_sig, sizeof(double) * (size_t)nl * V,
+                              hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+        for (uint32_t l = 0; l < nl; l++) {
+            const uint32_t *dp = c->depth_out + (i0 + l) * V;
+            uint32_t mx = 0;
+            for (uint32_t v = 0; v < V; v++)
+                if (dp[v] != ZG_NONE && dp[v] + 1 > mx) mx = dp[v] + 1;
+            c->levels_out[i0 + l] = mx;
+            if (c->shift_out) {
+                if (mx > c->max_levels) return ZG_ERR_INVALID_ARG;
+                for (uint32_t L = 0; L < mx; L++)
+                    c->shift_out[(i0 + l) * c->max_levels + L] =
+                        L < lv ? c->h_emax[((size_t)L * w.groups + q) * 64 + l] : 0;
+            }
+        }
+    }
+    return ZG_OK;
+}
+
+extern "C" int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sources, uint32_t *depth_out,
+                           double *sigma_out, int32_t *shift_out, uint32_t max_levels, uint32_t *levels_out)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!sources || !depth_out || !sigma_out || !levels_out) return ZG_ERR_INVALID_ARG;
+    for (uint64_t i = 0; i < n_sources; i++)
+        if (sources[i] >= g->V || !g->h_live[sources[i]]) return ZG_ERR_NODE_NOT_FOUND;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    ZG_TRY_RC(bc_ws_ensure(g, true));
+    PathsCtx c;
+    c.depth_out = depth_out;
+    c.sigma_out = sigma_out;
+    c.shift_out = shift_out;
+    c.max_levels = max_levels;
+    c.levels_out = levels_out;
+    c.d_dep = nullptr;
+    c.d_sig = nullptr;
+    uint32_t *d_src = nullptr;
+    int rc = zg_dmalloc(&c.d_dep, (size_t)64 * g->V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&c.d_sig, (size_t)64 * g->V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&d_src, n_sources);
+    if (rc == ZG_OK && n_sources)
+        rc = zg_hip_code(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, s));
+    zg_bc_info inf;
+    memset(&inf, 0, sizeof(inf));
+    if (rc == ZG_OK) rc = bc_sweep(g, d_src, n_sources, s, &inf, bc_paths_hook, &c);
+    (void)hipStreamSynchronize(s);
+    zg_dfree(c.d_dep);
+    zg_dfree(c.d_sig);
+    zg_dfree(d_src);
+    return rc;
+}

@@ -1,0 +1,219 @@
+// Shared definitions of the gfx950 backend: the resident graph handle, HIP
+// error plumbing and wave-level helpers (64-lane wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/zog_hip.h"
+
+#define ZG_NONE 0xFFFFFFFFu
+
+// A vertex's predecessor / successor list is cut into chunks of at most this
+// many arcs; one wavefront sweeps one chunk, so a hub is shared by several
+// wavefronts (hub split).
+#define ZG_CHUNK 256
+
+// Path counts are kept as FP64 with a per-(source, level) power-of-two shift:
+// once the largest binary exponent at a level passes this threshold, the
+// next level's counts are stored divided by 2^exponent (exact).
+#define ZG_SHIFT_T 512
+
+#define ZG_MAX_GROUPS 16
+
+// Error bits a Brandes kernel reports to the host.
+#define ZG_BC_ERR_SOURCE 1u     // a source id is out of range or not live
+#define ZG_BC_ERR_CAPACITY 2u   // a work list would overflow its buffer
+
+// Returns a ZG_ERR_* code for a HIP error.
+static inline int zg_hip_code(hipError_t e)
+{
+    if (e == hipSuccess) return ZG_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return ZG_ERR_GPU_OOM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return ZG_ERR_NO_DEVICE;
+    return ZG_ERR_HIP;
+}
+
+#define ZG_TRY(expr)                                   \
+    do {                                               \
+        hipError_t _e = (expr);                        \
+        if (_e != hipSuccess) return zg_hip_code(_e);  \
+    } while (0)
+
+#define ZG_TRY_RC(expr)                 \
+    do {                                \
+        int _rc = (expr);               \
+        if (_rc != ZG_OK) return _rc;   \
+    } while (0)
+
+template <typename T>
+static inline int zg_dmalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return zg_hip_code(e);
+    }
+    return ZG_OK;
+}
+
+template <typename T>
+static inline void zg_dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Brandes batched-sweep workspace (sized for `groups` 64-source groups).
+// Layouts, per group g (lane = source slot within the group):
+//   sig[g][v][lane]  FP64 stored path counts (row of 64 = 512 B per vertex)
+//   cc [g][v][lane]  FP64 (1 + delta) / sigma of finished vertices
+//   vis[g][v], fr[k][g][v]  u64 lane masks (visited, frontier buffers)
+//   flist/fmask[g][i]       per-level successor-chunk work lists
+//   hlist/hmask[g][i]       per-level lists of multi-chunk vertices
+//   emax[L][g][lane]        largest exponent seen at level L (shift k_L)
+struct zg_bc_ws {
+    int groups = 0;
+    bool with_depth = false;
+    double *sig = nullptr, *cc = nullptr, *hubacc = nullptr, *bcg = nullptr;
+    uint32_t *dep = nullptr;
+    uint64_t *vis = nullptr, *fr = nullptr;
+    uint32_t *cand = nullptr;            // [2][G][n_ich]
+    uint32_t *flist = nullptr;           // [G][fcap]
+    uint64_t *fmask = nullptr;
+    uint32_t *hlist = nullptr;           // [G][hcap]
+    uint64_t *hmask = nullptr;
+    int *emax = nullptr;                 // [lcap][G][64]
+    uint32_t lcap = 0;
+    uint32_t *src = nullptr;             // [G][64]
+    struct Counters {
+        uint32_t fcount, hcount, ncand[2];
+        unsigned long long arcs, reached;
+        uint32_t err;   // ZG_BC_ERR_* bits
+        uint32_t pad;
+    };
+    Counters *cnt = nullptr;             // [G]
+    Counters *h_cnt = nullptr;           // pinned host mirror
+    size_t fcap = 0, hcap = 0;
+    uint32_t last_levels = 0;            // levels used by the previous batch
+};
+
+// PageRank workspace: double-buffered ranks and gathered values.
+struct zg_pr_ws {
+    int prec = 0;              // 32 or 64
+    void *r = nullptr;         // [2][V] float or double
+    void *xh = nullptr;        // [2][V]
+    double *part = nullptr;    // [nblocks][2] block partial sums (l1, dangling)
+    float2 *seg32 = nullptr;   // large-row segment partials (FP32 Kahan pairs)
+    double *seg64 = nullptr;
+    void *state = nullptr;     // device iteration state
+    void *h_state = nullptr;   // pinned host mirror
+};
+
+struct zg_graph {
+    std::mutex mu;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t V = 0, n_live = 0;
+    uint64_t E = 0;
+    // resident CSR
+    uint32_t *out_ptr = nullptr, *out_col = nullptr;
+    uint32_t *in_ptr = nullptr, *in_col = nullptr;
+    uint8_t *live = nullptr;
+    std::vector<uint8_t> h_live;
+    std::vector<uint32_t> h_out_deg;
+    uint32_t n_dangling = 0;
+    // predecessor chunks (forward sweep work items)
+    uint32_t n_ich = 0;
+    uint32_t *ich_v = nullptr, *ich_e0 = nullptr, *ich_e1 = nullptr;
+    uint32_t n_inhub = 0;
+    uint32_t *inhub = nullptr;           // vertices with more than one predecessor chunk
+    // successor chunks (backward sweep work items)
+    uint32_t n_och = 0, max_och_per_v = 0;
+    uint32_t *och_first = nullptr, *och_v = nullptr, *och_e0 = nullptr, *och_e1 = nullptr;
+    uint32_t n_ohub = 0;
+    uint32_t *ohub_idx = nullptr;        // [V] dense index of multi-chunk vertices or ZG_NONE
+    // PageRank row bins
+    uint32_t n_small = 0, n_med = 0, n_large = 0, n_seg = 0;
+    uint32_t *pr_small = nullptr, *pr_med = nullptr;
+    uint32_t *pr_large = nullptr;        // [n_large] rows
+    uint32_t *pr_large_seg0 = nullptr;   // [n_large + 1] first segment of each large row
+    uint32_t *pr_seg_row = nullptr;      // [n_seg] large-row index of each segment
+    uint32_t *pr_seg_e0 = nullptr, *pr_seg_e1 = nullptr;
+    int groups = 4;
+    zg_bc_ws bc;
+    zg_pr_ws pr;
+    // per-kernel profiling (events around launches, summed on read)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_marks;   // (slot, index of the start event)
+    zg_profile prof{};
+};
+
+// Profiling helpers: zg_prof_begin returns the index of the start event (or
+// SIZE_MAX when off); zg_prof_end records the end event and remembers the
+// pair; zg_prof_collect waits for the stream and folds pairs into g->prof.
+size_t zg_prof_begin(zg_graph *g, hipStream_t s);
+void zg_prof_end(zg_graph *g, hipStream_t s, int slot, size_t start);
+int zg_prof_collect(zg_graph *g, hipStream_t s);
+void zg_prof_release(zg_graph *g);
+
+#define ZG_PROF_LAUNCH(slot, ...)                    \
+    do {                                             \
+        size_t _zp = zg_prof_begin(g, s);            \
+        hipLaunchKernelGGL(__VA_ARGS__);             \
+        zg_prof_end(g, s, (slot), _zp);              \
+    } while (0)
+
+int zg_bc_ws_release(zg_graph *g);
+int zg_pr_ws_release(zg_graph *g);
+
+// ---------------------------------------------------------------------------
+// Device helpers
+#define ZG_DEV __device__ __forceinline__
+
+ZG_DEV uint64_t zg_readlane64(uint64_t x, int j)
+{
+    uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)x, j);
+    uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+ZG_DEV uint32_t zg_readlane32(uint32_t x, int j)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, j);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wavefront.
+ZG_DEV uint32_t zg_wave_incl_scan(uint32_t x, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+ZG_DEV double zg_wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+ZG_DEV unsigned long long zg_wave_sum_u64(unsigned long long x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}

@@ -1,0 +1,327 @@
+// Resource build and lifetime: the device-resident successor / predecessor
+// CSR that replaces the linked-list ArrayGraph for both hot paths.
+//
+// Reference: buildGraph (lib/zog/resource_graph.ex:120-129) appends all nodes
+// then calls addEdge (priv/zog/src/models/array_graph.zig:88-108) in array
+// order; addEdge prepends, so successors() yields arcs in reverse insertion
+// order (array_graph.zig:94-104, 166-180).  PageRank's in-lists are filled by
+// scanning live sources in id order (centrality.zig:881-899), so each is
+// ascending by source with duplicates kept.
+#include "common.h"
+
+#include <algorithm>
+#include <new>
+
+#include "pr_bins.h"
+
+extern "C" const char *zg_strerror(int code)
+{
+    switch (code) {
+    case ZG_OK: return "ok";
+    case ZG_ERR_INVALID_ARG: return "invalid argument";
+    case ZG_ERR_NODE_NOT_FOUND: return "node not found";
+    case ZG_ERR_OOM: return "out of host memory";
+    case ZG_ERR_GPU_OOM: return "out of device memory";
+    case ZG_ERR_HIP: return "HIP runtime error";
+    case ZG_ERR_NO_DEVICE: return "no gfx950 device";
+    case ZG_ERR_TOO_LARGE: return "graph too large (more than 2^32-1 arcs or nodes)";
+    case ZG_ERR_INVALID_HANDLE: return "invalid graph handle";
+    default: return "unknown error";
+    }
+}
+
+size_t zg_prof_begin(zg_graph *g, hipStream_t s)
+{
+    if (!g->profiling) return SIZE_MAX;
+    while (g->ev_pool.size() < g->ev_used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return SIZE_MAX;
+        g->ev_pool.push_back(e);
+    }
+    size_t i = g->ev_used;
+    g->ev_used += 2;
+    (void)hipEventRecord(g->ev_pool[i], s);
+    return i;
+}
+
+void zg_prof_end(zg_graph *g, hipStream_t s, int slot, size_t start)
+{
+    if (start == SIZE_MAX) return;
+    (void)hipEventRecord(g->ev_pool[start + 1], s);
+    g->ev_marks.emplace_back(slot, start);
+}
+
+int zg_prof_collect(zg_graph *g, hipStream_t s)
+{
+    if (g->ev_marks.empty()) return ZG_OK;
+    ZG_TRY(hipStreamSynchronize(s));
+    for (auto &m : g->ev_marks) {
+        float ms = 0.f;
+        ZG_TRY(hipEventElapsedTime(&ms, g->ev_pool[m.second], g->ev_pool[m.second + 1]));
+        g->prof.ms[m.first] += ms;
+        g->prof.launches[m.first] += 1;
+    }
+    g->ev_marks.clear();
+    g->ev_used = 0;
+    return ZG_OK;
+}
+
+void zg_prof_release(zg_graph *g)
+{
+    for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
+    g->ev_pool.clear();
+    g->ev_marks.clear();
+    g->ev_used = 0;
+}
+
+extern "C" int zg_set_profiling(zg_graph *g, int on)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->profiling = on != 0;
+    memset(&g->prof, 0, sizeof(g->prof));
+    return ZG_OK;
+}
+
+extern "C" int zg_get_profile(zg_graph *g, zg_profile *out)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!out) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    *out = g->prof;
+    memset(&g->prof, 0, sizeof(g->prof));
+    return ZG_OK;
+}
+
+static void zg_free_device(zg_graph *g)
+{
+    zg_prof_release(g);
+    zg_bc_ws_release(g);
+    zg_pr_ws_release(g);
+    zg_dfree(g->out_ptr); zg_dfree(g->out_col); zg_dfree(g->in_ptr); zg_dfree(g->in_col);
+    zg_dfree(g->live);
+    zg_dfree(g->ich_v); zg_dfree(g->ich_e0); zg_dfree(g->ich_e1); zg_dfree(g->inhub);
+    zg_dfree(g->och_first); zg_dfree(g->och_v); zg_dfree(g->och_e0); zg_dfree(g->och_e1);
+    zg_dfree(g->ohub_idx);
+    zg_dfree(g->pr_small); zg_dfree(g->pr_med); zg_dfree(g->pr_large); zg_dfree(g->pr_large_seg0);
+    zg_dfree(g->pr_seg_row); zg_dfree(g->pr_seg_e0); zg_dfree(g->pr_seg_e1);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    g->stream = nullptr;
+}
+
+template <typename T>
+static int zg_upload(T **d, const std::vector<T> &h)
+{
+    ZG_TRY_RC(zg_dmalloc(d, h.size()));
+    if (!h.empty()) ZG_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return ZG_OK;
+}
+
+static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
+                    uint64_t m)
+{
+    const uint32_t V = g->V;
+    g->h_live.assign(V, 1);
+    if (live_mask)
+        for (uint32_t i = 0; i < V; i++) g->h_live[i] = live_mask[i] ? 1 : 0;
+    uint32_t n_live = 0;
+    for (uint32_t i = 0; i < V; i++) n_live += g->h_live[i];
+    g->n_live = n_live;
+
+    // Successor CSR.  Arcs touching a tombstoned node are dropped (the
+    // iterator skips them).  Within a row, arcs are stored newest first.
+    std::vector<uint32_t> out_ptr((size_t)V + 1, 0);
+    for (uint64_t e = 0; e < m; e++) {
+        uint32_t f = from[e], t = to[e];
+        if (f >= V || t >= V) return ZG_ERR_NODE_NOT_FOUND;
+        if (g->h_live[f] && g->h_live[t]) out_ptr[(size_t)f + 1]++;
+    }
+    for (uint32_t i = 0; i < V; i++) out_ptr[i + 1] += out_ptr[i];
+    const uint64_t E = out_ptr[V];
+    g->E = E;
+    std::vector<uint32_t> out_col(E);
+    {
+        // filling each row from its end in insertion order leaves the newest
+        // arc first
+        std::vector<uint32_t> fill(out_ptr.begin() + 1, out_ptr.end());
+        for (uint64_t e = 0; e < m; e++) {
+            uint32_t f = from[e], t = to[e];
+            if (g->h_live[f] && g->h_live[t]) out_col[--fill[f]] = t;
+        }
+    }
+    // Predecessor CSR: scan sources in id order.
+    std::vector<uint32_t> in_ptr((size_t)V + 1, 0);
+    for (uint64_t k = 0; k < E; k++) in_ptr[(size_t)out_col[k] + 1]++;
+    for (uint32_t i = 0; i < V; i++) in_ptr[i + 1] += in_ptr[i];
+    std::vector<uint32_t> in_col(E);
+    {
+        std::vector<uint32_t> fill(in_ptr.begin(), in_ptr.end() - 1);
+        for (uint32_t v = 0; v < V; v++)
+            for (uint32_t k = out_ptr[v]; k < out_ptr[(size_t)v + 1]; k++) in_col[fill[out_col[k]]++] = v;
+    }
+    g->h_out_deg.resize(V);
+    uint32_t n_dang = 0;
+    for (uint32_t v = 0; v < V; v++) {
+        g->h_out_deg[v] = out_ptr[(size_t)v + 1] - out_ptr[v];
+        if (g->h_live[v] && g->h_out_deg[v] == 0) n_dang++;
+    }
+    g->n_dangling = n_dang;
+
+    // Predecessor chunks: forward-sweep work items (vertices with no
+    // predecessor are reached only as sources and get no chunk).
+    std::vector<uint32_t> ich_v, ich_e0, ich_e1, inhub;
+    for (uint32_t v = 0; v < V; v++) {
+        uint32_t b = in_ptr[v], e = in_ptr[(size_t)v + 1];
+        if (b == e || !g->h_live[v]) continue;
+        if (e - b > ZG_CHUNK) inhub.push_back(v);
+        for (uint32_t c = b; c < e; c += ZG_CHUNK) {
+            ich_v.push_back(v);
+            ich_e0.push_back(c);
+            ich_e1.push_back(std::min<uint32_t>(c + ZG_CHUNK, e));
+        }
+    }
+    g->n_ich = (uint32_t)ich_v.size();
+    g->n_inhub = (uint32_t)inhub.size();
+    // Successor chunks: backward-sweep work items; every vertex has at least
+    // one (possibly empty) chunk because a leaf still publishes 1/sigma.
+    std::vector<uint32_t> och_first((size_t)V + 1), och_v, och_e0, och_e1, ohub_idx(V, ZG_NONE);
+    uint32_t max_k = 1, n_ohub = 0;
+    for (uint32_t v = 0; v < V; v++) {
+        och_first[v] = (uint32_t)och_v.size();
+        uint32_t b = out_ptr[v], e = out_ptr[(size_t)v + 1];
+        uint32_t k = 0;
+        if (b == e) {
+            och_v.push_back(v); och_e0.push_back(b); och_e1.push_back(e);
+            k = 1;
+        } else {
+            for (uint32_t c = b; c < e; c += ZG_CHUNK) {
+                och_v.push_back(v);
+                och_e0.push_back(c);
+                och_e1.push_back(std::min<uint32_t>(c + ZG_CHUNK, e));
+                k++;
+            }
+        }
+        if (k > 1) ohub_idx[v] = n_ohub++;
+        max_k = std::max(max_k, k);
+    }
+    och_first[V] = (uint32_t)och_v.size();
+    g->n_och = (uint32_t)och_v.size();
+    g->max_och_per_v = max_k;
+    g->n_ohub = n_ohub;
+
+    // PageRank row bins over live rows by predecessor count.
+    std::vector<uint32_t> small, med, large, large_seg0, seg_row, seg_e0, seg_e1;
+    for (uint32_t v = 0; v < V; v++) {
+        if (!g->h_live[v]) continue;
+        uint32_t d = in_ptr[(size_t)v + 1] - in_ptr[v];
+        if (d <= ZG_PR_SMALL) small.push_back(v);
+        else if (d <= ZG_PR_MED) med.push_back(v);
+        else {
+            large_seg0.push_back((uint32_t)seg_row.size());
+            for (uint32_t c = in_ptr[v]; c < in_ptr[(size_t)v + 1]; c += ZG_PR_SEG) {
+                seg_row.push_back((uint32_t)large.size());
+                seg_e0.push_back(c);
+                seg_e1.push_back(std::min<uint32_t>(c + ZG_PR_SEG, in_ptr[(size_t)v + 1]));
+            }
+            large.push_back(v);
+        }
+    }
+    large_seg0.push_back((uint32_t)seg_row.size());
+    g->n_small = (uint32_t)small.size();
+    g->n_med = (uint32_t)med.size();
+    g->n_large = (uint32_t)large.size();
+    g->n_seg = (uint32_t)seg_row.size();
+
+    ZG_TRY_RC(zg_upload(&g->out_ptr, out_ptr));
+    ZG_TRY_RC(zg_upload(&g->out_col, out_col));
+    ZG_TRY_RC(zg_upload(&g->in_ptr, in_ptr));
+    ZG_TRY_RC(zg_upload(&g->in_col, in_col));
+    ZG_TRY_RC(zg_upload(&g->live, g->h_live));
+    ZG_TRY_RC(zg_upload(&g->ich_v, ich_v));
+    ZG_TRY_RC(zg_upload(&g->ich_e0, ich_e0));
+    ZG_TRY_RC(zg_upload(&g->ich_e1, ich_e1));
+    ZG_TRY_RC(zg_upload(&g->inhub, inhub));
+    ZG_TRY_RC(zg_upload(&g->och_first, och_first));
+    ZG_TRY_RC(zg_upload(&g->och_v, och_v));
+    ZG_TRY_RC(zg_upload(&g->och_e0, och_e0));
+    ZG_TRY_RC(zg_upload(&g->och_e1, och_e1));
+    ZG_TRY_RC(zg_upload(&g->ohub_idx, ohub_idx));
+    ZG_TRY_RC(zg_upload(&g->pr_small, small));
+    ZG_TRY_RC(zg_upload(&g->pr_med, med));
+    ZG_TRY_RC(zg_upload(&g->pr_large, large));
+    ZG_TRY_RC(zg_upload(&g->pr_large_seg0, large_seg0));
+    ZG_TRY_RC(zg_upload(&g->pr_seg_row, seg_row));
+    ZG_TRY_RC(zg_upload(&g->pr_seg_e0, seg_e0));
+    ZG_TRY_RC(zg_upload(&g->pr_seg_e1, seg_e1));
+    return ZG_OK;
+}
+
+extern "C" int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
+                               const uint32_t *to, uint64_t m, zg_graph **out)
+{
+    if (!out) return ZG_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (m && (!from || !to)) return ZG_ERR_INVALID_ARG;
+    if (m >= 0xFFFFFFFFull || n_cap == 0xFFFFFFFFu) return ZG_ERR_TOO_LARGE;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return ZG_ERR_NO_DEVICE;
+    }
+    zg_graph *g = new (std::nothrow) zg_graph();
+    if (!g) return ZG_ERR_OOM;
+    g->V = n_cap;
+    int rc = ZG_OK;
+    hipError_t e = hipGetDevice(&g->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = zg_hip_code(e);
+    if (rc == ZG_OK) {
+        try {
+            rc = zg_build(g, live_mask, from, to, m);
+        } catch (const std::bad_alloc &) {
+            rc = ZG_ERR_OOM;
+        }
+    }
+    if (rc != ZG_OK) {
+        zg_free_device(g);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return ZG_OK;
+}
+
+extern "C" void zg_graph_destroy(zg_graph *g)
+{
+    if (!g) return;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        (void)hipSetDevice(g->device);
+        if (g->stream) (void)hipStreamSynchronize(g->stream);
+        zg_free_device(g);
+    }
+    delete g;
+}
+
+extern "C" int zg_graph_info(const zg_graph *g, uint32_t *n_cap, uint32_t *n_live, uint64_t *n_arcs)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (n_cap) *n_cap = g->V;
+    if (n_live) *n_live = g->n_live;
+    if (n_arcs) *n_arcs = g->E;
+    return ZG_OK;
+}
+
+extern "C" int zg_set_batch_groups(zg_graph *g, int groups)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (groups < 1 || groups > ZG_MAX_GROUPS) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    int prev = g->groups;
+    if (groups != g->groups) {
+        (void)hipSetDevice(g->device);
+        zg_bc_ws_release(g);
+        g->groups = groups;
+    }
+    return prev;
+}

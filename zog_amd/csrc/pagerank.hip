@@ -1,0 +1,477 @@
+// PageRank power iteration as a pull SpMV over the resident predecessor CSR.
+//
+// Reference: priv/zog/src/centrality.zig:871-965.  Per iteration
+//   sink   = sum over live v with out_deg(v) = 0 of r[v] / n          (:926-931)
+//   r'[v]  = (1 - d)/n + d * (sink + sum_{u in in(v), deg u > 0} r[u]/deg(u))
+//   l1     = sum |r'[v] - r[v]|                                        (:947-950)
+//   swap, then stop when l1 < tolerance (strict)                       (:952-957)
+//
+// gfx950 layout: ranks r and the gathered values x[u] = r[u]/deg(u) are FP32
+// (or FP64 in the FP64 mode), double-buffered by iteration parity.  Each row
+// sum is a compensated (Kahan) sum; partial sums of one row computed by
+// different lanes are merged with an error-free TwoSum.  The l1 change and the
+// dangling mass are reduced in FP64: every block writes its partials, and the
+// last block to finish (an atomic ticket) adds them in a fixed order, so the
+// result is deterministic, and prepares the next iteration's constant term
+// base = (1-d)/n + d*sink.  One kernel launch per iteration; launches after
+// convergence return at once, so several iterations are queued per host sync.
+#include "common.h"
+#include "pr_bins.h"
+
+struct PrState {
+    unsigned int ticket;
+    unsigned int done;
+    unsigned long long iter;
+    unsigned long long max_iter;
+    double base;      // (1-d)/n + d*sink for the next iteration to run
+    double last_l1;
+    double damping;
+    double inv_n;
+    double tol;
+};
+
+template <typename T>
+struct PrArgs {
+    const uint32_t *in_ptr, *in_col, *out_ptr;
+    const uint32_t *small, *med, *large, *large_seg0, *seg_e0, *seg_e1;
+    uint32_t n_small, n_med, n_large, n_seg;
+    uint32_t nb_small, nb_med;  // blocks of the small and medium bins
+    T *r0, *r1, *x0, *x1;       // parity 0 / 1 buffers
+    double *part;               // [gridDim.x][2]
+    T *seg;                     // [n_seg][2] Kahan pairs of long-row segments
+    PrState *st;
+};
+
+template <typename T>
+__device__ __forceinline__ void kahan_add(T &s, T &c, T x)
+{
+    T y = x - c;
+    T t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Merge two Kahan pairs (value = s - c) with an exact TwoSum of the heads.
+template <typename T>
+__device__ __forceinline__ void kahan_merge(T &s, T &c, T s2, T c2)
+{
+    T S = s + s2;
+    T bb = S - s;
+    T err = (s - (S - bb)) + (s2 - bb);
+    s = S;
+    c = (c + c2) - err;
+}
+
+template <typename T>
+__device__ __forceinline__ void wave_kahan_merge(T &s, T &c)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T s2 = __shfl_xor(s, off, 64);
+        T c2 = __shfl_xor(c, off, 64);
+        kahan_merge(s, c, s2, c2);
+    }
+}
+
+// New rank of row v from its compensated predecessor sum (value s - c);
+// accumulates the FP64 l1 change and dangling mass of this thread.
+template <typename T>
+__device__ __forceinline__ void pr_row_epilogue(const PrArgs<T> &a, const T *rold, T *rnew, T *xnew,
+                                                uint32_t v, T s, T c, double base, double damping,
+                                                double &l1, double &dang)
+{
+    double sum = (double)s - (double)c;
+    T rt = (T)(base + damping * sum);
+    rnew[v] = rt;
+    l1 += fabs((double)rt - (double)rold[v]);
+    uint32_t deg = a.out_ptr[v + 1] - a.out_ptr[v];
+    if (deg == 0) dang += (double)rt;
+    else xnew[v] = rt / (T)deg;
+}
+
+template <typename T>
+__global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
+{
+    __shared__ double red[2][ZG_PR_BLOCK / 64];
+    __shared__ T kred[2][ZG_PR_BLOCK / 64];
+    __shared__ int is_last;
+    PrState *st = a.st;
+    if (st->done) return;
+    const unsigned long long it = st->iter;
+    const bool odd = it & 1;
+    const T *x = odd ? a.x1 : a.x0;
+    const T *rold = odd ? a.r1 : a.r0;
+    T *rnew = odd ? a.r0 : a.r1;
+    T *xnew = odd ? a.x0 : a.x1;
+    const double base = st->base, damping = st->damping;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t b = blockIdx.x;
+    double l1 = 0.0, dang = 0.0;
+
+    if (b < a.nb_small) {
+        // one lane per row (<= ZG_PR_SMALL predecessors)
+        uint32_t i = b * ZG_PR_BLOCK + tid;
+        if (i < a.n_small) {
+            uint32_t v = a.small[i];
+            T s = 0, c = 0;
+            for (uint32_t e = a.in_ptr[v], e1 = a.in_ptr[v + 1]; e < e1; e++) kahan_add(s, c, x[a.in_col[e]]);
+            pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
+        }
+    } else if (b < a.nb_small + a.nb_med) {
+        // one wavefront per row
+        uint32_t i = (b - a.nb_small) * (ZG_PR_BLOCK / 64) + wv;
+        if (i < a.n_med) {
+            uint32_t v = a.med[i];
+            T s = 0, c = 0;
+            for (uint32_t e = a.in_ptr[v] + lane, e1 = a.in_ptr[v + 1]; e < e1; e += 64)
+                kahan_add(s, c, x[a.in_col[e]]);
+            wave_kahan_merge(s, c);
+            if (lane == 0) pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
+        }
+    } else {
+        // one block per segment of a long row; the last block finishes the row
+        uint32_t sg = b - a.nb_small - a.nb_med;
+        T s = 0, c = 0;
+        for (uint32_t e = a.seg_e0[sg] + tid, e1 = a.seg_e1[sg]; e < e1; e += ZG_PR_BLOCK)
+            kahan_add(s, c, x[a.in_col[e]]);
+        wave_kahan_merge(s, c);
+        if (lane == 0) {
+            kred[0][wv] = s;
+            kred[1][wv] = c;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            T S = kred[0][0], C = kred[1][0];
+            for (int w = 1; w < ZG_PR_BLOCK / 64; w++) kahan_merge(S, C, kred[0][w], kred[1][w]);
+            a.seg[2 * sg] = S;
+            a.seg[2 * sg + 1] = C;
+        }
+    }
+
+    // block partials of (l1, dangling mass)
+    l1 = zg_wave_sum(l1);
+    dang = zg_wave_sum(dang);
+    if (lane == 0) {
+        red[0][wv] = l1;
+        red[1][wv] = dang;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double L = 0.0, D = 0.0;
+        for (int w = 0; w < ZG_PR_BLOCK / 64; w++) {
+            L += red[0][w];
+            D += red[1][w];
+        }
+        a.part[2 * b] = L;
+        a.part[2 * b + 1] = D;
+        __threadfence();
+        unsigned int t = atomicAdd(&st->ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+
+    // Last block: finish the long rows from their segment partials, then add
+    // all block partials in a fixed order.
+    l1 = 0.0;
+    dang = 0.0;
+    for (uint32_t i = tid; i < a.n_large; i += ZG_PR_BLOCK) {
+        uint32_t s0 = a.large_seg0[i], s1 = a.large_seg0[i + 1];
+        T S = a.seg[2 * s0], C = a.seg[2 * s0 + 1];
+        for (uint32_t q = s0 + 1; q < s1; q++) kahan_merge(S, C, a.seg[2 * q], a.seg[2 * q + 1]);
+        pr_row_epilogue(a, rold, rnew, xnew, a.large[i], S, C, base, damping, l1, dang);
+    }
+    for (uint32_t q = tid; q < gridDim.x; q += ZG_PR_BLOCK) {
+        l1 += a.part[2 * q];
+        dang += a.part[2 * q + 1];
+    }
+    l1 = zg_wave_sum(l1);
+    dang = zg_wave_sum(dang);
+    __syncthreads();
+    if (lane == 0) {
+        red[0][wv] = l1;
+        red[1][wv] = dang;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double L = 0.0, D = 0.0;
+        for (int w = 0; w < ZG_PR_BLOCK / 64; w++) {
+            L += red[0][w];
+            D += red[1][w];
+        }
+        unsigned long long nit = it + 1;
+        st->iter = nit;
+        st->last_l1 = L;
+        st->base = (1.0 - damping) * st->inv_n + damping * (D * st->inv_n);
+        if (L < st->tol || nit >= st->max_iter) st->done = 1;
+        st->ticket = 0;
+        __threadfence();
+    }
+}
+
+// r <- src (uniform 1/n when src == nullptr) on live rows, 0 elsewhere;
+// x = r/deg on live rows with successors, 0 elsewhere; both parities.
+template <typename T, typename S>
+__global__ void pr_init_kernel(uint32_t V, const uint8_t *live, const uint32_t *out_ptr, const S *src,
+                               double uniform, T *r0, T *r1, T *x0, T *x1)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    T r = 0;
+    if (live[v]) r = src ? (T)src[v] : (T)uniform;
+    uint32_t deg = out_ptr[v + 1] - out_ptr[v];
+    r0[v] = r;
+    r1[v] = 0;
+    x0[v] = deg ? r / (T)deg : (T)0;
+    x1[v] = 0;
+}
+
+template <typename T>
+__global__ void pr_out_kernel(uint32_t V, const T *r, double *out)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) out[v] = (double)r[v];
+}
+
+__global__ void pr_single_kernel(uint32_t V, const uint8_t *live, double *out)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) out[v] = live[v] ? 1.0 : 0.0;
+}
+
+int zg_pr_ws_release(zg_graph *g)
+{
+    zg_pr_ws &w = g->pr;
+    zg_dfree(w.r);
+    zg_dfree(w.xh);
+    zg_dfree(w.part);
+    zg_dfree(w.seg32);
+    zg_dfree(w.seg64);
+    zg_dfree(w.state);
+    if (w.h_state) (void)hipHostFree(w.h_state);
+    w.h_state = nullptr;
+    return ZG_OK;
+}
+
+static uint32_t pr_grid(const zg_graph *g)
+{
+    uint32_t nb_small = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK;
+    uint32_t nb_med = (g->n_med + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
+    return nb_small + nb_med + g->n_seg;
+}
+
+static int pr_ws_ensure(zg_graph *g)
+{
+    zg_pr_ws &w = g->pr;
+    if (w.state) return ZG_OK;
+    size_t V = g->V;
+    // FP64 buffers (also used for the FP64 residual check) and FP32 buffers
+    // share one allocation: [r64 x2][x64 x2][r32 x2][x32 x2]
+    ZG_TRY_RC(zg_dmalloc((char **)&w.r, V * (4 * 8 + 4 * 4)));
+    w.xh = nullptr;
+    ZG_TRY_RC(zg_dmalloc(&w.part, 2 * (size_t)pr_grid(g)));
+    ZG_TRY_RC(zg_dmalloc(&w.seg32, g->n_seg));
+    ZG_TRY_RC(zg_dmalloc(&w.seg64, 2 * (size_t)g->n_seg));
+    ZG_TRY_RC(zg_dmalloc((char **)&w.state, 2 * sizeof(PrState)));
+    ZG_TRY(hipMemset(w.state, 0, 2 * sizeof(PrState)));
+    ZG_TRY(hipHostMalloc(&w.h_state, 2 * sizeof(PrState), hipHostMallocDefault));
+    return ZG_OK;
+}
+
+template <typename T>
+static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrState *st)
+{
+    PrArgs<T> a;
+    a.in_ptr = g->in_ptr;
+    a.in_col = g->in_col;
+    a.out_ptr = g->out_ptr;
+    a.small = g->pr_small;
+    a.med = g->pr_med;
+    a.large = g->pr_large;
+    a.large_seg0 = g->pr_large_seg0;
+    a.seg_e0 = g->pr_seg_e0;
+    a.seg_e1 = g->pr_seg_e1;
+    a.n_small = g->n_small;
+    a.n_med = g->n_med;
+    a.n_large = g->n_large;
+    a.n_seg = g->n_seg;
+    a.nb_small = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK;
+    a.nb_med = (g->n_med + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
+    a.r0 = r0;
+    a.r1 = r1;
+    a.x0 = x0;
+    a.x1 = x1;
+    a.part = g->pr.part;
+    a.seg = seg;
+    a.st = st;
+    return a;
+}
+
+// Runs the power iteration in precision T from the initialised buffers.
+// Returns with *host mirror of the final state filled.
+template <typename T>
+static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st, uint64_t max_iter)
+{
+    const uint32_t grid = pr_grid(g);
+    const uint64_t K = 8;
+    uint64_t launched = 0;
+    while (launched < max_iter) {
+        uint64_t k = max_iter - launched < K ? max_iter - launched : K;
+        for (uint64_t i = 0; i < k; i++)
+            ZG_PROF_LAUNCH(ZG_PROF_PR_ITER, pr_iter_kernel<T>, dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+        ZG_TRY(hipGetLastError());
+        launched += k;
+        ZG_TRY(hipMemcpyAsync(h_st, a.st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+        if (h_st->done) break;
+    }
+    if (max_iter == 0) {
+        ZG_TRY(hipMemcpyAsync(h_st, a.st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+    }
+    return ZG_OK;
+}
+
+static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                            int precision, double *d_out, hipStream_t s, zg_pr_info *info)
+{
+    const bool check = !(precision & 0x100);
+    precision &= 0xFF;
+    if (precision != ZG_PR_FP32 && precision != ZG_PR_FP64) return ZG_ERR_INVALID_ARG;
+    const uint32_t V = g->V;
+    const uint32_t n = g->n_live;
+    zg_pr_info inf;
+    memset(&inf, 0, sizeof(inf));
+    inf.precision = precision;
+    const dim3 vb(256), vg((V + 255) / 256);
+    if (n <= 1) {
+        // centrality.zig:875-879: n = 0 gives no scores, n = 1 gives 1.0
+        if (V) {
+            if (n == 0) ZG_TRY(hipMemsetAsync(d_out, 0, sizeof(double) * V, s));
+            else hipLaunchKernelGGL(pr_single_kernel, vg, vb, 0, s, V, g->live, d_out);
+            ZG_TRY(hipGetLastError());
+        }
+        ZG_TRY(hipStreamSynchronize(s));
+        if (info) *info = inf;
+        return ZG_OK;
+    }
+    ZG_TRY_RC(pr_ws_ensure(g));
+    double *r64 = (double *)g->pr.r;
+    double *x64 = r64 + 2 * (size_t)V;
+    float *r32 = (float *)(x64 + 2 * (size_t)V);
+    float *x32 = r32 + 2 * (size_t)V;
+    PrState *d_st = (PrState *)g->pr.state;
+    PrState *h_st = (PrState *)g->pr.h_state;
+
+    const double inv_n = 1.0 / (double)n;
+    PrState st0;
+    memset(&st0, 0, sizeof(st0));
+    st0.max_iter = max_iterations;
+    st0.damping = damping;
+    st0.inv_n = inv_n;
+    st0.tol = tolerance;
+    // sink of the uniform start vector: n_dangling * (1/n) / n
+    st0.base = (1.0 - damping) * inv_n + damping * ((double)g->n_dangling * inv_n * inv_n);
+    st0.done = max_iterations == 0 ? 1u : 0u;
+
+    hipEvent_t e0, e1;
+    ZG_TRY(hipEventCreate(&e0));
+    ZG_TRY(hipEventCreate(&e1));
+    int rc = ZG_OK;
+    uint64_t iters = 0;
+    bool final_odd = false;
+    do {
+        ZG_TRY(hipMemcpyAsync(d_st, &st0, sizeof(PrState), hipMemcpyHostToDevice, s));
+        if (precision == ZG_PR_FP32) {
+            hipLaunchKernelGGL((pr_init_kernel<float, float>), vg, vb, 0, s, V, g->live, g->out_ptr,
+                               (const float *)nullptr, inv_n, r32, r32 + V, x32, x32 + V);
+        } else {
+            hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr,
+                               (const double *)nullptr, inv_n, r64, r64 + V, x64, x64 + V);
+        }
+        ZG_TRY(hipGetLastError());
+        ZG_TRY(hipEventRecord(e0, s));
+        if (precision == ZG_PR_FP32) {
+            PrArgs<float> a = pr_args<float>(g, r32, r32 + V, x32, x32 + V, (float *)g->pr.seg32, d_st);
+            rc = pr_run<float>(g, s, a, h_st, max_iterations);
+        } else {
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st);
+            rc = pr_run<double>(g, s, a, h_st, max_iterations);
+        }
+        if (rc != ZG_OK) break;
+        ZG_TRY(hipEventRecord(e1, s));
+        ZG_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        ZG_TRY(hipEventElapsedTime(&ms, e0, e1));
+        inf.kernel_ms = ms;
+        iters = h_st->iter;
+        final_odd = iters & 1;
+        // launches queued past convergence returned at once: not profiled
+        if (g->ev_marks.size() > iters) g->ev_marks.resize(iters);
+        ZG_TRY_RC(zg_prof_collect(g, s));
+        inf.iterations = iters;
+        inf.last_l1 = h_st->last_l1;
+        if (precision == ZG_PR_FP32)
+            hipLaunchKernelGGL(pr_out_kernel<float>, vg, vb, 0, s, V, final_odd ? r32 + V : r32, d_out);
+        else
+            hipLaunchKernelGGL(pr_out_kernel<double>, vg, vb, 0, s, V, final_odd ? r64 + V : r64, d_out);
+        ZG_TRY(hipGetLastError());
+
+        if (check) {
+            // One FP64 iteration from the returned vector: its l1 change is
+            // ||G(r) - r||_1, and ||r - r*||_1 <= that / (1 - d).
+            PrState st1 = st0;
+            st1.base = h_st->base;  // constant term built from r's dangling mass
+            st1.max_iter = 1;
+            st1.tol = -1.0;
+            st1.done = 0;
+            ZG_TRY(hipMemcpyAsync(d_st + 1, &st1, sizeof(PrState), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr,
+                               (const double *)d_out, 0.0, r64, r64 + V, x64, x64 + V);
+            ZG_TRY(hipGetLastError());
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st + 1);
+            hipLaunchKernelGGL(pr_iter_kernel<double>, dim3(pr_grid(g)), dim3(ZG_PR_BLOCK), 0, s, a);
+            ZG_TRY(hipGetLastError());
+            ZG_TRY(hipMemcpyAsync(h_st + 1, d_st + 1, sizeof(PrState), hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+            inf.residual_l1 = h_st[1].last_l1;
+            inf.error_bound_l1 = damping < 1.0 ? inf.residual_l1 / (1.0 - damping) : INFINITY;
+        }
+    } while (0);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != ZG_OK) return rc;
+    ZG_TRY(hipStreamSynchronize(s));
+    if (info) *info = inf;
+    return ZG_OK;
+}
+
+extern "C" int zg_pagerank_device(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                                  int precision, double *d_out, void *stream, zg_pr_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!d_out && g->V) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    return zg_pagerank_impl(g, damping, max_iterations, tolerance, precision, d_out, s, info);
+}
+
+extern "C" int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                           int precision, double *out, zg_pr_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!out && g->V) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    double *d_out = nullptr;
+    ZG_TRY_RC(zg_dmalloc(&d_out, g->V));
+    int rc = zg_pagerank_impl(g, damping, max_iterations, tolerance, precision, d_out, g->stream, info);
+    if (rc == ZG_OK && g->V) {
+        hipError_t e = hipMemcpy(out, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = zg_hip_code(e);
+    }
+    zg_dfree(d_out);
+    return rc;
+}

@@ -1,0 +1,306 @@
+"""Python mirror of ``Zog.ResourceGraph`` for the ``:soa`` backend's two
+accelerated paths (lib/zog/resource_graph.ex), calling the gfx950 library
+through its C ABI (include/zog_hip.h).
+
+Reference surface kept:
+  * ``new(builder, backend="soa")``            resource_graph.ex:1393-1404
+  * ``read_edgelist(path, directed=True, backend="soa", integer_labels=False)``
+                                               resource_graph.ex:1474-1486, parse
+                                               :990-1084, undirected dedupe :953-988
+  * ``destroy(graph)``                         resource_graph.ex:1577-1580
+  * ``betweenness_unweighted(graph, raw=False)``  resource_graph.ex:1590-1601
+    (x0.5 for undirected builders before the raw branch, :2633-2637)
+  * ``pagerank(graph, damping=0.85, max_iterations=100, tolerance=0.0001,
+    raw=False)``                               resource_graph.ex:1671-1685
+  * ``raw=True`` returns the dense list indexed by u32 id; otherwise scores
+    are mapped to labels (map_scores, :2609-2619): integer keys for an
+    integer-labelled builder, else labels in insertion order.
+
+Extensions (keyword-only, absent in the reference): ``sources=`` for sampled
+Brandes, ``precision=`` for PageRank, ``info=`` dicts with per-run metrics,
+and ``betweenness_centrality`` as an alias of ``betweenness_unweighted``.
+Errors: a failing status raises ``ZogError`` (``NodeNotFoundError`` for an
+out-of-range id), as Zigler raises a returned Zig error in Elixir.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .soa import SoA
+
+BACKENDS = ("soa",)
+
+
+class Resource:
+    """Owner of one ``zg_graph`` handle (the GraphRes resource)."""
+
+    def __init__(self, n_cap: int, src: np.ndarray, dst: np.ndarray, live_mask=None):
+        L = _lib.lib()
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        if len(src) != len(dst):
+            raise ValueError("from/to arrays differ in length")
+        live = None if live_mask is None else np.ascontiguousarray(live_mask, dtype=np.uint8)
+        h = ctypes.c_void_p()
+        rc = L.zg_graph_create(int(n_cap), _ptr(live), _ptr(src), _ptr(dst), len(src), ctypes.byref(h))
+        _lib.check(rc, "zg_graph_create")
+        self._h = h
+        self._lock = threading.Lock()
+        self.n_cap = int(n_cap)
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise _lib.ZogError(_lib.ZG_ERR_INVALID_HANDLE, "released resource")
+        return self._h
+
+    def release(self):
+        with self._lock:
+            if self._h:
+                _lib.lib().zg_graph_destroy(self._h)
+                self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def info(self):
+        V = ctypes.c_uint32()
+        n = ctypes.c_uint32()
+        m = ctypes.c_uint64()
+        _lib.check(_lib.lib().zg_graph_info(self.handle, ctypes.byref(V), ctypes.byref(n), ctypes.byref(m)))
+        return int(V.value), int(n.value), int(m.value)
+
+
+def _ptr(a) -> Optional[ctypes.c_void_p]:
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class ResourceGraph(dict):
+    """``%{resource: ref, builder: SoA}`` of the reference."""
+
+    @property
+    def resource(self) -> Resource:
+        return self["resource"]
+
+    @property
+    def builder(self) -> SoA:
+        return self["builder"]
+
+
+def new(builder: SoA, backend: str = "soa", live_mask=None) -> ResourceGraph:
+    if backend not in BACKENDS:
+        raise ValueError(f"backend {backend!r} is not accelerated here (only :soa)")
+    src, dst, _w = builder.to_edge_arrays()
+    res = Resource(builder.node_count(), src, dst, live_mask)
+    return ResourceGraph(resource=res, builder=builder)
+
+
+def destroy(graph: ResourceGraph) -> str:
+    graph.resource.release()
+    return "ok"
+
+
+# ---------------------------------------------------------------------------
+# edge-list ingest (nif_read_edgelist, resource_graph.ex:990-1084)
+
+def _parse_edgelist(path: str, integer_labels: bool):
+    label_to_id = {}
+    labels = []
+    next_id = 0
+    edges = []
+    with open(path, "rb") as fh:
+        for raw in fh.read().split(b"\n"):
+            line = raw.strip(b" \t\r")
+            if not line or line[:1] == b"#":
+                continue
+            parts = [p for p in line.replace(b",", b" ").replace(b"\t", b" ").split(b" ") if p]
+            if len(parts) < 2:
+                continue
+            a, b = parts[0], parts[1]
+            if a.lower() == b"source" and b.lower() == b"target":
+                continue
+            w = 1.0
+            if len(parts) > 2:
+                try:
+                    w = float(parts[2])
+                except ValueError:
+                    w = 1.0
+            if integer_labels:
+                try:
+                    u, v = int(a), int(b)
+                except ValueError:
+                    continue
+                if not (0 <= u < 2**32 and 0 <= v < 2**32):
+                    continue
+                next_id = max(next_id, max(u, v) + 1)
+            else:
+                ids = []
+                for lab in (a, b):
+                    key = lab.decode()
+                    i = label_to_id.get(key)
+                    if i is None:
+                        i = next_id
+                        label_to_id[key] = i
+                        labels.append(key)
+                        next_id += 1
+                    ids.append(i)
+                u, v = ids
+            edges.append((u, v, w))
+    return edges, next_id, labels
+
+
+def _expand_edges(edges, directed: bool):
+    """addParsedEdges (resource_graph.ex:953-988): directed arcs in file order;
+    undirected edges sorted by (min, max), duplicates dropped, both arcs added
+    except for a self-loop (one arc)."""
+    if directed:
+        return [e[0] for e in edges], [e[1] for e in edges]
+    order = sorted(range(len(edges)), key=lambda i: (min(edges[i][0], edges[i][1]), max(edges[i][0], edges[i][1])))
+    src, dst = [], []
+    prev = None
+    for i in order:
+        u, v, _ = edges[i]
+        key = (min(u, v), max(u, v))
+        if key == prev:
+            continue
+        src.append(u)
+        dst.append(v)
+        if u != v:
+            src.append(v)
+            dst.append(u)
+        prev = key
+    return src, dst
+
+
+def read_edgelist(path: str, directed: bool = True, backend: str = "soa",
+                  integer_labels: bool = False) -> ResourceGraph:
+    if backend not in BACKENDS:
+        raise ValueError(f"backend {backend!r} is not accelerated here (only :soa)")
+    edges, next_id, labels = _parse_edgelist(os.path.expanduser(path), integer_labels)
+    src, dst = _expand_edges(edges, directed)
+    kind = "directed" if directed else "undirected"
+    # build_from_labels (resource_graph.ex:1531-1575): the builder carries only
+    # labels; its edge list stays empty
+    b = SoA(kind)
+    if integer_labels:
+        b.integer_labels = True
+        b.next_id = next_id
+    else:
+        for lab in labels:
+            b.ensure_node(lab)
+    res = Resource(next_id, np.asarray(src, dtype=np.uint32), np.asarray(dst, dtype=np.uint32))
+    return ResourceGraph(resource=res, builder=b)
+
+
+# ---------------------------------------------------------------------------
+# scores
+
+def _map_scores(builder: SoA, scores):
+    """map_scores (resource_graph.ex:2609-2619)."""
+    if builder.integer_labels:
+        return {i: s for i, s in enumerate(scores)}
+    return dict(zip(builder.all_labels(), scores))
+
+
+def _maybe_scale_undirected(builder: SoA, scores: np.ndarray) -> np.ndarray:
+    """maybe_scale_undirected (resource_graph.ex:2633-2637)."""
+    return scores * 0.5 if builder.kind == "undirected" else scores
+
+
+def betweenness_unweighted_array(graph: ResourceGraph, sources=None, info: Optional[dict] = None) -> np.ndarray:
+    """Unscaled directed-sum BC as f64[n_cap] (the NIF's return value)."""
+    res = graph.resource
+    out = np.zeros(res.n_cap, dtype=np.float64)
+    inf = _lib.ZgBcInfo()
+    if sources is None:
+        rc = _lib.lib().zg_bc_unweighted(res.handle, None, 0, _ptr(out), ctypes.byref(inf))
+    else:
+        s = np.ascontiguousarray(sources, dtype=np.uint32)
+        rc = _lib.lib().zg_bc_unweighted(res.handle, _ptr(s), len(s), _ptr(out), ctypes.byref(inf))
+    _lib.check(rc, "zg_bc_unweighted")
+    if info is not None:
+        info.update({k: getattr(inf, k) for k, _ in _lib.ZgBcInfo._fields_})
+    return out
+
+
+def betweenness_unweighted(graph: ResourceGraph, raw: bool = False, *, sources=None,
+                           info: Optional[dict] = None):
+    raw_scores = betweenness_unweighted_array(graph, sources=sources, info=info)
+    scores = _maybe_scale_undirected(graph.builder, raw_scores)
+    if raw:
+        return scores.tolist()
+    return _map_scores(graph.builder, scores.tolist())
+
+
+def betweenness_centrality(graph: ResourceGraph, raw: bool = False, **kw):
+    """Alias of ``betweenness_unweighted`` (same semantics)."""
+    return betweenness_unweighted(graph, raw=raw, **kw)
+
+
+def pagerank_array(graph: ResourceGraph, damping: float = 0.85, max_iterations: int = 100,
+                   tolerance: float = 0.0001, *, precision: int = 32,
+                   info: Optional[dict] = None) -> np.ndarray:
+    res = graph.resource
+    if max_iterations < 0:
+        raise ValueError("max_iterations must be non-negative")
+    out = np.zeros(res.n_cap, dtype=np.float64)
+    inf = _lib.ZgPrInfo()
+    rc = _lib.lib().zg_pagerank(res.handle, float(damping), int(max_iterations), float(tolerance),
+                                int(precision), _ptr(out), ctypes.byref(inf))
+    _lib.check(rc, "zg_pagerank")
+    if info is not None:
+        info.update({k: getattr(inf, k) for k, _ in _lib.ZgPrInfo._fields_})
+    return out
+
+
+def pagerank(graph: ResourceGraph, damping: float = 0.85, max_iterations: int = 100,
+             tolerance: float = 0.0001, raw: bool = False, *, precision: int = 32,
+             info: Optional[dict] = None):
+    scores = pagerank_array(graph, damping, max_iterations, tolerance, precision=precision, info=info)
+    if raw:
+        return scores.tolist()
+    return _map_scores(graph.builder, scores.tolist())
+
+
+def bc_paths(graph: ResourceGraph, sources, max_levels: int = 0):
+    """Per-source BFS depth (u32, 0xFFFFFFFF unreached), stored path counts
+    (f64) and per-level shifts; for parity checks."""
+    res = graph.resource
+    s = np.ascontiguousarray(sources, dtype=np.uint32)
+    V = res.n_cap
+    depth = np.empty((len(s), V), dtype=np.uint32)
+    sigma = np.empty((len(s), V), dtype=np.float64)
+    levels = np.empty(len(s), dtype=np.uint32)
+    ml = int(max_levels) if max_levels else V + 1
+    shift = np.zeros((len(s), ml), dtype=np.int32)
+    rc = _lib.lib().zg_bc_paths(res.handle, _ptr(s), len(s), _ptr(depth), _ptr(sigma), _ptr(shift), ml,
+                                _ptr(levels))
+    _lib.check(rc, "zg_bc_paths")
+    return depth, sigma, shift, levels
+
+
+def set_batch_groups(graph: ResourceGraph, groups: int) -> int:
+    rc = _lib.lib().zg_set_batch_groups(graph.resource.handle, int(groups))
+    if rc < 0:
+        _lib.check(rc, "zg_set_batch_groups")
+    return rc
+
+
+def set_profiling(graph: ResourceGraph, on: bool = True) -> None:
+    _lib.check(_lib.lib().zg_set_profiling(graph.resource.handle, 1 if on else 0), "zg_set_profiling")
+
+
+def get_profile(graph: ResourceGraph) -> dict:
+    """{kernel: (total device ms, launches)} since the last read."""
+    p = _lib.ZgProfile()
+    _lib.check(_lib.lib().zg_get_profile(graph.resource.handle, ctypes.byref(p)), "zg_get_profile")
+    return {name: (float(p.ms[i]), int(p.launches[i])) for i, name in enumerate(_lib.PROF_SLOTS)}
