@@ -12,24 +12,28 @@
 // with a single coalesced 512-byte access.  A batch sweeps up to
 // ZG_MAX_GROUPS groups in the same launches (blockIdx.y = group).
 //
-// Forward (level d -> d+1), pull over predecessor chunks: for a candidate
-// vertex v with unvisited lanes U, every predecessor u whose frontier mask
-// cur[u] meets U contributes sigma[u][l] for the lanes l in cur[u] & U.  The
-// lanes load 64 predecessors and their masks at once; a ballot picks the ones
-// that contribute, and each contributes one coalesced row read.  New lanes
-// are published in the next-frontier mask; a finalize kernel folds them into
-// the visited mask, appends the vertex's successor chunks (with the lane mask)
-// to the level-(d+1) work list using a wave-aggregated prefix sum, and
-// compacts the candidate list.
+// Forward (level d -> d+1), pull over predecessor chunks: for a vertex v with
+// unvisited lanes U, every predecessor u whose frontier mask cur[u] meets U
+// contributes sigma[u][l] for the lanes l in cur[u] & U.  The lanes load 64
+// predecessors and their masks at once; a ballot picks the ones that
+// contribute, and each contributes one coalesced row read.  New lanes are
+// published in the next-frontier mask.  A finalize kernel (fixed grid, one
+// contiguous chunk range per block) folds them into the visited mask and
+// appends the vertex's successor chunks (with the lane mask) to the
+// level-(d+1) work list, reserving space with one atomic per block.
 //
 // Backward (level d, from the deepest level up): for an entry (v, lanes at
 // depth d), delta_l(v) = sigma_l(v) * sum_{w in succ(v)} c_l(w), where
-// c_l(w) = (1 + delta_l(w)) / sigma_l(w) is published when w's level is done
-// and is still 0 (cleared at discovery) for lanes where w is not at d+1: an
-// arc v->w out of a reached v has depth(w) <= depth(v) + 1, so c_l(w) != 0
-// exactly when w is a BFS child of v for source l.  No depth compare, no
-// atomics for ordinary vertices, and the per-lane bc contributions are summed
-// across the wavefront before one read-modify-write per (vertex, level).
+// c_l(w) = (1 + delta_l(w)) / sigma_l(w) is published when w's level is done.
+// c is double-buffered by level parity: level d reads buffer (d+1)&1 and
+// writes buffer d&1, and both buffers are cleared for a lane when the lane
+// discovers w.  An arc v->w out of a reached v has depth(w) <= depth(v) + 1,
+// so in buffer (d+1)&1 the only lanes of w already written are those with
+// depth(w) = d+1: c_l(w) != 0 exactly when w is a BFS child of v for source
+// l, with no depth compare, and a same-level neighbour's fresh value (written
+// to the other buffer) is never seen.  No atomics for ordinary vertices; the
+// per-lane bc contributions are summed across the wavefront before one
+// read-modify-write per (vertex, level).
 //
 // Path counts are FP64 (exact integers below 2^53, equal to the reference's
 // u64 counts there) with a per-(source, level) power-of-two shift k_L taken
@@ -51,10 +55,10 @@ struct BcDev {
     const uint32_t *och_first, *och_v, *och_e0, *och_e1;
     const uint32_t *ohub_idx;
     const uint8_t *live;
-    double *sig, *cc, *hubacc, *bcg;
+    double *sig, *cc, *hubacc, *bcg;   // cc: [2][Gcap][V][64]
     uint32_t *dep;
     uint64_t *vis, *fr;         // fr: [3][Gcap][V]
-    uint32_t *cand;             // [2][Gcap][n_ich]
+    unsigned long long *stats;  // [Gcap][ZG_FIN_BLOCKS][2]
     uint32_t *flist;
     uint64_t *fmask;
     uint32_t *hlist;
@@ -73,6 +77,11 @@ struct BcDev {
 __device__ __forceinline__ uint64_t *fr_buf(const BcDev &a, int k)
 {
     return a.fr + (size_t)k * a.Gcap * a.V;
+}
+
+__device__ __forceinline__ double *cc_buf(const BcDev &a, int k)
+{
+    return a.cc + (size_t)k * a.Gcap * a.V * 64;
 }
 
 // Visited mask of every vertex starts as the group's unused lanes.
@@ -98,7 +107,6 @@ __global__ void bc_zero_hub_rows_kernel(BcDev a, const uint32_t *rows)
 __global__ void bc_source_kernel(BcDev a)
 {
     const uint32_t g = blockIdx.y, l = threadIdx.x;
-    if (l == 0) a.cnt[g].ncand[0] = a.n_ich;
     uint32_t s = a.src[g * 64 + l];
     unsigned long long arcs = 0, reached = 0;
     if (s != ZG_NONE) {
@@ -109,7 +117,8 @@ __global__ void bc_source_kernel(BcDev a)
             atomicOr((unsigned long long *)&a.vis[MASK(g, s)], (unsigned long long)bit);
             atomicOr((unsigned long long *)&fr_buf(a, 0)[MASK(g, s)], (unsigned long long)bit);
             a.sig[ROW(g, s) + l] = 1.0;
-            a.cc[ROW(g, s) + l] = 0.0;
+            cc_buf(a, 0)[ROW(g, s) + l] = 0.0;
+            cc_buf(a, 1)[ROW(g, s) + l] = 0.0;
             if (a.dep) a.dep[ROW(g, s) + l] = 0;
             uint32_t c0 = a.och_first[s], k = a.och_first[s + 1] - c0;
             uint32_t pos = atomicAdd(&a.cnt[g].fcount, k);
@@ -128,8 +137,9 @@ __global__ void bc_source_kernel(BcDev a)
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
     if (l == 0) {
-        atomicAdd(&a.cnt[g].arcs, arcs);
-        atomicAdd(&a.cnt[g].reached, reached);
+        // slot 0 belongs to finalize block 0, which runs strictly later
+        a.stats[(size_t)g * ZG_FIN_BLOCKS * 2] += arcs;
+        a.stats[(size_t)g * ZG_FIN_BLOCKS * 2 + 1] += reached;
     }
 }
 
@@ -138,11 +148,8 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d)
 {
     const uint32_t g = blockIdx.y;
     const int lane = threadIdx.x & 63;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[g].ncand[(d + 1) & 1] = 0;
-    const uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t nc = a.cnt[g].ncand[d & 1];
-    if (wid >= nc) return;
-    const uint32_t ch = d == 0 ? wid : a.cand[((size_t)(d & 1) * a.Gcap + g) * a.n_ich + wid];
+    const uint32_t ch = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ch >= a.n_ich) return;
     const uint32_t v = a.ich_v[ch], e0 = a.ich_e0[ch], e1 = a.ich_e1[ch];
     const uint64_t unvis = ~a.vis[MASK(g, v)];
     if (unvis == 0) return;
@@ -182,7 +189,8 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d)
             const int ex = ilogb(val);
             if (ex > ZG_SHIFT_T) atomicMax(&a.emax[EMAX(d + 1, g, lane)], ex);
         }
-        a.cc[r] = 0.0;
+        cc_buf(a, 0)[r] = 0.0;
+        cc_buf(a, 1)[r] = 0.0;
         if (a.dep) a.dep[r] = (uint32_t)(d + 1);
     }
     if (lane == 0) {
@@ -192,77 +200,63 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d)
     }
 }
 
-// Finalize level d: fold level d+1 into the visited masks, append its work
-// lists, compact the candidates, clear the frontier buffer of level d-1.
-__global__ __launch_bounds__(256) void bc_finalize_kernel(BcDev a, int d, GroupRange stale)
+// Exclusive prefix sum over a 1024-thread block (16 wavefronts); returns the
+// block total through *total.  All threads must call it.
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t x, uint32_t *lds, uint32_t *total)
 {
-    const uint32_t g = blockIdx.y;
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    if (i < stale.cnt[g]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t incl = zg_wave_incl_scan(x, lane);
+    if (lane == 63) lds[wv] = incl;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t t = threadIdx.x < 16 ? lds[threadIdx.x] : 0;
+        uint32_t ti = zg_wave_incl_scan(t, lane);
+        if (threadIdx.x < 16) lds[16 + threadIdx.x] = ti - t;  // exclusive wave offsets
+        if (threadIdx.x == 15) lds[32] = ti;
+    }
+    __syncthreads();
+    const uint32_t r = lds[16 + wv] + incl - x;
+    *total = lds[32];
+    __syncthreads();
+    return r;
+}
+
+// Finalize level d: fold level d+1 into the visited masks, append its work
+// lists, clear the frontier buffer of level d-1.  Block b owns predecessor
+// chunks [b*R, (b+1)*R); a vertex is handled by the block holding its first
+// chunk.
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, int d, GroupRange stale)
+{
+    __shared__ uint32_t lds[40];
+    __shared__ uint32_t base[2];
+    __shared__ unsigned long long sred[2][ZG_FIN_THREADS / 64];
+    const uint32_t g = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+    const int lane = tid & 63;
+    for (uint32_t i = b * ZG_FIN_THREADS + tid; i < stale.cnt[g]; i += ZG_FIN_BLOCKS * ZG_FIN_THREADS) {
         uint32_t oc = a.flist[g * a.fcap + stale.off[g] + i];
         fr_buf(a, (d + 2) % 3)[MASK(g, a.och_v[oc])] = 0;
     }
-    const uint32_t nc = a.cnt[g].ncand[d & 1];
-    uint32_t v = 0, ch = 0, kf = 0, kh = 0, kc = 0;
-    uint64_t nm = 0;
-    bool disc = false;
-    if (i < nc) {
-        ch = d == 0 ? i : a.cand[((size_t)(d & 1) * a.Gcap + g) * a.n_ich + i];
-        v = a.ich_v[ch];
-        nm = fr_buf(a, (d + 1) % 3)[MASK(g, v)];
-        const uint64_t vo = a.vis[MASK(g, v)];
-        disc = nm != 0 && a.ich_e0[ch] == a.in_ptr[v];
-        if (disc) {
-            a.vis[MASK(g, v)] = vo | nm;
-            kf = a.och_first[v + 1] - a.och_first[v];
-            kh = kf > 1 ? 1 : 0;
-        }
-        kc = (vo | nm) != ~0ull ? 1 : 0;
-    }
-    // wave-aggregated appends
-    const uint32_t sf = zg_wave_incl_scan(kf, lane);
-    const uint32_t sh = zg_wave_incl_scan(kh, lane);
-    const uint32_t sc = zg_wave_incl_scan(kc, lane);
-    const uint32_t tf = zg_readlane32(sf, 63), th = zg_readlane32(sh, 63), tc = zg_readlane32(sc, 63);
-    uint32_t bf = 0, bh = 0, bcand = 0;
-    if (lane == 63) {
-        if (tf) bf = atomicAdd(&a.cnt[g].fcount, tf);
-        if (th) bh = atomicAdd(&a.cnt[g].hcount, th);
-        if (tc) bcand = atomicAdd(&a.cnt[g].ncand[(d + 1) & 1], tc);
-    }
-    bf = zg_readlane32(bf, 63);
-    bh = zg_readlane32(bh, 63);
-    bcand = zg_readlane32(bcand, 63);
-    if (kf) {
-        const uint32_t pos = bf + sf - kf, c0 = a.och_first[v];
-        for (uint32_t q = 0; q < kf; q++) {
-            if (pos + q < a.fcap) {
-                a.flist[g * a.fcap + pos + q] = c0 + q;
-                a.fmask[g * a.fcap + pos + q] = nm;
-            } else {
-                atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
-            }
-        }
-    }
-    if (kh) {
-        const uint32_t pos = bh + sh - 1;
-        if (pos < a.hcap) {
-            a.hlist[g * a.hcap + pos] = v;
-            a.hmask[g * a.hcap + pos] = nm;
-        } else {
-            atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
-        }
-    }
-    if (kc) a.cand[((size_t)((d + 1) & 1) * a.Gcap + g) * a.n_ich + bcand + sc - 1] = ch;
-    // per-source work statistics
+    const uint64_t *next = fr_buf(a, (d + 1) % 3) + (size_t)g * a.V;
+    const uint32_t R = (a.n_ich + ZG_FIN_BLOCKS - 1) / ZG_FIN_BLOCKS;
+    const uint32_t c0 = min(b * R, a.n_ich), c1 = min(c0 + R, a.n_ich);
+
+    // pass A: fold masks, count list entries, statistics
+    uint32_t nf = 0, nh = 0;
     unsigned long long arcs = 0, reached = 0;
-    if (disc) {
+    for (uint32_t c = c0 + tid; c < c1; c += ZG_FIN_THREADS) {
+        const uint32_t v = a.ich_v[c];
+        if (a.ich_e0[c] != a.in_ptr[v]) continue;
+        const uint64_t nm = next[v];
+        if (!nm) continue;
+        a.vis[MASK(g, v)] |= nm;
+        const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
+        nf += kf;
+        nh += kf > 1 ? 1 : 0;
         const unsigned long long pc = __popcll(nm);
-        arcs = pc * (a.out_ptr[v + 1] - a.out_ptr[v]);
-        reached = pc;
-        // vertices summed by atomics: their level-(d+1) exponent is known only now
+        arcs += pc * (a.out_ptr[v + 1] - a.out_ptr[v]);
+        reached += pc;
         if (a.in_ptr[v + 1] - a.in_ptr[v] > ZG_CHUNK) {
+            // summed by atomics: the level-(d+1) exponent is known only now
             uint64_t mm = nm;
             while (mm) {
                 const int l = __builtin_ctzll(mm);
@@ -272,11 +266,73 @@ __global__ __launch_bounds__(256) void bc_finalize_kernel(BcDev a, int d, GroupR
             }
         }
     }
+    uint32_t tf, th;
+    (void)block_excl_scan_1024(nf, lds, &tf);
+    (void)block_excl_scan_1024(nh, lds, &th);
+    if (tid == 0) {
+        base[0] = tf ? atomicAdd(&a.cnt[g].fcount, tf) : 0;
+        base[1] = th ? atomicAdd(&a.cnt[g].hcount, th) : 0;
+    }
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
-    if (lane == 0 && (arcs | reached)) {
-        atomicAdd(&a.cnt[g].arcs, arcs);
-        atomicAdd(&a.cnt[g].reached, reached);
+    if (lane == 0) {
+        sred[0][tid >> 6] = arcs;
+        sred[1][tid >> 6] = reached;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long A = 0, Rr = 0;
+        for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
+            A += sred[0][w];
+            Rr += sred[1][w];
+        }
+        unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * 2;
+        st[0] += A;
+        st[1] += Rr;
+    }
+    if (tf == 0 && th == 0) return;
+
+    // pass B: write the entries, tile by tile
+    uint32_t fpos = base[0], hpos = base[1];
+    for (uint32_t t0 = c0; t0 < c1; t0 += ZG_FIN_THREADS) {
+        const uint32_t c = t0 + tid;
+        uint32_t v = 0, kf = 0, kh = 0;
+        uint64_t nm = 0;
+        if (c < c1) {
+            v = a.ich_v[c];
+            if (a.ich_e0[c] == a.in_ptr[v]) {
+                nm = next[v];
+                if (nm) {
+                    kf = a.och_first[v + 1] - a.och_first[v];
+                    kh = kf > 1 ? 1 : 0;
+                }
+            }
+        }
+        uint32_t sf_tot, sh_tot;
+        const uint32_t of = block_excl_scan_1024(kf, lds, &sf_tot);
+        const uint32_t oh = block_excl_scan_1024(kh, lds, &sh_tot);
+        if (kf) {
+            const uint32_t pos = fpos + of, q0 = a.och_first[v];
+            for (uint32_t q = 0; q < kf; q++) {
+                if (pos + q < a.fcap) {
+                    a.flist[g * a.fcap + pos + q] = q0 + q;
+                    a.fmask[g * a.fcap + pos + q] = nm;
+                } else {
+                    atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+                }
+            }
+        }
+        if (kh) {
+            const uint32_t pos = hpos + oh;
+            if (pos < a.hcap) {
+                a.hlist[g * a.hcap + pos] = v;
+                a.hmask[g * a.hcap + pos] = nm;
+            } else {
+                atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+            }
+        }
+        fpos += sf_tot;
+        hpos += sh_tot;
     }
 }
 
@@ -291,7 +347,7 @@ __device__ __forceinline__ void bc_finish(const BcDev &a, uint32_t g, uint32_t v
         const double sv = a.sig[r];
         const int k = a.emax[EMAX(d, g, lane)];
         const double delta = sv * ldexp(S, -k);
-        a.cc[r] = (1.0 + delta) / sv;
+        cc_buf(a, d & 1)[r] = (1.0 + delta) / sv;
         if (a.src[g * 64 + lane] != v) contrib = delta;
     }
     contrib = zg_wave_sum(contrib);
@@ -310,7 +366,7 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     const uint64_t mask = a.fmask[idx];
     const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
     const bool active = (mask >> lane) & 1;
-    const double *cg = a.cc + ROW(g, 0) + lane;
+    const double *cg = cc_buf(a, (d + 1) & 1) + ROW(g, 0) + lane;
     double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
@@ -388,7 +444,7 @@ int zg_bc_ws_release(zg_graph *g)
 {
     zg_bc_ws &w = g->bc;
     zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep);
-    zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.cand); zg_dfree(w.flist); zg_dfree(w.fmask);
+    zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.flist); zg_dfree(w.fmask);
     zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.src); zg_dfree(w.cnt);
     if (w.h_cnt) (void)hipHostFree(w.h_cnt);
     w.h_cnt = nullptr;
@@ -409,13 +465,13 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth)
     w.lcap = 256;
     int rc = ZG_OK;
     if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, G * V * 64);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, G * V * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, 2 * G * V * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hubacc, G * (size_t)g->n_ohub * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.bcg, G * V);
     if (rc == ZG_OK && with_depth) rc = zg_dmalloc(&w.dep, G * V * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.cand, 2 * G * (size_t)g->n_ich);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * 2);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.flist, G * w.fcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fmask, G * w.fcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hlist, G * w.hcap);
@@ -475,7 +531,7 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used)
     a.ohub_idx = g->ohub_idx;
     a.live = g->live;
     a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep;
-    a.vis = w.vis; a.fr = w.fr; a.cand = w.cand;
+    a.vis = w.vis; a.fr = w.fr; a.stats = w.stats;
     a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
     a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
     a.fcap = w.fcap; a.hcap = w.hcap;
@@ -511,46 +567,41 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStrea
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
         ZG_TRY(hipMemsetAsync(w.fr, 0, sizeof(uint64_t) * 3 * (size_t)Gcap * V, s));
         ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
+        ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * 2, s));
         ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
         if (w.dep) ZG_TRY(hipMemsetAsync(w.dep, 0xFF, sizeof(uint32_t) * (size_t)Gcap * V * 64, s));
         if (V) ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_init_vis_kernel, dim3(cdiv(V, 256), G), dim3(256), 0, s, a, valid);
         if (g->n_inhub) ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_hub_rows_kernel, dim3(g->n_inhub, G), dim3(64), 0, s, a, g->inhub);
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_source_kernel, dim3(1, G), dim3(64), 0, s, a);
         ZG_TRY(hipGetLastError());
-        // |F_0| per group = sum of successor-chunk counts of its sources:
-        // read back once with the level-0 counters below.
         for (uint32_t q = 0; q < G; q++) {
             foff[q].assign(1, 0);
             hoff[q].assign(1, 0);
         }
+        // level-0 list sizes (successor chunks of the sources)
         ZG_TRY(hipMemcpyAsync(w.h_cnt, w.cnt, sizeof(zg_bc_ws::Counters) * G, hipMemcpyDeviceToHost, s));
         ZG_TRY(hipStreamSynchronize(s));
-        std::vector<uint32_t> ncand(G);
         for (uint32_t q = 0; q < G; q++) {
             if (w.h_cnt[q].err & ZG_BC_ERR_SOURCE) return ZG_ERR_NODE_NOT_FOUND;
+            if (w.h_cnt[q].err) return ZG_ERR_HIP;
             foff[q].push_back(w.h_cnt[q].fcount);  // end of level 0
             hoff[q].push_back(w.h_cnt[q].hcount);
-            ncand[q] = g->n_ich;
         }
+        const uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(g->n_ich, 4));
         int d = 0;
         for (;;) {
             ZG_TRY_RC(bc_grow_levels(g, (uint32_t)d + 2, s));
             a.emax = w.emax;
-            uint32_t mc = 0;
-            for (uint32_t q = 0; q < G; q++) mc = std::max(mc, ncand[q]);
-            if (mc) ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(cdiv(mc, 4), G), dim3(256), 0, s, a, d);
-            else ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(1, G), dim3(256), 0, s, a, d);
+            if (g->n_ich) ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(fwd_blocks, G), dim3(256), 0, s, a, d);
             GroupRange stale;
             memset(&stale, 0, sizeof(stale));
-            uint32_t ms = 0;
             if (d >= 1)
                 for (uint32_t q = 0; q < G; q++) {
                     stale.off[q] = foff[q][d - 1];
                     stale.cnt[q] = foff[q][d] - foff[q][d - 1];
-                    ms = std::max(ms, stale.cnt[q]);
                 }
-            uint32_t mf = std::max(mc, ms);
-            if (mf) ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_kernel, dim3(cdiv(mf, 256), G), dim3(256), 0, s, a, d, stale);
+            ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_kernel, dim3(ZG_FIN_BLOCKS, G), dim3(ZG_FIN_THREADS), 0, s,
+                           a, d, stale);
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(w.h_cnt, w.cnt, sizeof(zg_bc_ws::Counters) * G, hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
@@ -561,15 +612,19 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStrea
                 if (fc > foff[q].back()) any = true;
                 foff[q].push_back(fc);
                 hoff[q].push_back(w.h_cnt[q].hcount);
-                ncand[q] = w.h_cnt[q].ncand[(d + 1) & 1];
             }
             if (!any) break;
             d++;
         }
         const int maxd = d;  // deepest non-empty level
-        for (uint32_t q = 0; q < G; q++) {
-            inf->arcs_traversed += w.h_cnt[q].arcs;
-            inf->nodes_reached += w.h_cnt[q].reached;
+        {
+            std::vector<unsigned long long> st((size_t)G * ZG_FIN_BLOCKS * 2);
+            ZG_TRY(hipMemcpyAsync(st.data(), w.stats, sizeof(unsigned long long) * st.size(), hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+            for (size_t i = 0; i < st.size(); i += 2) {
+                inf->arcs_traversed += st[i];
+                inf->nodes_reached += st[i + 1];
+            }
         }
         inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)maxd + 1);
         w.last_levels = (uint32_t)maxd + 2;

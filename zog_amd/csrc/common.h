@@ -25,6 +25,11 @@
 
 #define ZG_MAX_GROUPS 16
 
+// The finalize step runs as a fixed grid: each block owns a contiguous range
+// of predecessor chunks, so it reserves list space with one atomic per level.
+#define ZG_FIN_BLOCKS 256
+#define ZG_FIN_THREADS 1024
+
 // Error bits a Brandes kernel reports to the host.
 #define ZG_BC_ERR_SOURCE 1u     // a source id is out of range or not live
 #define ZG_BC_ERR_CAPACITY 2u   // a work list would overflow its buffer
@@ -74,8 +79,9 @@ static inline void zg_dfree(T *&p)
 // ---------------------------------------------------------------------------
 // Brandes batched-sweep workspace (sized for `groups` 64-source groups).
 // Layouts, per group g (lane = source slot within the group):
-//   sig[g][v][lane]  FP64 stored path counts (row of 64 = 512 B per vertex)
-//   cc [g][v][lane]  FP64 (1 + delta) / sigma of finished vertices
+//   sig[g][v][lane]    FP64 stored path counts (row of 64 = 512 B per vertex)
+//   cc [k][g][v][lane] FP64 (1 + delta) / sigma of finished vertices, k =
+//                      level parity (a level reads k = (d+1)&1, writes d&1)
 //   vis[g][v], fr[k][g][v]  u64 lane masks (visited, frontier buffers)
 //   flist/fmask[g][i]       per-level successor-chunk work lists
 //   hlist/hmask[g][i]       per-level lists of multi-chunk vertices
@@ -86,7 +92,7 @@ struct zg_bc_ws {
     double *sig = nullptr, *cc = nullptr, *hubacc = nullptr, *bcg = nullptr;
     uint32_t *dep = nullptr;
     uint64_t *vis = nullptr, *fr = nullptr;
-    uint32_t *cand = nullptr;            // [2][G][n_ich]
+    unsigned long long *stats = nullptr; // [G][ZG_FIN_BLOCKS][2] arcs, reached
     uint32_t *flist = nullptr;           // [G][fcap]
     uint64_t *fmask = nullptr;
     uint32_t *hlist = nullptr;           // [G][hcap]
@@ -95,8 +101,7 @@ struct zg_bc_ws {
     uint32_t lcap = 0;
     uint32_t *src = nullptr;             // [G][64]
     struct Counters {
-        uint32_t fcount, hcount, ncand[2];
-        unsigned long long arcs, reached;
+        uint32_t fcount, hcount;
         uint32_t err;   // ZG_BC_ERR_* bits
         uint32_t pad;
     };

@@ -10,17 +10,19 @@
 // (or FP64 in the FP64 mode), double-buffered by iteration parity.  Each row
 // sum is a compensated (Kahan) sum; partial sums of one row computed by
 // different lanes are merged with an error-free TwoSum.  The l1 change and the
-// dangling mass are reduced in FP64: every block writes its partials, and the
-// last block to finish (an atomic ticket) adds them in a fixed order, so the
-// result is deterministic, and prepares the next iteration's constant term
-// base = (1-d)/n + d*sink.  One kernel launch per iteration; launches after
-// convergence return at once, so several iterations are queued per host sync.
+// dangling mass are reduced in FP64: every block writes its partials, and a
+// one-block finish kernel adds them in a fixed order (deterministic result),
+// finishes rows longer than one segment and prepares the next iteration's
+// constant term base = (1-d)/n + d*sink.  The iteration kernel runs a fixed
+// grid that loops over its work units (no global atomics, no fences);
+// launches after convergence return at once, so several iterations are
+// queued per host sync.
 #include "common.h"
 #include "pr_bins.h"
 
 struct PrState {
-    unsigned int ticket;
     unsigned int done;
+    unsigned int pad;
     unsigned long long iter;
     unsigned long long max_iter;
     double base;      // (1-d)/n + d*sink for the next iteration to run
@@ -94,8 +96,7 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
 {
     __shared__ double red[2][ZG_PR_BLOCK / 64];
     __shared__ T kred[2][ZG_PR_BLOCK / 64];
-    __shared__ int is_last;
-    PrState *st = a.st;
+    const PrState *st = a.st;
     if (st->done) return;
     const unsigned long long it = st->iter;
     const bool odd = it & 1;
@@ -105,50 +106,53 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
     T *xnew = odd ? a.x0 : a.x1;
     const double base = st->base, damping = st->damping;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint32_t b = blockIdx.x;
+    const uint32_t units = a.nb_small + a.nb_med + a.n_seg;
     double l1 = 0.0, dang = 0.0;
 
-    if (b < a.nb_small) {
-        // one lane per row (<= ZG_PR_SMALL predecessors)
-        uint32_t i = b * ZG_PR_BLOCK + tid;
-        if (i < a.n_small) {
-            uint32_t v = a.small[i];
+    for (uint32_t b = blockIdx.x; b < units; b += gridDim.x) {
+        if (b < a.nb_small) {
+            // one lane per row (<= ZG_PR_SMALL predecessors)
+            const uint32_t i = b * ZG_PR_BLOCK + tid;
+            if (i < a.n_small) {
+                const uint32_t v = a.small[i];
+                T s = 0, c = 0;
+                for (uint32_t e = a.in_ptr[v], e1 = a.in_ptr[v + 1]; e < e1; e++) kahan_add(s, c, x[a.in_col[e]]);
+                pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
+            }
+        } else if (b < a.nb_small + a.nb_med) {
+            // one wavefront per row
+            const uint32_t i = (b - a.nb_small) * (ZG_PR_BLOCK / 64) + wv;
+            if (i < a.n_med) {
+                const uint32_t v = a.med[i];
+                T s = 0, c = 0;
+                for (uint32_t e = a.in_ptr[v] + lane, e1 = a.in_ptr[v + 1]; e < e1; e += 64)
+                    kahan_add(s, c, x[a.in_col[e]]);
+                wave_kahan_merge(s, c);
+                if (lane == 0) pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
+            }
+        } else {
+            // one block per segment of a long row; pr_finish_kernel finishes the row
+            const uint32_t sg = b - a.nb_small - a.nb_med;
             T s = 0, c = 0;
-            for (uint32_t e = a.in_ptr[v], e1 = a.in_ptr[v + 1]; e < e1; e++) kahan_add(s, c, x[a.in_col[e]]);
-            pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
-        }
-    } else if (b < a.nb_small + a.nb_med) {
-        // one wavefront per row
-        uint32_t i = (b - a.nb_small) * (ZG_PR_BLOCK / 64) + wv;
-        if (i < a.n_med) {
-            uint32_t v = a.med[i];
-            T s = 0, c = 0;
-            for (uint32_t e = a.in_ptr[v] + lane, e1 = a.in_ptr[v + 1]; e < e1; e += 64)
+            for (uint32_t e = a.seg_e0[sg] + tid, e1 = a.seg_e1[sg]; e < e1; e += ZG_PR_BLOCK)
                 kahan_add(s, c, x[a.in_col[e]]);
             wave_kahan_merge(s, c);
-            if (lane == 0) pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
-        }
-    } else {
-        // one block per segment of a long row; the last block finishes the row
-        uint32_t sg = b - a.nb_small - a.nb_med;
-        T s = 0, c = 0;
-        for (uint32_t e = a.seg_e0[sg] + tid, e1 = a.seg_e1[sg]; e < e1; e += ZG_PR_BLOCK)
-            kahan_add(s, c, x[a.in_col[e]]);
-        wave_kahan_merge(s, c);
-        if (lane == 0) {
-            kred[0][wv] = s;
-            kred[1][wv] = c;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            T S = kred[0][0], C = kred[1][0];
-            for (int w = 1; w < ZG_PR_BLOCK / 64; w++) kahan_merge(S, C, kred[0][w], kred[1][w]);
-            a.seg[2 * sg] = S;
-            a.seg[2 * sg + 1] = C;
+            if (lane == 0) {
+                kred[0][wv] = s;
+                kred[1][wv] = c;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                T S = kred[0][0], C = kred[1][0];
+                for (int w = 1; w < ZG_PR_BLOCK / 64; w++) kahan_merge(S, C, kred[0][w], kred[1][w]);
+                a.seg[2 * sg] = S;
+                a.seg[2 * sg + 1] = C;
+            }
+            __syncthreads();
         }
     }
 
-    // block partials of (l1, dangling mass)
+    // block partials of (l1, dangling mass); added in a fixed order later
     l1 = zg_wave_sum(l1);
     dang = zg_wave_sum(dang);
     if (lane == 0) {
@@ -162,33 +166,41 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
             L += red[0][w];
             D += red[1][w];
         }
-        a.part[2 * b] = L;
-        a.part[2 * b + 1] = D;
-        __threadfence();
-        unsigned int t = atomicAdd(&st->ticket, 1u);
-        is_last = (t == gridDim.x - 1);
+        a.part[2 * blockIdx.x] = L;
+        a.part[2 * blockIdx.x + 1] = D;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
 
-    // Last block: finish the long rows from their segment partials, then add
-    // all block partials in a fixed order.
-    l1 = 0.0;
-    dang = 0.0;
-    for (uint32_t i = tid; i < a.n_large; i += ZG_PR_BLOCK) {
-        uint32_t s0 = a.large_seg0[i], s1 = a.large_seg0[i + 1];
+#define ZG_PR_FINISH_THREADS 1024
+
+// One block: finish the long rows from their segment partials, add all block
+// partials in a fixed order, advance the iteration state.
+template <typename T>
+__global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<T> a, uint32_t nparts)
+{
+    __shared__ double red[2][ZG_PR_FINISH_THREADS / 64];
+    PrState *st = a.st;
+    if (st->done) return;
+    const unsigned long long it = st->iter;
+    const bool odd = it & 1;
+    const T *rold = odd ? a.r1 : a.r0;
+    T *rnew = odd ? a.r0 : a.r1;
+    T *xnew = odd ? a.x0 : a.x1;
+    const double base = st->base, damping = st->damping;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double l1 = 0.0, dang = 0.0;
+    for (uint32_t i = tid; i < a.n_large; i += ZG_PR_FINISH_THREADS) {
+        const uint32_t s0 = a.large_seg0[i], s1 = a.large_seg0[i + 1];
         T S = a.seg[2 * s0], C = a.seg[2 * s0 + 1];
         for (uint32_t q = s0 + 1; q < s1; q++) kahan_merge(S, C, a.seg[2 * q], a.seg[2 * q + 1]);
         pr_row_epilogue(a, rold, rnew, xnew, a.large[i], S, C, base, damping, l1, dang);
     }
-    for (uint32_t q = tid; q < gridDim.x; q += ZG_PR_BLOCK) {
+    for (uint32_t q = tid; q < nparts; q += ZG_PR_FINISH_THREADS) {
         l1 += a.part[2 * q];
         dang += a.part[2 * q + 1];
     }
     l1 = zg_wave_sum(l1);
     dang = zg_wave_sum(dang);
-    __syncthreads();
     if (lane == 0) {
         red[0][wv] = l1;
         red[1][wv] = dang;
@@ -196,17 +208,15 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
     __syncthreads();
     if (tid == 0) {
         double L = 0.0, D = 0.0;
-        for (int w = 0; w < ZG_PR_BLOCK / 64; w++) {
+        for (int w = 0; w < ZG_PR_FINISH_THREADS / 64; w++) {
             L += red[0][w];
             D += red[1][w];
         }
-        unsigned long long nit = it + 1;
+        const unsigned long long nit = it + 1;
         st->iter = nit;
         st->last_l1 = L;
         st->base = (1.0 - damping) * st->inv_n + damping * (D * st->inv_n);
         if (L < st->tol || nit >= st->max_iter) st->done = 1;
-        st->ticket = 0;
-        __threadfence();
     }
 }
 
@@ -254,11 +264,15 @@ int zg_pr_ws_release(zg_graph *g)
     return ZG_OK;
 }
 
+// Blocks of the iteration kernel: one per unit up to a fixed cap (the
+// kernel loops over units with a grid stride).
+#define ZG_PR_MAX_GRID 2048
 static uint32_t pr_grid(const zg_graph *g)
 {
     uint32_t nb_small = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK;
     uint32_t nb_med = (g->n_med + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
-    return nb_small + nb_med + g->n_seg;
+    uint32_t units = nb_small + nb_med + g->n_seg;
+    return units < ZG_PR_MAX_GRID ? (units ? units : 1) : ZG_PR_MAX_GRID;
 }
 
 static int pr_ws_ensure(zg_graph *g)
@@ -318,8 +332,12 @@ static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st,
     uint64_t launched = 0;
     while (launched < max_iter) {
         uint64_t k = max_iter - launched < K ? max_iter - launched : K;
-        for (uint64_t i = 0; i < k; i++)
-            ZG_PROF_LAUNCH(ZG_PROF_PR_ITER, pr_iter_kernel<T>, dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+        for (uint64_t i = 0; i < k; i++) {
+            const size_t p = zg_prof_begin(g, s);
+            hipLaunchKernelGGL(pr_iter_kernel<T>, dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(pr_finish_kernel<T>, dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, grid);
+            zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
+        }
         ZG_TRY(hipGetLastError());
         launched += k;
         ZG_TRY(hipMemcpyAsync(h_st, a.st, sizeof(PrState), hipMemcpyDeviceToHost, s));
@@ -432,6 +450,7 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             ZG_TRY(hipGetLastError());
             PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st + 1);
             hipLaunchKernelGGL(pr_iter_kernel<double>, dim3(pr_grid(g)), dim3(ZG_PR_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(pr_finish_kernel<double>, dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, pr_grid(g));
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(h_st + 1, d_st + 1, sizeof(PrState), hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
