@@ -133,6 +133,14 @@ int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sources, uint32
  * 256 sources per launch).  Returns the previous value, or <0. */
 int zg_set_batch_groups(zg_graph *g, int groups);
 
+/* Source grouping for the batched sweep (no reference counterpart; the
+ * result is the same sum, only FP64 summation order changes).  mode 0 keeps
+ * the caller's order; mode 1 (default) sorts the sources by their distance to
+ * the graph's hub core (the highest in-degree vertices) so the 64 sources of
+ * a group reach most vertices at the same BFS level.  Returns the previous
+ * mode, or <0. */
+int zg_set_source_order(zg_graph *g, int mode);
+
 /* Kernel-level profile of the most recent Brandes / PageRank call (no
  * reference counterpart; for the benchmark's roofline figures).  While
  * profiling is on, HIP events are recorded on the launch stream around every

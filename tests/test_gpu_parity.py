@@ -193,12 +193,13 @@ def test_rmat_small_sources_subset():
     check_bc(n, src, dst, sources=sources, groups=4)
 
 
-def test_batch_widths_agree():
+def test_batch_widths_and_source_order_agree():
     n, src, dst = gen.rmat(11, 1 << 14, 3)
     sources = np.arange(n, dtype=np.uint32)
     outs = []
-    for gr in (1, 3, 16):
+    for gr, order in ((1, 1), (3, 0), (16, 1), (4, 0)):
         g = make(n, src, dst, groups=gr)
+        rg.set_source_order(g, order)
         outs.append(rg.betweenness_unweighted_array(g, sources=sources))
         rg.destroy(g)
     for o in outs[1:]:
@@ -285,6 +286,25 @@ def test_pagerank_hub_rows_and_wiki_vote():
     dst = np.concatenate([np.full(30000, 5), rng.integers(0, 20000, 40000)]).astype(np.uint32)
     got, ref, it_ref, info = check_pr(20000, src, dst, iters=40, tol=0.0)
     assert np.abs(got - ref).sum() <= PR_L1
+
+
+def test_pagerank_every_row_bin():
+    # predecessor counts in every bin: 0, <=16, <=64, <=256, <=2048, one and
+    # several 4096-arc segments; parallel arcs and self-loops included
+    rng = np.random.default_rng(12)
+    n = 6000
+    degs = [0, 3, 16, 17, 64, 65, 200, 256, 257, 1500, 2048, 2049, 4096, 4097, 9000]
+    src, dst = [], []
+    for i, dgr in enumerate(degs):
+        src.append(rng.integers(0, n, dgr))
+        dst.append(np.full(dgr, 100 + i))
+    s2 = rng.integers(0, n, 30000)
+    d2 = rng.integers(0, n, 30000)
+    src = np.concatenate(src + [s2, np.array([7, 7, 7])]).astype(np.uint32)
+    dst = np.concatenate(dst + [d2, np.array([7, 7, 8])]).astype(np.uint32)
+    for prec, bar in ((32, PR_L1), (64, 1e-12)):
+        got, ref, it_ref, info = check_pr(n, src, dst, iters=30, tol=0.0, precision=prec)
+        assert np.abs(got - ref).sum() <= bar
 
 
 def test_pagerank_max_iterations_zero_and_damping():

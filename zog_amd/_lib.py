@@ -37,6 +37,7 @@ EXPORTED = (
     "zg_pagerank_device",
     "zg_bc_paths",
     "zg_set_batch_groups",
+    "zg_set_source_order",
     "zg_set_profiling",
     "zg_get_profile",
     "zg_strerror",
@@ -117,6 +118,8 @@ def lib():
     L.zg_bc_paths.restype = ctypes.c_int
     L.zg_set_batch_groups.argtypes = [P, ctypes.c_int]
     L.zg_set_batch_groups.restype = ctypes.c_int
+    L.zg_set_source_order.argtypes = [P, ctypes.c_int]
+    L.zg_set_source_order.restype = ctypes.c_int
     L.zg_set_profiling.argtypes = [P, ctypes.c_int]
     L.zg_set_profiling.restype = ctypes.c_int
     L.zg_get_profile.argtypes = [P, ctypes.POINTER(ZgProfile)]

@@ -57,6 +57,7 @@ struct BcDev {
     const uint8_t *live;
     double *sig, *cc, *hubacc, *bcg;   // cc: [2][Gcap][V][64]
     uint32_t *dep;
+    uint32_t *maxdep;           // [Gcap][V]
     uint64_t *vis, *fr;         // fr: [3][Gcap][V]
     unsigned long long *stats;  // [Gcap][ZG_FIN_BLOCKS][2]
     uint32_t *flist;
@@ -120,6 +121,7 @@ __global__ void bc_source_kernel(BcDev a)
             cc_buf(a, 0)[ROW(g, s) + l] = 0.0;
             cc_buf(a, 1)[ROW(g, s) + l] = 0.0;
             if (a.dep) a.dep[ROW(g, s) + l] = 0;
+            a.maxdep[MASK(g, s)] = 0;
             uint32_t c0 = a.och_first[s], k = a.och_first[s + 1] - c0;
             uint32_t pos = atomicAdd(&a.cnt[g].fcount, k);
             for (uint32_t q = 0; q < k; q++) {
@@ -166,13 +168,28 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d)
             f = cur[u] & unvis;
         }
         uint64_t act = __ballot(f != 0);
+        // contributing predecessors, eight row reads in flight per round
         while (act) {
-            const int j = __builtin_ctzll(act);
-            act &= act - 1;
-            const uint32_t uj = zg_readlane32(u, j);
-            const uint64_t fj = zg_readlane64(f, j);
-            m |= fj;
-            if ((fj >> lane) & 1) acc += sg[(size_t)uj * 64 + lane];
+            uint32_t uj[8];
+            uint64_t fj[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (act) {
+                    const int j = __builtin_ctzll(act);
+                    act &= act - 1;
+                    uj[q] = zg_readlane32(u, j);
+                    fj[q] = zg_readlane64(f, j);
+                } else {
+                    uj[q] = 0;
+                    fj[q] = 0;
+                }
+                m |= fj[q];
+            }
+            double val[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) val[q] = ((fj[q] >> lane) & 1) ? sg[(size_t)uj[q] * 64 + lane] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc += val[q];
         }
     }
     if (m == 0) return;
@@ -249,6 +266,7 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
         const uint64_t nm = next[v];
         if (!nm) continue;
         a.vis[MASK(g, v)] |= nm;
+        a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);  // levels only grow
         const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
         nf += kf;
         nh += kf > 1 ? 1 : 0;
@@ -367,28 +385,41 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
     const bool active = (mask >> lane) & 1;
     const double *cg = cc_buf(a, (d + 1) & 1) + ROW(g, 0) + lane;
-    double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
+    // Only successors that some lane reached at depth >= d+1 can be BFS
+    // children; the others' rows are not read.
+    const uint32_t *md = a.maxdep + (size_t)g * a.V;
+    double S = 0.0;
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
-        const uint32_t w = e < e1 ? a.out_col[e] : 0;
-        const int n = (int)min(64u, e1 - base);
-        int j = 0;
-        for (; j + 4 <= n; j += 4) {
-            const uint32_t w0 = zg_readlane32(w, j), w1 = zg_readlane32(w, j + 1);
-            const uint32_t w2 = zg_readlane32(w, j + 2), w3 = zg_readlane32(w, j + 3);
-            if (active) {
-                S0 += cg[(size_t)w0 * 64];
-                S1 += cg[(size_t)w1 * 64];
-                S2 += cg[(size_t)w2 * 64];
-                S3 += cg[(size_t)w3 * 64];
-            }
+        uint32_t w = 0;
+        bool keep = false;
+        if (e < e1) {
+            w = a.out_col[e];
+            keep = md[w] >= (uint32_t)(d + 1);
         }
-        for (; j < n; j++) {
-            const uint32_t wj = zg_readlane32(w, j);
-            if (active) S0 += cg[(size_t)wj * 64];
+        uint64_t act = __ballot(keep);
+        // eight row reads in flight per round
+        while (act) {
+            uint32_t wq[8];
+            bool okq[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                okq[q] = act != 0;
+                if (act) {
+                    const int j = __builtin_ctzll(act);
+                    act &= act - 1;
+                    wq[q] = zg_readlane32(w, j);
+                } else {
+                    wq[q] = 0;
+                }
+            }
+            double val[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) val[q] = (active && okq[q]) ? cg[(size_t)wq[q] * 64] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) S += val[q];
         }
     }
-    const double S = (S0 + S1) + (S2 + S3);
     const uint32_t hub = a.ohub_idx[v];
     if (hub != ZG_NONE) {
         if (active) atomicAdd(&a.hubacc[((size_t)g * a.n_ohub + hub) * 64 + lane], S);
@@ -443,7 +474,7 @@ __global__ void bc_export_kernel(uint32_t V, uint32_t g, const uint32_t *dep, co
 int zg_bc_ws_release(zg_graph *g)
 {
     zg_bc_ws &w = g->bc;
-    zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep);
+    zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep); zg_dfree(w.maxdep);
     zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.flist); zg_dfree(w.fmask);
     zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.src); zg_dfree(w.cnt);
     if (w.h_cnt) (void)hipHostFree(w.h_cnt);
@@ -469,6 +500,7 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth)
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hubacc, G * (size_t)g->n_ohub * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.bcg, G * V);
     if (rc == ZG_OK && with_depth) rc = zg_dmalloc(&w.dep, G * V * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.maxdep, G * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * 2);
@@ -530,7 +562,7 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used)
     a.och_first = g->och_first; a.och_v = g->och_v; a.och_e0 = g->och_e0; a.och_e1 = g->och_e1;
     a.ohub_idx = g->ohub_idx;
     a.live = g->live;
-    a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep;
+    a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep; a.maxdep = w.maxdep;
     a.vis = w.vis; a.fr = w.fr; a.stats = w.stats;
     a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
     a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
@@ -652,6 +684,76 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStrea
     return ZG_OK;
 }
 
+// One level of a reverse BFS from the hub core: a vertex joins level d+1 if
+// one of its successors sits at level d.
+__global__ void bc_coredist_kernel(uint32_t V, const uint32_t *out_ptr, const uint32_t *out_col, uint32_t *dist,
+                                   uint32_t d, uint32_t *changed)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V || dist[v] != ZG_NONE) return;
+    for (uint32_t e = out_ptr[v], e1 = out_ptr[v + 1]; e < e1; e++) {
+        if (dist[out_col[e]] == d) {
+            dist[v] = d + 1;
+            *changed = 1;
+            return;
+        }
+    }
+}
+
+#define ZG_CORE_SIZE 16
+#define ZG_CORE_MAX_LEVELS 64
+
+// Distance of every vertex to the ZG_CORE_SIZE highest in-degree vertices
+// (capped at ZG_CORE_MAX_LEVELS levels; unreachable = ZG_NONE), cached.
+static int bc_core_distances(zg_graph *g, hipStream_t s)
+{
+    if (!g->h_coredist.empty() || g->V == 0) return ZG_OK;
+    const uint32_t V = g->V;
+    std::vector<uint32_t> order(V);
+    for (uint32_t v = 0; v < V; v++) order[v] = v;
+    const uint32_t k = std::min<uint32_t>(ZG_CORE_SIZE, V);
+    std::partial_sort(order.begin(), order.begin() + k, order.end(), [&](uint32_t x, uint32_t y) {
+        if (g->h_in_deg[x] != g->h_in_deg[y]) return g->h_in_deg[x] > g->h_in_deg[y];
+        return x < y;
+    });
+    std::vector<uint32_t> dist(V, ZG_NONE);
+    for (uint32_t i = 0; i < k; i++)
+        if (g->h_live[order[i]]) dist[order[i]] = 0;
+    uint32_t *d_dist = nullptr, *d_changed = nullptr;
+    int rc = zg_dmalloc(&d_dist, V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&d_changed, 1);
+    if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(d_dist, dist.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice, s));
+    for (uint32_t d = 0; rc == ZG_OK && d < ZG_CORE_MAX_LEVELS; d++) {
+        uint32_t changed = 0;
+        rc = zg_hip_code(hipMemsetAsync(d_changed, 0, sizeof(uint32_t), s));
+        if (rc != ZG_OK) break;
+        hipLaunchKernelGGL(bc_coredist_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V, g->out_ptr, g->out_col, d_dist,
+                           d, d_changed);
+        rc = zg_hip_code(hipGetLastError());
+        if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(&changed, d_changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+        if (!changed) break;
+    }
+    if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(dist.data(), d_dist, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, s));
+    if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+    zg_dfree(d_dist);
+    zg_dfree(d_changed);
+    if (rc == ZG_OK) g->h_coredist.swap(dist);
+    return rc;
+}
+
+// Sorts sources so that each 64-source group holds sources at the same
+// distance from the hub core (then larger out-degree first, then id).
+static void bc_order_sources(const zg_graph *g, std::vector<uint32_t> &src)
+{
+    const std::vector<uint32_t> &cd = g->h_coredist;
+    std::stable_sort(src.begin(), src.end(), [&](uint32_t x, uint32_t y) {
+        if (cd[x] != cd[y]) return cd[x] < cd[y];
+        if (g->h_out_deg[x] != g->h_out_deg[y]) return g->h_out_deg[x] > g->h_out_deg[y];
+        return x < y;
+    });
+}
+
 static int bc_all_live_sources(zg_graph *g, std::vector<uint32_t> &out)
 {
     out.clear();
@@ -668,13 +770,25 @@ static int zg_bc_device_impl(zg_graph *g, const uint32_t *d_sources, uint64_t n_
     memset(&inf, 0, sizeof(inf));
     const uint32_t V = g->V;
     uint32_t *d_tmp = nullptr;
-    if (!d_sources) {
-        std::vector<uint32_t> all;
-        bc_all_live_sources(g, all);
-        n_sources = all.size();
-        ZG_TRY_RC(zg_dmalloc(&d_tmp, all.size()));
-        if (!all.empty())
-            ZG_TRY(hipMemcpyAsync(d_tmp, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice, s));
+    if (!d_sources || (g->source_order && n_sources > 64)) {
+        std::vector<uint32_t> list;
+        if (!d_sources) {
+            bc_all_live_sources(g, list);
+        } else {
+            list.resize(n_sources);
+            ZG_TRY(hipMemcpyAsync(list.data(), d_sources, sizeof(uint32_t) * n_sources, hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+            for (uint32_t x : list)
+                if (x >= V || !g->h_live[x]) return ZG_ERR_NODE_NOT_FOUND;
+        }
+        if (g->source_order && list.size() > 64) {
+            ZG_TRY_RC(bc_core_distances(g, s));
+            bc_order_sources(g, list);
+        }
+        n_sources = list.size();
+        ZG_TRY_RC(zg_dmalloc(&d_tmp, list.size()));
+        if (!list.empty())
+            ZG_TRY(hipMemcpyAsync(d_tmp, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, s));
         ZG_TRY(hipStreamSynchronize(s));
         d_sources = d_tmp;
     }

@@ -91,6 +91,7 @@ struct zg_bc_ws {
     bool with_depth = false;
     double *sig = nullptr, *cc = nullptr, *hubacc = nullptr, *bcg = nullptr;
     uint32_t *dep = nullptr;
+    uint32_t *maxdep = nullptr;          // [G][V] deepest level of v over the group's lanes
     uint64_t *vis = nullptr, *fr = nullptr;
     unsigned long long *stats = nullptr; // [G][ZG_FIN_BLOCKS][2] arcs, reached
     uint32_t *flist = nullptr;           // [G][fcap]
@@ -134,7 +135,9 @@ struct zg_graph {
     uint32_t *in_ptr = nullptr, *in_col = nullptr;
     uint8_t *live = nullptr;
     std::vector<uint8_t> h_live;
-    std::vector<uint32_t> h_out_deg;
+    std::vector<uint32_t> h_out_deg, h_in_deg;
+    std::vector<uint32_t> h_coredist;    // distance to the hub core (lazy)
+    int source_order = 1;
     uint32_t n_dangling = 0;
     // predecessor chunks (forward sweep work items)
     uint32_t n_ich = 0;
@@ -148,7 +151,8 @@ struct zg_graph {
     uint32_t *ohub_idx = nullptr;        // [V] dense index of multi-chunk vertices or ZG_NONE
     // PageRank row bins
     uint32_t n_small = 0, n_med = 0, n_large = 0, n_seg = 0;
-    uint32_t *pr_small = nullptr, *pr_med = nullptr;
+    uint32_t n_m4 = 0, n_m16 = 0, n_m64 = 0;   // n_med = n_m4 + n_m16 + n_m64
+    uint32_t *pr_small = nullptr, *pr_med = nullptr;   // pr_med = [m4 | m16 | m64]
     uint32_t *pr_large = nullptr;        // [n_large] rows
     uint32_t *pr_large_seg0 = nullptr;   // [n_large + 1] first segment of each large row
     uint32_t *pr_seg_row = nullptr;      // [n_seg] large-row index of each segment

@@ -160,8 +160,10 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
             for (uint32_t k = out_ptr[v]; k < out_ptr[(size_t)v + 1]; k++) in_col[fill[out_col[k]]++] = v;
     }
     g->h_out_deg.resize(V);
+    g->h_in_deg.resize(V);
     uint32_t n_dang = 0;
     for (uint32_t v = 0; v < V; v++) {
+        g->h_in_deg[v] = in_ptr[(size_t)v + 1] - in_ptr[v];
         g->h_out_deg[v] = out_ptr[(size_t)v + 1] - out_ptr[v];
         if (g->h_live[v] && g->h_out_deg[v] == 0) n_dang++;
     }
@@ -210,12 +212,14 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     g->n_ohub = n_ohub;
 
     // PageRank row bins over live rows by predecessor count.
-    std::vector<uint32_t> small, med, large, large_seg0, seg_row, seg_e0, seg_e1;
+    std::vector<uint32_t> small, m4, m16, m64, large, large_seg0, seg_row, seg_e0, seg_e1;
     for (uint32_t v = 0; v < V; v++) {
         if (!g->h_live[v]) continue;
         uint32_t d = in_ptr[(size_t)v + 1] - in_ptr[v];
         if (d <= ZG_PR_SMALL) small.push_back(v);
-        else if (d <= ZG_PR_MED) med.push_back(v);
+        else if (d <= ZG_PR_M4) m4.push_back(v);
+        else if (d <= ZG_PR_M16) m16.push_back(v);
+        else if (d <= ZG_PR_MED) m64.push_back(v);
         else {
             large_seg0.push_back((uint32_t)seg_row.size());
             for (uint32_t c = in_ptr[v]; c < in_ptr[(size_t)v + 1]; c += ZG_PR_SEG) {
@@ -228,6 +232,12 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     }
     large_seg0.push_back((uint32_t)seg_row.size());
     g->n_small = (uint32_t)small.size();
+    g->n_m4 = (uint32_t)m4.size();
+    g->n_m16 = (uint32_t)m16.size();
+    g->n_m64 = (uint32_t)m64.size();
+    std::vector<uint32_t> med(m4);
+    med.insert(med.end(), m16.begin(), m16.end());
+    med.insert(med.end(), m64.begin(), m64.end());
     g->n_med = (uint32_t)med.size();
     g->n_large = (uint32_t)large.size();
     g->n_seg = (uint32_t)seg_row.size();
@@ -310,6 +320,16 @@ extern "C" int zg_graph_info(const zg_graph *g, uint32_t *n_cap, uint32_t *n_liv
     if (n_live) *n_live = g->n_live;
     if (n_arcs) *n_arcs = g->E;
     return ZG_OK;
+}
+
+extern "C" int zg_set_source_order(zg_graph *g, int mode)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (mode != 0 && mode != 1) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    int prev = g->source_order;
+    g->source_order = mode;
+    return prev;
 }
 
 extern "C" int zg_set_batch_groups(zg_graph *g, int groups)

@@ -36,8 +36,8 @@ template <typename T>
 struct PrArgs {
     const uint32_t *in_ptr, *in_col, *out_ptr;
     const uint32_t *small, *med, *large, *large_seg0, *seg_e0, *seg_e1;
-    uint32_t n_small, n_med, n_large, n_seg;
-    uint32_t nb_small, nb_med;  // blocks of the small and medium bins
+    uint32_t n_small, n_m4, n_m16, n_m64, n_large, n_seg;
+    uint32_t nb_small, nb_m4, nb_m16, nb_m64;  // work units of each bin
     T *r0, *r1, *x0, *x1;       // parity 0 / 1 buffers
     double *part;               // [gridDim.x][2]
     T *seg;                     // [n_seg][2] Kahan pairs of long-row segments
@@ -75,6 +75,54 @@ __device__ __forceinline__ void wave_kahan_merge(T &s, T &c)
     }
 }
 
+// Kahan sum of x[col[e]] for e = e0, e0+step, ... < e1, four gathers in flight.
+template <typename T>
+__device__ __forceinline__ void pr_strided_kahan(const uint32_t *__restrict__ col, const T *__restrict__ x,
+                                                 uint32_t e0, uint32_t e1, uint32_t step, T &s, T &c)
+{
+    uint32_t e = e0;
+    for (; e + 3 * step < e1; e += 4 * step) {
+        const uint32_t c0 = col[e], c1 = col[e + step], c2 = col[e + 2 * step], c3 = col[e + 3 * step];
+        const T v0 = x[c0], v1 = x[c1], v2 = x[c2], v3 = x[c3];
+        kahan_add(s, c, v0);
+        kahan_add(s, c, v1);
+        kahan_add(s, c, v2);
+        kahan_add(s, c, v3);
+    }
+    for (; e < e1; e += step) kahan_add(s, c, x[col[e]]);
+}
+
+template <typename T>
+__device__ __forceinline__ void pr_row_epilogue(const PrArgs<T> &a, const T *rold, T *rnew, T *xnew,
+                                                uint32_t v, T s, T c, double base, double damping,
+                                                double &l1, double &dang);
+
+// Rows of one bin summed by groups of L lanes: unit u covers rows
+// [u*(256/L), (u+1)*(256/L)) of the bin; lane groups merge their Kahan pairs.
+template <typename T, int L>
+__device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint32_t *rows, uint32_t nrows,
+                                              uint32_t unit, const T *x, const T *rold, T *rnew, T *xnew,
+                                              double base, double damping, double &l1, double &dang)
+{
+    const int tid = threadIdx.x;
+    const uint32_t i = unit * (ZG_PR_BLOCK / L) + tid / L;
+    const uint32_t sub = tid % L;
+    const bool ok = i < nrows;
+    uint32_t v = 0;
+    T s = 0, c = 0;
+    if (ok) {
+        v = rows[i];
+        pr_strided_kahan(a.in_col, x, a.in_ptr[v] + sub, a.in_ptr[v + 1], (uint32_t)L, s, c);
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+        T s2 = __shfl_xor(s, off, 64);
+        T c2 = __shfl_xor(c, off, 64);
+        kahan_merge(s, c, s2, c2);
+    }
+    if (ok && sub == 0) pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
+}
+
 // New rank of row v from its compensated predecessor sum (value s - c);
 // accumulates the FP64 l1 change and dangling mass of this thread.
 template <typename T>
@@ -106,7 +154,7 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
     T *xnew = odd ? a.x0 : a.x1;
     const double base = st->base, damping = st->damping;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint32_t units = a.nb_small + a.nb_med + a.n_seg;
+    const uint32_t units = a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64 + a.n_seg;
     double l1 = 0.0, dang = 0.0;
 
     for (uint32_t b = blockIdx.x; b < units; b += gridDim.x) {
@@ -115,27 +163,33 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
             const uint32_t i = b * ZG_PR_BLOCK + tid;
             if (i < a.n_small) {
                 const uint32_t v = a.small[i];
+                const uint32_t e0 = a.in_ptr[v], deg = a.in_ptr[v + 1] - e0;
+                // all predecessor ids, then all gathers, in flight together
+                uint32_t col[ZG_PR_SMALL];
+                T val[ZG_PR_SMALL];
+#pragma unroll
+                for (int k = 0; k < ZG_PR_SMALL; k++) col[k] = (uint32_t)k < deg ? a.in_col[e0 + k] : 0u;
+#pragma unroll
+                for (int k = 0; k < ZG_PR_SMALL; k++) val[k] = (uint32_t)k < deg ? x[col[k]] : (T)0;
                 T s = 0, c = 0;
-                for (uint32_t e = a.in_ptr[v], e1 = a.in_ptr[v + 1]; e < e1; e++) kahan_add(s, c, x[a.in_col[e]]);
+#pragma unroll
+                for (int k = 0; k < ZG_PR_SMALL; k++)
+                    if ((uint32_t)k < deg) kahan_add(s, c, val[k]);
                 pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
             }
-        } else if (b < a.nb_small + a.nb_med) {
-            // one wavefront per row
-            const uint32_t i = (b - a.nb_small) * (ZG_PR_BLOCK / 64) + wv;
-            if (i < a.n_med) {
-                const uint32_t v = a.med[i];
-                T s = 0, c = 0;
-                for (uint32_t e = a.in_ptr[v] + lane, e1 = a.in_ptr[v + 1]; e < e1; e += 64)
-                    kahan_add(s, c, x[a.in_col[e]]);
-                wave_kahan_merge(s, c);
-                if (lane == 0) pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
-            }
+        } else if (b < a.nb_small + a.nb_m4) {
+            pr_rows_lanes<T, 4>(a, a.med, a.n_m4, b - a.nb_small, x, rold, rnew, xnew, base, damping, l1, dang);
+        } else if (b < a.nb_small + a.nb_m4 + a.nb_m16) {
+            pr_rows_lanes<T, 16>(a, a.med + a.n_m4, a.n_m16, b - a.nb_small - a.nb_m4, x, rold, rnew, xnew, base,
+                                 damping, l1, dang);
+        } else if (b < a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64) {
+            pr_rows_lanes<T, 64>(a, a.med + a.n_m4 + a.n_m16, a.n_m64, b - a.nb_small - a.nb_m4 - a.nb_m16, x, rold,
+                                 rnew, xnew, base, damping, l1, dang);
         } else {
             // one block per segment of a long row; pr_finish_kernel finishes the row
-            const uint32_t sg = b - a.nb_small - a.nb_med;
+            const uint32_t sg = b - a.nb_small - a.nb_m4 - a.nb_m16 - a.nb_m64;
             T s = 0, c = 0;
-            for (uint32_t e = a.seg_e0[sg] + tid, e1 = a.seg_e1[sg]; e < e1; e += ZG_PR_BLOCK)
-                kahan_add(s, c, x[a.in_col[e]]);
+            pr_strided_kahan(a.in_col, x, a.seg_e0[sg] + tid, a.seg_e1[sg], (uint32_t)ZG_PR_BLOCK, s, c);
             wave_kahan_merge(s, c);
             if (lane == 0) {
                 kred[0][wv] = s;
@@ -269,9 +323,8 @@ int zg_pr_ws_release(zg_graph *g)
 #define ZG_PR_MAX_GRID 2048
 static uint32_t pr_grid(const zg_graph *g)
 {
-    uint32_t nb_small = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK;
-    uint32_t nb_med = (g->n_med + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
-    uint32_t units = nb_small + nb_med + g->n_seg;
+    uint32_t units = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK + (g->n_m4 + 63) / 64 + (g->n_m16 + 15) / 16 +
+                     (g->n_m64 + 3) / 4 + g->n_seg;
     return units < ZG_PR_MAX_GRID ? (units ? units : 1) : ZG_PR_MAX_GRID;
 }
 
@@ -307,11 +360,15 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.seg_e0 = g->pr_seg_e0;
     a.seg_e1 = g->pr_seg_e1;
     a.n_small = g->n_small;
-    a.n_med = g->n_med;
+    a.n_m4 = g->n_m4;
+    a.n_m16 = g->n_m16;
+    a.n_m64 = g->n_m64;
     a.n_large = g->n_large;
     a.n_seg = g->n_seg;
     a.nb_small = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK;
-    a.nb_med = (g->n_med + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
+    a.nb_m4 = (g->n_m4 + ZG_PR_BLOCK / 4 - 1) / (ZG_PR_BLOCK / 4);
+    a.nb_m16 = (g->n_m16 + ZG_PR_BLOCK / 16 - 1) / (ZG_PR_BLOCK / 16);
+    a.nb_m64 = (g->n_m64 + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
     a.r0 = r0;
     a.r1 = r1;
     a.x0 = x0;

@@ -1,6 +1,10 @@
-// PageRank row binning by predecessor count (rows = live vertices).
+// PageRank row binning by predecessor count (rows = live vertices).  A row
+// is summed by a group of L lanes (one wavefront holds 64/L rows) so short
+// rows do not leave most of a wavefront idle.
 #pragma once
 #define ZG_PR_SMALL 16     // <= 16 predecessors: one lane per row
-#define ZG_PR_MED 2048     // <= 2048: one wavefront per row
+#define ZG_PR_M4 64        // <= 64: 4 lanes per row
+#define ZG_PR_M16 256      // <= 256: 16 lanes per row
+#define ZG_PR_MED 2048     // <= 2048: 64 lanes per row
 #define ZG_PR_SEG 4096     // longer rows: 256-lane blocks over 4096-arc segments
 #define ZG_PR_BLOCK 256

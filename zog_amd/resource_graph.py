@@ -295,6 +295,15 @@ def set_batch_groups(graph: ResourceGraph, groups: int) -> int:
     return rc
 
 
+def set_source_order(graph: ResourceGraph, mode: int) -> int:
+    """0 = sweep sources in the given order, 1 = group them by distance to the
+    hub core (default).  Returns the previous mode."""
+    rc = _lib.lib().zg_set_source_order(graph.resource.handle, int(mode))
+    if rc < 0:
+        _lib.check(rc, "zg_set_source_order")
+    return rc
+
+
 def set_profiling(graph: ResourceGraph, on: bool = True) -> None:
     _lib.check(_lib.lib().zg_set_profiling(graph.resource.handle, 1 if on else 0), "zg_set_profiling")
 
