@@ -1,21 +1,27 @@
 """Benchmark: exact Brandes (GTEPS) and PageRank (GB/s) on one MI355X.
 
-Workload (one "step"): R-MAT scale 20 (1,048,576 nodes, 2^24 distinct arcs,
-a=.57 b=.19 c=.19 d=.05, seeded), exact unweighted Brandes from 4,096 seeded
-sources swept 256 per launch, graph and sources resident in HBM, output left
-in HBM.  PageRank (FP32 pull SpMV with Kahan row sums, tol 1e-6 L1, at most
-100 iterations) is timed on the same resident graph and reported beside it.
+Default workload (one "step"), rmat20: R-MAT scale 20 (1,048,576 nodes, 2^24
+distinct arcs, a=.57 b=.19 c=.19 d=.05, seeded), exact unweighted Brandes
+from 4,096 seeded sources swept 256 per launch, graph and sources resident in
+HBM, output left in HBM.  PageRank (FP32 pull SpMV with Kahan row sums, tol
+1e-6 L1, at most 100 iterations) is timed on the same resident graph and
+reported beside it.  Other workloads (--workload): er10k (directed
+Erdos-Renyi n=10,000, avg out-degree 8, all 10,000 sources; PageRank tol
+1e-6), grid2048 (2048x2048 grid with 5% of edges removed, 1,024 seeded
+sources; PageRank to 1e-8 L1 in the mixed FP32/FP64 mode, at most 1,000
+iterations).
 
-Prints one JSON line (rank 0).  Units:
+Prints one JSON line.  Units:
   Brandes GTEPS   = sum over sources of arcs scanned by the forward sweep
                     (m_s) / seconds / 1e9
   algorithmic B   = 40 m_s + 64 n_s per source (u32 ids and depths, FP64
-                    counts, dependencies and scores); the forward kernel is
-                    charged 16 m_s + 24 n_s and the backward kernel
-                    24 m_s + 40 n_s
+                    counts, dependencies and scores); the forward sweep
+                    (pull and push kernels) is charged 16 m_s + 24 n_s and the
+                    backward kernel 24 m_s + 40 n_s
   PageRank GB/s   = iterations * (8 E + 20 n) bytes / seconds
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -31,35 +37,53 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def workload_spec(name):
+    from zog_amd import workloads as W
+    if name == "rmat20":
+        return dict(gen=W.rmat20, sources=lambda n, s, d: W.rmat20_sources(n, s, d, 4096),
+                    pr=dict(tol=1e-6, max_iter=100, precision=32), cpu_sources=16,
+                    desc="rmat20: R-MAT scale 20, 2^24 distinct arcs, exact Brandes from 4096 seeded sources, "
+                         "256 per launch; PageRank FP32-Kahan tol 1e-6 max 100")
+    if name == "er10k":
+        return dict(gen=W.er10k, sources=None, pr=dict(tol=1e-6, max_iter=100, precision=32), cpu_sources=None,
+                    desc="er10k: directed Erdos-Renyi n=10,000 p=8/(n-1), exact Brandes over all sources; "
+                         "PageRank FP32-Kahan tol 1e-6 max 100")
+    if name == "grid2048":
+        return dict(gen=W.grid2048, sources=lambda n, s, d: W.grid2048_sources(n, s, d, 1024),
+                    pr=dict(tol=1e-8, max_iter=1000, precision=96), cpu_sources=2, scaled=True,
+                    desc="grid2048: 2048x2048 grid, 419,226 of 8,384,512 edges removed, exact Brandes from 1024 "
+                         "seeded sources; PageRank mixed FP32/FP64 to 1e-8 L1, max 1000")
+    raise SystemExit(f"unknown workload {name}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048"])
     ap.add_argument("--groups", type=int, default=4, help="64-source groups per launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by source; see the sharding notes)", file=sys.stderr)
+              "(the path shards by source; see DESIGN notes on sharding)", file=sys.stderr)
         sys.exit(2)
-
-    import ctypes
 
     import numpy as np
     import torch
 
     from zog_amd import _lib
     from zog_amd import resource_graph as rg
-    from zog_amd import workloads
     from zog_amd.soa import SoA
 
+    spec = workload_spec(args.workload)
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     t0 = time.time()
-    n, src, dst = workloads.rmat20()
-    sources = workloads.rmat20_sources(n, src, dst, 4096)
-    log(f"[bench] rmat20 generated in {time.time() - t0:.1f}s: n={n} arcs={len(src)} sources={len(sources)}")
+    n, src, dst = spec["gen"]()
+    sources = spec["sources"](n, src, dst) if spec["sources"] else np.arange(n, dtype=np.uint32)
+    log(f"[bench] {args.workload} generated in {time.time() - t0:.1f}s: n={n} arcs={len(src)} sources={len(sources)}")
     t0 = time.time()
     g = rg.new(SoA.from_arrays("directed", n, src, dst))
     rg.set_batch_groups(g, args.groups)
@@ -82,43 +106,43 @@ def main():
         bc_step()
         log(f"[bench] warmup {i} done ({info.kernel_ms:.1f} ms)")
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
     w0 = time.perf_counter()
-    ev0.record(stream)
     arcs = 0
     reached = 0
     for i in range(args.steps):
         bc_step()
         arcs += info.arcs_traversed
         reached += info.nodes_reached
-    ev1.record(stream)
+        log(f"[bench] step {i} done")
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    ev_ms = ev0.elapsed_time(ev1)
-    log(f"[bench] {args.steps} steps in {wall:.3f}s (events {ev_ms:.1f} ms)")
+    log(f"[bench] {args.steps} steps in {wall:.3f}s")
     ms_per_step = wall * 1000.0 / args.steps
     gteps = arcs / wall / 1e9
     arcs_step = arcs // args.steps
     nodes_step = reached // args.steps
     levels = info.max_levels
+    push_levels, pull_levels = int(info.push_levels), int(info.pull_levels)
 
-    # per-kernel device time of one more step (events around every launch)
+    # per-kernel device time of one more step (HIP events around every launch
+    # on the launch stream)
     rg.set_profiling(g, True)
     bc_step()
     prof = rg.get_profile(g)
     rg.set_profiling(g, False)
-    fwd_ms, fwd_n = prof["bc_forward"]
+    log(f"[bench] profile {prof}")
+    fwd_ms = prof["bc_forward"][0] + prof["bc_push"][0]
+    fwd_n = prof["bc_forward"][1] + prof["bc_push"][1]
     bwd_ms, bwd_n = prof["bc_backward"]
     fwd_bytes = 16 * arcs_step + 24 * nodes_step
     bwd_bytes = 24 * arcs_step + 40 * nodes_step
     if fwd_ms >= bwd_ms:
-        dom, dom_bytes, dom_ms, dom_n = "bc_forward_kernel", fwd_bytes, fwd_ms, fwd_n
+        dom, dom_bytes, dom_ms, dom_n = "bc_forward_kernel+bc_push_kernel", fwd_bytes, fwd_ms, fwd_n
     else:
         dom, dom_bytes, dom_ms, dom_n = "bc_backward_kernel", bwd_bytes, bwd_ms, bwd_n
     achieved = dom_bytes / (dom_ms / 1000.0) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get(dom)
@@ -127,72 +151,72 @@ def main():
                 "kernel": dom, "launches": dom_n, "avg_launch_us": round(dom_ms * 1000.0 / max(dom_n, 1), 2),
                 "algorithmic_bytes_per_step": dom_bytes,
                 "kernel_ms_per_step": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}
-    log(f"[bench] profile {prof}")
 
     # PageRank on the same resident graph
+    prs = spec["pr"]
     d_pr = torch.empty(V, dtype=torch.float64, device=dev)
     pinfo = _lib.ZgPrInfo()
 
-    def pr_run(prec=_lib.ZG_PR_FP32 | _lib.ZG_PR_SKIP_RESIDUAL):
-        rc = L.zg_pagerank_device(h, 0.85, 100, 1e-6, prec, ctypes.c_void_p(d_pr.data_ptr()),
+    def pr_run(prec):
+        rc = L.zg_pagerank_device(h, 0.85, prs["max_iter"], prs["tol"], prec, ctypes.c_void_p(d_pr.data_ptr()),
                                   ctypes.c_void_p(stream.cuda_stream), ctypes.byref(pinfo))
         _lib.check(rc, "zg_pagerank_device")
 
-    pr_run()
+    fast = prs["precision"] | _lib.ZG_PR_SKIP_RESIDUAL
+    pr_run(fast)
     torch.cuda.synchronize()
     pr_reps = 5
     w0 = time.perf_counter()
     iters = 0
     for _ in range(pr_reps):
-        pr_run()
+        pr_run(fast)
         iters += pinfo.iterations
     torch.cuda.synchronize()
     pr_wall = time.perf_counter() - w0
     pr_bytes_iter = 8 * E + 20 * n_live
     pr_gbs = iters * pr_bytes_iter / pr_wall / 1e9
     rg.set_profiling(g, True)
-    pr_run(_lib.ZG_PR_FP32)
+    pr_run(prs["precision"])
     pprof = rg.get_profile(g)
     rg.set_profiling(g, False)
     pr_iters = int(pinfo.iterations)
     it_ms, it_n = pprof["pr_iter"]
     pr_ach = pr_bytes_iter * it_n / (it_ms / 1000.0) / 1e9 if it_ms > 0 else 0.0
     pr_result = d_pr.cpu().numpy()
-    log(f"[bench] pagerank {pr_iters} iterations, {pr_gbs:.1f} GB/s wall, kernel {pr_ach:.1f} GB/s, "
-        f"residual {pinfo.residual_l1:.3e}")
+    log(f"[bench] pagerank {pr_iters} iterations ({int(pinfo.fp64_iterations)} FP64), {pr_gbs:.1f} GB/s wall, "
+        f"kernel {pr_ach:.1f} GB/s, residual {pinfo.residual_l1:.3e}")
 
     cpu = None
-    parity = {}
+    parity = {"pagerank_fp64_residual_l1": float(pinfo.residual_l1),
+              "pagerank_error_bound_l1": float(pinfo.error_bound_l1)}
     if not args.no_cpu:
         from oracle import oracle as O
         threads = 1
-        t0 = time.time()
         og = O.Graph(n, src, dst)
-        log(f"[bench] oracle graph in {time.time() - t0:.1f}s")
-        sample = sources[:16]
+        sample = sources if spec["cpu_sources"] is None else sources[: spec["cpu_sources"]]
+        mode = O.SIGMA_SCALED if spec.get("scaled") else O.SIGMA_U64
         t0 = time.perf_counter()
-        ref_bc, _, arcs_cpu = og.betweenness(sources=sample, threads=threads, return_arcs=True)
+        ref_bc, _, arcs_cpu = og.betweenness(sources=sample, threads=threads, return_arcs=True, mode=mode)
         cpu_s = time.perf_counter() - t0
         cpu_gteps = float(arcs_cpu.sum()) / cpu_s / 1e9
-        log(f"[bench] oracle BC 16 sources in {cpu_s:.1f}s = {cpu_gteps:.4f} GTEPS")
+        log(f"[bench] oracle BC {len(sample)} sources in {cpu_s:.1f}s = {cpu_gteps:.4f} GTEPS")
         gpu_bc = rg.betweenness_unweighted_array(g, sources=sample)
         scale = max(1.0, float(np.abs(ref_bc).max()))
         rel = np.abs(gpu_bc - ref_bc) / np.maximum(np.abs(ref_bc), 1e-9 * scale)
-        parity["bc_max_rel_err_16_sources"] = float(rel.max())
+        parity[f"bc_max_rel_err_{len(sample)}_sources"] = float(rel.max())
         t0 = time.perf_counter()
-        ref_pr, ref_it, _ = og.pagerank(0.85, 100, 1e-6)
+        ref_pr, ref_it, _ = og.pagerank(0.85, prs["max_iter"], prs["tol"])
         cpu_pr_s = time.perf_counter() - t0
         parity["pagerank_l1_vs_fp64_oracle"] = float(np.abs(pr_result - ref_pr).sum())
         parity["pagerank_iterations_gpu_oracle"] = [pr_iters, ref_it]
-        parity["pagerank_fp64_residual_l1"] = float(pinfo.residual_l1)
         cpu = {"value": round(cpu_gteps, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
-               "sample": "oracle/oracle.c Brandes (restated centrality.zig:694-798), 1 thread, first 16 of "
-                         "the 4,096 sources on rmat20, host cores of the GPU box",
+               "sample": f"oracle/oracle.c Brandes (restated centrality.zig:694-798), 1 thread, {len(sample)} of "
+                         f"the {len(sources)} sources of {args.workload}, host cores of the GPU box",
                "pagerank_gbs": round(ref_it * pr_bytes_iter / cpu_pr_s / 1e9, 3),
-               "pagerank_sample": f"full FP64 oracle PageRank on rmat20 ({ref_it} iterations, 1 thread)"}
+               "pagerank_sample": f"full FP64 oracle PageRank on {args.workload} ({ref_it} iterations, 1 thread)"}
 
     line = {
-        "metric": "Brandes GTEPS (exact BC, rmat20, 4096 sources) + PageRank GB/s",
+        "metric": "Brandes GTEPS (exact BC) + PageRank GB/s",
         "value": round(gteps, 3),
         "unit": "GTEPS",
         "n_gpus": 1,
@@ -204,17 +228,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "rmat20: R-MAT scale 20, 2^24 distinct arcs, exact Brandes from 4096 seeded "
-                               "sources, 256 per launch; PageRank FP32-Kahan tol 1e-6 max 100",
-                   "nodes": n, "arcs": int(E), "sources": len(sources), "sources_per_launch": 64 * args.groups,
-                   "levels": levels, "arcs_per_step": arcs_step},
+        "config": {"workload": spec["desc"], "nodes": n, "arcs": int(E), "sources": len(sources),
+                   "sources_per_launch": 64 * args.groups, "levels": levels, "arcs_per_step": arcs_step,
+                   "push_levels": push_levels, "pull_levels": pull_levels},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pagerank": {"metric": "PageRank GB/s", "value": round(pr_gbs, 2), "unit": "GB/s",
-                     "iterations": pr_iters, "dtype": "f32 (Kahan rows, f64 reductions)",
+                     "iterations": pr_iters, "fp64_iterations": int(pinfo.fp64_iterations),
+                     "dtype": {32: "f32 (Kahan rows, f64 reductions)", 64: "f64",
+                               96: "f32 Kahan then f64"}[prs["precision"]],
                      "roofline": {"bound": "hbm", "achieved": round(pr_ach, 1), "peak": HBM_PEAK_GBS,
-                                  "unit": "GB/s", "frac": round(pr_ach / HBM_PEAK_GBS, 4),
-                                  "traffic": None, "kernel": "pr_iter_kernel<float>",
+                                  "unit": "GB/s", "frac": round(pr_ach / HBM_PEAK_GBS, 4), "traffic": None,
+                                  "kernel": "pr_iter_kernel+pr_finish_kernel",
                                   "avg_launch_us": round(it_ms * 1000.0 / max(it_n, 1), 2)}},
         "parity": parity,
     }

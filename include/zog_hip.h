@@ -41,7 +41,9 @@ typedef struct {
                                  sweep (arcs whose tail is reachable) */
     uint64_t nodes_reached;   /* sum over sources of reached vertices */
     uint32_t max_levels;      /* deepest BFS over all sources, in levels */
-    uint32_t batches;         /* launches of the batched multi-source sweep */
+    uint32_t batches;         /* batches of the multi-source sweep */
+    uint32_t push_levels;     /* forward levels run from the frontier (push) */
+    uint32_t pull_levels;     /* forward levels run over predecessor chunks (pull) */
     double kernel_ms;         /* device time of the sweep (HIP events) */
 } zg_bc_info;
 
@@ -55,13 +57,19 @@ typedef struct {
                                  L1 distance of the returned vector to the
                                  exact fixed point */
     double kernel_ms;         /* device time of the iterations (HIP events) */
-    int32_t precision;        /* 32 (FP32 Kahan) or 64 (FP64 iterations) */
+    int32_t precision;        /* ZG_PR_FP32, ZG_PR_FP64 or ZG_PR_MIXED */
     int32_t reserved;
+    uint64_t fp64_iterations; /* iterations run in FP64 by the mixed mode */
 } zg_pr_info;
 
 /* PageRank precision modes. */
 #define ZG_PR_FP32 32     /* FP32 values, Kahan row sums, FP64 reductions */
 #define ZG_PR_FP64 64     /* FP64 throughout (same data layout) */
+#define ZG_PR_MIXED 96    /* FP32 Kahan down to L1 change 1e-6, then FP64 from
+                             that iterate to the tolerance (for tolerances below
+                             the FP32 rounding floor) */
+#define ZG_PR_SKIP_RESIDUAL 0x100  /* or-ed into precision: skip the FP64
+                                      residual evaluation */
 
 /* Replaces `new/5` + `buildGraph` (lib/zog/resource_graph.ex:188-200,
  * :120-129) for the `:soa` backend: node ids 0..n_cap-1, arcs from[i]->to[i]
@@ -74,6 +82,20 @@ typedef struct {
  * n_cap returns ZG_ERR_NODE_NOT_FOUND (addEdge, array_graph.zig:89). */
 int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
                     const uint32_t *to, uint64_t m, zg_graph **out);
+
+/* Batch of independent graphs in one resource (no reference counterpart; the
+ * reference builds one resource per graph with new/5): graph i has
+ * node_counts[i] nodes and arc_counts[i] arcs whose local ids are in from/to,
+ * concatenated in graph order.  Ids are placed one graph after another, so
+ * results of zg_bc_unweighted over all sources (sources = NULL) are every
+ * graph's own BC, concatenated: paths never cross graphs.  The sweep limits
+ * each source group's work to the graphs its sources belong to.  PageRank on
+ * a batch of more than one graph returns ZG_ERR_INVALID_ARG. */
+int zg_graph_create_batch(uint32_t n_graphs, const uint32_t *node_counts, const uint64_t *arc_counts,
+                          const uint32_t *from, const uint32_t *to, zg_graph **out);
+
+/* Number of graphs in the resource (1 for zg_graph_create). */
+int zg_graph_components(const zg_graph *g, uint32_t *n_components);
 
 /* Replaces the resource destructor (resource_graph.ex:109-118); the NIF's
  * `nif_destroy` (:202-204) only releases the creation reference, so the
@@ -109,7 +131,7 @@ int zg_bc_unweighted_device(zg_graph *g, const uint32_t *d_sources, uint64_t n_s
  * `pagerank/4` NIF (resource_graph.ex:250-264) followed by extractScores:
  * out[n_cap] (0.0 for non-live ids).  Same iteration, dangling rule, strict
  * `l1 < tolerance` stop after the swap, n=0 and n=1 special cases.
- * precision is ZG_PR_FP32 or ZG_PR_FP64.  info may be NULL. */
+ * precision is ZG_PR_FP32, ZG_PR_FP64 or ZG_PR_MIXED.  info may be NULL. */
 int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
                 int precision, double *out, zg_pr_info *info);
 
@@ -133,6 +155,16 @@ int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sources, uint32
  * 256 sources per launch).  Returns the previous value, or <0. */
 int zg_set_batch_groups(zg_graph *g, int groups);
 
+/* Tuning parameters of the Brandes sweep (no reference counterpart; results
+ * are the same up to FP64 summation order).
+ *   ZG_TUNE_PUSH_ALPHA: a forward level runs from its frontier (push) when the
+ *   frontier's successor arcs times alpha are below the graph's arc count,
+ *   otherwise over all predecessor chunks (pull).  Default 8; 0 = always
+ *   pull; a tiny positive value = push whenever the frontier is non-empty.
+ * Returns 0 or <0. */
+#define ZG_TUNE_PUSH_ALPHA 1
+int zg_set_tuning(zg_graph *g, int param, double value);
+
 /* Source grouping for the batched sweep (no reference counterpart; the
  * result is the same sum, only FP64 summation order changes).  mode 0 keeps
  * the caller's order; mode 1 (default) sorts the sources by their distance to
@@ -153,6 +185,7 @@ int zg_set_source_order(zg_graph *g, int mode);
 #define ZG_PROF_BC_HUB 3
 #define ZG_PROF_BC_SETUP 4       /* per-batch resets, sources, final reduction */
 #define ZG_PROF_PR_ITER 5
+#define ZG_PROF_BC_PUSH 6
 #define ZG_PROF_SLOTS 8
 typedef struct {
     double ms[ZG_PROF_SLOTS];

@@ -125,3 +125,10 @@ def test_generators_deterministic():
     assert n == 20 and len(s) == 2 * (4 * 4 + 3 * 5 - 3)
     n, s, d = G.barabasi_albert(50, 4, 5)
     assert len(s) == 2 * (6 + 46 * 4)
+    assert np.array_equal(s, G.barabasi_albert(50, 4, 5)[1])
+    # clique first, then each new node's 4 distinct targets in ascending order
+    assert list(zip(s[:4], d[:4])) == [(0, 1), (1, 0), (0, 2), (2, 0)]
+    t = d[12:20:2]
+    assert np.all(s[12:20:2] == 4) and len(set(t.tolist())) == 4 and np.all(np.diff(t) > 0)
+    deg = np.bincount(s, minlength=50)
+    assert deg.min() >= 3 and deg.sum() == len(s)

@@ -206,6 +206,52 @@ def test_batch_widths_and_source_order_agree():
         assert_bc_close(o, outs[0])
 
 
+@pytest.mark.parametrize("alpha", [0.0, 1e-9, 8.0])
+def test_push_pull_directions(alpha):
+    # all-pull, all-push and the default switch give the oracle's BC, depths
+    # and path counts (hubs and parallel arcs included)
+    from oracle import oracle as O
+    n, src, dst = gen.rmat(12, 1 << 15, 1234)
+    src = np.concatenate([src, src[:500]])
+    dst = np.concatenate([dst, dst[:500]])
+    sources = gen.seeded_sources(n, src, dst, 200, 8)
+    og = O.Graph(n, src, dst)
+    ref, _ = og.betweenness(sources=sources)
+    g = make(n, src, dst)
+    rg.set_push_alpha(g, alpha)
+    info = {}
+    got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
+    assert_bc_close(got, ref)
+    if alpha == 0.0:
+        assert info["push_levels"] == 0
+    if alpha == 1e-9:
+        assert info["pull_levels"] == 0
+    depth, sigma, _, _ = rg.bc_paths(g, sources[:64], max_levels=64)
+    for i, s in enumerate(sources[:64].tolist()):
+        d_ref, su, *_ = og.bfs_paths(s)
+        assert np.array_equal(depth[i], d_ref)
+        r = d_ref != O.NONE_U32
+        assert np.array_equal(sigma[i][r], su[r].astype(np.float64))
+    rg.destroy(g)
+
+
+@pytest.mark.parametrize("groups", [1, 4, 16])
+def test_batch_of_uneven_graphs(groups):
+    # one resource holding graphs of uneven size: every graph's BC equals the
+    # oracle on that graph alone
+    from oracle import oracle as O
+    rng = np.random.default_rng(77)
+    graphs_ = [gen.barabasi_albert(int(n), 4, 100 + i) for i, n in enumerate([5, 70, 900, 130, 2500, 40])]
+    graphs_.append(graphs.random_multigraph(rng, 300, 1200))
+    graphs_.append((1, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
+    info = {}
+    outs = rg.betweenness_unweighted_batch(graphs_, info=info, groups=groups)
+    assert info["n_sources"] == sum(g[0] for g in graphs_)
+    for (n, src, dst), got in zip(graphs_, outs):
+        ref, _ = O.Graph(n, src, dst).betweenness()
+        assert_bc_close(got, ref)
+
+
 def test_depth_sigma_bit_exact():
     from oracle import oracle as O
     n, src, dst = gen.rmat(12, 1 << 15, 99)
@@ -305,6 +351,23 @@ def test_pagerank_every_row_bin():
     for prec, bar in ((32, PR_L1), (64, 1e-12)):
         got, ref, it_ref, info = check_pr(n, src, dst, iters=30, tol=0.0, precision=prec)
         assert np.abs(got - ref).sum() <= bar
+
+
+def test_pagerank_mixed_precision_tight_tolerance():
+    # a tolerance below the FP32 rounding floor: FP32 Kahan down to 1e-6, then
+    # FP64 from that iterate; the result meets the FP64 oracle's fixed point
+    from oracle import oracle as O
+    n, src, dst = gen.grid_2d(60, 60, seed=3, delete=300)
+    og = O.Graph(n, src, dst)
+    ref, it_ref, l1_ref = og.pagerank(0.85, 1000, 1e-10)
+    g = make(n, src, dst)
+    info = {}
+    got = rg.pagerank_array(g, 0.85, 1000, 1e-10, precision=96, info=info)
+    assert info["last_l1"] < 1e-10 and info["fp64_iterations"] > 0
+    assert abs(info["iterations"] - it_ref) <= 2
+    assert np.abs(got - ref).sum() <= 1e-8
+    assert info["error_bound_l1"] <= 1e-8
+    rg.destroy(g)
 
 
 def test_pagerank_max_iterations_zero_and_damping():

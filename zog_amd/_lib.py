@@ -24,11 +24,15 @@ ZG_ERR_INVALID_HANDLE = -8
 
 ZG_PR_FP32 = 32
 ZG_PR_FP64 = 64
+ZG_PR_MIXED = 96
 ZG_PR_SKIP_RESIDUAL = 0x100
+ZG_TUNE_PUSH_ALPHA = 1
 
 # Every symbol include/zog_hip.h declares.
 EXPORTED = (
     "zg_graph_create",
+    "zg_graph_create_batch",
+    "zg_graph_components",
     "zg_graph_destroy",
     "zg_graph_info",
     "zg_bc_unweighted",
@@ -38,12 +42,13 @@ EXPORTED = (
     "zg_bc_paths",
     "zg_set_batch_groups",
     "zg_set_source_order",
+    "zg_set_tuning",
     "zg_set_profiling",
     "zg_get_profile",
     "zg_strerror",
 )
 
-PROF_SLOTS = ("bc_forward", "bc_finalize", "bc_backward", "bc_hub", "bc_setup", "pr_iter")
+PROF_SLOTS = ("bc_forward", "bc_finalize", "bc_backward", "bc_hub", "bc_setup", "pr_iter", "bc_push")
 
 
 class ZgBcInfo(ctypes.Structure):
@@ -53,6 +58,8 @@ class ZgBcInfo(ctypes.Structure):
         ("nodes_reached", ctypes.c_uint64),
         ("max_levels", ctypes.c_uint32),
         ("batches", ctypes.c_uint32),
+        ("push_levels", ctypes.c_uint32),
+        ("pull_levels", ctypes.c_uint32),
         ("kernel_ms", ctypes.c_double),
     ]
 
@@ -66,6 +73,7 @@ class ZgPrInfo(ctypes.Structure):
         ("kernel_ms", ctypes.c_double),
         ("precision", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("fp64_iterations", ctypes.c_uint64),
     ]
 
 
@@ -100,6 +108,10 @@ def lib():
     P = ctypes.c_void_p
     L.zg_graph_create.argtypes = [ctypes.c_uint32, P, P, P, ctypes.c_uint64, ctypes.POINTER(P)]
     L.zg_graph_create.restype = ctypes.c_int
+    L.zg_graph_create_batch.argtypes = [ctypes.c_uint32, P, P, P, P, ctypes.POINTER(P)]
+    L.zg_graph_create_batch.restype = ctypes.c_int
+    L.zg_graph_components.argtypes = [P, P]
+    L.zg_graph_components.restype = ctypes.c_int
     L.zg_graph_destroy.argtypes = [P]
     L.zg_graph_destroy.restype = None
     L.zg_graph_info.argtypes = [P, P, P, P]
@@ -118,6 +130,8 @@ def lib():
     L.zg_bc_paths.restype = ctypes.c_int
     L.zg_set_batch_groups.argtypes = [P, ctypes.c_int]
     L.zg_set_batch_groups.restype = ctypes.c_int
+    L.zg_set_tuning.argtypes = [P, ctypes.c_int, ctypes.c_double]
+    L.zg_set_tuning.restype = ctypes.c_int
     L.zg_set_source_order.argtypes = [P, ctypes.c_int]
     L.zg_set_source_order.restype = ctypes.c_int
     L.zg_set_profiling.argtypes = [P, ctypes.c_int]

@@ -41,9 +41,19 @@
 // high-diameter graphs never overflow; every k is 0 until then.
 #include "common.h"
 
+#include <algorithm>
+
 struct GroupRange {
     uint32_t off[ZG_MAX_GROUPS];
     uint32_t cnt[ZG_MAX_GROUPS];
+};
+
+// Vertex range [vlo, vhi) and predecessor-chunk range [clo, chi) a group
+// works on: the components (graphs) its sources belong to.
+struct GroupSpan {
+    uint32_t vlo[ZG_MAX_GROUPS], vhi[ZG_MAX_GROUPS];
+    uint32_t clo[ZG_MAX_GROUPS], chi[ZG_MAX_GROUPS];
+    uint32_t used[ZG_MAX_GROUPS];   // source slots in use
 };
 
 struct BcDev {
@@ -60,6 +70,7 @@ struct BcDev {
     uint32_t *maxdep;           // [Gcap][V]
     uint64_t *vis, *fr;         // fr: [3][Gcap][V]
     unsigned long long *stats;  // [Gcap][ZG_FIN_BLOCKS][2]
+    uint32_t *touched;          // [Gcap][V]
     uint32_t *flist;
     uint64_t *fmask;
     uint32_t *hlist;
@@ -85,23 +96,31 @@ __device__ __forceinline__ double *cc_buf(const BcDev &a, int k)
     return a.cc + (size_t)k * a.Gcap * a.V * 64;
 }
 
-// Visited mask of every vertex starts as the group's unused lanes.
-__global__ void bc_init_vis_kernel(BcDev a, GroupRange valid_lanes)
+// Batch reset over a group's vertex span: the visited mask starts as the
+// group's unused lanes, the frontier buffers as empty.
+__global__ void bc_reset_kernel(BcDev a, GroupSpan sp)
 {
-    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t g = blockIdx.y;
-    if (v >= a.V) return;
-    uint32_t nl = valid_lanes.cnt[g];
-    uint64_t used = nl >= 64 ? ~0ull : ((1ull << nl) - 1);
+    const uint32_t g = blockIdx.y;
+    const uint32_t v = sp.vlo[g] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= sp.vhi[g]) return;
+    const uint32_t nl = sp.used[g];
+    const uint64_t used = nl >= 64 ? ~0ull : ((1ull << nl) - 1);
     a.vis[MASK(g, v)] = ~used;
+    fr_buf(a, 0)[MASK(g, v)] = 0;
+    fr_buf(a, 1)[MASK(g, v)] = 0;
+    fr_buf(a, 2)[MASK(g, v)] = 0;
 }
 
-// Vertices with several predecessor chunks accumulate sigma with atomics:
-// clear their rows first.
-__global__ void bc_zero_hub_rows_kernel(BcDev a, const uint32_t *rows)
+// Path-count rows of a group's span start at zero (push levels and
+// multi-chunk vertices add into them); exported depths start unreached.
+__global__ void bc_zero_rows_kernel(BcDev a, GroupSpan sp)
 {
-    uint32_t v = rows[blockIdx.x];
-    a.sig[ROW(blockIdx.y, v) + threadIdx.x] = 0.0;
+    const uint32_t g = blockIdx.y;
+    const uint32_t v = sp.vlo[g] + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= sp.vhi[g]) return;
+    const size_t r = ROW(g, v) + (threadIdx.x & 63);
+    a.sig[r] = 0.0;
+    if (a.dep) a.dep[r] = ZG_NONE;
 }
 
 // Level 0: every lane's source gets depth 0, sigma 1 and joins frontier 0.
@@ -139,6 +158,7 @@ __global__ void bc_source_kernel(BcDev a)
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
     if (l == 0) {
+        a.cnt[g].farcs = arcs;
         // slot 0 belongs to finalize block 0, which runs strictly later
         a.stats[(size_t)g * ZG_FIN_BLOCKS * 2] += arcs;
         a.stats[(size_t)g * ZG_FIN_BLOCKS * 2 + 1] += reached;
@@ -146,12 +166,12 @@ __global__ void bc_source_kernel(BcDev a)
 }
 
 // Forward level d: discover level d+1 by pulling over predecessor chunks.
-__global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d)
+__global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSpan sp)
 {
     const uint32_t g = blockIdx.y;
     const int lane = threadIdx.x & 63;
-    const uint32_t ch = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (ch >= a.n_ich) return;
+    const uint32_t ch = sp.clo[g] + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ch >= sp.chi[g]) return;
     const uint32_t v = a.ich_v[ch], e0 = a.ich_e0[ch], e1 = a.ich_e1[ch];
     const uint64_t unvis = ~a.vis[MASK(g, v)];
     if (unvis == 0) return;
@@ -217,6 +237,72 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d)
     }
 }
 
+__device__ __forceinline__ void push_flush_touched(const BcDev &a, uint32_t g, int lane, uint32_t tbuf, int tn)
+{
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&a.cnt[g].tcount, (uint32_t)tn);
+    base = zg_readlane32(base, 0);
+    if (lane < tn) {
+        if (base + lane < a.V) a.touched[(size_t)g * a.V + base + lane] = tbuf;
+        else atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+    }
+}
+
+// Forward level d, push over the level-d work list (sparse frontiers): each
+// entry's lanes add their count into every successor not yet visited by that
+// lane (FP64 atomics; exact for integer counts below 2^53).
+__global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d, GroupRange R)
+{
+    const uint32_t g = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= R.cnt[g]) return;
+    const size_t idx = g * a.fcap + R.off[g] + wid;
+    const uint32_t oc = a.flist[idx];
+    const uint64_t mask = a.fmask[idx];
+    const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
+    const bool mine = (mask >> lane) & 1;
+    const double val = mine ? ldexp(a.sig[ROW(g, v) + lane], -a.emax[EMAX(d, g, lane)]) : 0.0;
+    const uint64_t *visg = a.vis + (size_t)g * a.V;
+    uint64_t *next = fr_buf(a, (d + 1) % 3) + (size_t)g * a.V;
+    uint32_t tbuf = 0;  // lane k holds the k-th vertex this wavefront reached first
+    int tn = 0;
+    for (uint32_t base = e0; base < e1; base += 64) {
+        const uint32_t e = base + lane;
+        uint32_t w = 0;
+        uint64_t m = 0;
+        if (e < e1) {
+            w = a.out_col[e];
+            m = mask & ~visg[w];
+        }
+        uint64_t act = __ballot(m != 0);
+        while (act) {
+            const int j = __builtin_ctzll(act);
+            act &= act - 1;
+            const uint32_t wj = zg_readlane32(w, j);
+            const uint64_t mj = zg_readlane64(m, j);
+            if ((mj >> lane) & 1) {
+                const size_t r = ROW(g, wj) + lane;
+                atomicAdd(&a.sig[r], val);
+                cc_buf(a, 0)[r] = 0.0;
+                cc_buf(a, 1)[r] = 0.0;
+                if (a.dep) a.dep[r] = (uint32_t)(d + 1);
+            }
+            unsigned long long old = 0;
+            if (lane == 0) old = atomicOr((unsigned long long *)&next[wj], (unsigned long long)mj);
+            if (zg_readlane64(old, 0) == 0) {
+                // first to reach wj at this level: record it for the sparse finalize
+                if (lane == tn) tbuf = wj;
+                if (++tn == 64) {
+                    push_flush_touched(a, g, lane, tbuf, tn);
+                    tn = 0;
+                }
+            }
+        }
+    }
+    if (tn) push_flush_touched(a, g, lane, tbuf, tn);
+}
+
 // Exclusive prefix sum over a 1024-thread block (16 wavefronts); returns the
 // block total through *total.  All threads must call it.
 __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t x, uint32_t *lds, uint32_t *total)
@@ -242,11 +328,12 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t x, uint32_t *l
 // lists, clear the frontier buffer of level d-1.  Block b owns predecessor
 // chunks [b*R, (b+1)*R); a vertex is handled by the block holding its first
 // chunk.
-__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, int d, GroupRange stale)
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, int d, GroupRange stale, int pushed,
+                                                                   GroupSpan sp)
 {
     __shared__ uint32_t lds[40];
     __shared__ uint32_t base[2];
-    __shared__ unsigned long long sred[2][ZG_FIN_THREADS / 64];
+    __shared__ unsigned long long sred[3][ZG_FIN_THREADS / 64];
     const uint32_t g = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
     const int lane = tid & 63;
     for (uint32_t i = b * ZG_FIN_THREADS + tid; i < stale.cnt[g]; i += ZG_FIN_BLOCKS * ZG_FIN_THREADS) {
@@ -254,12 +341,13 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
         fr_buf(a, (d + 2) % 3)[MASK(g, a.och_v[oc])] = 0;
     }
     const uint64_t *next = fr_buf(a, (d + 1) % 3) + (size_t)g * a.V;
-    const uint32_t R = (a.n_ich + ZG_FIN_BLOCKS - 1) / ZG_FIN_BLOCKS;
-    const uint32_t c0 = min(b * R, a.n_ich), c1 = min(c0 + R, a.n_ich);
+    const uint32_t nch = sp.chi[g] - sp.clo[g];
+    const uint32_t R = (nch + ZG_FIN_BLOCKS - 1) / ZG_FIN_BLOCKS;
+    const uint32_t c0 = sp.clo[g] + min(b * R, nch), c1 = sp.clo[g] + min(b * R + R, nch);
 
     // pass A: fold masks, count list entries, statistics
     uint32_t nf = 0, nh = 0;
-    unsigned long long arcs = 0, reached = 0;
+    unsigned long long arcs = 0, reached = 0, farcs = 0;
     for (uint32_t c = c0 + tid; c < c1; c += ZG_FIN_THREADS) {
         const uint32_t v = a.ich_v[c];
         if (a.ich_e0[c] != a.in_ptr[v]) continue;
@@ -271,9 +359,11 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
         nf += kf;
         nh += kf > 1 ? 1 : 0;
         const unsigned long long pc = __popcll(nm);
-        arcs += pc * (a.out_ptr[v + 1] - a.out_ptr[v]);
+        const uint32_t od = a.out_ptr[v + 1] - a.out_ptr[v];
+        arcs += pc * od;
         reached += pc;
-        if (a.in_ptr[v + 1] - a.in_ptr[v] > ZG_CHUNK) {
+        farcs += od;
+        if (pushed || a.in_ptr[v + 1] - a.in_ptr[v] > ZG_CHUNK) {
             // summed by atomics: the level-(d+1) exponent is known only now
             uint64_t mm = nm;
             while (mm) {
@@ -293,20 +383,24 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
     }
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
+    farcs = zg_wave_sum_u64(farcs);
     if (lane == 0) {
         sred[0][tid >> 6] = arcs;
         sred[1][tid >> 6] = reached;
+        sred[2][tid >> 6] = farcs;
     }
     __syncthreads();
     if (tid == 0) {
-        unsigned long long A = 0, Rr = 0;
+        unsigned long long A = 0, Rr = 0, F = 0;
         for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
             A += sred[0][w];
             Rr += sred[1][w];
+            F += sred[2][w];
         }
         unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * 2;
         st[0] += A;
         st[1] += Rr;
+        if (F) atomicAdd(&a.cnt[g].farcs, F);
     }
     if (tf == 0 && th == 0) return;
 
@@ -325,6 +419,117 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
                     kh = kf > 1 ? 1 : 0;
                 }
             }
+        }
+        uint32_t sf_tot, sh_tot;
+        const uint32_t of = block_excl_scan_1024(kf, lds, &sf_tot);
+        const uint32_t oh = block_excl_scan_1024(kh, lds, &sh_tot);
+        if (kf) {
+            const uint32_t pos = fpos + of, q0 = a.och_first[v];
+            for (uint32_t q = 0; q < kf; q++) {
+                if (pos + q < a.fcap) {
+                    a.flist[g * a.fcap + pos + q] = q0 + q;
+                    a.fmask[g * a.fcap + pos + q] = nm;
+                } else {
+                    atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+                }
+            }
+        }
+        if (kh) {
+            const uint32_t pos = hpos + oh;
+            if (pos < a.hcap) {
+                a.hlist[g * a.hcap + pos] = v;
+                a.hmask[g * a.hcap + pos] = nm;
+            } else {
+                atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
+            }
+        }
+        fpos += sf_tot;
+        hpos += sh_tot;
+    }
+}
+
+// Finalize level d after a push level: same as bc_finalize_kernel, over the
+// vertices the push reached first (a fixed grid striding over the list).
+#define ZG_SPARSE_FIN_BLOCKS 64
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDev a, int d, GroupRange stale)
+{
+    __shared__ uint32_t lds[40];
+    __shared__ uint32_t base[2];
+    __shared__ unsigned long long sred[3][ZG_FIN_THREADS / 64];
+    const uint32_t g = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+    const int lane = tid & 63;
+    for (uint32_t i = b * ZG_FIN_THREADS + tid; i < stale.cnt[g]; i += ZG_SPARSE_FIN_BLOCKS * ZG_FIN_THREADS) {
+        uint32_t oc = a.flist[g * a.fcap + stale.off[g] + i];
+        fr_buf(a, (d + 2) % 3)[MASK(g, a.och_v[oc])] = 0;
+    }
+    const uint64_t *next = fr_buf(a, (d + 1) % 3) + (size_t)g * a.V;
+    const uint32_t *tl = a.touched + (size_t)g * a.V;
+    const uint32_t nt = min(a.cnt[g].tcount, a.V);
+    const uint32_t R = (nt + ZG_SPARSE_FIN_BLOCKS - 1) / ZG_SPARSE_FIN_BLOCKS;
+    const uint32_t t0 = min(b * R, nt), t1 = min(t0 + R, nt);
+
+    uint32_t nf = 0, nh = 0;
+    unsigned long long arcs = 0, reached = 0, farcs = 0;
+    for (uint32_t t = t0 + tid; t < t1; t += ZG_FIN_THREADS) {
+        const uint32_t v = tl[t];
+        const uint64_t nm = next[v];
+        a.vis[MASK(g, v)] |= nm;
+        a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);
+        const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
+        nf += kf;
+        nh += kf > 1 ? 1 : 0;
+        const unsigned long long pc = __popcll(nm);
+        const uint32_t od = a.out_ptr[v + 1] - a.out_ptr[v];
+        arcs += pc * od;
+        reached += pc;
+        farcs += od;
+        uint64_t mm = nm;
+        while (mm) {
+            const int l = __builtin_ctzll(mm);
+            mm &= mm - 1;
+            const int ex = ilogb(a.sig[ROW(g, v) + l]);
+            if (ex > ZG_SHIFT_T) atomicMax(&a.emax[EMAX(d + 1, g, l)], ex);
+        }
+    }
+    uint32_t tf, th;
+    (void)block_excl_scan_1024(nf, lds, &tf);
+    (void)block_excl_scan_1024(nh, lds, &th);
+    if (tid == 0) {
+        base[0] = tf ? atomicAdd(&a.cnt[g].fcount, tf) : 0;
+        base[1] = th ? atomicAdd(&a.cnt[g].hcount, th) : 0;
+    }
+    arcs = zg_wave_sum_u64(arcs);
+    reached = zg_wave_sum_u64(reached);
+    farcs = zg_wave_sum_u64(farcs);
+    if (lane == 0) {
+        sred[0][tid >> 6] = arcs;
+        sred[1][tid >> 6] = reached;
+        sred[2][tid >> 6] = farcs;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long A = 0, Rr = 0, F = 0;
+        for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
+            A += sred[0][w];
+            Rr += sred[1][w];
+            F += sred[2][w];
+        }
+        unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * 2;
+        st[0] += A;
+        st[1] += Rr;
+        if (F) atomicAdd(&a.cnt[g].farcs, F);
+    }
+    if (tf == 0 && th == 0) return;
+    uint32_t fpos = base[0], hpos = base[1];
+    for (uint32_t tt = t0; tt < t1; tt += ZG_FIN_THREADS) {
+        const uint32_t t = tt + tid;
+        uint32_t v = 0, kf = 0, kh = 0;
+        uint64_t nm = 0;
+        if (t < t1) {
+            v = tl[t];
+            nm = next[v];
+            kf = a.och_first[v + 1] - a.och_first[v];
+            kh = kf > 1 ? 1 : 0;
         }
         uint32_t sf_tot, sh_tot;
         const uint32_t of = block_excl_scan_1024(kf, lds, &sf_tot);
@@ -475,7 +680,7 @@ int zg_bc_ws_release(zg_graph *g)
 {
     zg_bc_ws &w = g->bc;
     zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep); zg_dfree(w.maxdep);
-    zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.flist); zg_dfree(w.fmask);
+    zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.touched); zg_dfree(w.flist); zg_dfree(w.fmask);
     zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.src); zg_dfree(w.cnt);
     if (w.h_cnt) (void)hipHostFree(w.h_cnt);
     w.h_cnt = nullptr;
@@ -504,6 +709,7 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth)
     if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * 2);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.touched, G * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.flist, G * w.fcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fmask, G * w.fcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hlist, G * w.hcap);
@@ -563,7 +769,7 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used)
     a.ohub_idx = g->ohub_idx;
     a.live = g->live;
     a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep; a.maxdep = w.maxdep;
-    a.vis = w.vis; a.fr = w.fr; a.stats = w.stats;
+    a.vis = w.vis; a.fr = w.fr; a.stats = w.stats; a.touched = w.touched;
     a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
     a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
     a.fcap = w.fcap; a.hcap = w.hcap;
@@ -579,8 +785,8 @@ typedef int (*bc_batch_hook)(zg_graph *g, hipStream_t s, uint64_t first, uint32_
 
 // Runs the batched sweep over d_sources[0..n) accumulating into the group
 // accumulators; the caller reduces them.
-static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStream_t s, zg_bc_info *inf,
-                    bc_batch_hook hook, void *user)
+static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_sources, uint64_t n, hipStream_t s,
+                    zg_bc_info *inf, bc_batch_hook hook, void *user)
 {
     zg_bc_ws &w = g->bc;
     const uint32_t V = g->V;
@@ -591,19 +797,43 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStrea
         const uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)Gcap * 64, n - b0);
         const uint32_t G = cdiv(nb, 64);
         BcDev a = bc_dev(g, G);
-        GroupRange valid;
-        memset(&valid, 0, sizeof(valid));
-        for (uint32_t q = 0; q < G; q++) valid.cnt[q] = std::min<uint32_t>(64, nb - 64 * q);
+        // per-group spans: the components its sources belong to
+        GroupSpan sp;
+        memset(&sp, 0, sizeof(sp));
+        uint32_t max_v = 0, max_c = 0;
+        for (uint32_t q = 0; q < G; q++) {
+            sp.used[q] = std::min<uint32_t>(64, nb - 64 * q);
+            if (h_sources) {
+                uint32_t lo = ZG_NONE, hi = 0;
+                for (uint32_t l = 0; l < sp.used[q]; l++) {
+                    const uint32_t x = h_sources[b0 + 64 * q + l];
+                    lo = std::min(lo, x);
+                    hi = std::max(hi, x);
+                }
+                const std::vector<uint32_t> &co = g->h_comp_off;
+                const size_t c_lo = std::upper_bound(co.begin(), co.end(), lo) - co.begin() - 1;
+                const size_t c_hi = std::upper_bound(co.begin(), co.end(), hi) - co.begin() - 1;
+                sp.vlo[q] = co[c_lo];
+                sp.vhi[q] = co[c_hi + 1];
+            } else {
+                sp.vlo[q] = 0;
+                sp.vhi[q] = V;
+            }
+            sp.clo[q] = g->h_ich_first[sp.vlo[q]];
+            sp.chi[q] = g->h_ich_first[sp.vhi[q]];
+            max_v = std::max(max_v, sp.vhi[q] - sp.vlo[q]);
+            max_c = std::max(max_c, sp.chi[q] - sp.clo[q]);
+        }
         // batch reset
         ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * Gcap * 64, s));
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
-        ZG_TRY(hipMemsetAsync(w.fr, 0, sizeof(uint64_t) * 3 * (size_t)Gcap * V, s));
         ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
         ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * 2, s));
         ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
-        if (w.dep) ZG_TRY(hipMemsetAsync(w.dep, 0xFF, sizeof(uint32_t) * (size_t)Gcap * V * 64, s));
-        if (V) ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_init_vis_kernel, dim3(cdiv(V, 256), G), dim3(256), 0, s, a, valid);
-        if (g->n_inhub) ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_hub_rows_kernel, dim3(g->n_inhub, G), dim3(64), 0, s, a, g->inhub);
+        if (max_v) {
+            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reset_kernel, dim3(cdiv(max_v, 256), G), dim3(256), 0, s, a, sp);
+            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_rows_kernel, dim3(cdiv(max_v, 4), G), dim3(256), 0, s, a, sp);
+        }
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_source_kernel, dim3(1, G), dim3(64), 0, s, a);
         ZG_TRY(hipGetLastError());
         for (uint32_t q = 0; q < G; q++) {
@@ -619,12 +849,17 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStrea
             foff[q].push_back(w.h_cnt[q].fcount);  // end of level 0
             hoff[q].push_back(w.h_cnt[q].hcount);
         }
-        const uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(g->n_ich, 4));
+        const uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(max_c, 4));
+        std::vector<unsigned long long> farcs(G);
+        for (uint32_t q = 0; q < G; q++) farcs[q] = w.h_cnt[q].farcs;
         int d = 0;
         for (;;) {
             ZG_TRY_RC(bc_grow_levels(g, (uint32_t)d + 2, s));
             a.emax = w.emax;
-            if (g->n_ich) ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(fwd_blocks, G), dim3(256), 0, s, a, d);
+            // direction: push from a small frontier, pull over predecessor chunks otherwise
+            unsigned long long fmax = 0;
+            for (uint32_t q = 0; q < G; q++) fmax = std::max(fmax, farcs[q]);
+            const bool push = g->push_alpha > 0.0 && (double)fmax * g->push_alpha < (double)g->E;
             GroupRange stale;
             memset(&stale, 0, sizeof(stale));
             if (d >= 1)
@@ -632,8 +867,29 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStrea
                     stale.off[q] = foff[q][d - 1];
                     stale.cnt[q] = foff[q][d] - foff[q][d - 1];
                 }
-            ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_kernel, dim3(ZG_FIN_BLOCKS, G), dim3(ZG_FIN_THREADS), 0, s,
-                           a, d, stale);
+            // reset the touched-vertex count and the successor-arc count
+            ZG_TRY(hipMemset2DAsync(&w.cnt[0].tcount, sizeof(zg_bc_ws::Counters), 0,
+                                    sizeof(uint32_t) + sizeof(unsigned long long), G, s));
+            if (push) {
+                GroupRange F;
+                memset(&F, 0, sizeof(F));
+                uint32_t mf = 0;
+                for (uint32_t q = 0; q < G; q++) {
+                    F.off[q] = foff[q][d];
+                    F.cnt[q] = foff[q][d + 1] - foff[q][d];
+                    mf = std::max(mf, F.cnt[q]);
+                }
+                if (mf) ZG_PROF_LAUNCH(ZG_PROF_BC_PUSH, bc_push_kernel, dim3(cdiv(mf, 4), G), dim3(256), 0, s, a, d, F);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_sparse_kernel, dim3(ZG_SPARSE_FIN_BLOCKS, G),
+                               dim3(ZG_FIN_THREADS), 0, s, a, d, stale);
+                inf->push_levels++;
+            } else {
+                if (max_c)
+                    ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(fwd_blocks, G), dim3(256), 0, s, a, d, sp);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_kernel, dim3(ZG_FIN_BLOCKS, G), dim3(ZG_FIN_THREADS), 0,
+                               s, a, d, stale, 0, sp);
+                inf->pull_levels++;
+            }
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(w.h_cnt, w.cnt, sizeof(zg_bc_ws::Counters) * G, hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
@@ -644,6 +900,7 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, hipStrea
                 if (fc > foff[q].back()) any = true;
                 foff[q].push_back(fc);
                 hoff[q].push_back(w.h_cnt[q].hcount);
+                farcs[q] = w.h_cnt[q].farcs;
             }
             if (!any) break;
             d++;
@@ -709,16 +966,24 @@ static int bc_core_distances(zg_graph *g, hipStream_t s)
 {
     if (!g->h_coredist.empty() || g->V == 0) return ZG_OK;
     const uint32_t V = g->V;
-    std::vector<uint32_t> order(V);
-    for (uint32_t v = 0; v < V; v++) order[v] = v;
-    const uint32_t k = std::min<uint32_t>(ZG_CORE_SIZE, V);
-    std::partial_sort(order.begin(), order.begin() + k, order.end(), [&](uint32_t x, uint32_t y) {
-        if (g->h_in_deg[x] != g->h_in_deg[y]) return g->h_in_deg[x] > g->h_in_deg[y];
-        return x < y;
-    });
     std::vector<uint32_t> dist(V, ZG_NONE);
-    for (uint32_t i = 0; i < k; i++)
-        if (g->h_live[order[i]]) dist[order[i]] = 0;
+    const std::vector<uint32_t> &co = g->h_comp_off;
+    const size_t ncomp = co.size() - 1;
+    // the core of each component: its highest in-degree live vertices
+    // (ZG_CORE_SIZE of them in a single graph, one per graph in a batch)
+    const uint32_t per = ncomp == 1 ? ZG_CORE_SIZE : 1;
+    std::vector<uint32_t> order;
+    for (size_t c = 0; c < ncomp; c++) {
+        order.clear();
+        for (uint32_t v = co[c]; v < co[c + 1]; v++)
+            if (g->h_live[v]) order.push_back(v);
+        const size_t k = std::min<size_t>(per, order.size());
+        std::partial_sort(order.begin(), order.begin() + k, order.end(), [&](uint32_t x, uint32_t y) {
+            if (g->h_in_deg[x] != g->h_in_deg[y]) return g->h_in_deg[x] > g->h_in_deg[y];
+            return x < y;
+        });
+        for (size_t i = 0; i < k; i++) dist[order[i]] = 0;
+    }
     uint32_t *d_dist = nullptr, *d_changed = nullptr;
     int rc = zg_dmalloc(&d_dist, V);
     if (rc == ZG_OK) rc = zg_dmalloc(&d_changed, 1);
@@ -742,12 +1007,20 @@ static int bc_core_distances(zg_graph *g, hipStream_t s)
     return rc;
 }
 
-// Sorts sources so that each 64-source group holds sources at the same
-// distance from the hub core (then larger out-degree first, then id).
+// Sorts sources so that each 64-source group holds sources of one graph
+// (component) at the same distance from its hub core (then larger out-degree
+// first, then id).  Components are contiguous id ranges, so comparing ids
+// across components orders by component.
 static void bc_order_sources(const zg_graph *g, std::vector<uint32_t> &src)
 {
     const std::vector<uint32_t> &cd = g->h_coredist;
+    const std::vector<uint32_t> &co = g->h_comp_off;
+    auto comp = [&](uint32_t x) { return std::upper_bound(co.begin(), co.end(), x) - co.begin(); };
     std::stable_sort(src.begin(), src.end(), [&](uint32_t x, uint32_t y) {
+        if (co.size() > 2) {
+            const auto cx = comp(x), cy = comp(y);
+            if (cx != cy) return cx < cy;
+        }
         if (cd[x] != cd[y]) return cd[x] < cd[y];
         if (g->h_out_deg[x] != g->h_out_deg[y]) return g->h_out_deg[x] > g->h_out_deg[y];
         return x < y;
@@ -769,35 +1042,38 @@ static int zg_bc_device_impl(zg_graph *g, const uint32_t *d_sources, uint64_t n_
     zg_bc_info inf;
     memset(&inf, 0, sizeof(inf));
     const uint32_t V = g->V;
-    uint32_t *d_tmp = nullptr;
-    if (!d_sources || (g->source_order && n_sources > 64)) {
-        std::vector<uint32_t> list;
-        if (!d_sources) {
-            bc_all_live_sources(g, list);
-        } else {
-            list.resize(n_sources);
+    // the sweep needs the source list on the host (ordering, per-group spans)
+    std::vector<uint32_t> list;
+    if (!d_sources) {
+        bc_all_live_sources(g, list);
+    } else {
+        list.resize(n_sources);
+        if (n_sources) {
             ZG_TRY(hipMemcpyAsync(list.data(), d_sources, sizeof(uint32_t) * n_sources, hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
-            for (uint32_t x : list)
-                if (x >= V || !g->h_live[x]) return ZG_ERR_NODE_NOT_FOUND;
         }
+        for (uint32_t x : list)
+            if (x >= V || !g->h_live[x]) return ZG_ERR_NODE_NOT_FOUND;
+    }
+    uint32_t *d_tmp = nullptr;
+    if (!d_sources || (g->source_order && list.size() > 64)) {
         if (g->source_order && list.size() > 64) {
             ZG_TRY_RC(bc_core_distances(g, s));
             bc_order_sources(g, list);
         }
-        n_sources = list.size();
         ZG_TRY_RC(zg_dmalloc(&d_tmp, list.size()));
         if (!list.empty())
             ZG_TRY(hipMemcpyAsync(d_tmp, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, s));
         ZG_TRY(hipStreamSynchronize(s));
         d_sources = d_tmp;
     }
+    n_sources = list.size();
     int rc = bc_ws_ensure(g, false);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e0, s));
-    if (rc == ZG_OK) rc = bc_sweep(g, d_sources, n_sources, s, &inf, nullptr, nullptr);
+    if (rc == ZG_OK) rc = bc_sweep(g, d_sources, list.data(), n_sources, s, &inf, nullptr, nullptr);
     if (rc == ZG_OK && V) {
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reduce_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V,
                        (uint32_t)g->bc.groups, g->bc.bcg, g->live, d_out);
@@ -934,7 +1210,7 @@ extern "C" int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sour
         rc = zg_hip_code(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, s));
     zg_bc_info inf;
     memset(&inf, 0, sizeof(inf));
-    if (rc == ZG_OK) rc = bc_sweep(g, d_src, n_sources, s, &inf, bc_paths_hook, &c);
+    if (rc == ZG_OK) rc = bc_sweep(g, d_src, sources, n_sources, s, &inf, bc_paths_hook, &c);
     (void)hipStreamSynchronize(s);
     zg_dfree(c.d_dep);
     zg_dfree(c.d_sig);

@@ -94,6 +94,7 @@ struct zg_bc_ws {
     uint32_t *maxdep = nullptr;          // [G][V] deepest level of v over the group's lanes
     uint64_t *vis = nullptr, *fr = nullptr;
     unsigned long long *stats = nullptr; // [G][ZG_FIN_BLOCKS][2] arcs, reached
+    uint32_t *touched = nullptr;         // [G][V] vertices reached by a push level
     uint32_t *flist = nullptr;           // [G][fcap]
     uint64_t *fmask = nullptr;
     uint32_t *hlist = nullptr;           // [G][hcap]
@@ -103,8 +104,9 @@ struct zg_bc_ws {
     uint32_t *src = nullptr;             // [G][64]
     struct Counters {
         uint32_t fcount, hcount;
-        uint32_t err;   // ZG_BC_ERR_* bits
-        uint32_t pad;
+        uint32_t err;               // ZG_BC_ERR_* bits
+        uint32_t tcount;            // vertices first reached by the current push level
+        unsigned long long farcs;   // successor arcs of the newest level (vertex-level)
     };
     Counters *cnt = nullptr;             // [G]
     Counters *h_cnt = nullptr;           // pinned host mirror
@@ -138,10 +140,15 @@ struct zg_graph {
     std::vector<uint32_t> h_out_deg, h_in_deg;
     std::vector<uint32_t> h_coredist;    // distance to the hub core (lazy)
     int source_order = 1;
+    double push_alpha = 8.0;
     uint32_t n_dangling = 0;
+    // disjoint components given at creation (zg_graph_create_batch): vertex
+    // offsets, n_comp + 1 entries; a plain graph is one component [0, V)
+    std::vector<uint32_t> h_comp_off;
     // predecessor chunks (forward sweep work items)
     uint32_t n_ich = 0;
     uint32_t *ich_v = nullptr, *ich_e0 = nullptr, *ich_e1 = nullptr;
+    std::vector<uint32_t> h_ich_first;   // [V+1] first predecessor chunk of each vertex
     uint32_t n_inhub = 0;
     uint32_t *inhub = nullptr;           // vertices with more than one predecessor chunk
     // successor chunks (backward sweep work items)
@@ -153,6 +160,7 @@ struct zg_graph {
     uint32_t n_small = 0, n_med = 0, n_large = 0, n_seg = 0;
     uint32_t n_m4 = 0, n_m16 = 0, n_m64 = 0;   // n_med = n_m4 + n_m16 + n_m64
     uint32_t *pr_small = nullptr, *pr_med = nullptr;   // pr_med = [m4 | m16 | m64]
+    uint4 *pr_desc = nullptr;            // [small | m4 | m16 | m64] rows: {v, first arc, in-deg, out-deg}
     uint32_t *pr_large = nullptr;        // [n_large] rows
     uint32_t *pr_large_seg0 = nullptr;   // [n_large + 1] first segment of each large row
     uint32_t *pr_seg_row = nullptr;      // [n_seg] large-row index of each segment

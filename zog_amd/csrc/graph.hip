@@ -103,7 +103,7 @@ static void zg_free_device(zg_graph *g)
     zg_dfree(g->ich_v); zg_dfree(g->ich_e0); zg_dfree(g->ich_e1); zg_dfree(g->inhub);
     zg_dfree(g->och_first); zg_dfree(g->och_v); zg_dfree(g->och_e0); zg_dfree(g->och_e1);
     zg_dfree(g->ohub_idx);
-    zg_dfree(g->pr_small); zg_dfree(g->pr_med); zg_dfree(g->pr_large); zg_dfree(g->pr_large_seg0);
+    zg_dfree(g->pr_small); zg_dfree(g->pr_med); zg_dfree(g->pr_desc); zg_dfree(g->pr_large); zg_dfree(g->pr_large_seg0);
     zg_dfree(g->pr_seg_row); zg_dfree(g->pr_seg_e0); zg_dfree(g->pr_seg_e1);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     g->stream = nullptr;
@@ -121,6 +121,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
                     uint64_t m)
 {
     const uint32_t V = g->V;
+    if (g->h_comp_off.empty()) g->h_comp_off = {0u, V};
     g->h_live.assign(V, 1);
     if (live_mask)
         for (uint32_t i = 0; i < V; i++) g->h_live[i] = live_mask[i] ? 1 : 0;
@@ -172,7 +173,9 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     // Predecessor chunks: forward-sweep work items (vertices with no
     // predecessor are reached only as sources and get no chunk).
     std::vector<uint32_t> ich_v, ich_e0, ich_e1, inhub;
+    g->h_ich_first.assign((size_t)V + 1, 0);
     for (uint32_t v = 0; v < V; v++) {
+        g->h_ich_first[v] = (uint32_t)ich_v.size();
         uint32_t b = in_ptr[v], e = in_ptr[(size_t)v + 1];
         if (b == e || !g->h_live[v]) continue;
         if (e - b > ZG_CHUNK) inhub.push_back(v);
@@ -183,6 +186,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
         }
     }
     g->n_ich = (uint32_t)ich_v.size();
+    g->h_ich_first[V] = g->n_ich;
     g->n_inhub = (uint32_t)inhub.size();
     // Successor chunks: backward-sweep work items; every vertex has at least
     // one (possibly empty) chunk because a leaf still publishes 1/sigma.
@@ -239,6 +243,11 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     med.insert(med.end(), m16.begin(), m16.end());
     med.insert(med.end(), m64.begin(), m64.end());
     g->n_med = (uint32_t)med.size();
+    std::vector<uint4> desc;
+    desc.reserve(small.size() + med.size());
+    for (const std::vector<uint32_t> *bin : {&small, &med})
+        for (uint32_t v : *bin)
+            desc.push_back(make_uint4(v, in_ptr[v], in_ptr[(size_t)v + 1] - in_ptr[v], out_ptr[(size_t)v + 1] - out_ptr[v]));
     g->n_large = (uint32_t)large.size();
     g->n_seg = (uint32_t)seg_row.size();
 
@@ -258,6 +267,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     ZG_TRY_RC(zg_upload(&g->ohub_idx, ohub_idx));
     ZG_TRY_RC(zg_upload(&g->pr_small, small));
     ZG_TRY_RC(zg_upload(&g->pr_med, med));
+    ZG_TRY_RC(zg_upload(&g->pr_desc, desc));
     ZG_TRY_RC(zg_upload(&g->pr_large, large));
     ZG_TRY_RC(zg_upload(&g->pr_large_seg0, large_seg0));
     ZG_TRY_RC(zg_upload(&g->pr_seg_row, seg_row));
@@ -266,13 +276,9 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     return ZG_OK;
 }
 
-extern "C" int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
-                               const uint32_t *to, uint64_t m, zg_graph **out)
+static int zg_create_common(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
+                            uint64_t m, std::vector<uint32_t> comp_off, zg_graph **out)
 {
-    if (!out) return ZG_ERR_INVALID_ARG;
-    *out = nullptr;
-    if (m && (!from || !to)) return ZG_ERR_INVALID_ARG;
-    if (m >= 0xFFFFFFFFull || n_cap == 0xFFFFFFFFu) return ZG_ERR_TOO_LARGE;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         (void)hipGetLastError();
@@ -287,6 +293,7 @@ extern "C" int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const u
     if (e != hipSuccess) rc = zg_hip_code(e);
     if (rc == ZG_OK) {
         try {
+            g->h_comp_off = std::move(comp_off);
             rc = zg_build(g, live_mask, from, to, m);
         } catch (const std::bad_alloc &) {
             rc = ZG_ERR_OOM;
@@ -298,6 +305,60 @@ extern "C" int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const u
         return rc;
     }
     *out = g;
+    return ZG_OK;
+}
+
+extern "C" int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
+                               const uint32_t *to, uint64_t m, zg_graph **out)
+{
+    if (!out) return ZG_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (m && (!from || !to)) return ZG_ERR_INVALID_ARG;
+    if (m >= 0xFFFFFFFFull || n_cap == 0xFFFFFFFFu) return ZG_ERR_TOO_LARGE;
+    return zg_create_common(n_cap, live_mask, from, to, m, {}, out);
+}
+
+extern "C" int zg_graph_create_batch(uint32_t n_graphs, const uint32_t *node_counts, const uint64_t *arc_counts,
+                                     const uint32_t *from, const uint32_t *to, zg_graph **out)
+{
+    if (!out) return ZG_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (n_graphs && (!node_counts || !arc_counts)) return ZG_ERR_INVALID_ARG;
+    uint64_t V = 0, M = 0;
+    for (uint32_t i = 0; i < n_graphs; i++) {
+        V += node_counts[i];
+        M += arc_counts[i];
+    }
+    if (M && (!from || !to)) return ZG_ERR_INVALID_ARG;
+    if (M >= 0xFFFFFFFFull || V >= 0xFFFFFFFFull) return ZG_ERR_TOO_LARGE;
+    std::vector<uint32_t> comp_off, gf, gt;
+    try {
+        comp_off.resize((size_t)n_graphs + 1);
+        gf.resize(M);
+        gt.resize(M);
+        uint64_t vo = 0, mo = 0;
+        for (uint32_t i = 0; i < n_graphs; i++) {
+            comp_off[i] = (uint32_t)vo;
+            for (uint64_t k = 0; k < arc_counts[i]; k++) {
+                const uint32_t f = from[mo + k], t = to[mo + k];
+                if (f >= node_counts[i] || t >= node_counts[i]) return ZG_ERR_NODE_NOT_FOUND;
+                gf[mo + k] = (uint32_t)(vo + f);
+                gt[mo + k] = (uint32_t)(vo + t);
+            }
+            vo += node_counts[i];
+            mo += arc_counts[i];
+        }
+        comp_off[n_graphs] = (uint32_t)vo;
+    } catch (const std::bad_alloc &) {
+        return ZG_ERR_OOM;
+    }
+    return zg_create_common((uint32_t)V, nullptr, gf.data(), gt.data(), M, std::move(comp_off), out);
+}
+
+extern "C" int zg_graph_components(const zg_graph *g, uint32_t *n_components)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (n_components) *n_components = (uint32_t)g->h_comp_off.size() - 1;
     return ZG_OK;
 }
 
@@ -320,6 +381,19 @@ extern "C" int zg_graph_info(const zg_graph *g, uint32_t *n_cap, uint32_t *n_liv
     if (n_live) *n_live = g->n_live;
     if (n_arcs) *n_arcs = g->E;
     return ZG_OK;
+}
+
+extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    switch (param) {
+    case ZG_TUNE_PUSH_ALPHA:
+        if (!(value >= 0.0)) return ZG_ERR_INVALID_ARG;
+        g->push_alpha = value;
+        return ZG_OK;
+    default: return ZG_ERR_INVALID_ARG;
+    }
 }
 
 extern "C" int zg_set_source_order(zg_graph *g, int mode)

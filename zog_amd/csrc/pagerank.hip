@@ -20,6 +20,8 @@
 #include "common.h"
 #include "pr_bins.h"
 
+#include <algorithm>
+
 struct PrState {
     unsigned int done;
     unsigned int pad;
@@ -36,6 +38,7 @@ template <typename T>
 struct PrArgs {
     const uint32_t *in_ptr, *in_col, *out_ptr;
     const uint32_t *small, *med, *large, *large_seg0, *seg_e0, *seg_e1;
+    const uint4 *desc;          // [small | m4 | m16 | m64]: {v, first arc, in-degree, out-degree}
     uint32_t n_small, n_m4, n_m16, n_m64, n_large, n_seg;
     uint32_t nb_small, nb_m4, nb_m16, nb_m64;  // work units of each bin
     T *r0, *r1, *x0, *x1;       // parity 0 / 1 buffers
@@ -97,10 +100,23 @@ __device__ __forceinline__ void pr_row_epilogue(const PrArgs<T> &a, const T *rol
                                                 uint32_t v, T s, T c, double base, double damping,
                                                 double &l1, double &dang);
 
+// Same, with the old rank and the out-degree already loaded.
+template <typename T>
+__device__ __forceinline__ void pr_row_finish(T *rnew, T *xnew, uint32_t v, T r_old, uint32_t outdeg, T s, T c,
+                                              double base, double damping, double &l1, double &dang)
+{
+    const double sum = (double)s - (double)c;
+    const T rt = (T)(base + damping * sum);
+    rnew[v] = rt;
+    l1 += fabs((double)rt - (double)r_old);
+    if (outdeg == 0) dang += (double)rt;
+    else xnew[v] = rt / (T)outdeg;
+}
+
 // Rows of one bin summed by groups of L lanes: unit u covers rows
 // [u*(256/L), (u+1)*(256/L)) of the bin; lane groups merge their Kahan pairs.
 template <typename T, int L>
-__device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint32_t *rows, uint32_t nrows,
+__device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *rows, uint32_t nrows,
                                               uint32_t unit, const T *x, const T *rold, T *rnew, T *xnew,
                                               double base, double damping, double &l1, double &dang)
 {
@@ -108,11 +124,12 @@ __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint32_t
     const uint32_t i = unit * (ZG_PR_BLOCK / L) + tid / L;
     const uint32_t sub = tid % L;
     const bool ok = i < nrows;
-    uint32_t v = 0;
-    T s = 0, c = 0;
+    uint4 dsc = make_uint4(0, 0, 0, 0);
+    T s = 0, c = 0, r_old = 0;
     if (ok) {
-        v = rows[i];
-        pr_strided_kahan(a.in_col, x, a.in_ptr[v] + sub, a.in_ptr[v + 1], (uint32_t)L, s, c);
+        dsc = rows[i];
+        if (sub == 0) r_old = rold[dsc.x];
+        pr_strided_kahan(a.in_col, x, dsc.y + sub, dsc.y + dsc.z, (uint32_t)L, s, c);
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) {
@@ -120,7 +137,7 @@ __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint32_t
         T c2 = __shfl_xor(c, off, 64);
         kahan_merge(s, c, s2, c2);
     }
-    if (ok && sub == 0) pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
+    if (ok && sub == 0) pr_row_finish(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
 }
 
 // New rank of row v from its compensated predecessor sum (value s - c);
@@ -162,29 +179,31 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
             // one lane per row (<= ZG_PR_SMALL predecessors)
             const uint32_t i = b * ZG_PR_BLOCK + tid;
             if (i < a.n_small) {
-                const uint32_t v = a.small[i];
-                const uint32_t e0 = a.in_ptr[v], deg = a.in_ptr[v + 1] - e0;
+                const uint4 dsc = a.desc[i];
+                const uint32_t e0 = dsc.y, deg = dsc.z;
                 // all predecessor ids, then all gathers, in flight together
                 uint32_t col[ZG_PR_SMALL];
                 T val[ZG_PR_SMALL];
 #pragma unroll
                 for (int k = 0; k < ZG_PR_SMALL; k++) col[k] = (uint32_t)k < deg ? a.in_col[e0 + k] : 0u;
+                const T r_old = rold[dsc.x];
 #pragma unroll
                 for (int k = 0; k < ZG_PR_SMALL; k++) val[k] = (uint32_t)k < deg ? x[col[k]] : (T)0;
                 T s = 0, c = 0;
 #pragma unroll
                 for (int k = 0; k < ZG_PR_SMALL; k++)
                     if ((uint32_t)k < deg) kahan_add(s, c, val[k]);
-                pr_row_epilogue(a, rold, rnew, xnew, v, s, c, base, damping, l1, dang);
+                pr_row_finish(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
             }
         } else if (b < a.nb_small + a.nb_m4) {
-            pr_rows_lanes<T, 4>(a, a.med, a.n_m4, b - a.nb_small, x, rold, rnew, xnew, base, damping, l1, dang);
+            pr_rows_lanes<T, 4>(a, a.desc + a.n_small, a.n_m4, b - a.nb_small, x, rold, rnew, xnew, base, damping, l1,
+                                dang);
         } else if (b < a.nb_small + a.nb_m4 + a.nb_m16) {
-            pr_rows_lanes<T, 16>(a, a.med + a.n_m4, a.n_m16, b - a.nb_small - a.nb_m4, x, rold, rnew, xnew, base,
-                                 damping, l1, dang);
+            pr_rows_lanes<T, 16>(a, a.desc + a.n_small + a.n_m4, a.n_m16, b - a.nb_small - a.nb_m4, x, rold, rnew,
+                                 xnew, base, damping, l1, dang);
         } else if (b < a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64) {
-            pr_rows_lanes<T, 64>(a, a.med + a.n_m4 + a.n_m16, a.n_m64, b - a.nb_small - a.nb_m4 - a.nb_m16, x, rold,
-                                 rnew, xnew, base, damping, l1, dang);
+            pr_rows_lanes<T, 64>(a, a.desc + a.n_small + a.n_m4 + a.n_m16, a.n_m64, b - a.nb_small - a.nb_m4 - a.nb_m16,
+                                 x, rold, rnew, xnew, base, damping, l1, dang);
         } else {
             // one block per segment of a long row; pr_finish_kernel finishes the row
             const uint32_t sg = b - a.nb_small - a.nb_m4 - a.nb_m16 - a.nb_m64;
@@ -321,6 +340,10 @@ int zg_pr_ws_release(zg_graph *g)
 // Blocks of the iteration kernel: one per unit up to a fixed cap (the
 // kernel loops over units with a grid stride).
 #define ZG_PR_MAX_GRID 2048
+
+// The mixed mode leaves FP32 once the L1 change drops below this (the FP32
+// rounding floor of a probability vector is ~1e-7 in L1).
+#define ZG_PR_MIXED_SWITCH 1e-6
 static uint32_t pr_grid(const zg_graph *g)
 {
     uint32_t units = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK + (g->n_m4 + 63) / 64 + (g->n_m16 + 15) / 16 +
@@ -355,6 +378,7 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.out_ptr = g->out_ptr;
     a.small = g->pr_small;
     a.med = g->pr_med;
+    a.desc = g->pr_desc;
     a.large = g->pr_large;
     a.large_seg0 = g->pr_large_seg0;
     a.seg_e0 = g->pr_seg_e0;
@@ -411,9 +435,11 @@ static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st,
 static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
                             int precision, double *d_out, hipStream_t s, zg_pr_info *info)
 {
-    const bool check = !(precision & 0x100);
+    const bool check = !(precision & ZG_PR_SKIP_RESIDUAL);
     precision &= 0xFF;
-    if (precision != ZG_PR_FP32 && precision != ZG_PR_FP64) return ZG_ERR_INVALID_ARG;
+    if (precision != ZG_PR_FP32 && precision != ZG_PR_FP64 && precision != ZG_PR_MIXED) return ZG_ERR_INVALID_ARG;
+    const bool mixed = precision == ZG_PR_MIXED;
+    if (g->h_comp_off.size() > 2) return ZG_ERR_INVALID_ARG;  // one PageRank per resource
     const uint32_t V = g->V;
     const uint32_t n = g->n_live;
     zg_pr_info inf;
@@ -454,11 +480,14 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
     ZG_TRY(hipEventCreate(&e0));
     ZG_TRY(hipEventCreate(&e1));
     int rc = ZG_OK;
-    uint64_t iters = 0;
-    bool final_odd = false;
     do {
-        ZG_TRY(hipMemcpyAsync(d_st, &st0, sizeof(PrState), hipMemcpyHostToDevice, s));
-        if (precision == ZG_PR_FP32) {
+        // phase 1: FP64 for ZG_PR_FP64, FP32 otherwise (the mixed mode stops
+        // it at the FP32 noise floor and finishes in FP64)
+        const bool fp64 = precision == ZG_PR_FP64;
+        PrState st = st0;
+        if (mixed) st.tol = std::max(tolerance, ZG_PR_MIXED_SWITCH);
+        ZG_TRY(hipMemcpyAsync(d_st, &st, sizeof(PrState), hipMemcpyHostToDevice, s));
+        if (!fp64) {
             hipLaunchKernelGGL((pr_init_kernel<float, float>), vg, vb, 0, s, V, g->live, g->out_ptr,
                                (const float *)nullptr, inv_n, r32, r32 + V, x32, x32 + V);
         } else {
@@ -467,7 +496,7 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
         }
         ZG_TRY(hipGetLastError());
         ZG_TRY(hipEventRecord(e0, s));
-        if (precision == ZG_PR_FP32) {
+        if (!fp64) {
             PrArgs<float> a = pr_args<float>(g, r32, r32 + V, x32, x32 + V, (float *)g->pr.seg32, d_st);
             rc = pr_run<float>(g, s, a, h_st, max_iterations);
         } else {
@@ -475,19 +504,44 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             rc = pr_run<double>(g, s, a, h_st, max_iterations);
         }
         if (rc != ZG_OK) break;
+        uint64_t iters = h_st->iter;
+        // launches queued past convergence returned at once: not profiled
+        if (g->ev_marks.size() > iters) g->ev_marks.resize(iters);
+        ZG_TRY_RC(zg_prof_collect(g, s));
+        bool final64 = fp64;
+        if (mixed && !(h_st->last_l1 < tolerance) && iters < max_iterations) {
+            // phase 2: FP64 from the FP32 iterate; the buffers of parity
+            // (iters & 1) hold the start vector, the state carries on
+            final64 = true;
+            const bool odd = iters & 1;
+            const float *rs = odd ? r32 + V : r32;
+            double *ra = odd ? r64 + V : r64, *rb = odd ? r64 : r64 + V;
+            double *xa = odd ? x64 + V : x64, *xb = odd ? x64 : x64 + V;
+            hipLaunchKernelGGL((pr_init_kernel<double, float>), vg, vb, 0, s, V, g->live, g->out_ptr, rs, 0.0, ra,
+                               rb, xa, xb);
+            ZG_TRY(hipGetLastError());
+            PrState st2 = *h_st;
+            st2.tol = tolerance;
+            st2.done = 0;
+            ZG_TRY(hipMemcpyAsync(d_st, &st2, sizeof(PrState), hipMemcpyHostToDevice, s));
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st);
+            rc = pr_run<double>(g, s, a, h_st, max_iterations - iters);
+            if (rc != ZG_OK) break;
+            const uint64_t more = h_st->iter - iters;
+            if (g->ev_marks.size() > more) g->ev_marks.resize(more);
+            ZG_TRY_RC(zg_prof_collect(g, s));
+            inf.fp64_iterations = more;
+            iters = h_st->iter;
+        }
         ZG_TRY(hipEventRecord(e1, s));
         ZG_TRY(hipEventSynchronize(e1));
         float ms = 0.f;
         ZG_TRY(hipEventElapsedTime(&ms, e0, e1));
         inf.kernel_ms = ms;
-        iters = h_st->iter;
-        final_odd = iters & 1;
-        // launches queued past convergence returned at once: not profiled
-        if (g->ev_marks.size() > iters) g->ev_marks.resize(iters);
-        ZG_TRY_RC(zg_prof_collect(g, s));
+        const bool final_odd = iters & 1;
         inf.iterations = iters;
         inf.last_l1 = h_st->last_l1;
-        if (precision == ZG_PR_FP32)
+        if (!final64)
             hipLaunchKernelGGL(pr_out_kernel<float>, vg, vb, 0, s, V, final_odd ? r32 + V : r32, d_out);
         else
             hipLaunchKernelGGL(pr_out_kernel<double>, vg, vb, 0, s, V, final_odd ? r64 + V : r64, d_out);

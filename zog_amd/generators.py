@@ -3,8 +3,9 @@
 Every generator returns ``(n, src, dst)``: node count and u32 arc arrays in
 SoA insertion order (an undirected edge contributes (u, v) then (v, u), as
 ``Zog.SoA.add_edge`` does, lib/zog/soa.ex:110-128).  All randomness comes from
-numpy's PCG64 seeded explicitly, so a (generator, parameters, seed) triple
-always yields the same arrays.
+numpy's PCG64 seeded explicitly (the Barabasi-Albert generator, written in C
+in csrc/gen.c, uses xoshiro256** seeded by SplitMix64), so a (generator,
+parameters, seed) triple always yields the same arrays.
 
   * ``erdos_renyi_directed``: G(n, p) by the Batagelj-Brandes geometric-skip
     method over the n(n-1) ordered pairs, the method of
@@ -17,7 +18,8 @@ always yields the same arrays.
     removed uniformly without replacement.
   * ``barabasi_albert``: preferential attachment as in
     lib/zog/generator.ex:131-198 (clique of m, degree-weighted target list,
-    m distinct targets per new node, ascending).
+    m distinct targets per new node, ascending); ``ba_batch`` draws the
+    node counts of a batch log-uniformly.
 """
 from __future__ import annotations
 
@@ -61,11 +63,8 @@ def erdos_renyi_directed(n: int, p: float, seed: int):
 def rmat(scale: int, n_arcs: int, seed: int, a=0.57, b=0.19, c=0.19):
     rng = _rng(seed)
     n = 1 << scale
-    seen_keys = []
     n_unique = 0
-    uniq = np.zeros(0, dtype=np.uint64)
     first_pos = np.zeros(0, dtype=np.int64)
-    offset = 0
     all_keys = []
     while n_unique < n_arcs:
         M = max(int((n_arcs - n_unique) * 1.15), 1 << 16)
@@ -80,7 +79,6 @@ def rmat(scale: int, n_arcs: int, seed: int, a=0.57, b=0.19, c=0.19):
             dst |= dbit << sh
         keys = (src << np.uint64(32)) | dst
         all_keys.append(keys)
-        offset += M
         cat = np.concatenate(all_keys)
         uniq, first_pos = np.unique(cat, return_index=True)
         n_unique = len(uniq)
@@ -115,49 +113,45 @@ def grid_2d(rows: int, cols: int, seed: int = 0, delete: int = 0):
     return rows * cols, src, dst
 
 
+_gen_lib = None
+
+
+def _genlib():
+    global _gen_lib
+    if _gen_lib is None:
+        import ctypes
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzog_gen.so")
+        L = ctypes.CDLL(path)
+        L.zgen_ba_arcs.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.zgen_ba_arcs.restype = ctypes.c_uint64
+        L.zgen_barabasi_albert.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p,
+                                           ctypes.c_void_p]
+        L.zgen_barabasi_albert.restype = ctypes.c_uint64
+        _gen_lib = L
+    return _gen_lib
+
+
 def barabasi_albert(n: int, m: int, seed: int):
-    """Undirected BA graph (both arcs per edge)."""
+    """Undirected BA graph (both arcs per edge), generated natively."""
     if not (n > m >= 1):
         raise ValueError("need n > m >= 1")
-    rng = _rng(seed)
-    eu, ev = [], []
-    for uu in range(m - 1):
-        for vv in range(uu + 1, m):
-            eu.append(uu)
-            ev.append(vv)
-    cap = m * (m - 1) + (n - m) * 2 * m + 1
-    deg = np.empty(cap, dtype=np.int64)
-    if m > 1:
-        base = np.repeat(np.arange(m), m - 1)
-    else:
-        base = np.zeros(1, dtype=np.int64)
-    deg[: len(base)] = base
-    size = len(base)
-    # draw uniform positions in bulk; rejection of repeats follows the method
-    buf = rng.random(1 << 16)
-    bi = 0
-    for uu in range(m, n):
-        chosen = []
-        while len(chosen) < m:
-            if bi == len(buf):
-                buf = rng.random(1 << 16)
-                bi = 0
-            node = int(deg[int(buf[bi] * size)])
-            bi += 1
-            if node not in chosen:
-                chosen.append(node)
-        chosen.sort()
-        for vv in chosen:
-            eu.append(uu)
-            ev.append(vv)
-        add = [uu] + chosen + [uu] * (m - 1)
-        deg[size: size + len(add)] = add
-        size += len(add)
-    eu = np.asarray(eu, dtype=np.int64)
-    ev = np.asarray(ev, dtype=np.int64)
-    src = np.stack([eu, ev], axis=-1).reshape(-1).astype(np.uint32)
-    dst = np.stack([ev, eu], axis=-1).reshape(-1).astype(np.uint32)
+    L = _genlib()
+    k = int(L.zgen_ba_arcs(n, m))
+    src = np.empty(k, dtype=np.uint32)
+    dst = np.empty(k, dtype=np.uint32)
+    got = int(L.zgen_barabasi_albert(n, m, int(seed) & ((1 << 64) - 1), src.ctypes.data, dst.ctypes.data))
+    if got != k:
+        raise RuntimeError("barabasi_albert generation failed")
     return n, src, dst
+
+
+def ba_batch(n_graphs: int, m: int, lo: int, hi: int, seed: int):
+    """n_graphs BA graphs with node counts floor(exp(U(ln lo, ln hi))), seeded."""
+    rng = _rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), n_graphs))).astype(np.int64)
+    sizes = np.maximum(sizes, m + 1)
+    return [barabasi_albert(int(sz), m, seed * 1000003 + i) for i, sz in enumerate(sizes)]
 
 
 def seeded_sources(n: int, src: np.ndarray, dst: np.ndarray, k: int, seed: int) -> np.ndarray:

@@ -84,6 +84,47 @@ def _ptr(a) -> Optional[ctypes.c_void_p]:
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
+class BatchResource(Resource):
+    """One handle over several independent graphs (zg_graph_create_batch)."""
+
+    def __init__(self, graphs):
+        L = _lib.lib()
+        ns = np.ascontiguousarray([int(g[0]) for g in graphs], dtype=np.uint32)
+        ms = np.ascontiguousarray([len(g[1]) for g in graphs], dtype=np.uint64)
+        src = np.ascontiguousarray(np.concatenate([np.asarray(g[1], np.uint32) for g in graphs]) if graphs
+                                   else np.zeros(0, np.uint32), dtype=np.uint32)
+        dst = np.ascontiguousarray(np.concatenate([np.asarray(g[2], np.uint32) for g in graphs]) if graphs
+                                   else np.zeros(0, np.uint32), dtype=np.uint32)
+        h = ctypes.c_void_p()
+        rc = L.zg_graph_create_batch(len(graphs), _ptr(ns), _ptr(ms), _ptr(src), _ptr(dst), ctypes.byref(h))
+        _lib.check(rc, "zg_graph_create_batch")
+        self._h = h
+        self._lock = threading.Lock()
+        self.n_cap = int(ns.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
+
+
+def betweenness_unweighted_batch(graphs, info: Optional[dict] = None, groups: Optional[int] = None):
+    """Exact unweighted BC of every graph in `graphs` ((n, src, dst) triples
+    of local ids) in one batched sweep; returns one f64 array per graph
+    (directed sums, as betweenness_unweighted_array)."""
+    res = BatchResource(graphs)
+    try:
+        if groups:
+            rc = _lib.lib().zg_set_batch_groups(res.handle, int(groups))
+            if rc < 0:
+                _lib.check(rc, "zg_set_batch_groups")
+        out = np.zeros(res.n_cap, dtype=np.float64)
+        inf = _lib.ZgBcInfo()
+        rc = _lib.lib().zg_bc_unweighted(res.handle, None, 0, _ptr(out), ctypes.byref(inf))
+        _lib.check(rc, "zg_bc_unweighted")
+        if info is not None:
+            info.update({k: getattr(inf, k) for k, _ in _lib.ZgBcInfo._fields_})
+        return [out[res.offsets[i]: res.offsets[i + 1]] for i in range(len(graphs))]
+    finally:
+        res.release()
+
+
 class ResourceGraph(dict):
     """``%{resource: ref, builder: SoA}`` of the reference."""
 
@@ -293,6 +334,13 @@ def set_batch_groups(graph: ResourceGraph, groups: int) -> int:
     if rc < 0:
         _lib.check(rc, "zg_set_batch_groups")
     return rc
+
+
+def set_push_alpha(graph: ResourceGraph, alpha: float) -> None:
+    """Forward levels run push-style when frontier arcs * alpha < arcs
+    (0 = always pull)."""
+    _lib.check(_lib.lib().zg_set_tuning(graph.resource.handle, _lib.ZG_TUNE_PUSH_ALPHA, float(alpha)),
+               "zg_set_tuning")
 
 
 def set_source_order(graph: ResourceGraph, mode: int) -> int:
