@@ -163,6 +163,13 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   pull; a tiny positive value = push whenever the frontier is non-empty.
  * Returns 0 or <0. */
 #define ZG_TUNE_PUSH_ALPHA 1
+/*   ZG_TUNE_ENGINE: ZG_ENGINE_AUTO (default: the per-source queue engine for
+ *   graphs whose BFS from the hub core does not settle within 64 levels, the
+ *   64-lane engine otherwise), ZG_ENGINE_LANES or ZG_ENGINE_QUEUE. */
+#define ZG_TUNE_ENGINE 2
+#define ZG_ENGINE_AUTO 0
+#define ZG_ENGINE_LANES 1
+#define ZG_ENGINE_QUEUE 2
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the

@@ -252,6 +252,39 @@ def test_batch_of_uneven_graphs(groups):
         assert_bc_close(got, ref)
 
 
+@pytest.mark.parametrize("case", ["rmat", "multigraph", "hubs", "grid"])
+def test_queue_engine(case):
+    # the per-source queue engine against the oracle on the same inputs as the
+    # 64-lane engine: parallel arcs, self-loops, hubs, counts beyond 2^64
+    from oracle import oracle as O
+    rng = np.random.default_rng(31)
+    mode = O.SIGMA_U64
+    if case == "rmat":
+        n, src, dst = gen.rmat(12, 1 << 15, 4321)
+        sources = gen.seeded_sources(n, src, dst, 300, 3)
+    elif case == "multigraph":
+        n, src, dst = graphs.random_multigraph(rng, 400, 2500)
+        sources = None
+    elif case == "hubs":
+        n = 2000
+        src = np.concatenate([rng.integers(0, n, 3000), np.full(1200, 9), rng.integers(0, n, 5000)]).astype(np.uint32)
+        dst = np.concatenate([np.full(3000, 4), rng.integers(0, n, 1200), rng.integers(0, n, 5000)]).astype(np.uint32)
+        sources = np.arange(0, n, 7, dtype=np.uint32)
+    else:
+        n, src, dst = gen.grid_2d(200, 200, seed=5, delete=2000)
+        sources = gen.seeded_sources(n, src, dst, 70, 9)
+        mode = O.SIGMA_SCALED
+    og = O.Graph(n, src, dst)
+    ref, _ = og.betweenness(sources=sources, mode=mode)
+    g = make(n, src, dst)
+    rg.set_engine(g, "queue")
+    info = {}
+    got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
+    assert_bc_close(got, ref)
+    assert info["arcs_traversed"] > 0
+    rg.destroy(g)
+
+
 def test_depth_sigma_bit_exact():
     from oracle import oracle as O
     n, src, dst = gen.rmat(12, 1 << 15, 99)

@@ -27,6 +27,10 @@ ZG_PR_FP64 = 64
 ZG_PR_MIXED = 96
 ZG_PR_SKIP_RESIDUAL = 0x100
 ZG_TUNE_PUSH_ALPHA = 1
+ZG_TUNE_ENGINE = 2
+ZG_ENGINE_AUTO = 0
+ZG_ENGINE_LANES = 1
+ZG_ENGINE_QUEUE = 2
 
 # Every symbol include/zog_hip.h declares.
 EXPORTED = (

@@ -998,6 +998,7 @@ static int bc_core_distances(zg_graph *g, hipStream_t s)
         if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(&changed, d_changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
         if (!changed) break;
+        if (d + 1 == ZG_CORE_MAX_LEVELS) g->high_diameter = true;  // still growing at the cap
     }
     if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(dist.data(), d_dist, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, s));
     if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
@@ -1055,12 +1056,13 @@ static int zg_bc_device_impl(zg_graph *g, const uint32_t *d_sources, uint64_t n_
         for (uint32_t x : list)
             if (x >= V || !g->h_live[x]) return ZG_ERR_NODE_NOT_FOUND;
     }
+    // engine: the per-source queue engine for high-diameter graphs
+    if (g->engine == ZG_ENGINE_AUTO || (g->source_order && list.size() > 64)) ZG_TRY_RC(bc_core_distances(g, s));
+    const bool queue_engine =
+        g->engine == ZG_ENGINE_QUEUE || (g->engine == ZG_ENGINE_AUTO && g->high_diameter);
     uint32_t *d_tmp = nullptr;
     if (!d_sources || (g->source_order && list.size() > 64)) {
-        if (g->source_order && list.size() > 64) {
-            ZG_TRY_RC(bc_core_distances(g, s));
-            bc_order_sources(g, list);
-        }
+        if (g->source_order && list.size() > 64) bc_order_sources(g, list);
         ZG_TRY_RC(zg_dmalloc(&d_tmp, list.size()));
         if (!list.empty())
             ZG_TRY(hipMemcpyAsync(d_tmp, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, s));
@@ -1068,6 +1070,27 @@ static int zg_bc_device_impl(zg_graph *g, const uint32_t *d_sources, uint64_t n_
         d_sources = d_tmp;
     }
     n_sources = list.size();
+    if (queue_engine) {
+        hipEvent_t q0 = nullptr, q1 = nullptr;
+        int qrc = zg_hip_code(hipEventCreate(&q0));
+        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventCreate(&q1));
+        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventRecord(q0, s));
+        if (qrc == ZG_OK) qrc = zg_bcq_sweep(g, d_sources, n_sources, d_out, s, &inf);
+        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventRecord(q1, s));
+        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventSynchronize(q1));
+        if (qrc == ZG_OK) {
+            float ms = 0.f;
+            qrc = zg_hip_code(hipEventElapsedTime(&ms, q0, q1));
+            inf.kernel_ms = ms;
+        }
+        if (q0) (void)hipEventDestroy(q0);
+        if (q1) (void)hipEventDestroy(q1);
+        (void)hipStreamSynchronize(s);
+        if (qrc == ZG_OK) qrc = zg_prof_collect(g, s);
+        zg_dfree(d_tmp);
+        if (qrc == ZG_OK && info) *info = inf;
+        return qrc;
+    }
     int rc = bc_ws_ensure(g, false);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));

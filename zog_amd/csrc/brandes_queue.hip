@@ -1,0 +1,428 @@
+// Unweighted Brandes, per-source queue engine, for high-diameter graphs.
+//
+// Reference: priv/zog/src/centrality.zig:694-798.  The reference keeps, per
+// source, a BFS queue whose order is also the stack of the dependency sweep;
+// this engine keeps exactly that per source, for S sources at once:
+//   depth[s][v] (u32), sig[s][v] (FP64 stored path counts), cc[k][s][v]
+//   (FP64 (1+delta)/sigma, double-buffered by level parity), queue[s][i],
+//   lstart[L][s] (queue offset of level L), qlen[s].
+// One lane handles one (source, frontier vertex) pair; a wavefront always
+// holds pairs of a single source, so discoveries are appended to that
+// source's queue with one atomic per wavefront (ballot + prefix).
+//
+// Why a second engine: the 64-lane engine (brandes.hip) shares one row
+// access among the group's sources, which pays when they reach a vertex at
+// the same level (small-world graphs).  On a high-diameter graph with
+// scattered sources each vertex sits at ~64 different depths, so a row
+// access serves one lane; here every access serves its own pair.
+//
+// Forward level d: for v at depth d and each arc v->w: if depth[w] is unset,
+// CAS it to d+1 (the winner appends w); if depth[w] == d+1, atomically add
+// sigma'[v] * 2^-k_d to sigma[w] (exact for integer counts below 2^53).  A
+// close step over the new level clears c, records the level's largest
+// exponent (shift k_{d+1}) and the next level offset.  Backward level d:
+// delta(v) = sigma'(v) 2^-k_d sum_{w in succ(v)} c(w), with c(w) nonzero
+// exactly for BFS children (same argument as the 64-lane engine), then
+// bc[v] += delta(v) for v != s (FP64 atomics across sources).
+//
+// The level loop is device-driven: every kernel reads its per-source ranges
+// from lstart/qlen, so the host queues ZG_Q_LEVELS_PER_SYNC levels at a time
+// and syncs only to find the last level.
+#include "common.h"
+
+#include <algorithm>
+
+#define ZG_Q_MAX_SOURCES (64 * ZG_MAX_GROUPS)
+#define ZG_Q_GRID 1024
+#define ZG_Q_LEVELS_PER_SYNC 32
+
+struct BcqDev {
+    uint32_t V, S, lcap;
+    const uint32_t *out_ptr, *out_col;
+    const uint8_t *live;
+    uint32_t *depth;
+    double *sig, *cc;               // cc: [2][S][V]
+    uint32_t *queue;                // [S][V]
+    uint32_t *lstart;               // [lcap][S]
+    uint32_t *qlen;                 // [S]
+    int *kshift;                    // [lcap][S]
+    uint32_t *newcnt;               // [lcap] entries added at each level
+    const uint32_t *src;            // [S]
+    double *bc;                     // [V]
+    unsigned long long *stats;      // [2] arcs, reached
+};
+
+#define QROW(s) ((size_t)(s) * a.V)
+
+// Per-source ranges of this kernel's level into LDS, and the prefix of
+// 64-pair chunk counts; returns the total number of chunks.
+template <typename RangeFn>
+__device__ __forceinline__ uint32_t bcq_chunks(const BcqDev &a, RangeFn range, uint32_t *s_lo, uint32_t *s_hi,
+                                               uint32_t *s_pre)
+{
+    __shared__ uint32_t wtot[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < a.S; base += blockDim.x) {
+        const uint32_t s = base + tid;
+        uint32_t lo = 0, hi = 0;
+        if (s < a.S) range(s, lo, hi);
+        const uint32_t cnt = hi > lo ? (hi - lo + 63) / 64 : 0;
+        if (s < a.S) {
+            s_lo[s] = lo;
+            s_hi[s] = hi;
+        }
+        const uint32_t incl = zg_wave_incl_scan(cnt, lane);
+        if (lane == 63) wtot[wv] = incl;
+        __syncthreads();
+        uint32_t off = carry;
+        for (int w = 0; w < wv; w++) off += wtot[w];
+        if (s < a.S) s_pre[s] = off + incl - cnt;
+        uint32_t tot = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += wtot[w];
+        carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) s_pre[a.S] = carry;
+    __syncthreads();
+    return s_pre[a.S];
+}
+
+// Source owning chunk c: the last s with s_pre[s] <= c.
+__device__ __forceinline__ uint32_t bcq_owner(const uint32_t *s_pre, uint32_t S, uint32_t c)
+{
+    uint32_t lo = 0, hi = S;  // s_pre[lo] <= c < s_pre[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void bcq_init_kernel(BcqDev a)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.S) return;
+    const uint32_t v = a.src[s];
+    if (v == ZG_NONE) {
+        a.qlen[s] = 0;
+        a.lstart[s] = 0;
+        a.lstart[a.S + s] = 0;
+        return;
+    }
+    a.depth[QROW(s) + v] = 0;
+    a.sig[QROW(s) + v] = 1.0;
+    a.cc[QROW(s) + v] = 0.0;
+    a.cc[(size_t)a.S * a.V + QROW(s) + v] = 0.0;
+    a.queue[QROW(s)] = v;
+    a.qlen[s] = 1;
+    a.lstart[s] = 0;        // level 0 starts at 0
+    a.lstart[a.S + s] = 1;  // level 1 starts at 1
+    atomicAdd(&a.stats[0], (unsigned long long)(a.out_ptr[v + 1] - a.out_ptr[v]));
+    atomicAdd(&a.stats[1], 1ull);
+}
+
+__global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
+{
+    __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
+    const uint32_t total = bcq_chunks(
+        a,
+        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
+            lo = a.lstart[(size_t)d * a.S + s];
+            hi = a.lstart[(size_t)(d + 1) * a.S + s];
+        },
+        s_lo, s_hi, s_pre);
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+        const uint32_t s = bcq_owner(s_pre, a.S, c);
+        const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
+        const bool valid = idx < s_hi[s];
+        const uint32_t v = valid ? a.queue[QROW(s) + idx] : 0;
+        const double val = valid ? ldexp(a.sig[QROW(s) + v], -a.kshift[(size_t)d * a.S + s]) : 0.0;
+        const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
+        uint32_t maxdeg = deg;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor((int)maxdeg, off, 64));
+        uint32_t *dep = a.depth + QROW(s);
+        double *sg = a.sig + QROW(s);
+        for (uint32_t k = 0; k < maxdeg; k++) {
+            bool won = false;
+            uint32_t w = 0;
+            if (k < deg) {
+                w = a.out_col[e0 + k];
+                uint32_t dw = dep[w];
+                if (dw == ZG_NONE) {
+                    const uint32_t old = atomicCAS(&dep[w], ZG_NONE, (uint32_t)(d + 1));
+                    won = old == ZG_NONE;
+                    dw = won ? (uint32_t)(d + 1) : old;
+                }
+                if (dw == (uint32_t)(d + 1)) atomicAdd(&sg[w], val);
+            }
+            const uint64_t bw = __ballot(won);
+            if (bw) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&a.qlen[s], (uint32_t)__popcll(bw));
+                base = zg_readlane32(base, 0);
+                if (won) a.queue[QROW(s) + base + __popcll(bw & ((1ull << lane) - 1))] = w;
+            }
+        }
+    }
+}
+
+// Close level L (just appended): clear c, shift exponent, statistics, and
+// the offset of level L+1.
+__global__ __launch_bounds__(256) void bcq_close_kernel(BcqDev a, int L)
+{
+    __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
+    __shared__ unsigned long long red[2][4];
+    const uint32_t total = bcq_chunks(
+        a,
+        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
+            lo = a.lstart[(size_t)L * a.S + s];
+            hi = a.qlen[s];
+        },
+        s_lo, s_hi, s_pre);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (blockIdx.x == 0) {
+        uint32_t added = 0;
+        for (uint32_t s = tid; s < a.S; s += blockDim.x) {
+            a.lstart[(size_t)(L + 1) * a.S + s] = s_hi[s];
+            added += s_hi[s] - s_lo[s];
+        }
+        added = (uint32_t)zg_wave_sum_u64(added);
+        if (lane == 0) atomicAdd(&a.newcnt[L], added);
+    }
+    unsigned long long arcs = 0, reached = 0;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const size_t cc1 = (size_t)a.S * a.V;
+    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + wv; c < total; c += waves) {
+        const uint32_t s = bcq_owner(s_pre, a.S, c);
+        const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
+        int ex = -100000;
+        if (idx < s_hi[s]) {
+            const uint32_t w = a.queue[QROW(s) + idx];
+            a.cc[QROW(s) + w] = 0.0;
+            a.cc[cc1 + QROW(s) + w] = 0.0;
+            arcs += a.out_ptr[w + 1] - a.out_ptr[w];
+            reached += 1;
+            ex = ilogb(a.sig[QROW(s) + w]);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) ex = max(ex, __shfl_xor(ex, off, 64));
+        if (lane == 0 && ex > ZG_SHIFT_T) atomicMax(&a.kshift[(size_t)L * a.S + s], ex);
+    }
+    arcs = zg_wave_sum_u64(arcs);
+    reached = zg_wave_sum_u64(reached);
+    if (lane == 0) {
+        red[0][wv] = arcs;
+        red[1][wv] = reached;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long A = 0, R = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            A += red[0][w];
+            R += red[1][w];
+        }
+        if (A) atomicAdd(&a.stats[0], A);
+        if (R) atomicAdd(&a.stats[1], R);
+    }
+}
+
+__global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
+{
+    __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
+    const uint32_t total = bcq_chunks(
+        a,
+        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
+            lo = a.lstart[(size_t)d * a.S + s];
+            hi = a.lstart[(size_t)(d + 1) * a.S + s];
+        },
+        s_lo, s_hi, s_pre);
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const double *cin = a.cc + (size_t)((d + 1) & 1) * a.S * a.V;
+    double *cout = a.cc + (size_t)(d & 1) * a.S * a.V;
+    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+        const uint32_t s = bcq_owner(s_pre, a.S, c);
+        const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
+        if (idx >= s_hi[s]) continue;
+        const uint32_t v = a.queue[QROW(s) + idx];
+        double S = 0.0;
+        for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e++) S += cin[QROW(s) + a.out_col[e]];
+        const double sv = a.sig[QROW(s) + v];
+        const double delta = sv * ldexp(S, -a.kshift[(size_t)d * a.S + s]);
+        cout[QROW(s) + v] = (1.0 + delta) / sv;
+        if (v != a.src[s]) atomicAdd(&a.bc[v], delta);
+    }
+}
+
+__global__ void bcq_out_kernel(uint32_t V, const double *bc, const uint8_t *live, double *out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) out[v] = live[v] ? bc[v] : 0.0;
+}
+
+int zg_bcq_ws_release(zg_graph *g)
+{
+    zg_bcq_ws &w = g->bcq;
+    zg_dfree(w.depth); zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.queue); zg_dfree(w.lstart);
+    zg_dfree(w.qlen); zg_dfree(w.kshift); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.bc);
+    zg_dfree(w.stats);
+    w.S = 0;
+    w.lcap = 0;
+    return ZG_OK;
+}
+
+static int bcq_ws_ensure(zg_graph *g, uint32_t S)
+{
+    zg_bcq_ws &w = g->bcq;
+    if (w.S == S) return ZG_OK;
+    zg_bcq_ws_release(g);
+    const size_t V = g->V;
+    w.lcap = 1024;
+    int rc = ZG_OK;
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.depth, S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, 2 * S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.queue, S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.lstart, (size_t)w.lcap * S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.qlen, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.kshift, (size_t)w.lcap * S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.newcnt, w.lcap);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.src, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.bc, V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, 2);
+    if (rc != ZG_OK) {
+        zg_bcq_ws_release(g);
+        return rc;
+    }
+    w.S = S;
+    return ZG_OK;
+}
+
+// Grows the per-level tables to hold level `need` (prefix copied; level-major
+// layout keeps the used prefix contiguous).
+static int bcq_grow_levels(zg_graph *g, uint32_t need, uint32_t used, hipStream_t s)
+{
+    zg_bcq_ws &w = g->bcq;
+    if (need < w.lcap) return ZG_OK;
+    uint32_t nl = w.lcap;
+    while (nl <= need) nl *= 2;
+    uint32_t *ls = nullptr, *nc = nullptr;
+    int *ks = nullptr;
+    ZG_TRY_RC(zg_dmalloc(&ls, (size_t)nl * w.S));
+    int rc = zg_dmalloc(&ks, (size_t)nl * w.S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&nc, nl);
+    if (rc != ZG_OK) {
+        zg_dfree(ls);
+        zg_dfree(ks);
+        return rc;
+    }
+    ZG_TRY(hipMemsetAsync(ks, 0, sizeof(int) * (size_t)nl * w.S, s));
+    ZG_TRY(hipMemsetAsync(nc, 0, sizeof(uint32_t) * nl, s));
+    ZG_TRY(hipMemcpyAsync(ls, w.lstart, sizeof(uint32_t) * (size_t)std::min(used, w.lcap) * w.S,
+                          hipMemcpyDeviceToDevice, s));
+    ZG_TRY(hipMemcpyAsync(ks, w.kshift, sizeof(int) * (size_t)std::min(used, w.lcap) * w.S, hipMemcpyDeviceToDevice,
+                          s));
+    ZG_TRY(hipMemcpyAsync(nc, w.newcnt, sizeof(uint32_t) * std::min(used, w.lcap), hipMemcpyDeviceToDevice, s));
+    ZG_TRY(hipStreamSynchronize(s));
+    zg_dfree(w.lstart);
+    zg_dfree(w.kshift);
+    zg_dfree(w.newcnt);
+    w.lstart = ls;
+    w.kshift = ks;
+    w.newcnt = nc;
+    w.lcap = nl;
+    return ZG_OK;
+}
+
+static BcqDev bcq_dev(zg_graph *g)
+{
+    zg_bcq_ws &w = g->bcq;
+    BcqDev a;
+    a.V = g->V;
+    a.S = w.S;
+    a.lcap = w.lcap;
+    a.out_ptr = g->out_ptr;
+    a.out_col = g->out_col;
+    a.live = g->live;
+    a.depth = w.depth;
+    a.sig = w.sig;
+    a.cc = w.cc;
+    a.queue = w.queue;
+    a.lstart = w.lstart;
+    a.qlen = w.qlen;
+    a.kshift = w.kshift;
+    a.newcnt = w.newcnt;
+    a.src = w.src;
+    a.bc = w.bc;
+    a.stats = w.stats;
+    return a;
+}
+
+int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double *d_out, hipStream_t s, zg_bc_info *inf)
+{
+    const uint32_t V = g->V;
+    const uint32_t S = 64u * (uint32_t)g->groups;
+    ZG_TRY_RC(bcq_ws_ensure(g, S));
+    zg_bcq_ws &w = g->bcq;
+    ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
+    std::vector<uint32_t> newcnt;
+    for (uint64_t b0 = 0; b0 < n; b0 += S) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(S, n - b0);
+        ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * S, s));
+        ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
+        ZG_TRY(hipMemsetAsync(w.depth, 0xFF, sizeof(uint32_t) * (size_t)S * V, s));
+        ZG_TRY(hipMemsetAsync(w.sig, 0, sizeof(double) * (size_t)S * V, s));
+        ZG_TRY(hipMemsetAsync(w.kshift, 0, sizeof(int) * (size_t)w.lcap * S, s));
+        ZG_TRY(hipMemsetAsync(w.newcnt, 0, sizeof(uint32_t) * w.lcap, s));
+        ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 2, s));
+        BcqDev a = bcq_dev(g);
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bcq_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, a);
+        ZG_TRY(hipGetLastError());
+        // forward: queue levels in chunks, sync to find the last one
+        uint32_t d = 0;
+        int maxd = -1;
+        while (maxd < 0) {
+            ZG_TRY_RC(bcq_grow_levels(g, d + ZG_Q_LEVELS_PER_SYNC + 2, d + 2, s));
+            a = bcq_dev(g);
+            for (uint32_t k = 0; k < ZG_Q_LEVELS_PER_SYNC; k++) {
+                ZG_PROF_LAUNCH(ZG_PROF_BC_PUSH, bcq_forward_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a, (int)(d + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bcq_close_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a,
+                               (int)(d + k + 1));
+            }
+            ZG_TRY(hipGetLastError());
+            newcnt.resize(ZG_Q_LEVELS_PER_SYNC);
+            ZG_TRY(hipMemcpyAsync(newcnt.data(), w.newcnt + d + 1, sizeof(uint32_t) * ZG_Q_LEVELS_PER_SYNC,
+                                  hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+            for (uint32_t k = 0; k < ZG_Q_LEVELS_PER_SYNC; k++)
+                if (newcnt[k] == 0) {
+                    maxd = (int)(d + k);  // level d+k+1 is empty
+                    break;
+                }
+            d += ZG_Q_LEVELS_PER_SYNC;
+            inf->push_levels += ZG_Q_LEVELS_PER_SYNC;
+        }
+        for (int L = maxd; L >= 1; L--)
+            ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, bcq_backward_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a, L);
+        ZG_TRY(hipGetLastError());
+        unsigned long long st[2];
+        ZG_TRY(hipMemcpyAsync(st, w.stats, sizeof(st), hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+        inf->arcs_traversed += st[0];
+        inf->nodes_reached += st[1];
+        inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)maxd + 1);
+        inf->batches++;
+    }
+    if (V) {
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bcq_out_kernel, dim3((V + 255) / 256), dim3(256), 0, s, V, w.bc, g->live,
+                       d_out);
+        ZG_TRY(hipGetLastError());
+    }
+    inf->n_sources = n;
+    return ZG_OK;
+}

@@ -114,6 +114,22 @@ struct zg_bc_ws {
     uint32_t last_levels = 0;            // levels used by the previous batch
 };
 
+// Per-source queue engine workspace (brandes_queue.hip), S sources per batch.
+struct zg_bcq_ws {
+    uint32_t S = 0, lcap = 0;
+    uint32_t *depth = nullptr;           // [S][V]
+    double *sig = nullptr;               // [S][V]
+    double *cc = nullptr;                // [2][S][V]
+    uint32_t *queue = nullptr;           // [S][V]
+    uint32_t *lstart = nullptr;          // [lcap][S]
+    uint32_t *qlen = nullptr;            // [S]
+    int *kshift = nullptr;               // [lcap][S]
+    uint32_t *newcnt = nullptr;          // [lcap]
+    uint32_t *src = nullptr;             // [S]
+    double *bc = nullptr;                // [V]
+    unsigned long long *stats = nullptr; // [2]
+};
+
 // PageRank workspace: double-buffered ranks and gathered values.
 struct zg_pr_ws {
     int prec = 0;              // 32 or 64
@@ -166,7 +182,10 @@ struct zg_graph {
     uint32_t *pr_seg_row = nullptr;      // [n_seg] large-row index of each segment
     uint32_t *pr_seg_e0 = nullptr, *pr_seg_e1 = nullptr;
     int groups = 4;
+    int engine = 0;                      // ZG_ENGINE_*
+    bool high_diameter = false;          // set with h_coredist
     zg_bc_ws bc;
+    zg_bcq_ws bcq;
     zg_pr_ws pr;
     // per-kernel profiling (events around launches, summed on read)
     bool profiling = false;
@@ -192,7 +211,9 @@ void zg_prof_release(zg_graph *g);
     } while (0)
 
 int zg_bc_ws_release(zg_graph *g);
+int zg_bcq_ws_release(zg_graph *g);
 int zg_pr_ws_release(zg_graph *g);
+int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double *d_out, hipStream_t s, zg_bc_info *inf);
 
 // ---------------------------------------------------------------------------
 // Device helpers

@@ -97,6 +97,7 @@ static void zg_free_device(zg_graph *g)
 {
     zg_prof_release(g);
     zg_bc_ws_release(g);
+    zg_bcq_ws_release(g);
     zg_pr_ws_release(g);
     zg_dfree(g->out_ptr); zg_dfree(g->out_col); zg_dfree(g->in_ptr); zg_dfree(g->in_col);
     zg_dfree(g->live);
@@ -392,6 +393,10 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
         if (!(value >= 0.0)) return ZG_ERR_INVALID_ARG;
         g->push_alpha = value;
         return ZG_OK;
+    case ZG_TUNE_ENGINE:
+        if (value != ZG_ENGINE_AUTO && value != ZG_ENGINE_LANES && value != ZG_ENGINE_QUEUE) return ZG_ERR_INVALID_ARG;
+        g->engine = (int)value;
+        return ZG_OK;
     default: return ZG_ERR_INVALID_ARG;
     }
 }
@@ -415,6 +420,7 @@ extern "C" int zg_set_batch_groups(zg_graph *g, int groups)
     if (groups != g->groups) {
         (void)hipSetDevice(g->device);
         zg_bc_ws_release(g);
+        zg_bcq_ws_release(g);
         g->groups = groups;
     }
     return prev;

@@ -343,6 +343,13 @@ def set_push_alpha(graph: ResourceGraph, alpha: float) -> None:
                "zg_set_tuning")
 
 
+def set_engine(graph: ResourceGraph, engine: str) -> None:
+    """'auto' (default), 'lanes' (64 sources per wavefront) or 'queue'
+    (per-source queues, for high-diameter graphs)."""
+    code = {"auto": _lib.ZG_ENGINE_AUTO, "lanes": _lib.ZG_ENGINE_LANES, "queue": _lib.ZG_ENGINE_QUEUE}[engine]
+    _lib.check(_lib.lib().zg_set_tuning(graph.resource.handle, _lib.ZG_TUNE_ENGINE, float(code)), "zg_set_tuning")
+
+
 def set_source_order(graph: ResourceGraph, mode: int) -> int:
     """0 = sweep sources in the given order, 1 = group them by distance to the
     hub core (default).  Returns the previous mode."""
