@@ -48,6 +48,10 @@ def workload_spec(name):
         return dict(gen=W.er10k, sources=None, pr=dict(tol=1e-6, max_iter=100, precision=32), cpu_sources=None,
                     desc="er10k: directed Erdos-Renyi n=10,000 p=8/(n-1), exact Brandes over all sources; "
                          "PageRank FP32-Kahan tol 1e-6 max 100")
+    if name == "ba512":
+        return dict(gen=None, sources=None, pr=None, cpu_sources=8,
+                    desc="ba512: 512 undirected Barabasi-Albert graphs (m=4), node counts log-uniform in "
+                         "[1,000, 100,000], seeded; exact all-sources Brandes for every graph in one batched resource")
     if name == "grid2048":
         return dict(gen=W.grid2048, sources=lambda n, s, d: W.grid2048_sources(n, s, d, 1024),
                     pr=dict(tol=1e-8, max_iter=1000, precision=96), cpu_sources=2, scaled=True,
@@ -56,13 +60,23 @@ def workload_spec(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def apply_tuning(g, args):
+    from zog_amd import resource_graph as rg
+    if args.pull_grid is not None:
+        rg.set_pull_grid(g, args.pull_grid)
+    if args.alpha is not None:
+        rg.set_push_alpha(g, args.alpha)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048"])
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512"])
     ap.add_argument("--groups", type=int, default=4, help="64-source groups per launch")
+    ap.add_argument("--pull-grid", type=int, default=None, help="pull workgroups per group (0 = one wave per chunk)")
+    ap.add_argument("--alpha", type=float, default=None, help="push/pull switch factor")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -78,6 +92,8 @@ def main():
     from zog_amd.soa import SoA
 
     spec = workload_spec(args.workload)
+    if args.workload == "ba512":
+        return bench_batch(args, spec)
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     t0 = time.time()
@@ -87,6 +103,7 @@ def main():
     t0 = time.time()
     g = rg.new(SoA.from_arrays("directed", n, src, dst))
     rg.set_batch_groups(g, args.groups)
+    apply_tuning(g, args)
     V, n_live, E = g.resource.info()
     log(f"[bench] resource built in {time.time() - t0:.1f}s")
     L = _lib.lib()
@@ -245,6 +262,123 @@ def main():
     }
     print(json.dumps(line), flush=True)
     rg.destroy(g)
+
+
+def bench_batch(args, spec):
+    """Batched uneven graphs: one resource, all sources of every graph."""
+    import numpy as np
+    import torch
+
+    from zog_amd import _lib
+    from zog_amd import generators as gen
+    from zog_amd import resource_graph as rg
+    from zog_amd import workloads as W
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    graphs = gen.ba_batch(512, 4, 1000, 100000, W.SEEDS["ba512"])
+    n_tot = sum(g[0] for g in graphs)
+    arcs_tot = sum(len(g[1]) for g in graphs)
+    log(f"[bench] ba512 generated in {time.time() - t0:.1f}s: {len(graphs)} graphs, {n_tot} nodes, {arcs_tot} arcs")
+    t0 = time.time()
+    res = rg.BatchResource(graphs)
+    L = _lib.lib()
+    h = res.handle
+    rc = L.zg_set_batch_groups(h, args.groups)
+    if rc < 0:
+        _lib.check(rc, "zg_set_batch_groups")
+    apply_tuning(rg.ResourceGraph(resource=res, builder=None), args)
+    log(f"[bench] batch resource built in {time.time() - t0:.1f}s")
+    d_bc = torch.empty(n_tot, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    info = _lib.ZgBcInfo()
+
+    def bc_step():
+        rc = L.zg_bc_unweighted_device(h, None, 0, ctypes.c_void_p(d_bc.data_ptr()),
+                                       ctypes.c_void_p(stream.cuda_stream), ctypes.byref(info))
+        _lib.check(rc, "zg_bc_unweighted_device")
+
+    for i in range(args.warmup):
+        bc_step()
+        log(f"[bench] warmup {i} done")
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    arcs = reached = 0
+    for i in range(args.steps):
+        bc_step()
+        arcs += info.arcs_traversed
+        reached += info.nodes_reached
+        log(f"[bench] step {i} done ({info.batches} batches, {info.max_levels} levels max)")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    gteps = arcs / wall / 1e9
+    arcs_step = arcs // args.steps
+    nodes_step = reached // args.steps
+    L.zg_set_profiling(h, 1)
+    bc_step()
+    p = _lib.ZgProfile()
+    L.zg_get_profile(h, ctypes.byref(p))
+    L.zg_set_profiling(h, 0)
+    prof = {name: (float(p.ms[i]), int(p.launches[i])) for i, name in enumerate(_lib.PROF_SLOTS)}
+    log(f"[bench] profile {prof}")
+    fwd_ms = prof["bc_forward"][0] + prof["bc_push"][0]
+    fwd_n = prof["bc_forward"][1] + prof["bc_push"][1]
+    bwd_ms, bwd_n = prof["bc_backward"]
+    fwd_bytes = 16 * arcs_step + 24 * nodes_step
+    bwd_bytes = 24 * arcs_step + 40 * nodes_step
+    if fwd_ms >= bwd_ms:
+        dom, dom_bytes, dom_ms, dom_n = "bc_forward_kernel+bc_push_kernel", fwd_bytes, fwd_ms, fwd_n
+    else:
+        dom, dom_bytes, dom_ms, dom_n = "bc_backward_kernel", bwd_bytes, bwd_ms, bwd_n
+    achieved = dom_bytes / (dom_ms / 1000.0) / 1e9
+    out = d_bc.cpu().numpy()
+    parity = {}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        offs = res.offsets
+        small = sorted(range(len(graphs)), key=lambda i: graphs[i][0])[:3]
+        worst = 0.0
+        for i in small:
+            n, s, dd = graphs[i]
+            ref, _ = O.Graph(n, s, dd).betweenness()
+            got = out[offs[i]: offs[i + 1]]
+            scale = max(1.0, float(np.abs(ref).max()))
+            worst = max(worst, float((np.abs(got - ref) / np.maximum(np.abs(ref), 1e-9 * scale)).max()))
+        parity["bc_max_rel_err_3_smallest_graphs_all_sources"] = worst
+        rng = np.random.default_rng(5)
+        pick = rng.choice(len(graphs), 16, replace=False)
+        t_cpu = 0.0
+        arcs_cpu = 0
+        for i in pick:
+            n, s, dd = graphs[i]
+            og = O.Graph(n, s, dd)
+            srcs = np.arange(0, n, max(1, n // spec["cpu_sources"]), dtype=np.uint32)[: spec["cpu_sources"]]
+            t1 = time.perf_counter()
+            _, _, a_s = og.betweenness(sources=srcs, return_arcs=True)
+            t_cpu += time.perf_counter() - t1
+            arcs_cpu += int(a_s.sum())
+        cpu = {"value": round(arcs_cpu / t_cpu / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": "oracle/oracle.c Brandes, 1 thread, 8 sources in each of 16 random graphs of the batch"}
+    line = {
+        "metric": "Brandes GTEPS (exact BC, batched graphs)",
+        "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1000.0 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": spec["desc"], "graphs": len(graphs), "nodes": n_tot, "arcs": arcs_tot,
+                   "sources": n_tot, "sources_per_launch": 64 * args.groups, "batches": int(info.batches),
+                   "arcs_per_step": arcs_step, "levels": int(info.max_levels),
+                   "push_levels": int(info.push_levels), "pull_levels": int(info.pull_levels)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "launches": dom_n,
+                     "avg_launch_us": round(dom_ms * 1000.0 / max(dom_n, 1), 2),
+                     "kernel_ms_per_step": {k: round(v[0], 3) for k, v in prof.items() if v[1]}},
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    res.release()
 
 
 if __name__ == "__main__":

@@ -151,7 +151,7 @@ int zg_pagerank_device(zg_graph *g, double damping, uint64_t max_iterations, dou
 int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sources, uint32_t *depth_out,
                 double *sigma_out, int32_t *shift_out, uint32_t max_levels, uint32_t *levels_out);
 
-/* Sets the number of 64-source groups swept per batch (1..16, default 4 =
+/* Sets the number of 64-source groups swept per batch (1..64, default 4 =
  * 256 sources per launch).  Returns the previous value, or <0. */
 int zg_set_batch_groups(zg_graph *g, int groups);
 
@@ -170,6 +170,10 @@ int zg_set_batch_groups(zg_graph *g, int groups);
 #define ZG_ENGINE_AUTO 0
 #define ZG_ENGINE_LANES 1
 #define ZG_ENGINE_QUEUE 2
+/*   ZG_TUNE_PULL_GRID: workgroups per 64-source group for a pull level (they
+ *   stride over the predecessor chunks); 0 = one wavefront per chunk.
+ *   Default 4096. */
+#define ZG_TUNE_PULL_GRID 3
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the
@@ -205,6 +209,63 @@ int zg_set_profiling(zg_graph *g, int on);
 /* Copies the profile of the most recent call(s) since profiling was turned
  * on or last read, then clears it. */
 int zg_get_profile(zg_graph *g, zg_profile *out);
+
+/* ---- Sampled Brandes and learned pivot selection -------------------------
+ * None of the entry points below has a counterpart in the reference: they
+ * serve sampled betweenness (weighted sources) and the node-scoring network
+ * that proposes the sampled sources. */
+
+/* Weighted multi-source Brandes: out_bc[v] = sum_i weights[i] *
+ * delta_{sources[i]}(v) over v != sources[i] (weights NULL = all 1, as
+ * zg_bc_unweighted; repeated sources count repeatedly), and the per-source
+ * dependency totals out_totals[i] = sum_v delta_{sources[i]}(v) =
+ * sum over t reachable from sources[i], t != sources[i], of (dist - 1)
+ * (every shortest s-t path has dist - 1 interior vertices).  sources NULL =
+ * every live vertex in id order (weights must then be NULL).  Either output
+ * may be NULL, not both.  Host buffers; ZG_ERR_NODE_NOT_FOUND as
+ * zg_bc_unweighted, ZG_ERR_INVALID_ARG for a non-finite weight. */
+int zg_bc_weighted(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources,
+                   double *out_bc, double *out_totals, zg_bc_info *info);
+
+/* Core numbers of the graph taken as undirected and simple (self-loops
+ * dropped, each neighbour once), out[v] for v < n_cap (0 for tombstones),
+ * as coreNumbers in priv/zog/src/connectivity.zig:5-101. */
+int zg_core_numbers(zg_graph *g, uint32_t *out);
+
+/* Node features, ZG_NFEAT floats per node into d_features[n_cap][8] (device
+ * memory): log1p(in+out degree), log1p(in degree), log1p(out degree),
+ * log1p(sum of the successors' out-degrees), log(n * PageRank) (damping
+ * 0.85, tol 1e-6, FP32), log1p(core number), each standardised over the live
+ * nodes; column 6 = 1, column 7 = 0 (rows of tombstones are 0).  One graph
+ * per handle (ZG_ERR_INVALID_ARG on a batch handle of several graphs). */
+#define ZG_NFEAT 8
+int zg_node_features(zg_graph *g, float *d_features, void *stream);
+
+/* Node-scoring network ZG_NFEAT -> 128 -> 128 -> 1 with ReLU hidden layers,
+ * BF16 operands on the matrix cores and FP32 accumulation.  Parameters are
+ * ZG_MLP_PARAMS floats: W1[8][128], b1[128], W2[128][128], b2[128],
+ * w3[128], b3 (x @ W convention).  The model lives on the device that is
+ * current at zg_mlp_create. */
+#define ZG_MLP_HIDDEN 128
+#define ZG_MLP_PARAMS 17793
+#define ZG_MLP_MAX_BATCH 65536
+typedef struct zg_mlp zg_mlp;
+int zg_mlp_create(uint64_t seed, zg_mlp **out);
+void zg_mlp_destroy(zg_mlp *m);
+int zg_mlp_get_params(zg_mlp *m, float *out);
+/* Sets the parameters and resets the Adam state. */
+int zg_mlp_set_params(zg_mlp *m, const float *in);
+/* d_y[i] = network(d_x[i]) for n rows of d_x[n][ZG_NFEAT] (device memory);
+ * returns when d_y is written. */
+int zg_mlp_forward(zg_mlp *m, const float *d_x, uint64_t n, float *d_y, void *stream);
+/* Gradient of the mean squared error (1/n) sum (y_i - target_i)^2 over a
+ * batch of n <= ZG_MLP_MAX_BATCH rows (device memory), kept for
+ * zg_mlp_adam; optionally copied to grad_out[ZG_MLP_PARAMS] and the loss to
+ * *loss_out (host memory).  Returns when done, as zg_mlp_adam. */
+int zg_mlp_gradient(zg_mlp *m, const float *d_x, const float *d_target, uint64_t n, float *grad_out,
+                    double *loss_out, void *stream);
+/* One Adam step with the last gradient (bias-corrected moments). */
+int zg_mlp_adam(zg_mlp *m, float lr, float beta1, float beta2, float eps, void *stream);
 
 /* Human-readable name of a status code. */
 const char *zg_strerror(int code);

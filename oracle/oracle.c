@@ -28,6 +28,8 @@
  *     is taken,
  *   - per-source export of BFS depths and path counts,
  *   - an N-thread source-chunked BC (the pattern of centrality.zig:216-256).
+ *   - core numbers, coreNumbers in priv/zog/src/connectivity.zig:5-101
+ *     (input to the learned pivot scorer's features).
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -537,5 +539,124 @@ int zo_pagerank(const zo_graph *g, double damping, uint64_t max_iterations, doub
     if (iters_out) *iters_out = it;
     if (l1_out) *l1_out = l1_norm;
     free(nodes); free(in_ptr); free(out_deg); free(in_col); free(fill); free(ranks); free(new_ranks);
+    return ZO_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Core numbers (connectivity.zig:5-101)                                      */
+/* ------------------------------------------------------------------------- */
+
+/* out[n_cap]: core number of every live node of the graph taken as
+ * undirected and simple (0 for non-live ids).  As the reference: the
+ * neighbour lists are built from each live node's successors (self-loops
+ * skipped, a pair added once, :27-45), then bucket peeling in increasing
+ * degree, a node's core number being the bucket it is popped from and each
+ * unprocessed neighbour's degree dropping by one into bucket max(deg, i)
+ * (:47-97).  A per-node mark replaces the reference's linear scan for a
+ * repeated neighbour; the lists and the peeling order are the same. */
+static int cmp_u64(const void *a, const void *b)
+{
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+int zo_core_numbers(const zo_graph *g, uint32_t *out)
+{
+    const uint32_t V = g->n_cap;
+    for (uint32_t i = 0; i < V; i++) out[i] = 0;
+    if (V == 0) return ZO_OK;
+    uint32_t *deg = calloc(V, sizeof(uint32_t));
+    uint64_t *cnt = calloc((size_t)V + 1, sizeof(uint64_t));
+    if (!deg || !cnt) {
+        free(deg); free(cnt);
+        return ZO_ERR_OOM;
+    }
+    /* pass 1: collect the undirected simple pairs (u < v keys, sorted) */
+    uint64_t m2 = 0;
+    for (uint32_t u = 0; u < V; u++) {
+        if (g->node_deleted[u]) continue;
+        uint32_t cur = g->first_edge[u], t;
+        while ((t = zo_next_succ(g, &cur)) != ZO_NONE) m2 += t != u;
+    }
+    uint64_t *keys = malloc(sizeof(uint64_t) * (m2 ? m2 : 1));
+    if (!keys) {
+        free(deg); free(cnt);
+        return ZO_ERR_OOM;
+    }
+    uint64_t k = 0;
+    for (uint32_t u = 0; u < V; u++) {
+        if (g->node_deleted[u]) continue;
+        uint32_t cur = g->first_edge[u], t;
+        while ((t = zo_next_succ(g, &cur)) != ZO_NONE)
+            if (t != u) keys[k++] = u < t ? ((uint64_t)u << 32 | t) : ((uint64_t)t << 32 | u);
+    }
+    /* sort + unique (insertion order of the lists does not change core numbers) */
+    qsort(keys, m2, sizeof(uint64_t), cmp_u64);
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < m2; i++)
+        if (m == 0 || keys[i] != keys[m - 1]) keys[m++] = keys[i];
+    for (uint64_t i = 0; i < m; i++) {
+        cnt[(keys[i] >> 32) + 1]++;
+        cnt[(uint32_t)keys[i] + 1]++;
+    }
+    for (uint32_t i = 0; i < V; i++) cnt[i + 1] += cnt[i];
+    uint32_t *adj = malloc(sizeof(uint32_t) * (m ? 2 * m : 1));
+    uint64_t *fill = malloc(sizeof(uint64_t) * V);
+    if (!adj || !fill) {
+        free(deg); free(cnt); free(keys); free(adj); free(fill);
+        return ZO_ERR_OOM;
+    }
+    memcpy(fill, cnt, sizeof(uint64_t) * V);
+    for (uint64_t i = 0; i < m; i++) {
+        const uint32_t u = (uint32_t)(keys[i] >> 32), v = (uint32_t)keys[i];
+        adj[fill[u]++] = v;
+        adj[fill[v]++] = u;
+    }
+    free(keys);
+    free(fill);
+    uint32_t max_deg = 0;
+    for (uint32_t v = 0; v < V; v++) {
+        deg[v] = (uint32_t)(cnt[v + 1] - cnt[v]);
+        if (deg[v] > max_deg) max_deg = deg[v];
+    }
+    /* buckets as stacks in one array with per-bucket linked storage: every
+     * push is an entry (node, next) so repeated pushes of a node are kept */
+    uint64_t cap = (uint64_t)V + 2 * m + 1, used = 0;
+    uint32_t *e_node = malloc(sizeof(uint32_t) * cap);
+    uint64_t *e_next = malloc(sizeof(uint64_t) * cap);
+    uint64_t *head = malloc(sizeof(uint64_t) * ((size_t)max_deg + 1));
+    uint8_t *done = calloc(V, 1);
+    if (!e_node || !e_next || !head || !done) {
+        free(deg); free(cnt); free(adj); free(e_node); free(e_next); free(head); free(done);
+        return ZO_ERR_OOM;
+    }
+    const uint64_t NIL = ~(uint64_t)0;
+    for (uint32_t i = 0; i <= max_deg; i++) head[i] = NIL;
+    for (uint32_t v = 0; v < V; v++) {
+        if (g->node_deleted[v]) continue;
+        e_node[used] = v;
+        e_next[used] = head[deg[v]];
+        head[deg[v]] = used++;
+    }
+    for (uint32_t i = 0; i <= max_deg; i++) {
+        while (head[i] != NIL) {
+            const uint64_t e = head[i];
+            head[i] = e_next[e];
+            const uint32_t u = e_node[e];
+            if (done[u]) continue;
+            out[u] = i;
+            done[u] = 1;
+            for (uint64_t a = cnt[u]; a < cnt[u + 1]; a++) {
+                const uint32_t v = adj[a];
+                if (done[v]) continue;
+                if (deg[v] > 0) deg[v]--;
+                const uint32_t b = deg[v] > i ? deg[v] : i;
+                e_node[used] = v;
+                e_next[used] = head[b];
+                head[b] = used++;
+            }
+        }
+    }
+    free(deg); free(cnt); free(adj); free(e_node); free(e_next); free(head); free(done);
     return ZO_OK;
 }

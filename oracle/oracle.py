@@ -52,6 +52,8 @@ def lib():
         L.zo_bfs_paths.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P]
         L.zo_bfs_paths.restype = ctypes.c_int
         L.zo_pagerank.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P, P, P]
+        L.zo_core_numbers.argtypes = [P, P]
+        L.zo_core_numbers.restype = ctypes.c_int
         L.zo_pagerank.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -142,3 +144,11 @@ class Graph:
         if rc != 0:
             raise OracleError(f"zo_pagerank failed: {rc}")
         return out, int(it.value), float(l1.value)
+
+    def core_numbers(self) -> np.ndarray:
+        """u32[n_cap] core numbers of the undirected simple view (0 for non-live)."""
+        out = np.zeros(self.n_cap, dtype=np.uint32)
+        rc = lib().zo_core_numbers(self._h, _ptr(out))
+        if rc != 0:
+            raise OracleError(f"zo_core_numbers failed: {rc}")
+        return out

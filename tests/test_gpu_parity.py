@@ -217,11 +217,13 @@ def test_push_pull_directions(alpha):
     sources = gen.seeded_sources(n, src, dst, 200, 8)
     og = O.Graph(n, src, dst)
     ref, _ = og.betweenness(sources=sources)
+    _, _, arcs = og.betweenness(sources=sources, return_arcs=True)
     g = make(n, src, dst)
     rg.set_push_alpha(g, alpha)
     info = {}
     got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
     assert_bc_close(got, ref)
+    assert info["arcs_traversed"] == int(arcs.sum())
     if alpha == 0.0:
         assert info["push_levels"] == 0
     if alpha == 1e-9:
@@ -235,7 +237,7 @@ def test_push_pull_directions(alpha):
     rg.destroy(g)
 
 
-@pytest.mark.parametrize("groups", [1, 4, 16])
+@pytest.mark.parametrize("groups", [1, 4, 16, 64])
 def test_batch_of_uneven_graphs(groups):
     # one resource holding graphs of uneven size: every graph's BC equals the
     # oracle on that graph alone
@@ -250,6 +252,40 @@ def test_batch_of_uneven_graphs(groups):
     for (n, src, dst), got in zip(graphs_, outs):
         ref, _ = O.Graph(n, src, dst).betweenness()
         assert_bc_close(got, ref)
+
+
+def test_batch_groups_spanning_several_graphs():
+    # one handle over several graphs; sources of one graph, then sources
+    # scattered over all graphs (a group spans several graphs and the
+    # per-group state grows), then depths and path counts of scattered groups
+    from oracle import oracle as O
+    graphs_ = [gen.barabasi_albert(n, 4, 300 + i) for i, n in enumerate([200, 1500, 60, 800, 5])]
+    res = rg.BatchResource(graphs_)
+    off = res.offsets
+    n = int(off[-1])
+    src = np.concatenate([g[1] + np.uint32(o) for g, o in zip(graphs_, off[:-1])])
+    dst = np.concatenate([g[2] + np.uint32(o) for g, o in zip(graphs_, off[:-1])])
+    og = O.Graph(n, src, dst)
+    g = rg.ResourceGraph(resource=res, builder=SoA.from_arrays("directed", n, src, dst))
+    rng = np.random.default_rng(5)
+    one = np.arange(off[1], off[2], dtype=np.uint32)
+    scattered = rng.permutation(n)[:700].astype(np.uint32)
+    for order in (1, 0):
+        rg.set_source_order(g, order)
+        for sources in (one, scattered, one):
+            ref, _, arcs = og.betweenness(sources=sources, return_arcs=True)
+            info = {}
+            got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
+            assert_bc_close(got, ref)
+            assert info["arcs_traversed"] == int(arcs.sum())
+    pick = scattered[:70]
+    depth, sigma, _, _ = rg.bc_paths(g, pick, max_levels=64)
+    for i, s in enumerate(pick.tolist()):
+        d_ref, su, *_ = og.bfs_paths(s)
+        assert np.array_equal(depth[i], d_ref)
+        r = d_ref != O.NONE_U32
+        assert np.array_equal(sigma[i][r], su[r].astype(np.float64))
+    res.release()
 
 
 @pytest.mark.parametrize("case", ["rmat", "multigraph", "hubs", "grid"])
@@ -281,7 +317,8 @@ def test_queue_engine(case):
     info = {}
     got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
     assert_bc_close(got, ref)
-    assert info["arcs_traversed"] > 0
+    _, _, arcs = og.betweenness(sources=sources, mode=mode, return_arcs=True)
+    assert info["arcs_traversed"] == int(arcs.sum())
     rg.destroy(g)
 
 

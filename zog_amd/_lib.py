@@ -31,6 +31,7 @@ ZG_TUNE_ENGINE = 2
 ZG_ENGINE_AUTO = 0
 ZG_ENGINE_LANES = 1
 ZG_ENGINE_QUEUE = 2
+ZG_TUNE_PULL_GRID = 3
 
 # Every symbol include/zog_hip.h declares.
 EXPORTED = (
@@ -50,7 +51,22 @@ EXPORTED = (
     "zg_set_profiling",
     "zg_get_profile",
     "zg_strerror",
+    "zg_bc_weighted",
+    "zg_core_numbers",
+    "zg_node_features",
+    "zg_mlp_create",
+    "zg_mlp_destroy",
+    "zg_mlp_get_params",
+    "zg_mlp_set_params",
+    "zg_mlp_forward",
+    "zg_mlp_gradient",
+    "zg_mlp_adam",
 )
+
+ZG_NFEAT = 8
+ZG_MLP_HIDDEN = 128
+ZG_MLP_PARAMS = 17793
+ZG_MLP_MAX_BATCH = 65536
 
 PROF_SLOTS = ("bc_forward", "bc_finalize", "bc_backward", "bc_hub", "bc_setup", "pr_iter", "bc_push")
 
@@ -101,6 +117,22 @@ class NodeNotFoundError(ZogError):
 _lib = None
 
 
+def _init_torch_first():
+    """PyTorch-ROCm ships its own HIP runtime next to the one this library
+    links (/opt/rocm).  Both sit on the same HSA runtime, so device pointers
+    pass between them, but torch's runtime finds no GPU if it initialises
+    after ours; when torch is present it is initialised first.  Work does
+    not order across the two runtimes' queues: the Python wrappers
+    synchronise at every hand-over, and only stream 0 (the null stream) is
+    passed from torch to this library."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def lib():
     """Load libzog_hip.so; raises OSError if it has not been built."""
     global _lib
@@ -108,6 +140,7 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise OSError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    _init_torch_first()
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     L.zg_graph_create.argtypes = [ctypes.c_uint32, P, P, P, ctypes.c_uint64, ctypes.POINTER(P)]
@@ -136,6 +169,26 @@ def lib():
     L.zg_set_batch_groups.restype = ctypes.c_int
     L.zg_set_tuning.argtypes = [P, ctypes.c_int, ctypes.c_double]
     L.zg_set_tuning.restype = ctypes.c_int
+    L.zg_bc_weighted.argtypes = [P, P, P, ctypes.c_uint64, P, P, ctypes.POINTER(ZgBcInfo)]
+    L.zg_bc_weighted.restype = ctypes.c_int
+    L.zg_core_numbers.argtypes = [P, P]
+    L.zg_core_numbers.restype = ctypes.c_int
+    L.zg_node_features.argtypes = [P, P, P]
+    L.zg_node_features.restype = ctypes.c_int
+    L.zg_mlp_create.argtypes = [ctypes.c_uint64, ctypes.POINTER(P)]
+    L.zg_mlp_create.restype = ctypes.c_int
+    L.zg_mlp_destroy.argtypes = [P]
+    L.zg_mlp_destroy.restype = None
+    L.zg_mlp_get_params.argtypes = [P, P]
+    L.zg_mlp_get_params.restype = ctypes.c_int
+    L.zg_mlp_set_params.argtypes = [P, P]
+    L.zg_mlp_set_params.restype = ctypes.c_int
+    L.zg_mlp_forward.argtypes = [P, P, ctypes.c_uint64, P, P]
+    L.zg_mlp_forward.restype = ctypes.c_int
+    L.zg_mlp_gradient.argtypes = [P, P, P, ctypes.c_uint64, P, P, P]
+    L.zg_mlp_gradient.restype = ctypes.c_int
+    L.zg_mlp_adam.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, P]
+    L.zg_mlp_adam.restype = ctypes.c_int
     L.zg_set_source_order.argtypes = [P, ctypes.c_int]
     L.zg_set_source_order.restype = ctypes.c_int
     L.zg_set_profiling.argtypes = [P, ctypes.c_int]

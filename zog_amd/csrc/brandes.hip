@@ -42,6 +42,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <cmath>
 
 struct GroupRange {
     uint32_t off[ZG_MAX_GROUPS];
@@ -58,6 +59,8 @@ struct GroupSpan {
 
 struct BcDev {
     uint32_t V;
+    uint32_t Vloc;              // per-group capacity: state of vertex v of group g sits at v - vlo[g]
+    uint32_t vlo[ZG_MAX_GROUPS];
     uint32_t G;                 // groups in this batch (gridDim.y)
     uint32_t n_ich;
     const uint32_t *in_ptr, *in_col, *out_ptr, *out_col;
@@ -76,24 +79,42 @@ struct BcDev {
     uint32_t *hlist;
     uint64_t *hmask;
     int *emax;                  // [L][Gcap][64]
+    uint32_t *foff, *hoff;      // [L][Gcap] level offsets in flist / hlist
+    uint32_t *modes;            // [L][Gcap]
     const uint32_t *src;        // [Gcap][64]
+    const double *wts;          // [Gcap][64] source weights, or null (all 1)
+    unsigned long long *dsum;   // [Gcap][64] dependency totals, or null (not wanted)
+    double push_alpha;
+    unsigned long long Eg[ZG_MAX_GROUPS];  // arcs inside each group's span
     zg_bc_ws::Counters *cnt;
     size_t fcap, hcap;
     uint32_t Gcap, n_ohub;
 };
 
-#define ROW(g, v) (((size_t)(g) * a.V + (v)) * 64)
-#define MASK(g, v) ((size_t)(g) * a.V + (v))
+// Direction of a group's level.
+#define ZG_MODE_IDLE 0u    // the level is empty: the group's sweep is done
+#define ZG_MODE_PULL 1u
+#define ZG_MODE_PUSH 2u
+
+// Levels queued per host sync: the first chunk, doubling up to the cap.
+#define ZG_LEVEL_CHUNK0 12
+#define ZG_LEVEL_CHUNK_MAX 64
+// Fixed grid of the push kernel (it strides over the level's entries).
+#define ZG_PUSH_GRID 1024
+
+// Per-group state is indexed by the vertex's position in the group's span.
+#define MASK(g, v) ((size_t)(g) * a.Vloc + ((v) - a.vlo[g]))
+#define ROW(g, v) (MASK(g, v) * 64)
 #define EMAX(L, g, l) ((((size_t)(L)) * a.Gcap + (g)) * 64 + (l))
 
 __device__ __forceinline__ uint64_t *fr_buf(const BcDev &a, int k)
 {
-    return a.fr + (size_t)k * a.Gcap * a.V;
+    return a.fr + (size_t)k * a.Gcap * a.Vloc;
 }
 
 __device__ __forceinline__ double *cc_buf(const BcDev &a, int k)
 {
-    return a.cc + (size_t)k * a.Gcap * a.V * 64;
+    return a.cc + (size_t)k * a.Gcap * a.Vloc * 64;
 }
 
 // Batch reset over a group's vertex span: the visited mask starts as the
@@ -158,25 +179,42 @@ __global__ void bc_source_kernel(BcDev a)
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
     if (l == 0) {
-        a.cnt[g].farcs = arcs;
+        a.cnt[g].farcs[0] = arcs;
         // slot 0 belongs to finalize block 0, which runs strictly later
         a.stats[(size_t)g * ZG_FIN_BLOCKS * 2] += arcs;
         a.stats[(size_t)g * ZG_FIN_BLOCKS * 2 + 1] += reached;
     }
 }
 
-// Forward level d: discover level d+1 by pulling over predecessor chunks.
-__global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSpan sp)
+// Opens level d for every group: records where the level's work lists end
+// (everything finalize(d-1) appended), picks the direction from the level's
+// successor-arc count, and clears the counters level d accumulates into.
+__global__ void bc_level_kernel(BcDev a, int d)
 {
-    const uint32_t g = blockIdx.y;
-    const int lane = threadIdx.x & 63;
-    const uint32_t ch = sp.clo[g] + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (ch >= sp.chi[g]) return;
+    const uint32_t g = threadIdx.x;
+    if (g >= a.G) return;
+    zg_bc_ws::Counters &c = a.cnt[g];
+    const uint32_t fc = c.fcount;
+    a.foff[(size_t)(d + 1) * a.Gcap + g] = fc;
+    a.hoff[(size_t)(d + 1) * a.Gcap + g] = c.hcount;
+    const uint32_t start = a.foff[(size_t)d * a.Gcap + g];
+    const unsigned long long fa = c.farcs[d & 1];
+    uint32_t mode = ZG_MODE_IDLE;
+    if (fc > start)
+        mode = (a.push_alpha > 0.0 && (double)fa * a.push_alpha < (double)a.Eg[g]) ? ZG_MODE_PUSH : ZG_MODE_PULL;
+    c.mode = mode;
+    a.modes[(size_t)d * a.Gcap + g] = mode;
+    c.tcount = 0;
+    c.farcs[(d + 1) & 1] = 0;
+}
+
+// One predecessor chunk of the pull at level d (discovers level d+1).
+__device__ __forceinline__ void bc_pull_chunk(const BcDev &a, uint32_t g, int lane, int d, uint32_t ch)
+{
     const uint32_t v = a.ich_v[ch], e0 = a.ich_e0[ch], e1 = a.ich_e1[ch];
     const uint64_t unvis = ~a.vis[MASK(g, v)];
     if (unvis == 0) return;
-    const uint64_t *cur = fr_buf(a, d % 3) + (size_t)g * a.V;
-    const double *sg = a.sig + ROW(g, 0);
+    const uint64_t *cur = fr_buf(a, d % 3);
     double acc = 0.0;
     uint64_t m = 0;
     for (uint32_t base = e0; base < e1; base += 64) {
@@ -185,7 +223,7 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSp
         uint64_t f = 0;
         if (e < e1) {
             u = a.in_col[e];
-            f = cur[u] & unvis;
+            f = cur[MASK(g, u)] & unvis;
         }
         uint64_t act = __ballot(f != 0);
         // contributing predecessors, eight row reads in flight per round
@@ -207,7 +245,7 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSp
             }
             double val[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) val[q] = ((fj[q] >> lane) & 1) ? sg[(size_t)uj[q] * 64 + lane] : 0.0;
+            for (int q = 0; q < 8; q++) val[q] = ((fj[q] >> lane) & 1) ? a.sig[ROW(g, uj[q]) + lane] : 0.0;
 #pragma unroll
             for (int q = 0; q < 8; q++) acc += val[q];
         }
@@ -237,13 +275,26 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSp
     }
 }
 
+
+// Forward level d: discover level d+1 by pulling over predecessor chunks (a
+// fixed grid striding over the group's chunks).
+__global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSpan sp)
+{
+    const uint32_t g = blockIdx.y;
+    if (a.cnt[g].mode != ZG_MODE_PULL) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t nch = sp.chi[g] - sp.clo[g];
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < nch; i += gridDim.x * 4)
+        bc_pull_chunk(a, g, lane, d, sp.clo[g] + i);
+}
+
 __device__ __forceinline__ void push_flush_touched(const BcDev &a, uint32_t g, int lane, uint32_t tbuf, int tn)
 {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&a.cnt[g].tcount, (uint32_t)tn);
     base = zg_readlane32(base, 0);
     if (lane < tn) {
-        if (base + lane < a.V) a.touched[(size_t)g * a.V + base + lane] = tbuf;
+        if (base + lane < a.Vloc) a.touched[(size_t)g * a.Vloc + base + lane] = tbuf;
         else atomicOr(&a.cnt[g].err, ZG_BC_ERR_CAPACITY);
     }
 }
@@ -251,29 +302,30 @@ __device__ __forceinline__ void push_flush_touched(const BcDev &a, uint32_t g, i
 // Forward level d, push over the level-d work list (sparse frontiers): each
 // entry's lanes add their count into every successor not yet visited by that
 // lane (FP64 atomics; exact for integer counts below 2^53).
-__global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d, GroupRange R)
+__global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d)
 {
     const uint32_t g = blockIdx.y;
+    if (a.cnt[g].mode != ZG_MODE_PUSH) return;
     const int lane = threadIdx.x & 63;
-    const uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wid >= R.cnt[g]) return;
-    const size_t idx = g * a.fcap + R.off[g] + wid;
+    const uint32_t off = a.foff[(size_t)d * a.Gcap + g];
+    const uint32_t cnt = a.foff[(size_t)(d + 1) * a.Gcap + g] - off;
+    uint64_t *next = fr_buf(a, (d + 1) % 3);
+    uint32_t tbuf = 0;  // lane k holds the k-th vertex this wavefront reached first
+    int tn = 0;
+    for (uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6); wid < cnt; wid += gridDim.x * 4) {
+    const size_t idx = g * a.fcap + off + wid;
     const uint32_t oc = a.flist[idx];
     const uint64_t mask = a.fmask[idx];
     const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
     const bool mine = (mask >> lane) & 1;
     const double val = mine ? ldexp(a.sig[ROW(g, v) + lane], -a.emax[EMAX(d, g, lane)]) : 0.0;
-    const uint64_t *visg = a.vis + (size_t)g * a.V;
-    uint64_t *next = fr_buf(a, (d + 1) % 3) + (size_t)g * a.V;
-    uint32_t tbuf = 0;  // lane k holds the k-th vertex this wavefront reached first
-    int tn = 0;
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
         uint32_t w = 0;
         uint64_t m = 0;
         if (e < e1) {
             w = a.out_col[e];
-            m = mask & ~visg[w];
+            m = mask & ~a.vis[MASK(g, w)];
         }
         uint64_t act = __ballot(m != 0);
         while (act) {
@@ -289,7 +341,7 @@ __global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d, GroupRange
                 if (a.dep) a.dep[r] = (uint32_t)(d + 1);
             }
             unsigned long long old = 0;
-            if (lane == 0) old = atomicOr((unsigned long long *)&next[wj], (unsigned long long)mj);
+            if (lane == 0) old = atomicOr((unsigned long long *)&next[MASK(g, wj)], (unsigned long long)mj);
             if (zg_readlane64(old, 0) == 0) {
                 // first to reach wj at this level: record it for the sparse finalize
                 if (lane == tn) tbuf = wj;
@@ -300,7 +352,39 @@ __global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d, GroupRange
             }
         }
     }
+    }
     if (tn) push_flush_touched(a, g, lane, tbuf, tn);
+}
+
+// Per-lane counts of the vertices a finalize block adds to level d+1; each
+// adds d to its lane's dependency total sum_t (dist(s,t) - 1).
+__device__ __forceinline__ void bc_count_lanes(uint32_t *lcnt, uint64_t nm)
+{
+    while (nm) {
+        const int l = __builtin_ctzll(nm);
+        nm &= nm - 1;
+        atomicAdd(&lcnt[l], 1u);
+    }
+}
+
+__device__ __forceinline__ void bc_flush_lanes(const BcDev &a, uint32_t g, int d, const uint32_t *lcnt)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < 64 && lcnt[t] && d > 0)
+        atomicAdd(&a.dsum[g * 64 + t], (unsigned long long)lcnt[t] * (unsigned long long)d);
+}
+
+// Clears the frontier buffer of level d-1 (it becomes level d+2's) for the
+// vertices of level d-1's work list.
+__device__ __forceinline__ void bc_clear_stale(const BcDev &a, int d, uint32_t g, uint32_t i0, uint32_t step)
+{
+    if (d < 1) return;
+    const uint32_t off = a.foff[(size_t)(d - 1) * a.Gcap + g];
+    const uint32_t n = a.foff[(size_t)d * a.Gcap + g] - off;
+    for (uint32_t i = i0; i < n; i += step) {
+        const uint32_t oc = a.flist[g * a.fcap + off + i];
+        fr_buf(a, (d + 2) % 3)[MASK(g, a.och_v[oc])] = 0;
+    }
 }
 
 // Exclusive prefix sum over a 1024-thread block (16 wavefronts); returns the
@@ -328,21 +412,23 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t x, uint32_t *l
 // lists, clear the frontier buffer of level d-1.  Block b owns predecessor
 // chunks [b*R, (b+1)*R); a vertex is handled by the block holding its first
 // chunk.
-__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, int d, GroupRange stale, int pushed,
-                                                                   GroupSpan sp)
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, int d, int pushed, GroupSpan sp)
 {
     __shared__ uint32_t lds[40];
     __shared__ uint32_t base[2];
     __shared__ unsigned long long sred[3][ZG_FIN_THREADS / 64];
     const uint32_t g = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+    if (a.cnt[g].mode != ZG_MODE_PULL) return;
     const int lane = tid & 63;
-    for (uint32_t i = b * ZG_FIN_THREADS + tid; i < stale.cnt[g]; i += ZG_FIN_BLOCKS * ZG_FIN_THREADS) {
-        uint32_t oc = a.flist[g * a.fcap + stale.off[g] + i];
-        fr_buf(a, (d + 2) % 3)[MASK(g, a.och_v[oc])] = 0;
+    bc_clear_stale(a, d, g, b * ZG_FIN_THREADS + tid, gridDim.x * ZG_FIN_THREADS);
+    const uint64_t *next = fr_buf(a, (d + 1) % 3);
+    __shared__ uint32_t lcnt[64];
+    if (a.dsum) {
+        if (tid < 64) lcnt[tid] = 0;
+        __syncthreads();
     }
-    const uint64_t *next = fr_buf(a, (d + 1) % 3) + (size_t)g * a.V;
     const uint32_t nch = sp.chi[g] - sp.clo[g];
-    const uint32_t R = (nch + ZG_FIN_BLOCKS - 1) / ZG_FIN_BLOCKS;
+    const uint32_t R = (nch + gridDim.x - 1) / gridDim.x;
     const uint32_t c0 = sp.clo[g] + min(b * R, nch), c1 = sp.clo[g] + min(b * R + R, nch);
 
     // pass A: fold masks, count list entries, statistics
@@ -351,10 +437,11 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
     for (uint32_t c = c0 + tid; c < c1; c += ZG_FIN_THREADS) {
         const uint32_t v = a.ich_v[c];
         if (a.ich_e0[c] != a.in_ptr[v]) continue;
-        const uint64_t nm = next[v];
+        const uint64_t nm = next[MASK(g, v)];
         if (!nm) continue;
         a.vis[MASK(g, v)] |= nm;
         a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);  // levels only grow
+        if (a.dsum) bc_count_lanes(lcnt, nm);
         const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
         nf += kf;
         nh += kf > 1 ? 1 : 0;
@@ -390,6 +477,7 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
         sred[2][tid >> 6] = farcs;
     }
     __syncthreads();
+    if (a.dsum) bc_flush_lanes(a, g, d, lcnt);
     if (tid == 0) {
         unsigned long long A = 0, Rr = 0, F = 0;
         for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
@@ -400,7 +488,7 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
         unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * 2;
         st[0] += A;
         st[1] += Rr;
-        if (F) atomicAdd(&a.cnt[g].farcs, F);
+        if (F) atomicAdd(&a.cnt[g].farcs[(d + 1) & 1], F);
     }
     if (tf == 0 && th == 0) return;
 
@@ -413,7 +501,7 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
         if (c < c1) {
             v = a.ich_v[c];
             if (a.ich_e0[c] == a.in_ptr[v]) {
-                nm = next[v];
+                nm = next[MASK(g, v)];
                 if (nm) {
                     kf = a.och_first[v + 1] - a.och_first[v];
                     kh = kf > 1 ? 1 : 0;
@@ -451,30 +539,34 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
 // Finalize level d after a push level: same as bc_finalize_kernel, over the
 // vertices the push reached first (a fixed grid striding over the list).
 #define ZG_SPARSE_FIN_BLOCKS 64
-__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDev a, int d, GroupRange stale)
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDev a, int d)
 {
     __shared__ uint32_t lds[40];
     __shared__ uint32_t base[2];
     __shared__ unsigned long long sred[3][ZG_FIN_THREADS / 64];
     const uint32_t g = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+    if (a.cnt[g].mode != ZG_MODE_PUSH) return;
     const int lane = tid & 63;
-    for (uint32_t i = b * ZG_FIN_THREADS + tid; i < stale.cnt[g]; i += ZG_SPARSE_FIN_BLOCKS * ZG_FIN_THREADS) {
-        uint32_t oc = a.flist[g * a.fcap + stale.off[g] + i];
-        fr_buf(a, (d + 2) % 3)[MASK(g, a.och_v[oc])] = 0;
+    bc_clear_stale(a, d, g, b * ZG_FIN_THREADS + tid, gridDim.x * ZG_FIN_THREADS);
+    const uint64_t *next = fr_buf(a, (d + 1) % 3);
+    const uint32_t *tl = a.touched + (size_t)g * a.Vloc;
+    const uint32_t nt = min(a.cnt[g].tcount, a.Vloc);
+    __shared__ uint32_t lcnt[64];
+    if (a.dsum) {
+        if (tid < 64) lcnt[tid] = 0;
+        __syncthreads();
     }
-    const uint64_t *next = fr_buf(a, (d + 1) % 3) + (size_t)g * a.V;
-    const uint32_t *tl = a.touched + (size_t)g * a.V;
-    const uint32_t nt = min(a.cnt[g].tcount, a.V);
-    const uint32_t R = (nt + ZG_SPARSE_FIN_BLOCKS - 1) / ZG_SPARSE_FIN_BLOCKS;
+    const uint32_t R = (nt + gridDim.x - 1) / gridDim.x;
     const uint32_t t0 = min(b * R, nt), t1 = min(t0 + R, nt);
 
     uint32_t nf = 0, nh = 0;
     unsigned long long arcs = 0, reached = 0, farcs = 0;
     for (uint32_t t = t0 + tid; t < t1; t += ZG_FIN_THREADS) {
         const uint32_t v = tl[t];
-        const uint64_t nm = next[v];
+        const uint64_t nm = next[MASK(g, v)];
         a.vis[MASK(g, v)] |= nm;
         a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);
+        if (a.dsum) bc_count_lanes(lcnt, nm);
         const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
         nf += kf;
         nh += kf > 1 ? 1 : 0;
@@ -507,6 +599,7 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDe
         sred[2][tid >> 6] = farcs;
     }
     __syncthreads();
+    if (a.dsum) bc_flush_lanes(a, g, d, lcnt);
     if (tid == 0) {
         unsigned long long A = 0, Rr = 0, F = 0;
         for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
@@ -517,7 +610,7 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDe
         unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * 2;
         st[0] += A;
         st[1] += Rr;
-        if (F) atomicAdd(&a.cnt[g].farcs, F);
+        if (F) atomicAdd(&a.cnt[g].farcs[(d + 1) & 1], F);
     }
     if (tf == 0 && th == 0) return;
     uint32_t fpos = base[0], hpos = base[1];
@@ -527,7 +620,7 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDe
         uint64_t nm = 0;
         if (t < t1) {
             v = tl[t];
-            nm = next[v];
+            nm = next[MASK(g, v)];
             kf = a.och_first[v + 1] - a.och_first[v];
             kh = kf > 1 ? 1 : 0;
         }
@@ -571,7 +664,7 @@ __device__ __forceinline__ void bc_finish(const BcDev &a, uint32_t g, uint32_t v
         const int k = a.emax[EMAX(d, g, lane)];
         const double delta = sv * ldexp(S, -k);
         cc_buf(a, d & 1)[r] = (1.0 + delta) / sv;
-        if (a.src[g * 64 + lane] != v) contrib = delta;
+        if (a.src[g * 64 + lane] != v) contrib = a.wts ? delta * a.wts[g * 64 + lane] : delta;
     }
     contrib = zg_wave_sum(contrib);
     if (lane == 0) a.bcg[(size_t)g * a.V + v] += contrib;
@@ -589,10 +682,9 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     const uint64_t mask = a.fmask[idx];
     const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
     const bool active = (mask >> lane) & 1;
-    const double *cg = cc_buf(a, (d + 1) & 1) + ROW(g, 0) + lane;
+    const double *cg = cc_buf(a, (d + 1) & 1);
     // Only successors that some lane reached at depth >= d+1 can be BFS
     // children; the others' rows are not read.
-    const uint32_t *md = a.maxdep + (size_t)g * a.V;
     double S = 0.0;
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
@@ -600,7 +692,7 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
         bool keep = false;
         if (e < e1) {
             w = a.out_col[e];
-            keep = md[w] >= (uint32_t)(d + 1);
+            keep = a.maxdep[MASK(g, w)] >= (uint32_t)(d + 1);
         }
         uint64_t act = __ballot(keep);
         // eight row reads in flight per round
@@ -620,7 +712,7 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
             }
             double val[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) val[q] = (active && okq[q]) ? cg[(size_t)wq[q] * 64] : 0.0;
+            for (int q = 0; q < 8; q++) val[q] = (active && okq[q]) ? cg[ROW(g, wq[q]) + lane] : 0.0;
 #pragma unroll
             for (int q = 0; q < 8; q++) S += val[q];
         }
@@ -663,17 +755,23 @@ __global__ void bc_reduce_kernel(uint32_t V, uint32_t G, const double *bcg, cons
     out[v] = live[v] ? s : 0.0;
 }
 
-// Depth / sigma export of one group: [lane][v] planes.
-__global__ void bc_export_kernel(uint32_t V, uint32_t g, const uint32_t *dep, const double *sig,
-                                 uint32_t *dep_out, double *sig_out)
+// Depth / sigma export of one group: [lane][v] planes; vertices outside the
+// group's span [vlo, vhi) are unreached.
+__global__ void bc_export_kernel(uint32_t V, uint32_t Vloc, uint32_t vlo, uint32_t vhi, uint32_t g,
+                                 const uint32_t *dep, const double *sig, uint32_t *dep_out, double *sig_out)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t l = blockIdx.y;
     if (v >= V) return;
-    size_t r = ((size_t)g * V + v) * 64 + l;
-    uint32_t dd = dep[r];
+    uint32_t dd = ZG_NONE;
+    double sv = 0.0;
+    if (v >= vlo && v < vhi) {
+        const size_t r = ((size_t)g * Vloc + (v - vlo)) * 64 + l;
+        dd = dep[r];
+        sv = dd == ZG_NONE ? 0.0 : sig[r];
+    }
     dep_out[(size_t)l * V + v] = dd;
-    sig_out[(size_t)l * V + v] = dd == ZG_NONE ? 0.0 : sig[r];
+    sig_out[(size_t)l * V + v] = sv;
 }
 
 int zg_bc_ws_release(zg_graph *g)
@@ -681,41 +779,86 @@ int zg_bc_ws_release(zg_graph *g)
     zg_bc_ws &w = g->bc;
     zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep); zg_dfree(w.maxdep);
     zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.touched); zg_dfree(w.flist); zg_dfree(w.fmask);
-    zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.src); zg_dfree(w.cnt);
+    zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.foff); zg_dfree(w.hoff); zg_dfree(w.modes); zg_dfree(w.src); zg_dfree(w.cnt);
+    zg_dfree(w.wts); zg_dfree(w.dsum);
     if (w.h_cnt) (void)hipHostFree(w.h_cnt);
     w.h_cnt = nullptr;
     w.groups = 0;
     w.lcap = 0;
+    w.vloc = 0;
+    w.fcap = w.hcap = 0;
     w.with_depth = false;
     return ZG_OK;
 }
 
-static int bc_ws_ensure(zg_graph *g, bool with_depth)
+// Vertex span of a group: the components its sources belong to (components
+// are contiguous id ranges; a single graph is one component).
+static void bc_group_span(const zg_graph *g, const uint32_t *src, uint32_t n, uint32_t *vlo, uint32_t *vhi)
+{
+    const std::vector<uint32_t> &co = g->h_comp_off;
+    if (co.size() <= 2 || n == 0) {
+        *vlo = 0;
+        *vhi = g->V;
+        return;
+    }
+    uint32_t lo = ZG_NONE, hi = 0;
+    for (uint32_t l = 0; l < n; l++) {
+        lo = std::min(lo, src[l]);
+        hi = std::max(hi, src[l]);
+    }
+    const size_t c_lo = std::upper_bound(co.begin(), co.end(), lo) - co.begin() - 1;
+    const size_t c_hi = std::upper_bound(co.begin(), co.end(), hi) - co.begin() - 1;
+    *vlo = co[c_lo];
+    *vhi = co[c_hi + 1];
+}
+
+// Sizes the lane-engine state for the sweep over h_sources[0..n): each group
+// holds its span's vertices (Vloc of them) and list entries for its span's
+// successor chunks and multi-chunk vertices.
+static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources, uint64_t n)
 {
     zg_bc_ws &w = g->bc;
-    if (w.groups == g->groups && (w.with_depth || !with_depth)) return ZG_OK;
+    uint32_t vloc = 1, och = 0, hubs = 0;
+    for (uint64_t b0 = 0; b0 < n; b0 += 64) {
+        uint32_t lo, hi;
+        bc_group_span(g, h_sources + b0, (uint32_t)std::min<uint64_t>(64, n - b0), &lo, &hi);
+        vloc = std::max(vloc, hi - lo);
+        och = std::max(och, g->h_och_first[hi] - g->h_och_first[lo]);
+        hubs = std::max(hubs, g->h_hub_first[hi] - g->h_hub_first[lo]);
+    }
+    const size_t need_f = 64 * (size_t)och + 64 * (size_t)g->max_och_per_v;
+    const size_t need_h = 64 * (size_t)hubs + 64;
+    if (w.groups == g->groups && (w.with_depth || !with_depth) && w.vloc >= vloc && w.fcap >= need_f &&
+        w.hcap >= need_h)
+        return ZG_OK;
     zg_bc_ws_release(g);
     const size_t G = g->groups, V = g->V;
-    w.fcap = 64 * (size_t)g->n_och + 64 * (size_t)g->max_och_per_v;
-    w.hcap = 64 * (size_t)g->n_ohub + 64;
+    const size_t VL = vloc;
+    w.fcap = need_f;
+    w.hcap = need_h;
     w.lcap = 256;
     int rc = ZG_OK;
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, G * V * 64);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, 2 * G * V * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, G * VL * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, 2 * G * VL * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hubacc, G * (size_t)g->n_ohub * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.bcg, G * V);
-    if (rc == ZG_OK && with_depth) rc = zg_dmalloc(&w.dep, G * V * 64);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.maxdep, G * V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * V);
+    if (rc == ZG_OK && with_depth) rc = zg_dmalloc(&w.dep, G * VL * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.maxdep, G * VL);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * VL);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * 2);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.touched, G * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.touched, G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.flist, G * w.fcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fmask, G * w.fcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hlist, G * w.hcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hmask, G * w.hcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.emax, (size_t)w.lcap * G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.foff, (size_t)w.lcap * G);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.hoff, (size_t)w.lcap * G);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.modes, (size_t)w.lcap * G);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.src, G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.wts, G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.dsum, G * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.cnt, G);
     if (rc == ZG_OK) {
         hipError_t e = hipHostMalloc((void **)&w.h_cnt, G * sizeof(zg_bc_ws::Counters), hipHostMallocDefault);
@@ -731,35 +874,62 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth)
         return rc;
     }
     w.groups = (int)G;
+    w.vloc = vloc;
     w.with_depth = with_depth;
     w.last_levels = w.lcap;
     return ZG_OK;
 }
 
-// Grows the per-level shift table to hold level `need` (copies the prefix).
+// Grows the per-level tables (shifts, list offsets, directions) to hold
+// level `need`, keeping their contents.
+template <typename T>
+static int bc_grow_table(T **tab, size_t old_rows, size_t new_rows, size_t row, hipStream_t s)
+{
+    T *nt = nullptr;
+    ZG_TRY_RC(zg_dmalloc(&nt, new_rows * row));
+    ZG_TRY(hipMemsetAsync(nt, 0, sizeof(T) * new_rows * row, s));
+    ZG_TRY(hipMemcpyAsync(nt, *tab, sizeof(T) * old_rows * row, hipMemcpyDeviceToDevice, s));
+    ZG_TRY(hipStreamSynchronize(s));
+    zg_dfree(*tab);
+    *tab = nt;
+    return ZG_OK;
+}
+
 static int bc_grow_levels(zg_graph *g, uint32_t need, hipStream_t s)
 {
     zg_bc_ws &w = g->bc;
     if (need < w.lcap) return ZG_OK;
     uint32_t nl = w.lcap;
     while (nl <= need) nl *= 2;
-    int *ne = nullptr;
-    const size_t row = (size_t)w.groups * 64;
-    ZG_TRY_RC(zg_dmalloc(&ne, (size_t)nl * row));
-    ZG_TRY(hipMemsetAsync(ne, 0, sizeof(int) * (size_t)nl * row, s));
-    ZG_TRY(hipMemcpyAsync(ne, w.emax, sizeof(int) * (size_t)w.lcap * row, hipMemcpyDeviceToDevice, s));
-    ZG_TRY(hipStreamSynchronize(s));
-    zg_dfree(w.emax);
-    w.emax = ne;
+    const size_t G = (size_t)w.groups;
+    ZG_TRY_RC(bc_grow_table(&w.emax, w.lcap, nl, G * 64, s));
+    ZG_TRY_RC(bc_grow_table(&w.foff, w.lcap, nl, G, s));
+    ZG_TRY_RC(bc_grow_table(&w.hoff, w.lcap, nl, G, s));
+    ZG_TRY_RC(bc_grow_table(&w.modes, w.lcap, nl, G, s));
     w.lcap = nl;
     return ZG_OK;
 }
 
-static BcDev bc_dev(zg_graph *g, uint32_t groups_used)
+// Per-batch callback used by the depth/sigma export.
+typedef int (*bc_batch_hook)(zg_graph *g, hipStream_t s, uint64_t first, uint32_t n_in_batch,
+                             uint32_t groups_used, void *user);
+
+// Options of one sweep: per-source weights (device, sweep order), per-source
+// dependency totals (host, sweep order), per-batch hook.
+struct BcSweepOpt {
+    const double *d_weights = nullptr;
+    double *h_totals = nullptr;
+    bc_batch_hook hook = nullptr;
+    void *user = nullptr;
+};
+
+static BcDev bc_dev(zg_graph *g, uint32_t groups_used, const GroupSpan &sp, const BcSweepOpt &opt)
 {
     zg_bc_ws &w = g->bc;
     BcDev a;
     a.V = g->V;
+    a.Vloc = w.vloc;
+    for (int q = 0; q < ZG_MAX_GROUPS; q++) a.vlo[q] = sp.vlo[q];
     a.G = groups_used;
     a.Gcap = (uint32_t)w.groups;
     a.n_ich = g->n_ich;
@@ -772,6 +942,12 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used)
     a.vis = w.vis; a.fr = w.fr; a.stats = w.stats; a.touched = w.touched;
     a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
     a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
+    a.wts = opt.d_weights ? w.wts : nullptr;
+    a.dsum = opt.h_totals ? w.dsum : nullptr;
+    a.foff = w.foff; a.hoff = w.hoff; a.modes = w.modes;
+    a.push_alpha = g->push_alpha;
+    for (int q = 0; q < ZG_MAX_GROUPS; q++)
+        a.Eg[q] = sp.vhi[q] > sp.vlo[q] ? g->h_in_ptr[sp.vhi[q]] - g->h_in_ptr[sp.vlo[q]] : 0;
     a.fcap = w.fcap; a.hcap = w.hcap;
     a.n_ohub = g->n_ohub;
     return a;
@@ -779,140 +955,122 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used)
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// Per-batch callback used by the depth/sigma export.
-typedef int (*bc_batch_hook)(zg_graph *g, hipStream_t s, uint64_t first, uint32_t n_in_batch,
-                             uint32_t groups_used, void *user);
-
 // Runs the batched sweep over d_sources[0..n) accumulating into the group
 // accumulators; the caller reduces them.
 static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_sources, uint64_t n, hipStream_t s,
-                    zg_bc_info *inf, bc_batch_hook hook, void *user)
+                    zg_bc_info *inf, const BcSweepOpt &opt)
 {
     zg_bc_ws &w = g->bc;
     const uint32_t V = g->V;
     const uint32_t Gcap = (uint32_t)w.groups;
     ZG_TRY(hipMemsetAsync(w.bcg, 0, sizeof(double) * (size_t)Gcap * V, s));
+    // per-block work statistics accumulate over all batches
+    ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * 2, s));
     std::vector<std::vector<uint32_t>> foff(Gcap), hoff(Gcap);
     for (uint64_t b0 = 0; b0 < n; b0 += (uint64_t)Gcap * 64) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)Gcap * 64, n - b0);
         const uint32_t G = cdiv(nb, 64);
-        BcDev a = bc_dev(g, G);
         // per-group spans: the components its sources belong to
         GroupSpan sp;
         memset(&sp, 0, sizeof(sp));
         uint32_t max_v = 0, max_c = 0;
         for (uint32_t q = 0; q < G; q++) {
             sp.used[q] = std::min<uint32_t>(64, nb - 64 * q);
-            if (h_sources) {
-                uint32_t lo = ZG_NONE, hi = 0;
-                for (uint32_t l = 0; l < sp.used[q]; l++) {
-                    const uint32_t x = h_sources[b0 + 64 * q + l];
-                    lo = std::min(lo, x);
-                    hi = std::max(hi, x);
-                }
-                const std::vector<uint32_t> &co = g->h_comp_off;
-                const size_t c_lo = std::upper_bound(co.begin(), co.end(), lo) - co.begin() - 1;
-                const size_t c_hi = std::upper_bound(co.begin(), co.end(), hi) - co.begin() - 1;
-                sp.vlo[q] = co[c_lo];
-                sp.vhi[q] = co[c_hi + 1];
-            } else {
-                sp.vlo[q] = 0;
-                sp.vhi[q] = V;
-            }
+            bc_group_span(g, h_sources + b0 + 64 * q, sp.used[q], &sp.vlo[q], &sp.vhi[q]);
+            if (sp.vhi[q] - sp.vlo[q] > w.vloc) return ZG_ERR_INVALID_ARG;  // workspace sized for other sources
             sp.clo[q] = g->h_ich_first[sp.vlo[q]];
             sp.chi[q] = g->h_ich_first[sp.vhi[q]];
             max_v = std::max(max_v, sp.vhi[q] - sp.vlo[q]);
             max_c = std::max(max_c, sp.chi[q] - sp.clo[q]);
         }
+        BcDev a = bc_dev(g, G, sp, opt);
         // batch reset
         ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * Gcap * 64, s));
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
+        if (opt.d_weights)
+            ZG_TRY(hipMemcpyAsync(w.wts, opt.d_weights + b0, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
+        if (opt.h_totals) ZG_TRY(hipMemsetAsync(w.dsum, 0, sizeof(unsigned long long) * Gcap * 64, s));
         ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
-        ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * 2, s));
         ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
         if (max_v) {
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reset_kernel, dim3(cdiv(max_v, 256), G), dim3(256), 0, s, a, sp);
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_rows_kernel, dim3(cdiv(max_v, 4), G), dim3(256), 0, s, a, sp);
         }
+        ZG_TRY(hipMemsetAsync(w.foff, 0, sizeof(uint32_t) * Gcap, s));  // level 0 starts at 0
+        ZG_TRY(hipMemsetAsync(w.hoff, 0, sizeof(uint32_t) * Gcap, s));
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_source_kernel, dim3(1, G), dim3(64), 0, s, a);
         ZG_TRY(hipGetLastError());
-        for (uint32_t q = 0; q < G; q++) {
-            foff[q].assign(1, 0);
-            hoff[q].assign(1, 0);
-        }
-        // level-0 list sizes (successor chunks of the sources)
-        ZG_TRY(hipMemcpyAsync(w.h_cnt, w.cnt, sizeof(zg_bc_ws::Counters) * G, hipMemcpyDeviceToHost, s));
-        ZG_TRY(hipStreamSynchronize(s));
-        for (uint32_t q = 0; q < G; q++) {
-            if (w.h_cnt[q].err & ZG_BC_ERR_SOURCE) return ZG_ERR_NODE_NOT_FOUND;
-            if (w.h_cnt[q].err) return ZG_ERR_HIP;
-            foff[q].push_back(w.h_cnt[q].fcount);  // end of level 0
-            hoff[q].push_back(w.h_cnt[q].hcount);
-        }
-        const uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(max_c, 4));
-        std::vector<unsigned long long> farcs(G);
-        for (uint32_t q = 0; q < G; q++) farcs[q] = w.h_cnt[q].farcs;
-        int d = 0;
-        for (;;) {
-            ZG_TRY_RC(bc_grow_levels(g, (uint32_t)d + 2, s));
-            a.emax = w.emax;
-            // direction: push from a small frontier, pull over predecessor chunks otherwise
-            unsigned long long fmax = 0;
-            for (uint32_t q = 0; q < G; q++) fmax = std::max(fmax, farcs[q]);
-            const bool push = g->push_alpha > 0.0 && (double)fmax * g->push_alpha < (double)g->E;
-            GroupRange stale;
-            memset(&stale, 0, sizeof(stale));
-            if (d >= 1)
-                for (uint32_t q = 0; q < G; q++) {
-                    stale.off[q] = foff[q][d - 1];
-                    stale.cnt[q] = foff[q][d] - foff[q][d - 1];
-                }
-            // reset the touched-vertex count and the successor-arc count
-            ZG_TRY(hipMemset2DAsync(&w.cnt[0].tcount, sizeof(zg_bc_ws::Counters), 0,
-                                    sizeof(uint32_t) + sizeof(unsigned long long), G, s));
-            if (push) {
-                GroupRange F;
-                memset(&F, 0, sizeof(F));
-                uint32_t mf = 0;
-                for (uint32_t q = 0; q < G; q++) {
-                    F.off[q] = foff[q][d];
-                    F.cnt[q] = foff[q][d + 1] - foff[q][d];
-                    mf = std::max(mf, F.cnt[q]);
-                }
-                if (mf) ZG_PROF_LAUNCH(ZG_PROF_BC_PUSH, bc_push_kernel, dim3(cdiv(mf, 4), G), dim3(256), 0, s, a, d, F);
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_sparse_kernel, dim3(ZG_SPARSE_FIN_BLOCKS, G),
-                               dim3(ZG_FIN_THREADS), 0, s, a, d, stale);
-                inf->push_levels++;
-            } else {
+        // Forward sweep, device-driven: each level opens with bc_level_kernel
+        // (offsets, direction, counter resets); push, pull and both finalize
+        // variants are queued and all but the chosen ones return at once.
+        // The host syncs once per chunk of levels to find the last one.
+        uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(max_c, 4));
+        if (g->pull_grid) fwd_blocks = std::min(fwd_blocks, g->pull_grid);
+        // grids of the other level kernels follow the largest span
+        const uint32_t fin_blocks = std::min<uint32_t>(ZG_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(max_c, 4096)));
+        const uint32_t sfin_blocks = std::min<uint32_t>(ZG_SPARSE_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(max_v, 8192)));
+        const uint32_t push_blocks = std::min<uint32_t>(ZG_PUSH_GRID, std::max<uint32_t>(1, cdiv(max_v, 32)));
+        std::vector<uint32_t> h_off, h_hoff, h_modes;  // [L][Gcap] copies
+        int d = 0, maxd = -1;
+        uint32_t chunk = ZG_LEVEL_CHUNK0;
+        while (maxd < 0) {
+            ZG_TRY_RC(bc_grow_levels(g, (uint32_t)d + chunk + 2, s));
+            a = bc_dev(g, G, sp, opt);
+            for (uint32_t k = 0; k < chunk; k++) {
+                const int L = d + (int)k;
+                ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_level_kernel, dim3(1), dim3(64), 0, s, a, L);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_PUSH, bc_push_kernel, dim3(push_blocks, G), dim3(256), 0, s, a, L);
                 if (max_c)
-                    ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(fwd_blocks, G), dim3(256), 0, s, a, d, sp);
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_kernel, dim3(ZG_FIN_BLOCKS, G), dim3(ZG_FIN_THREADS), 0,
-                               s, a, d, stale, 0, sp);
-                inf->pull_levels++;
+                    ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, bc_forward_kernel, dim3(fwd_blocks, G), dim3(256), 0, s, a, L, sp);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_sparse_kernel, dim3(sfin_blocks, G),
+                               dim3(ZG_FIN_THREADS), 0, s, a, L);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bc_finalize_kernel, dim3(fin_blocks, G), dim3(ZG_FIN_THREADS), 0,
+                               s, a, L, 0, sp);
             }
             ZG_TRY(hipGetLastError());
+            // offsets of levels up to d+chunk, directions of levels d..d+chunk-1
+            const size_t rows = (size_t)d + chunk + 1;
+            h_off.resize(rows * Gcap);
+            h_hoff.resize(rows * Gcap);
+            h_modes.resize(rows * Gcap);
+            ZG_TRY(hipMemcpyAsync(h_off.data(), w.foff, sizeof(uint32_t) * rows * Gcap, hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipMemcpyAsync(h_hoff.data(), w.hoff, sizeof(uint32_t) * rows * Gcap, hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipMemcpyAsync(h_modes.data(), w.modes, sizeof(uint32_t) * rows * Gcap, hipMemcpyDeviceToHost, s));
             ZG_TRY(hipMemcpyAsync(w.h_cnt, w.cnt, sizeof(zg_bc_ws::Counters) * G, hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
-            bool any = false;
             for (uint32_t q = 0; q < G; q++) {
+                if (w.h_cnt[q].err & ZG_BC_ERR_SOURCE) return ZG_ERR_NODE_NOT_FOUND;
                 if (w.h_cnt[q].err) return ZG_ERR_HIP;  // work-list capacity exceeded
-                uint32_t fc = w.h_cnt[q].fcount;
-                if (fc > foff[q].back()) any = true;
-                foff[q].push_back(fc);
-                hoff[q].push_back(w.h_cnt[q].hcount);
-                farcs[q] = w.h_cnt[q].farcs;
             }
-            if (!any) break;
-            d++;
+            // level L holds entries [off[L], off[L+1]); the level after the
+            // chunk ends at fcount
+            bool tail_busy = false;
+            for (uint32_t q = 0; q < G; q++) tail_busy |= w.h_cnt[q].fcount > h_off[(size_t)(d + chunk) * Gcap + q];
+            if (tail_busy) {
+                d += (int)chunk;
+                chunk = std::min<uint32_t>(chunk * 2, ZG_LEVEL_CHUNK_MAX);
+                continue;
+            }
+            for (int L = d + (int)chunk - 1; L >= 0 && maxd < 0; L--)
+                for (uint32_t q = 0; q < G; q++)
+                    if (h_off[(size_t)(L + 1) * Gcap + q] > h_off[(size_t)L * Gcap + q]) maxd = L;
+            if (maxd < 0) maxd = 0;
         }
-        const int maxd = d;  // deepest non-empty level
-        {
-            std::vector<unsigned long long> st((size_t)G * ZG_FIN_BLOCKS * 2);
-            ZG_TRY(hipMemcpyAsync(st.data(), w.stats, sizeof(unsigned long long) * st.size(), hipMemcpyDeviceToHost, s));
-            ZG_TRY(hipStreamSynchronize(s));
-            for (size_t i = 0; i < st.size(); i += 2) {
-                inf->arcs_traversed += st[i];
-                inf->nodes_reached += st[i + 1];
+        for (int L = 0; L <= maxd; L++) {
+            bool push = false, pull = false;
+            for (uint32_t q = 0; q < G; q++) {
+                push |= h_modes[(size_t)L * Gcap + q] == ZG_MODE_PUSH;
+                pull |= h_modes[(size_t)L * Gcap + q] == ZG_MODE_PULL;
+            }
+            inf->push_levels += push;
+            inf->pull_levels += pull;
+        }
+        for (uint32_t q = 0; q < G; q++) {
+            foff[q].resize((size_t)maxd + 2);
+            hoff[q].resize((size_t)maxd + 2);
+            for (int L = 0; L <= maxd + 1; L++) {
+                foff[q][L] = h_off[(size_t)L * Gcap + q];
+                hoff[q][L] = h_hoff[(size_t)L * Gcap + q];
             }
         }
         inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)maxd + 1);
@@ -935,7 +1093,22 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         }
         ZG_TRY(hipGetLastError());
         inf->batches++;
-        if (hook) ZG_TRY_RC(hook(g, s, b0, nb, G, user));
+        if (opt.h_totals) {
+            std::vector<unsigned long long> t(nb);
+            ZG_TRY(hipMemcpyAsync(t.data(), w.dsum, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+            for (uint32_t i = 0; i < nb; i++) opt.h_totals[b0 + i] = (double)t[i];
+        }
+        if (opt.hook) ZG_TRY_RC(opt.hook(g, s, b0, nb, G, opt.user));
+    }
+    {
+        std::vector<unsigned long long> st((size_t)Gcap * ZG_FIN_BLOCKS * 2);
+        ZG_TRY(hipMemcpyAsync(st.data(), w.stats, sizeof(unsigned long long) * st.size(), hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+        for (size_t i = 0; i < st.size(); i += 2) {
+            inf->arcs_traversed += st[i];
+            inf->nodes_reached += st[i + 1];
+        }
     }
     inf->n_sources = n;
     return ZG_OK;
@@ -1008,16 +1181,20 @@ static int bc_core_distances(zg_graph *g, hipStream_t s)
     return rc;
 }
 
-// Sorts sources so that each 64-source group holds sources of one graph
-// (component) at the same distance from its hub core (then larger out-degree
-// first, then id).  Components are contiguous id ranges, so comparing ids
-// across components orders by component.
-static void bc_order_sources(const zg_graph *g, std::vector<uint32_t> &src)
+// Order in which the sources are swept: each 64-source group holds sources
+// of one graph (component) at the same distance from its hub core (then
+// larger out-degree first, then id).  Components are contiguous id ranges,
+// so comparing ids across components orders by component.  Returns the
+// positions of src in sweep order.
+static void bc_order_perm(const zg_graph *g, const std::vector<uint32_t> &src, std::vector<uint64_t> &perm)
 {
     const std::vector<uint32_t> &cd = g->h_coredist;
     const std::vector<uint32_t> &co = g->h_comp_off;
     auto comp = [&](uint32_t x) { return std::upper_bound(co.begin(), co.end(), x) - co.begin(); };
-    std::stable_sort(src.begin(), src.end(), [&](uint32_t x, uint32_t y) {
+    perm.resize(src.size());
+    for (size_t i = 0; i < perm.size(); i++) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](uint64_t i, uint64_t j) {
+        const uint32_t x = src[i], y = src[j];
         if (co.size() > 2) {
             const auto cx = comp(x), cy = comp(y);
             if (cx != cy) return cx < cy;
@@ -1028,79 +1205,96 @@ static void bc_order_sources(const zg_graph *g, std::vector<uint32_t> &src)
     });
 }
 
-static int bc_all_live_sources(zg_graph *g, std::vector<uint32_t> &out)
+static void bc_all_live_sources(zg_graph *g, std::vector<uint32_t> &out)
 {
     out.clear();
     out.reserve(g->n_live);
     for (uint32_t v = 0; v < g->V; v++)
         if (g->h_live[v]) out.push_back(v);
-    return ZG_OK;
 }
 
-static int zg_bc_device_impl(zg_graph *g, const uint32_t *d_sources, uint64_t n_sources, double *d_out,
-                             hipStream_t s, zg_bc_info *info)
+// One Brandes call: sources (device list, or null = every live vertex in id
+// order), optional per-source weights and dependency totals (host, in the
+// caller's source order), optional output (device [V]).
+struct BcCall {
+    const uint32_t *d_sources = nullptr;
+    uint64_t n_sources = 0;
+    const double *h_weights = nullptr;
+    double *h_totals = nullptr;
+    double *d_out = nullptr;
+};
+
+static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_bc_info *info)
 {
     zg_bc_info inf;
     memset(&inf, 0, sizeof(inf));
     const uint32_t V = g->V;
     // the sweep needs the source list on the host (ordering, per-group spans)
     std::vector<uint32_t> list;
-    if (!d_sources) {
+    if (!call.d_sources) {
         bc_all_live_sources(g, list);
     } else {
-        list.resize(n_sources);
-        if (n_sources) {
-            ZG_TRY(hipMemcpyAsync(list.data(), d_sources, sizeof(uint32_t) * n_sources, hipMemcpyDeviceToHost, s));
+        list.resize(call.n_sources);
+        if (call.n_sources) {
+            ZG_TRY(hipMemcpyAsync(list.data(), call.d_sources, sizeof(uint32_t) * call.n_sources,
+                                  hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
         }
         for (uint32_t x : list)
             if (x >= V || !g->h_live[x]) return ZG_ERR_NODE_NOT_FOUND;
     }
+    const uint64_t n = list.size();
+    const bool order = g->source_order && n > 64;
     // engine: the per-source queue engine for high-diameter graphs
-    if (g->engine == ZG_ENGINE_AUTO || (g->source_order && list.size() > 64)) ZG_TRY_RC(bc_core_distances(g, s));
-    const bool queue_engine =
-        g->engine == ZG_ENGINE_QUEUE || (g->engine == ZG_ENGINE_AUTO && g->high_diameter);
-    uint32_t *d_tmp = nullptr;
-    if (!d_sources || (g->source_order && list.size() > 64)) {
-        if (g->source_order && list.size() > 64) bc_order_sources(g, list);
-        ZG_TRY_RC(zg_dmalloc(&d_tmp, list.size()));
-        if (!list.empty())
-            ZG_TRY(hipMemcpyAsync(d_tmp, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, s));
-        ZG_TRY(hipStreamSynchronize(s));
-        d_sources = d_tmp;
+    if (g->engine == ZG_ENGINE_AUTO || order) ZG_TRY_RC(bc_core_distances(g, s));
+    const bool queue_engine = g->engine == ZG_ENGINE_QUEUE || (g->engine == ZG_ENGINE_AUTO && g->high_diameter);
+    std::vector<uint64_t> perm;
+    if (order) {
+        bc_order_perm(g, list, perm);
+        std::vector<uint32_t> sorted(n);
+        for (uint64_t i = 0; i < n; i++) sorted[i] = list[perm[i]];
+        list.swap(sorted);
     }
-    n_sources = list.size();
-    if (queue_engine) {
-        hipEvent_t q0 = nullptr, q1 = nullptr;
-        int qrc = zg_hip_code(hipEventCreate(&q0));
-        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventCreate(&q1));
-        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventRecord(q0, s));
-        if (qrc == ZG_OK) qrc = zg_bcq_sweep(g, d_sources, n_sources, d_out, s, &inf);
-        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventRecord(q1, s));
-        if (qrc == ZG_OK) qrc = zg_hip_code(hipEventSynchronize(q1));
-        if (qrc == ZG_OK) {
-            float ms = 0.f;
-            qrc = zg_hip_code(hipEventElapsedTime(&ms, q0, q1));
-            inf.kernel_ms = ms;
-        }
-        if (q0) (void)hipEventDestroy(q0);
-        if (q1) (void)hipEventDestroy(q1);
-        (void)hipStreamSynchronize(s);
-        if (qrc == ZG_OK) qrc = zg_prof_collect(g, s);
-        zg_dfree(d_tmp);
-        if (qrc == ZG_OK && info) *info = inf;
-        return qrc;
+    std::vector<double> wsorted, tsorted;
+    if (call.h_weights) {
+        wsorted.resize(n);
+        for (uint64_t i = 0; i < n; i++) wsorted[i] = call.h_weights[order ? perm[i] : i];
     }
-    int rc = bc_ws_ensure(g, false);
+    if (call.h_totals) tsorted.assign(n, 0.0);
+    uint32_t *d_list = nullptr;
+    double *d_w = nullptr, *d_acc = nullptr;
+    int rc = zg_dmalloc(&d_list, n);
+    if (rc == ZG_OK && n)
+        rc = zg_hip_code(hipMemcpyAsync(d_list, list.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    if (rc == ZG_OK && call.h_weights) {
+        rc = zg_dmalloc(&d_w, n);
+        if (rc == ZG_OK && n)
+            rc = zg_hip_code(hipMemcpyAsync(d_w, wsorted.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    }
+    double *d_out = call.d_out;
+    if (rc == ZG_OK && !d_out) {  // totals only: the scores go to scratch
+        rc = zg_dmalloc(&d_acc, V);
+        d_out = d_acc;
+    }
+    if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e0, s));
-    if (rc == ZG_OK) rc = bc_sweep(g, d_sources, list.data(), n_sources, s, &inf, nullptr, nullptr);
-    if (rc == ZG_OK && V) {
-        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reduce_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V,
-                       (uint32_t)g->bc.groups, g->bc.bcg, g->live, d_out);
-        rc = zg_hip_code(hipGetLastError());
+    if (queue_engine) {
+        if (rc == ZG_OK)
+            rc = zg_bcq_sweep(g, d_list, n, d_w, call.h_totals ? tsorted.data() : nullptr, d_out, s, &inf);
+    } else {
+        if (rc == ZG_OK) rc = bc_ws_ensure(g, false, list.data(), n);
+        BcSweepOpt opt;
+        opt.d_weights = d_w;
+        opt.h_totals = call.h_totals ? tsorted.data() : nullptr;
+        if (rc == ZG_OK) rc = bc_sweep(g, d_list, list.data(), n, s, &inf, opt);
+        if (rc == ZG_OK && V) {
+            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reduce_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V,
+                           (uint32_t)g->bc.groups, g->bc.bcg, g->live, d_out);
+            rc = zg_hip_code(hipGetLastError());
+        }
     }
     if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e1, s));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventSynchronize(e1));
@@ -1113,7 +1307,11 @@ static int zg_bc_device_impl(zg_graph *g, const uint32_t *d_sources, uint64_t n_
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(s);
     if (rc == ZG_OK) rc = zg_prof_collect(g, s);
-    zg_dfree(d_tmp);
+    zg_dfree(d_list);
+    zg_dfree(d_w);
+    zg_dfree(d_acc);
+    if (rc == ZG_OK && call.h_totals)
+        for (uint64_t i = 0; i < n; i++) call.h_totals[order ? perm[i] : i] = tsorted[i];
     if (rc == ZG_OK && info) *info = inf;
     return rc;
 }
@@ -1126,7 +1324,43 @@ extern "C" int zg_bc_unweighted_device(zg_graph *g, const uint32_t *d_sources, u
     std::lock_guard<std::mutex> lk(g->mu);
     ZG_TRY(hipSetDevice(g->device));
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
-    return zg_bc_device_impl(g, d_sources, n_sources, d_out_bc, s, info);
+    BcCall call;
+    call.d_sources = d_sources;
+    call.n_sources = n_sources;
+    call.d_out = d_out_bc;
+    return zg_bc_device_impl(g, call, s, info);
+}
+
+// Host-buffer form shared by zg_bc_unweighted and zg_bc_weighted.
+static int bc_host_call(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources,
+                        double *out_bc, double *out_totals, zg_bc_info *info)
+{
+    if (sources)
+        for (uint64_t i = 0; i < n_sources; i++)
+            if (sources[i] >= g->V || !g->h_live[sources[i]]) return ZG_ERR_NODE_NOT_FOUND;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    uint32_t *d_src = nullptr;
+    double *d_out = nullptr;
+    int rc = out_bc ? zg_dmalloc(&d_out, g->V) : ZG_OK;
+    if (rc == ZG_OK && sources) {
+        rc = zg_dmalloc(&d_src, n_sources);
+        if (rc == ZG_OK && n_sources)
+            rc = zg_hip_code(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, s));
+    }
+    BcCall call;
+    call.d_sources = sources ? d_src : nullptr;
+    call.n_sources = sources ? n_sources : 0;
+    call.h_weights = weights;
+    call.h_totals = out_totals;
+    call.d_out = d_out;
+    if (rc == ZG_OK) rc = zg_bc_device_impl(g, call, s, info);
+    if (rc == ZG_OK && out_bc && g->V)
+        rc = zg_hip_code(hipMemcpy(out_bc, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
+    zg_dfree(d_src);
+    zg_dfree(d_out);
+    return rc;
 }
 
 extern "C" int zg_bc_unweighted(zg_graph *g, const uint32_t *sources, uint64_t n_sources, double *out_bc,
@@ -1134,30 +1368,23 @@ extern "C" int zg_bc_unweighted(zg_graph *g, const uint32_t *sources, uint64_t n
 {
     if (!g) return ZG_ERR_INVALID_HANDLE;
     if (!out_bc && g->V) return ZG_ERR_INVALID_ARG;
-    if (sources && n_sources) {
+    return bc_host_call(g, sources, nullptr, n_sources, out_bc, nullptr, info);
+}
+
+extern "C" int zg_bc_weighted(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources,
+                              double *out_bc, double *out_totals, zg_bc_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!out_bc && !out_totals) return ZG_ERR_INVALID_ARG;
+    if (weights && !sources) return ZG_ERR_INVALID_ARG;
+    if (weights)
         for (uint64_t i = 0; i < n_sources; i++)
-            if (sources[i] >= g->V || !g->h_live[sources[i]]) return ZG_ERR_NODE_NOT_FOUND;
-    }
-    std::lock_guard<std::mutex> lk(g->mu);
-    ZG_TRY(hipSetDevice(g->device));
-    hipStream_t s = g->stream;
-    uint32_t *d_src = nullptr;
-    double *d_out = nullptr;
-    int rc = zg_dmalloc(&d_out, g->V);
-    if (rc == ZG_OK && sources) {
-        rc = zg_dmalloc(&d_src, n_sources);
-        if (rc == ZG_OK && n_sources)
-            rc = zg_hip_code(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, s));
-    }
-    if (rc == ZG_OK) rc = zg_bc_device_impl(g, sources ? d_src : nullptr, sources ? n_sources : 0, d_out, s, info);
-    if (rc == ZG_OK && g->V)
-        rc = zg_hip_code(hipMemcpy(out_bc, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
-    zg_dfree(d_src);
-    zg_dfree(d_out);
-    return rc;
+            if (!std::isfinite(weights[i])) return ZG_ERR_INVALID_ARG;
+    return bc_host_call(g, sources, weights, n_sources, out_bc, out_totals, info);
 }
 
 struct PathsCtx {
+    const uint32_t *sources;
     uint32_t *depth_out;
     double *sigma_out;
     int32_t *shift_out;
@@ -1179,8 +1406,10 @@ static int bc_paths_hook(zg_graph *g, hipStream_t s, uint64_t first, uint32_t nb
     for (uint32_t q = 0; q < G; q++) {
         const uint32_t nl = std::min<uint32_t>(64, nb - 64 * q);
         if (V) {
-            hipLaunchKernelGGL(bc_export_kernel, dim3(cdiv(V, 256), 64), dim3(256), 0, s, V, q, w.dep, w.sig,
-                               c->d_dep, c->d_sig);
+            uint32_t vlo, vhi;
+            bc_group_span(g, c->sources + first + 64ull * q, nl, &vlo, &vhi);
+            hipLaunchKernelGGL(bc_export_kernel, dim3(cdiv(V, 256), 64), dim3(256), 0, s, V, w.vloc, vlo, vhi, q,
+                               w.dep, w.sig, c->d_dep, c->d_sig);
             ZG_TRY(hipGetLastError());
         }
         const uint64_t i0 = first + 64ull * q;
@@ -1216,8 +1445,9 @@ extern "C" int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sour
     std::lock_guard<std::mutex> lk(g->mu);
     ZG_TRY(hipSetDevice(g->device));
     hipStream_t s = g->stream;
-    ZG_TRY_RC(bc_ws_ensure(g, true));
+    ZG_TRY_RC(bc_ws_ensure(g, true, sources, n_sources));
     PathsCtx c;
+    c.sources = sources;
     c.depth_out = depth_out;
     c.sigma_out = sigma_out;
     c.shift_out = shift_out;
@@ -1233,7 +1463,10 @@ extern "C" int zg_bc_paths(zg_graph *g, const uint32_t *sources, uint64_t n_sour
         rc = zg_hip_code(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, s));
     zg_bc_info inf;
     memset(&inf, 0, sizeof(inf));
-    if (rc == ZG_OK) rc = bc_sweep(g, d_src, sources, n_sources, s, &inf, bc_paths_hook, &c);
+    BcSweepOpt opt;
+    opt.hook = bc_paths_hook;
+    opt.user = &c;
+    if (rc == ZG_OK) rc = bc_sweep(g, d_src, sources, n_sources, s, &inf, opt);
     (void)hipStreamSynchronize(s);
     zg_dfree(c.d_dep);
     zg_dfree(c.d_sig);

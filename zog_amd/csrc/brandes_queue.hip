@@ -32,7 +32,7 @@
 
 #include <algorithm>
 
-#define ZG_Q_MAX_SOURCES (64 * ZG_MAX_GROUPS)
+#define ZG_Q_MAX_SOURCES 1024  // sources per batch (LDS range tables)
 #define ZG_Q_GRID 1024
 #define ZG_Q_LEVELS_PER_SYNC 32
 
@@ -48,6 +48,7 @@ struct BcqDev {
     int *kshift;                    // [lcap][S]
     uint32_t *newcnt;               // [lcap] entries added at each level
     const uint32_t *src;            // [S]
+    const double *wts;              // [S] source weights, or null (all 1)
     double *bc;                     // [V]
     unsigned long long *stats;      // [2] arcs, reached
 };
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
         const double sv = a.sig[QROW(s) + v];
         const double delta = sv * ldexp(S, -a.kshift[(size_t)d * a.S + s]);
         cout[QROW(s) + v] = (1.0 + delta) / sv;
-        if (v != a.src[s]) atomicAdd(&a.bc[v], delta);
+        if (v != a.src[s]) atomicAdd(&a.bc[v], a.wts ? delta * a.wts[s] : delta);
     }
 }
 
@@ -269,7 +270,7 @@ int zg_bcq_ws_release(zg_graph *g)
 {
     zg_bcq_ws &w = g->bcq;
     zg_dfree(w.depth); zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.queue); zg_dfree(w.lstart);
-    zg_dfree(w.qlen); zg_dfree(w.kshift); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.bc);
+    zg_dfree(w.qlen); zg_dfree(w.kshift); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.wts); zg_dfree(w.bc);
     zg_dfree(w.stats);
     w.S = 0;
     w.lcap = 0;
@@ -293,6 +294,7 @@ static int bcq_ws_ensure(zg_graph *g, uint32_t S)
     if (rc == ZG_OK) rc = zg_dmalloc(&w.kshift, (size_t)w.lcap * S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.newcnt, w.lcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.src, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.wts, S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.bc, V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, 2);
     if (rc != ZG_OK) {
@@ -339,7 +341,7 @@ static int bcq_grow_levels(zg_graph *g, uint32_t need, uint32_t used, hipStream_
     return ZG_OK;
 }
 
-static BcqDev bcq_dev(zg_graph *g)
+static BcqDev bcq_dev(zg_graph *g, bool weighted)
 {
     zg_bcq_ws &w = g->bcq;
     BcqDev a;
@@ -358,15 +360,17 @@ static BcqDev bcq_dev(zg_graph *g)
     a.kshift = w.kshift;
     a.newcnt = w.newcnt;
     a.src = w.src;
+    a.wts = weighted ? w.wts : nullptr;
     a.bc = w.bc;
     a.stats = w.stats;
     return a;
 }
 
-int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double *d_out, hipStream_t s, zg_bc_info *inf)
+int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const double *d_weights, double *h_totals,
+                 double *d_out, hipStream_t s, zg_bc_info *inf)
 {
     const uint32_t V = g->V;
-    const uint32_t S = 64u * (uint32_t)g->groups;
+    const uint32_t S = std::min<uint32_t>(64u * (uint32_t)g->groups, ZG_Q_MAX_SOURCES);
     ZG_TRY_RC(bcq_ws_ensure(g, S));
     zg_bcq_ws &w = g->bcq;
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
@@ -375,12 +379,14 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double *d_o
         const uint32_t nb = (uint32_t)std::min<uint64_t>(S, n - b0);
         ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * S, s));
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
+        if (d_weights)
+            ZG_TRY(hipMemcpyAsync(w.wts, d_weights + b0, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
         ZG_TRY(hipMemsetAsync(w.depth, 0xFF, sizeof(uint32_t) * (size_t)S * V, s));
         ZG_TRY(hipMemsetAsync(w.sig, 0, sizeof(double) * (size_t)S * V, s));
         ZG_TRY(hipMemsetAsync(w.kshift, 0, sizeof(int) * (size_t)w.lcap * S, s));
         ZG_TRY(hipMemsetAsync(w.newcnt, 0, sizeof(uint32_t) * w.lcap, s));
         ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 2, s));
-        BcqDev a = bcq_dev(g);
+        BcqDev a = bcq_dev(g, d_weights != nullptr);
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bcq_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, a);
         ZG_TRY(hipGetLastError());
         // forward: queue levels in chunks, sync to find the last one
@@ -388,7 +394,7 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double *d_o
         int maxd = -1;
         while (maxd < 0) {
             ZG_TRY_RC(bcq_grow_levels(g, d + ZG_Q_LEVELS_PER_SYNC + 2, d + 2, s));
-            a = bcq_dev(g);
+            a = bcq_dev(g, d_weights != nullptr);
             for (uint32_t k = 0; k < ZG_Q_LEVELS_PER_SYNC; k++) {
                 ZG_PROF_LAUNCH(ZG_PROF_BC_PUSH, bcq_forward_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a, (int)(d + k));
                 ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bcq_close_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a,
@@ -417,6 +423,18 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double *d_o
         inf->nodes_reached += st[1];
         inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)maxd + 1);
         inf->batches++;
+        if (h_totals) {
+            // dependency total of a source: sum over its levels L of (L - 1) x level size
+            std::vector<uint32_t> ls((size_t)(maxd + 2) * S);
+            ZG_TRY(hipMemcpyAsync(ls.data(), w.lstart, sizeof(uint32_t) * ls.size(), hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+            for (uint32_t i = 0; i < nb; i++) {
+                double t = 0.0;
+                for (int L = 2; L <= maxd; L++)
+                    t += (double)(L - 1) * (double)(ls[(size_t)(L + 1) * S + i] - ls[(size_t)L * S + i]);
+                h_totals[b0 + i] = t;
+            }
+        }
     }
     if (V) {
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bcq_out_kernel, dim3((V + 255) / 256), dim3(256), 0, s, V, w.bc, g->live,

@@ -23,7 +23,7 @@
 // next level's counts are stored divided by 2^exponent (exact).
 #define ZG_SHIFT_T 512
 
-#define ZG_MAX_GROUPS 16
+#define ZG_MAX_GROUPS 64
 
 // The finalize step runs as a fixed grid: each block owns a contiguous range
 // of predecessor chunks, so it reserves list space with one atomic per level.
@@ -88,6 +88,7 @@ static inline void zg_dfree(T *&p)
 //   emax[L][g][lane]        largest exponent seen at level L (shift k_L)
 struct zg_bc_ws {
     int groups = 0;
+    uint32_t vloc = 0;                   // per-group vertex capacity of the state arrays
     bool with_depth = false;
     double *sig = nullptr, *cc = nullptr, *hubacc = nullptr, *bcg = nullptr;
     uint32_t *dep = nullptr;
@@ -100,13 +101,19 @@ struct zg_bc_ws {
     uint32_t *hlist = nullptr;           // [G][hcap]
     uint64_t *hmask = nullptr;
     int *emax = nullptr;                 // [lcap][G][64]
+    uint32_t *foff = nullptr, *hoff = nullptr;  // [lcap][G] level offsets in flist / hlist
+    uint32_t *modes = nullptr;           // [lcap][G] direction taken at each level
     uint32_t lcap = 0;
     uint32_t *src = nullptr;             // [G][64]
+    double *wts = nullptr;               // [G][64] per-source weights of a weighted sweep
+    unsigned long long *dsum = nullptr;  // [G][64] per-source dependency totals (integers)
     struct Counters {
         uint32_t fcount, hcount;
         uint32_t err;               // ZG_BC_ERR_* bits
         uint32_t tcount;            // vertices first reached by the current push level
-        unsigned long long farcs;   // successor arcs of the newest level (vertex-level)
+        unsigned long long farcs[2];  // successor arcs of a level's vertices, by level parity
+        uint32_t mode;              // ZG_MODE_* of the current level
+        uint32_t pad;
     };
     Counters *cnt = nullptr;             // [G]
     Counters *h_cnt = nullptr;           // pinned host mirror
@@ -126,6 +133,7 @@ struct zg_bcq_ws {
     int *kshift = nullptr;               // [lcap][S]
     uint32_t *newcnt = nullptr;          // [lcap]
     uint32_t *src = nullptr;             // [S]
+    double *wts = nullptr;               // [S] per-source weights of a weighted sweep
     double *bc = nullptr;                // [V]
     unsigned long long *stats = nullptr; // [2]
 };
@@ -164,6 +172,7 @@ struct zg_graph {
     // predecessor chunks (forward sweep work items)
     uint32_t n_ich = 0;
     uint32_t *ich_v = nullptr, *ich_e0 = nullptr, *ich_e1 = nullptr;
+    std::vector<uint32_t> h_in_ptr;      // [V+1] host copy of the predecessor CSR offsets
     std::vector<uint32_t> h_ich_first;   // [V+1] first predecessor chunk of each vertex
     uint32_t n_inhub = 0;
     uint32_t *inhub = nullptr;           // vertices with more than one predecessor chunk
@@ -171,6 +180,8 @@ struct zg_graph {
     uint32_t n_och = 0, max_och_per_v = 0;
     uint32_t *och_first = nullptr, *och_v = nullptr, *och_e0 = nullptr, *och_e1 = nullptr;
     uint32_t n_ohub = 0;
+    std::vector<uint32_t> h_och_first;   // [V+1] first successor chunk of each vertex
+    std::vector<uint32_t> h_hub_first;   // [V+1] multi-chunk vertices before each vertex
     uint32_t *ohub_idx = nullptr;        // [V] dense index of multi-chunk vertices or ZG_NONE
     // PageRank row bins
     uint32_t n_small = 0, n_med = 0, n_large = 0, n_seg = 0;
@@ -183,6 +194,8 @@ struct zg_graph {
     uint32_t *pr_seg_e0 = nullptr, *pr_seg_e1 = nullptr;
     int groups = 4;
     int engine = 0;                      // ZG_ENGINE_*
+    uint32_t *und_ptr = nullptr, *und_col = nullptr;  // simple undirected CSR (core numbers), built on demand
+    uint32_t pull_grid = 4096;           // pull workgroups per source group (0 = one wave per chunk)
     bool high_diameter = false;          // set with h_coredist
     zg_bc_ws bc;
     zg_bcq_ws bcq;
@@ -213,7 +226,8 @@ void zg_prof_release(zg_graph *g);
 int zg_bc_ws_release(zg_graph *g);
 int zg_bcq_ws_release(zg_graph *g);
 int zg_pr_ws_release(zg_graph *g);
-int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double *d_out, hipStream_t s, zg_bc_info *inf);
+int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const double *d_weights, double *h_totals,
+                 double *d_out, hipStream_t s, zg_bc_info *inf);
 
 // ---------------------------------------------------------------------------
 // Device helpers

@@ -100,6 +100,7 @@ static void zg_free_device(zg_graph *g)
     zg_bcq_ws_release(g);
     zg_pr_ws_release(g);
     zg_dfree(g->out_ptr); zg_dfree(g->out_col); zg_dfree(g->in_ptr); zg_dfree(g->in_col);
+    zg_dfree(g->und_ptr); zg_dfree(g->und_col);
     zg_dfree(g->live);
     zg_dfree(g->ich_v); zg_dfree(g->ich_e0); zg_dfree(g->ich_e1); zg_dfree(g->inhub);
     zg_dfree(g->och_first); zg_dfree(g->och_v); zg_dfree(g->och_e0); zg_dfree(g->och_e1);
@@ -213,6 +214,9 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     }
     och_first[V] = (uint32_t)och_v.size();
     g->n_och = (uint32_t)och_v.size();
+    g->h_och_first = och_first;
+    g->h_hub_first.assign((size_t)V + 1, 0);
+    for (uint32_t v = 0; v < V; v++) g->h_hub_first[v + 1] = g->h_hub_first[v] + (ohub_idx[v] != ZG_NONE ? 1 : 0);
     g->max_och_per_v = max_k;
     g->n_ohub = n_ohub;
 
@@ -255,6 +259,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     ZG_TRY_RC(zg_upload(&g->out_ptr, out_ptr));
     ZG_TRY_RC(zg_upload(&g->out_col, out_col));
     ZG_TRY_RC(zg_upload(&g->in_ptr, in_ptr));
+    g->h_in_ptr = in_ptr;
     ZG_TRY_RC(zg_upload(&g->in_col, in_col));
     ZG_TRY_RC(zg_upload(&g->live, g->h_live));
     ZG_TRY_RC(zg_upload(&g->ich_v, ich_v));
@@ -396,6 +401,10 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
     case ZG_TUNE_ENGINE:
         if (value != ZG_ENGINE_AUTO && value != ZG_ENGINE_LANES && value != ZG_ENGINE_QUEUE) return ZG_ERR_INVALID_ARG;
         g->engine = (int)value;
+        return ZG_OK;
+    case ZG_TUNE_PULL_GRID:
+        if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;
+        g->pull_grid = (uint32_t)value;
         return ZG_OK;
     default: return ZG_ERR_INVALID_ARG;
     }

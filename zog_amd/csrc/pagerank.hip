@@ -577,6 +577,13 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
     return ZG_OK;
 }
 
+// For other parts of the library that already hold the handle's lock.
+int zg_pagerank_dev(zg_graph *g, double damping, uint64_t max_iterations, double tolerance, int precision,
+                    double *d_out, hipStream_t s, zg_pr_info *info)
+{
+    return zg_pagerank_impl(g, damping, max_iterations, tolerance, precision, d_out, s, info);
+}
+
 extern "C" int zg_pagerank_device(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
                                   int precision, double *d_out, void *stream, zg_pr_info *info)
 {

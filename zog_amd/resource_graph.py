@@ -343,6 +343,13 @@ def set_push_alpha(graph: ResourceGraph, alpha: float) -> None:
                "zg_set_tuning")
 
 
+def set_pull_grid(graph: ResourceGraph, blocks: int) -> None:
+    """Workgroups per 64-source group on pull levels (0 = one wavefront per
+    predecessor chunk)."""
+    _lib.check(_lib.lib().zg_set_tuning(graph.resource.handle, _lib.ZG_TUNE_PULL_GRID, float(blocks)),
+               "zg_set_tuning")
+
+
 def set_engine(graph: ResourceGraph, engine: str) -> None:
     """'auto' (default), 'lanes' (64 sources per wavefront) or 'queue'
     (per-source queues, for high-diameter graphs)."""
