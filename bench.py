@@ -9,7 +9,14 @@ reported beside it.  Other workloads (--workload): er10k (directed
 Erdos-Renyi n=10,000, avg out-degree 8, all 10,000 sources; PageRank tol
 1e-6), grid2048 (2048x2048 grid with 5% of edges removed, 1,024 seeded
 sources; PageRank to 1e-8 L1 in the mixed FP32/FP64 mode, at most 1,000
-iterations).
+iterations).  ba512 (512 undirected Barabasi-Albert graphs, m=4, node counts
+log-uniform in [1,000, 100,000], all sources of every graph in one batched
+resource).  pivots (learned vs uniform pivots for sampled Brandes: the
+node-scoring network is trained on the GPU on exact dependency totals of
+R-MAT graphs of scales 14-18, then both pivot samplers are judged against
+exact Brandes on an R-MAT graph of --eval-scale; the value is the ratio of
+the sources uniform pivots need to reach top-1% overlap 0.95 to the sources
+learned pivots need).
 
 Prints one JSON line.  Units:
   Brandes GTEPS   = sum over sources of arcs scanned by the forward sweep
@@ -35,6 +42,113 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def bench_pivots(args):
+    """Learned vs uniform pivots for sampled Brandes (see the module doc)."""
+    import numpy as np
+    import torch
+
+    from zog_amd import _lib
+    from zog_amd import generators as gen
+    from zog_amd import pivots as PV
+    from zog_amd import resource_graph as rg
+    from zog_amd import workloads as W
+    from zog_amd.soa import SoA
+
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    X, T = PV.rmat_training_set(per_scale=args.train_per_scale, log=log)
+    t_label = time.time() - t0
+    log(f"[bench] {X.shape[0]} training rows labelled in {t_label:.1f}s")
+    sc = PV.Scorer(seed=7)
+    p = sc.params
+    p[-1] = float(T.mean())  # output bias starts at the mean target
+    sc.params = p
+    t0 = time.time()
+    losses = sc.fit(X, T, epochs=args.epochs, batch=_lib.ZG_MLP_MAX_BATCH, lr=1e-3, log=log)
+    t_train = time.time() - t0
+    train_rows = int(X.shape[0])
+    del X, T
+    scale = args.eval_scale
+    n, src, dst = gen.rmat(scale, 16 << scale, W.SEEDS["rmat22"] if scale == 22 else W.SEEDS["rmat20"] + scale)
+    g = rg.new(SoA.from_arrays("directed", n, src, dst))
+    V, n_live, E = g.resource.info()
+    live_ids = np.arange(V)
+    t0 = time.time()
+    info = {}
+    exact, true_tot = PV.bc_weighted(g, None, None, want_bc=True, want_totals=True, info=info)
+    t_exact = time.time() - t0
+    log(f"[bench] exact Brandes on R-MAT {scale}: {t_exact:.1f}s, {info['arcs_traversed'] / t_exact / 1e9:.1f} GTEPS")
+    pred, _ = PV.predict_totals(g, sc)
+    from scipy.stats import spearmanr
+    rho = float(spearmanr(pred, true_tot).statistic)
+    # inference throughput of the scorer on this graph's features
+    feats = PV.node_features(g)
+    sc.forward(feats)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sc.forward(feats)
+    mlp_s = (time.perf_counter() - t0) / reps
+    flops = 2.0 * V * (8 * 128 + 128 * 128 + 128)
+    rng = np.random.default_rng(W.SEEDS["rmat20"] + 99)
+    ks = []
+    k = 2048
+    while k < n_live:
+        ks.append(k)
+        k *= 2
+    # reference line: uniform over the sources that have successors (a
+    # source without successors contributes nothing)
+    nonsink = np.nonzero(np.bincount(src, minlength=V) > 0)[0]
+    curves = {"uniform": [], "learned": [], "uniform_nonsink": []}
+    for method in ("uniform", "learned", "uniform_nonsink"):
+        for k in ks:
+            ov, taus, used = [], [], []
+            for r in range(args.reps):
+                if method == "uniform_nonsink":
+                    est, m = PV.sampled_betweenness(g, k, rng, None, nonsink)
+                else:
+                    est, m = PV.sampled_betweenness(g, k, rng, pred if method == "learned" else None, live_ids)
+                ov.append(PV.top_overlap(est, exact))
+                used.append(m)
+                if r == 0:
+                    taus.append(PV.kendall_tau(est, exact))
+            curves[method].append({"k": k, "sources": float(np.mean(used)), "top1_overlap": float(np.mean(ov)),
+                                   "top1_overlap_min": float(np.min(ov)), "kendall_tau": taus[0]})
+            log(f"[bench] {method} k={k}: overlap {np.mean(ov):.4f} (min {np.min(ov):.4f}) tau {taus[0]:.4f}")
+
+    def needed(c):
+        for row in c:
+            if row["top1_overlap_min"] >= 0.95:
+                return row["sources"]
+        return None
+
+    nu, nl = needed(curves["uniform"]), needed(curves["learned"])
+    ratio = (nu / nl) if (nu and nl) else None
+    at2048 = {m: curves[m][0] for m in curves}
+    line = {
+        "metric": "learned-pivot sampled Brandes: sources to reach top-1% overlap >= 0.95, uniform / learned",
+        "value": round(ratio, 3) if ratio else None, "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(t_exact * 1000, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16-mfma/f64", "data": "synthetic",
+        "config": {"workload": f"pivots: train on {args.train_per_scale} R-MAT graphs per scale 14..18, "
+                               f"judge on R-MAT scale {scale} (exact Brandes over all sources)",
+                   "eval_nodes": V, "eval_arcs": E, "train_rows": train_rows,
+                   "epochs": args.epochs, "reps": args.reps},
+        "learned": {"uniform_sources_needed": nu, "learned_sources_needed": nl,
+                    "uniform_nonsink_sources_needed": needed(curves["uniform_nonsink"]), "at_2048": at2048,
+                    "overlap": "share of the top-1% estimates whose exact BC reaches the exact top-1% threshold",
+                    "curves": curves, "spearman_pred_vs_exact_totals": rho, "final_train_loss": losses[-1],
+                    "label_s": round(t_label, 1), "train_s": round(t_train, 1), "exact_s": round(t_exact, 1)},
+        "roofline": {"bound": "mfma", "achieved": round(flops / mlp_s / 1e12, 3), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(flops / mlp_s / 1e12 / 2500.0, 5), "traffic": None,
+                     "kernel": "mlp_forward_kernel", "avg_call_us": round(mlp_s * 1e6, 1),
+                     "note": "wall time per zg_mlp_forward call (launch and synchronisation included)"},
+    }
+    print(json.dumps(line), flush=True)
+    rg.destroy(g)
+    sc.close()
 
 
 def workload_spec(name):
@@ -73,7 +187,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512"])
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512", "pivots"])
+    ap.add_argument("--train-per-scale", type=int, default=40, help="pivots: R-MAT training graphs per scale 14..18")
+    ap.add_argument("--epochs", type=int, default=30, help="pivots: training epochs")
+    ap.add_argument("--eval-scale", type=int, default=20, help="pivots: R-MAT scale of the judged graph")
+    ap.add_argument("--reps", type=int, default=3, help="pivots: samples per pivot budget")
     ap.add_argument("--groups", type=int, default=4, help="64-source groups per launch")
     ap.add_argument("--pull-grid", type=int, default=None, help="pull workgroups per group (0 = one wave per chunk)")
     ap.add_argument("--alpha", type=float, default=None, help="push/pull switch factor")
@@ -91,6 +209,8 @@ def main():
     from zog_amd import resource_graph as rg
     from zog_amd.soa import SoA
 
+    if args.workload == "pivots":
+        return bench_pivots(args)
     spec = workload_spec(args.workload)
     if args.workload == "ba512":
         return bench_batch(args, spec)

@@ -8,6 +8,12 @@
  * ascending order, then appends u, the targets, and u another m-1 times.
  * Undirected: every edge is emitted as the arcs (u, v), (v, u).
  *
+ * R-MAT (Chakrabarti, Zhan & Faloutsos): each arc picks one quadrant per
+ * bit level with probabilities a, b, c, 1-a-b-c (source bit set in the lower
+ * half, target bit in the right half); arcs are drawn until n_arcs distinct
+ * (source, target) pairs exist and are kept in first-drawn order;
+ * self-loops are kept.
+ *
  * Random numbers: xoshiro256** (Blackman & Vigna) seeded by SplitMix64.
  */
 #include <stdint.h>
@@ -119,5 +125,54 @@ uint64_t zgen_barabasi_albert(uint32_t n, uint32_t m, uint64_t seed, uint32_t *s
     }
     free(deg);
     free(chosen);
+    return k;
+}
+
+/* Writes n_arcs distinct R-MAT arcs on 2^scale nodes; returns n_arcs, or 0
+ * for invalid parameters or when memory runs out. */
+uint64_t zgen_rmat(uint32_t scale, uint64_t n_arcs, uint64_t seed, double a, double b, double c, uint32_t *src,
+                   uint32_t *dst)
+{
+    if (scale < 1 || scale > 31 || !(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) return 0;
+    if (n_arcs > (1ull << scale) * (1ull << scale)) return 0;
+    /* open-addressing set of keys src<<32|dst (+1 so 0 marks empty) */
+    uint64_t cap = 1024;
+    while (cap < 2 * n_arcs + 1024) cap <<= 1;
+    uint64_t *set = (uint64_t *)calloc(cap, sizeof(uint64_t));
+    if (!set) return 0;
+    zgen_rng r;
+    rng_seed(&r, seed);
+    /* quadrant thresholds on a 32-bit uniform */
+    const double two32 = 4294967296.0;
+    const uint64_t ta = (uint64_t)(a * two32), tab = (uint64_t)((a + b) * two32), tabc = (uint64_t)((a + b + c) * two32);
+    uint64_t k = 0;
+    while (k < n_arcs) {
+        uint32_t s = 0, d = 0;
+        for (uint32_t lvl = 0; lvl < scale; lvl += 2) {
+            const uint64_t x = rng_next(&r);
+            for (uint32_t h = 0; h < 2 && lvl + h < scale; h++) {
+                const uint64_t u = (x >> (32 * h)) & 0xFFFFFFFFull;
+                const uint32_t bit = scale - 1 - (lvl + h);
+                const uint32_t sb = u >= tab, db = (u >= ta && u < tab) || u >= tabc;
+                s |= sb << bit;
+                d |= db << bit;
+            }
+        }
+        const uint64_t key = ((uint64_t)s << 32 | d) + 1;
+        uint64_t h = (key * 0x9E3779B97F4A7C15ull) >> 17;
+        for (;;) {
+            h &= cap - 1;
+            if (set[h] == key) break;
+            if (set[h] == 0) {
+                set[h] = key;
+                src[k] = s;
+                dst[k] = d;
+                k++;
+                break;
+            }
+            h++;
+        }
+    }
+    free(set);
     return k;
 }

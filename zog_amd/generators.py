@@ -61,32 +61,17 @@ def erdos_renyi_directed(n: int, p: float, seed: int):
 
 
 def rmat(scale: int, n_arcs: int, seed: int, a=0.57, b=0.19, c=0.19):
-    rng = _rng(seed)
+    """R-MAT on 2^scale nodes: n_arcs distinct arcs in first-drawn order
+    (self-loops kept), generated natively."""
+    import ctypes
+    L = _genlib()
     n = 1 << scale
-    n_unique = 0
-    first_pos = np.zeros(0, dtype=np.int64)
-    all_keys = []
-    while n_unique < n_arcs:
-        M = max(int((n_arcs - n_unique) * 1.15), 1 << 16)
-        src = np.zeros(M, dtype=np.uint64)
-        dst = np.zeros(M, dtype=np.uint64)
-        for lvl in range(scale):
-            u = rng.random(M)
-            sbit = (u >= a + b).astype(np.uint64)
-            dbit = (((u >= a) & (u < a + b)) | (u >= a + b + c)).astype(np.uint64)
-            sh = np.uint64(scale - 1 - lvl)
-            src |= sbit << sh
-            dst |= dbit << sh
-        keys = (src << np.uint64(32)) | dst
-        all_keys.append(keys)
-        cat = np.concatenate(all_keys)
-        uniq, first_pos = np.unique(cat, return_index=True)
-        n_unique = len(uniq)
-        all_keys = [cat]
-    order = np.sort(first_pos)[:n_arcs]
-    keys = all_keys[0][order]
-    src = (keys >> np.uint64(32)).astype(np.uint32)
-    dst = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    src = np.empty(n_arcs, dtype=np.uint32)
+    dst = np.empty(n_arcs, dtype=np.uint32)
+    got = L.zgen_rmat(scale, n_arcs, int(seed) & ((1 << 64) - 1), a, b, c,
+                      ctypes.c_void_p(src.ctypes.data), ctypes.c_void_p(dst.ctypes.data))
+    if got != n_arcs:
+        raise ValueError("invalid R-MAT parameters")
     return n, src, dst
 
 
@@ -128,6 +113,9 @@ def _genlib():
         L.zgen_barabasi_albert.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p,
                                            ctypes.c_void_p]
         L.zgen_barabasi_albert.restype = ctypes.c_uint64
+        L.zgen_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.zgen_rmat.restype = ctypes.c_uint64
         _gen_lib = L
     return _gen_lib
 

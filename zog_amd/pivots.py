@@ -171,7 +171,7 @@ def predicted_totals(y: np.ndarray, n_live: int) -> np.ndarray:
     return np.maximum(np.expm1(np.asarray(y, np.float64)), 0.0) * n_live
 
 
-def inclusion_probabilities(score: np.ndarray, k: int, eps: float = 0.1) -> np.ndarray:
+def inclusion_probabilities(score: np.ndarray, k: int, eps: float = 0.05) -> np.ndarray:
     """pi_s = min(1, c q_s) with q the (1-eps, eps) mixture of score/sum and
     uniform, c chosen so that sum pi = k (iterative capping)."""
     score = np.maximum(np.asarray(score, np.float64), 0.0)
@@ -202,7 +202,7 @@ def systematic_sample(pi: np.ndarray, order: np.ndarray, rng) -> np.ndarray:
     return order[picks > 0]
 
 
-def learned_pivots(pred_totals: np.ndarray, live_ids: np.ndarray, k: int, rng, eps: float = 0.1):
+def learned_pivots(pred_totals: np.ndarray, live_ids: np.ndarray, k: int, rng, eps: float = 0.05):
     """(sources, weights) of a learned-pivot sample of about k sources."""
     pi = inclusion_probabilities(pred_totals, k, eps)
     order = np.argsort(-pred_totals, kind="stable")
@@ -219,14 +219,94 @@ def uniform_pivots(live_ids: np.ndarray, k: int, rng):
 
 
 def top_overlap(est: np.ndarray, exact: np.ndarray, frac: float = 0.01) -> float:
-    """|top-k(est) & top-k(exact)| / k for k = ceil(frac * n) (ties by id)."""
+    """Share of the k = ceil(frac * n) highest estimates whose exact score
+    reaches the exact k-th largest (ties with the k-th value count as
+    hits, so the measure does not depend on how exact ties are broken)."""
+    exact = np.asarray(exact)
     n = len(exact)
     k = max(1, int(np.ceil(frac * n)))
-    te = np.lexsort((np.arange(n), -np.asarray(exact)))[:k]
-    tg = np.lexsort((np.arange(n), -np.asarray(est)))[:k]
-    return len(np.intersect1d(te, tg)) / k
+    kth = np.partition(exact, n - k)[n - k]
+    top_est = np.lexsort((np.arange(n), -np.asarray(est)))[:k]
+    return float(np.count_nonzero(exact[top_est] >= kth)) / k
 
 
 def kendall_tau(est: np.ndarray, exact: np.ndarray) -> float:
     from scipy.stats import kendalltau
     return float(kendalltau(est, exact).statistic)
+
+
+# ---------------------------------------------------------------------------
+# training data, sampled BC, evaluation
+
+
+def graph_labels(graph: ResourceGraph):
+    """(features [n_live][8] cuda, targets [n_live] cuda, live ids) of one
+    graph: the network's inputs and the exact per-source dependency totals
+    from the GPU Brandes sweep (every live vertex a source)."""
+    import torch
+    res = graph.resource
+    V, n_live, _ = res.info()
+    live_ids = np.nonzero(res.live_mask())[0].astype(np.int64)
+    feats = node_features(graph)
+    _, tot = bc_weighted(graph, None, None, want_bc=False, want_totals=True)
+    t = torch.from_numpy(dependency_target(tot, n_live)).cuda()
+    idx = torch.from_numpy(live_ids).cuda()
+    return feats.index_select(0, idx).contiguous(), t, live_ids
+
+
+def rmat_training_set(scales=(14, 15, 16, 17, 18), per_scale=40, seed=0x20C00004, edge_factor=16, log=None,
+                      workers=4):
+    """Training inputs and labels from per_scale R-MAT graphs at each scale
+    (a=.57 b=.19 c=.19, edge_factor * 2^scale distinct arcs), generated on
+    `workers` host threads while the GPU labels the previous ones."""
+    import concurrent.futures as cf
+
+    import torch
+
+    from . import generators as gen
+    from .soa import SoA
+    from . import resource_graph as rg
+    jobs = [(sc, seed + 1000 * sc + i) for sc in scales for i in range(per_scale)]
+    xs, ts = [], []
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = [ex.submit(gen.rmat, sc, edge_factor << sc, sd) for sc, sd in jobs]
+        for j, f in enumerate(futs):
+            n, src, dst = f.result()
+            g = rg.new(SoA.from_arrays("directed", n, src, dst))
+            x, t, _ = graph_labels(g)
+            rg.destroy(g)
+            xs.append(x)
+            ts.append(t)
+            if log and (j + 1) % max(1, len(jobs) // 10) == 0:
+                log(f"[pivots] labelled {j + 1}/{len(jobs)} training graphs")
+    return torch.cat(xs), torch.cat(ts)
+
+
+def predict_totals(graph: ResourceGraph, scorer: Scorer):
+    """Predicted dependency totals of every live vertex (id order) and the
+    live ids."""
+    res = graph.resource
+    V, n_live, _ = res.info()
+    live_ids = np.nonzero(res.live_mask())[0].astype(np.int64)
+    feats = node_features(graph)
+    y = scorer.forward(feats).cpu().numpy()[live_ids]
+    return predicted_totals(y, n_live), live_ids
+
+
+def sampled_betweenness(graph: ResourceGraph, k: int, rng, pred_totals=None, live_ids=None, eps: float = 0.05,
+                        info: Optional[dict] = None):
+    """Estimated BC from about k pivots: learned pivots when pred_totals is
+    given, uniform otherwise; exact Brandes when k covers every source."""
+    res = graph.resource
+    V, n_live, _ = res.info()
+    if live_ids is None:
+        live_ids = np.nonzero(res.live_mask())[0].astype(np.int64)
+    if k >= len(live_ids):
+        bc, _ = bc_weighted(graph, live_ids, None, info=info)
+        return bc, len(live_ids)
+    if pred_totals is None:
+        src, w = uniform_pivots(live_ids, k, rng)
+    else:
+        src, w = learned_pivots(pred_totals, live_ids, k, rng, eps)
+    bc, _ = bc_weighted(graph, src, w, info=info)
+    return bc, len(src)

@@ -53,6 +53,11 @@ class Resource:
         self._h = h
         self._lock = threading.Lock()
         self.n_cap = int(n_cap)
+        self._live = np.ones(self.n_cap, bool) if live is None else live[: self.n_cap].astype(bool)
+
+    def live_mask(self) -> np.ndarray:
+        """bool[n_cap]: the live ids (nodeIds, array_graph.zig:268-284)."""
+        return self._live
 
     @property
     def handle(self):
@@ -101,6 +106,7 @@ class BatchResource(Resource):
         self._h = h
         self._lock = threading.Lock()
         self.n_cap = int(ns.sum())
+        self._live = np.ones(self.n_cap, bool)
         self.offsets = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
 
 
