@@ -44,6 +44,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def bench_copy(args):
+    """A plain 1 GiB device-to-device copy: the known-size reference that
+    scales the FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
+    import torch
+    torch.cuda.set_device(0)
+    nbytes = 1 << 30
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    print(json.dumps({"metric": "device copy", "value": round(2 * nbytes / dt / 1e9, 1), "unit": "GB/s",
+                      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+                      "data": "synthetic", "config": {"workload": "copy: 1 GiB read + 1 GiB written per step"}}),
+          flush=True)
+
+
 def bench_pivots(args):
     """Learned vs uniform pivots for sampled Brandes (see the module doc)."""
     import numpy as np
@@ -187,7 +210,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512", "pivots"])
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512", "pivots", "copy"])
     ap.add_argument("--train-per-scale", type=int, default=40, help="pivots: R-MAT training graphs per scale 14..18")
     ap.add_argument("--epochs", type=int, default=30, help="pivots: training epochs")
     ap.add_argument("--eval-scale", type=int, default=20, help="pivots: R-MAT scale of the judged graph")
@@ -195,6 +218,7 @@ def main():
     ap.add_argument("--groups", type=int, default=4, help="64-source groups per launch")
     ap.add_argument("--pull-grid", type=int, default=None, help="pull workgroups per group (0 = one wave per chunk)")
     ap.add_argument("--alpha", type=float, default=None, help="push/pull switch factor")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -211,6 +235,8 @@ def main():
 
     if args.workload == "pivots":
         return bench_pivots(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     spec = workload_spec(args.workload)
     if args.workload == "ba512":
         return bench_batch(args, spec)
@@ -224,6 +250,14 @@ def main():
     g = rg.new(SoA.from_arrays("directed", n, src, dst))
     rg.set_batch_groups(g, args.groups)
     apply_tuning(g, args)
+    tuned = None
+    if not args.no_autotune and args.pull_grid is None and args.alpha is None and args.workload != "grid2048":
+        # batch width is part of the workload (64 * --groups sources per
+        # launch); the tuner picks the pull grid and the push/pull switch
+        from zog_amd import autotune
+        t0 = time.time()
+        res_t = autotune.tune(g.resource, sources[: 128 * args.groups], groups=(args.groups,), log=log)
+        tuned = dict(res_t["best"], seconds=round(time.time() - t0, 1))
     V, n_live, E = g.resource.info()
     log(f"[bench] resource built in {time.time() - t0:.1f}s")
     L = _lib.lib()
@@ -282,9 +316,16 @@ def main():
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get(dom)
+            ent = json.load(fh).get(dom)
+        if ent:
+            traffic = round(ent["hbm_bytes_per_dispatch"])
+    measured = {}
+    if traffic:
+        mgbs = traffic * dom_n / (dom_ms / 1000.0) / 1e9
+        measured = {"measured_hbm_GBs": round(mgbs, 1), "measured_frac": round(mgbs / HBM_PEAK_GBS, 4),
+                    "traffic_source": f"profiles/traffic_{args.workload}.json (FETCH_SIZE x2 + WRITE_SIZE x1)"}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, **measured,
                 "kernel": dom, "launches": dom_n, "avg_launch_us": round(dom_ms * 1000.0 / max(dom_n, 1), 2),
                 "algorithmic_bytes_per_step": dom_bytes,
                 "kernel_ms_per_step": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}
@@ -365,7 +406,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": spec["desc"], "nodes": n, "arcs": int(E), "sources": len(sources),
+        "config": {"workload": spec["desc"], "nodes": n, "arcs": int(E), "sources": len(sources), "autotuned": tuned,
                    "sources_per_launch": 64 * args.groups, "levels": levels, "arcs_per_step": arcs_step,
                    "push_levels": push_levels, "pull_levels": pull_levels},
         "roofline": roofline,
@@ -410,6 +451,15 @@ def bench_batch(args, spec):
         _lib.check(rc, "zg_set_batch_groups")
     apply_tuning(rg.ResourceGraph(resource=res, builder=None), args)
     log(f"[bench] batch resource built in {time.time() - t0:.1f}s")
+    tuned = None
+    if not args.no_autotune and args.pull_grid is None and args.alpha is None:
+        from zog_amd import autotune
+        t0 = time.time()
+        free, _ = torch.cuda.mem_get_info()
+        res_t = autotune.tune(res, autotune.batch_sample(res.offsets), groups=(16, 32, 64), alphas=(4.0, 8.0, 16.0, 32.0),
+                              pull_grids=(4096, 16384), repeats=1, max_state_bytes=0.5 * free,
+                              span_vertices=int(np.diff(res.offsets).max()), log=log)
+        tuned = dict(res_t["best"], seconds=round(time.time() - t0, 1))
     d_bc = torch.empty(n_tot, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
     info = _lib.ZgBcInfo()
@@ -487,7 +537,8 @@ def bench_batch(args, spec):
         "ms_per_step": round(wall * 1000.0 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": spec["desc"], "graphs": len(graphs), "nodes": n_tot, "arcs": arcs_tot,
-                   "sources": n_tot, "sources_per_launch": 64 * args.groups, "batches": int(info.batches),
+                   "sources": n_tot, "sources_per_launch": 64 * (tuned["groups"] if tuned else args.groups),
+                   "autotuned": tuned, "batches": int(info.batches),
                    "arcs_per_step": arcs_step, "levels": int(info.max_levels),
                    "push_levels": int(info.push_levels), "pull_levels": int(info.pull_levels)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

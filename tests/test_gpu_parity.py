@@ -446,3 +446,19 @@ def test_pagerank_max_iterations_zero_and_damping():
     assert info["iterations"] == 0 and np.allclose(got, 1.0 / n, rtol=1e-7)
     got, ref, it_ref, info = check_pr(n, src, dst, iters=100, tol=0.0, damping=0.99, precision=64)
     assert np.abs(got - ref).sum() <= 1e-12
+
+
+def test_autotuner_keeps_results():
+    # every candidate launch shape gives the oracle's BC; the tuner leaves the
+    # fastest one set on the handle
+    from oracle import oracle as O
+    from zog_amd import autotune
+    n, src, dst = gen.rmat(12, 1 << 15, 4321)
+    sources = gen.seeded_sources(n, src, dst, 300, 2)
+    ref, _ = O.Graph(n, src, dst).betweenness(sources=sources)
+    g = make(n, src, dst)
+    out = autotune.tune(g.resource, sources[:128], groups=(1, 2), alphas=(0.0, 8.0), pull_grids=(64, 0), repeats=1)
+    assert len(out["trials"]) == 8 and out["best"] in out["trials"]
+    got = rg.betweenness_unweighted_array(g, sources=sources)
+    assert_bc_close(got, ref)
+    rg.destroy(g)

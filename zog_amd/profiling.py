@@ -1,0 +1,110 @@
+"""Summaries of rocprofv3 CSV output for the measurement files.
+
+  python -m zog_amd.profiling stats <run_kernel_stats.csv>
+      markdown table of the kernel statistics (per kernel: calls, total ms,
+      average / min / max microseconds, share of device time)
+
+  python -m zog_amd.profiling traffic --fetch F.csv --write W.csv \
+      --copy-fetch CF.csv --copy-write CW.csv --copy-bytes 1073741824 -o out.json
+      HBM bytes per dispatch of every kernel from one FETCH_SIZE and one
+      WRITE_SIZE counter-collection run (KiB per dispatch), each scaled by
+      what the same counter reads for a plain copy of known size (copy
+      kernels: the largest dispatch of the calibration run); the Brandes
+      forward pull and push kernels are also summed under
+      "bc_forward_kernel+bc_push_kernel", as bench.py reports them together.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import sys
+from collections import defaultdict
+
+
+def _short(name: str) -> str:
+    """Kernel name without its argument list or template arguments."""
+    base = name.split("(")[0].strip()
+    if base.startswith("void "):
+        base = base[5:]
+    return base
+
+
+def kernel_stats_markdown(path: str) -> str:
+    rows = list(csv.DictReader(open(path)))
+    out = ["| Kernel | Calls | Total ms | Avg us | Min us | Max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        out.append("| `%s` | %s | %.2f | %.1f | %.1f | %.1f | %.2f |" % (
+            r["Name"][:70], r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
+            float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    return "\n".join(out)
+
+
+def _counter_per_kernel(path: str, counter: str):
+    """{kernel: (sum of counter over dispatches, dispatches)} and the
+    largest single dispatch value."""
+    tot = defaultdict(float)
+    cnt = defaultdict(int)
+    biggest = 0.0
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        v = float(r["Counter_Value"])
+        k = _short(r["Kernel_Name"])
+        tot[k] += v
+        cnt[k] += 1
+        biggest = max(biggest, v)
+    return tot, cnt, biggest
+
+
+def traffic(fetch, write, copy_fetch, copy_write, copy_bytes):
+    ft, fc, _ = _counter_per_kernel(fetch, "FETCH_SIZE")
+    wt, wc, _ = _counter_per_kernel(write, "WRITE_SIZE")
+    _, _, cf = _counter_per_kernel(copy_fetch, "FETCH_SIZE")
+    _, _, cw = _counter_per_kernel(copy_write, "WRITE_SIZE")
+    # counters are in KiB; a copy of copy_bytes reads and writes copy_bytes
+    f_scale = copy_bytes / (cf * 1024.0)
+    w_scale = copy_bytes / (cw * 1024.0)
+    out = {"_calibration": {"copy_bytes": copy_bytes, "copy_fetch_kib": cf, "copy_write_kib": cw,
+                            "fetch_factor": f_scale, "write_factor": w_scale}}
+    for k in sorted(set(ft) | set(wt)):
+        n = max(fc.get(k, 0), wc.get(k, 0))
+        if not n:
+            continue
+        fb = ft.get(k, 0.0) * 1024.0 * f_scale / max(fc.get(k, 1), 1)
+        wb = wt.get(k, 0.0) * 1024.0 * w_scale / max(wc.get(k, 1), 1)
+        out[k] = {"dispatches": n, "fetch_bytes_per_dispatch": fb, "write_bytes_per_dispatch": wb,
+                  "hbm_bytes_per_dispatch": fb + wb}
+    a, b = "bc_forward_kernel", "bc_push_kernel"
+    if a in out and b in out:
+        n = out[a]["dispatches"] + out[b]["dispatches"]
+        tot = out[a]["hbm_bytes_per_dispatch"] * out[a]["dispatches"] + \
+            out[b]["hbm_bytes_per_dispatch"] * out[b]["dispatches"]
+        out[a + "+" + b] = {"dispatches": n, "hbm_bytes_per_dispatch": tot / n}
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
+    t.add_argument("--copy-bytes", type=int, default=1 << 30)
+    t.add_argument("-o", "--out", required=True)
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        print(kernel_stats_markdown(a.csv))
+    else:
+        res = traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes)
+        with open(a.out, "w") as fh:
+            json.dump(res, fh, indent=1)
+        json.dump(res, sys.stdout, indent=1)
+
+
+if __name__ == "__main__":
+    main()
