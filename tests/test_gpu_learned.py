@@ -161,3 +161,40 @@ def test_mlp_training_reduces_loss():
     ref = -1e-2 * (g / (np.abs(g) + 1e-8))  # bias-corrected first step: m/sqrt(v) = sign(g)
     assert np.allclose(step, ref, atol=2e-6)
     sc.close()
+
+
+def test_learned_pivots_gain_over_uniform():
+    # train the scorer on small R-MAT graphs (labels from the GPU sweep),
+    # then at the same source budget learned pivots must beat uniform pivots
+    # on the top-1% overlap with exact Brandes, and rank sources well
+    from scipy.stats import spearmanr
+    say("learned pivots")
+    X, T = PV.rmat_training_set(scales=(10, 11, 12), per_scale=4, seed=77, workers=2)
+    sc = PV.Scorer(seed=5)
+    p = sc.params
+    p[-1] = float(T.mean())
+    sc.params = p
+    losses = sc.fit(X, T, epochs=40, batch=8192, lr=2e-3)
+    assert losses[-1] < losses[0]
+    n, s, d = gen.rmat(13, 16 << 13, 4242)
+    g = make(n, s, d)
+    exact, tot = PV.bc_weighted(g, None, None, want_bc=True, want_totals=True)
+    pred, live_ids = PV.predict_totals(g, sc)
+    rho = spearmanr(pred, tot).statistic
+    rng = np.random.default_rng(3)
+    k = n // 2
+    ov = {"uniform": [], "learned": []}
+    for _ in range(4):
+        est, m = PV.sampled_betweenness(g, k, rng, None, live_ids)
+        ov["uniform"].append(PV.top_overlap(est, exact))
+        est, m = PV.sampled_betweenness(g, k, rng, pred, live_ids)
+        assert abs(m - k) <= 1
+        ov["learned"].append(PV.top_overlap(est, exact))
+    say("spearman", rho, "overlap uniform", np.mean(ov["uniform"]), "learned", np.mean(ov["learned"]))
+    assert rho > 0.8
+    assert np.mean(ov["learned"]) > np.mean(ov["uniform"]) + 0.02
+    # the full budget falls back to exact Brandes
+    est, m = PV.sampled_betweenness(g, n, rng, pred, live_ids)
+    assert m == n and np.allclose(est, exact, rtol=1e-12, atol=0)
+    rg.destroy(g)
+    sc.close()

@@ -115,3 +115,42 @@ def test_features_closed_form():
     assert np.allclose(f2[:, :6].mean(0), 0.0, atol=1e-12)
     sd = f2[:, :6].std(0)
     assert np.all((np.abs(sd - 1) < 1e-9) | (sd < 1e-12))
+
+
+def test_inclusion_probabilities_and_systematic_sample():
+    from zog_amd import pivots as PV
+    rng = np.random.default_rng(0)
+    score = np.concatenate([rng.exponential(1.0, 900), np.zeros(100), [1e6]])
+    for k in (1, 10, 200, 1000):
+        pi = PV.inclusion_probabilities(score, k, eps=0.05)
+        assert abs(pi.sum() - k) < 1e-6 and np.all(pi <= 1.0) and np.all(pi > 0)
+        if k >= 10:
+            assert pi[-1] == 1.0  # the dominant score is taken for sure
+        order = np.argsort(-score, kind="stable")
+        picks = [PV.systematic_sample(pi, order, np.random.default_rng(r)) for r in range(200)]
+        assert all(abs(len(p) - k) <= 1 and len(set(p.tolist())) == len(p) for p in picks)
+        freq = np.bincount(np.concatenate(picks), minlength=len(score)) / 200
+        assert np.abs(freq - pi).max() < 0.15  # empirical inclusion frequency tracks pi
+    # Horvitz-Thompson: unbiased for a fixed vector of per-source values
+    vals = rng.random(len(score))
+    pi = PV.inclusion_probabilities(score, 300, eps=0.2)
+    order = np.argsort(-score, kind="stable")
+    ests = []
+    for r in range(3000):
+        p = PV.systematic_sample(pi, order, np.random.default_rng(10_000 + r))
+        ests.append(np.sum(vals[p] / pi[p]))
+    assert abs(np.mean(ests) - vals.sum()) < 0.01 * vals.sum()
+
+
+def test_top_overlap_ties_and_uniform_pivots():
+    from zog_amd import pivots as PV
+    exact = np.array([5.0, 4.0, 4.0, 4.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0] * 10)
+    # the top 1% (1 node) by estimate is any node tied at the exact maximum
+    est = exact.copy()
+    assert PV.top_overlap(est, exact, frac=0.01) == 1.0
+    # top 4%: threshold 4.0; estimates that pick tied nodes in another order still hit
+    est2 = exact + np.linspace(0, 0.1, len(exact))
+    assert PV.top_overlap(est2, exact, frac=0.04) == 1.0
+    assert PV.top_overlap(-exact, exact, frac=0.04) == 0.0
+    src, w = PV.uniform_pivots(np.arange(100), 30, np.random.default_rng(1))
+    assert len(set(src.tolist())) == 30 and np.allclose(w, 100 / 30)
