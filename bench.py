@@ -256,7 +256,8 @@ def main():
         # launch); the tuner picks the pull grid and the push/pull switch
         from zog_amd import autotune
         t0 = time.time()
-        res_t = autotune.tune(g.resource, sources[: 128 * args.groups], groups=(args.groups,), log=log)
+        res_t = autotune.tune(g.resource, sources[: 128 * args.groups], groups=(args.groups,), orders=(1, 2),
+                              log=log)
         tuned = dict(res_t["best"], seconds=round(time.time() - t0, 1))
     V, n_live, E = g.resource.info()
     log(f"[bench] resource built in {time.time() - t0:.1f}s")
@@ -456,8 +457,8 @@ def bench_batch(args, spec):
         from zog_amd import autotune
         t0 = time.time()
         free, _ = torch.cuda.mem_get_info()
-        res_t = autotune.tune(res, autotune.batch_sample(res.offsets), groups=(16, 32, 64), alphas=(4.0, 8.0, 16.0, 32.0),
-                              pull_grids=(4096, 16384), repeats=1, max_state_bytes=0.5 * free,
+        res_t = autotune.tune(res, autotune.batch_sample(res.offsets), groups=(32, 64), alphas=(4.0, 8.0, 16.0),
+                              pull_grids=(4096,), orders=(1, 2), repeats=1, max_state_bytes=0.5 * free,
                               span_vertices=int(np.diff(res.offsets).max()), log=log)
         tuned = dict(res_t["best"], seconds=round(time.time() - t0, 1))
     d_bc = torch.empty(n_tot, dtype=torch.float64, device=dev)

@@ -180,8 +180,10 @@ int zg_set_tuning(zg_graph *g, int param, double value);
  * result is the same sum, only FP64 summation order changes).  mode 0 keeps
  * the caller's order; mode 1 (default) sorts the sources by their distance to
  * the graph's hub core (the highest in-degree vertices) so the 64 sources of
- * a group reach most vertices at the same BFS level.  Returns the previous
- * mode, or <0. */
+ * a group reach most vertices at the same BFS level; mode 2 sorts them by
+ * their discovery rank in a breadth-first search from the hub core over
+ * predecessor arcs, so a group holds sources that are close to each other.
+ * Returns the previous mode, or <0. */
 int zg_set_source_order(zg_graph *g, int mode);
 
 /* Kernel-level profile of the most recent Brandes / PageRank call (no

@@ -197,7 +197,7 @@ def test_batch_widths_and_source_order_agree():
     n, src, dst = gen.rmat(11, 1 << 14, 3)
     sources = np.arange(n, dtype=np.uint32)
     outs = []
-    for gr, order in ((1, 1), (3, 0), (16, 1), (4, 0)):
+    for gr, order in ((1, 1), (3, 0), (16, 1), (4, 0), (5, 2), (64, 2)):
         g = make(n, src, dst, groups=gr)
         rg.set_source_order(g, order)
         outs.append(rg.betweenness_unweighted_array(g, sources=sources))
@@ -270,7 +270,7 @@ def test_batch_groups_spanning_several_graphs():
     rng = np.random.default_rng(5)
     one = np.arange(off[1], off[2], dtype=np.uint32)
     scattered = rng.permutation(n)[:700].astype(np.uint32)
-    for order in (1, 0):
+    for order in (1, 0, 2):
         rg.set_source_order(g, order)
         for sources in (one, scattered, one):
             ref, _, arcs = og.betweenness(sources=sources, return_arcs=True)
@@ -457,8 +457,9 @@ def test_autotuner_keeps_results():
     sources = gen.seeded_sources(n, src, dst, 300, 2)
     ref, _ = O.Graph(n, src, dst).betweenness(sources=sources)
     g = make(n, src, dst)
-    out = autotune.tune(g.resource, sources[:128], groups=(1, 2), alphas=(0.0, 8.0), pull_grids=(64, 0), repeats=1)
-    assert len(out["trials"]) == 8 and out["best"] in out["trials"]
+    out = autotune.tune(g.resource, sources[:128], groups=(1, 2), alphas=(0.0, 8.0), pull_grids=(64, 0),
+                        orders=(1, 2), repeats=1)
+    assert len(out["trials"]) == 16 and out["best"] in out["trials"]
     got = rg.betweenness_unweighted_array(g, sources=sources)
     assert_bc_close(got, ref)
     rg.destroy(g)

@@ -6,7 +6,9 @@ The candidates are the knobs the library exposes per handle:
   * the push/pull switch factor alpha (a level pushes from its frontier when
     the frontier's successor arcs times alpha are below the arcs of the
     group's span; ZG_TUNE_PUSH_ALPHA),
-  * workgroups per group of the pull kernel (ZG_TUNE_PULL_GRID).
+  * workgroups per group of the pull kernel (ZG_TUNE_PULL_GRID),
+  * the source grouping (zg_set_source_order: by distance to the hub core, or
+    by discovery rank in a search from it).
 Each candidate sweeps a fixed sample of the sources (the same sample for all)
 on the device; the tuner keeps the one with the most traversed arcs per
 kernel millisecond (info.kernel_ms is HIP-event time of the sweep) and leaves
@@ -25,8 +27,11 @@ from . import _lib
 from .resource_graph import _ptr
 
 
-def _set(handle, groups, alpha, pull_grid):
+def _set(handle, groups, alpha, pull_grid, order):
     L = _lib.lib()
+    rc = L.zg_set_source_order(handle, int(order))
+    if rc < 0:
+        _lib.check(rc, "zg_set_source_order")
     rc = L.zg_set_batch_groups(handle, int(groups))
     if rc < 0:
         _lib.check(rc, "zg_set_batch_groups")
@@ -43,9 +48,9 @@ def _sweep(handle, n_cap, sources):
 
 
 def tune(resource, sample: np.ndarray, groups: Sequence[int] = (4,), alphas: Sequence[float] = (4.0, 8.0, 16.0),
-         pull_grids: Sequence[int] = (4096, 16384, 0), repeats: int = 2, max_state_bytes: Optional[float] = None,
-         span_vertices: Optional[int] = None, log=None) -> dict:
-    """Times every (groups, alpha, pull_grid) candidate on `sample` (u32
+         pull_grids: Sequence[int] = (4096, 16384, 0), orders: Sequence[int] = (1,), repeats: int = 2,
+         max_state_bytes: Optional[float] = None, span_vertices: Optional[int] = None, log=None) -> dict:
+    """Times every (groups, alpha, pull_grid, order) candidate on `sample` (u32
     source ids of `resource`) and sets the fastest.  Candidates whose
     per-group state (about 1.6 KiB per vertex of a group's span, at most
     span_vertices, default all) would exceed max_state_bytes are skipped.  Returns {"best": {...}, "trials": [...]}."""
@@ -54,10 +59,10 @@ def tune(resource, sample: np.ndarray, groups: Sequence[int] = (4,), alphas: Seq
     n_cap = resource.n_cap
     trials = []
     best = None
-    for gr, al, pg in itertools.product(groups, alphas, pull_grids):
+    for gr, al, pg, od in itertools.product(groups, alphas, pull_grids, orders):
         if max_state_bytes is not None and gr * 1.6 * 1024 * (span_vertices or n_cap) > max_state_bytes:
             continue
-        _set(h, gr, al, pg)
+        _set(h, gr, al, pg, od)
         _sweep(h, n_cap, sample)  # warm: workspace sized and first-touch done
         ms, arcs = [], 0
         for _ in range(repeats):
@@ -66,7 +71,8 @@ def tune(resource, sample: np.ndarray, groups: Sequence[int] = (4,), alphas: Seq
             arcs = inf.arcs_traversed
         t = float(np.median(ms))
         rate = arcs / (t * 1e-3) / 1e9 if t > 0 else 0.0
-        row = {"groups": gr, "alpha": al, "pull_grid": pg, "kernel_ms": round(t, 3), "gteps": round(rate, 2)}
+        row = {"groups": gr, "alpha": al, "pull_grid": pg, "order": od, "kernel_ms": round(t, 3),
+               "gteps": round(rate, 2)}
         trials.append(row)
         if log:
             log(f"[autotune] {row}")
@@ -74,7 +80,7 @@ def tune(resource, sample: np.ndarray, groups: Sequence[int] = (4,), alphas: Seq
             best = row
     if best is None:
         raise ValueError("no candidate fits")
-    _set(h, best["groups"], best["alpha"], best["pull_grid"])
+    _set(h, best["groups"], best["alpha"], best["pull_grid"], best["order"])
     return {"best": best, "trials": trials}
 
 

@@ -1181,9 +1181,48 @@ static int bc_core_distances(zg_graph *g, hipStream_t s)
     return rc;
 }
 
+// Discovery rank of every vertex in a breadth-first search from the hub core
+// over predecessor arcs (the search that fills the core distances; siblings
+// get neighbouring ranks); vertices it does not reach follow in id order.
+static int bc_bfs_rank(zg_graph *g, hipStream_t s)
+{
+    if (!g->h_bfsrank.empty() || g->V == 0) return ZG_OK;
+    ZG_TRY_RC(bc_core_distances(g, s));
+    const uint32_t V = g->V;
+    std::vector<uint32_t> col(g->E);
+    if (g->E) {
+        ZG_TRY(hipMemcpyAsync(col.data(), g->in_col, sizeof(uint32_t) * g->E, hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+    }
+    const std::vector<uint32_t> &ip = g->h_in_ptr;
+    std::vector<uint32_t> rank(V, ZG_NONE), queue;
+    queue.reserve(V);
+    for (uint32_t v = 0; v < V; v++)
+        if (g->h_coredist[v] == 0) {
+            rank[v] = (uint32_t)queue.size();
+            queue.push_back(v);
+        }
+    for (size_t h = 0; h < queue.size(); h++) {
+        const uint32_t w = queue[h];
+        for (uint32_t e = ip[w]; e < ip[w + 1]; e++) {
+            const uint32_t u = col[e];
+            if (rank[u] == ZG_NONE && g->h_live[u]) {
+                rank[u] = (uint32_t)queue.size();
+                queue.push_back(u);
+            }
+        }
+    }
+    uint32_t next = (uint32_t)queue.size();
+    for (uint32_t v = 0; v < V; v++)
+        if (rank[v] == ZG_NONE) rank[v] = next++;
+    g->h_bfsrank.swap(rank);
+    return ZG_OK;
+}
+
 // Order in which the sources are swept: each 64-source group holds sources
 // of one graph (component) at the same distance from its hub core (then
-// larger out-degree first, then id).  Components are contiguous id ranges,
+// larger out-degree first, then id), or (mode 2) in the order a search from
+// the hub core discovers them.  Components are contiguous id ranges,
 // so comparing ids across components orders by component.  Returns the
 // positions of src in sweep order.
 static void bc_order_perm(const zg_graph *g, const std::vector<uint32_t> &src, std::vector<uint64_t> &perm)
@@ -1193,12 +1232,14 @@ static void bc_order_perm(const zg_graph *g, const std::vector<uint32_t> &src, s
     auto comp = [&](uint32_t x) { return std::upper_bound(co.begin(), co.end(), x) - co.begin(); };
     perm.resize(src.size());
     for (size_t i = 0; i < perm.size(); i++) perm[i] = i;
+    const bool by_rank = g->source_order == 2;
     std::stable_sort(perm.begin(), perm.end(), [&](uint64_t i, uint64_t j) {
         const uint32_t x = src[i], y = src[j];
         if (co.size() > 2) {
             const auto cx = comp(x), cy = comp(y);
             if (cx != cy) return cx < cy;
         }
+        if (by_rank) return g->h_bfsrank[x] < g->h_bfsrank[y];
         if (cd[x] != cd[y]) return cd[x] < cd[y];
         if (g->h_out_deg[x] != g->h_out_deg[y]) return g->h_out_deg[x] > g->h_out_deg[y];
         return x < y;
@@ -1247,6 +1288,7 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
     const bool order = g->source_order && n > 64;
     // engine: the per-source queue engine for high-diameter graphs
     if (g->engine == ZG_ENGINE_AUTO || order) ZG_TRY_RC(bc_core_distances(g, s));
+    if (order && g->source_order == 2) ZG_TRY_RC(bc_bfs_rank(g, s));
     const bool queue_engine = g->engine == ZG_ENGINE_QUEUE || (g->engine == ZG_ENGINE_AUTO && g->high_diameter);
     std::vector<uint64_t> perm;
     if (order) {

@@ -163,6 +163,7 @@ struct zg_graph {
     std::vector<uint8_t> h_live;
     std::vector<uint32_t> h_out_deg, h_in_deg;
     std::vector<uint32_t> h_coredist;    // distance to the hub core (lazy)
+    std::vector<uint32_t> h_bfsrank;     // discovery rank in a search from the hub core (lazy)
     int source_order = 1;
     double push_alpha = 8.0;
     uint32_t n_dangling = 0;

@@ -413,7 +413,7 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
 extern "C" int zg_set_source_order(zg_graph *g, int mode)
 {
     if (!g) return ZG_ERR_INVALID_HANDLE;
-    if (mode != 0 && mode != 1) return ZG_ERR_INVALID_ARG;
+    if (mode < 0 || mode > 2) return ZG_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(g->mu);
     int prev = g->source_order;
     g->source_order = mode;

@@ -365,7 +365,8 @@ def set_engine(graph: ResourceGraph, engine: str) -> None:
 
 def set_source_order(graph: ResourceGraph, mode: int) -> int:
     """0 = sweep sources in the given order, 1 = group them by distance to the
-    hub core (default).  Returns the previous mode."""
+    hub core (default), 2 = by discovery rank in a search from the hub core.
+    Returns the previous mode."""
     rc = _lib.lib().zg_set_source_order(graph.resource.handle, int(mode))
     if rc < 0:
         _lib.check(rc, "zg_set_source_order")
