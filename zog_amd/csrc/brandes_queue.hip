@@ -370,7 +370,19 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
                  double *d_out, hipStream_t s, zg_bc_info *inf)
 {
     const uint32_t V = g->V;
-    const uint32_t S = std::min<uint32_t>(64u * (uint32_t)g->groups, ZG_Q_MAX_SOURCES);
+    // sources per batch: as many as the call has (rounded up to 64), at most
+    // ZG_Q_MAX_SOURCES, and as many as half the free device memory holds at
+    // 32 bytes of state per source and vertex; an existing larger workspace
+    // is kept
+    const uint32_t want = (uint32_t)std::min<uint64_t>(ZG_Q_MAX_SOURCES, (n + 63) / 64 * 64);
+    uint32_t S = g->bcq.S;
+    if (S < want) {
+        size_t fr = 0, tot = 0;
+        ZG_TRY(hipMemGetInfo(&fr, &tot));
+        fr += (size_t)g->bcq.S * V * 32;  // released before the new one is made
+        const uint64_t fit = (uint64_t)(0.5 * (double)fr) / (32ull * (V ? V : 1));
+        S = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(want, fit) / 64 * 64);
+    }
     ZG_TRY_RC(bcq_ws_ensure(g, S));
     zg_bcq_ws &w = g->bcq;
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));

@@ -2,7 +2,9 @@
 // is summed by a group of L lanes (one wavefront holds 64/L rows) so short
 // rows do not leave most of a wavefront idle.
 #pragma once
+#ifndef ZG_PR_SMALL
 #define ZG_PR_SMALL 16     // <= 16 predecessors: one lane per row
+#endif
 #define ZG_PR_M4 64        // <= 64: 4 lanes per row
 #define ZG_PR_M16 256      // <= 256: 16 lanes per row
 #define ZG_PR_MED 2048     // <= 2048: 64 lanes per row
