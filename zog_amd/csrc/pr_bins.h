@@ -3,10 +3,14 @@
 // rows do not leave most of a wavefront idle.
 #pragma once
 #ifndef ZG_PR_SMALL
-#define ZG_PR_SMALL 16     // <= 16 predecessors: one lane per row
+#define ZG_PR_SMALL 4      // <= 4 predecessors: one lane per row
 #endif
+#ifndef ZG_PR_M4
 #define ZG_PR_M4 64        // <= 64: 4 lanes per row
+#endif
+#ifndef ZG_PR_M16
 #define ZG_PR_M16 256      // <= 256: 16 lanes per row
+#endif
 #define ZG_PR_MED 2048     // <= 2048: 64 lanes per row
 #define ZG_PR_SEG 4096     // longer rows: 256-lane blocks over 4096-arc segments
 #define ZG_PR_BLOCK 256
