@@ -189,7 +189,10 @@ struct zg_graph {
     uint32_t n_m4 = 0, n_m16 = 0, n_m64 = 0;   // n_med = n_m4 + n_m16 + n_m64
     uint32_t *pr_small = nullptr, *pr_med = nullptr;   // pr_med = [m4 | m16 | m64]
     uint4 *pr_desc = nullptr;            // [small | m4 | m16 | m64] rows: {v, first arc, in-deg, out-deg}
-    uint32_t *pr_large = nullptr;        // [n_large] rows
+    uint32_t *pr_large = nullptr;        // [n_large] rows (PageRank ids)
+    uint32_t *pr_large_deg = nullptr;    // [n_large] out-degrees of the large rows
+    uint32_t *pr_id = nullptr;           // [V] PageRank id of each vertex (by decreasing out-degree)
+    uint32_t *pr_col = nullptr;          // [E] predecessor CSR columns as PageRank ids
     uint32_t *pr_large_seg0 = nullptr;   // [n_large + 1] first segment of each large row
     uint32_t *pr_seg_row = nullptr;      // [n_seg] large-row index of each segment
     uint32_t *pr_seg_e0 = nullptr, *pr_seg_e1 = nullptr;

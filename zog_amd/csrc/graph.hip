@@ -107,6 +107,7 @@ static void zg_free_device(zg_graph *g)
     zg_dfree(g->ohub_idx);
     zg_dfree(g->pr_small); zg_dfree(g->pr_med); zg_dfree(g->pr_desc); zg_dfree(g->pr_large); zg_dfree(g->pr_large_seg0);
     zg_dfree(g->pr_seg_row); zg_dfree(g->pr_seg_e0); zg_dfree(g->pr_seg_e1);
+    zg_dfree(g->pr_large_deg); zg_dfree(g->pr_id); zg_dfree(g->pr_col);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     g->stream = nullptr;
 }
@@ -220,8 +221,20 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     g->max_och_per_v = max_k;
     g->n_ohub = n_ohub;
 
+    // PageRank ids: vertices by decreasing out-degree, so the rank values
+    // gathered most often share cache lines; the predecessor columns are
+    // relabelled in place (each row keeps its arc order).
+    std::vector<uint32_t> pr_order(V), pr_id(V);
+    for (uint32_t v = 0; v < V; v++) pr_order[v] = v;
+    std::stable_sort(pr_order.begin(), pr_order.end(), [&](uint32_t x, uint32_t y) {
+        return out_ptr[(size_t)x + 1] - out_ptr[x] > out_ptr[(size_t)y + 1] - out_ptr[y];
+    });
+    for (uint32_t i = 0; i < V; i++) pr_id[pr_order[i]] = i;
+    std::vector<uint32_t> pr_col(in_col.size());
+    for (size_t e = 0; e < in_col.size(); e++) pr_col[e] = pr_id[in_col[e]];
+
     // PageRank row bins over live rows by predecessor count.
-    std::vector<uint32_t> small, m4, m16, m64, large, large_seg0, seg_row, seg_e0, seg_e1;
+    std::vector<uint32_t> small, m4, m16, m64, large, large_deg, large_seg0, seg_row, seg_e0, seg_e1;
     for (uint32_t v = 0; v < V; v++) {
         if (!g->h_live[v]) continue;
         uint32_t d = in_ptr[(size_t)v + 1] - in_ptr[v];
@@ -236,7 +249,8 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
                 seg_e0.push_back(c);
                 seg_e1.push_back(std::min<uint32_t>(c + ZG_PR_SEG, in_ptr[(size_t)v + 1]));
             }
-            large.push_back(v);
+            large.push_back(pr_id[v]);
+            large_deg.push_back(out_ptr[(size_t)v + 1] - out_ptr[v]);
         }
     }
     large_seg0.push_back((uint32_t)seg_row.size());
@@ -252,7 +266,8 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     desc.reserve(small.size() + med.size());
     for (const std::vector<uint32_t> *bin : {&small, &med})
         for (uint32_t v : *bin)
-            desc.push_back(make_uint4(v, in_ptr[v], in_ptr[(size_t)v + 1] - in_ptr[v], out_ptr[(size_t)v + 1] - out_ptr[v]));
+            desc.push_back(make_uint4(pr_id[v], in_ptr[v], in_ptr[(size_t)v + 1] - in_ptr[v],
+                                      out_ptr[(size_t)v + 1] - out_ptr[v]));
     g->n_large = (uint32_t)large.size();
     g->n_seg = (uint32_t)seg_row.size();
 
@@ -275,6 +290,9 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     ZG_TRY_RC(zg_upload(&g->pr_med, med));
     ZG_TRY_RC(zg_upload(&g->pr_desc, desc));
     ZG_TRY_RC(zg_upload(&g->pr_large, large));
+    ZG_TRY_RC(zg_upload(&g->pr_large_deg, large_deg));
+    ZG_TRY_RC(zg_upload(&g->pr_id, pr_id));
+    ZG_TRY_RC(zg_upload(&g->pr_col, pr_col));
     ZG_TRY_RC(zg_upload(&g->pr_large_seg0, large_seg0));
     ZG_TRY_RC(zg_upload(&g->pr_seg_row, seg_row));
     ZG_TRY_RC(zg_upload(&g->pr_seg_e0, seg_e0));

@@ -7,7 +7,10 @@
 //   swap, then stop when l1 < tolerance (strict)                       (:952-957)
 //
 // gfx950 layout: ranks r and the gathered values x[u] = r[u]/deg(u) are FP32
-// (or FP64 in the FP64 mode), double-buffered by iteration parity.  Each row
+// (or FP64 in the FP64 mode), double-buffered by iteration parity, indexed by
+// PageRank id (vertices by decreasing out-degree: the values gathered most
+// often share cache lines; rows keep the reference's arc order, so the sums
+// are unchanged).  Each row
 // sum is a compensated (Kahan) sum; partial sums of one row computed by
 // different lanes are merged with an error-free TwoSum.  The l1 change and the
 // dangling mass are reduced in FP64: every block writes its partials, and a
@@ -36,8 +39,8 @@ struct PrState {
 
 template <typename T>
 struct PrArgs {
-    const uint32_t *in_ptr, *in_col, *out_ptr;
-    const uint32_t *small, *med, *large, *large_seg0, *seg_e0, *seg_e1;
+    const uint32_t *in_ptr, *in_col, *out_ptr;   // in_col: predecessors as PageRank ids
+    const uint32_t *small, *med, *large, *large_deg, *large_seg0, *seg_e0, *seg_e1;
     const uint4 *desc;          // [small | m4 | m16 | m64]: {v, first arc, in-degree, out-degree}
     uint32_t n_small, n_m4, n_m16, n_m64, n_large, n_seg;
     uint32_t nb_small, nb_m4, nb_m16, nb_m64;  // work units of each bin
@@ -95,10 +98,6 @@ __device__ __forceinline__ void pr_strided_kahan(const uint32_t *__restrict__ co
     for (; e < e1; e += step) kahan_add(s, c, x[col[e]]);
 }
 
-template <typename T>
-__device__ __forceinline__ void pr_row_epilogue(const PrArgs<T> &a, const T *rold, T *rnew, T *xnew,
-                                                uint32_t v, T s, T c, double base, double damping,
-                                                double &l1, double &dang);
 
 // Same, with the old rank and the out-degree already loaded.
 template <typename T>
@@ -138,22 +137,6 @@ __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *r
         kahan_merge(s, c, s2, c2);
     }
     if (ok && sub == 0) pr_row_finish(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
-}
-
-// New rank of row v from its compensated predecessor sum (value s - c);
-// accumulates the FP64 l1 change and dangling mass of this thread.
-template <typename T>
-__device__ __forceinline__ void pr_row_epilogue(const PrArgs<T> &a, const T *rold, T *rnew, T *xnew,
-                                                uint32_t v, T s, T c, double base, double damping,
-                                                double &l1, double &dang)
-{
-    double sum = (double)s - (double)c;
-    T rt = (T)(base + damping * sum);
-    rnew[v] = rt;
-    l1 += fabs((double)rt - (double)rold[v]);
-    uint32_t deg = a.out_ptr[v + 1] - a.out_ptr[v];
-    if (deg == 0) dang += (double)rt;
-    else xnew[v] = rt / (T)deg;
 }
 
 template <typename T>
@@ -266,7 +249,8 @@ __global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<
         const uint32_t s0 = a.large_seg0[i], s1 = a.large_seg0[i + 1];
         T S = a.seg[2 * s0], C = a.seg[2 * s0 + 1];
         for (uint32_t q = s0 + 1; q < s1; q++) kahan_merge(S, C, a.seg[2 * q], a.seg[2 * q + 1]);
-        pr_row_epilogue(a, rold, rnew, xnew, a.large[i], S, C, base, damping, l1, dang);
+        const uint32_t v = a.large[i];
+        pr_row_finish(rnew, xnew, v, rold[v], a.large_deg[i], S, C, base, damping, l1, dang);
     }
     for (uint32_t q = tid; q < nparts; q += ZG_PR_FINISH_THREADS) {
         l1 += a.part[2 * q];
@@ -294,27 +278,30 @@ __global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<
 }
 
 // r <- src (uniform 1/n when src == nullptr) on live rows, 0 elsewhere;
-// x = r/deg on live rows with successors, 0 elsewhere; both parities.
+// x = r/deg on live rows with successors, 0 elsewhere; both parities.  The
+// buffers are in PageRank ids; src is in PageRank ids too when src_pr, else
+// in vertex ids.
 template <typename T, typename S>
-__global__ void pr_init_kernel(uint32_t V, const uint8_t *live, const uint32_t *out_ptr, const S *src,
-                               double uniform, T *r0, T *r1, T *x0, T *x1)
+__global__ void pr_init_kernel(uint32_t V, const uint8_t *live, const uint32_t *out_ptr, const uint32_t *pr_id,
+                               const S *src, bool src_pr, double uniform, T *r0, T *r1, T *x0, T *x1)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
+    const uint32_t p = pr_id[v];
     T r = 0;
-    if (live[v]) r = src ? (T)src[v] : (T)uniform;
+    if (live[v]) r = src ? (T)src[src_pr ? p : v] : (T)uniform;
     uint32_t deg = out_ptr[v + 1] - out_ptr[v];
-    r0[v] = r;
-    r1[v] = 0;
-    x0[v] = deg ? r / (T)deg : (T)0;
-    x1[v] = 0;
+    r0[p] = r;
+    r1[p] = 0;
+    x0[p] = deg ? r / (T)deg : (T)0;
+    x1[p] = 0;
 }
 
 template <typename T>
-__global__ void pr_out_kernel(uint32_t V, const T *r, double *out)
+__global__ void pr_out_kernel(uint32_t V, const uint32_t *pr_id, const T *r, double *out)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) out[v] = (double)r[v];
+    if (v < V) out[v] = (double)r[pr_id[v]];
 }
 
 __global__ void pr_single_kernel(uint32_t V, const uint8_t *live, double *out)
@@ -374,12 +361,13 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
 {
     PrArgs<T> a;
     a.in_ptr = g->in_ptr;
-    a.in_col = g->in_col;
+    a.in_col = g->pr_col;
     a.out_ptr = g->out_ptr;
     a.small = g->pr_small;
     a.med = g->pr_med;
     a.desc = g->pr_desc;
     a.large = g->pr_large;
+    a.large_deg = g->pr_large_deg;
     a.large_seg0 = g->pr_large_seg0;
     a.seg_e0 = g->pr_seg_e0;
     a.seg_e1 = g->pr_seg_e1;
@@ -488,11 +476,11 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
         if (mixed) st.tol = std::max(tolerance, ZG_PR_MIXED_SWITCH);
         ZG_TRY(hipMemcpyAsync(d_st, &st, sizeof(PrState), hipMemcpyHostToDevice, s));
         if (!fp64) {
-            hipLaunchKernelGGL((pr_init_kernel<float, float>), vg, vb, 0, s, V, g->live, g->out_ptr,
-                               (const float *)nullptr, inv_n, r32, r32 + V, x32, x32 + V);
+            hipLaunchKernelGGL((pr_init_kernel<float, float>), vg, vb, 0, s, V, g->live, g->out_ptr, g->pr_id,
+                               (const float *)nullptr, false, inv_n, r32, r32 + V, x32, x32 + V);
         } else {
-            hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr,
-                               (const double *)nullptr, inv_n, r64, r64 + V, x64, x64 + V);
+            hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr, g->pr_id,
+                               (const double *)nullptr, false, inv_n, r64, r64 + V, x64, x64 + V);
         }
         ZG_TRY(hipGetLastError());
         ZG_TRY(hipEventRecord(e0, s));
@@ -517,8 +505,8 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             const float *rs = odd ? r32 + V : r32;
             double *ra = odd ? r64 + V : r64, *rb = odd ? r64 : r64 + V;
             double *xa = odd ? x64 + V : x64, *xb = odd ? x64 : x64 + V;
-            hipLaunchKernelGGL((pr_init_kernel<double, float>), vg, vb, 0, s, V, g->live, g->out_ptr, rs, 0.0, ra,
-                               rb, xa, xb);
+            hipLaunchKernelGGL((pr_init_kernel<double, float>), vg, vb, 0, s, V, g->live, g->out_ptr, g->pr_id, rs,
+                               true, 0.0, ra, rb, xa, xb);
             ZG_TRY(hipGetLastError());
             PrState st2 = *h_st;
             st2.tol = tolerance;
@@ -542,9 +530,9 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
         inf.iterations = iters;
         inf.last_l1 = h_st->last_l1;
         if (!final64)
-            hipLaunchKernelGGL(pr_out_kernel<float>, vg, vb, 0, s, V, final_odd ? r32 + V : r32, d_out);
+            hipLaunchKernelGGL(pr_out_kernel<float>, vg, vb, 0, s, V, g->pr_id, final_odd ? r32 + V : r32, d_out);
         else
-            hipLaunchKernelGGL(pr_out_kernel<double>, vg, vb, 0, s, V, final_odd ? r64 + V : r64, d_out);
+            hipLaunchKernelGGL(pr_out_kernel<double>, vg, vb, 0, s, V, g->pr_id, final_odd ? r64 + V : r64, d_out);
         ZG_TRY(hipGetLastError());
 
         if (check) {
@@ -556,8 +544,8 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             st1.tol = -1.0;
             st1.done = 0;
             ZG_TRY(hipMemcpyAsync(d_st + 1, &st1, sizeof(PrState), hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr,
-                               (const double *)d_out, 0.0, r64, r64 + V, x64, x64 + V);
+            hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr, g->pr_id,
+                               (const double *)d_out, false, 0.0, r64, r64 + V, x64, x64 + V);
             ZG_TRY(hipGetLastError());
             PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st + 1);
             hipLaunchKernelGGL(pr_iter_kernel<double>, dim3(pr_grid(g)), dim3(ZG_PR_BLOCK), 0, s, a);
