@@ -98,11 +98,30 @@ def bench_pivots(args):
     g = rg.new(SoA.from_arrays("directed", n, src, dst))
     V, n_live, E = g.resource.info()
     live_ids = np.arange(V)
+    log(f"[bench] judged graph R-MAT {scale} built: {V} nodes, {E} arcs")
+    # exact Brandes over every source, in slices so that progress shows
     t0 = time.time()
-    info = {}
-    exact, true_tot = PV.bc_weighted(g, None, None, want_bc=True, want_totals=True, info=info)
-    t_exact = time.time() - t0
-    log(f"[bench] exact Brandes on R-MAT {scale}: {t_exact:.1f}s, {info['arcs_traversed'] / t_exact / 1e9:.1f} GTEPS")
+    if args.exact_cache and os.path.exists(args.exact_cache):
+        with np.load(args.exact_cache) as z:
+            exact, true_tot, t_exact = z["exact"], z["totals"], float(z["seconds"])
+        log(f"[bench] exact scores read from {args.exact_cache} (computed in {t_exact:.1f}s)")
+    else:
+        exact = np.zeros(V)
+        true_tot = np.zeros(V)
+        arcs_done = 0
+        sl = 1 << 18
+        for a in range(0, V, sl):
+            info = {}
+            ids = live_ids[a:a + sl].astype(np.uint32)
+            b, tt = PV.bc_weighted(g, ids, None, want_bc=True, want_totals=True, info=info)
+            exact += b
+            true_tot[a:a + sl] = tt
+            arcs_done += info["arcs_traversed"]
+            log(f"[bench] exact Brandes: {min(a + sl, V)}/{V} sources, {time.time() - t0:.1f}s")
+        t_exact = time.time() - t0
+        log(f"[bench] exact Brandes on R-MAT {scale}: {t_exact:.1f}s, {arcs_done / t_exact / 1e9:.1f} GTEPS")
+        if args.exact_cache:
+            np.savez(args.exact_cache, exact=exact, totals=true_tot, seconds=t_exact)
     pred, _ = PV.predict_totals(g, sc)
     from scipy.stats import spearmanr
     rho = float(spearmanr(pred, true_tot).statistic)
@@ -116,11 +135,14 @@ def bench_pivots(args):
     mlp_s = (time.perf_counter() - t0) / reps
     flops = 2.0 * V * (8 * 128 + 128 * 128 + 128)
     rng = np.random.default_rng(W.SEEDS["rmat20"] + 99)
-    ks = []
-    k = 2048
-    while k < n_live:
-        ks.append(k)
-        k *= 2
+    if args.ks:
+        ks = [int(x) for x in args.ks.split(",")]
+    else:
+        ks = []
+        k = 2048
+        while k < n_live:
+            ks.append(k)
+            k *= 2
     # reference line: uniform over the sources that have successors (a
     # source without successors contributes nothing)
     nonsink = np.nonzero(np.bincount(src, minlength=V) > 0)[0]
@@ -130,9 +152,10 @@ def bench_pivots(args):
             ov, taus, used = [], [], []
             for r in range(args.reps):
                 if method == "uniform_nonsink":
-                    est, m = PV.sampled_betweenness(g, k, rng, None, nonsink)
+                    est, m = PV.sampled_betweenness(g, k, rng, None, nonsink, log=log)
                 else:
-                    est, m = PV.sampled_betweenness(g, k, rng, pred if method == "learned" else None, live_ids)
+                    est, m = PV.sampled_betweenness(g, k, rng, pred if method == "learned" else None, live_ids,
+                                                    log=log)
                 ov.append(PV.top_overlap(est, exact))
                 used.append(m)
                 if r == 0:
@@ -215,6 +238,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=30, help="pivots: training epochs")
     ap.add_argument("--eval-scale", type=int, default=20, help="pivots: R-MAT scale of the judged graph")
     ap.add_argument("--reps", type=int, default=3, help="pivots: samples per pivot budget")
+    ap.add_argument("--ks", default=None, help="pivots: comma-separated pivot budgets (default 2048, 4096, ... < n)")
+    ap.add_argument("--exact-cache", default=None, help="pivots: .npz of the exact scores (read if present, else written)")
     ap.add_argument("--groups", type=int, default=4, help="64-source groups per launch")
     ap.add_argument("--pull-grid", type=int, default=None, help="pull workgroups per group (0 = one wave per chunk)")
     ap.add_argument("--alpha", type=float, default=None, help="push/pull switch factor")
@@ -252,12 +277,18 @@ def main():
     apply_tuning(g, args)
     tuned = None
     if not args.no_autotune and args.pull_grid is None and args.alpha is None and args.workload != "grid2048":
-        # batch width is part of the workload (64 * --groups sources per
-        # launch); the tuner picks the pull grid and the push/pull switch
+        # rmat20 fixes the batch width (256 sources per launch); er10k sweeps
+        # every source, so the tuner also picks the width there, on all of
+        # them; otherwise it picks the pull grid, the push/pull switch and the
+        # source grouping on two batches' worth of sources
         from zog_amd import autotune
         t0 = time.time()
-        res_t = autotune.tune(g.resource, sources[: 128 * args.groups], groups=(args.groups,), orders=(1, 2),
-                              log=log)
+        if args.workload == "er10k":
+            res_t = autotune.tune(g.resource, sources, groups=(4, 16, 64), alphas=(8.0,), pull_grids=(4096, 0),
+                                  orders=(1, 2), repeats=1, log=log)
+        else:
+            res_t = autotune.tune(g.resource, sources[: 128 * args.groups], groups=(args.groups,), orders=(1, 2),
+                                  log=log)
         tuned = dict(res_t["best"], seconds=round(time.time() - t0, 1))
     V, n_live, E = g.resource.info()
     log(f"[bench] resource built in {time.time() - t0:.1f}s")
@@ -408,7 +439,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": spec["desc"], "nodes": n, "arcs": int(E), "sources": len(sources), "autotuned": tuned,
-                   "sources_per_launch": 64 * args.groups, "levels": levels, "arcs_per_step": arcs_step,
+                   "sources_per_launch": 64 * (tuned["groups"] if tuned else args.groups), "levels": levels, "arcs_per_step": arcs_step,
                    "push_levels": push_levels, "pull_levels": pull_levels},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -294,7 +294,7 @@ def predict_totals(graph: ResourceGraph, scorer: Scorer):
 
 
 def sampled_betweenness(graph: ResourceGraph, k: int, rng, pred_totals=None, live_ids=None, eps: float = 0.05,
-                        info: Optional[dict] = None):
+                        info: Optional[dict] = None, slice_sources: int = 1 << 18, log=None):
     """Estimated BC from about k pivots: learned pivots when pred_totals is
     given, uniform otherwise; exact Brandes when k covers every source."""
     res = graph.resource
@@ -308,5 +308,10 @@ def sampled_betweenness(graph: ResourceGraph, k: int, rng, pred_totals=None, liv
         src, w = uniform_pivots(live_ids, k, rng)
     else:
         src, w = learned_pivots(pred_totals, live_ids, k, rng, eps)
-    bc, _ = bc_weighted(graph, src, w, info=info)
+    bc = np.zeros(res.n_cap)
+    for a in range(0, len(src), slice_sources):
+        part, _ = bc_weighted(graph, src[a:a + slice_sources], w[a:a + slice_sources], info=info)
+        bc += part
+        if log and len(src) > slice_sources:
+            log(f"[pivots] sampled Brandes: {min(a + slice_sources, len(src))}/{len(src)} sources")
     return bc, len(src)
