@@ -174,6 +174,10 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   stride over the predecessor chunks); 0 = one wavefront per chunk.
  *   Default 4096. */
 #define ZG_TUNE_PULL_GRID 3
+/*   ZG_TUNE_GRAPH: 1 (default) replays the forward levels from a captured
+ *   HIP graph (the level index lives on the device), 0 launches every level
+ *   kernel from the host.  Per-kernel profiling always launches directly. */
+#define ZG_TUNE_GRAPH 4
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the

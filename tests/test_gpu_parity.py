@@ -207,7 +207,8 @@ def test_batch_widths_and_source_order_agree():
 
 
 @pytest.mark.parametrize("alpha", [0.0, 1e-9, 8.0])
-def test_push_pull_directions(alpha):
+@pytest.mark.parametrize("graph", [1, 0])
+def test_push_pull_directions(alpha, graph):
     # all-pull, all-push and the default switch give the oracle's BC, depths
     # and path counts (hubs and parallel arcs included)
     from oracle import oracle as O
@@ -220,6 +221,8 @@ def test_push_pull_directions(alpha):
     _, _, arcs = og.betweenness(sources=sources, return_arcs=True)
     g = make(n, src, dst)
     rg.set_push_alpha(g, alpha)
+    from zog_amd import _lib
+    _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_GRAPH, float(graph)), "zg_set_tuning")
     info = {}
     got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
     assert_bc_close(got, ref)

@@ -32,6 +32,7 @@ ZG_ENGINE_AUTO = 0
 ZG_ENGINE_LANES = 1
 ZG_ENGINE_QUEUE = 2
 ZG_TUNE_PULL_GRID = 3
+ZG_TUNE_GRAPH = 4
 
 # Every symbol include/zog_hip.h declares.
 EXPORTED = (

@@ -102,6 +102,15 @@ struct BcDev {
 // Fixed grid of the push kernel (it strides over the level's entries).
 #define ZG_PUSH_GRID 1024
 
+// Arguments of the graph-replayed level kernels, kept in device memory so one
+// captured graph serves every chunk of levels of every batch: the sweep's
+// BcDev, the batch's spans, and the device-side level counter.
+struct BcLevelArgs {
+    BcDev a;
+    GroupSpan sp;
+    int *cur;
+};
+
 // Per-group state is indexed by the vertex's position in the group's span.
 #define MASK(g, v) ((size_t)(g) * a.Vloc + ((v) - a.vlo[g]))
 #define ROW(g, v) (MASK(g, v) * 64)
@@ -189,7 +198,7 @@ __global__ void bc_source_kernel(BcDev a)
 // Opens level d for every group: records where the level's work lists end
 // (everything finalize(d-1) appended), picks the direction from the level's
 // successor-arc count, and clears the counters level d accumulates into.
-__global__ void bc_level_kernel(BcDev a, int d)
+__device__ __forceinline__ void bc_level_body(const BcDev &a, int d)
 {
     const uint32_t g = threadIdx.x;
     if (g >= a.G) return;
@@ -206,6 +215,17 @@ __global__ void bc_level_kernel(BcDev a, int d)
     a.modes[(size_t)d * a.Gcap + g] = mode;
     c.tcount = 0;
     c.farcs[(d + 1) & 1] = 0;
+}
+
+__global__ void bc_level_kernel(BcDev a, int d) { bc_level_body(a, d); }
+
+// Graph-replayed form: the level index lives on the device and advances here.
+__global__ void bc_level_kernel_g(const BcLevelArgs *__restrict__ p)
+{
+    const int d = *p->cur + 1;
+    __syncthreads();
+    if (threadIdx.x == 0) *p->cur = d;
+    bc_level_body(p->a, d);
 }
 
 // One predecessor chunk of the pull at level d (discovers level d+1).
@@ -278,7 +298,7 @@ __device__ __forceinline__ void bc_pull_chunk(const BcDev &a, uint32_t g, int la
 
 // Forward level d: discover level d+1 by pulling over predecessor chunks (a
 // fixed grid striding over the group's chunks).
-__global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSpan sp)
+__device__ __forceinline__ void bc_forward_body(const BcDev &a, int d, const GroupSpan &sp)
 {
     const uint32_t g = blockIdx.y;
     if (a.cnt[g].mode != ZG_MODE_PULL) return;
@@ -286,6 +306,13 @@ __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSp
     const uint32_t nch = sp.chi[g] - sp.clo[g];
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < nch; i += gridDim.x * 4)
         bc_pull_chunk(a, g, lane, d, sp.clo[g] + i);
+}
+
+__global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSpan sp) { bc_forward_body(a, d, sp); }
+
+__global__ __launch_bounds__(256) void bc_forward_kernel_g(const BcLevelArgs *__restrict__ p)
+{
+    bc_forward_body(p->a, *p->cur, p->sp);
 }
 
 __device__ __forceinline__ void push_flush_touched(const BcDev &a, uint32_t g, int lane, uint32_t tbuf, int tn)
@@ -302,7 +329,7 @@ __device__ __forceinline__ void push_flush_touched(const BcDev &a, uint32_t g, i
 // Forward level d, push over the level-d work list (sparse frontiers): each
 // entry's lanes add their count into every successor not yet visited by that
 // lane (FP64 atomics; exact for integer counts below 2^53).
-__global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d)
+__device__ __forceinline__ void bc_push_body(const BcDev &a, int d)
 {
     const uint32_t g = blockIdx.y;
     if (a.cnt[g].mode != ZG_MODE_PUSH) return;
@@ -355,6 +382,10 @@ __global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d)
     }
     if (tn) push_flush_touched(a, g, lane, tbuf, tn);
 }
+
+__global__ __launch_bounds__(256) void bc_push_kernel(BcDev a, int d) { bc_push_body(a, d); }
+
+__global__ __launch_bounds__(256) void bc_push_kernel_g(const BcLevelArgs *__restrict__ p) { bc_push_body(p->a, *p->cur); }
 
 // Per-lane counts of the vertices a finalize block adds to level d+1; each
 // adds d to its lane's dependency total sum_t (dist(s,t) - 1).
@@ -412,7 +443,7 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t x, uint32_t *l
 // lists, clear the frontier buffer of level d-1.  Block b owns predecessor
 // chunks [b*R, (b+1)*R); a vertex is handled by the block holding its first
 // chunk.
-__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, int d, int pushed, GroupSpan sp)
+__device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int pushed, const GroupSpan &sp)
 {
     __shared__ uint32_t lds[40];
     __shared__ uint32_t base[2];
@@ -536,10 +567,20 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, in
     }
 }
 
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel(BcDev a, int d, int pushed, GroupSpan sp)
+{
+    bc_finalize_body(a, d, pushed, sp);
+}
+
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel_g(const BcLevelArgs *__restrict__ p)
+{
+    bc_finalize_body(p->a, *p->cur, 0, p->sp);
+}
+
 // Finalize level d after a push level: same as bc_finalize_kernel, over the
 // vertices the push reached first (a fixed grid striding over the list).
 #define ZG_SPARSE_FIN_BLOCKS 64
-__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDev a, int d)
+__device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
 {
     __shared__ uint32_t lds[40];
     __shared__ uint32_t base[2];
@@ -650,6 +691,16 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDe
         fpos += sf_tot;
         hpos += sh_tot;
     }
+}
+
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel(BcDev a, int d)
+{
+    bc_finalize_sparse_body(a, d);
+}
+
+__global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel_g(const BcLevelArgs *__restrict__ p)
+{
+    bc_finalize_sparse_body(p->a, *p->cur);
 }
 
 // delta / c / bc for lanes `active` of vertex v at level d, given S = sum of
@@ -781,6 +832,14 @@ int zg_bc_ws_release(zg_graph *g)
     zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.touched); zg_dfree(w.flist); zg_dfree(w.fmask);
     zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.foff); zg_dfree(w.hoff); zg_dfree(w.modes); zg_dfree(w.src); zg_dfree(w.cnt);
     zg_dfree(w.wts); zg_dfree(w.dsum);
+    zg_dfree(w.cur);
+    if (w.level_args) (void)hipFree(w.level_args);
+    if (w.h_level_args) (void)hipHostFree(w.h_level_args);
+    if (w.graph) (void)hipGraphExecDestroy(w.graph);
+    w.level_args = w.h_level_args = nullptr;
+    w.graph = nullptr;
+    memset(w.graph_dims, 0, sizeof(w.graph_dims));
+    w.cmax = 0;
     if (w.h_cnt) (void)hipHostFree(w.h_cnt);
     w.h_cnt = nullptr;
     w.groups = 0;
@@ -818,18 +877,19 @@ static void bc_group_span(const zg_graph *g, const uint32_t *src, uint32_t n, ui
 static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources, uint64_t n)
 {
     zg_bc_ws &w = g->bc;
-    uint32_t vloc = 1, och = 0, hubs = 0;
+    uint32_t vloc = 1, och = 0, hubs = 0, cmax = 0;
     for (uint64_t b0 = 0; b0 < n; b0 += 64) {
         uint32_t lo, hi;
         bc_group_span(g, h_sources + b0, (uint32_t)std::min<uint64_t>(64, n - b0), &lo, &hi);
         vloc = std::max(vloc, hi - lo);
+        cmax = std::max(cmax, g->h_ich_first[hi] - g->h_ich_first[lo]);
         och = std::max(och, g->h_och_first[hi] - g->h_och_first[lo]);
         hubs = std::max(hubs, g->h_hub_first[hi] - g->h_hub_first[lo]);
     }
     const size_t need_f = 64 * (size_t)och + 64 * (size_t)g->max_och_per_v;
     const size_t need_h = 64 * (size_t)hubs + 64;
     if (w.groups == g->groups && (w.with_depth || !with_depth) && w.vloc >= vloc && w.fcap >= need_f &&
-        w.hcap >= need_h)
+        w.hcap >= need_h && w.cmax >= cmax)
         return ZG_OK;
     zg_bc_ws_release(g);
     const size_t G = g->groups, V = g->V;
@@ -859,6 +919,9 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
     if (rc == ZG_OK) rc = zg_dmalloc(&w.src, G * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.wts, G * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.dsum, G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cur, 1);
+    if (rc == ZG_OK) rc = zg_hip_code(hipMalloc(&w.level_args, sizeof(BcLevelArgs)));
+    if (rc == ZG_OK) rc = zg_hip_code(hipHostMalloc(&w.h_level_args, sizeof(BcLevelArgs), hipHostMallocDefault));
     if (rc == ZG_OK) rc = zg_dmalloc(&w.cnt, G);
     if (rc == ZG_OK) {
         hipError_t e = hipHostMalloc((void **)&w.h_cnt, G * sizeof(zg_bc_ws::Counters), hipHostMallocDefault);
@@ -875,6 +938,7 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
     }
     w.groups = (int)G;
     w.vloc = vloc;
+    w.cmax = cmax;
     w.with_depth = with_depth;
     w.last_levels = w.lcap;
     return ZG_OK;
@@ -907,6 +971,44 @@ static int bc_grow_levels(zg_graph *g, uint32_t need, hipStream_t s)
     ZG_TRY_RC(bc_grow_table(&w.hoff, w.lcap, nl, G, s));
     ZG_TRY_RC(bc_grow_table(&w.modes, w.lcap, nl, G, s));
     w.lcap = nl;
+    return ZG_OK;
+}
+
+#define ZG_GRAPH_LEVELS 4  // levels per captured graph; level chunks are multiples of it
+static_assert(ZG_LEVEL_CHUNK0 % ZG_GRAPH_LEVELS == 0 && ZG_LEVEL_CHUNK_MAX % ZG_GRAPH_LEVELS == 0,
+              "level chunks replay whole graphs");
+
+// Captures ZG_GRAPH_LEVELS forward levels (level, push, pull, sparse and dense
+// finalize kernels) reading their arguments from w.level_args; the grids are
+// fixed by the workspace (dims: push, pull, sparse finalize, dense finalize
+// blocks per group, groups), so one graph serves every chunk of every batch.
+static int bc_capture_levels(zg_graph *g, const uint32_t dims[5])
+{
+    zg_bc_ws &w = g->bc;
+    if (w.graph && !memcmp(dims, w.graph_dims, sizeof(w.graph_dims))) return ZG_OK;
+    if (w.graph) (void)hipGraphExecDestroy(w.graph);
+    w.graph = nullptr;
+    if (!g->cap_stream) ZG_TRY(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+    hipStream_t cs = g->cap_stream;
+    const BcLevelArgs *p = (const BcLevelArgs *)w.level_args;
+    const uint32_t G = dims[4];
+    ZG_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < ZG_GRAPH_LEVELS; k++) {
+        hipLaunchKernelGGL(bc_level_kernel_g, dim3(1), dim3(64), 0, cs, p);
+        hipLaunchKernelGGL(bc_push_kernel_g, dim3(dims[0], G), dim3(256), 0, cs, p);
+        hipLaunchKernelGGL(bc_forward_kernel_g, dim3(dims[1], G), dim3(256), 0, cs, p);
+        hipLaunchKernelGGL(bc_finalize_sparse_kernel_g, dim3(dims[2], G), dim3(ZG_FIN_THREADS), 0, cs, p);
+        hipLaunchKernelGGL(bc_finalize_kernel_g, dim3(dims[3], G), dim3(ZG_FIN_THREADS), 0, cs, p);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&w.graph, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        w.graph = nullptr;
+        return zg_hip_code(e);
+    }
+    memcpy(w.graph_dims, dims, sizeof(w.graph_dims));
     return ZG_OK;
 }
 
@@ -963,6 +1065,7 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
     zg_bc_ws &w = g->bc;
     const uint32_t V = g->V;
     const uint32_t Gcap = (uint32_t)w.groups;
+    const bool use_graph = g->use_graph && !g->profiling;
     ZG_TRY(hipMemsetAsync(w.bcg, 0, sizeof(double) * (size_t)Gcap * V, s));
     // per-block work statistics accumulate over all batches
     ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * 2, s));
@@ -1004,19 +1107,37 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         // (offsets, direction, counter resets); push, pull and both finalize
         // variants are queued and all but the chosen ones return at once.
         // The host syncs once per chunk of levels to find the last one.
-        uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(max_c, 4));
+        // Replayed from a captured graph, the grids follow the workspace's
+        // largest span (the same for every batch); launched directly, they
+        // follow this batch's largest span.
+        const uint32_t span_c = use_graph ? w.cmax : max_c, span_v = use_graph ? w.vloc : max_v;
+        uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(span_c, 4));
         if (g->pull_grid) fwd_blocks = std::min(fwd_blocks, g->pull_grid);
-        // grids of the other level kernels follow the largest span
-        const uint32_t fin_blocks = std::min<uint32_t>(ZG_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(max_c, 4096)));
-        const uint32_t sfin_blocks = std::min<uint32_t>(ZG_SPARSE_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(max_v, 8192)));
-        const uint32_t push_blocks = std::min<uint32_t>(ZG_PUSH_GRID, std::max<uint32_t>(1, cdiv(max_v, 32)));
+        const uint32_t fin_blocks = std::min<uint32_t>(ZG_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(span_c, 4096)));
+        const uint32_t sfin_blocks = std::min<uint32_t>(ZG_SPARSE_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(span_v, 8192)));
+        const uint32_t push_blocks = std::min<uint32_t>(ZG_PUSH_GRID, std::max<uint32_t>(1, cdiv(span_v, 32)));
+        if (use_graph) {
+            const uint32_t dims[5] = {push_blocks, fwd_blocks, sfin_blocks, fin_blocks, Gcap};
+            ZG_TRY_RC(bc_capture_levels(g, dims));
+            ZG_TRY(hipMemsetAsync(w.cur, 0xFF, sizeof(int), s));  // level counter = -1
+        }
         std::vector<uint32_t> h_off, h_hoff, h_modes;  // [L][Gcap] copies
         int d = 0, maxd = -1;
         uint32_t chunk = ZG_LEVEL_CHUNK0;
         while (maxd < 0) {
             ZG_TRY_RC(bc_grow_levels(g, (uint32_t)d + chunk + 2, s));
             a = bc_dev(g, G, sp, opt);
-            for (uint32_t k = 0; k < chunk; k++) {
+            if (use_graph) {
+                // the previous copy from the staging buffer has run: the stream
+                // was synchronised after it
+                BcLevelArgs *h = (BcLevelArgs *)w.h_level_args;
+                h->a = a;
+                h->sp = sp;
+                h->cur = w.cur;
+                ZG_TRY(hipMemcpyAsync(w.level_args, h, sizeof(BcLevelArgs), hipMemcpyHostToDevice, s));
+                for (uint32_t k = 0; k < chunk; k += ZG_GRAPH_LEVELS) ZG_TRY(hipGraphLaunch(w.graph, s));
+            }
+            for (uint32_t k = 0; k < (use_graph ? 0u : chunk); k++) {
                 const int L = d + (int)k;
                 ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_level_kernel, dim3(1), dim3(64), 0, s, a, L);
                 ZG_PROF_LAUNCH(ZG_PROF_BC_PUSH, bc_push_kernel, dim3(push_blocks, G), dim3(256), 0, s, a, L);

@@ -107,6 +107,12 @@ struct zg_bc_ws {
     uint32_t *src = nullptr;             // [G][64]
     double *wts = nullptr;               // [G][64] per-source weights of a weighted sweep
     unsigned long long *dsum = nullptr;  // [G][64] per-source dependency totals (integers)
+    uint32_t cmax = 0;                   // most predecessor chunks in one group's span
+    void *level_args = nullptr;          // device arguments of the graph-replayed level kernels
+    void *h_level_args = nullptr;        // pinned staging copy of them
+    int *cur = nullptr;                  // device level counter of the replayed levels
+    hipGraphExec_t graph = nullptr;      // ZG_GRAPH_LEVELS captured levels
+    uint32_t graph_dims[5] = {0, 0, 0, 0, 0};  // push, pull, sparse/dense finalize grids, groups
     struct Counters {
         uint32_t fcount, hcount;
         uint32_t err;               // ZG_BC_ERR_* bits
@@ -154,6 +160,8 @@ struct zg_graph {
     std::mutex mu;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t cap_stream = nullptr;    // stream the level graph is captured on (never launched on)
+    bool use_graph = true;               // replay captured level chunks (ZG_TUNE_GRAPH)
     uint32_t V = 0, n_live = 0;
     uint64_t E = 0;
     // resident CSR

@@ -109,6 +109,7 @@ static void zg_free_device(zg_graph *g)
     zg_dfree(g->pr_seg_row); zg_dfree(g->pr_seg_e0); zg_dfree(g->pr_seg_e1);
     zg_dfree(g->pr_large_deg); zg_dfree(g->pr_id); zg_dfree(g->pr_col);
     if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
     g->stream = nullptr;
 }
 
@@ -419,6 +420,10 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
     case ZG_TUNE_ENGINE:
         if (value != ZG_ENGINE_AUTO && value != ZG_ENGINE_LANES && value != ZG_ENGINE_QUEUE) return ZG_ERR_INVALID_ARG;
         g->engine = (int)value;
+        return ZG_OK;
+    case ZG_TUNE_GRAPH:
+        if (value != 0.0 && value != 1.0) return ZG_ERR_INVALID_ARG;
+        g->use_graph = value != 0.0;
         return ZG_OK;
     case ZG_TUNE_PULL_GRID:
         if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;
