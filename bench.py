@@ -225,6 +225,12 @@ def apply_tuning(g, args):
     from zog_amd import resource_graph as rg
     if args.no_graph:
         _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_GRAPH, 0.0), "zg_set_tuning")
+    if args.pr_cache is not None:
+        _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_PR_CACHE, float(args.pr_cache)),
+                   "zg_set_tuning")
+    if args.pr_grid is not None:
+        _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_PR_GRID, float(args.pr_grid)),
+                   "zg_set_tuning")
     if args.pull_grid is not None:
         rg.set_pull_grid(g, args.pull_grid)
     if args.alpha is not None:
@@ -248,6 +254,8 @@ def main():
     ap.add_argument("--alpha", type=float, default=None, help="push/pull switch factor")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-graph", action="store_true", help="launch every level kernel from the host")
+    ap.add_argument("--pr-cache", type=int, default=None, help="PageRank values staged in LDS (0 = none)")
+    ap.add_argument("--pr-grid", type=int, default=None, help="PageRank iteration blocks")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -178,6 +178,12 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   HIP graph (the level index lives on the device), 0 launches every level
  *   kernel from the host.  Per-kernel profiling always launches directly. */
 #define ZG_TUNE_GRAPH 4
+/*   ZG_TUNE_PR_CACHE: PageRank values of the most-gathered vertices each
+ *   iteration block copies into LDS (default: as many as 16 KiB hold, when
+ *   they serve at least 10% of the gathers; 0 = none).  ZG_TUNE_PR_GRID:
+ *   PageRank iteration blocks (0 = default 2048). */
+#define ZG_TUNE_PR_CACHE 5
+#define ZG_TUNE_PR_GRID 6
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the

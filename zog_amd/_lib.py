@@ -33,6 +33,8 @@ ZG_ENGINE_LANES = 1
 ZG_ENGINE_QUEUE = 2
 ZG_TUNE_PULL_GRID = 3
 ZG_TUNE_GRAPH = 4
+ZG_TUNE_PR_CACHE = 5
+ZG_TUNE_PR_GRID = 6
 
 # Every symbol include/zog_hip.h declares.
 EXPORTED = (

@@ -207,7 +207,10 @@ struct zg_graph {
     int groups = 4;
     int engine = 0;                      // ZG_ENGINE_*
     uint32_t *und_ptr = nullptr, *und_col = nullptr;  // simple undirected CSR (core numbers), built on demand
-    uint32_t pull_grid = 4096;           // pull workgroups per source group (0 = one wave per chunk)
+    uint32_t pull_grid = 4096;
+    uint32_t pr_xcache = 1u << 30;       // PageRank values staged in LDS (capped by the LDS size; 0 = none)
+    bool pr_xcache_useful = false;       // the cached values carry a useful share of the gathers
+    uint32_t pr_grid_cap = 0;            // PageRank iteration blocks (0 = default)           // pull workgroups per source group (0 = one wave per chunk)
     bool high_diameter = false;          // set with h_coredist
     zg_bc_ws bc;
     zg_bcq_ws bcq;

@@ -231,6 +231,13 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
         return out_ptr[(size_t)x + 1] - out_ptr[x] > out_ptr[(size_t)y + 1] - out_ptr[y];
     });
     for (uint32_t i = 0; i < V; i++) pr_id[pr_order[i]] = i;
+    {
+        // stage the hottest values in LDS only if they serve >= 10% of the gathers
+        const uint32_t K = std::min<uint32_t>(V, ZG_PR_XCACHE_BYTES / sizeof(float));
+        uint64_t hot = 0;
+        for (uint32_t i = 0; i < K; i++) hot += out_ptr[(size_t)pr_order[i] + 1] - out_ptr[pr_order[i]];
+        g->pr_xcache_useful = E > 0 && hot * 10 >= E;
+    }
     std::vector<uint32_t> pr_col(in_col.size());
     for (size_t e = 0; e < in_col.size(); e++) pr_col[e] = pr_id[in_col[e]];
 
@@ -420,6 +427,15 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
     case ZG_TUNE_ENGINE:
         if (value != ZG_ENGINE_AUTO && value != ZG_ENGINE_LANES && value != ZG_ENGINE_QUEUE) return ZG_ERR_INVALID_ARG;
         g->engine = (int)value;
+        return ZG_OK;
+    case ZG_TUNE_PR_CACHE:
+        if (!(value >= 0.0 && value <= 4294967295.0)) return ZG_ERR_INVALID_ARG;
+        g->pr_xcache = (uint32_t)value;
+        return ZG_OK;
+    case ZG_TUNE_PR_GRID:
+        if (!(value >= 0.0 && value <= 65536.0)) return ZG_ERR_INVALID_ARG;
+        g->pr_grid_cap = (uint32_t)value;
+        (void)zg_pr_ws_release(g);  // partial buffers are sized by the grid
         return ZG_OK;
     case ZG_TUNE_GRAPH:
         if (value != 0.0 && value != 1.0) return ZG_ERR_INVALID_ARG;

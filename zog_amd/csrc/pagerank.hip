@@ -44,6 +44,7 @@ struct PrArgs {
     const uint4 *desc;          // [small | m4 | m16 | m64]: {v, first arc, in-degree, out-degree}
     uint32_t n_small, n_m4, n_m16, n_m64, n_large, n_seg;
     uint32_t nb_small, nb_m4, nb_m16, nb_m64;  // work units of each bin
+    uint32_t xcache;            // values staged in LDS (0 = none)
     T *r0, *r1, *x0, *x1;       // parity 0 / 1 buffers
     double *part;               // [gridDim.x][2]
     T *seg;                     // [n_seg][2] Kahan pairs of long-row segments
@@ -81,9 +82,19 @@ __device__ __forceinline__ void wave_kahan_merge(T &s, T &c)
     }
 }
 
-// Kahan sum of x[col[e]] for e = e0, e0+step, ... < e1, four gathers in flight.
+// Gathered values: ids below K (the most-gathered vertices, PageRank ids
+// being ordered by decreasing out-degree) come from the block's LDS copy.
 template <typename T>
-__device__ __forceinline__ void pr_strided_kahan(const uint32_t *__restrict__ col, const T *__restrict__ x,
+struct PrGather {
+    const T *__restrict__ x;
+    const T *lds;
+    uint32_t K;
+    __device__ __forceinline__ T operator[](uint32_t u) const { return u < K ? lds[u] : x[u]; }
+};
+
+// Kahan sum of x[col[e]] for e = e0, e0+step, ... < e1, four gathers in flight.
+template <typename T, typename X>
+__device__ __forceinline__ void pr_strided_kahan(const uint32_t *__restrict__ col, const X &x,
                                                  uint32_t e0, uint32_t e1, uint32_t step, T &s, T &c)
 {
     uint32_t e = e0;
@@ -114,9 +125,9 @@ __device__ __forceinline__ void pr_row_finish(T *rnew, T *xnew, uint32_t v, T r_
 
 // Rows of one bin summed by groups of L lanes: unit u covers rows
 // [u*(256/L), (u+1)*(256/L)) of the bin; lane groups merge their Kahan pairs.
-template <typename T, int L>
+template <typename T, int L, typename X>
 __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *rows, uint32_t nrows,
-                                              uint32_t unit, const T *x, const T *rold, T *rnew, T *xnew,
+                                              uint32_t unit, const X &x, const T *rold, T *rnew, T *xnew,
                                               double base, double damping, double &l1, double &dang)
 {
     const int tid = threadIdx.x;
@@ -148,7 +159,13 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
     if (st->done) return;
     const unsigned long long it = st->iter;
     const bool odd = it & 1;
-    const T *x = odd ? a.x1 : a.x0;
+    const T *xg = odd ? a.x1 : a.x0;
+    // the ZG_PR_XCACHE_BYTES most-gathered values (lowest PageRank ids) in LDS
+    __shared__ T xs[ZG_PR_XCACHE_BYTES / sizeof(T)];
+    const uint32_t K = min(a.xcache, (uint32_t)(ZG_PR_XCACHE_BYTES / sizeof(T)));
+    for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) xs[i] = xg[i];
+    __syncthreads();
+    const PrGather<T> x{xg, xs, K};
     const T *rold = odd ? a.r1 : a.r0;
     T *rnew = odd ? a.r0 : a.r1;
     T *xnew = odd ? a.x0 : a.x1;
@@ -335,7 +352,8 @@ static uint32_t pr_grid(const zg_graph *g)
 {
     uint32_t units = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK + (g->n_m4 + 63) / 64 + (g->n_m16 + 15) / 16 +
                      (g->n_m64 + 3) / 4 + g->n_seg;
-    return units < ZG_PR_MAX_GRID ? (units ? units : 1) : ZG_PR_MAX_GRID;
+    const uint32_t cap = g->pr_grid_cap ? g->pr_grid_cap : ZG_PR_MAX_GRID;
+    return units < cap ? (units ? units : 1) : cap;
 }
 
 static int pr_ws_ensure(zg_graph *g)
@@ -381,6 +399,7 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.nb_m4 = (g->n_m4 + ZG_PR_BLOCK / 4 - 1) / (ZG_PR_BLOCK / 4);
     a.nb_m16 = (g->n_m16 + ZG_PR_BLOCK / 16 - 1) / (ZG_PR_BLOCK / 16);
     a.nb_m64 = (g->n_m64 + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
+    a.xcache = g->pr_xcache_useful ? std::min<uint32_t>(g->pr_xcache, g->V) : 0;
     a.r0 = r0;
     a.r1 = r1;
     a.x0 = x0;
