@@ -83,6 +83,14 @@ typedef struct {
 int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
                     const uint32_t *to, uint64_t m, zg_graph **out);
 
+/* Same as zg_graph_create, keeping the arc weights: weight[i] (f64) is the
+ * weight of arc i, as buildGraph stores it with addEdge(f, t, w)
+ * (lib/zog/resource_graph.ex:120-129; weight is the `weight` list new/5
+ * takes, :188).  Only zg_bc_f64 reads the weights; the other entry points
+ * behave as on a zg_graph_create handle. */
+int zg_graph_create_weighted(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
+                             const uint32_t *to, const double *weight, uint64_t m, zg_graph **out);
+
 /* Batch of independent graphs in one resource (no reference counterpart; the
  * reference builds one resource per graph with new/5): graph i has
  * node_counts[i] nodes and arc_counts[i] arcs whose local ids are in from/to,
@@ -118,6 +126,25 @@ int zg_graph_info(const zg_graph *g, uint32_t *n_cap, uint32_t *n_live, uint64_t
  * info may be NULL. */
 int zg_bc_unweighted(zg_graph *g, const uint32_t *sources, uint64_t n_sources, double *out_bc,
                      zg_bc_info *info);
+
+/* Replaces zog.centrality.betweennessF64 on an ArrayGraph(void, f64)
+ * (priv/zog/src/centrality.zig:1355-1357 -> betweenness, :529-667) as
+ * called by `nif_betweenness_f64` (resource_graph.ex:217-226) followed by
+ * extractScores: single-source shortest paths under the arc weights (sums
+ * added in path order, a path is shortest when its sum equals the distance
+ * exactly), path counts over every such arc, unnormalised directed-sum
+ * dependencies into out_bc[n_cap] (0.0 for non-live ids).  The handle must
+ * come from zg_graph_create_weighted (else ZG_ERR_INVALID_ARG when it has
+ * arcs).  sources / n_sources as zg_bc_unweighted (NULL = every live node,
+ * the reference).  info may be NULL.
+ * A live arc with a NaN, negative or infinite weight is ZG_ERR_INVALID_ARG
+ * (the reference's Dijkstra assumes finite non-negative weights and returns
+ * order-dependent values otherwise).  Weight 0 is accepted; where arcs of
+ * weight 0 join vertices at equal distance, the reference's values depend on
+ * its heap's pop order, while this path counts every tight arc in DAG order
+ * and leaves vertices on a zero-weight cycle without paths.  Path counts are
+ * FP64 (exact below 2^53; the reference's are u64). */
+int zg_bc_f64(zg_graph *g, const uint32_t *sources, uint64_t n_sources, double *out_bc, zg_bc_info *info);
 
 /* Same computation with device-resident source list and output (both in HBM,
  * d_sources may be NULL for all live nodes) on the caller's HIP stream

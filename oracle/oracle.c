@@ -30,6 +30,9 @@
  *   - an N-thread source-chunked BC (the pattern of centrality.zig:216-256).
  *   - core numbers, coreNumbers in priv/zog/src/connectivity.zig:5-101
  *     (input to the learned pivot scorer's features).
+ *   - weighted Brandes, betweenness (centrality.zig:529-667) as
+ *     betweennessF64 instantiates it (:1355-1357: f64 weights, 0.0, a + b,
+ *     std.math.order).
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -58,6 +61,7 @@ typedef struct {
     uint32_t *to;
     uint32_t *next_edge;    /* ZO_NONE = end of list */
     uint8_t *edge_deleted;
+    double *weight;         /* edge data (f64); NULL until zo_graph_set_weights */
 } zo_graph;
 
 void zo_graph_destroy(zo_graph *g)
@@ -68,6 +72,7 @@ void zo_graph_destroy(zo_graph *g)
     free(g->to);
     free(g->next_edge);
     free(g->edge_deleted);
+    free(g->weight);
     free(g);
 }
 
@@ -659,4 +664,187 @@ int zo_core_numbers(const zo_graph *g, uint32_t *out)
     }
     free(deg); free(cnt); free(adj); free(e_node); free(e_next); free(head); free(done);
     return ZO_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Weighted Brandes (centrality.zig:529-667, betweennessF64 :1355-1357)       */
+/* ------------------------------------------------------------------------- */
+
+/* Edge data of arc e (arc order of zo_graph_create). */
+int zo_graph_set_weights(zo_graph *g, const double *w, uint64_t m)
+{
+    if (m != g->m) return ZO_ERR_ARG;
+    double *c = (double *)malloc(sizeof(double) * (m ? m : 1));
+    if (!c) return ZO_ERR_OOM;
+    for (uint64_t e = 0; e < m; e++) c[e] = w[e];
+    free(g->weight);
+    g->weight = c;
+    return ZO_OK;
+}
+
+/* As zo_next_succ, also returning the arc. */
+static inline uint32_t zo_next_arc(const zo_graph *g, uint32_t *cur, uint32_t *arc)
+{
+    while (*cur != ZO_NONE) {
+        uint32_t e = *cur;
+        *cur = g->next_edge[e];
+        if (!g->edge_deleted[e]) {
+            uint32_t w = g->to[e];
+            if (!g->node_deleted[w]) {
+                *arc = e;
+                return w;
+            }
+        }
+    }
+    return ZO_NONE;
+}
+
+/* Binary min-heap of (distance, node) keyed by distance: append + sift up,
+ * pop = move the last item to the root + sift down (the smaller child, the
+ * right one only when strictly smaller; stop when the moved item is strictly
+ * smaller than that child). */
+typedef struct {
+    double d;
+    uint32_t node;
+} zo_item;
+
+static void zo_heap_push(zo_item *h, size_t *n, zo_item x)
+{
+    size_t i = (*n)++;
+    while (i > 0) {
+        size_t p = (i - 1) >> 1;
+        if (!(x.d < h[p].d)) break;
+        h[i] = h[p];
+        i = p;
+    }
+    h[i] = x;
+}
+
+static zo_item zo_heap_pop(zo_item *h, size_t *n)
+{
+    zo_item top = h[0];
+    zo_item last = h[--(*n)];
+    if (*n == 0) return top;
+    size_t i = 0;
+    for (;;) {
+        size_t c = 2 * i + 1;
+        if (c >= *n) break;
+        if (c + 1 < *n && h[c + 1].d < h[c].d) c++;
+        if (last.d < h[c].d) break;
+        h[i] = h[c];
+        i = c;
+    }
+    h[i] = last;
+    return top;
+}
+
+/* Unnormalised weighted BC into out[n_cap] over sources[0..n) (NULL = every
+ * live node in id order).  Per source (:592-647): dist unset, sigma 0, delta
+ * 0, preds cleared; the source at 0 with sigma 1; pop the closest item, skip
+ * it if its distance is above the node's recorded one, push the node on the
+ * stack, relax its arcs: strictly shorter -> new distance, sigma copied,
+ * preds reset to {v}, pushed; equal -> sigma added, v appended to preds;
+ * unset -> as strictly shorter.  Backward (:649-663): pop the stack, for each
+ * pred u: delta[u] += (sigma[u] / sigma[v]) * (1 + delta[v]); then
+ * bc[v] += delta[v] unless v is the source.  sigma is u64 (the reference's
+ * usize).  Returns 0, or <0 (no weights set, out of memory). */
+int zo_bc_f64(const zo_graph *g, const uint32_t *sources, uint64_t n_sources, double *out)
+{
+    const uint32_t V = g->n_cap;
+    if (!g->weight) return ZO_ERR_ARG;
+    for (uint32_t i = 0; i < V; i++) out[i] = 0.0;
+    double *dist = malloc(sizeof(double) * (V ? V : 1));
+    uint8_t *has = malloc(V ? V : 1);
+    uint64_t *sigma = malloc(sizeof(uint64_t) * (V ? V : 1));
+    double *delta = malloc(sizeof(double) * (V ? V : 1));
+    uint32_t *stack = malloc(sizeof(uint32_t) * (V ? V : 1));
+    /* preds as per-node growable arrays */
+    uint32_t **pr = calloc(V ? V : 1, sizeof(uint32_t *));
+    uint32_t *pn = calloc(V ? V : 1, sizeof(uint32_t)), *pc = calloc(V ? V : 1, sizeof(uint32_t));
+    size_t hcap = (size_t)g->m + 2;
+    zo_item *heap = malloc(sizeof(zo_item) * hcap);
+    int rc = ZO_OK;
+    if (!dist || !has || !sigma || !delta || !stack || !pr || !pn || !pc || !heap) rc = ZO_ERR_OOM;
+    uint64_t ns = sources ? n_sources : V;
+    for (uint64_t k = 0; rc == ZO_OK && k < ns; k++) {
+        uint32_t s = sources ? sources[k] : (uint32_t)k;
+        if (!sources && g->node_deleted[s]) continue;
+        if (s >= V || g->node_deleted[s]) {
+            rc = ZO_ERR_NODE_NOT_FOUND;
+            break;
+        }
+        for (uint32_t i = 0; i < V; i++) {
+            has[i] = 0;
+            sigma[i] = 0;
+            delta[i] = 0.0;
+            pn[i] = 0;
+        }
+        size_t hn = 0, sn = 0;
+        dist[s] = 0.0;
+        has[s] = 1;
+        sigma[s] = 1;
+        zo_heap_push(heap, &hn, (zo_item){0.0, s});
+        while (hn > 0) {
+            zo_item it = zo_heap_pop(heap, &hn);
+            const double dv = it.d;
+            const uint32_t v = it.node;
+            const double best = has[v] ? dist[v] : dv;
+            if (dv > best) continue;
+            stack[sn++] = v;
+            uint32_t cur = g->first_edge[v], arc, w;
+            while ((w = zo_next_arc(g, &cur, &arc)) != ZO_NONE) {
+                const double nd = dv + g->weight[arc];
+                int shorter = !has[w] || nd < dist[w];
+                int equal = has[w] && nd == dist[w];
+                if (shorter) {
+                    dist[w] = nd;
+                    has[w] = 1;
+                    sigma[w] = sigma[v];
+                    pn[w] = 0;
+                } else if (equal) {
+                    sigma[w] += sigma[v];
+                }
+                if (shorter || equal) {
+                    if (pn[w] == pc[w]) {
+                        uint32_t nc = pc[w] ? 2 * pc[w] : 4;
+                        uint32_t *np = realloc(pr[w], sizeof(uint32_t) * nc);
+                        if (!np) {
+                            rc = ZO_ERR_OOM;
+                            break;
+                        }
+                        pr[w] = np;
+                        pc[w] = nc;
+                    }
+                    pr[w][pn[w]++] = v;
+                }
+                if (shorter) {
+                    if (hn == hcap) {
+                        hcap *= 2;
+                        zo_item *nh = realloc(heap, sizeof(zo_item) * hcap);
+                        if (!nh) {
+                            rc = ZO_ERR_OOM;
+                            break;
+                        }
+                        heap = nh;
+                    }
+                    zo_heap_push(heap, &hn, (zo_item){nd, w});
+                }
+            }
+            if (rc != ZO_OK) break;
+        }
+        while (rc == ZO_OK && sn > 0) {
+            const uint32_t v = stack[--sn];
+            const double sv = (double)sigma[v];
+            const double dlt = delta[v];
+            for (uint32_t i = 0; i < pn[v]; i++) {
+                const uint32_t u = pr[v][i];
+                delta[u] += ((double)sigma[u] / sv) * (1.0 + dlt);
+            }
+            if (v != s) out[v] += dlt;
+        }
+    }
+    for (uint32_t i = 0; pr && i < V; i++) free(pr[i]);
+    free(pr); free(pn); free(pc); free(heap);
+    free(dist); free(has); free(sigma); free(delta); free(stack);
+    return rc;
 }

@@ -54,6 +54,10 @@ def lib():
         L.zo_pagerank.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P, P, P]
         L.zo_core_numbers.argtypes = [P, P]
         L.zo_core_numbers.restype = ctypes.c_int
+        L.zo_graph_set_weights.argtypes = [P, P, ctypes.c_uint64]
+        L.zo_graph_set_weights.restype = ctypes.c_int
+        L.zo_bc_f64.argtypes = [P, P, ctypes.c_uint64, P]
+        L.zo_bc_f64.restype = ctypes.c_int
         L.zo_pagerank.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -151,4 +155,20 @@ class Graph:
         rc = lib().zo_core_numbers(self._h, _ptr(out))
         if rc != 0:
             raise OracleError(f"zo_core_numbers failed: {rc}")
+        return out
+
+    def set_weights(self, weights):
+        """Edge data f64[m] in arc order."""
+        self.weights = np.ascontiguousarray(weights, dtype=np.float64)
+        rc = lib().zo_graph_set_weights(self._h, _ptr(self.weights), len(self.weights))
+        if rc != 0:
+            raise OracleError(f"zo_graph_set_weights failed: {rc}")
+
+    def betweenness_f64(self, sources=None):
+        """Unnormalised weighted BC f64[n_cap] (betweennessF64)."""
+        out = np.zeros(self.n_cap, dtype=np.float64)
+        src = None if sources is None else np.ascontiguousarray(sources, dtype=np.uint32)
+        rc = lib().zo_bc_f64(self._h, _ptr(src), 0 if src is None else len(src), _ptr(out))
+        if rc != 0:
+            raise OracleError(f"zo_bc_f64 failed: {rc}")
         return out

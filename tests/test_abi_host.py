@@ -45,6 +45,8 @@ def test_null_handle_is_rejected():
     assert L.zg_set_batch_groups(None, 4) == _lib.ZG_ERR_INVALID_HANDLE
     L.zg_graph_destroy(None)
     assert L.zg_graph_create(3, None, None, None, 0, None) == _lib.ZG_ERR_INVALID_ARG
+    assert L.zg_bc_f64(None, None, 0, None, None) == _lib.ZG_ERR_INVALID_HANDLE
+    assert L.zg_graph_create_weighted(3, None, None, None, None, 0, None) == _lib.ZG_ERR_INVALID_ARG
 
 
 def test_errors_raise():
@@ -103,11 +105,15 @@ def test_edgelist_parse_labels_first_seen(tmp_path):
 def test_undirected_dedupe_rule():
     # addParsedEdges (resource_graph.ex:953-988): sort by (min, max), drop
     # repeats of an unordered pair, both arcs unless a self-loop
-    edges = [(2, 1, 1.0), (0, 1, 1.0), (1, 2, 1.0), (3, 3, 1.0), (1, 0, 1.0)]
-    src, dst = rg._expand_edges(edges, directed=False)
+    # repeats of an unordered pair (the first in input order keeps its
+    # weight: the sort is stable), both arcs unless a self-loop
+    edges = [(2, 1, 1.0), (0, 1, 4.0), (1, 2, 2.0), (3, 3, 3.0), (1, 0, 5.0)]
+    src, dst, w = rg._expand_edges(edges, directed=False)
     assert list(zip(src, dst)) == [(0, 1), (1, 0), (2, 1), (1, 2), (3, 3)]
-    src, dst = rg._expand_edges(edges, directed=True)
+    assert w == [4.0, 4.0, 1.0, 1.0, 3.0]
+    src, dst, w = rg._expand_edges(edges, directed=True)
     assert list(zip(src, dst)) == [(2, 1), (0, 1), (1, 2), (3, 3), (1, 0)]
+    assert w == [1.0, 4.0, 2.0, 3.0, 5.0]
 
 
 def test_generators_deterministic():

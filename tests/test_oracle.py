@@ -377,3 +377,111 @@ def test_er10k_checksum_and_threads(oracle_mod):
         checksum += int(np.where(depth != oracle_mod.NONE_U32, depth.astype(np.int64) - 1, 0).clip(0).sum())
     bc20, _ = g.betweenness(sources=srcs[:20])
     assert abs(bc20.sum() - checksum) <= 1e-9 * checksum
+
+
+# --- weighted Brandes (centrality.zig:529-667, betweennessF64 :1355-1357) ---
+
+def brute_bc_weighted(n, src, dst, w):
+    """BC(v) from every simple s-t arc sequence whose weight sum (in path
+    order, as Dijkstra adds) is the smallest, parallel arcs separately."""
+    adj = [[] for _ in range(n)]
+    for u, v, x in zip(src.tolist(), dst.tolist(), w.tolist()):
+        adj[u].append((v, x))
+    bc = [Fraction(0)] * n
+    for s in range(n):
+        best = {}
+        paths = {}
+
+        def walk(u, total, seen, inner):
+            for v, x in adj[u]:
+                if v in seen:
+                    continue
+                t = total + x
+                if v not in best or t < best[v]:
+                    best[v] = t
+                    paths[v] = [inner]
+                elif t == best[v]:
+                    paths[v].append(inner)
+                walk(v, t, seen | {v}, inner + [v])
+
+        walk(s, 0.0, {s}, [])
+        for t, plist in paths.items():
+            if t == s:
+                continue
+            # keep only the shortest ones (walk recorded every improvement)
+            good = [p for p in plist]
+            k = len(good)
+            for p in good:
+                for v in p:
+                    bc[v] += Fraction(1, k)
+    return [float(x) for x in bc]
+
+
+def test_weighted_reference_cases(oracle_mod):
+    O = oracle_mod
+    # centrality_test.exs:55-65 and centrality.zig:1436-1454: bridge on a chain
+    g = O.Graph(3, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    g.set_weights([1.0, 1.0])
+    bc = g.betweenness_f64()
+    assert bc[1] > bc[0] and bc[1] > bc[2]
+    # centrality_test.exs:67-79: the 10.0 shortcut is longer than A->B->C
+    g = O.Graph(3, np.array([0, 1, 0], np.uint32), np.array([1, 2, 2], np.uint32))
+    g.set_weights([1.0, 1.0, 10.0])
+    assert g.betweenness_f64().tolist() == [0.0, 1.0, 0.0]
+    g.set_weights([1.0, 1.0, 1.5])
+    assert g.betweenness_f64().tolist() == [0.0, 0.0, 0.0]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_weighted_brute_force(oracle_mod, seed):
+    # integer-valued weights so that ties (equal sums) are exact
+    O = oracle_mod
+    rng = np.random.default_rng(500 + seed)
+    n = 7
+    m = 16
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    w = rng.integers(1, 4, len(src)).astype(np.float64)
+    g = O.Graph(n, src, dst)
+    g.set_weights(w)
+    got = g.betweenness_f64()
+    ref = brute_bc_weighted(n, src, dst, w)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (got, ref)
+
+
+def test_weighted_unit_weights_equal_unweighted(oracle_mod):
+    # all weights 1.0: the same shortest paths (parallel arcs included)
+    O = oracle_mod
+    rng = np.random.default_rng(9)
+    n, src, dst = graphs.random_multigraph(rng, 200, 1500)
+    g = O.Graph(n, src, dst)
+    g.set_weights(np.ones(len(src)))
+    ref, _ = g.betweenness()
+    got = g.betweenness_f64()
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-9)
+
+
+def test_weighted_networkx(oracle_mod):
+    nx = pytest.importorskip("networkx")
+    O = oracle_mod
+    rng = np.random.default_rng(13)
+    n = 120
+    src = rng.integers(0, n, 900).astype(np.uint32)
+    dst = rng.integers(0, n, 900).astype(np.uint32)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    _, first = np.unique(src.astype(np.int64) * n + dst, return_index=True)
+    first.sort()
+    src, dst = src[first], dst[first]
+    w = rng.uniform(0.1, 3.0, len(src))
+    g = O.Graph(n, src, dst)
+    g.set_weights(w)
+    got = g.betweenness_f64()
+    G = nx.DiGraph()
+    G.add_nodes_from(range(n))
+    for a, b, x in zip(src.tolist(), dst.tolist(), w.tolist()):
+        G.add_edge(a, b, weight=x)
+    ref = nx.betweenness_centrality(G, normalized=False, weight="weight")
+    assert np.allclose(got, [ref[v] for v in range(n)], rtol=1e-12, atol=1e-9)

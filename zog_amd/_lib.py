@@ -39,12 +39,14 @@ ZG_TUNE_PR_GRID = 6
 # Every symbol include/zog_hip.h declares.
 EXPORTED = (
     "zg_graph_create",
+    "zg_graph_create_weighted",
     "zg_graph_create_batch",
     "zg_graph_components",
     "zg_graph_destroy",
     "zg_graph_info",
     "zg_bc_unweighted",
     "zg_bc_unweighted_device",
+    "zg_bc_f64",
     "zg_pagerank",
     "zg_pagerank_device",
     "zg_bc_paths",
@@ -148,6 +150,8 @@ def lib():
     P = ctypes.c_void_p
     L.zg_graph_create.argtypes = [ctypes.c_uint32, P, P, P, ctypes.c_uint64, ctypes.POINTER(P)]
     L.zg_graph_create.restype = ctypes.c_int
+    L.zg_graph_create_weighted.argtypes = [ctypes.c_uint32, P, P, P, P, ctypes.c_uint64, ctypes.POINTER(P)]
+    L.zg_graph_create_weighted.restype = ctypes.c_int
     L.zg_graph_create_batch.argtypes = [ctypes.c_uint32, P, P, P, P, ctypes.POINTER(P)]
     L.zg_graph_create_batch.restype = ctypes.c_int
     L.zg_graph_components.argtypes = [P, P]
@@ -160,6 +164,8 @@ def lib():
     L.zg_bc_unweighted.restype = ctypes.c_int
     L.zg_bc_unweighted_device.argtypes = [P, P, ctypes.c_uint64, P, P, ctypes.POINTER(ZgBcInfo)]
     L.zg_bc_unweighted_device.restype = ctypes.c_int
+    L.zg_bc_f64.argtypes = [P, P, ctypes.c_uint64, P, ctypes.POINTER(ZgBcInfo)]
+    L.zg_bc_f64.restype = ctypes.c_int
     L.zg_pagerank.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int, P,
                               ctypes.POINTER(ZgPrInfo)]
     L.zg_pagerank.restype = ctypes.c_int

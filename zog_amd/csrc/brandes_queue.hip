@@ -29,6 +29,7 @@
 // from lstart/qlen, so the host queues ZG_Q_LEVELS_PER_SYNC levels at a time
 // and syncs only to find the last level.
 #include "common.h"
+#include "queue_map.h"
 
 #include <algorithm>
 
@@ -54,52 +55,6 @@ struct BcqDev {
 };
 
 #define QROW(s) ((size_t)(s) * a.V)
-
-// Per-source ranges of this kernel's level into LDS, and the prefix of
-// 64-pair chunk counts; returns the total number of chunks.
-template <typename RangeFn>
-__device__ __forceinline__ uint32_t bcq_chunks(const BcqDev &a, RangeFn range, uint32_t *s_lo, uint32_t *s_hi,
-                                               uint32_t *s_pre)
-{
-    __shared__ uint32_t wtot[4];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < a.S; base += blockDim.x) {
-        const uint32_t s = base + tid;
-        uint32_t lo = 0, hi = 0;
-        if (s < a.S) range(s, lo, hi);
-        const uint32_t cnt = hi > lo ? (hi - lo + 63) / 64 : 0;
-        if (s < a.S) {
-            s_lo[s] = lo;
-            s_hi[s] = hi;
-        }
-        const uint32_t incl = zg_wave_incl_scan(cnt, lane);
-        if (lane == 63) wtot[wv] = incl;
-        __syncthreads();
-        uint32_t off = carry;
-        for (int w = 0; w < wv; w++) off += wtot[w];
-        if (s < a.S) s_pre[s] = off + incl - cnt;
-        uint32_t tot = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += wtot[w];
-        carry += tot;
-        __syncthreads();
-    }
-    if (tid == 0) s_pre[a.S] = carry;
-    __syncthreads();
-    return s_pre[a.S];
-}
-
-// Source owning chunk c: the last s with s_pre[s] <= c.
-__device__ __forceinline__ uint32_t bcq_owner(const uint32_t *s_pre, uint32_t S, uint32_t c)
-{
-    uint32_t lo = 0, hi = S;  // s_pre[lo] <= c < s_pre[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_pre[mid] <= c) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __global__ void bcq_init_kernel(BcqDev a)
 {
@@ -127,8 +82,8 @@ __global__ void bcq_init_kernel(BcqDev a)
 __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
 {
     __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
-    const uint32_t total = bcq_chunks(
-        a,
+    const uint32_t total = zg_q_chunks(
+        a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
             lo = a.lstart[(size_t)d * a.S + s];
             hi = a.lstart[(size_t)(d + 1) * a.S + s];
@@ -137,7 +92,7 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
     const int lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
-        const uint32_t s = bcq_owner(s_pre, a.S, c);
+        const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
         const uint32_t v = valid ? a.queue[QROW(s) + idx] : 0;
@@ -178,8 +133,8 @@ __global__ __launch_bounds__(256) void bcq_close_kernel(BcqDev a, int L)
 {
     __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
     __shared__ unsigned long long red[2][4];
-    const uint32_t total = bcq_chunks(
-        a,
+    const uint32_t total = zg_q_chunks(
+        a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
             lo = a.lstart[(size_t)L * a.S + s];
             hi = a.qlen[s];
@@ -199,7 +154,7 @@ __global__ __launch_bounds__(256) void bcq_close_kernel(BcqDev a, int L)
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const size_t cc1 = (size_t)a.S * a.V;
     for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + wv; c < total; c += waves) {
-        const uint32_t s = bcq_owner(s_pre, a.S, c);
+        const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         int ex = -100000;
         if (idx < s_hi[s]) {
@@ -235,8 +190,8 @@ __global__ __launch_bounds__(256) void bcq_close_kernel(BcqDev a, int L)
 __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
 {
     __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
-    const uint32_t total = bcq_chunks(
-        a,
+    const uint32_t total = zg_q_chunks(
+        a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
             lo = a.lstart[(size_t)d * a.S + s];
             hi = a.lstart[(size_t)(d + 1) * a.S + s];
@@ -247,7 +202,7 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
     const double *cin = a.cc + (size_t)((d + 1) & 1) * a.S * a.V;
     double *cout = a.cc + (size_t)(d & 1) * a.S * a.V;
     for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
-        const uint32_t s = bcq_owner(s_pre, a.S, c);
+        const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         if (idx >= s_hi[s]) continue;
         const uint32_t v = a.queue[QROW(s) + idx];

@@ -144,6 +144,27 @@ struct zg_bcq_ws {
     unsigned long long *stats = nullptr; // [2]
 };
 
+// Weighted (f64 arc weight) engine workspace (brandes_weighted.hip), S
+// sources per batch.
+struct zg_wbc_ws {
+    uint32_t S = 0, lcap = 0;
+    unsigned long long *dist = nullptr;  // [S][V] distance bits (all ones = unreached)
+    double *sig = nullptr, *cc = nullptr;  // [S][V]
+    uint32_t *indeg = nullptr;           // [S][V]
+    uint32_t *stamp = nullptr;           // [S][V]
+    uint32_t *fr = nullptr;              // [2][S][V]
+    uint32_t *fcnt = nullptr;            // [2][S]
+    uint32_t *reach = nullptr;           // [S][V]
+    uint32_t *rcnt = nullptr;            // [S]
+    uint32_t *order = nullptr;           // [S][V]
+    uint32_t *qlen = nullptr;            // [S]
+    uint32_t *lstart = nullptr;          // [lcap][S]
+    uint32_t *tot = nullptr;             // [lcap]
+    uint32_t *src = nullptr;             // [S]
+    double *bc = nullptr;                // [V]
+    unsigned long long *stats = nullptr; // [2]
+};
+
 // PageRank workspace: double-buffered ranks and gathered values.
 struct zg_pr_ws {
     int prec = 0;              // 32 or 64
@@ -206,14 +227,17 @@ struct zg_graph {
     uint32_t *pr_seg_e0 = nullptr, *pr_seg_e1 = nullptr;
     int groups = 4;
     int engine = 0;                      // ZG_ENGINE_*
+    double *out_w = nullptr;             // [E] arc weights in successor-CSR order (weighted resources)
+    bool weights_ok = true;              // every live arc's weight is finite and non-negative
     uint32_t *und_ptr = nullptr, *und_col = nullptr;  // simple undirected CSR (core numbers), built on demand
     uint32_t pull_grid = 4096;
     uint32_t pr_xcache = 1u << 30;       // PageRank values staged in LDS (capped by the LDS size; 0 = none)
     bool pr_xcache_useful = false;       // the cached values carry a useful share of the gathers
-    uint32_t pr_grid_cap = 0;            // PageRank iteration blocks (0 = default)           // pull workgroups per source group (0 = one wave per chunk)
+    uint32_t pr_grid_cap = 0;            // PageRank iteration blocks (0 = default)
     bool high_diameter = false;          // set with h_coredist
     zg_bc_ws bc;
     zg_bcq_ws bcq;
+    zg_wbc_ws wbc;
     zg_pr_ws pr;
     // per-kernel profiling (events around launches, summed on read)
     bool profiling = false;
@@ -240,6 +264,7 @@ void zg_prof_release(zg_graph *g);
 
 int zg_bc_ws_release(zg_graph *g);
 int zg_bcq_ws_release(zg_graph *g);
+int zg_wbc_ws_release(zg_graph *g);
 int zg_pr_ws_release(zg_graph *g);
 int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const double *d_weights, double *h_totals,
                  double *d_out, hipStream_t s, zg_bc_info *inf);

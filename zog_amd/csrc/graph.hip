@@ -9,6 +9,8 @@
 // ascending by source with duplicates kept.
 #include "common.h"
 
+#include <cmath>
+
 #include <algorithm>
 #include <new>
 
@@ -98,8 +100,9 @@ static void zg_free_device(zg_graph *g)
     zg_prof_release(g);
     zg_bc_ws_release(g);
     zg_bcq_ws_release(g);
+    zg_wbc_ws_release(g);
     zg_pr_ws_release(g);
-    zg_dfree(g->out_ptr); zg_dfree(g->out_col); zg_dfree(g->in_ptr); zg_dfree(g->in_col);
+    zg_dfree(g->out_ptr); zg_dfree(g->out_col); zg_dfree(g->in_ptr); zg_dfree(g->in_col); zg_dfree(g->out_w);
     zg_dfree(g->und_ptr); zg_dfree(g->und_col);
     zg_dfree(g->live);
     zg_dfree(g->ich_v); zg_dfree(g->ich_e0); zg_dfree(g->ich_e1); zg_dfree(g->inhub);
@@ -122,7 +125,7 @@ static int zg_upload(T **d, const std::vector<T> &h)
 }
 
 static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
-                    uint64_t m)
+                    const double *weight, uint64_t m)
 {
     const uint32_t V = g->V;
     if (g->h_comp_off.empty()) g->h_comp_off = {0u, V};
@@ -145,15 +148,25 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     const uint64_t E = out_ptr[V];
     g->E = E;
     std::vector<uint32_t> out_col(E);
+    std::vector<double> out_w(weight ? E : 0);
     {
         // filling each row from its end in insertion order leaves the newest
         // arc first
         std::vector<uint32_t> fill(out_ptr.begin() + 1, out_ptr.end());
         for (uint64_t e = 0; e < m; e++) {
             uint32_t f = from[e], t = to[e];
-            if (g->h_live[f] && g->h_live[t]) out_col[--fill[f]] = t;
+            if (g->h_live[f] && g->h_live[t]) {
+                const uint32_t k = --fill[f];
+                out_col[k] = t;
+                if (weight) {
+                    out_w[k] = weight[e];
+                    // NaN, negative or infinite: no shortest paths under the reference's semantics
+                    if (!(weight[e] >= 0.0) || weight[e] == HUGE_VAL) g->weights_ok = false;
+                }
+            }
         }
     }
+    if (weight) ZG_TRY_RC(zg_upload(&g->out_w, out_w));
     // Predecessor CSR: scan sources in id order.
     std::vector<uint32_t> in_ptr((size_t)V + 1, 0);
     for (uint64_t k = 0; k < E; k++) in_ptr[(size_t)out_col[k] + 1]++;
@@ -309,7 +322,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
 }
 
 static int zg_create_common(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
-                            uint64_t m, std::vector<uint32_t> comp_off, zg_graph **out)
+                            const double *weight, uint64_t m, std::vector<uint32_t> comp_off, zg_graph **out)
 {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -326,7 +339,7 @@ static int zg_create_common(uint32_t n_cap, const uint8_t *live_mask, const uint
     if (rc == ZG_OK) {
         try {
             g->h_comp_off = std::move(comp_off);
-            rc = zg_build(g, live_mask, from, to, m);
+            rc = zg_build(g, live_mask, from, to, weight, m);
         } catch (const std::bad_alloc &) {
             rc = ZG_ERR_OOM;
         }
@@ -347,7 +360,17 @@ extern "C" int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const u
     *out = nullptr;
     if (m && (!from || !to)) return ZG_ERR_INVALID_ARG;
     if (m >= 0xFFFFFFFFull || n_cap == 0xFFFFFFFFu) return ZG_ERR_TOO_LARGE;
-    return zg_create_common(n_cap, live_mask, from, to, m, {}, out);
+    return zg_create_common(n_cap, live_mask, from, to, nullptr, m, {}, out);
+}
+
+extern "C" int zg_graph_create_weighted(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
+                                        const uint32_t *to, const double *weight, uint64_t m, zg_graph **out)
+{
+    if (!out) return ZG_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (m && (!from || !to || !weight)) return ZG_ERR_INVALID_ARG;
+    if (m >= 0xFFFFFFFFull || n_cap == 0xFFFFFFFFu) return ZG_ERR_TOO_LARGE;
+    return zg_create_common(n_cap, live_mask, from, to, weight, m, {}, out);
 }
 
 extern "C" int zg_graph_create_batch(uint32_t n_graphs, const uint32_t *node_counts, const uint64_t *arc_counts,
@@ -384,7 +407,7 @@ extern "C" int zg_graph_create_batch(uint32_t n_graphs, const uint32_t *node_cou
     } catch (const std::bad_alloc &) {
         return ZG_ERR_OOM;
     }
-    return zg_create_common((uint32_t)V, nullptr, gf.data(), gt.data(), M, std::move(comp_off), out);
+    return zg_create_common((uint32_t)V, nullptr, gf.data(), gt.data(), nullptr, M, std::move(comp_off), out);
 }
 
 extern "C" int zg_graph_components(const zg_graph *g, uint32_t *n_components)

@@ -10,6 +10,8 @@ Reference surface kept:
   * ``destroy(graph)``                         resource_graph.ex:1577-1580
   * ``betweenness_unweighted(graph, raw=False)``  resource_graph.ex:1590-1601
     (x0.5 for undirected builders before the raw branch, :2633-2637)
+  * ``betweenness_f64(graph, raw=False)``       resource_graph.ex:1610-1621
+    (arc weights from the builder, as buildGraph keeps them, :120-129)
   * ``pagerank(graph, damping=0.85, max_iterations=100, tolerance=0.0001,
     raw=False)``                               resource_graph.ex:1671-1685
   * ``raw=True`` returns the dense list indexed by u32 id; otherwise scores
@@ -40,7 +42,7 @@ BACKENDS = ("soa",)
 class Resource:
     """Owner of one ``zg_graph`` handle (the GraphRes resource)."""
 
-    def __init__(self, n_cap: int, src: np.ndarray, dst: np.ndarray, live_mask=None):
+    def __init__(self, n_cap: int, src: np.ndarray, dst: np.ndarray, live_mask=None, weights=None):
         L = _lib.lib()
         src = np.ascontiguousarray(src, dtype=np.uint32)
         dst = np.ascontiguousarray(dst, dtype=np.uint32)
@@ -48,8 +50,17 @@ class Resource:
             raise ValueError("from/to arrays differ in length")
         live = None if live_mask is None else np.ascontiguousarray(live_mask, dtype=np.uint8)
         h = ctypes.c_void_p()
-        rc = L.zg_graph_create(int(n_cap), _ptr(live), _ptr(src), _ptr(dst), len(src), ctypes.byref(h))
-        _lib.check(rc, "zg_graph_create")
+        if weights is None:
+            rc = L.zg_graph_create(int(n_cap), _ptr(live), _ptr(src), _ptr(dst), len(src), ctypes.byref(h))
+            _lib.check(rc, "zg_graph_create")
+        else:
+            w = np.ascontiguousarray(weights, dtype=np.float64)
+            if len(w) != len(src):
+                raise ValueError("weight array differs in length from the arcs")
+            rc = L.zg_graph_create_weighted(int(n_cap), _ptr(live), _ptr(src), _ptr(dst), _ptr(w), len(src),
+                                            ctypes.byref(h))
+            _lib.check(rc, "zg_graph_create_weighted")
+        self.weighted = weights is not None
         self._h = h
         self._lock = threading.Lock()
         self.n_cap = int(n_cap)
@@ -105,6 +116,7 @@ class BatchResource(Resource):
         _lib.check(rc, "zg_graph_create_batch")
         self._h = h
         self._lock = threading.Lock()
+        self.weighted = False
         self.n_cap = int(ns.sum())
         self._live = np.ones(self.n_cap, bool)
         self.offsets = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
@@ -144,10 +156,12 @@ class ResourceGraph(dict):
 
 
 def new(builder: SoA, backend: str = "soa", live_mask=None) -> ResourceGraph:
+    """new/5 with the builder's arcs and weights (resource_graph.ex:1393-1404,
+    buildGraph :120-129)."""
     if backend not in BACKENDS:
         raise ValueError(f"backend {backend!r} is not accelerated here (only :soa)")
-    src, dst, _w = builder.to_edge_arrays()
-    res = Resource(builder.node_count(), src, dst, live_mask)
+    src, dst, w = builder.to_edge_arrays()
+    res = Resource(builder.node_count(), src, dst, live_mask, weights=w)
     return ResourceGraph(resource=res, builder=builder)
 
 
@@ -210,22 +224,24 @@ def _expand_edges(edges, directed: bool):
     undirected edges sorted by (min, max), duplicates dropped, both arcs added
     except for a self-loop (one arc)."""
     if directed:
-        return [e[0] for e in edges], [e[1] for e in edges]
+        return [e[0] for e in edges], [e[1] for e in edges], [e[2] for e in edges]
     order = sorted(range(len(edges)), key=lambda i: (min(edges[i][0], edges[i][1]), max(edges[i][0], edges[i][1])))
-    src, dst = [], []
+    src, dst, wts = [], [], []
     prev = None
     for i in order:
-        u, v, _ = edges[i]
+        u, v, w = edges[i]
         key = (min(u, v), max(u, v))
         if key == prev:
             continue
         src.append(u)
         dst.append(v)
+        wts.append(w)
         if u != v:
             src.append(v)
             dst.append(u)
+            wts.append(w)
         prev = key
-    return src, dst
+    return src, dst, wts
 
 
 def read_edgelist(path: str, directed: bool = True, backend: str = "soa",
@@ -233,7 +249,7 @@ def read_edgelist(path: str, directed: bool = True, backend: str = "soa",
     if backend not in BACKENDS:
         raise ValueError(f"backend {backend!r} is not accelerated here (only :soa)")
     edges, next_id, labels = _parse_edgelist(os.path.expanduser(path), integer_labels)
-    src, dst = _expand_edges(edges, directed)
+    src, dst, wts = _expand_edges(edges, directed)
     kind = "directed" if directed else "undirected"
     # build_from_labels (resource_graph.ex:1531-1575): the builder carries only
     # labels; its edge list stays empty
@@ -244,7 +260,8 @@ def read_edgelist(path: str, directed: bool = True, backend: str = "soa",
     else:
         for lab in labels:
             b.ensure_node(lab)
-    res = Resource(next_id, np.asarray(src, dtype=np.uint32), np.asarray(dst, dtype=np.uint32))
+    res = Resource(next_id, np.asarray(src, dtype=np.uint32), np.asarray(dst, dtype=np.uint32),
+                   weights=np.asarray(wts, dtype=np.float64))
     return ResourceGraph(resource=res, builder=b)
 
 
@@ -282,6 +299,33 @@ def betweenness_unweighted_array(graph: ResourceGraph, sources=None, info: Optio
 def betweenness_unweighted(graph: ResourceGraph, raw: bool = False, *, sources=None,
                            info: Optional[dict] = None):
     raw_scores = betweenness_unweighted_array(graph, sources=sources, info=info)
+    scores = _maybe_scale_undirected(graph.builder, raw_scores)
+    if raw:
+        return scores.tolist()
+    return _map_scores(graph.builder, scores.tolist())
+
+
+def betweenness_f64_array(graph: ResourceGraph, sources=None, info: Optional[dict] = None) -> np.ndarray:
+    """Unscaled directed-sum weighted BC as f64[n_cap] (nif_betweenness_f64,
+    resource_graph.ex:217-226)."""
+    res = graph.resource
+    out = np.zeros(res.n_cap, dtype=np.float64)
+    inf = _lib.ZgBcInfo()
+    if sources is None:
+        rc = _lib.lib().zg_bc_f64(res.handle, None, 0, _ptr(out), ctypes.byref(inf))
+    else:
+        s = np.ascontiguousarray(sources, dtype=np.uint32)
+        rc = _lib.lib().zg_bc_f64(res.handle, _ptr(s), len(s), _ptr(out), ctypes.byref(inf))
+    _lib.check(rc, "zg_bc_f64")
+    if info is not None:
+        info.update({k: getattr(inf, k) for k, _ in _lib.ZgBcInfo._fields_})
+    return out
+
+
+def betweenness_f64(graph: ResourceGraph, raw: bool = False, *, sources=None, info: Optional[dict] = None):
+    """Weighted betweenness (resource_graph.ex:1610-1621): x0.5 for an
+    undirected builder, then the raw list or label-mapped scores."""
+    raw_scores = betweenness_f64_array(graph, sources=sources, info=info)
     scores = _maybe_scale_undirected(graph.builder, raw_scores)
     if raw:
         return scores.tolist()
