@@ -197,6 +197,135 @@ def bench_pivots(args):
     sc.close()
 
 
+def bench_weighted(args):
+    """Weighted Brandes (zg_bc_f64) on rmat20's graph and 4,096 sources with
+    seeded arc weights uniform in (0, 1]."""
+    import numpy as np
+    import torch
+
+    from zog_amd import _lib
+    from zog_amd import resource_graph as rg
+    from zog_amd import workloads as W
+    from zog_amd.soa import SoA
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    n, src, dst = W.rmat20()
+    w = W.rmat20w_weights(len(src))
+    sources = W.rmat20_sources(n, src, dst, 4096)
+    log(f"[bench] rmat20w generated in {time.time() - t0:.1f}s: n={n} arcs={len(src)} sources={len(sources)}")
+    g = rg.new(SoA.from_arrays("directed", n, src, dst, w))
+    V, n_live, E = g.resource.info()
+    L = _lib.lib()
+    h = g.resource.handle
+    out = np.zeros(V, dtype=np.float64)
+    info = _lib.ZgBcInfo()
+
+    def step(srcs=sources):
+        rc = L.zg_bc_f64(h, ctypes.c_void_p(srcs.ctypes.data), len(srcs), ctypes.c_void_p(out.ctypes.data),
+                         ctypes.byref(info))
+        _lib.check(rc, "zg_bc_f64")
+
+    def set_batch(b):
+        _lib.check(L.zg_set_tuning(h, _lib.ZG_TUNE_WEIGHTED_BATCH, float(b)), "zg_set_tuning")
+
+    def set_delta(d):
+        _lib.check(L.zg_set_tuning(h, _lib.ZG_TUNE_WEIGHTED_DELTA, float(d)), "zg_set_tuning")
+
+    tuned = None
+    if args.wbatch is not None:
+        set_batch(args.wbatch)
+    if args.wdelta is not None:
+        set_delta(args.wdelta)
+    if not args.no_autotune and args.wdelta is None:
+        # bound step of the near frontier (0 = the library's default), on a
+        # quarter of the sources: small steps relax each vertex fewer times,
+        # large ones run fewer rounds
+        sub = np.ascontiguousarray(sources[:1024])
+        t_best = None
+        for d in (0.0, 0.01, 0.04, float("inf")):
+            set_delta(d)
+            step(sub)
+            log(f"[bench] tune: delta {d}: {info.kernel_ms:.1f} ms for {len(sub)} sources, "
+                f"{info.push_levels} rounds, relaxed/arcs {info.arcs_relaxed / max(info.arcs_traversed, 1):.2f}")
+            if t_best is None or info.kernel_ms < t_best[0]:
+                t_best = (info.kernel_ms, d)
+        set_delta(t_best[1])
+        tuned = {"delta": t_best[1] if t_best[1] != float("inf") else "inf"}
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup {i} done ({info.kernel_ms:.1f} ms)")
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    arcs = relaxed = reached = 0
+    kernel_ms = 0.0
+    for i in range(args.steps):
+        step()
+        arcs += info.arcs_traversed
+        relaxed += info.arcs_relaxed
+        reached += info.nodes_reached
+        kernel_ms += info.kernel_ms
+        log(f"[bench] step {i} done ({info.kernel_ms:.1f} ms)")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    K = args.steps
+    arcs_step, relaxed_step, nodes_step = arcs // K, relaxed // K, reached // K
+    rg.set_profiling(g, True)
+    step()
+    prof = rg.get_profile(g)
+    rg.set_profiling(g, False)
+    log(f"[bench] profile {prof}")
+    # algorithmic bytes per pass: an arc is its head id (4 B), its weight
+    # (8 B) and the head's distance (8 B); a vertex its row bounds (8 B), its
+    # distance (8 B) and, in the path-count and dependency passes, its sigma
+    # or c value (8 B)
+    kb = {"wbc_relax_kernel": (20 * relaxed_step, prof["bc_forward"]),
+          "wbc_count_kernel+wbc_peel_kernel": (40 * arcs_step + 48 * nodes_step, prof["bc_dag"]),
+          "wbc_backward_kernel": (20 * arcs_step + 24 * nodes_step, prof["bc_backward"])}
+    dom = max(kb, key=lambda k: kb[k][1][0])
+    dom_bytes, (dom_ms, dom_n) = kb[dom]
+    achieved = dom_bytes / (dom_ms / 1000.0) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "launches": dom_n,
+                "avg_launch_us": round(dom_ms * 1000.0 / max(dom_n, 1), 2),
+                "algorithmic_bytes_per_step": dom_bytes,
+                "kernel_ms_per_step": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}
+    cpu = None
+    parity = {}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        og = O.Graph(n, src, dst)
+        og.set_weights(w)
+        sample = sources[:6]
+        t0 = time.perf_counter()
+        ref = og.betweenness_f64(sources=sample)
+        cpu_s = time.perf_counter() - t0
+        got = rg.betweenness_f64_array(g, sources=sample, info=(ginf := {}))
+        cpu_gteps = ginf["arcs_traversed"] / cpu_s / 1e9
+        scale = max(1.0, float(np.abs(ref).max()))
+        parity[f"bc_max_rel_err_{len(sample)}_sources"] = float(
+            (np.abs(got - ref) / np.maximum(np.abs(ref), 1e-9 * scale)).max())
+        log(f"[bench] oracle weighted BC {len(sample)} sources in {cpu_s:.1f}s = {cpu_gteps:.4f} GTEPS")
+        cpu = {"value": round(cpu_gteps, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": f"oracle/oracle.c weighted Brandes (restated centrality.zig:529-667, binary heap), "
+                         f"1 thread, {len(sample)} of the {len(sources)} sources, host cores of the GPU box"}
+    line = {
+        "metric": "weighted Brandes GTEPS (exact betweennessF64)",
+        "value": round(arcs / wall / 1e9, 3), "unit": "GTEPS", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1000.0 / K, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "rmat20w: R-MAT scale 20, 2^24 distinct arcs, seeded f64 weights uniform in (0, 1], "
+                               "exact weighted Brandes from 4096 seeded sources",
+                   "nodes": n, "arcs": int(E), "sources": len(sources), "arcs_per_step": arcs_step,
+                   "arcs_relaxed_per_step": relaxed_step, "relax_rounds": int(info.push_levels),
+                   "max_path_arcs": int(info.max_levels) - 1, "batches": int(info.batches), "autotuned": tuned,
+                   "kernel_ms_per_step": round(kernel_ms / K, 1)},
+        "roofline": roofline, "cpu_baseline": cpu, "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    rg.destroy(g)
+
+
 def workload_spec(name):
     from zog_amd import workloads as W
     if name == "rmat20":
@@ -242,7 +371,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512", "pivots", "copy"])
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512", "rmat20w", "pivots", "copy"])
     ap.add_argument("--train-per-scale", type=int, default=40, help="pivots: R-MAT training graphs per scale 14..18")
     ap.add_argument("--epochs", type=int, default=30, help="pivots: training epochs")
     ap.add_argument("--eval-scale", type=int, default=20, help="pivots: R-MAT scale of the judged graph")
@@ -257,6 +386,8 @@ def main():
     ap.add_argument("--pr-cache", type=int, default=None, help="PageRank values staged in LDS (0 = none)")
     ap.add_argument("--pr-grid", type=int, default=None, help="PageRank iteration blocks")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--wbatch", type=int, default=None, help="rmat20w: sources per batch (multiple of 64)")
+    ap.add_argument("--wdelta", type=float, default=None, help="rmat20w: near-frontier bound step (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -274,6 +405,8 @@ def main():
         return bench_pivots(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "rmat20w":
+        return bench_weighted(args)
     spec = workload_spec(args.workload)
     if args.workload == "ba512":
         return bench_batch(args, spec)

@@ -40,17 +40,24 @@ typedef struct {
     uint64_t arcs_traversed;  /* sum over sources of arcs scanned by the forward
                                  sweep (arcs whose tail is reachable) */
     uint64_t nodes_reached;   /* sum over sources of reached vertices */
-    uint32_t max_levels;      /* deepest BFS over all sources, in levels */
+    uint32_t max_levels;      /* deepest BFS over all sources, in levels
+                                 (zg_bc_f64: most arcs on a shortest path, + 1) */
     uint32_t batches;         /* batches of the multi-source sweep */
-    uint32_t push_levels;     /* forward levels run from the frontier (push) */
+    uint32_t push_levels;     /* forward levels run from the frontier (push;
+                                 zg_bc_f64: distance rounds launched) */
     uint32_t pull_levels;     /* forward levels run over predecessor chunks (pull) */
     double kernel_ms;         /* device time of the sweep (HIP events) */
+    uint64_t arcs_relaxed;    /* zg_bc_f64: arcs scanned by the distance rounds,
+                                 a vertex's arcs once per round it is queued
+                                 in (0 for the unweighted sweeps) */
 } zg_bc_info;
 
-/* Per-run information for PageRank. */
+/* Per-run information for PageRank (and the eigenvector, Katz and alpha
+ * iterations below). */
 typedef struct {
     uint64_t iterations;      /* power iterations performed */
-    double last_l1;           /* L1 change of the last iteration, reduced in FP64 */
+    double last_l1;           /* L1 change of the last iteration, reduced in FP64
+                                 (eigenvector: the L2 change) */
     double residual_l1;       /* ||G(r) - r||_1 of the returned vector,
                                  evaluated in FP64 from the stored FP32 ranks */
     double error_bound_l1;    /* residual_l1 / (1 - damping): bound on the
@@ -58,7 +65,7 @@ typedef struct {
                                  exact fixed point */
     double kernel_ms;         /* device time of the iterations (HIP events) */
     int32_t precision;        /* ZG_PR_FP32, ZG_PR_FP64 or ZG_PR_MIXED */
-    int32_t reserved;
+    int32_t outcome;          /* zg_eigenvector: ZG_EIG_* (0 otherwise) */
     uint64_t fp64_iterations; /* iterations run in FP64 by the mixed mode */
 } zg_pr_info;
 
@@ -167,6 +174,37 @@ int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations, double tol
 int zg_pagerank_device(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
                        int precision, double *d_out, void *stream, zg_pr_info *info);
 
+/* How zg_eigenvector's iteration ended (zg_pr_info.outcome). */
+#define ZG_EIG_CONVERGED 0    /* L2 change below tolerance: the new iterate */
+#define ZG_EIG_OSCILLATING 1  /* iterate within tolerance of the one two steps
+                                 back: the normalised average of the last two */
+#define ZG_EIG_ZERO 2         /* the iterate vanished: 1/sqrt(n) everywhere */
+#define ZG_EIG_CAP 3          /* max_iterations reached: the last iterate */
+#define ZG_EIG_TRIVIAL 4      /* at most one live node: 1.0 */
+
+/* Replaces the `eigenvector/3` NIF (resource_graph.ex:266-275) over
+ * zog.centrality.eigenvector (priv/zog/src/centrality.zig:1050-1201) on an
+ * ArrayGraph, followed by extractScores: out[n_cap] (0.0 for non-live ids).
+ * Power iteration over the predecessor lists (utils.zig:52-73) in FP64 from
+ * 1 + i/1000 for the i-th live id, L2-normalised each step, with the
+ * reference's zero-vector, oscillation (compared with the iterate two steps
+ * back), convergence and iteration-cap rules in that order.  info may be
+ * NULL. */
+int zg_eigenvector(zg_graph *g, uint64_t max_iterations, double tolerance, double *out, zg_pr_info *info);
+
+/* Replaces the `katz/5` NIF (resource_graph.ex:277-286) over
+ * zog.centrality.katz (centrality.zig:1212-1267): x <- alpha * sum over
+ * in(v) of x + beta from x = beta, FP64, stop after the step whose L1 change
+ * is below tolerance.  out[n_cap]. */
+int zg_katz(zg_graph *g, double alpha, double beta, uint64_t max_iterations, double tolerance, double *out,
+            zg_pr_info *info);
+
+/* Replaces the `alpha_centrality/5` NIF (resource_graph.ex:288-297) over
+ * zog.centrality.alphaCentrality (centrality.zig:1278-1333): as zg_katz with
+ * x <- alpha * sum from x = initial.  out[n_cap]. */
+int zg_alpha_centrality(zg_graph *g, double alpha, double initial, uint64_t max_iterations, double tolerance,
+                        double *out, zg_pr_info *info);
+
 /* Per-source BFS export for parity checks (no reference counterpart; the
  * reference keeps these workspaces private, centrality.zig:697-703): for each
  * of n_sources sources, depth_out[i*n_cap + v] (0xFFFFFFFF = unreached) and
@@ -211,6 +249,17 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   PageRank iteration blocks (0 = default 2048). */
 #define ZG_TUNE_PR_CACHE 5
 #define ZG_TUNE_PR_GRID 6
+/*   ZG_TUNE_WEIGHTED_BATCH: sources per batch of zg_bc_f64, a multiple of 64
+ *   up to 1024 (0 = default: up to 1024, as device memory allows). */
+#define ZG_TUNE_WEIGHTED_BATCH 7
+/*   ZG_TUNE_WEIGHTED_DELTA: zg_bc_f64 relaxes, per source, only vertices
+ *   whose distance is within a bound that rises in steps of this value once
+ *   nothing under it is left (0 = default: 0.6 x mean arc weight / mean
+ *   out-degree; +inf = no bound).  Only the amount of work changes. */
+#define ZG_TUNE_WEIGHTED_DELTA 8
+/*   ZG_TUNE_WEIGHTED_GRID: workgroups of zg_bc_f64's per-round kernels
+ *   (0 = default 4096). */
+#define ZG_TUNE_WEIGHTED_GRID 9
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the
@@ -228,7 +277,10 @@ int zg_set_source_order(zg_graph *g, int mode);
  * profiling is on, HIP events are recorded on the launch stream around every
  * kernel launch, and the per-kernel device time is summed.  PageRank counts
  * only launches that ran an iteration (not the no-op launches queued past
- * convergence). */
+ * convergence).  zg_bc_f64 reports its distance rounds (relaxation and
+ * far-pile splits) as ZG_PROF_BC_FORWARD,
+ * its dependency rounds as ZG_PROF_BC_BACKWARD and its round bookkeeping as
+ * ZG_PROF_BC_SETUP. */
 #define ZG_PROF_BC_FORWARD 0
 #define ZG_PROF_BC_FINALIZE 1
 #define ZG_PROF_BC_BACKWARD 2
@@ -236,6 +288,7 @@ int zg_set_source_order(zg_graph *g, int mode);
 #define ZG_PROF_BC_SETUP 4       /* per-batch resets, sources, final reduction */
 #define ZG_PROF_PR_ITER 5
 #define ZG_PROF_BC_PUSH 6
+#define ZG_PROF_BC_DAG 7         /* zg_bc_f64: tight-arc counts and path-count rounds */
 #define ZG_PROF_SLOTS 8
 typedef struct {
     double ms[ZG_PROF_SLOTS];

@@ -33,6 +33,9 @@
  *   - weighted Brandes, betweenness (centrality.zig:529-667) as
  *     betweennessF64 instantiates it (:1355-1357: f64 weights, 0.0, a + b,
  *     std.math.order).
+ *   - eigenvector (centrality.zig:1050-1201), Katz (:1212-1267) and alpha
+ *     centrality (:1278-1333) over collectNodes / buildInNeighbors
+ *     (utils.zig:42-73).
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -847,4 +850,217 @@ int zo_bc_f64(const zo_graph *g, const uint32_t *sources, uint64_t n_sources, do
     free(pr); free(pn); free(pc); free(heap);
     free(dist); free(has); free(sigma); free(delta); free(stack);
     return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Eigenvector, Katz and alpha centrality (centrality.zig:1050-1201,          */
+/* :1212-1267, :1278-1333)                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* collectNodes (utils.zig:42-49) and buildInNeighbors (utils.zig:52-73): the
+ * live ids ascending, and for each live `from` in that order, `from` appended
+ * to in[to] for every successor (parallel arcs repeated, self-loops kept). */
+typedef struct {
+    uint32_t n;
+    uint32_t *nodes;
+    uint64_t *in_ptr; /* [V + 1] */
+    uint32_t *in_col;
+} zo_inlists;
+
+static void zo_inlists_free(zo_inlists *L)
+{
+    free(L->nodes);
+    free(L->in_ptr);
+    free(L->in_col);
+}
+
+static int zo_inlists_build(const zo_graph *g, zo_inlists *L)
+{
+    const uint32_t V = g->n_cap;
+    memset(L, 0, sizeof(*L));
+    L->nodes = malloc(sizeof(uint32_t) * (V ? V : 1));
+    L->in_ptr = calloc((size_t)V + 1, sizeof(uint64_t));
+    if (!L->nodes || !L->in_ptr) {
+        zo_inlists_free(L);
+        return ZO_ERR_OOM;
+    }
+    for (uint32_t i = 0; i < V; i++)
+        if (!g->node_deleted[i]) L->nodes[L->n++] = i;
+    for (uint32_t k = 0; k < L->n; k++) {
+        uint32_t cur = g->first_edge[L->nodes[k]], t;
+        while ((t = zo_next_succ(g, &cur)) != ZO_NONE) L->in_ptr[t + 1]++;
+    }
+    for (uint32_t i = 0; i < V; i++) L->in_ptr[i + 1] += L->in_ptr[i];
+    const uint64_t E = L->in_ptr[V];
+    L->in_col = malloc(sizeof(uint32_t) * (E ? E : 1));
+    uint64_t *fill = malloc(sizeof(uint64_t) * (V ? V : 1));
+    if (!L->in_col || !fill) {
+        free(fill);
+        zo_inlists_free(L);
+        return ZO_ERR_OOM;
+    }
+    memcpy(fill, L->in_ptr, sizeof(uint64_t) * V);
+    for (uint32_t k = 0; k < L->n; k++) {
+        const uint32_t v = L->nodes[k];
+        uint32_t cur = g->first_edge[v], t;
+        while ((t = zo_next_succ(g, &cur)) != ZO_NONE) L->in_col[fill[t]++] = v;
+    }
+    free(fill);
+    return ZO_OK;
+}
+
+/* sum over in(v) of x, in list order (the `orelse 0.0` lookups of the
+ * reference: every in-neighbour is live, so it has a value) */
+static double zo_in_sum(const zo_inlists *L, const double *x, uint32_t v)
+{
+    double s = 0.0;
+    for (uint64_t e = L->in_ptr[v]; e < L->in_ptr[v + 1]; e++) s += x[L->in_col[e]];
+    return s;
+}
+
+/* Eigenvector centrality (centrality.zig:1050-1201).  out[n_cap] (0.0 for
+ * non-live).  *iters_out = iterations run, *outcome_out: 0 converged,
+ * 1 oscillation (average returned), 2 zero vector (uniform returned),
+ * 3 iteration cap (last iterate returned), 4 at most one node.  The L2 norms
+ * are summed in node order (the reference sums the first one in hash-map
+ * order, so its last bits may differ). */
+int zo_eigenvector(const zo_graph *g, uint64_t max_iterations, double tolerance, double *out,
+                   uint64_t *iters_out, int *outcome_out)
+{
+    const uint32_t V = g->n_cap;
+    for (uint32_t i = 0; i < V; i++) out[i] = 0.0;
+    zo_inlists L;
+    int rc = zo_inlists_build(g, &L);
+    if (rc) return rc;
+    const uint32_t n = L.n;
+    *iters_out = 0;
+    if (n <= 1) { /* :1061-1064 */
+        for (uint32_t k = 0; k < n; k++) out[L.nodes[k]] = 1.0;
+        *outcome_out = 4;
+        zo_inlists_free(&L);
+        return ZO_OK;
+    }
+    double *pp = calloc(V, sizeof(double)), *cur = calloc(V, sizeof(double)), *nw = calloc(V, sizeof(double));
+    if (!pp || !cur || !nw) {
+        free(pp); free(cur); free(nw);
+        zo_inlists_free(&L);
+        return ZO_ERR_OOM;
+    }
+    for (uint32_t k = 0; k < n; k++) cur[L.nodes[k]] = 1.0 + (double)k / 1000.0; /* :1078-1080 */
+    int have_pp = 0;
+    int outcome = 3;
+    uint64_t it = 0;
+    for (; it < max_iterations; it++) {
+        for (uint32_t k = 0; k < n; k++) nw[L.nodes[k]] = zo_in_sum(&L, cur, L.nodes[k]);
+        double l2 = 0.0;
+        for (uint32_t k = 0; k < n; k++) l2 += nw[L.nodes[k]] * nw[L.nodes[k]];
+        if (l2 > 0.0) {
+            const double norm = sqrt(l2);
+            for (uint32_t k = 0; k < n; k++) nw[L.nodes[k]] /= norm;
+        }
+        if (l2 == 0.0) { /* :1103-1116 */
+            const double u = 1.0 / sqrt((double)n);
+            for (uint32_t k = 0; k < n; k++) out[L.nodes[k]] = u;
+            outcome = 2;
+            break;
+        }
+        double d1 = 0.0;
+        for (uint32_t k = 0; k < n; k++) {
+            const double d = nw[L.nodes[k]] - cur[L.nodes[k]];
+            d1 += d * d;
+        }
+        if (d1 > 0.0) d1 = sqrt(d1);
+        int osc = 0;
+        if (have_pp) { /* :1130-1142 */
+            double d2 = 0.0;
+            for (uint32_t k = 0; k < n; k++) {
+                const double d = nw[L.nodes[k]] - pp[L.nodes[k]];
+                d2 += d * d;
+            }
+            if (d2 > 0.0) d2 = sqrt(d2);
+            osc = d2 < tolerance;
+        }
+        if (osc) { /* :1144-1167: normalised average of the last two */
+            double a2 = 0.0;
+            for (uint32_t k = 0; k < n; k++) {
+                const uint32_t v = L.nodes[k];
+                out[v] = (nw[v] + cur[v]) / 2.0;
+                a2 += out[v] * out[v];
+            }
+            if (a2 > 0.0) {
+                const double an = sqrt(a2);
+                for (uint32_t k = 0; k < n; k++) out[L.nodes[k]] /= an;
+            }
+            outcome = 1;
+            break;
+        }
+        if (d1 < tolerance) { /* :1169-1181 */
+            for (uint32_t k = 0; k < n; k++) out[L.nodes[k]] = nw[L.nodes[k]];
+            outcome = 0;
+            break;
+        }
+        /* rotate: prev_prev <- curr <- new (:1183-1188) */
+        double *t = pp;
+        pp = cur;
+        cur = nw;
+        nw = t;
+        have_pp = 1;
+    }
+    if (outcome == 3)
+        for (uint32_t k = 0; k < n; k++) out[L.nodes[k]] = cur[L.nodes[k]];
+    *iters_out = outcome == 3 ? it : it + 1;
+    *outcome_out = outcome;
+    free(pp); free(cur); free(nw);
+    zo_inlists_free(&L);
+    return ZO_OK;
+}
+
+/* Katz (centrality.zig:1212-1267) when has_beta, else alpha centrality
+ * (:1278-1333) with start value `start`: x <- alpha * sum_in(x) (+ beta),
+ * L1 change in node order, swap, stop when it is below tolerance. */
+int zo_katz_alpha(const zo_graph *g, double alpha, double beta, int has_beta, double start,
+                  uint64_t max_iterations, double tolerance, double *out, uint64_t *iters_out,
+                  double *l1_out)
+{
+    const uint32_t V = g->n_cap;
+    for (uint32_t i = 0; i < V; i++) out[i] = 0.0;
+    zo_inlists L;
+    int rc = zo_inlists_build(g, &L);
+    if (rc) return rc;
+    const uint32_t n = L.n;
+    *iters_out = 0;
+    *l1_out = 0.0;
+    if (n == 0) {
+        zo_inlists_free(&L);
+        return ZO_OK;
+    }
+    double *cur = calloc(V, sizeof(double)), *nw = calloc(V, sizeof(double));
+    if (!cur || !nw) {
+        free(cur); free(nw);
+        zo_inlists_free(&L);
+        return ZO_ERR_OOM;
+    }
+    for (uint32_t k = 0; k < n; k++) cur[L.nodes[k]] = start;
+    uint64_t it = 0;
+    double l1 = 0.0;
+    while (it < max_iterations) {
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t v = L.nodes[k];
+            const double s = zo_in_sum(&L, cur, v);
+            nw[v] = has_beta ? alpha * s + beta : alpha * s;
+        }
+        l1 = 0.0;
+        for (uint32_t k = 0; k < n; k++) l1 += fabs(nw[L.nodes[k]] - cur[L.nodes[k]]);
+        double *t = cur;
+        cur = nw;
+        nw = t;
+        it++;
+        if (l1 < tolerance) break;
+    }
+    for (uint32_t k = 0; k < n; k++) out[L.nodes[k]] = cur[L.nodes[k]];
+    *iters_out = it;
+    *l1_out = l1;
+    free(cur); free(nw);
+    zo_inlists_free(&L);
+    return ZO_OK;
 }

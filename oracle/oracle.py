@@ -57,6 +57,9 @@ def lib():
         L.zo_graph_set_weights.argtypes = [P, P, ctypes.c_uint64]
         L.zo_graph_set_weights.restype = ctypes.c_int
         L.zo_bc_f64.argtypes = [P, P, ctypes.c_uint64, P]
+        L.zo_eigenvector.argtypes = [P, ctypes.c_uint64, ctypes.c_double, P, P, P]
+        L.zo_katz_alpha.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_uint64, ctypes.c_double, P, P, P]
         L.zo_bc_f64.restype = ctypes.c_int
         L.zo_pagerank.restype = ctypes.c_int
         _lib = L
@@ -156,6 +159,37 @@ class Graph:
         if rc != 0:
             raise OracleError(f"zo_core_numbers failed: {rc}")
         return out
+
+    # outcomes of eigenvector(): how the reference's loop ended
+    EIG_CONVERGED, EIG_OSCILLATING, EIG_ZERO, EIG_CAP, EIG_TRIVIAL = range(5)
+
+    def eigenvector(self, max_iterations=100, tolerance=1e-4):
+        """Eigenvector centrality f64[n_cap]; returns (scores, iterations, outcome)."""
+        out = np.zeros(self.n_cap, dtype=np.float64)
+        it = ctypes.c_uint64()
+        oc = ctypes.c_int()
+        rc = lib().zo_eigenvector(self._h, max_iterations, tolerance, _ptr(out), ctypes.byref(it), ctypes.byref(oc))
+        if rc != 0:
+            raise OracleError(f"zo_eigenvector failed: {rc}")
+        return out, int(it.value), int(oc.value)
+
+    def katz(self, alpha=0.1, beta=1.0, max_iterations=100, tolerance=1e-4):
+        """Katz centrality f64[n_cap]; returns (scores, iterations, last_l1)."""
+        return self._katz_alpha(alpha, beta, True, beta, max_iterations, tolerance)
+
+    def alpha_centrality(self, alpha=0.5, initial=1.0, max_iterations=100, tolerance=1e-4):
+        """Alpha centrality f64[n_cap]; returns (scores, iterations, last_l1)."""
+        return self._katz_alpha(alpha, 0.0, False, initial, max_iterations, tolerance)
+
+    def _katz_alpha(self, alpha, beta, has_beta, start, max_iterations, tolerance):
+        out = np.zeros(self.n_cap, dtype=np.float64)
+        it = ctypes.c_uint64()
+        l1 = ctypes.c_double()
+        rc = lib().zo_katz_alpha(self._h, alpha, beta, int(has_beta), start, max_iterations, tolerance, _ptr(out),
+                                 ctypes.byref(it), ctypes.byref(l1))
+        if rc != 0:
+            raise OracleError(f"zo_katz_alpha failed: {rc}")
+        return out, int(it.value), float(l1.value)
 
     def set_weights(self, weights):
         """Edge data f64[m] in arc order."""

@@ -119,9 +119,19 @@ def test_several_source_batches_and_sampled_sources():
     sources = rng.integers(0, n, 300).astype(np.uint32)
     sources[:4] = sources[4]
     got = rg.betweenness_f64_array(g, sources=sources)
-    rg.destroy(g)
-    ok, err = close(got, oracle_bc(n, src, dst, w, sources=sources))
+    ref = oracle_bc(n, src, dst, w, sources=sources)
+    ok, err = close(got, ref)
     assert ok, err
+    # a narrower batch width: five batches, the last one partly empty
+    L = _lib.lib()
+    _lib.check(L.zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_WEIGHTED_BATCH, 64.0))
+    info = {}
+    got = rg.betweenness_f64_array(g, sources=sources, info=info)
+    assert info["batches"] == 5
+    ok, err = close(got, ref)
+    assert ok, err
+    assert L.zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_WEIGHTED_BATCH, 100.0) == _lib.ZG_ERR_INVALID_ARG
+    rg.destroy(g)
 
 
 def test_grid_high_diameter():
@@ -150,6 +160,49 @@ def test_rmat_uniform_weights():
     rg.destroy(g)
     ok, err = close(got, oracle_bc(n, src, dst, w, sources=sources))
     assert ok, err
+
+
+@pytest.mark.parametrize("delta", [0.01, 0.3, float("inf")])
+def test_near_far_bound_steps(delta):
+    # the bound step changes only the amount of work, never the result
+    say("weighted delta", delta)
+    rng = np.random.default_rng(21)
+    n, src, dst = gen.rmat(11, 1 << 15, 17)
+    w = rng.uniform(0.01, 1.0, len(src))
+    sources = rng.choice(n, 200, replace=False).astype(np.uint32)
+    g = make(n, src, dst, w)
+    _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_WEIGHTED_DELTA, delta))
+    info = {}
+    got = rg.betweenness_f64_array(g, sources=sources, info=info)
+    rg.destroy(g)
+    ok, err = close(got, oracle_bc(n, src, dst, w, sources=sources))
+    assert ok, err
+    say("  rounds", info["push_levels"], "arcs relaxed / arcs", info["arcs_relaxed"] / info["arcs_traversed"])
+
+
+def test_hub_pieces():
+    # two hubs of 9,000 and 20,000 arcs (cut into pieces for many
+    # wavefronts) on a random graph, with small bound steps so that the hubs
+    # are relaxed in rounds of their own
+    say("weighted hubs")
+    rng = np.random.default_rng(8)
+    n = 30000
+    hs = np.concatenate([np.zeros(9000, np.uint32), np.full(20000, 1, np.uint32)])
+    hd = np.concatenate([rng.choice(n, 9000, replace=False), rng.choice(n, 20000, replace=False)]).astype(np.uint32)
+    rs = rng.integers(0, n, 60000).astype(np.uint32)
+    rd = rng.integers(0, n, 60000).astype(np.uint32)
+    src = np.concatenate([rs[:30000], hs, rs[30000:]])
+    dst = np.concatenate([rd[:30000], hd, rd[30000:]])
+    w = rng.uniform(0.05, 1.0, len(src))
+    sources = np.concatenate([[0, 1], rng.choice(n, 150, replace=False)]).astype(np.uint32)
+    g = make(n, src, dst, w)
+    ref = oracle_bc(n, src, dst, w, sources=sources)
+    for delta in (0.02, float("inf")):
+        _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_WEIGHTED_DELTA, delta))
+        got = rg.betweenness_f64_array(g, sources=sources)
+        ok, err = close(got, ref)
+        assert ok, (delta, err)
+    rg.destroy(g)
 
 
 def test_unit_weights_equal_unweighted_engine():

@@ -485,3 +485,101 @@ def test_weighted_networkx(oracle_mod):
         G.add_edge(a, b, weight=x)
     ref = nx.betweenness_centrality(G, normalized=False, weight="weight")
     assert np.allclose(got, [ref[v] for v in range(n)], rtol=1e-12, atol=1e-9)
+
+
+# --- eigenvector / Katz / alpha (centrality.zig:1050-1333) -------------------
+
+def _dense_in(n, src, dst, live=None):
+    """A^T as a dense matrix over the live nodes: row v holds the number of
+    arcs u -> v (parallel arcs counted, self-loops kept)."""
+    M = np.zeros((n, n))
+    for u, v in zip(src.tolist(), dst.tolist()):
+        if live is None or (live[u] and live[v]):
+            M[v, u] += 1.0
+    return M
+
+
+def test_power_reference_cases(oracle_mod):
+    O = oracle_mod
+    # centrality.zig:1478-1498 / centrality_test.exs:166-178: a 3-cycle
+    g = O.Graph(3, np.array([0, 1, 2], np.uint32), np.array([1, 2, 0], np.uint32))
+    ev, it, oc = g.eigenvector(100, 1e-4)
+    assert abs(ev[0] - ev[1]) < 0.01 and abs(ev[1] - ev[2]) < 0.01
+    assert oc == O.Graph.EIG_CAP and it == 100  # a 3-cycle rotates: no convergence, no period 2
+    # centrality_test.exs:180-192: undirected star, the hub scores highest
+    s, d = graphs.undirected([(0, 1), (0, 2), (0, 3)])
+    ev, _, _ = O.Graph(4, s, d).eigenvector()
+    assert ev[0] > ev[1] and ev[0] > ev[2] and ev[0] > ev[3]
+    # centrality_test.exs:196-206: B (1) receives from A (0) and C (2)
+    g = O.Graph(3, np.array([0, 2], np.uint32), np.array([1, 1], np.uint32))
+    k, _, _ = g.katz(0.1, 1.0)
+    assert k[1] > k[0] and k[1] > k[2]
+    # centrality_test.exs:208-217 and centrality.zig:1501-1518: at least beta
+    g = O.Graph(3, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    k, _, _ = g.katz(0.1, 1.0, 100, 1e-4)
+    assert np.all(k >= 1.0)
+    # centrality.zig:1520-1538: path 0 -> 1 -> 2, alpha 0.5, start 1.0
+    a, it, _ = g.alpha_centrality(0.5, 1.0, 100, 1e-4)
+    assert a[2] >= a[0]
+    assert a.tolist() == [0.0, 0.0, 0.0] and it == 4  # alpha^k A^k x0 dies after 3 steps
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_katz_alpha_match_dense_iterates(oracle_mod, seed):
+    # the k-th iterate of x <- alpha A^T x (+ beta) from a dense matrix;
+    # tolerance -1 never stops early
+    O = oracle_mod
+    rng = np.random.default_rng(60 + seed)
+    n, src, dst = graphs.random_multigraph(rng, 40, 160)
+    live = (rng.random(n) > 0.1).astype(np.uint8)
+    g = O.Graph(n, src, dst, live)
+    M = _dense_in(n, src, dst, live)
+    lv = live.astype(bool)
+    for k in (0, 1, 2, 7):
+        x = np.where(lv, 1.0, 0.0)
+        for _ in range(k):
+            x = np.where(lv, 0.05 * (M @ x) + 1.0, 0.0)
+        got, it, _ = g.katz(0.05, 1.0, k, -1.0)
+        assert it == k and np.allclose(got, x, rtol=1e-13, atol=0)
+        y = np.where(lv, 0.7, 0.0)
+        for _ in range(k):
+            y = np.where(lv, 0.3 * (M @ y), 0.0)
+        got, it, _ = g.alpha_centrality(0.3, 0.7, k, -1.0)
+        assert it == k and np.allclose(got, y, rtol=1e-13, atol=0)
+    # converged Katz is the resolvent beta (I - alpha A^T)^-1 1
+    rho = max(abs(np.linalg.eigvals(M)))
+    alpha = 0.5 / max(rho, 1.0)
+    got, it, l1 = g.katz(alpha, 1.0, 10_000, 1e-13)
+    idx = np.nonzero(lv)[0]
+    ref = np.linalg.solve(np.eye(len(idx)) - alpha * M[np.ix_(idx, idx)], np.ones(len(idx)))
+    assert l1 < 1e-13 and np.allclose(got[idx], ref, rtol=1e-11)
+
+
+def test_eigenvector_closed_forms(oracle_mod):
+    O = oracle_mod
+    # strongly connected and aperiodic (a 5-cycle with a chord and a
+    # self-loop): the Perron vector of A^T, unit L2 norm
+    pairs = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 2), (3, 3)]
+    s, d = graphs.arcs(pairs)
+    ev, it, oc = O.Graph(5, s, d).eigenvector(1000, 1e-12)
+    M = _dense_in(5, s, d)
+    w, V = np.linalg.eig(M)
+    p = np.real(V[:, np.argmax(np.real(w))])
+    p = np.abs(p) / np.linalg.norm(p)
+    assert oc == O.Graph.EIG_CONVERGED and np.allclose(ev, p, atol=1e-10)
+    # undirected star with k leaves: period-2 iterates, detected as an
+    # oscillation; the normalised average is the Perron vector
+    k = 5
+    s, d = graphs.undirected([(0, i) for i in range(1, k + 1)])
+    ev, it, oc = O.Graph(k + 1, s, d).eigenvector(1000, 1e-9)
+    assert oc == O.Graph.EIG_OSCILLATING
+    assert np.allclose(ev, [1 / np.sqrt(2)] + [1 / np.sqrt(2 * k)] * k, atol=1e-9)
+    # a DAG reaches the zero vector: uniform 1/sqrt(n)
+    s, d = graphs.arcs([(0, 1), (1, 2), (0, 2), (2, 3)])
+    ev, it, oc = O.Graph(4, s, d).eigenvector()
+    assert oc == O.Graph.EIG_ZERO and it == 4 and np.allclose(ev, 0.5)
+    # one node: 1.0; no iterations: the start vector 1 + i/1000 over live nodes
+    assert O.Graph(1, s[:0], d[:0]).eigenvector()[0].tolist() == [1.0]
+    live = np.array([1, 0, 1, 1], np.uint8)
+    ev, it, _ = O.Graph(4, s, d, live).eigenvector(0, 1e-4)
+    assert it == 0 and ev.tolist() == [1.0, 0.0, 1.001, 1.002]

@@ -35,6 +35,10 @@ ZG_TUNE_PULL_GRID = 3
 ZG_TUNE_GRAPH = 4
 ZG_TUNE_PR_CACHE = 5
 ZG_TUNE_PR_GRID = 6
+ZG_TUNE_WEIGHTED_BATCH = 7
+ZG_TUNE_WEIGHTED_DELTA = 8
+ZG_TUNE_WEIGHTED_GRID = 9
+ZG_EIG_CONVERGED, ZG_EIG_OSCILLATING, ZG_EIG_ZERO, ZG_EIG_CAP, ZG_EIG_TRIVIAL = range(5)
 
 # Every symbol include/zog_hip.h declares.
 EXPORTED = (
@@ -49,6 +53,9 @@ EXPORTED = (
     "zg_bc_f64",
     "zg_pagerank",
     "zg_pagerank_device",
+    "zg_eigenvector",
+    "zg_katz",
+    "zg_alpha_centrality",
     "zg_bc_paths",
     "zg_set_batch_groups",
     "zg_set_source_order",
@@ -73,7 +80,7 @@ ZG_MLP_HIDDEN = 128
 ZG_MLP_PARAMS = 17793
 ZG_MLP_MAX_BATCH = 65536
 
-PROF_SLOTS = ("bc_forward", "bc_finalize", "bc_backward", "bc_hub", "bc_setup", "pr_iter", "bc_push")
+PROF_SLOTS = ("bc_forward", "bc_finalize", "bc_backward", "bc_hub", "bc_setup", "pr_iter", "bc_push", "bc_dag")
 
 
 class ZgBcInfo(ctypes.Structure):
@@ -86,6 +93,7 @@ class ZgBcInfo(ctypes.Structure):
         ("push_levels", ctypes.c_uint32),
         ("pull_levels", ctypes.c_uint32),
         ("kernel_ms", ctypes.c_double),
+        ("arcs_relaxed", ctypes.c_uint64),
     ]
 
 
@@ -97,7 +105,7 @@ class ZgPrInfo(ctypes.Structure):
         ("error_bound_l1", ctypes.c_double),
         ("kernel_ms", ctypes.c_double),
         ("precision", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("outcome", ctypes.c_int32),
         ("fp64_iterations", ctypes.c_uint64),
     ]
 
@@ -172,6 +180,14 @@ def lib():
     L.zg_pagerank_device.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                                      P, P, ctypes.POINTER(ZgPrInfo)]
     L.zg_pagerank_device.restype = ctypes.c_int
+    L.zg_eigenvector.argtypes = [P, ctypes.c_uint64, ctypes.c_double, P, ctypes.POINTER(ZgPrInfo)]
+    L.zg_eigenvector.restype = ctypes.c_int
+    L.zg_katz.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P,
+                          ctypes.POINTER(ZgPrInfo)]
+    L.zg_katz.restype = ctypes.c_int
+    L.zg_alpha_centrality.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P,
+                                      ctypes.POINTER(ZgPrInfo)]
+    L.zg_alpha_centrality.restype = ctypes.c_int
     L.zg_bc_paths.argtypes = [P, P, ctypes.c_uint64, P, P, P, ctypes.c_uint32, P]
     L.zg_bc_paths.restype = ctypes.c_int
     L.zg_set_batch_groups.argtypes = [P, ctypes.c_int]

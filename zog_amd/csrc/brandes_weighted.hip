@@ -8,13 +8,19 @@
 // reverse order; bc(v) += delta(v) for v != s, unnormalised.
 //
 // GPU form, per source (up to 1,024 sources per batch, each with its own
-// rows; one wavefront per 64 entries of one source's list):
+// rows; one wavefront per 64 entries of one source's list, a vertex with
+// more than ZG_W_WIDE arcs walked by the whole wavefront):
 //   1. distances by frontier relaxation: every vertex whose distance dropped
 //      relaxes its arcs with a 64-bit atomic min on the distance's bit
 //      pattern (non-negative doubles order as unsigned integers); a vertex
-//      is queued at most once per round.  It ends at the least fixed point
-//      d(w) = min over arcs of fl(d(v) + w(v,w)), the distances Dijkstra
-//      computes with the same additions.
+//      is queued at most once per round.  A vertex whose new distance is
+//      above the source's bound waits in a far pile instead; when a source's
+//      frontier runs dry the bound rises by delta (or to the least far
+//      distance) and the far entries under it form the next frontier (the
+//      near-far form of delta-stepping), which saves relaxing a vertex again
+//      and again on the way down to its distance.  It ends at the least fixed
+//      point d(w) = min over arcs of fl(d(v) + w(v,w)), the distances
+//      Dijkstra computes with the same additions.
 //   2. the tight arcs (fl(d(v) + w) == d(w) bit for bit, v != w, w not the
 //      source; with positive weights neither exception is ever tight) form the
 //      shortest-path DAG: count their in-arcs, then peel it in rounds (a
@@ -31,8 +37,12 @@
 #include <algorithm>
 
 #define ZG_W_MAX_SOURCES 1024
-#define ZG_W_GRID 1024
+#define ZG_W_GRID 4096  // default workgroups of the per-round kernels
 #define ZG_W_ROUNDS_PER_SYNC 16
+#define ZG_W_WIDE 32  // vertices with more arcs are walked by a whole wavefront
+#define ZG_W_HUB 4096  // ... and with more than this, cut into pieces for many wavefronts
+#define ZG_W_PIECE 2048
+#define ZG_W_HUB_CAP (1u << 20)  // hub pieces per relax round
 #define ZG_W_UNSET 0xFFFFFFFFFFFFFFFFull
 
 struct WbcDev {
@@ -48,12 +58,27 @@ struct WbcDev {
     uint32_t *reach;            // [S][V] reached vertices
     uint32_t *rcnt;             // [S]
     uint32_t *order;            // [S][V] DAG order, round by round
+    uint32_t *tbits;            // [S][tw] tight-arc bits in successor-CSR order
+    size_t tw;                  // words per source
     uint32_t *qlen;             // [S]
     uint32_t *lstart;           // [lcap][S] round starts in order[]
     uint32_t *tot;              // [lcap] frontier sizes (relax) / entries added (peel)
     const uint32_t *src;        // [S]
     double *bc;                 // [V]
-    unsigned long long *stats;  // [2] arcs, reached
+    unsigned long long *stats;  // [3] arcs, reached, arcs relaxed
+    // near-far split of the relax frontier: a vertex whose new distance is
+    // above its source's bound waits in the far pile until the bound passes it
+    uint32_t *far;              // [2][S][V] far piles (fsel[s] is the live one)
+    uint32_t *farcnt;           // [2][S]
+    uint32_t *farflag;          // [S][V] 0 never far, 1 in the far pile, 2 within the bound
+    uint32_t *fsel;             // [S]
+    uint32_t *split;            // [S] this round moves far entries under the raised bound
+    unsigned long long *thr;    // [S] the bound (distance bits)
+    unsigned long long *fmin;   // [S] least distance recorded for a far entry (bits)
+    double delta;               // bound step
+    uint32_t *hcnt;             // [1] hub pieces listed this round
+    uint32_t *hub_v, *hub_a;    // [hcap] hub vertex; source << 22 | piece
+    uint32_t hcap;
 };
 
 #define WROW(s) ((size_t)(s) * a.V)
@@ -79,6 +104,12 @@ __global__ void wbc_init_kernel(WbcDev a)
     if (s >= a.S) return;
     a.fcnt[a.S + s] = 0;
     a.lstart[s] = 0;
+    a.farcnt[s] = 0;
+    a.farcnt[a.S + s] = 0;
+    a.fsel[s] = 0;
+    a.split[s] = 0;
+    a.thr[s] = zg_f64_to_bits(a.delta);
+    a.fmin[s] = ZG_W_UNSET;
     const uint32_t v = a.src[s];
     if (v == ZG_NONE) {
         a.fcnt[s] = 0;
@@ -88,6 +119,7 @@ __global__ void wbc_init_kernel(WbcDev a)
         return;
     }
     a.dist[WROW(s) + v] = 0ull;  // +0.0
+    a.farflag[WROW(s) + v] = 2;
     a.sig[WROW(s) + v] = 1.0;
     a.stamp[WROW(s) + v] = 0;
     a.fr[WROW(s)] = v;
@@ -99,16 +131,33 @@ __global__ void wbc_init_kernel(WbcDev a)
     a.lstart[a.S + s] = 1;
 }
 
-// Before relax round r: the size of its frontier (tot[r]) and an empty
-// buffer for the next one.  One block.
+// Before relax round r, per source: the work left (frontier and far pile,
+// summed into tot[r]), an empty buffer for the next frontier, and, for a
+// source whose frontier is empty while its far pile is not, a raised bound
+// max(bound + delta, least far distance) and a fresh far pile for the split
+// kernel to move the entries under it into.  One block.
 __global__ void wbc_round_kernel(WbcDev a, int r)
 {
     __shared__ unsigned long long red[16];
     const int cur = r & 1, nxt = cur ^ 1;
     unsigned long long t = 0;
+    if (threadIdx.x == 0) *a.hcnt = 0;
     for (uint32_t s = threadIdx.x; s < a.S; s += blockDim.x) {
-        t += a.fcnt[(size_t)cur * a.S + s];
+        const uint32_t nf = a.fcnt[(size_t)cur * a.S + s], fs = a.fsel[s];
+        const uint32_t nfar = a.farcnt[(size_t)fs * a.S + s];
+        t += (unsigned long long)nf + nfar;
         a.fcnt[(size_t)nxt * a.S + s] = 0;
+        const bool sp = nf == 0 && nfar > 0;
+        a.split[s] = sp;
+        if (sp) {
+            double T = zg_bits_to_f64(a.thr[s]) + a.delta;
+            const unsigned long long fm = a.fmin[s];
+            if (fm != ZG_W_UNSET) T = fmax(T, zg_bits_to_f64(fm));
+            a.thr[s] = zg_f64_to_bits(T);
+            a.fmin[s] = ZG_W_UNSET;
+            a.fsel[s] = fs ^ 1;
+            a.farcnt[(size_t)(fs ^ 1) * a.S + s] = 0;
+        }
     }
     t = zg_wave_sum_u64(t);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
@@ -120,15 +169,30 @@ __global__ void wbc_round_kernel(WbcDev a, int r)
     }
 }
 
-__global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
+// Least of a lane value over the wavefront (64-bit).
+__device__ __forceinline__ unsigned long long zg_wave_min_u64(unsigned long long x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long y = __shfl_xor(x, off, 64);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+// Splits the far piles of the sources flagged by wbc_round_kernel: entries
+// still in the pile and within the raised bound join round r's frontier,
+// the others stay (and give the least far distance); entries whose vertex
+// has since come within the bound are dropped.
+__global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r)
 {
     __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
-    const int cur = r & 1, nxt = cur ^ 1;
+    const int cur = r & 1;
     const uint32_t total = zg_q_chunks(
         a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
             lo = 0;
-            hi = a.fcnt[(size_t)cur * a.S + s];
+            hi = a.split[s] ? a.farcnt[(size_t)(a.fsel[s] ^ 1) * a.S + s] : 0;
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63;
@@ -137,30 +201,235 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
-        unsigned long long *dist = a.dist + WROW(s);
+        const uint32_t fs = a.fsel[s];
+        const unsigned long long Tb = a.thr[s];
+        const uint32_t v = valid ? a.far[((size_t)(fs ^ 1) * a.S + s) * a.V + idx] : 0;
+        bool move = false, keep = false;
+        unsigned long long d = ZG_W_UNSET;
+        if (valid && a.farflag[WROW(s) + v] == 1) {
+            d = a.dist[WROW(s) + v];
+            if (d <= Tb) move = atomicExch(&a.farflag[WROW(s) + v], 2u) == 1u;
+            else keep = true;
+        }
+        wbc_append(move, v, &a.fcnt[(size_t)cur * a.S + s], a.fr + ((size_t)cur * a.S + s) * a.V, lane);
+        wbc_append(keep, v, &a.farcnt[(size_t)fs * a.S + s], a.far + ((size_t)fs * a.S + s) * a.V, lane);
+        const unsigned long long m = zg_wave_min_u64(keep ? d : ZG_W_UNSET);
+        if (lane == 0 && m != ZG_W_UNSET) atomicMin(&a.fmin[s], m);
+    }
+}
+
+// Tight arc: v != w, w not the source, fl(d(v) + weight) == d(w) bit for bit.
+__device__ __forceinline__ bool wbc_tight(const WbcDev &a, const unsigned long long *dist, uint32_t v, uint32_t src,
+                                          double dv, uint32_t e, uint32_t &w)
+{
+    w = a.out_col[e];
+    return w != v && w != src && zg_f64_to_bits(dv + a.out_w[e]) == dist[w];
+}
+
+__device__ __forceinline__ double zg_readlane_f64(double x, int j)
+{
+    return zg_bits_to_f64(zg_readlane64(zg_f64_to_bits(x), j));
+}
+
+// Walks the tight arcs of the arc range [e, e1) through the bit words that
+// cover it.
+struct WbcTightIt {
+    uint32_t e, e1, base, bits;
+};
+
+__device__ __forceinline__ bool wbc_next_tight(WbcTightIt &it, const uint32_t *tb, uint32_t &arc)
+{
+    while (it.bits == 0) {
+        if (it.e >= it.e1) return false;
+        const uint32_t wi = it.e >> 5;
+        const uint64_t wend = ((uint64_t)wi + 1) << 5;
+        uint32_t b = tb[wi] & (~0u << (it.e & 31));
+        if ((uint64_t)it.e1 < wend) b &= (1u << (it.e1 & 31)) - 1;
+        it.base = wi << 5;
+        it.bits = b;
+        it.e = (uint32_t)std::min<uint64_t>(wend, it.e1);
+    }
+    arc = it.base + __builtin_ctz(it.bits);
+    it.bits &= it.bits - 1;
+    return true;
+}
+
+// Bit words covering a wide vertex's arcs [ej, ej + dj), and the range of
+// word k of them (empty past the last word).
+__device__ __forceinline__ uint32_t wbc_words(uint32_t ej, uint32_t dj)
+{
+    return ((ej + dj - 1) >> 5) - (ej >> 5) + 1;
+}
+
+__device__ __forceinline__ WbcTightIt wbc_word_share(uint32_t ej, uint32_t dj, uint32_t k)
+{
+    const uint32_t w0 = ej >> 5;
+    WbcTightIt it{0, 0, 0, 0};
+    if (k < wbc_words(ej, dj)) {
+        const uint64_t lo = (uint64_t)(w0 + k) << 5;
+        it.e = (uint32_t)std::max<uint64_t>(lo, ej);
+        it.e1 = (uint32_t)std::min<uint64_t>(lo + 32, (uint64_t)ej + dj);
+    }
+    return it;
+}
+
+// One source's view of relax round r.
+struct WbcRelax {
+    unsigned long long *dist;
+    uint32_t *stamp, *farflag, *ncnt, *nfr, *rcnt, *reach, *fcnt, *far;
+    unsigned long long Tb;
+    uint32_t r1;
+    unsigned long long fm;  // least far distance this lane recorded
+
+    __device__ __forceinline__ WbcRelax(const WbcDev &a, uint32_t s, int r)
+    {
+        const int nxt = (r & 1) ^ 1;
+        dist = a.dist + WROW(s);
+        stamp = a.stamp + WROW(s);
+        farflag = a.farflag + WROW(s);
+        ncnt = &a.fcnt[(size_t)nxt * a.S + s];
+        nfr = a.fr + ((size_t)nxt * a.S + s) * a.V;
+        rcnt = &a.rcnt[s];
+        reach = a.reach + WROW(s);
+        const uint32_t fs = a.fsel[s];
+        fcnt = &a.farcnt[(size_t)fs * a.S + s];
+        far = a.far + ((size_t)fs * a.S + s) * a.V;
+        Tb = a.thr[s];
+        r1 = (uint32_t)(r + 1);
+        fm = ZG_W_UNSET;
+    }
+
+    // lowers d(w) to fl(d + weight); reports a first reach, a queueing for
+    // the next round (within the bound) or a first far entry
+    __device__ __forceinline__ void arc(const WbcDev &a, double d, uint32_t e, bool &first, bool &queue, bool &fnew,
+                                        uint32_t &w)
+    {
+        w = a.out_col[e];
+        const unsigned long long nb = zg_f64_to_bits(d + a.out_w[e]);
+        if (nb < dist[w]) {
+            const unsigned long long old = atomicMin(&dist[w], nb);
+            if (nb < old) {
+                first = old == ZG_W_UNSET;
+                if (nb <= Tb) {
+                    queue = atomicExch(&stamp[w], r1) != r1;
+                    farflag[w] = 2;
+                } else {
+                    fnew = atomicCAS(&farflag[w], 0u, 1u) == 0u;
+                    fm = nb < fm ? nb : fm;
+                }
+            }
+        }
+    }
+
+    // The appends of one step; every lane of the wavefront calls it.
+    __device__ __forceinline__ void append(bool first, bool queue, bool fnew, uint32_t w, int lane)
+    {
+        wbc_append(queue, w, ncnt, nfr, lane);
+        wbc_append(fnew, w, fcnt, far, lane);
+        wbc_append(first, w, rcnt, reach, lane);
+    }
+
+    // The whole wavefront over the arcs [e0, e1) of a vertex at distance d.
+    __device__ __forceinline__ void wave_range(const WbcDev &a, double d, uint32_t e0, uint32_t e1, int lane)
+    {
+        for (uint32_t k0 = e0; k0 < e1; k0 += 64) {
+            bool first = false, queue = false, fnew = false;
+            uint32_t w = 0;
+            if (k0 + lane < e1) arc(a, d, k0 + lane, first, queue, fnew, w);
+            append(first, queue, fnew, w, lane);
+        }
+    }
+
+    // Folds the lanes' least far distance into the source's; all lanes call it.
+    __device__ __forceinline__ void flush(const WbcDev &a, uint32_t s, int lane)
+    {
+        const unsigned long long m = zg_wave_min_u64(fm);
+        if (lane == 0 && m != ZG_W_UNSET) atomicMin(&a.fmin[s], m);
+    }
+};
+
+// Relax round r over every source's frontier.  Vertices with more than
+// ZG_W_WIDE arcs are walked by the whole wavefront, one at a time, and those
+// with more than ZG_W_HUB arcs are cut into pieces of ZG_W_PIECE arcs for
+// wbc_hub_kernel (inline when its list is full); the others are walked by
+// their own lane.
+__global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
+{
+    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
+    const int cur = r & 1;
+    const uint32_t total = zg_q_chunks(
+        a.S,
+        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
+            lo = 0;
+            hi = a.fcnt[(size_t)cur * a.S + s];
+        },
+        s_lo, s_hi, s_pre);
+    __shared__ unsigned long long red[4];
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    unsigned long long scanned = 0;
+    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+        const uint32_t s = zg_q_owner(s_pre, a.S, c);
+        const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
+        const bool valid = idx < s_hi[s];
+        WbcRelax R(a, s, r);
         const uint32_t v = valid ? a.fr[((size_t)cur * a.S + s) * a.V + idx] : 0;
-        const double dv = valid ? zg_bits_to_f64(dist[v]) : 0.0;  // latest value
+        const double dv = valid ? zg_bits_to_f64(R.dist[v]) : 0.0;  // latest value
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
-        uint32_t maxdeg = deg;
+        scanned += deg;
+        for (uint64_t wide = __ballot(deg > ZG_W_WIDE); wide; wide &= wide - 1) {
+            const int j = __builtin_ctzll(wide);
+            const uint32_t vj = zg_readlane32(v, j), ej = zg_readlane32(e0, j), dj = zg_readlane32(deg, j);
+            if (dj > ZG_W_HUB) {
+                const uint32_t pieces = (dj + ZG_W_PIECE - 1) / ZG_W_PIECE;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.hcnt, pieces);
+                base = zg_readlane32(base, 0);
+                const bool fits = (uint64_t)base + pieces <= a.hcap;
+                for (uint32_t k = lane; k < pieces && base + k < a.hcap; k += 64) {
+                    a.hub_v[base + k] = fits ? vj : ZG_NONE;  // claimed slots of a list that is full stay empty
+                    a.hub_a[base + k] = (s << 22) | k;
+                }
+                if (fits) continue;
+            }
+            R.wave_range(a, zg_readlane_f64(dv, j), ej, ej + dj, lane);
+        }
+        const uint32_t mine = deg <= ZG_W_WIDE ? deg : 0;
+        uint32_t maxdeg = mine;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor((int)maxdeg, off, 64));
         for (uint32_t k = 0; k < maxdeg; k++) {
-            bool first = false, queue = false;
+            bool first = false, queue = false, fnew = false;
             uint32_t w = 0;
-            if (k < deg) {
-                w = a.out_col[e0 + k];
-                const unsigned long long nb = zg_f64_to_bits(dv + a.out_w[e0 + k]);
-                if (nb < dist[w]) {
-                    const unsigned long long old = atomicMin(&dist[w], nb);
-                    if (nb < old) {
-                        first = old == ZG_W_UNSET;
-                        queue = atomicExch(&a.stamp[WROW(s) + w], (uint32_t)(r + 1)) != (uint32_t)(r + 1);
-                    }
-                }
-            }
-            wbc_append(queue, w, &a.fcnt[(size_t)nxt * a.S + s], a.fr + ((size_t)nxt * a.S + s) * a.V, lane);
-            wbc_append(first, w, &a.rcnt[s], a.reach + WROW(s), lane);
+            if (k < mine) R.arc(a, dv, e0 + k, first, queue, fnew, w);
+            R.append(first, queue, fnew, w, lane);
         }
+        R.flush(a, s, lane);
+    }
+    scanned = zg_wave_sum_u64(scanned);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = scanned;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = red[0] + red[1] + red[2] + red[3];
+        if (t) atomicAdd(&a.stats[2], t);
+    }
+}
+
+// The hub pieces of relax round r: one wavefront per piece.
+__global__ __launch_bounds__(256) void wbc_hub_kernel(WbcDev a, int r)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t H = min(*a.hcnt, a.hcap);
+    for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < H; i += waves) {
+        const uint32_t v = a.hub_v[i];
+        if (v == ZG_NONE) continue;
+        const uint32_t item = a.hub_a[i], s = item >> 22, piece = item & ((1u << 22) - 1);
+        WbcRelax R(a, s, r);
+        const uint32_t b1 = a.out_ptr[v + 1];
+        const uint32_t e0 = a.out_ptr[v] + piece * ZG_W_PIECE, e1 = min(b1, e0 + ZG_W_PIECE);
+        R.wave_range(a, zg_bits_to_f64(R.dist[v]), e0, e1, lane);
+        R.flush(a, s, lane);
     }
 }
 
@@ -182,18 +451,34 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
     for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + wv; c < total; c += waves) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
-        if (idx >= s_hi[s]) continue;
+        const bool valid = idx < s_hi[s];
         const unsigned long long *dist = a.dist + WROW(s);
-        const uint32_t v = a.reach[WROW(s) + idx], src = a.src[s];
-        const double dv = zg_bits_to_f64(dist[v]);
-        const uint32_t e0 = a.out_ptr[v], e1 = a.out_ptr[v + 1];
-        arcs += e1 - e0;
-        reached += 1;
-        for (uint32_t e = e0; e < e1; e++) {
-            const uint32_t w = a.out_col[e];
-            if (w != v && w != src && zg_f64_to_bits(dv + a.out_w[e]) == dist[w])
-                atomicAdd(&a.indeg[WROW(s) + w], 1u);
+        uint32_t *indeg = a.indeg + WROW(s), *tb = a.tbits + s * a.tw;
+        const uint32_t v = valid ? a.reach[WROW(s) + idx] : 0, src = a.src[s];
+        const double dv = valid ? zg_bits_to_f64(dist[v]) : 0.0;
+        const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
+        arcs += deg;
+        reached += valid ? 1 : 0;
+        for (uint64_t wide = __ballot(deg > ZG_W_WIDE); wide; wide &= wide - 1) {
+            const int j = __builtin_ctzll(wide);
+            const uint32_t vj = zg_readlane32(v, j), ej = zg_readlane32(e0, j), dj = zg_readlane32(deg, j);
+            const double dvj = zg_readlane_f64(dv, j);
+            for (uint32_t k = lane; k < dj; k += 64) {
+                uint32_t w;
+                if (wbc_tight(a, dist, vj, src, dvj, ej + k, w)) {
+                    atomicAdd(&indeg[w], 1u);
+                    atomicOr(&tb[(ej + k) >> 5], 1u << ((ej + k) & 31));
+                }
+            }
         }
+        if (deg <= ZG_W_WIDE)
+            for (uint32_t k = 0; k < deg; k++) {
+                uint32_t w;
+                if (wbc_tight(a, dist, v, src, dv, e0 + k, w)) {
+                    atomicAdd(&indeg[w], 1u);
+                    atomicOr(&tb[(e0 + k) >> 5], 1u << ((e0 + k) & 31));
+                }
+            }
     }
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
@@ -231,26 +516,36 @@ __global__ __launch_bounds__(256) void wbc_peel_kernel(WbcDev a, int r)
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
-        const unsigned long long *dist = a.dist + WROW(s);
-        const uint32_t v = valid ? a.order[WROW(s) + idx] : 0, src = a.src[s];
-        const double dv = valid ? zg_bits_to_f64(dist[v]) : 0.0;
-        const double sv = valid ? a.sig[WROW(s) + v] : 0.0;
+        double *sig = a.sig + WROW(s);
+        uint32_t *indeg = a.indeg + WROW(s), *qlen = &a.qlen[s], *order = a.order + WROW(s);
+        const uint32_t *tb = a.tbits + s * a.tw;
+        const uint32_t v = valid ? order[idx] : 0;
+        const double sv = valid ? sig[v] : 0.0;
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
-        uint32_t maxdeg = deg;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor((int)maxdeg, off, 64));
-        for (uint32_t k = 0; k < maxdeg; k++) {
-            bool ready = false;
-            uint32_t w = 0;
-            if (k < deg) {
-                w = a.out_col[e0 + k];
-                if (w != v && w != src && zg_f64_to_bits(dv + a.out_w[e0 + k]) == dist[w]) {
-                    atomicAdd(&a.sig[WROW(s) + w], sv);
-                    ready = atomicSub(&a.indeg[WROW(s) + w], 1u) == 1u;
+        // one tight arc per lane and step; every lane takes part in the append
+        auto walk = [&](WbcTightIt it, double sg) {
+            for (;;) {
+                uint32_t e = 0;
+                const bool has = wbc_next_tight(it, tb, e);
+                if (!__ballot(has)) break;
+                bool ready = false;
+                uint32_t w = 0;
+                if (has) {
+                    w = a.out_col[e];
+                    atomicAdd(&sig[w], sg);
+                    ready = atomicSub(&indeg[w], 1u) == 1u;
                 }
+                wbc_append(ready, w, qlen, order, lane);
             }
-            wbc_append(ready, w, &a.qlen[s], a.order + WROW(s), lane);
+        };
+        for (uint64_t wide = __ballot(deg > ZG_W_WIDE); wide; wide &= wide - 1) {
+            const int j = __builtin_ctzll(wide);
+            const uint32_t ej = zg_readlane32(e0, j), dj = zg_readlane32(deg, j);
+            const double svj = zg_readlane_f64(sv, j);
+            for (uint32_t k0 = 0, nw = wbc_words(ej, dj); k0 < nw; k0 += 64)
+                walk(wbc_word_share(ej, dj, k0 + lane), svj);
         }
+        walk(WbcTightIt{e0, deg <= ZG_W_WIDE ? e0 + deg : e0, 0, 0}, sv);
     }
 }
 
@@ -289,19 +584,35 @@ __global__ __launch_bounds__(256) void wbc_backward_kernel(WbcDev a, int r)
     for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
-        if (idx >= s_hi[s]) continue;
-        const unsigned long long *dist = a.dist + WROW(s);
-        const uint32_t v = a.order[WROW(s) + idx], src = a.src[s];
-        const double dv = zg_bits_to_f64(dist[v]);
-        double S = 0.0;
-        for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e++) {
-            const uint32_t w = a.out_col[e];
-            if (w != v && w != src && zg_f64_to_bits(dv + a.out_w[e]) == dist[w]) S += a.cc[WROW(s) + w];
+        const bool valid = idx < s_hi[s];
+        double *cc = a.cc + WROW(s);
+        const uint32_t *tb = a.tbits + s * a.tw;
+        const uint32_t v = valid ? a.order[WROW(s) + idx] : 0, src = a.src[s];
+        const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
+        double S = 0.0;  // sum of c(w) over v's tight successors
+        for (uint64_t wide = __ballot(deg > ZG_W_WIDE); wide; wide &= wide - 1) {
+            const int j = __builtin_ctzll(wide);
+            const uint32_t ej = zg_readlane32(e0, j), dj = zg_readlane32(deg, j);
+            double part = 0.0;
+            for (uint32_t k0 = 0, nw = wbc_words(ej, dj); k0 < nw; k0 += 64) {
+                WbcTightIt it = wbc_word_share(ej, dj, k0 + lane);
+                uint32_t e;
+                while (wbc_next_tight(it, tb, e)) part += cc[a.out_col[e]];
+            }
+            part = zg_wave_sum(part);
+            if (lane == j) S = part;
         }
-        const double sv = a.sig[WROW(s) + v];
-        const double delta = sv * S;
-        a.cc[WROW(s) + v] = (1.0 + delta) / sv;
-        if (v != src) atomicAdd(&a.bc[v], delta);
+        if (deg <= ZG_W_WIDE) {
+            WbcTightIt it{e0, e0 + deg, 0, 0};
+            uint32_t e;
+            while (wbc_next_tight(it, tb, e)) S += cc[a.out_col[e]];
+        }
+        if (valid) {
+            const double sv = a.sig[WROW(s) + v];
+            const double delta = sv * S;
+            cc[v] = (1.0 + delta) / sv;
+            if (v != src) atomicAdd(&a.bc[v], delta);
+        }
     }
 }
 
@@ -316,10 +627,18 @@ int zg_wbc_ws_release(zg_graph *g)
     zg_wbc_ws &w = g->wbc;
     zg_dfree(w.dist); zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.indeg); zg_dfree(w.stamp); zg_dfree(w.fr);
     zg_dfree(w.fcnt); zg_dfree(w.reach); zg_dfree(w.rcnt); zg_dfree(w.order); zg_dfree(w.qlen);
-    zg_dfree(w.lstart); zg_dfree(w.tot); zg_dfree(w.src); zg_dfree(w.bc); zg_dfree(w.stats);
+    zg_dfree(w.lstart); zg_dfree(w.tot); zg_dfree(w.src); zg_dfree(w.bc); zg_dfree(w.stats); zg_dfree(w.tbits);
+    zg_dfree(w.far); zg_dfree(w.farcnt); zg_dfree(w.farflag); zg_dfree(w.fsel); zg_dfree(w.split);
+    zg_dfree(w.thr); zg_dfree(w.fmin); zg_dfree(w.hcnt); zg_dfree(w.hub_v); zg_dfree(w.hub_a);
     w.S = 0;
     w.lcap = 0;
     return ZG_OK;
+}
+
+// Tight-arc bit words per source.
+static size_t zg_wbc_words(const zg_graph *g)
+{
+    return (size_t)((g->E + 31) / 32) + 1;
 }
 
 static int wbc_ws_ensure(zg_graph *g, uint32_t S)
@@ -344,7 +663,18 @@ static int wbc_ws_ensure(zg_graph *g, uint32_t S)
     if (rc == ZG_OK) rc = zg_dmalloc(&w.tot, w.lcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.src, S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.bc, V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, 2);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, 3);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.tbits, (size_t)S * zg_wbc_words(g));
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.far, 2 * SV);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.farcnt, 2 * (size_t)S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.farflag, SV);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fsel, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.split, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.thr, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fmin, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.hcnt, 1);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.hub_v, ZG_W_HUB_CAP);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.hub_a, ZG_W_HUB_CAP);
     if (rc != ZG_OK) {
         zg_wbc_ws_release(g);
         return rc;
@@ -379,6 +709,20 @@ static int wbc_grow(zg_graph *g, uint32_t need, uint32_t used, hipStream_t s)
     return ZG_OK;
 }
 
+// Bound step of the near frontier: the tuned value, else 0.6 x the mean arc
+// weight / the mean out-degree (+inf when there is nothing to scale by).
+// Smaller steps relax each vertex fewer times but run more rounds, each of
+// which scans the far piles of the sources it raises the bound of; on R-MAT
+// scale 20 with weights uniform in (0, 1] the step 0.02 (= 0.6 x 0.5 / 16)
+// was the fastest of 0.002 ... 0.4.
+static double zg_wbc_delta(const zg_graph *g)
+{
+    if (g->wbc_delta > 0.0) return g->wbc_delta;
+    const double live = (double)std::max<uint32_t>(g->n_live, 1);
+    const double d = g->E ? 0.6 * g->wmean / ((double)g->E / live) : 0.0;
+    return d > 0.0 ? d : HUGE_VAL;
+}
+
 static WbcDev wbc_dev(zg_graph *g)
 {
     zg_wbc_ws &w = g->wbc;
@@ -392,6 +736,12 @@ static WbcDev wbc_dev(zg_graph *g)
     a.dist = w.dist; a.sig = w.sig; a.cc = w.cc; a.indeg = w.indeg; a.stamp = w.stamp; a.fr = w.fr;
     a.fcnt = w.fcnt; a.reach = w.reach; a.rcnt = w.rcnt; a.order = w.order; a.qlen = w.qlen;
     a.lstart = w.lstart; a.tot = w.tot; a.src = w.src; a.bc = w.bc; a.stats = w.stats;
+    a.tbits = w.tbits;
+    a.tw = zg_wbc_words(g);
+    a.far = w.far; a.farcnt = w.farcnt; a.farflag = w.farflag; a.fsel = w.fsel; a.split = w.split;
+    a.thr = w.thr; a.fmin = w.fmin;
+    a.hcnt = w.hcnt; a.hub_v = w.hub_v; a.hub_a = w.hub_a; a.hcap = ZG_W_HUB_CAP;
+    a.delta = zg_wbc_delta(g);
     return a;
 }
 
@@ -405,22 +755,26 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
         return ZG_OK;
     }
     // sources per batch: as many as the call has (rounded up to 64), at most
-    // ZG_W_MAX_SOURCES, and as many as half the free device memory holds at
-    // 48 bytes of state per source and vertex
-    const uint32_t want = (uint32_t)std::min<uint64_t>(ZG_W_MAX_SOURCES, (n + 63) / 64 * 64);
+    // the tuned width (default ZG_W_MAX_SOURCES), and as many as half the
+    // free device memory holds (60 bytes per vertex and 4 per 32 arcs of
+    // state per source); a larger workspace from an earlier call is kept
+    const uint64_t per_src = 60ull * V + 4ull * zg_wbc_words(g);
+    const uint32_t cap = g->wbc_batch ? g->wbc_batch : ZG_W_MAX_SOURCES;
+    const uint32_t want = (uint32_t)std::min<uint64_t>(cap, (n + 63) / 64 * 64);
     uint32_t S = g->wbc.S;
-    if (S < want) {
+    if (S < want || S > cap) {
         size_t fr = 0, tot = 0;
         ZG_TRY(hipMemGetInfo(&fr, &tot));
-        fr += (size_t)g->wbc.S * V * 48;
-        const uint64_t fit = (uint64_t)(0.5 * (double)fr) / (48ull * (V ? V : 1));
+        fr += (size_t)g->wbc.S * per_src;
+        const uint64_t fit = (uint64_t)(0.5 * (double)fr) / per_src;
         S = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(want, fit) / 64 * 64);
     }
     ZG_TRY_RC(wbc_ws_ensure(g, S));
     zg_wbc_ws &w = g->wbc;
     const size_t SV = (size_t)S * V;
+    const uint32_t grid = g->wbc_grid ? g->wbc_grid : ZG_W_GRID;
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
-    ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 2, s));
+    ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 3, s));
     std::vector<uint32_t> tot(ZG_W_ROUNDS_PER_SYNC);
     for (uint64_t b0 = 0; b0 < n; b0 += S) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(S, n - b0);
@@ -431,8 +785,10 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
         ZG_TRY(hipMemsetAsync(w.cc, 0, sizeof(double) * SV, s));
         ZG_TRY(hipMemsetAsync(w.indeg, 0, sizeof(uint32_t) * SV, s));
         ZG_TRY(hipMemsetAsync(w.stamp, 0xFF, sizeof(uint32_t) * SV, s));
+        ZG_TRY(hipMemsetAsync(w.tbits, 0, sizeof(uint32_t) * S * zg_wbc_words(g), s));
+        ZG_TRY(hipMemsetAsync(w.farflag, 0, sizeof(uint32_t) * SV, s));
         WbcDev a = wbc_dev(g);
-        hipLaunchKernelGGL(wbc_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, a);
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, a);
         ZG_TRY(hipGetLastError());
         // 1. distances: relax rounds, queued ZG_W_ROUNDS_PER_SYNC at a time
         uint32_t r = 0;
@@ -440,8 +796,10 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             ZG_TRY_RC(wbc_grow(g, r + ZG_W_ROUNDS_PER_SYNC + 2, r + 1, s));
             a = wbc_dev(g);
             for (uint32_t k = 0; k < ZG_W_ROUNDS_PER_SYNC; k++) {
-                hipLaunchKernelGGL(wbc_round_kernel, dim3(1), dim3(1024), 0, s, a, (int)(r + k));
-                hipLaunchKernelGGL(wbc_relax_kernel, dim3(ZG_W_GRID), dim3(256), 0, s, a, (int)(r + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_round_kernel, dim3(1), dim3(1024), 0, s, a, (int)(r + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_relax_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_hub_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
             }
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(tot.data(), w.tot + r, sizeof(uint32_t) * ZG_W_ROUNDS_PER_SYNC,
@@ -451,15 +809,15 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             r += ZG_W_ROUNDS_PER_SYNC;
         }
         // 2. the tight DAG: in-arc counts, then peel rounds
-        hipLaunchKernelGGL(wbc_count_kernel, dim3(ZG_W_GRID), dim3(256), 0, s, a);
+        ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_count_kernel, dim3(grid), dim3(256), 0, s, a);
         ZG_TRY(hipGetLastError());
         int last = -1;
         for (uint32_t p = 0; last < 0;) {
             ZG_TRY_RC(wbc_grow(g, p + ZG_W_ROUNDS_PER_SYNC + 2, p + 2, s));
             a = wbc_dev(g);
             for (uint32_t k = 0; k < ZG_W_ROUNDS_PER_SYNC; k++) {
-                hipLaunchKernelGGL(wbc_peel_kernel, dim3(ZG_W_GRID), dim3(256), 0, s, a, (int)(p + k));
-                hipLaunchKernelGGL(wbc_close_kernel, dim3(1), dim3(1024), 0, s, a, (int)(p + k + 1));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_peel_kernel, dim3(grid), dim3(256), 0, s, a, (int)(p + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_close_kernel, dim3(1), dim3(1024), 0, s, a, (int)(p + k + 1));
             }
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(tot.data(), w.tot + p + 1, sizeof(uint32_t) * ZG_W_ROUNDS_PER_SYNC,
@@ -475,21 +833,22 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
         // 3. dependencies, rounds in reverse (round 0 is the source)
         a = wbc_dev(g);
         for (int q = last; q >= 1; q--)
-            hipLaunchKernelGGL(wbc_backward_kernel, dim3(ZG_W_GRID), dim3(256), 0, s, a, q);
+            ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, wbc_backward_kernel, dim3(grid), dim3(256), 0, s, a, q);
         ZG_TRY(hipGetLastError());
         inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)last + 1);
         inf->push_levels += r;
         inf->batches++;
     }
-    unsigned long long st[2] = {0, 0};
+    unsigned long long st[3] = {0, 0, 0};
     ZG_TRY(hipMemcpyAsync(st, w.stats, sizeof(st), hipMemcpyDeviceToHost, s));
     if (V) {
-        hipLaunchKernelGGL(wbc_out_kernel, dim3((V + 255) / 256), dim3(256), 0, s, V, w.bc, g->live, d_out);
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_out_kernel, dim3((V + 255) / 256), dim3(256), 0, s, V, w.bc, g->live, d_out);
         ZG_TRY(hipGetLastError());
     }
     ZG_TRY(hipStreamSynchronize(s));
     inf->arcs_traversed += st[0];
     inf->nodes_reached += st[1];
+    inf->arcs_relaxed += st[2];
     inf->n_sources = n;
     return ZG_OK;
 }
@@ -532,6 +891,7 @@ extern "C" int zg_bc_f64(zg_graph *g, const uint32_t *sources, uint64_t n_source
         rc = zg_hip_code(hipEventElapsedTime(&ms, e0, e1));
         inf.kernel_ms = ms;
     }
+    if (rc == ZG_OK) rc = zg_prof_collect(g, s);
     if (rc == ZG_OK && g->V) rc = zg_hip_code(hipMemcpy(out_bc, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

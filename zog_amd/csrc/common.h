@@ -160,9 +160,16 @@ struct zg_wbc_ws {
     uint32_t *qlen = nullptr;            // [S]
     uint32_t *lstart = nullptr;          // [lcap][S]
     uint32_t *tot = nullptr;             // [lcap]
+    uint32_t *tbits = nullptr;           // [S][(E + 31) / 32 + 1] tight-arc bits
+    uint32_t *far = nullptr;             // [2][S][V] far piles
+    uint32_t *farcnt = nullptr;          // [2][S]
+    uint32_t *farflag = nullptr;         // [S][V]
+    uint32_t *fsel = nullptr, *split = nullptr;  // [S]
+    unsigned long long *thr = nullptr, *fmin = nullptr;  // [S]
+    uint32_t *hcnt = nullptr, *hub_v = nullptr, *hub_a = nullptr;  // hub pieces of a relax round
     uint32_t *src = nullptr;             // [S]
     double *bc = nullptr;                // [V]
-    unsigned long long *stats = nullptr; // [2]
+    unsigned long long *stats = nullptr; // [3]
 };
 
 // PageRank workspace: double-buffered ranks and gathered values.
@@ -229,6 +236,10 @@ struct zg_graph {
     int engine = 0;                      // ZG_ENGINE_*
     double *out_w = nullptr;             // [E] arc weights in successor-CSR order (weighted resources)
     bool weights_ok = true;              // every live arc's weight is finite and non-negative
+    uint32_t wbc_batch = 0;              // weighted sweep: sources per batch (0 = default)
+    double wbc_delta = 0.0;              // weighted sweep: bound step of the near frontier (0 = default)
+    uint32_t wbc_grid = 0;               // weighted sweep: workgroups per round kernel (0 = default)
+    double wmean = 0.0;                  // mean weight of the live arcs
     uint32_t *und_ptr = nullptr, *und_col = nullptr;  // simple undirected CSR (core numbers), built on demand
     uint32_t pull_grid = 4096;
     uint32_t pr_xcache = 1u << 30;       // PageRank values staged in LDS (capped by the LDS size; 0 = none)

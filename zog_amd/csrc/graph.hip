@@ -149,6 +149,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     g->E = E;
     std::vector<uint32_t> out_col(E);
     std::vector<double> out_w(weight ? E : 0);
+    double wsum = 0.0;
     {
         // filling each row from its end in insertion order leaves the newest
         // arc first
@@ -160,12 +161,14 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
                 out_col[k] = t;
                 if (weight) {
                     out_w[k] = weight[e];
+                    wsum += weight[e];
                     // NaN, negative or infinite: no shortest paths under the reference's semantics
                     if (!(weight[e] >= 0.0) || weight[e] == HUGE_VAL) g->weights_ok = false;
                 }
             }
         }
     }
+    g->wmean = E ? wsum / (double)E : 0.0;
     if (weight) ZG_TRY_RC(zg_upload(&g->out_w, out_w));
     // Predecessor CSR: scan sources in id order.
     std::vector<uint32_t> in_ptr((size_t)V + 1, 0);
@@ -463,6 +466,19 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
     case ZG_TUNE_GRAPH:
         if (value != 0.0 && value != 1.0) return ZG_ERR_INVALID_ARG;
         g->use_graph = value != 0.0;
+        return ZG_OK;
+    case ZG_TUNE_WEIGHTED_BATCH:
+        if (!(value >= 0.0 && value <= 1024.0) || value != (double)(64 * (uint32_t)(value / 64)))
+            return ZG_ERR_INVALID_ARG;
+        g->wbc_batch = (uint32_t)value;
+        return ZG_OK;
+    case ZG_TUNE_WEIGHTED_DELTA:
+        if (!(value >= 0.0)) return ZG_ERR_INVALID_ARG;  // +inf allowed: no bound
+        g->wbc_delta = value;
+        return ZG_OK;
+    case ZG_TUNE_WEIGHTED_GRID:
+        if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;
+        g->wbc_grid = (uint32_t)value;
         return ZG_OK;
     case ZG_TUNE_PULL_GRID:
         if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;

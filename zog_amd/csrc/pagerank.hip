@@ -27,15 +27,24 @@
 
 struct PrState {
     unsigned int done;
-    unsigned int pad;
+    int outcome;      // eigenvector: how the iteration ended (ZG_EIG_*)
     unsigned long long iter;
     unsigned long long max_iter;
-    double base;      // (1-d)/n + d*sink for the next iteration to run
-    double last_l1;
-    double damping;
+    double base;      // PageRank: (1-d)/n + d*sink for the next iteration; Katz: beta
+    double last_l1;   // eigenvector: the L2 change
+    double damping;   // PageRank: d; Katz / alpha: alpha
     double inv_n;
     double tol;
+    double sumsq;     // eigenvector: sum of squares of the raw iterate
 };
+
+// What one iteration computes from the gathered sum of a row:
+//   ZG_PW_PR      PageRank: r' = base + d * sum, gathered x' = r'/deg
+//   ZG_PW_AFFINE  Katz / alpha: r' = alpha * sum + beta, gathered x' = r'
+//   ZG_PW_RAW     eigenvector: y = sum (normalised by eig_norm_kernel)
+#define ZG_PW_PR 0
+#define ZG_PW_AFFINE 1
+#define ZG_PW_RAW 2
 
 template <typename T>
 struct PrArgs {
@@ -46,6 +55,7 @@ struct PrArgs {
     uint32_t nb_small, nb_m4, nb_m16, nb_m64;  // work units of each bin
     uint32_t xcache;            // values staged in LDS (0 = none)
     T *r0, *r1, *x0, *x1;       // parity 0 / 1 buffers
+    T *rb0, *rb1, *rb2;         // ZG_PW_RAW: the rotating (older, current, new) iterates
     double *part;               // [gridDim.x][2]
     T *seg;                     // [n_seg][2] Kahan pairs of long-row segments
     PrState *st;
@@ -110,22 +120,56 @@ __device__ __forceinline__ void pr_strided_kahan(const uint32_t *__restrict__ co
 }
 
 
-// Same, with the old rank and the out-degree already loaded.
-template <typename T>
+// Same, with the old rank and the out-degree already loaded.  l1 gathers
+// the L1 change (PageRank, Katz / alpha) or the sum of squares (raw sums).
+template <typename T, int MODE>
 __device__ __forceinline__ void pr_row_finish(T *rnew, T *xnew, uint32_t v, T r_old, uint32_t outdeg, T s, T c,
                                               double base, double damping, double &l1, double &dang)
 {
     const double sum = (double)s - (double)c;
-    const T rt = (T)(base + damping * sum);
-    rnew[v] = rt;
-    l1 += fabs((double)rt - (double)r_old);
-    if (outdeg == 0) dang += (double)rt;
-    else xnew[v] = rt / (T)outdeg;
+    if constexpr (MODE == ZG_PW_PR) {
+        const T rt = (T)(base + damping * sum);
+        rnew[v] = rt;
+        l1 += fabs((double)rt - (double)r_old);
+        if (outdeg == 0) dang += (double)rt;
+        else xnew[v] = rt / (T)outdeg;
+    } else if constexpr (MODE == ZG_PW_AFFINE) {
+        const T rt = (T)__dadd_rn(__dmul_rn(damping, sum), base);  // alpha * sum + beta, two roundings
+        rnew[v] = rt;
+        l1 += fabs((double)rt - (double)r_old);
+    } else {
+        const T rt = (T)sum;
+        rnew[v] = rt;
+        l1 += (double)rt * (double)rt;
+    }
 }
+
+// The buffers of iteration `it`: gathered values, old and new iterates.
+template <typename T, int MODE>
+struct PrBufs {
+    const T *xg, *rold;
+    T *rnew, *xnew;
+    __device__ __forceinline__ PrBufs(const PrArgs<T> &a, unsigned long long it)
+    {
+        if constexpr (MODE == ZG_PW_RAW) {
+            const unsigned k = (unsigned)(it % 3);
+            xg = k == 0 ? a.rb0 : (k == 1 ? a.rb1 : a.rb2);
+            rold = xg;
+            rnew = k == 0 ? a.rb1 : (k == 1 ? a.rb2 : a.rb0);
+            xnew = rnew;
+        } else {
+            const bool odd = it & 1;
+            xg = odd ? a.x1 : a.x0;
+            rold = odd ? a.r1 : a.r0;
+            rnew = odd ? a.r0 : a.r1;
+            xnew = odd ? a.x0 : a.x1;
+        }
+    }
+};
 
 // Rows of one bin summed by groups of L lanes: unit u covers rows
 // [u*(256/L), (u+1)*(256/L)) of the bin; lane groups merge their Kahan pairs.
-template <typename T, int L, typename X>
+template <typename T, int MODE, int L, typename X>
 __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *rows, uint32_t nrows,
                                               uint32_t unit, const X &x, const T *rold, T *rnew, T *xnew,
                                               double base, double damping, double &l1, double &dang)
@@ -138,7 +182,7 @@ __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *r
     T s = 0, c = 0, r_old = 0;
     if (ok) {
         dsc = rows[i];
-        if (sub == 0) r_old = rold[dsc.x];
+        if (MODE != ZG_PW_RAW && sub == 0) r_old = rold[dsc.x];
         pr_strided_kahan(a.in_col, x, dsc.y + sub, dsc.y + dsc.z, (uint32_t)L, s, c);
     }
 #pragma unroll
@@ -147,10 +191,10 @@ __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *r
         T c2 = __shfl_xor(c, off, 64);
         kahan_merge(s, c, s2, c2);
     }
-    if (ok && sub == 0) pr_row_finish(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
+    if (ok && sub == 0) pr_row_finish<T, MODE>(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
 }
 
-template <typename T>
+template <typename T, int MODE>
 __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
 {
     __shared__ double red[2][ZG_PR_BLOCK / 64];
@@ -158,17 +202,17 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
     const PrState *st = a.st;
     if (st->done) return;
     const unsigned long long it = st->iter;
-    const bool odd = it & 1;
-    const T *xg = odd ? a.x1 : a.x0;
+    const PrBufs<T, MODE> B(a, it);
+    const T *xg = B.xg;
     // the ZG_PR_XCACHE_BYTES most-gathered values (lowest PageRank ids) in LDS
     __shared__ T xs[ZG_PR_XCACHE_BYTES / sizeof(T)];
     const uint32_t K = min(a.xcache, (uint32_t)(ZG_PR_XCACHE_BYTES / sizeof(T)));
     for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) xs[i] = xg[i];
     __syncthreads();
     const PrGather<T> x{xg, xs, K};
-    const T *rold = odd ? a.r1 : a.r0;
-    T *rnew = odd ? a.r0 : a.r1;
-    T *xnew = odd ? a.x0 : a.x1;
+    const T *rold = B.rold;
+    T *rnew = B.rnew;
+    T *xnew = B.xnew;
     const double base = st->base, damping = st->damping;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t units = a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64 + a.n_seg;
@@ -186,23 +230,23 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
                 T val[ZG_PR_SMALL];
 #pragma unroll
                 for (int k = 0; k < ZG_PR_SMALL; k++) col[k] = (uint32_t)k < deg ? a.in_col[e0 + k] : 0u;
-                const T r_old = rold[dsc.x];
+                const T r_old = MODE != ZG_PW_RAW ? rold[dsc.x] : (T)0;
 #pragma unroll
                 for (int k = 0; k < ZG_PR_SMALL; k++) val[k] = (uint32_t)k < deg ? x[col[k]] : (T)0;
                 T s = 0, c = 0;
 #pragma unroll
                 for (int k = 0; k < ZG_PR_SMALL; k++)
                     if ((uint32_t)k < deg) kahan_add(s, c, val[k]);
-                pr_row_finish(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
+                pr_row_finish<T, MODE>(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
             }
         } else if (b < a.nb_small + a.nb_m4) {
-            pr_rows_lanes<T, 4>(a, a.desc + a.n_small, a.n_m4, b - a.nb_small, x, rold, rnew, xnew, base, damping, l1,
+            pr_rows_lanes<T, MODE, 4>(a, a.desc + a.n_small, a.n_m4, b - a.nb_small, x, rold, rnew, xnew, base, damping, l1,
                                 dang);
         } else if (b < a.nb_small + a.nb_m4 + a.nb_m16) {
-            pr_rows_lanes<T, 16>(a, a.desc + a.n_small + a.n_m4, a.n_m16, b - a.nb_small - a.nb_m4, x, rold, rnew,
+            pr_rows_lanes<T, MODE, 16>(a, a.desc + a.n_small + a.n_m4, a.n_m16, b - a.nb_small - a.nb_m4, x, rold, rnew,
                                  xnew, base, damping, l1, dang);
         } else if (b < a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64) {
-            pr_rows_lanes<T, 64>(a, a.desc + a.n_small + a.n_m4 + a.n_m16, a.n_m64, b - a.nb_small - a.nb_m4 - a.nb_m16,
+            pr_rows_lanes<T, MODE, 64>(a, a.desc + a.n_small + a.n_m4 + a.n_m16, a.n_m64, b - a.nb_small - a.nb_m4 - a.nb_m16,
                                  x, rold, rnew, xnew, base, damping, l1, dang);
         } else {
             // one block per segment of a long row; pr_finish_kernel finishes the row
@@ -248,17 +292,17 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
 
 // One block: finish the long rows from their segment partials, add all block
 // partials in a fixed order, advance the iteration state.
-template <typename T>
+template <typename T, int MODE>
 __global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<T> a, uint32_t nparts)
 {
     __shared__ double red[2][ZG_PR_FINISH_THREADS / 64];
     PrState *st = a.st;
     if (st->done) return;
     const unsigned long long it = st->iter;
-    const bool odd = it & 1;
-    const T *rold = odd ? a.r1 : a.r0;
-    T *rnew = odd ? a.r0 : a.r1;
-    T *xnew = odd ? a.x0 : a.x1;
+    const PrBufs<T, MODE> B(a, it);
+    const T *rold = B.rold;
+    T *rnew = B.rnew;
+    T *xnew = B.xnew;
     const double base = st->base, damping = st->damping;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double l1 = 0.0, dang = 0.0;
@@ -267,7 +311,8 @@ __global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<
         T S = a.seg[2 * s0], C = a.seg[2 * s0 + 1];
         for (uint32_t q = s0 + 1; q < s1; q++) kahan_merge(S, C, a.seg[2 * q], a.seg[2 * q + 1]);
         const uint32_t v = a.large[i];
-        pr_row_finish(rnew, xnew, v, rold[v], a.large_deg[i], S, C, base, damping, l1, dang);
+        pr_row_finish<T, MODE>(rnew, xnew, v, MODE != ZG_PW_RAW ? rold[v] : (T)0, a.large_deg[i], S, C, base,
+                               damping, l1, dang);
     }
     for (uint32_t q = tid; q < nparts; q += ZG_PR_FINISH_THREADS) {
         l1 += a.part[2 * q];
@@ -286,11 +331,15 @@ __global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<
             L += red[0][w];
             D += red[1][w];
         }
-        const unsigned long long nit = it + 1;
-        st->iter = nit;
-        st->last_l1 = L;
-        st->base = (1.0 - damping) * st->inv_n + damping * (D * st->inv_n);
-        if (L < st->tol || nit >= st->max_iter) st->done = 1;
+        if constexpr (MODE == ZG_PW_RAW) {
+            st->sumsq = L;  // eig_norm_kernel / eig_decide_kernel finish the iteration
+        } else {
+            const unsigned long long nit = it + 1;
+            st->iter = nit;
+            st->last_l1 = L;
+            if constexpr (MODE == ZG_PW_PR) st->base = (1.0 - damping) * st->inv_n + damping * (D * st->inv_n);
+            if (L < st->tol || nit >= st->max_iter) st->done = 1;
+        }
     }
 }
 
@@ -404,6 +453,7 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.r1 = r1;
     a.x0 = x0;
     a.x1 = x1;
+    a.rb0 = a.rb1 = a.rb2 = nullptr;
     a.part = g->pr.part;
     a.seg = seg;
     a.st = st;
@@ -422,8 +472,8 @@ static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st,
         uint64_t k = max_iter - launched < K ? max_iter - launched : K;
         for (uint64_t i = 0; i < k; i++) {
             const size_t p = zg_prof_begin(g, s);
-            hipLaunchKernelGGL(pr_iter_kernel<T>, dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
-            hipLaunchKernelGGL(pr_finish_kernel<T>, dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, grid);
+            hipLaunchKernelGGL((pr_iter_kernel<T, ZG_PW_PR>), dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((pr_finish_kernel<T, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, grid);
             zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
         }
         ZG_TRY(hipGetLastError());
@@ -567,8 +617,9 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
                                (const double *)d_out, false, 0.0, r64, r64 + V, x64, x64 + V);
             ZG_TRY(hipGetLastError());
             PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st + 1);
-            hipLaunchKernelGGL(pr_iter_kernel<double>, dim3(pr_grid(g)), dim3(ZG_PR_BLOCK), 0, s, a);
-            hipLaunchKernelGGL(pr_finish_kernel<double>, dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, pr_grid(g));
+            hipLaunchKernelGGL((pr_iter_kernel<double, ZG_PW_PR>), dim3(pr_grid(g)), dim3(ZG_PR_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a,
+                               pr_grid(g));
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(h_st + 1, d_st + 1, sizeof(PrState), hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
@@ -618,4 +669,325 @@ extern "C" int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations,
     }
     zg_dfree(d_out);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Eigenvector, Katz and alpha centrality: the same pull over the predecessor
+// lists (collectNodes / buildInNeighbors, priv/zog/src/utils.zig:42-73, are
+// the live ids ascending and, per vertex, its predecessors ascending with
+// parallel arcs repeated), in FP64 with the PageRank kernels' row bins.
+//   eigenvector (centrality.zig:1050-1201): y = sum over in(v) of x, then
+//     y /= ||y||_2 (when nonzero), then in order: a zero y ends with the
+//     uniform 1/sqrt(n); from the second iteration on, ||y - x_prev_prev||_2
+//     below tol ends with the normalised average (y + x)/2; ||y - x||_2
+//     below tol ends with y; after max_iterations the last y is returned.
+//     Start 1 + i/1000 for the i-th live id; n <= 1 gives 1.0.
+//   Katz (:1212-1267): x' = alpha * sum + beta from beta; alpha centrality
+//     (:1278-1333): x' = alpha * sum from `initial`; the L1 change, then the
+//     swap, then stop when it is below tol.
+
+// y /= ||y|| on the new iterate; block partials of ||y - x||^2 and
+// ||y - x_prev_prev||^2.
+__global__ __launch_bounds__(256) void eig_norm_kernel(PrArgs<double> a, uint32_t V)
+{
+    __shared__ double red[2][4];
+    const PrState *st = a.st;
+    if (st->done) return;
+    const unsigned long long it = st->iter;
+    const unsigned k = (unsigned)(it % 3);
+    const double *cur = k == 0 ? a.rb0 : (k == 1 ? a.rb1 : a.rb2);
+    double *nw = k == 0 ? a.rb1 : (k == 1 ? a.rb2 : a.rb0);
+    const double *pp = k == 0 ? a.rb2 : (k == 1 ? a.rb0 : a.rb1);
+    const double s2 = st->sumsq, norm = sqrt(s2);
+    double d1 = 0.0, d2 = 0.0;
+    if (s2 > 0.0)
+        for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < V; p += gridDim.x * blockDim.x) {
+            const double y = nw[p] / norm;
+            nw[p] = y;
+            const double e1 = y - cur[p], e2 = y - pp[p];
+            d1 += e1 * e1;
+            d2 += e2 * e2;
+        }
+    d1 = zg_wave_sum(d1);
+    d2 = zg_wave_sum(d2);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = d1;
+        red[1][threadIdx.x >> 6] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.part[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        a.part[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+// The eigenvector iteration's end-of-step checks, in the reference's order.
+// One block.
+__global__ __launch_bounds__(1024) void eig_decide_kernel(PrArgs<double> a, uint32_t nparts)
+{
+    __shared__ double red[2][16];
+    PrState *st = a.st;
+    if (st->done) return;
+    double d1 = 0.0, d2 = 0.0;
+    for (uint32_t q = threadIdx.x; q < nparts; q += blockDim.x) {
+        d1 += a.part[2 * q];
+        d2 += a.part[2 * q + 1];
+    }
+    d1 = zg_wave_sum(d1);
+    d2 = zg_wave_sum(d2);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = d1;
+        red[1][threadIdx.x >> 6] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double D1 = 0.0, D2 = 0.0;
+    for (int w = 0; w < 16; w++) {
+        D1 += red[0][w];
+        D2 += red[1][w];
+    }
+    const unsigned long long it = st->iter;
+    if (st->sumsq == 0.0) {  // the zero vector
+        st->outcome = ZG_EIG_ZERO;
+        st->done = 1;
+        return;
+    }
+    const double l2 = D1 > 0.0 ? sqrt(D1) : D1;
+    const double l2b = D2 > 0.0 ? sqrt(D2) : D2;
+    st->last_l1 = l2;
+    if (it >= 1 && l2b < st->tol) {
+        st->outcome = ZG_EIG_OSCILLATING;
+        st->done = 1;
+    } else if (l2 < st->tol) {
+        st->outcome = ZG_EIG_CONVERGED;
+        st->done = 1;
+    } else {
+        st->iter = it + 1;
+        if (it + 1 >= st->max_iter) {
+            st->outcome = ZG_EIG_CAP;
+            st->done = 1;
+        }
+    }
+}
+
+// out[p] = (u[p] + w[p]) / 2 and block partials of its sum of squares.
+__global__ __launch_bounds__(256) void eig_avg_kernel(uint32_t V, const double *u, const double *w, double *out,
+                                                      double *part)
+{
+    __shared__ double red[4];
+    double s2 = 0.0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < V; p += gridDim.x * blockDim.x) {
+        const double y = (u[p] + w[p]) / 2.0;
+        out[p] = y;
+        s2 += y * y;
+    }
+    s2 = zg_wave_sum(s2);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out /= sqrt(sum of the partials) when positive (each block adds the
+// partials in the same order).
+__global__ __launch_bounds__(256) void eig_scale_kernel(uint32_t V, double *out, const double *part, uint32_t nparts)
+{
+    __shared__ double tot;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t q = 0; q < nparts; q++) t += part[q];
+        tot = t;
+    }
+    __syncthreads();
+    if (!(tot > 0.0)) return;
+    const double norm = sqrt(tot);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < V; p += gridDim.x * blockDim.x) out[p] /= norm;
+}
+
+__global__ void pw_fill_kernel(uint32_t V, const uint8_t *live, double value, double *out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) out[v] = live[v] ? value : 0.0;
+}
+
+// b0[pr_id[v]] = src[v] (or `fill`) on live rows, 0 elsewhere; b1 = 0 (and
+// b2 = 0 when given).
+__global__ void pw_init_kernel(uint32_t V, const uint8_t *live, const uint32_t *pr_id, const double *src, double fill,
+                               double *b0, double *b1, double *b2)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t p = pr_id[v];
+    b0[p] = live[v] ? (src ? src[v] : fill) : 0.0;
+    b1[p] = 0.0;
+    if (b2) b2[p] = 0.0;
+}
+
+// kind: ZG_PW_AFFINE (alpha, beta, start) or ZG_PW_RAW (eigenvector).
+static int zg_power_impl(zg_graph *g, int kind, double alpha, double beta, double start, uint64_t max_iterations,
+                         double tolerance, double *d_out, hipStream_t s, zg_pr_info *info)
+{
+    if (g->h_comp_off.size() > 2) return ZG_ERR_INVALID_ARG;  // one iteration per resource
+    const uint32_t V = g->V, n = g->n_live;
+    zg_pr_info inf;
+    memset(&inf, 0, sizeof(inf));
+    inf.precision = ZG_PR_FP64;
+    const dim3 vb(256), vg((V + 255) / 256);
+    if (n == 0 || (kind == ZG_PW_RAW && n == 1)) {
+        // Katz / alpha of no nodes: no scores; eigenvector of <= 1 node: 1.0
+        if (V) {
+            if (n == 0) ZG_TRY(hipMemsetAsync(d_out, 0, sizeof(double) * V, s));
+            else hipLaunchKernelGGL(pr_single_kernel, vg, vb, 0, s, V, g->live, d_out);
+            ZG_TRY(hipGetLastError());
+        }
+        ZG_TRY(hipStreamSynchronize(s));
+        inf.outcome = kind == ZG_PW_RAW ? ZG_EIG_TRIVIAL : 0;
+        if (info) *info = inf;
+        return ZG_OK;
+    }
+    ZG_TRY_RC(pr_ws_ensure(g));
+    double *r64 = (double *)g->pr.r;
+    double *b0 = r64, *b1 = r64 + V, *b2 = r64 + 2 * (size_t)V;
+    PrState *d_st = (PrState *)g->pr.state;
+    PrState *h_st = (PrState *)g->pr.h_state;
+    PrState st0;
+    memset(&st0, 0, sizeof(st0));
+    st0.max_iter = max_iterations;
+    st0.damping = alpha;
+    st0.base = beta;
+    st0.inv_n = 1.0 / (double)n;
+    st0.tol = tolerance;
+    st0.done = max_iterations == 0 ? 1u : 0u;
+    st0.outcome = ZG_EIG_CAP;
+    double *d_start = nullptr;
+    if (kind == ZG_PW_RAW) {
+        // the start vector 1 + i/1000 over the live ids in ascending order
+        std::vector<double> h((size_t)V, 0.0);
+        uint64_t i = 0;
+        for (uint32_t v = 0; v < V; v++)
+            if (g->h_live[v]) h[v] = 1.0 + (double)(i++) / 1000.0;
+        ZG_TRY_RC(zg_dmalloc(&d_start, V));
+        hipError_t e = hipMemcpyAsync(d_start, h.data(), sizeof(double) * V, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            zg_dfree(d_start);
+            return zg_hip_code(e);
+        }
+    }
+    hipLaunchKernelGGL(pw_init_kernel, vg, vb, 0, s, V, g->live, g->pr_id, (const double *)d_start, start, b0, b1,
+                       kind == ZG_PW_RAW ? b2 : nullptr);
+    ZG_TRY(hipMemcpyAsync(d_st, &st0, sizeof(PrState), hipMemcpyHostToDevice, s));
+    ZG_TRY(hipGetLastError());
+    PrArgs<double> a = pr_args<double>(g, b0, b1, b0, b1, g->pr.seg64, d_st);  // Katz / alpha gather x = r
+    a.rb0 = b0;
+    a.rb1 = b1;
+    a.rb2 = b2;
+    const uint32_t grid = pr_grid(g);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = zg_hip_code(hipEventCreate(&e0));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e0, s));
+    uint64_t launched = 0;
+    const uint64_t K = 8;
+    while (rc == ZG_OK && launched < max_iterations) {
+        const uint64_t k = std::min<uint64_t>(K, max_iterations - launched);
+        for (uint64_t i = 0; i < k; i++) {
+            const size_t p = zg_prof_begin(g, s);
+            if (kind == ZG_PW_RAW) {
+                hipLaunchKernelGGL((pr_iter_kernel<double, ZG_PW_RAW>), dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+                hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_RAW>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s,
+                                   a, grid);
+                hipLaunchKernelGGL(eig_norm_kernel, dim3(grid), dim3(256), 0, s, a, V);
+                hipLaunchKernelGGL(eig_decide_kernel, dim3(1), dim3(1024), 0, s, a, grid);
+            } else {
+                hipLaunchKernelGGL((pr_iter_kernel<double, ZG_PW_AFFINE>), dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+                hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_AFFINE>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0,
+                                   s, a, grid);
+            }
+            zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
+        }
+        rc = zg_hip_code(hipGetLastError());
+        launched += k;
+        if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(h_st, d_st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+        if (rc == ZG_OK && h_st->done) break;
+    }
+    if (rc == ZG_OK && max_iterations == 0) {
+        rc = zg_hip_code(hipMemcpyAsync(h_st, d_st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+    }
+    if (rc == ZG_OK) {
+        // iterations that ran (the eigenvector counts its last, deciding one)
+        const uint64_t it = h_st->iter;
+        const int oc = h_st->outcome;
+        uint64_t ran = it;
+        if (kind == ZG_PW_RAW && max_iterations > 0 && oc != ZG_EIG_CAP) ran = it + 1;
+        if (g->ev_marks.size() > ran) g->ev_marks.resize(ran);
+        rc = zg_prof_collect(g, s);
+        inf.iterations = ran;
+        inf.last_l1 = h_st->last_l1;
+        inf.outcome = kind == ZG_PW_RAW ? oc : 0;
+        const double *res = nullptr;
+        if (kind != ZG_PW_RAW) {
+            res = (it & 1) ? b1 : b0;
+        } else if (oc == ZG_EIG_ZERO) {
+            hipLaunchKernelGGL(pw_fill_kernel, vg, vb, 0, s, V, g->live, 1.0 / sqrt((double)n), d_out);
+        } else if (oc == ZG_EIG_OSCILLATING) {
+            const unsigned k3 = (unsigned)(it % 3);
+            const double *cur = k3 == 0 ? b0 : (k3 == 1 ? b1 : b2);
+            const double *nw = k3 == 0 ? b1 : (k3 == 1 ? b2 : b0);
+            double *tmp = k3 == 0 ? b2 : (k3 == 1 ? b0 : b1);
+            hipLaunchKernelGGL(eig_avg_kernel, dim3(grid), dim3(256), 0, s, V, nw, cur, tmp, g->pr.part);
+            hipLaunchKernelGGL(eig_scale_kernel, dim3(grid), dim3(256), 0, s, V, tmp, (const double *)g->pr.part,
+                               grid);
+            res = tmp;
+        } else {
+            // converged: the new iterate; cap (or no iteration): the current one
+            const unsigned k3 = (unsigned)((oc == ZG_EIG_CONVERGED ? it + 1 : it) % 3);
+            res = k3 == 0 ? b0 : (k3 == 1 ? b1 : b2);
+        }
+        if (res) hipLaunchKernelGGL(pr_out_kernel<double>, vg, vb, 0, s, V, g->pr_id, res, d_out);
+        if (rc == ZG_OK) rc = zg_hip_code(hipGetLastError());
+        if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e1, s));
+        if (rc == ZG_OK) rc = zg_hip_code(hipEventSynchronize(e1));
+        float ms = 0.f;
+        if (rc == ZG_OK) rc = zg_hip_code(hipEventElapsedTime(&ms, e0, e1));
+        inf.kernel_ms = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    zg_dfree(d_start);
+    if (rc == ZG_OK && info) *info = inf;
+    return rc;
+}
+
+static int zg_power_host(zg_graph *g, int kind, double alpha, double beta, double start, uint64_t max_iterations,
+                         double tolerance, double *out, zg_pr_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!out && g->V) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    double *d_out = nullptr;
+    ZG_TRY_RC(zg_dmalloc(&d_out, g->V));
+    int rc = zg_power_impl(g, kind, alpha, beta, start, max_iterations, tolerance, d_out, g->stream, info);
+    if (rc == ZG_OK && g->V) rc = zg_hip_code(hipMemcpy(out, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
+    zg_dfree(d_out);
+    return rc;
+}
+
+extern "C" int zg_eigenvector(zg_graph *g, uint64_t max_iterations, double tolerance, double *out, zg_pr_info *info)
+{
+    return zg_power_host(g, ZG_PW_RAW, 0.0, 0.0, 0.0, max_iterations, tolerance, out, info);
+}
+
+extern "C" int zg_katz(zg_graph *g, double alpha, double beta, uint64_t max_iterations, double tolerance, double *out,
+                       zg_pr_info *info)
+{
+    return zg_power_host(g, ZG_PW_AFFINE, alpha, beta, beta, max_iterations, tolerance, out, info);
+}
+
+extern "C" int zg_alpha_centrality(zg_graph *g, double alpha, double initial, uint64_t max_iterations,
+                                   double tolerance, double *out, zg_pr_info *info)
+{
+    return zg_power_host(g, ZG_PW_AFFINE, alpha, 0.0, initial, max_iterations, tolerance, out, info);
 }

@@ -14,6 +14,12 @@ Reference surface kept:
     (arc weights from the builder, as buildGraph keeps them, :120-129)
   * ``pagerank(graph, damping=0.85, max_iterations=100, tolerance=0.0001,
     raw=False)``                               resource_graph.ex:1671-1685
+  * ``eigenvector(graph, max_iterations=100, tolerance=0.0001, raw=False)``
+                                               resource_graph.ex:1697-1709
+  * ``katz(graph, alpha=0.1, beta=1.0, max_iterations=100, tolerance=0.0001,
+    raw=False)``                               resource_graph.ex:1723-1737
+  * ``alpha_centrality(graph, alpha=0.5, initial=1.0, max_iterations=100,
+    tolerance=0.0001, raw=False)``             resource_graph.ex:1751-1765
   * ``raw=True`` returns the dense list indexed by u32 id; otherwise scores
     are mapped to labels (map_scores, :2609-2619): integer keys for an
     integer-labelled builder, else labels in insertion order.
@@ -335,6 +341,44 @@ def betweenness_f64(graph: ResourceGraph, raw: bool = False, *, sources=None, in
 def betweenness_centrality(graph: ResourceGraph, raw: bool = False, **kw):
     """Alias of ``betweenness_unweighted`` (same semantics)."""
     return betweenness_unweighted(graph, raw=raw, **kw)
+
+
+def _power(graph: ResourceGraph, fn: str, args, raw: bool, info: Optional[dict]):
+    res = graph.resource
+    out = np.zeros(res.n_cap, dtype=np.float64)
+    inf = _lib.ZgPrInfo()
+    L = _lib.lib()
+    _lib.check(getattr(L, fn)(res.handle, *args, _ptr(out), ctypes.byref(inf)), fn)
+    if info is not None:
+        info.update({k: getattr(inf, k) for k, _ in _lib.ZgPrInfo._fields_})
+    if raw:
+        return out.tolist()
+    return _map_scores(graph.builder, out.tolist())
+
+
+def eigenvector(graph: ResourceGraph, max_iterations: int = 100, tolerance: float = 0.0001, raw: bool = False,
+                *, info: Optional[dict] = None):
+    """Eigenvector centrality (resource_graph.ex:1697-1709; NIF :266-275)."""
+    if max_iterations < 0:
+        raise ValueError("max_iterations must be >= 0")
+    return _power(graph, "zg_eigenvector", (int(max_iterations), float(tolerance)), raw, info)
+
+
+def katz(graph: ResourceGraph, alpha: float = 0.1, beta: float = 1.0, max_iterations: int = 100,
+         tolerance: float = 0.0001, raw: bool = False, *, info: Optional[dict] = None):
+    """Katz centrality (resource_graph.ex:1723-1737; NIF :277-286)."""
+    if max_iterations < 0:
+        raise ValueError("max_iterations must be >= 0")
+    return _power(graph, "zg_katz", (float(alpha), float(beta), int(max_iterations), float(tolerance)), raw, info)
+
+
+def alpha_centrality(graph: ResourceGraph, alpha: float = 0.5, initial: float = 1.0, max_iterations: int = 100,
+                     tolerance: float = 0.0001, raw: bool = False, *, info: Optional[dict] = None):
+    """Alpha centrality (resource_graph.ex:1751-1765; NIF :288-297)."""
+    if max_iterations < 0:
+        raise ValueError("max_iterations must be >= 0")
+    return _power(graph, "zg_alpha_centrality",
+                  (float(alpha), float(initial), int(max_iterations), float(tolerance)), raw, info)
 
 
 def pagerank_array(graph: ResourceGraph, damping: float = 0.85, max_iterations: int = 100,
