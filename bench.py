@@ -326,6 +326,88 @@ def bench_weighted(args):
     rg.destroy(g)
 
 
+def bench_power(args):
+    """Eigenvector, Katz and alpha centrality on rmat20's graph: 100
+    iterations each (tolerance 0 never stops early), FP64."""
+    import numpy as np
+    import torch
+
+    from zog_amd import _lib
+    from zog_amd import resource_graph as rg
+    from zog_amd import workloads as W
+    from zog_amd.soa import SoA
+    torch.cuda.set_device(0)
+    n, src, dst = W.rmat20()
+    g = rg.new(SoA.from_arrays("directed", n, src, dst))
+    V, n_live, E = g.resource.info()
+    iters = 100
+    # alpha below 1 / spectral radius keeps Katz and alpha finite: the
+    # largest in-degree bounds the radius from below, the arc count above
+    alpha = 1.0 / float(E)
+    runs = {"eigenvector": lambda inf: rg.eigenvector(g, iters, 0.0, raw=True, info=inf),
+            "katz": lambda inf: rg.katz(g, alpha, 1.0, iters, 0.0, raw=True, info=inf),
+            "alpha_centrality": lambda inf: rg.alpha_centrality(g, alpha, 1.0, iters, 0.0, raw=True, info=inf)}
+    # per iteration: predecessor id (4 B) and gathered value (8 B) per arc;
+    # row descriptor (16 B), old and new value (16 B) per row; the
+    # eigenvector's normalising pass reads three iterates and writes one
+    bytes_iter = {"eigenvector": 12 * E + 64 * n_live, "katz": 12 * E + 32 * n_live,
+                  "alpha_centrality": 12 * E + 32 * n_live}
+    res = {}
+    for name, fn in runs.items():
+        inf = {}
+        fn(inf)  # warm-up
+        times = []
+        for _ in range(args.steps):
+            fn(inf)
+            times.append(inf["kernel_ms"])
+            log(f"[bench] {name}: {inf['iterations']} iterations in {inf['kernel_ms']:.1f} ms")
+        ms = float(np.median(times))
+        rg.set_profiling(g, True)
+        fn(inf)
+        prof = rg.get_profile(g)
+        rg.set_profiling(g, False)
+        it_ms, it_n = prof["pr_iter"]
+        gbs = bytes_iter[name] * inf["iterations"] / (ms / 1000.0) / 1e9
+        ach = bytes_iter[name] * it_n / (it_ms / 1000.0) / 1e9 if it_ms > 0 else 0.0
+        res[name] = {"value": round(gbs, 1), "unit": "GB/s", "iterations": int(inf["iterations"]),
+                     "ms": round(ms, 2), "bytes_per_iteration": bytes_iter[name],
+                     "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                                  "kernel": "pr_iter_kernel+pr_finish_kernel" +
+                                            ("+eig_norm_kernel+eig_decide_kernel" if name == "eigenvector" else ""),
+                                  "avg_iteration_us": round(it_ms * 1000.0 / max(it_n, 1), 2)}}
+    cpu = None
+    parity = {}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        og = O.Graph(n, src, dst)
+        k = 10
+        t0 = time.perf_counter()
+        ref, _, _ = og.katz(alpha, 1.0, k, 0.0)
+        cpu_s = time.perf_counter() - t0
+        got = np.array(rg.katz(g, alpha, 1.0, k, 0.0, raw=True))
+        parity["katz_max_rel_err_10_iterations"] = float((np.abs(got - ref) / np.abs(ref).max()).max())
+        ref, _, _ = og.eigenvector(k, 0.0)
+        got = np.array(rg.eigenvector(g, k, 0.0, raw=True))
+        parity["eigenvector_max_abs_err_10_iterations"] = float(np.abs(got - ref).max())
+        cpu = {"value": round(bytes_iter["katz"] * k / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/oracle.c Katz (restated centrality.zig:1212-1267), {k} iterations on the same "
+                         f"graph, 1 thread, host cores of the GPU box"}
+        log(f"[bench] oracle Katz {k} iterations in {cpu_s:.1f}s")
+    line = {
+        "metric": "power-iteration GB/s (Katz; eigenvector and alpha beside it)",
+        "value": res["katz"]["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": res["katz"]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "power: R-MAT scale 20, 2^24 distinct arcs; eigenvector, Katz (alpha = 1/arcs, "
+                               "beta 1) and alpha centrality (alpha = 1/arcs, initial 1), 100 iterations each",
+                   "nodes": n, "arcs": int(E)},
+        "roofline": res["katz"]["roofline"], "methods": res, "cpu_baseline": cpu, "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    rg.destroy(g)
+
+
 def workload_spec(name):
     from zog_amd import workloads as W
     if name == "rmat20":
@@ -371,7 +453,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512", "rmat20w", "pivots", "copy"])
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "er10k", "grid2048", "ba512", "rmat20w", "power", "pivots", "copy"])
     ap.add_argument("--train-per-scale", type=int, default=40, help="pivots: R-MAT training graphs per scale 14..18")
     ap.add_argument("--epochs", type=int, default=30, help="pivots: training epochs")
     ap.add_argument("--eval-scale", type=int, default=20, help="pivots: R-MAT scale of the judged graph")
@@ -407,6 +489,8 @@ def main():
         return bench_copy(args)
     if args.workload == "rmat20w":
         return bench_weighted(args)
+    if args.workload == "power":
+        return bench_power(args)
     spec = workload_spec(args.workload)
     if args.workload == "ba512":
         return bench_batch(args, spec)
