@@ -98,6 +98,20 @@ int zg_graph_create(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *fr
 int zg_graph_create_weighted(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from,
                              const uint32_t *to, const double *weight, uint64_t m, zg_graph **out);
 
+/* Arc count from which zg_graph_create* build the CSR on the GPU (stable
+ * radix sorts; default 2^18; below it, host loops).  Process-wide; the
+ * arrays are the same either way.  No reference counterpart. */
+int zg_set_build_threshold(uint64_t min_arcs);
+
+/* Copies the resident successor CSR (out_ptr[n_cap+1], out_col[arcs]: each
+ * row in the reference's iteration order, newest arc first), predecessor CSR
+ * (in_ptr, in_col: each row ascending by source, parallel arcs repeated) and
+ * the PageRank ids (pr_id[n_cap]) to host buffers; any pointer may be NULL.
+ * Returns 1 if the CSR was built on the GPU, 0 if on the host, <0 on error.
+ * No reference counterpart (parity checks, callers that want the arrays). */
+int zg_graph_csr(zg_graph *g, uint32_t *out_ptr, uint32_t *out_col, uint32_t *in_ptr, uint32_t *in_col,
+                 uint32_t *pr_id);
+
 /* Batch of independent graphs in one resource (no reference counterpart; the
  * reference builds one resource per graph with new/5): graph i has
  * node_counts[i] nodes and arc_counts[i] arcs whose local ids are in from/to,
@@ -319,9 +333,16 @@ int zg_get_profile(zg_graph *g, zg_profile *out);
 int zg_bc_weighted(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources,
                    double *out_bc, double *out_totals, zg_bc_info *info);
 
+/* Replaces `nif_node_degrees` (resource_graph.ex:562-599, Elixir
+ * node_degrees/1 :2130-2136): out[v] = live out-arcs of v (arcs to
+ * tombstoned ids skipped; 0 for a tombstoned v, whose arcs removeNode
+ * deleted) for v < n_cap.  Read from the handle's host copy of the degrees. */
+int zg_node_degrees(zg_graph *g, uint32_t *out);
+
 /* Core numbers of the graph taken as undirected and simple (self-loops
  * dropped, each neighbour once), out[v] for v < n_cap (0 for tombstones),
- * as coreNumbers in priv/zog/src/connectivity.zig:5-101. */
+ * as coreNumbers in priv/zog/src/connectivity.zig:5-101 (the
+ * `nif_core_numbers` NIF, resource_graph.ex:622-628). */
 int zg_core_numbers(zg_graph *g, uint32_t *out);
 
 /* Node features, ZG_NFEAT floats per node into d_features[n_cap][8] (device

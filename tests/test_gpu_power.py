@@ -163,3 +163,40 @@ def test_errors():
     g = make(0, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
     assert rg.katz(g, raw=True) == [] and rg.eigenvector(g, raw=True) == []
     rg.destroy(g)
+
+
+def test_node_degrees_core_numbers_and_copy_in_out_entry_points():
+    # resource_graph.ex:2130-2136 / :562-599 and :2171-2192; Zog.Centrality
+    # (centrality.ex:191-312) builds a resource per call on the same kernels
+    say("degrees, cores, Zog.Centrality")
+    from zog_amd import centrality as C
+    rng = np.random.default_rng(31)
+    n, s, d = graphs.random_multigraph(rng, 500, 4000)
+    live = (rng.random(n) > 0.05).astype(np.uint8)
+    g = make(n, s, d, live)
+    deg = rg.node_degrees(g)
+    ref = np.zeros(n, np.int64)
+    for a, b in zip(s.tolist(), d.tolist()):
+        if live[a] and live[b]:
+            ref[a] += 1
+    assert deg == ref.tolist()
+    cores = rg.core_numbers(g, raw=True)
+    assert cores == og(n, s, d, live).core_numbers().tolist()
+    assert rg.core_numbers(g) == {i: c for i, c in enumerate(cores)}
+    rg.destroy(g)
+    # labelled builder: scores keyed by label, x0.5 for the undirected BC
+    b = SoA.from_list("undirected", [("a", "b", 1.0), ("b", "c", 2.0), ("c", "d", 1.0), ("a", "c", 5.0)])
+    bc = C.betweenness_unweighted(b)
+    g = rg.new(b)
+    assert bc == rg.betweenness_unweighted(g)
+    assert C.betweenness_f64(b) == rg.betweenness_f64(g)
+    assert C.pagerank(b) == rg.pagerank(g)
+    assert C.eigenvector(b) == rg.eigenvector(g)
+    assert C.katz(b) == rg.katz(g)
+    assert C.alpha_centrality(b, alpha=0.2) == rg.alpha_centrality(g, alpha=0.2)
+    assert rg.node_degrees(g) == [2, 2, 3, 1]
+    rg.destroy(g)
+    # the weighted path through the copy-in/out entry point: a -> c costs 3
+    # via b, less than the direct 5.0, so b carries (a, c) and (c, a)
+    w = C.betweenness_f64(b)
+    assert w["b"] > 0.0 and w["a"] == 0.0

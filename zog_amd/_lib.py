@@ -45,6 +45,8 @@ EXPORTED = (
     "zg_graph_create",
     "zg_graph_create_weighted",
     "zg_graph_create_batch",
+    "zg_set_build_threshold",
+    "zg_graph_csr",
     "zg_graph_components",
     "zg_graph_destroy",
     "zg_graph_info",
@@ -65,6 +67,7 @@ EXPORTED = (
     "zg_strerror",
     "zg_bc_weighted",
     "zg_core_numbers",
+    "zg_node_degrees",
     "zg_node_features",
     "zg_mlp_create",
     "zg_mlp_destroy",
@@ -160,6 +163,10 @@ def lib():
     L.zg_graph_create.restype = ctypes.c_int
     L.zg_graph_create_weighted.argtypes = [ctypes.c_uint32, P, P, P, P, ctypes.c_uint64, ctypes.POINTER(P)]
     L.zg_graph_create_weighted.restype = ctypes.c_int
+    L.zg_set_build_threshold.argtypes = [ctypes.c_uint64]
+    L.zg_set_build_threshold.restype = ctypes.c_int
+    L.zg_graph_csr.argtypes = [P, P, P, P, P, P]
+    L.zg_graph_csr.restype = ctypes.c_int
     L.zg_graph_create_batch.argtypes = [ctypes.c_uint32, P, P, P, P, ctypes.POINTER(P)]
     L.zg_graph_create_batch.restype = ctypes.c_int
     L.zg_graph_components.argtypes = [P, P]
@@ -196,6 +203,8 @@ def lib():
     L.zg_set_tuning.restype = ctypes.c_int
     L.zg_bc_weighted.argtypes = [P, P, P, ctypes.c_uint64, P, P, ctypes.POINTER(ZgBcInfo)]
     L.zg_bc_weighted.restype = ctypes.c_int
+    L.zg_node_degrees.argtypes = [P, P]
+    L.zg_node_degrees.restype = ctypes.c_int
     L.zg_core_numbers.argtypes = [P, P]
     L.zg_core_numbers.restype = ctypes.c_int
     L.zg_node_features.argtypes = [P, P, P]

@@ -236,6 +236,7 @@ struct zg_graph {
     int engine = 0;                      // ZG_ENGINE_*
     double *out_w = nullptr;             // [E] arc weights in successor-CSR order (weighted resources)
     bool weights_ok = true;              // every live arc's weight is finite and non-negative
+    bool built_on_gpu = false;           // the CSR came from zg_build_csr_gpu
     uint32_t wbc_batch = 0;              // weighted sweep: sources per batch (0 = default)
     double wbc_delta = 0.0;              // weighted sweep: bound step of the near frontier (0 = default)
     uint32_t wbc_grid = 0;               // weighted sweep: workgroups per round kernel (0 = default)
@@ -273,6 +274,8 @@ void zg_prof_release(zg_graph *g);
         zg_prof_end(g, s, (slot), _zp);              \
     } while (0)
 
+int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
+                     std::vector<uint32_t> &out_ptr, std::vector<uint32_t> &in_ptr, std::vector<uint32_t> &pr_order);
 int zg_bc_ws_release(zg_graph *g);
 int zg_bcq_ws_release(zg_graph *g);
 int zg_wbc_ws_release(zg_graph *g);

@@ -95,6 +95,15 @@ extern "C" int zg_get_profile(zg_graph *g, zg_profile *out)
     return ZG_OK;
 }
 
+// Graphs with at least this many arcs build their CSR on the GPU.
+static uint64_t g_build_gpu_min_arcs = 1u << 18;
+
+extern "C" int zg_set_build_threshold(uint64_t min_arcs)
+{
+    g_build_gpu_min_arcs = min_arcs;
+    return ZG_OK;
+}
+
 static void zg_free_device(zg_graph *g)
 {
     zg_prof_release(g);
@@ -124,21 +133,18 @@ static int zg_upload(T **d, const std::vector<T> &h)
     return ZG_OK;
 }
 
-static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
-                    const double *weight, uint64_t m)
+// Host form of the CSR build (zg_build_csr_gpu in graph_build.hip is the
+// device form, used from ZG_GPU_BUILD_MIN_ARCS arcs on; both produce the
+// same arrays): the successor CSR, newest arc first in every row, arcs with
+// a tombstoned endpoint dropped (the iterator skips them); the predecessor
+// CSR by scanning sources in id order; PageRank ids by decreasing
+// out-degree (stable); uploads them with the weights and the live mask.
+static int zg_build_csr_host(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
+                             std::vector<uint32_t> &out_ptr, std::vector<uint32_t> &in_ptr,
+                             std::vector<uint32_t> &pr_order)
 {
     const uint32_t V = g->V;
-    if (g->h_comp_off.empty()) g->h_comp_off = {0u, V};
-    g->h_live.assign(V, 1);
-    if (live_mask)
-        for (uint32_t i = 0; i < V; i++) g->h_live[i] = live_mask[i] ? 1 : 0;
-    uint32_t n_live = 0;
-    for (uint32_t i = 0; i < V; i++) n_live += g->h_live[i];
-    g->n_live = n_live;
-
-    // Successor CSR.  Arcs touching a tombstoned node are dropped (the
-    // iterator skips them).  Within a row, arcs are stored newest first.
-    std::vector<uint32_t> out_ptr((size_t)V + 1, 0);
+    out_ptr.assign((size_t)V + 1, 0);
     for (uint64_t e = 0; e < m; e++) {
         uint32_t f = from[e], t = to[e];
         if (f >= V || t >= V) return ZG_ERR_NODE_NOT_FOUND;
@@ -170,8 +176,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     }
     g->wmean = E ? wsum / (double)E : 0.0;
     if (weight) ZG_TRY_RC(zg_upload(&g->out_w, out_w));
-    // Predecessor CSR: scan sources in id order.
-    std::vector<uint32_t> in_ptr((size_t)V + 1, 0);
+    in_ptr.assign((size_t)V + 1, 0);
     for (uint64_t k = 0; k < E; k++) in_ptr[(size_t)out_col[k] + 1]++;
     for (uint32_t i = 0; i < V; i++) in_ptr[i + 1] += in_ptr[i];
     std::vector<uint32_t> in_col(E);
@@ -180,6 +185,46 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
         for (uint32_t v = 0; v < V; v++)
             for (uint32_t k = out_ptr[v]; k < out_ptr[(size_t)v + 1]; k++) in_col[fill[out_col[k]]++] = v;
     }
+    pr_order.resize(V);
+    for (uint32_t v = 0; v < V; v++) pr_order[v] = v;
+    std::stable_sort(pr_order.begin(), pr_order.end(), [&](uint32_t x, uint32_t y) {
+        return out_ptr[(size_t)x + 1] - out_ptr[x] > out_ptr[(size_t)y + 1] - out_ptr[y];
+    });
+    std::vector<uint32_t> pr_id(V);
+    for (uint32_t i = 0; i < V; i++) pr_id[pr_order[i]] = i;
+    std::vector<uint32_t> pr_col(E);
+    for (size_t e = 0; e < E; e++) pr_col[e] = pr_id[in_col[e]];
+    ZG_TRY_RC(zg_upload(&g->out_ptr, out_ptr));
+    ZG_TRY_RC(zg_upload(&g->out_col, out_col));
+    ZG_TRY_RC(zg_upload(&g->in_ptr, in_ptr));
+    ZG_TRY_RC(zg_upload(&g->in_col, in_col));
+    ZG_TRY_RC(zg_upload(&g->live, g->h_live));
+    ZG_TRY_RC(zg_upload(&g->pr_id, pr_id));
+    ZG_TRY_RC(zg_upload(&g->pr_col, pr_col));
+    return ZG_OK;
+}
+
+static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
+                    const double *weight, uint64_t m)
+{
+    const uint32_t V = g->V;
+    if (g->h_comp_off.empty()) g->h_comp_off = {0u, V};
+    g->h_live.assign(V, 1);
+    if (live_mask)
+        for (uint32_t i = 0; i < V; i++) g->h_live[i] = live_mask[i] ? 1 : 0;
+    uint32_t n_live = 0;
+    for (uint32_t i = 0; i < V; i++) n_live += g->h_live[i];
+    g->n_live = n_live;
+
+    std::vector<uint32_t> out_ptr, in_ptr, pr_order;
+    const uint64_t gpu_min = g_build_gpu_min_arcs;
+    if (m >= gpu_min) ZG_TRY_RC(zg_build_csr_gpu(g, from, to, weight, m, out_ptr, in_ptr, pr_order));
+    else ZG_TRY_RC(zg_build_csr_host(g, from, to, weight, m, out_ptr, in_ptr, pr_order));
+    g->built_on_gpu = m >= gpu_min;
+    const uint64_t E = g->E;
+    std::vector<uint32_t> pr_id(V);
+    for (uint32_t i = 0; i < V; i++) pr_id[pr_order[i]] = i;
+
     g->h_out_deg.resize(V);
     g->h_in_deg.resize(V);
     uint32_t n_dang = 0;
@@ -238,24 +283,14 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     g->max_och_per_v = max_k;
     g->n_ohub = n_ohub;
 
-    // PageRank ids: vertices by decreasing out-degree, so the rank values
-    // gathered most often share cache lines; the predecessor columns are
-    // relabelled in place (each row keeps its arc order).
-    std::vector<uint32_t> pr_order(V), pr_id(V);
-    for (uint32_t v = 0; v < V; v++) pr_order[v] = v;
-    std::stable_sort(pr_order.begin(), pr_order.end(), [&](uint32_t x, uint32_t y) {
-        return out_ptr[(size_t)x + 1] - out_ptr[x] > out_ptr[(size_t)y + 1] - out_ptr[y];
-    });
-    for (uint32_t i = 0; i < V; i++) pr_id[pr_order[i]] = i;
     {
-        // stage the hottest values in LDS only if they serve >= 10% of the gathers
+        // stage the hottest PageRank values in LDS only if they serve >= 10%
+        // of the gathers (PageRank ids order vertices by decreasing out-degree)
         const uint32_t K = std::min<uint32_t>(V, ZG_PR_XCACHE_BYTES / sizeof(float));
         uint64_t hot = 0;
         for (uint32_t i = 0; i < K; i++) hot += out_ptr[(size_t)pr_order[i] + 1] - out_ptr[pr_order[i]];
         g->pr_xcache_useful = E > 0 && hot * 10 >= E;
     }
-    std::vector<uint32_t> pr_col(in_col.size());
-    for (size_t e = 0; e < in_col.size(); e++) pr_col[e] = pr_id[in_col[e]];
 
     // PageRank row bins over live rows by predecessor count.
     std::vector<uint32_t> small, m4, m16, m64, large, large_deg, large_seg0, seg_row, seg_e0, seg_e1;
@@ -295,12 +330,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     g->n_large = (uint32_t)large.size();
     g->n_seg = (uint32_t)seg_row.size();
 
-    ZG_TRY_RC(zg_upload(&g->out_ptr, out_ptr));
-    ZG_TRY_RC(zg_upload(&g->out_col, out_col));
-    ZG_TRY_RC(zg_upload(&g->in_ptr, in_ptr));
     g->h_in_ptr = in_ptr;
-    ZG_TRY_RC(zg_upload(&g->in_col, in_col));
-    ZG_TRY_RC(zg_upload(&g->live, g->h_live));
     ZG_TRY_RC(zg_upload(&g->ich_v, ich_v));
     ZG_TRY_RC(zg_upload(&g->ich_e0, ich_e0));
     ZG_TRY_RC(zg_upload(&g->ich_e1, ich_e1));
@@ -315,8 +345,6 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     ZG_TRY_RC(zg_upload(&g->pr_desc, desc));
     ZG_TRY_RC(zg_upload(&g->pr_large, large));
     ZG_TRY_RC(zg_upload(&g->pr_large_deg, large_deg));
-    ZG_TRY_RC(zg_upload(&g->pr_id, pr_id));
-    ZG_TRY_RC(zg_upload(&g->pr_col, pr_col));
     ZG_TRY_RC(zg_upload(&g->pr_large_seg0, large_seg0));
     ZG_TRY_RC(zg_upload(&g->pr_seg_row, seg_row));
     ZG_TRY_RC(zg_upload(&g->pr_seg_e0, seg_e0));
@@ -438,6 +466,32 @@ extern "C" int zg_graph_info(const zg_graph *g, uint32_t *n_cap, uint32_t *n_liv
     if (n_cap) *n_cap = g->V;
     if (n_live) *n_live = g->n_live;
     if (n_arcs) *n_arcs = g->E;
+    return ZG_OK;
+}
+
+extern "C" int zg_graph_csr(zg_graph *g, uint32_t *out_ptr, uint32_t *out_col, uint32_t *in_ptr, uint32_t *in_col,
+                            uint32_t *pr_id)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    const size_t V1 = (size_t)g->V + 1, E = g->E;
+    if (out_ptr) ZG_TRY(hipMemcpy(out_ptr, g->out_ptr, sizeof(uint32_t) * V1, hipMemcpyDeviceToHost));
+    if (out_col && E) ZG_TRY(hipMemcpy(out_col, g->out_col, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
+    if (in_ptr) ZG_TRY(hipMemcpy(in_ptr, g->in_ptr, sizeof(uint32_t) * V1, hipMemcpyDeviceToHost));
+    if (in_col && E) ZG_TRY(hipMemcpy(in_col, g->in_col, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
+    if (pr_id && g->V) ZG_TRY(hipMemcpy(pr_id, g->pr_id, sizeof(uint32_t) * g->V, hipMemcpyDeviceToHost));
+    return g->built_on_gpu ? 1 : 0;
+}
+
+extern "C" int zg_node_degrees(zg_graph *g, uint32_t *out)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!out && g->V) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    // arcs of tombstoned endpoints are not in the successor CSR, as removeNode
+    // deletes a node's outgoing and incoming edges (array_graph.zig:128-153)
+    for (uint32_t v = 0; v < g->V; v++) out[v] = g->h_out_deg[v];
     return ZG_OK;
 }
 

@@ -20,6 +20,8 @@ Reference surface kept:
     raw=False)``                               resource_graph.ex:1723-1737
   * ``alpha_centrality(graph, alpha=0.5, initial=1.0, max_iterations=100,
     tolerance=0.0001, raw=False)``             resource_graph.ex:1751-1765
+  * ``node_degrees(graph)``                    resource_graph.ex:2130-2136
+  * ``core_numbers(graph, raw=False)``         resource_graph.ex:2171-2192
   * ``raw=True`` returns the dense list indexed by u32 id; otherwise scores
     are mapped to labels (map_scores, :2609-2619): integer keys for an
     integer-labelled builder, else labels in insertion order.
@@ -379,6 +381,27 @@ def alpha_centrality(graph: ResourceGraph, alpha: float = 0.5, initial: float = 
         raise ValueError("max_iterations must be >= 0")
     return _power(graph, "zg_alpha_centrality",
                   (float(alpha), float(initial), int(max_iterations), float(tolerance)), raw, info)
+
+
+def node_degrees(graph: ResourceGraph):
+    """Live out-degrees by u32 id (resource_graph.ex:2130-2136; NIF :562-599)."""
+    res = graph.resource
+    out = np.zeros(res.n_cap, dtype=np.uint32)
+    _lib.check(_lib.lib().zg_node_degrees(res.handle, _ptr(out)), "zg_node_degrees")
+    return out.tolist()
+
+
+def core_numbers(graph: ResourceGraph, raw: bool = False):
+    """Core numbers (resource_graph.ex:2171-2192; NIF :622-628): the raw
+    list by u32 id, or labels in insertion order mapped to their core number."""
+    res = graph.resource
+    out = np.zeros(res.n_cap, dtype=np.uint32)
+    _lib.check(_lib.lib().zg_core_numbers(res.handle, _ptr(out)), "zg_core_numbers")
+    cores = out.tolist()
+    if raw:
+        return cores
+    labels = graph.builder.all_labels()
+    return {labels[i]: c for i, c in enumerate(cores)}
 
 
 def pagerank_array(graph: ResourceGraph, damping: float = 0.85, max_iterations: int = 100,
