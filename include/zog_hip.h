@@ -31,6 +31,7 @@ extern "C" {
 #define ZG_ERR_NO_DEVICE (-6)        /* no usable gfx950 device */
 #define ZG_ERR_TOO_LARGE (-7)        /* more than 2^32-1 arcs or nodes */
 #define ZG_ERR_INVALID_HANDLE (-8)   /* NULL or destroyed handle */
+#define ZG_ERR_IO (-9)               /* a file could not be opened or read */
 
 typedef struct zg_graph zg_graph;
 
@@ -111,6 +112,27 @@ int zg_set_build_threshold(uint64_t min_arcs);
  * No reference counterpart (parity checks, callers that want the arrays). */
 int zg_graph_csr(zg_graph *g, uint32_t *out_ptr, uint32_t *out_col, uint32_t *in_ptr, uint32_t *in_col,
                  uint32_t *pr_id);
+
+/* Replaces the parsing half of `nif_read_edgelist` (resource_graph.ex:
+ * 990-1084) with `addParsedEdges` (:953-988): reads an edge list (lines of
+ * `src dst [weight]`, separators any of space, tab, comma; `#` comments; a
+ * `source target` header; weight 1.0 when absent or not a number) with
+ * integer ids (integer_labels != 0; node count = largest id + 1; lines whose
+ * ids do not parse as u32 are skipped) or labels numbered by first
+ * appearance, and expands it into arcs: file order when directed, else one
+ * arc pair per unordered pair (the first in file order, by a stable sort on
+ * (min, max); one arc for a self-loop).  The arcs then go to
+ * zg_graph_create_weighted.  Host memory only.  ZG_ERR_IO when the file
+ * cannot be read. */
+typedef struct zg_edgelist zg_edgelist;
+int zg_edgelist_read(const char *path, int directed, int integer_labels, zg_edgelist **out);
+/* The expanded arcs and the node count; pointers stay valid until free. */
+int zg_edgelist_arcs(const zg_edgelist *el, const uint32_t **from, const uint32_t **to, const double **weight,
+                     uint64_t *n_arcs, uint32_t *n_nodes);
+/* Labels in id order: label i is blob[offsets[i] .. offsets[i+1]) (none
+ * with integer labels). */
+int zg_edgelist_labels(const zg_edgelist *el, const char **blob, const uint64_t **offsets, uint32_t *n_labels);
+void zg_edgelist_free(zg_edgelist *el);
 
 /* Batch of independent graphs in one resource (no reference counterpart; the
  * reference builds one resource per graph with new/5): graph i has

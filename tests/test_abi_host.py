@@ -84,36 +84,72 @@ def test_map_scores_and_scaling():
     assert rg._map_scores(i, [0.5, 0.25, 0.25]) == {0: 0.5, 1: 0.25, 2: 0.25}
 
 
+def _read(tmp_path, text, directed=True, integer_labels=False, name="e.txt"):
+    p = tmp_path / name
+    p.write_bytes(text.encode() if isinstance(text, str) else text)
+    return rg._read_edgelist_native(str(p), directed, integer_labels)
+
+
 def test_edgelist_parse_integer_labels(tmp_path):
-    # resource_graph_test.exs:545-571 input; placeholder ids up to max id
-    p = tmp_path / "e.txt"
-    p.write_text("# comment\nsource target\n0 1 1.5\n1 2 2.0\n\n2 0 1.0\nx 3\n5,4\n")
-    edges, next_id, labels = rg._parse_edgelist(str(p), True)
-    assert next_id == 6
-    assert [(u, v) for u, v, _ in edges] == [(0, 1), (1, 2), (2, 0), (5, 4)]
-    assert edges[0][2] == 1.5
+    # resource_graph_test.exs:545-571 input; placeholder ids up to max id;
+    # a comment, a header, a blank line, a non-numeric id (skipped), commas
+    src, dst, w, n, labels = _read(tmp_path, "# comment\nsource target\n0 1 1.5\n1 2 2.0\n\n2 0 1.0\nx 3\n5,4\n",
+                                   integer_labels=True)
+    assert n == 6 and labels is None
+    assert list(zip(src.tolist(), dst.tolist())) == [(0, 1), (1, 2), (2, 0), (5, 4)]
+    assert w.tolist() == [1.5, 2.0, 1.0, 1.0]
 
 
 def test_edgelist_parse_labels_first_seen(tmp_path):
-    p = tmp_path / "e.txt"
-    p.write_text("b a\na c\tw\nc,b,2.5\n")
-    edges, next_id, labels = rg._parse_edgelist(str(p), False)
-    assert labels == ["b", "a", "c"] and next_id == 3
-    assert [(u, v, w) for u, v, w in edges] == [(0, 1, 1.0), (1, 2, 1.0), (2, 0, 2.5)]
+    # a tab before a non-numeric weight (1.0), commas, a last line without
+    # a newline, CRLF line ends, a header in any case
+    src, dst, w, n, labels = _read(tmp_path, "Source TARGET\r\nb a\r\na c\tw\nc,b,2.5")
+    assert labels == ["b", "a", "c"] and n == 3
+    assert list(zip(src.tolist(), dst.tolist(), w.tolist())) == [(0, 1, 1.0), (1, 2, 1.0), (2, 0, 2.5)]
 
 
-def test_undirected_dedupe_rule():
-    # addParsedEdges (resource_graph.ex:953-988): sort by (min, max), drop
-    # repeats of an unordered pair, both arcs unless a self-loop
-    # repeats of an unordered pair (the first in input order keeps its
-    # weight: the sort is stable), both arcs unless a self-loop
-    edges = [(2, 1, 1.0), (0, 1, 4.0), (1, 2, 2.0), (3, 3, 3.0), (1, 0, 5.0)]
-    src, dst, w = rg._expand_edges(edges, directed=False)
-    assert list(zip(src, dst)) == [(0, 1), (1, 0), (2, 1), (1, 2), (3, 3)]
-    assert w == [4.0, 4.0, 1.0, 1.0, 3.0]
-    src, dst, w = rg._expand_edges(edges, directed=True)
-    assert list(zip(src, dst)) == [(2, 1), (0, 1), (1, 2), (3, 3), (1, 0)]
-    assert w == [1.0, 4.0, 2.0, 3.0, 5.0]
+def test_edgelist_number_forms(tmp_path):
+    # std.fmt.parseInt(u32, _, 10) and std.fmt.parseFloat(f64, _): '+',
+    # underscores between digits, exponents, hex floats, inf; anything else
+    # skips the line (ids) or gives weight 1.0
+    text = "\n".join(["+1 2 1e-3", "1_0 3 0x1p3", "4 5 -inf", "_4 5 2", "4 5_ 2", "-1 5 2", "4294967296 1 2",
+                      "6 7 1_5.0_0", "6 7 1__5", "6 7 abc", "6 7 nan(1)", "6 7 +2.5", "8 9 .5", "8 9 1."])
+    src, dst, w, n, _ = _read(tmp_path, text, integer_labels=True)
+    assert list(zip(src.tolist(), dst.tolist())) == [(1, 2), (10, 3), (4, 5), (6, 7), (6, 7), (6, 7), (6, 7),
+                                                      (6, 7), (8, 9), (8, 9)]
+    assert w[:3].tolist() == [1e-3, 8.0, float("-inf")]
+    assert w[3:].tolist() == [15.0, 1.0, 1.0, 1.0, 2.5, 0.5, 1.0]
+    assert n == 11
+
+
+def test_undirected_dedupe_rule(tmp_path):
+    # addParsedEdges (resource_graph.ex:953-988): a stable sort by (min, max),
+    # the first arc of an unordered pair in file order keeps its weight,
+    # both arcs unless a self-loop
+    text = "2 1 1.0\n0 1 4.0\n1 2 2.0\n3 3 3.0\n1 0 5.0\n"
+    src, dst, w, _, _ = _read(tmp_path, text, directed=False, integer_labels=True)
+    assert list(zip(src.tolist(), dst.tolist())) == [(0, 1), (1, 0), (2, 1), (1, 2), (3, 3)]
+    assert w.tolist() == [4.0, 4.0, 1.0, 1.0, 3.0]
+    src, dst, w, _, _ = _read(tmp_path, text, directed=True, integer_labels=True)
+    assert list(zip(src.tolist(), dst.tolist())) == [(2, 1), (0, 1), (1, 2), (3, 3), (1, 0)]
+    assert w.tolist() == [1.0, 4.0, 2.0, 3.0, 5.0]
+
+
+def test_edgelist_reference_fixture_and_errors(tmp_path):
+    # the reference's fixture test/fixtures/wiki_vote_sample.txt read as
+    # labels: ids in first-seen order, as tests/graphs.wiki_vote builds them
+    from tests import graphs
+    src, dst, w, n, labels = rg._read_edgelist_native(graphs.WIKI_VOTE, True, False)
+    n2, s2, d2 = graphs.wiki_vote()
+    assert n == n2 and np.array_equal(src, s2) and np.array_equal(dst, d2) and np.all(w == 1.0)
+    with pytest.raises(FileNotFoundError):
+        rg._read_edgelist_native(str(tmp_path / "missing.txt"), True, False)
+    # an id whose node count would not fit in u32
+    with pytest.raises(_lib.ZogError):
+        _read(tmp_path, "4294967295 0\n", integer_labels=True)
+    # empty file
+    src, dst, w, n, labels = _read(tmp_path, "", integer_labels=False)
+    assert len(src) == 0 and n == 0 and labels == []
 
 
 def test_generators_deterministic():

@@ -21,6 +21,7 @@ ZG_ERR_HIP = -5
 ZG_ERR_NO_DEVICE = -6
 ZG_ERR_TOO_LARGE = -7
 ZG_ERR_INVALID_HANDLE = -8
+ZG_ERR_IO = -9
 
 ZG_PR_FP32 = 32
 ZG_PR_FP64 = 64
@@ -47,6 +48,10 @@ EXPORTED = (
     "zg_graph_create_batch",
     "zg_set_build_threshold",
     "zg_graph_csr",
+    "zg_edgelist_read",
+    "zg_edgelist_arcs",
+    "zg_edgelist_labels",
+    "zg_edgelist_free",
     "zg_graph_components",
     "zg_graph_destroy",
     "zg_graph_info",
@@ -165,6 +170,15 @@ def lib():
     L.zg_graph_create_weighted.restype = ctypes.c_int
     L.zg_set_build_threshold.argtypes = [ctypes.c_uint64]
     L.zg_set_build_threshold.restype = ctypes.c_int
+    L.zg_edgelist_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
+    L.zg_edgelist_read.restype = ctypes.c_int
+    L.zg_edgelist_arcs.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
+                                   ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]
+    L.zg_edgelist_arcs.restype = ctypes.c_int
+    L.zg_edgelist_labels.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(ctypes.c_uint32)]
+    L.zg_edgelist_labels.restype = ctypes.c_int
+    L.zg_edgelist_free.argtypes = [P]
+    L.zg_edgelist_free.restype = None
     L.zg_graph_csr.argtypes = [P, P, P, P, P, P]
     L.zg_graph_csr.restype = ctypes.c_int
     L.zg_graph_create_batch.argtypes = [ctypes.c_uint32, P, P, P, P, ctypes.POINTER(P)]

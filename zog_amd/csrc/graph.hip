@@ -28,6 +28,7 @@ extern "C" const char *zg_strerror(int code)
     case ZG_ERR_NO_DEVICE: return "no gfx950 device";
     case ZG_ERR_TOO_LARGE: return "graph too large (more than 2^32-1 arcs or nodes)";
     case ZG_ERR_INVALID_HANDLE: return "invalid graph handle";
+    case ZG_ERR_IO: return "file could not be read";
     default: return "unknown error";
     }
 }

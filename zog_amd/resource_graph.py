@@ -5,8 +5,10 @@ through its C ABI (include/zog_hip.h).
 Reference surface kept:
   * ``new(builder, backend="soa")``            resource_graph.ex:1393-1404
   * ``read_edgelist(path, directed=True, backend="soa", integer_labels=False)``
-                                               resource_graph.ex:1474-1486, parse
-                                               :990-1084, undirected dedupe :953-988
+                                               resource_graph.ex:1474-1486; parse
+                                               :990-1084 and undirected dedupe
+                                               :953-988 in the library's native
+                                               reader (zg_edgelist_read)
   * ``destroy(graph)``                         resource_graph.ex:1577-1580
   * ``betweenness_unweighted(graph, raw=False)``  resource_graph.ex:1590-1601
     (x0.5 for undirected builders before the raw branch, :2633-2637)
@@ -181,83 +183,52 @@ def destroy(graph: ResourceGraph) -> str:
 # ---------------------------------------------------------------------------
 # edge-list ingest (nif_read_edgelist, resource_graph.ex:990-1084)
 
-def _parse_edgelist(path: str, integer_labels: bool):
-    label_to_id = {}
-    labels = []
-    next_id = 0
-    edges = []
-    with open(path, "rb") as fh:
-        for raw in fh.read().split(b"\n"):
-            line = raw.strip(b" \t\r")
-            if not line or line[:1] == b"#":
-                continue
-            parts = [p for p in line.replace(b",", b" ").replace(b"\t", b" ").split(b" ") if p]
-            if len(parts) < 2:
-                continue
-            a, b = parts[0], parts[1]
-            if a.lower() == b"source" and b.lower() == b"target":
-                continue
-            w = 1.0
-            if len(parts) > 2:
-                try:
-                    w = float(parts[2])
-                except ValueError:
-                    w = 1.0
-            if integer_labels:
-                try:
-                    u, v = int(a), int(b)
-                except ValueError:
-                    continue
-                if not (0 <= u < 2**32 and 0 <= v < 2**32):
-                    continue
-                next_id = max(next_id, max(u, v) + 1)
-            else:
-                ids = []
-                for lab in (a, b):
-                    key = lab.decode()
-                    i = label_to_id.get(key)
-                    if i is None:
-                        i = next_id
-                        label_to_id[key] = i
-                        labels.append(key)
-                        next_id += 1
-                    ids.append(i)
-                u, v = ids
-            edges.append((u, v, w))
-    return edges, next_id, labels
+def _read_edgelist_native(path: str, directed: bool, integer_labels: bool):
+    """Parse and expand an edge list with the library's reader
+    (zg_edgelist_read): (src u32[], dst u32[], weight f64[], node count,
+    labels in id order or None for integer labels)."""
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = L.zg_edgelist_read(os.fsencode(path), int(bool(directed)), int(bool(integer_labels)), ctypes.byref(h))
+    if rc == _lib.ZG_ERR_IO:
+        raise FileNotFoundError(path)
+    _lib.check(rc, "zg_edgelist_read")
+    try:
+        pf, pt, pw = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        m = ctypes.c_uint64()
+        nn = ctypes.c_uint32()
+        _lib.check(L.zg_edgelist_arcs(h, ctypes.byref(pf), ctypes.byref(pt), ctypes.byref(pw), ctypes.byref(m),
+                                      ctypes.byref(nn)), "zg_edgelist_arcs")
+        M = int(m.value)
 
+        def arr(p, ct, dt):
+            if M == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(M,)).copy()
 
-def _expand_edges(edges, directed: bool):
-    """addParsedEdges (resource_graph.ex:953-988): directed arcs in file order;
-    undirected edges sorted by (min, max), duplicates dropped, both arcs added
-    except for a self-loop (one arc)."""
-    if directed:
-        return [e[0] for e in edges], [e[1] for e in edges], [e[2] for e in edges]
-    order = sorted(range(len(edges)), key=lambda i: (min(edges[i][0], edges[i][1]), max(edges[i][0], edges[i][1])))
-    src, dst, wts = [], [], []
-    prev = None
-    for i in order:
-        u, v, w = edges[i]
-        key = (min(u, v), max(u, v))
-        if key == prev:
-            continue
-        src.append(u)
-        dst.append(v)
-        wts.append(w)
-        if u != v:
-            src.append(v)
-            dst.append(u)
-            wts.append(w)
-        prev = key
-    return src, dst, wts
+        src = arr(pf, ctypes.c_uint32, np.uint32)
+        dst = arr(pt, ctypes.c_uint32, np.uint32)
+        w = arr(pw, ctypes.c_double, np.float64)
+        labels = None
+        if not integer_labels:
+            pb, po = ctypes.c_void_p(), ctypes.c_void_p()
+            nl = ctypes.c_uint32()
+            _lib.check(L.zg_edgelist_labels(h, ctypes.byref(pb), ctypes.byref(po), ctypes.byref(nl)),
+                       "zg_edgelist_labels")
+            n_l = int(nl.value)
+            offs = np.ctypeslib.as_array(ctypes.cast(po, ctypes.POINTER(ctypes.c_uint64)), shape=(n_l + 1,))
+            blob = ctypes.string_at(pb, int(offs[-1])) if n_l else b""
+            labels = [blob[int(offs[i]):int(offs[i + 1])].decode() for i in range(n_l)]
+        return src, dst, w, int(nn.value), labels
+    finally:
+        L.zg_edgelist_free(h)
 
 
 def read_edgelist(path: str, directed: bool = True, backend: str = "soa",
                   integer_labels: bool = False) -> ResourceGraph:
     if backend not in BACKENDS:
         raise ValueError(f"backend {backend!r} is not accelerated here (only :soa)")
-    edges, next_id, labels = _parse_edgelist(os.path.expanduser(path), integer_labels)
-    src, dst, wts = _expand_edges(edges, directed)
+    src, dst, wts, next_id, labels = _read_edgelist_native(os.path.expanduser(path), directed, integer_labels)
     kind = "directed" if directed else "undirected"
     # build_from_labels (resource_graph.ex:1531-1575): the builder carries only
     # labels; its edge list stays empty
@@ -268,8 +239,7 @@ def read_edgelist(path: str, directed: bool = True, backend: str = "soa",
     else:
         for lab in labels:
             b.ensure_node(lab)
-    res = Resource(next_id, np.asarray(src, dtype=np.uint32), np.asarray(dst, dtype=np.uint32),
-                   weights=np.asarray(wts, dtype=np.float64))
+    res = Resource(next_id, src, dst, weights=wts)
     return ResourceGraph(resource=res, builder=b)
 
 
