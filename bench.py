@@ -633,7 +633,7 @@ def main():
         threads = 1
         og = O.Graph(n, src, dst)
         sample = sources if spec["cpu_sources"] is None else sources[: spec["cpu_sources"]]
-        mode = O.SIGMA_SCALED if spec.get("scaled") else O.SIGMA_U64
+        mode = O.SIGMA_EXP if spec.get("scaled") else O.SIGMA_U64
         t0 = time.perf_counter()
         ref_bc, _, arcs_cpu = og.betweenness(sources=sample, threads=threads, return_arcs=True, mode=mode)
         cpu_s = time.perf_counter() - t0

@@ -26,6 +26,8 @@
  *     power-of-two shift, so counts beyond 2^64 (and beyond the FP64 range)
  *     stay usable; it reduces to the reference's formula whenever no shift
  *     is taken,
+ *   - an "exponent" mode that gives every count its own binary exponent, for
+ *     graphs whose counts on one level span more than the FP64 range,
  *   - per-source export of BFS depths and path counts,
  *   - an N-thread source-chunked BC (the pattern of centrality.zig:216-256).
  *   - core numbers, coreNumbers in priv/zog/src/connectivity.zig:5-101
@@ -179,7 +181,15 @@ uint64_t zo_successors(const zo_graph *g, uint32_t v, uint32_t *out, uint64_t ca
  * reference's (on FP64 counts). */
 #define ZO_SIGMA_U64 0
 #define ZO_SIGMA_SCALED 1
+#define ZO_SIGMA_EXP 2
 #define ZO_SHIFT_T 512
+/* ZO_SIGMA_EXP keeps every count as a mantissa in [1, 2) (sigma_f) and its
+ * own binary exponent (sexp): sigma = sigma_f * 2^sexp.  Counts are added in
+ * the reference's order (:763), aligned to the larger exponent and
+ * renormalised; below 2^53 this is exact.  sigma_u/sigma_w in the backward
+ * sweep is (sigma_f[u] / sigma_f[w]) * 2^(sexp[u] - sexp[w]).  Unlike the
+ * per-level shift, it represents levels whose counts span more than the
+ * FP64 range (a 2048 x 2048 grid: 1 to 2^4000 on one level). */
 
 typedef struct {
     /* per-source workspaces (centrality.zig:697-726) */
@@ -192,6 +202,7 @@ typedef struct {
     uint32_t *pred_head, *pred_tail, *pred_node, *pred_next;
     uint64_t pred_cap;
     int32_t *shift;     /* k_L per level (scaled mode) */
+    int32_t *sexp;      /* per-vertex exponent (exponent mode) */
 } zo_ws;
 
 static void zo_ws_free(zo_ws *w)
@@ -199,6 +210,7 @@ static void zo_ws_free(zo_ws *w)
     free(w->dist); free(w->sigma_u); free(w->sigma_f); free(w->delta); free(w->queue);
     free(w->pred_head); free(w->pred_tail); free(w->pred_node); free(w->pred_next);
     free(w->shift);
+    free(w->sexp);
 }
 
 static int zo_ws_init(zo_ws *w, const zo_graph *g)
@@ -216,8 +228,9 @@ static int zo_ws_init(zo_ws *w, const zo_graph *g)
     w->pred_node = malloc(sizeof(uint32_t) * w->pred_cap);
     w->pred_next = malloc(sizeof(uint32_t) * w->pred_cap);
     w->shift = malloc(sizeof(int32_t) * (V + 1));
+    w->sexp = malloc(sizeof(int32_t) * V);
     if (!w->dist || !w->sigma_u || !w->sigma_f || !w->delta || !w->queue || !w->pred_head ||
-        !w->pred_tail || !w->pred_node || !w->pred_next || !w->shift) {
+        !w->pred_tail || !w->pred_node || !w->pred_next || !w->shift || !w->sexp) {
         zo_ws_free(w);
         return ZO_ERR_OOM;
     }
@@ -245,6 +258,7 @@ static zo_src_stats zo_brandes_one(const zo_graph *g, zo_ws *w, uint32_t s, int 
         w->delta[i] = 0.0;
         w->pred_head[i] = ZO_NONE;
         w->pred_tail[i] = ZO_NONE;
+        w->sexp[i] = 0;
     }
     uint64_t npred = 0;
     uint32_t qlen = 0, head = 0;
@@ -284,7 +298,24 @@ static zo_src_stats zo_brandes_one(const zo_graph *g, zo_ws *w, uint32_t s, int 
                 uint64_t sum;
                 if (__builtin_add_overflow(w->sigma_u[t], w->sigma_u[v], &sum)) st.overflow = 1;
                 w->sigma_u[t] = sum;
-                w->sigma_f[t] += ldexp(w->sigma_f[v], -k_cur);
+                if (mode == ZO_SIGMA_EXP) {
+                    /* (f_t, e_t) += (f_v, e_v), then normalise f_t to [1, 2) */
+                    double f = w->sigma_f[t];
+                    int32_t e = w->sexp[t];
+                    if (f == 0.0) {
+                        f = w->sigma_f[v];
+                        e = w->sexp[v];
+                    } else {
+                        const int32_t E = e > w->sexp[v] ? e : w->sexp[v];
+                        f = ldexp(f, e - E) + ldexp(w->sigma_f[v], w->sexp[v] - E);
+                        e = E;
+                    }
+                    const int k = ilogb(f);
+                    w->sigma_f[t] = ldexp(f, -k);
+                    w->sexp[t] = e + k;
+                } else {
+                    w->sigma_f[t] += ldexp(w->sigma_f[v], -k_cur);
+                }
                 /* preds[t].append(v) */
                 uint32_t p = (uint32_t)npred++;
                 w->pred_node[p] = v;
@@ -308,6 +339,12 @@ static zo_src_stats zo_brandes_one(const zo_graph *g, zo_ws *w, uint32_t s, int 
                 uint32_t u = w->pred_node[p];
                 double c = ((double)w->sigma_u[u] / sigma_v) * (1.0 + delta_v);
                 w->delta[u] += c;
+            }
+        } else if (mode == ZO_SIGMA_EXP) {
+            for (uint32_t p = w->pred_head[v]; p != ZO_NONE; p = w->pred_next[p]) {
+                uint32_t u = w->pred_node[p];
+                double ratio = ldexp(w->sigma_f[u] / w->sigma_f[v], w->sexp[u] - w->sexp[v]);
+                w->delta[u] += ratio * (1.0 + delta_v);
             }
         } else {
             double sigma_v = w->sigma_f[v];
@@ -416,6 +453,29 @@ int zo_bc_unweighted(const zo_graph *g, const uint32_t *sources, uint64_t n_sour
     free(th);
     free(all);
     return rc;
+}
+
+/* Per-source export of the exponent mode: depth, count mantissa in [1, 2)
+ * and exponent per vertex (0 / 0 when unreached). */
+int zo_bfs_paths_exp(const zo_graph *g, uint32_t s, uint32_t *depth_out, double *mant_out, int32_t *exp_out)
+{
+    if (!zo_has_node(g, s)) return ZO_ERR_NODE_NOT_FOUND;
+    zo_ws ws;
+    if (zo_ws_init(&ws, g) != ZO_OK) return ZO_ERR_OOM;
+    double *scratch = calloc(g->n_cap ? g->n_cap : 1, sizeof(double));
+    if (!scratch) {
+        zo_ws_free(&ws);
+        return ZO_ERR_OOM;
+    }
+    zo_brandes_one(g, &ws, s, ZO_SIGMA_EXP, scratch);
+    for (uint32_t i = 0; i < g->n_cap; i++) {
+        if (depth_out) depth_out[i] = ws.dist[i];
+        if (mant_out) mant_out[i] = ws.sigma_f[i];
+        if (exp_out) exp_out[i] = ws.sexp[i];
+    }
+    free(scratch);
+    zo_ws_free(&ws);
+    return ZO_OK;
 }
 
 /* Per-source BFS export: depth (ZO_NONE if unreached), path counts as u64

@@ -22,6 +22,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 SIGMA_U64 = 0
 SIGMA_SCALED = 1
+SIGMA_EXP = 2
 NONE_U32 = 0xFFFFFFFF
 
 _lib = None
@@ -51,6 +52,8 @@ def lib():
         L.zo_bc_unweighted.restype = ctypes.c_int
         L.zo_bfs_paths.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P]
         L.zo_bfs_paths.restype = ctypes.c_int
+        L.zo_bfs_paths_exp.argtypes = [P, ctypes.c_uint32, P, P, P]
+        L.zo_bfs_paths_exp.restype = ctypes.c_int
         L.zo_pagerank.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P, P, P]
         L.zo_core_numbers.argtypes = [P, P]
         L.zo_core_numbers.restype = ctypes.c_int
@@ -140,6 +143,18 @@ class Graph:
         if rc != 0:
             raise OracleError(f"zo_bfs_paths failed: {rc}")
         return depth, su, sf, shift[: levels.value].copy(), int(arcs.value), bool(ovf.value)
+
+    def bfs_paths_exp(self, s: int):
+        """(depth u32[V], count mantissa f64[V] in [1, 2), exponent i32[V]) of
+        the per-vertex-exponent mode: sigma = mantissa * 2^exponent."""
+        V = self.n_cap
+        depth = np.empty(V, dtype=np.uint32)
+        mant = np.empty(V, dtype=np.float64)
+        ex = np.empty(V, dtype=np.int32)
+        rc = lib().zo_bfs_paths_exp(self._h, s, _ptr(depth), _ptr(mant), _ptr(ex))
+        if rc != 0:
+            raise OracleError(f"zo_bfs_paths_exp failed: {rc}")
+        return depth, mant, ex
 
     def pagerank(self, damping=0.85, max_iterations=100, tolerance=1e-4):
         """FP64 PageRank, f64[n_cap]; returns (ranks, iterations, last_l1)."""

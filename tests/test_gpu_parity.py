@@ -360,6 +360,28 @@ def test_grid_overflowing_path_counts():
     rg.destroy(g)
 
 
+def test_queue_engine_counts_beyond_one_level_fp64_range():
+    # 1600x1600 grid: a level's path counts span more than the FP64 range
+    # (1 on the edges, ~2^1590 mid-grid from a corner); the queue engine's
+    # per-vertex exponents against the oracle's, BC finite, depths equal
+    from oracle import oracle as O
+    R = 1600
+    n, src, dst = gen.grid_2d(R, R)
+    sources = np.array([0, R - 1, (R // 2) * R + R // 2, R * R - 1], np.uint32)
+    og = O.Graph(n, src, dst)
+    ref, _ = og.betweenness(sources=sources, mode=O.SIGMA_EXP)
+    assert np.all(np.isfinite(ref))
+    g = make(n, src, dst)
+    rg.set_engine(g, "queue")
+    info = {}
+    got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
+    rg.destroy(g)
+    assert np.all(np.isfinite(got))
+    assert_bc_close(got, ref)
+    depth = og.bfs_paths(0)[0]
+    assert info["max_levels"] == int(depth.max()) + 1
+
+
 def test_dependency_checksum_rmat():
     # sum_v BC(v) over a source set = sum over sources of sum_t (dist(s,t) - 1)
     n, src, dst = gen.rmat(13, 1 << 17, 42)

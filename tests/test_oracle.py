@@ -360,6 +360,55 @@ def test_scaled_sigma_shift_beyond_fp64_range(oracle_mod):
     assert np.allclose(B, B.T, rtol=1e-9, atol=1e-6)
 
 
+def test_exp_sigma_matches_bigint_and_scaled(oracle_mod):
+    # per-vertex exponents: the 40x40 exact-ratio case and the 300x300 case
+    # where both FP64 modes are valid
+    O = oracle_mod
+    n, src, dst = _grid(40, 40)
+    g = O.Graph(n, src, dst)
+    a, _ = g.betweenness(sources=[0], mode=O.SIGMA_EXP)
+    b, _ = g.betweenness(sources=[0], mode=O.SIGMA_SCALED)
+    assert np.allclose(a, b, rtol=1e-13, atol=0)
+    depth, mant, ex = g.bfs_paths_exp(0)
+    dist, sig, _, _ = _bigint_sigma(n, src, dst, 0)
+    for v in range(0, n, 37):
+        got = mant[v] * 2.0 ** ex[v]
+        if sig[v] < 2 ** 53:
+            assert got == float(sig[v])  # exact
+        else:
+            assert abs(got - float(sig[v])) <= 1e-13 * float(sig[v])  # rounded sums
+        assert 1.0 <= mant[v] < 2.0
+    n, src, dst = _grid(300, 300)
+    g = O.Graph(n, src, dst)
+    a, _ = g.betweenness(sources=[0, 299, 45150], mode=O.SIGMA_EXP)
+    b, _ = g.betweenness(sources=[0, 299, 45150], mode=O.SIGMA_SCALED)
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+
+
+def test_exp_sigma_beyond_one_level_fp64_range(oracle_mod):
+    # 1600x1600 grid from a corner: a level's counts run from 1 (on the
+    # grid's edges) to ~2^(level - log2 level), and the shifts taken for the
+    # largest add up past 2^1074, so the per-level mode loses the small
+    # counts (and goes non-finite) while per-vertex exponents keep them all
+    O = oracle_mod
+    R = 1600
+    n, src, dst = _grid(R, R)
+    g = O.Graph(n, src, dst)
+    sc, _ = g.betweenness(sources=[0], mode=O.SIGMA_SCALED)
+    assert not np.all(np.isfinite(sc))
+    bc, _ = g.betweenness(sources=[0], mode=O.SIGMA_EXP)
+    assert np.all(np.isfinite(bc))
+    depth, mant, ex = g.bfs_paths_exp(0)
+    for r, c in [(0, 0), (0, R - 1), (1, 1), (R // 2, R // 2), (R - 1, R - 1), (700, 13), (3, 1590), (1599, 2)]:
+        v = r * R + c
+        log2_true = math.log2(math.comb(r + c, r))  # lattice paths from the corner
+        assert abs(math.log2(mant[v]) + int(ex[v]) - log2_true) < 1e-12 * max(1.0, log2_true)
+    checksum = int((depth.astype(np.int64) - 1).clip(0).sum())
+    assert abs(bc.sum() - checksum) <= 1e-9 * checksum
+    B = bc.reshape(R, R)
+    assert np.allclose(B, B.T, rtol=1e-9, atol=1e-6)
+
+
 # --- the parity configuration ------------------------------------------------
 
 def test_er10k_checksum_and_threads(oracle_mod):

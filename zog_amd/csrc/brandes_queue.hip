@@ -3,12 +3,17 @@
 // Reference: priv/zog/src/centrality.zig:694-798.  The reference keeps, per
 // source, a BFS queue whose order is also the stack of the dependency sweep;
 // this engine keeps exactly that per source, for S sources at once:
-//   depth[s][v] (u32), sig[s][v] (FP64 stored path counts), cc[k][s][v]
-//   (FP64 (1+delta)/sigma, double-buffered by level parity), queue[s][i],
-//   lstart[L][s] (queue offset of level L), qlen[s].
-// One lane handles one (source, frontier vertex) pair; a wavefront always
-// holds pairs of a single source, so discoveries are appended to that
-// source's queue with one atomic per wavefront (ballot + prefix).
+//   de[s][v]   {depth (u32), exponent (i32)} in one 8-byte word
+//   sig[s][v]  path-count mantissa (FP64): sigma = sig * 2^exponent, sig in
+//              [1, 2) -- every count has its own exponent, so a level whose
+//              counts run from 1 to 2^4000 (a 2048 x 2048 grid) is still
+//              represented; counts below 2^53 are exact
+//   cc[s][v]   (1 + delta) / sig, the dependency term c = (1+delta)/sigma
+//              scaled by 2^exponent
+//   queue[s][i], lstart[L][s] (queue offset of level L), qlen[s].
+// One lane handles one (source, vertex) pair; a wavefront always holds pairs
+// of a single source, so discoveries are appended to that source's queue
+// with one atomic per wavefront (ballot + prefix).
 //
 // Why a second engine: the 64-lane engine (brandes.hip) shares one row
 // access among the group's sources, which pays when they reach a vertex at
@@ -16,14 +21,15 @@
 // scattered sources each vertex sits at ~64 different depths, so a row
 // access serves one lane; here every access serves its own pair.
 //
-// Forward level d: for v at depth d and each arc v->w: if depth[w] is unset,
-// CAS it to d+1 (the winner appends w); if depth[w] == d+1, atomically add
-// sigma'[v] * 2^-k_d to sigma[w] (exact for integer counts below 2^53).  A
-// close step over the new level clears c, records the level's largest
-// exponent (shift k_{d+1}) and the next level offset.  Backward level d:
-// delta(v) = sigma'(v) 2^-k_d sum_{w in succ(v)} c(w), with c(w) nonzero
-// exactly for BFS children (same argument as the 64-lane engine), then
-// bc[v] += delta(v) for v != s (FP64 atomics across sources).
+// Forward level d (one launch): for v at depth d, first its path count,
+// pulled from its predecessors at depth d-1 over the predecessor CSR (the
+// sum of their counts, parallel arcs counted once each, aligned to the
+// largest exponent, then normalised), then for each arc v->w with depth[w]
+// unset, a CAS to d+1 (the winner appends w).  Backward level d: delta(v) =
+// sig(v) * sum over successors w at depth d+1 of cc(w) 2^(e_v - e_w)
+// (e_w >= e_v since sigma_w >= sigma_v, so only negligible terms can
+// underflow), cc(v) = (1 + delta(v)) / sig(v), bc[v] += delta(v) for v != s
+// (FP64 atomics across sources).
 //
 // The level loop is device-driven: every kernel reads its per-source ranges
 // from lstart/qlen, so the host queues ZG_Q_LEVELS_PER_SYNC levels at a time
@@ -32,21 +38,22 @@
 #include "queue_map.h"
 
 #include <algorithm>
+#include <climits>
 
 #define ZG_Q_MAX_SOURCES 1024  // sources per batch (LDS range tables)
 #define ZG_Q_GRID 1024
 #define ZG_Q_LEVELS_PER_SYNC 32
+#define ZG_Q_WIDE 32           // vertices with more predecessors are summed by the whole wavefront
+#define ZG_Q_BYTES_PER_SV 28   // workspace bytes per (source, vertex)
 
 struct BcqDev {
     uint32_t V, S, lcap;
-    const uint32_t *out_ptr, *out_col;
-    const uint8_t *live;
-    uint32_t *depth;
-    double *sig, *cc;               // cc: [2][S][V]
+    const uint32_t *out_ptr, *out_col, *in_ptr, *in_col;
+    uint2 *de;                      // [S][V] {depth, exponent}
+    double *sig, *cc;               // [S][V]
     uint32_t *queue;                // [S][V]
     uint32_t *lstart;               // [lcap][S]
     uint32_t *qlen;                 // [S]
-    int *kshift;                    // [lcap][S]
     uint32_t *newcnt;               // [lcap] entries added at each level
     const uint32_t *src;            // [S]
     const double *wts;              // [S] source weights, or null (all 1)
@@ -55,6 +62,13 @@ struct BcqDev {
 };
 
 #define QROW(s) ((size_t)(s) * a.V)
+
+// 2^k * x (ldexp saturates to 0 or infinity; exponents of path counts stay
+// far inside the int range, and only those of reached vertices are used).
+__device__ __forceinline__ double zg_scale(double x, int k)
+{
+    return ldexp(x, k);
+}
 
 __global__ void bcq_init_kernel(BcqDev a)
 {
@@ -67,10 +81,8 @@ __global__ void bcq_init_kernel(BcqDev a)
         a.lstart[a.S + s] = 0;
         return;
     }
-    a.depth[QROW(s) + v] = 0;
+    a.de[QROW(s) + v] = make_uint2(0u, 0u);
     a.sig[QROW(s) + v] = 1.0;
-    a.cc[QROW(s) + v] = 0.0;
-    a.cc[(size_t)a.S * a.V + QROW(s) + v] = 0.0;
     a.queue[QROW(s)] = v;
     a.qlen[s] = 1;
     a.lstart[s] = 0;        // level 0 starts at 0
@@ -82,6 +94,7 @@ __global__ void bcq_init_kernel(BcqDev a)
 __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
 {
     __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
+    __shared__ unsigned long long red[2][4];
     const uint32_t total = zg_q_chunks(
         a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
@@ -89,32 +102,80 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
             hi = a.lstart[(size_t)(d + 1) * a.S + s];
         },
         s_lo, s_hi, s_pre);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+    const uint32_t dprev = (uint32_t)(d - 1);
+    unsigned long long arcs = 0, reached = 0;
+    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + wv; c < total; c += waves) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
+        uint2 *de = a.de + QROW(s);
+        double *sg = a.sig + QROW(s);
         const uint32_t v = valid ? a.queue[QROW(s) + idx] : 0;
-        const double val = valid ? ldexp(a.sig[QROW(s) + v], -a.kshift[(size_t)d * a.S + s]) : 0.0;
+        if (d > 0) {
+            // path count of v from its predecessors at depth d-1
+            const uint32_t i0 = valid ? a.in_ptr[v] : 0, ideg = valid ? a.in_ptr[v + 1] - i0 : 0;
+            // narrow rows: one lane, the sum rescaled whenever a larger exponent appears
+            int E = 0;
+            double x = 0.0;
+            if (ideg <= ZG_Q_WIDE) {
+                for (uint32_t k = 0; k < ideg; k++) {
+                    const uint32_t u = a.in_col[i0 + k];
+                    const uint2 q = de[u];
+                    if (q.x != dprev) continue;
+                    const int eu = (int)q.y;
+                    if (x == 0.0) E = eu;
+                    else if (eu > E) {
+                        x = zg_scale(x, E - eu);
+                        E = eu;
+                    }
+                    x += zg_scale(sg[u], eu - E);
+                }
+            }
+            // wide rows: the whole wavefront, largest exponent first, then the sum
+            for (uint64_t wide = __ballot(ideg > ZG_Q_WIDE); wide; wide &= wide - 1) {
+                const int j = __builtin_ctzll(wide);
+                const uint32_t ij = zg_readlane32(i0, j), dj = zg_readlane32(ideg, j);
+                int emax = INT_MIN;
+                for (uint32_t k = lane; k < dj; k += 64) {
+                    const uint2 q = de[a.in_col[ij + k]];
+                    if (q.x == dprev) emax = max(emax, (int)q.y);
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) emax = max(emax, __shfl_xor(emax, off, 64));
+                double part = 0.0;
+                for (uint32_t k = lane; k < dj; k += 64) {
+                    const uint32_t u = a.in_col[ij + k];
+                    const uint2 q = de[u];
+                    if (q.x == dprev) part += zg_scale(sg[u], (int)q.y - emax);
+                }
+                part = zg_wave_sum(part);
+                if (lane == j) {
+                    x = part;
+                    E = emax;
+                }
+            }
+            if (valid) {
+                const int k = ilogb(x);  // x >= 1: v has a predecessor at depth d-1
+                sg[v] = ldexp(x, -k);
+                de[v].y = (uint32_t)(E + k);
+            }
+        }
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
+        if (valid && d > 0) {  // level 0 (the source) is counted by bcq_init_kernel
+            arcs += deg;
+            reached += 1;
+        }
         uint32_t maxdeg = deg;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor((int)maxdeg, off, 64));
-        uint32_t *dep = a.depth + QROW(s);
-        double *sg = a.sig + QROW(s);
         for (uint32_t k = 0; k < maxdeg; k++) {
             bool won = false;
             uint32_t w = 0;
             if (k < deg) {
                 w = a.out_col[e0 + k];
-                uint32_t dw = dep[w];
-                if (dw == ZG_NONE) {
-                    const uint32_t old = atomicCAS(&dep[w], ZG_NONE, (uint32_t)(d + 1));
-                    won = old == ZG_NONE;
-                    dw = won ? (uint32_t)(d + 1) : old;
-                }
-                if (dw == (uint32_t)(d + 1)) atomicAdd(&sg[w], val);
+                if (de[w].x == ZG_NONE) won = atomicCAS(&de[w].x, ZG_NONE, (uint32_t)(d + 1)) == ZG_NONE;
             }
             const uint64_t bw = __ballot(won);
             if (bw) {
@@ -125,50 +186,6 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
             }
         }
     }
-}
-
-// Close level L (just appended): clear c, shift exponent, statistics, and
-// the offset of level L+1.
-__global__ __launch_bounds__(256) void bcq_close_kernel(BcqDev a, int L)
-{
-    __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
-    __shared__ unsigned long long red[2][4];
-    const uint32_t total = zg_q_chunks(
-        a.S,
-        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
-            lo = a.lstart[(size_t)L * a.S + s];
-            hi = a.qlen[s];
-        },
-        s_lo, s_hi, s_pre);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (blockIdx.x == 0) {
-        uint32_t added = 0;
-        for (uint32_t s = tid; s < a.S; s += blockDim.x) {
-            a.lstart[(size_t)(L + 1) * a.S + s] = s_hi[s];
-            added += s_hi[s] - s_lo[s];
-        }
-        added = (uint32_t)zg_wave_sum_u64(added);
-        if (lane == 0) atomicAdd(&a.newcnt[L], added);
-    }
-    unsigned long long arcs = 0, reached = 0;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    const size_t cc1 = (size_t)a.S * a.V;
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + wv; c < total; c += waves) {
-        const uint32_t s = zg_q_owner(s_pre, a.S, c);
-        const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
-        int ex = -100000;
-        if (idx < s_hi[s]) {
-            const uint32_t w = a.queue[QROW(s) + idx];
-            a.cc[QROW(s) + w] = 0.0;
-            a.cc[cc1 + QROW(s) + w] = 0.0;
-            arcs += a.out_ptr[w + 1] - a.out_ptr[w];
-            reached += 1;
-            ex = ilogb(a.sig[QROW(s) + w]);
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) ex = max(ex, __shfl_xor(ex, off, 64));
-        if (lane == 0 && ex > ZG_SHIFT_T) atomicMax(&a.kshift[(size_t)L * a.S + s], ex);
-    }
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
     if (lane == 0) {
@@ -176,15 +193,26 @@ __global__ __launch_bounds__(256) void bcq_close_kernel(BcqDev a, int L)
         red[1][wv] = reached;
     }
     __syncthreads();
-    if (tid == 0) {
-        unsigned long long A = 0, R = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
-            A += red[0][w];
-            R += red[1][w];
-        }
+    if (threadIdx.x == 0) {
+        const unsigned long long A = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const unsigned long long R = red[1][0] + red[1][1] + red[1][2] + red[1][3];
         if (A) atomicAdd(&a.stats[0], A);
         if (R) atomicAdd(&a.stats[1], R);
     }
+}
+
+// Close level L (just appended): the offset of level L+1 and the level's
+// size.  One block.
+__global__ __launch_bounds__(1024) void bcq_close_kernel(BcqDev a, int L)
+{
+    uint32_t added = 0;
+    for (uint32_t s = threadIdx.x; s < a.S; s += blockDim.x) {
+        const uint32_t q = a.qlen[s];
+        a.lstart[(size_t)(L + 1) * a.S + s] = q;
+        added += q - a.lstart[(size_t)L * a.S + s];
+    }
+    added = (uint32_t)zg_wave_sum_u64(added);
+    if ((threadIdx.x & 63) == 0 && added) atomicAdd(&a.newcnt[L], added);
 }
 
 __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
@@ -199,18 +227,24 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    const double *cin = a.cc + (size_t)((d + 1) & 1) * a.S * a.V;
-    double *cout = a.cc + (size_t)(d & 1) * a.S * a.V;
+    const uint32_t dnext = (uint32_t)(d + 1);
     for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         if (idx >= s_hi[s]) continue;
+        const uint2 *de = a.de + QROW(s);
+        const double *cc = a.cc + QROW(s);
         const uint32_t v = a.queue[QROW(s) + idx];
+        const int ev = (int)de[v].y;
         double S = 0.0;
-        for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e++) S += cin[QROW(s) + a.out_col[e]];
+        for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e++) {
+            const uint32_t w = a.out_col[e];
+            const uint2 q = de[w];
+            if (q.x == dnext) S += zg_scale(cc[w], ev - (int)q.y);
+        }
         const double sv = a.sig[QROW(s) + v];
-        const double delta = sv * ldexp(S, -a.kshift[(size_t)d * a.S + s]);
-        cout[QROW(s) + v] = (1.0 + delta) / sv;
+        const double delta = sv * S;
+        a.cc[QROW(s) + v] = (1.0 + delta) / sv;
         if (v != a.src[s]) atomicAdd(&a.bc[v], a.wts ? delta * a.wts[s] : delta);
     }
 }
@@ -224,8 +258,8 @@ __global__ void bcq_out_kernel(uint32_t V, const double *bc, const uint8_t *live
 int zg_bcq_ws_release(zg_graph *g)
 {
     zg_bcq_ws &w = g->bcq;
-    zg_dfree(w.depth); zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.queue); zg_dfree(w.lstart);
-    zg_dfree(w.qlen); zg_dfree(w.kshift); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.wts); zg_dfree(w.bc);
+    zg_dfree(w.de); zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.queue); zg_dfree(w.lstart);
+    zg_dfree(w.qlen); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.wts); zg_dfree(w.bc);
     zg_dfree(w.stats);
     w.S = 0;
     w.lcap = 0;
@@ -240,13 +274,12 @@ static int bcq_ws_ensure(zg_graph *g, uint32_t S)
     const size_t V = g->V;
     w.lcap = 1024;
     int rc = ZG_OK;
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.depth, S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.de, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, S * V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, 2 * S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.queue, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.lstart, (size_t)w.lcap * S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.qlen, S);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.kshift, (size_t)w.lcap * S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.newcnt, w.lcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.src, S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.wts, S);
@@ -269,28 +302,20 @@ static int bcq_grow_levels(zg_graph *g, uint32_t need, uint32_t used, hipStream_
     uint32_t nl = w.lcap;
     while (nl <= need) nl *= 2;
     uint32_t *ls = nullptr, *nc = nullptr;
-    int *ks = nullptr;
     ZG_TRY_RC(zg_dmalloc(&ls, (size_t)nl * w.S));
-    int rc = zg_dmalloc(&ks, (size_t)nl * w.S);
-    if (rc == ZG_OK) rc = zg_dmalloc(&nc, nl);
+    int rc = zg_dmalloc(&nc, nl);
     if (rc != ZG_OK) {
         zg_dfree(ls);
-        zg_dfree(ks);
         return rc;
     }
-    ZG_TRY(hipMemsetAsync(ks, 0, sizeof(int) * (size_t)nl * w.S, s));
     ZG_TRY(hipMemsetAsync(nc, 0, sizeof(uint32_t) * nl, s));
     ZG_TRY(hipMemcpyAsync(ls, w.lstart, sizeof(uint32_t) * (size_t)std::min(used, w.lcap) * w.S,
                           hipMemcpyDeviceToDevice, s));
-    ZG_TRY(hipMemcpyAsync(ks, w.kshift, sizeof(int) * (size_t)std::min(used, w.lcap) * w.S, hipMemcpyDeviceToDevice,
-                          s));
     ZG_TRY(hipMemcpyAsync(nc, w.newcnt, sizeof(uint32_t) * std::min(used, w.lcap), hipMemcpyDeviceToDevice, s));
     ZG_TRY(hipStreamSynchronize(s));
     zg_dfree(w.lstart);
-    zg_dfree(w.kshift);
     zg_dfree(w.newcnt);
     w.lstart = ls;
-    w.kshift = ks;
     w.newcnt = nc;
     w.lcap = nl;
     return ZG_OK;
@@ -305,14 +330,14 @@ static BcqDev bcq_dev(zg_graph *g, bool weighted)
     a.lcap = w.lcap;
     a.out_ptr = g->out_ptr;
     a.out_col = g->out_col;
-    a.live = g->live;
-    a.depth = w.depth;
+    a.in_ptr = g->in_ptr;
+    a.in_col = g->in_col;
+    a.de = w.de;
     a.sig = w.sig;
     a.cc = w.cc;
     a.queue = w.queue;
     a.lstart = w.lstart;
     a.qlen = w.qlen;
-    a.kshift = w.kshift;
     a.newcnt = w.newcnt;
     a.src = w.src;
     a.wts = weighted ? w.wts : nullptr;
@@ -327,15 +352,15 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
     const uint32_t V = g->V;
     // sources per batch: as many as the call has (rounded up to 64), at most
     // ZG_Q_MAX_SOURCES, and as many as half the free device memory holds at
-    // 32 bytes of state per source and vertex; an existing larger workspace
-    // is kept
+    // ZG_Q_BYTES_PER_SV bytes of state per source and vertex; an existing
+    // larger workspace is kept
     const uint32_t want = (uint32_t)std::min<uint64_t>(ZG_Q_MAX_SOURCES, (n + 63) / 64 * 64);
     uint32_t S = g->bcq.S;
     if (S < want) {
         size_t fr = 0, tot = 0;
         ZG_TRY(hipMemGetInfo(&fr, &tot));
-        fr += (size_t)g->bcq.S * V * 32;  // released before the new one is made
-        const uint64_t fit = (uint64_t)(0.5 * (double)fr) / (32ull * (V ? V : 1));
+        fr += (size_t)g->bcq.S * V * ZG_Q_BYTES_PER_SV;  // released before the new one is made
+        const uint64_t fit = (uint64_t)(0.5 * (double)fr) / ((uint64_t)ZG_Q_BYTES_PER_SV * (V ? V : 1));
         S = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(want, fit) / 64 * 64);
     }
     ZG_TRY_RC(bcq_ws_ensure(g, S));
@@ -348,9 +373,7 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
         if (d_weights)
             ZG_TRY(hipMemcpyAsync(w.wts, d_weights + b0, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
-        ZG_TRY(hipMemsetAsync(w.depth, 0xFF, sizeof(uint32_t) * (size_t)S * V, s));
-        ZG_TRY(hipMemsetAsync(w.sig, 0, sizeof(double) * (size_t)S * V, s));
-        ZG_TRY(hipMemsetAsync(w.kshift, 0, sizeof(int) * (size_t)w.lcap * S, s));
+        ZG_TRY(hipMemsetAsync(w.de, 0xFF, sizeof(uint2) * (size_t)S * V, s));  // depth unset
         ZG_TRY(hipMemsetAsync(w.newcnt, 0, sizeof(uint32_t) * w.lcap, s));
         ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 2, s));
         BcqDev a = bcq_dev(g, d_weights != nullptr);
@@ -364,8 +387,7 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
             a = bcq_dev(g, d_weights != nullptr);
             for (uint32_t k = 0; k < ZG_Q_LEVELS_PER_SYNC; k++) {
                 ZG_PROF_LAUNCH(ZG_PROF_BC_PUSH, bcq_forward_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a, (int)(d + k));
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bcq_close_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a,
-                               (int)(d + k + 1));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bcq_close_kernel, dim3(1), dim3(1024), 0, s, a, (int)(d + k + 1));
             }
             ZG_TRY(hipGetLastError());
             newcnt.resize(ZG_Q_LEVELS_PER_SYNC);

@@ -130,13 +130,12 @@ struct zg_bc_ws {
 // Per-source queue engine workspace (brandes_queue.hip), S sources per batch.
 struct zg_bcq_ws {
     uint32_t S = 0, lcap = 0;
-    uint32_t *depth = nullptr;           // [S][V]
-    double *sig = nullptr;               // [S][V]
-    double *cc = nullptr;                // [2][S][V]
+    uint2 *de = nullptr;                 // [S][V] {depth, path-count exponent}
+    double *sig = nullptr;               // [S][V] path-count mantissa
+    double *cc = nullptr;                // [S][V]
     uint32_t *queue = nullptr;           // [S][V]
     uint32_t *lstart = nullptr;          // [lcap][S]
     uint32_t *qlen = nullptr;            // [S]
-    int *kshift = nullptr;               // [lcap][S]
     uint32_t *newcnt = nullptr;          // [lcap]
     uint32_t *src = nullptr;             // [S]
     double *wts = nullptr;               // [S] per-source weights of a weighted sweep
@@ -237,6 +236,7 @@ struct zg_graph {
     double *out_w = nullptr;             // [E] arc weights in successor-CSR order (weighted resources)
     bool weights_ok = true;              // every live arc's weight is finite and non-negative
     bool built_on_gpu = false;           // the CSR came from zg_build_csr_gpu
+    uint32_t max_in_deg = 0;             // largest in-degree
     uint32_t wbc_batch = 0;              // weighted sweep: sources per batch (0 = default)
     double wbc_delta = 0.0;              // weighted sweep: bound step of the near frontier (0 = default)
     uint32_t wbc_grid = 0;               // weighted sweep: workgroups per round kernel (0 = default)

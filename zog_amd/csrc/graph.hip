@@ -232,6 +232,7 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     for (uint32_t v = 0; v < V; v++) {
         g->h_in_deg[v] = in_ptr[(size_t)v + 1] - in_ptr[v];
         g->h_out_deg[v] = out_ptr[(size_t)v + 1] - out_ptr[v];
+        g->max_in_deg = std::max(g->max_in_deg, g->h_in_deg[v]);
         if (g->h_live[v] && g->h_out_deg[v] == 0) n_dang++;
     }
     g->n_dangling = n_dang;
