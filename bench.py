@@ -376,6 +376,12 @@ def bench_power(args):
                                   "kernel": "pr_iter_kernel+pr_finish_kernel" +
                                             ("+eig_norm_kernel+eig_decide_kernel" if name == "eigenvector" else ""),
                                   "avg_iteration_us": round(it_ms * 1000.0 / max(it_n, 1), 2)}}
+    floor_us = ctypes.c_double()
+    _lib.check(_lib.lib().zg_gather_floor(g.resource.handle, 64, 20, ctypes.byref(floor_us)), "zg_gather_floor")
+    for name in res:
+        it_us = res[name]["roofline"]["avg_iteration_us"]
+        res[name]["roofline"]["gather_floor_us"] = round(floor_us.value, 2)
+        res[name]["roofline"]["frac_of_gather_floor"] = round(floor_us.value / it_us, 3) if it_us > 0 else None
     cpu = None
     parity = {}
     if not args.no_cpu:
@@ -622,6 +628,10 @@ def main():
     it_ms, it_n = pprof["pr_iter"]
     pr_ach = pr_bytes_iter * it_n / (it_ms / 1000.0) / 1e9 if it_ms > 0 else 0.0
     pr_result = d_pr.cpu().numpy()
+    # the floor of a pull over these columns: one plain gather pass (FP32)
+    floor_us = ctypes.c_double()
+    _lib.check(L.zg_gather_floor(h, 32, 20, ctypes.byref(floor_us)), "zg_gather_floor")
+    it_us = it_ms * 1000.0 / max(it_n, 1)
     log(f"[bench] pagerank {pr_iters} iterations ({int(pinfo.fp64_iterations)} FP64), {pr_gbs:.1f} GB/s wall, "
         f"kernel {pr_ach:.1f} GB/s, residual {pinfo.residual_l1:.3e}")
 
@@ -679,7 +689,9 @@ def main():
                      "roofline": {"bound": "hbm", "achieved": round(pr_ach, 1), "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": round(pr_ach / HBM_PEAK_GBS, 4), "traffic": None,
                                   "kernel": "pr_iter_kernel+pr_finish_kernel",
-                                  "avg_launch_us": round(it_ms * 1000.0 / max(it_n, 1), 2)}},
+                                  "avg_launch_us": round(it_us, 2),
+                                  "gather_floor_us": round(floor_us.value, 2),
+                                  "frac_of_gather_floor": round(floor_us.value / it_us, 3) if it_us > 0 else None}},
         "parity": parity,
     }
     print(json.dumps(line), flush=True)

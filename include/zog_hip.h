@@ -241,6 +241,12 @@ int zg_katz(zg_graph *g, double alpha, double beta, uint64_t max_iterations, dou
 int zg_alpha_centrality(zg_graph *g, double alpha, double initial, uint64_t max_iterations, double tolerance,
                         double *out, zg_pr_info *info);
 
+/* Measurement only (no reference counterpart): the device time of one
+ * plain gather pass sum_e x[col[e]] over the resident predecessor columns
+ * (one thread per arc; precision 32 or 64; averaged over `reps` passes after
+ * a warm-up), the floor for any pull iteration over these columns. */
+int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_per_pass);
+
 /* Per-source BFS export for parity checks (no reference counterpart; the
  * reference keeps these workspaces private, centrality.zig:697-703): for each
  * of n_sources sources, depth_out[i*n_cap + v] (0xFFFFFFFF = unreached) and

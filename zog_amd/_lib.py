@@ -63,6 +63,7 @@ EXPORTED = (
     "zg_eigenvector",
     "zg_katz",
     "zg_alpha_centrality",
+    "zg_gather_floor",
     "zg_bc_paths",
     "zg_set_batch_groups",
     "zg_set_source_order",
@@ -209,6 +210,8 @@ def lib():
     L.zg_alpha_centrality.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P,
                                       ctypes.POINTER(ZgPrInfo)]
     L.zg_alpha_centrality.restype = ctypes.c_int
+    L.zg_gather_floor.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.zg_gather_floor.restype = ctypes.c_int
     L.zg_bc_paths.argtypes = [P, P, ctypes.c_uint64, P, P, P, ctypes.c_uint32, P]
     L.zg_bc_paths.restype = ctypes.c_int
     L.zg_set_batch_groups.argtypes = [P, ctypes.c_int]

@@ -991,3 +991,55 @@ extern "C" int zg_alpha_centrality(zg_graph *g, double alpha, double initial, ui
 {
     return zg_power_host(g, ZG_PW_AFFINE, alpha, 0.0, initial, max_iterations, tolerance, out, info);
 }
+
+// ---------------------------------------------------------------------------
+// Gather floor: the time of one pass sum_e x[pr_col[e]] over the resident
+// predecessor columns (PageRank ids), one thread per arc, no row structure.
+// It bounds what any pull iteration over these columns can reach.
+
+template <typename T>
+__global__ __launch_bounds__(256) void pw_gather_kernel(const uint32_t *__restrict__ col, const T *__restrict__ x,
+                                                        uint32_t E, T *sink)
+{
+    T s = 0;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) s += x[col[e]];
+    if (s == (T)-1) sink[0] = s;  // never true (x is zero): keeps the loads
+}
+
+extern "C" int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_per_pass)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if ((precision != 32 && precision != 64) || reps < 1 || !us_per_pass) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    const uint32_t V = g->V, E = (uint32_t)g->E;
+    void *x = nullptr;
+    ZG_TRY(hipMalloc(&x, (size_t)(precision / 8) * (V ? V : 1) + 64));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = zg_hip_code(hipMemsetAsync(x, 0, (size_t)(precision / 8) * (V ? V : 1) + 64, s));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
+    const dim3 grid(8192), blk(256);
+    for (int pass = 0; pass < 2 && rc == ZG_OK; pass++) {  // the first pass warms the caches
+        if (pass) rc = zg_hip_code(hipEventRecord(e0, s));
+        for (int i = 0; i < reps && rc == ZG_OK && E; i++) {
+            if (precision == 32)
+                hipLaunchKernelGGL(pw_gather_kernel<float>, grid, blk, 0, s, g->pr_col, (const float *)x, E,
+                                   (float *)x + (V ? V : 1));
+            else
+                hipLaunchKernelGGL(pw_gather_kernel<double>, grid, blk, 0, s, g->pr_col, (const double *)x, E,
+                                   (double *)x + (V ? V : 1));
+        }
+        if (rc == ZG_OK) rc = zg_hip_code(hipGetLastError());
+        if (pass && rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e1, s));
+    }
+    float ms = 0.f;
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventSynchronize(e1));
+    if (rc == ZG_OK) rc = zg_hip_code(hipEventElapsedTime(&ms, e0, e1));
+    if (rc == ZG_OK) *us_per_pass = 1000.0 * ms / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(x);
+    return rc;
+}
