@@ -120,17 +120,28 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
             int E = 0;
             double x = 0.0;
             if (ideg <= ZG_Q_WIDE) {
-                for (uint32_t k = 0; k < ideg; k++) {
-                    const uint32_t u = a.in_col[i0 + k];
-                    const uint2 q = de[u];
-                    if (q.x != dprev) continue;
-                    const int eu = (int)q.y;
-                    if (x == 0.0) E = eu;
-                    else if (eu > E) {
-                        x = zg_scale(x, E - eu);
-                        E = eu;
+                // four arcs at a time: their loads overlap instead of chaining
+                for (uint32_t k0 = 0; k0 < ideg; k0 += 4) {
+                    uint32_t u[4];
+                    uint2 q[4];
+                    double sv[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) u[t] = k0 + t < ideg ? a.in_col[i0 + k0 + t] : 0u;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) q[t] = k0 + t < ideg ? de[u[t]] : make_uint2(ZG_NONE, 0u);
+#pragma unroll
+                    for (int t = 0; t < 4; t++) sv[t] = q[t].x == dprev ? sg[u[t]] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (q[t].x != dprev) continue;
+                        const int eu = (int)q[t].y;
+                        if (x == 0.0) E = eu;
+                        else if (eu > E) {
+                            x = zg_scale(x, E - eu);
+                            E = eu;
+                        }
+                        x += zg_scale(sv[t], eu - E);
                     }
-                    x += zg_scale(sg[u], eu - E);
                 }
             }
             // wide rows: the whole wavefront, largest exponent first, then the sum
@@ -170,19 +181,25 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
         uint32_t maxdeg = deg;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor((int)maxdeg, off, 64));
-        for (uint32_t k = 0; k < maxdeg; k++) {
-            bool won = false;
-            uint32_t w = 0;
-            if (k < deg) {
-                w = a.out_col[e0 + k];
-                if (de[w].x == ZG_NONE) won = atomicCAS(&de[w].x, ZG_NONE, (uint32_t)(d + 1)) == ZG_NONE;
-            }
-            const uint64_t bw = __ballot(won);
-            if (bw) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&a.qlen[s], (uint32_t)__popcll(bw));
-                base = zg_readlane32(base, 0);
-                if (won) a.queue[QROW(s) + base + __popcll(bw & ((1ull << lane) - 1))] = w;
+        for (uint32_t k0 = 0; k0 < maxdeg; k0 += 4) {
+            // the heads and their depths of four arcs first, then the claims
+            uint32_t w[4], dw[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) w[t] = k0 + t < deg ? a.out_col[e0 + k0 + t] : 0u;
+#pragma unroll
+            for (int t = 0; t < 4; t++) dw[t] = k0 + t < deg ? de[w[t]].x : 0u;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                bool won = false;
+                if (k0 + t < deg && dw[t] == ZG_NONE)
+                    won = atomicCAS(&de[w[t]].x, ZG_NONE, (uint32_t)(d + 1)) == ZG_NONE;
+                const uint64_t bw = __ballot(won);
+                if (bw) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&a.qlen[s], (uint32_t)__popcll(bw));
+                    base = zg_readlane32(base, 0);
+                    if (won) a.queue[QROW(s) + base + __popcll(bw & ((1ull << lane) - 1))] = w[t];
+                }
             }
         }
     }
@@ -237,10 +254,20 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
         const uint32_t v = a.queue[QROW(s) + idx];
         const int ev = (int)de[v].y;
         double S = 0.0;
-        for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e++) {
-            const uint32_t w = a.out_col[e];
-            const uint2 q = de[w];
-            if (q.x == dnext) S += zg_scale(cc[w], ev - (int)q.y);
+        for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e += 4) {
+            // four arcs at a time: heads, then depths and exponents, then terms
+            uint32_t w[4];
+            uint2 q[4];
+            double cw[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) w[t] = e + t < e1 ? a.out_col[e + t] : 0u;
+#pragma unroll
+            for (int t = 0; t < 4; t++) q[t] = e + t < e1 ? de[w[t]] : make_uint2(ZG_NONE, 0u);
+#pragma unroll
+            for (int t = 0; t < 4; t++) cw[t] = q[t].x == dnext ? cc[w[t]] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (q[t].x == dnext) S += zg_scale(cw[t], ev - (int)q[t].y);
         }
         const double sv = a.sig[QROW(s) + v];
         const double delta = sv * S;
