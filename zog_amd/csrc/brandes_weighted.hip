@@ -321,6 +321,27 @@ struct WbcRelax {
         }
     }
 
+    // As arc(), with the head, the weight and a read of the head's
+    // distance already loaded (a stale read is never below the current one)
+    __device__ __forceinline__ void arc_loaded(double d, uint32_t w, double wt, unsigned long long dw, bool &first,
+                                               bool &queue, bool &fnew)
+    {
+        const unsigned long long nb = zg_f64_to_bits(d + wt);
+        if (nb < dw) {
+            const unsigned long long old = atomicMin(&dist[w], nb);
+            if (nb < old) {
+                first = old == ZG_W_UNSET;
+                if (nb <= Tb) {
+                    queue = atomicExch(&stamp[w], r1) != r1;
+                    farflag[w] = 2;
+                } else {
+                    fnew = atomicCAS(&farflag[w], 0u, 1u) == 0u;
+                    fm = nb < fm ? nb : fm;
+                }
+            }
+        }
+    }
+
     // The appends of one step; every lane of the wavefront calls it.
     __device__ __forceinline__ void append(bool first, bool queue, bool fnew, uint32_t w, int lane)
     {
@@ -398,11 +419,24 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
         uint32_t maxdeg = mine;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor((int)maxdeg, off, 64));
-        for (uint32_t k = 0; k < maxdeg; k++) {
-            bool first = false, queue = false, fnew = false;
-            uint32_t w = 0;
-            if (k < mine) R.arc(a, dv, e0 + k, first, queue, fnew, w);
-            R.append(first, queue, fnew, w, lane);
+        for (uint32_t k0 = 0; k0 < maxdeg; k0 += 4) {
+            // heads, weights and distances of four arcs first, so the loads overlap
+            uint32_t w[4];
+            double wt[4];
+            unsigned long long dw[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                w[t] = k0 + t < mine ? a.out_col[e0 + k0 + t] : 0u;
+                wt[t] = k0 + t < mine ? a.out_w[e0 + k0 + t] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) dw[t] = k0 + t < mine ? R.dist[w[t]] : 0ull;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                bool first = false, queue = false, fnew = false;
+                if (k0 + t < mine) R.arc_loaded(dv, w[t], wt[t], dw[t], first, queue, fnew);
+                R.append(first, queue, fnew, w[t], lane);
+            }
         }
         R.flush(a, s, lane);
     }
@@ -472,11 +506,24 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
             }
         }
         if (deg <= ZG_W_WIDE)
-            for (uint32_t k = 0; k < deg; k++) {
-                uint32_t w;
-                if (wbc_tight(a, dist, v, src, dv, e0 + k, w)) {
-                    atomicAdd(&indeg[w], 1u);
-                    atomicOr(&tb[(e0 + k) >> 5], 1u << ((e0 + k) & 31));
+            for (uint32_t k0 = 0; k0 < deg; k0 += 4) {
+                uint32_t w[4];
+                double wt[4];
+                unsigned long long dw[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    w[t] = k0 + t < deg ? a.out_col[e0 + k0 + t] : 0u;
+                    wt[t] = k0 + t < deg ? a.out_w[e0 + k0 + t] : 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) dw[t] = k0 + t < deg ? dist[w[t]] : ZG_W_UNSET;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const uint32_t e = e0 + k0 + t;
+                    if (k0 + t < deg && w[t] != v && w[t] != src && zg_f64_to_bits(dv + wt[t]) == dw[t]) {
+                        atomicAdd(&indeg[w[t]], 1u);
+                        atomicOr(&tb[e >> 5], 1u << (e & 31));
+                    }
                 }
             }
     }
