@@ -44,6 +44,11 @@
 #define ZG_W_PIECE 2048
 #define ZG_W_HUB_CAP (1u << 20)  // hub pieces per relax round
 #define ZG_W_UNSET 0xFFFFFFFFFFFFFFFFull
+#define ZG_W_WINDOW 8  // the window pile spans this many bound steps
+
+// far pile p, buffer b, source s
+#define FARROW(p, b, s) (((size_t)((p) * 2 + (b)) * a.S + (s)) * a.V)
+#define FARCNT(p, b, s) (((size_t)((p) * 2 + (b)) * a.S + (s)))
 
 struct WbcDev {
     uint32_t V, S, lcap;
@@ -67,14 +72,16 @@ struct WbcDev {
     double *bc;                 // [V]
     unsigned long long *stats;  // [3] arcs, reached, arcs relaxed
     // near-far split of the relax frontier: a vertex whose new distance is
-    // above its source's bound waits in the far pile until the bound passes it
-    uint32_t *far;              // [2][S][V] far piles (fsel[s] is the live one)
-    uint32_t *farcnt;           // [2][S]
-    uint32_t *farflag;          // [S][V] 0 never far, 1 in the far pile, 2 within the bound
-    uint32_t *fsel;             // [S]
-    uint32_t *split;            // [S] this round moves far entries under the raised bound
+    // above its source's bound waits in a far pile until the bound passes it;
+    // pile 0 (the window) holds distances below wlim, pile 1 the rest
+    uint32_t *far;              // [2 piles][2 buffers][S][V] (fsel[p][s] is the live buffer)
+    uint32_t *farcnt;           // [2][2][S]
+    uint32_t *farflag;          // [S][V] 0 never far, 1 in the window, 3 in pile 1, 2 within the bound
+    uint32_t *fsel;             // [2][S]
+    uint32_t *split;            // [2][S] this round splits the window / scans pile 1
     unsigned long long *thr;    // [S] the bound (distance bits)
-    unsigned long long *fmin;   // [S] least distance recorded for a far entry (bits)
+    unsigned long long *wlim;   // [S] the window's end (distance bits)
+    unsigned long long *fmin;   // [2][S] least distance recorded for a pile's entry (bits)
     double delta;               // bound step
     uint32_t *hcnt;             // [1] hub pieces listed this round
     uint32_t *hub_v, *hub_a;    // [hcap] hub vertex; source << 22 | piece
@@ -104,12 +111,15 @@ __global__ void wbc_init_kernel(WbcDev a)
     if (s >= a.S) return;
     a.fcnt[a.S + s] = 0;
     a.lstart[s] = 0;
-    a.farcnt[s] = 0;
-    a.farcnt[a.S + s] = 0;
-    a.fsel[s] = 0;
-    a.split[s] = 0;
+    for (int p = 0; p < 2; p++) {
+        a.farcnt[FARCNT(p, 0, s)] = 0;
+        a.farcnt[FARCNT(p, 1, s)] = 0;
+        a.fsel[p * a.S + s] = 0;
+        a.split[p * a.S + s] = 0;
+        a.fmin[p * a.S + s] = ZG_W_UNSET;
+    }
     a.thr[s] = zg_f64_to_bits(a.delta);
-    a.fmin[s] = ZG_W_UNSET;
+    a.wlim[s] = zg_f64_to_bits(a.delta * (1 + ZG_W_WINDOW));
     const uint32_t v = a.src[s];
     if (v == ZG_NONE) {
         a.fcnt[s] = 0;
@@ -131,11 +141,14 @@ __global__ void wbc_init_kernel(WbcDev a)
     a.lstart[a.S + s] = 1;
 }
 
-// Before relax round r, per source: the work left (frontier and far pile,
+// Before relax round r, per source: the work left (frontier and far piles,
 // summed into tot[r]), an empty buffer for the next frontier, and, for a
-// source whose frontier is empty while its far pile is not, a raised bound
-// max(bound + delta, least far distance) and a fresh far pile for the split
-// kernel to move the entries under it into.  One block.
+// source whose frontier is empty while a far pile is not, a raised bound
+// max(bound + delta, least window distance) -- or, when the window is empty,
+// max(bound + delta, least pile-1 distance) -- with a fresh window buffer for
+// wbc_split_kernel; pile 1 is scanned too (and the window moved on by
+// ZG_W_WINDOW steps past the bound) when the window is empty or the bound
+// passes its end.  One block.
 __global__ void wbc_round_kernel(WbcDev a, int r)
 {
     __shared__ unsigned long long red[16];
@@ -143,21 +156,32 @@ __global__ void wbc_round_kernel(WbcDev a, int r)
     unsigned long long t = 0;
     if (threadIdx.x == 0) *a.hcnt = 0;
     for (uint32_t s = threadIdx.x; s < a.S; s += blockDim.x) {
-        const uint32_t nf = a.fcnt[(size_t)cur * a.S + s], fs = a.fsel[s];
-        const uint32_t nfar = a.farcnt[(size_t)fs * a.S + s];
-        t += (unsigned long long)nf + nfar;
+        const uint32_t nf = a.fcnt[(size_t)cur * a.S + s];
+        const uint32_t fw = a.fsel[s], fo = a.fsel[a.S + s];
+        const uint32_t nw = a.farcnt[FARCNT(0, fw, s)], no = a.farcnt[FARCNT(1, fo, s)];
+        t += (unsigned long long)nf + nw + no;
         a.fcnt[(size_t)nxt * a.S + s] = 0;
-        const bool sp = nf == 0 && nfar > 0;
-        a.split[s] = sp;
-        if (sp) {
+        bool sw = false, so = false;
+        if (nf == 0 && (nw > 0 || no > 0)) {
             double T = zg_bits_to_f64(a.thr[s]) + a.delta;
-            const unsigned long long fm = a.fmin[s];
-            if (fm != ZG_W_UNSET) T = fmax(T, zg_bits_to_f64(fm));
+            const unsigned long long fmw = a.fmin[s], fmo = a.fmin[a.S + s];
+            if (nw > 0 && fmw != ZG_W_UNSET) T = fmax(T, zg_bits_to_f64(fmw));
+            if (nw == 0 && fmo != ZG_W_UNSET) T = fmax(T, zg_bits_to_f64(fmo));
             a.thr[s] = zg_f64_to_bits(T);
+            sw = nw > 0;
+            so = no > 0 && (nw == 0 || zg_f64_to_bits(T) >= a.wlim[s]);
+            if (so) {
+                a.wlim[s] = zg_f64_to_bits(T + a.delta * ZG_W_WINDOW);
+                a.fmin[a.S + s] = ZG_W_UNSET;
+                a.fsel[a.S + s] = fo ^ 1;
+                a.farcnt[FARCNT(1, fo ^ 1, s)] = 0;
+            }
             a.fmin[s] = ZG_W_UNSET;
-            a.fsel[s] = fs ^ 1;
-            a.farcnt[(size_t)(fs ^ 1) * a.S + s] = 0;
+            a.fsel[s] = fw ^ 1;
+            a.farcnt[FARCNT(0, fw ^ 1, s)] = 0;
         }
+        a.split[s] = sw;
+        a.split[a.S + s] = so;
     }
     t = zg_wave_sum_u64(t);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
@@ -180,11 +204,14 @@ __device__ __forceinline__ unsigned long long zg_wave_min_u64(unsigned long long
     return x;
 }
 
-// Splits the far piles of the sources flagged by wbc_round_kernel: entries
-// still in the pile and within the raised bound join round r's frontier,
-// the others stay (and give the least far distance); entries whose vertex
-// has since come within the bound are dropped.
-__global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r)
+// Splits far pile p of the sources flagged by wbc_round_kernel.  Window
+// (p = 0): entries still in it and within the raised bound join round r's
+// frontier, the others stay.  Pile 1: entries within the bound join the
+// frontier, those below the window's new end move to the window, the others
+// stay.  Entries whose vertex has left the pile (come within the bound, or
+// moved to the window) are dropped; each pile's least remaining distance is
+// recorded.
+__global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r, int p)
 {
     __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
     const int cur = r & 1;
@@ -192,29 +219,37 @@ __global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r)
         a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
             lo = 0;
-            hi = a.split[s] ? a.farcnt[(size_t)(a.fsel[s] ^ 1) * a.S + s] : 0;
+            hi = a.split[(size_t)p * a.S + s] ? a.farcnt[FARCNT(p, a.fsel[(size_t)p * a.S + s] ^ 1, s)] : 0;
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t mine = p == 0 ? 1u : 3u;
     for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
-        const uint32_t fs = a.fsel[s];
-        const unsigned long long Tb = a.thr[s];
-        const uint32_t v = valid ? a.far[((size_t)(fs ^ 1) * a.S + s) * a.V + idx] : 0;
-        bool move = false, keep = false;
+        const uint32_t fs = a.fsel[(size_t)p * a.S + s], fw = a.fsel[s];
+        const unsigned long long Tb = a.thr[s], WL = a.wlim[s];
+        uint32_t *flag = a.farflag + WROW(s);
+        const uint32_t v = valid ? a.far[FARROW(p, fs ^ 1, s) + idx] : 0;
+        bool move = false, to_window = false, keep = false;
         unsigned long long d = ZG_W_UNSET;
-        if (valid && a.farflag[WROW(s) + v] == 1) {
+        if (valid && flag[v] == mine) {
             d = a.dist[WROW(s) + v];
-            if (d <= Tb) move = atomicExch(&a.farflag[WROW(s) + v], 2u) == 1u;
+            if (d <= Tb) move = atomicExch(&flag[v], 2u) == mine;
+            else if (p == 1 && d < WL) to_window = atomicCAS(&flag[v], 3u, 1u) == 3u;
             else keep = true;
         }
         wbc_append(move, v, &a.fcnt[(size_t)cur * a.S + s], a.fr + ((size_t)cur * a.S + s) * a.V, lane);
-        wbc_append(keep, v, &a.farcnt[(size_t)fs * a.S + s], a.far + ((size_t)fs * a.S + s) * a.V, lane);
-        const unsigned long long m = zg_wave_min_u64(keep ? d : ZG_W_UNSET);
-        if (lane == 0 && m != ZG_W_UNSET) atomicMin(&a.fmin[s], m);
+        wbc_append(keep, v, &a.farcnt[FARCNT(p, fs, s)], a.far + FARROW(p, fs, s), lane);
+        if (p == 1) wbc_append(to_window, v, &a.farcnt[FARCNT(0, fw, s)], a.far + FARROW(0, fw, s), lane);
+        const unsigned long long mk = zg_wave_min_u64(keep ? d : ZG_W_UNSET);
+        if (lane == 0 && mk != ZG_W_UNSET) atomicMin(&a.fmin[(size_t)p * a.S + s], mk);
+        if (p == 1) {
+            const unsigned long long mw = zg_wave_min_u64(to_window ? d : ZG_W_UNSET);
+            if (lane == 0 && mw != ZG_W_UNSET) atomicMin(&a.fmin[s], mw);
+        }
     }
 }
 
@@ -276,10 +311,10 @@ __device__ __forceinline__ WbcTightIt wbc_word_share(uint32_t ej, uint32_t dj, u
 // One source's view of relax round r.
 struct WbcRelax {
     unsigned long long *dist;
-    uint32_t *stamp, *farflag, *ncnt, *nfr, *rcnt, *reach, *fcnt, *far;
-    unsigned long long Tb;
+    uint32_t *stamp, *farflag, *ncnt, *nfr, *rcnt, *reach, *wcnt, *win, *ocnt, *ovf;
+    unsigned long long Tb, WL;
     uint32_t r1;
-    unsigned long long fm;  // least far distance this lane recorded
+    unsigned long long fmw, fmo;  // least window / pile-1 distance this lane recorded
 
     __device__ __forceinline__ WbcRelax(const WbcDev &a, uint32_t s, int r)
     {
@@ -291,18 +326,21 @@ struct WbcRelax {
         nfr = a.fr + ((size_t)nxt * a.S + s) * a.V;
         rcnt = &a.rcnt[s];
         reach = a.reach + WROW(s);
-        const uint32_t fs = a.fsel[s];
-        fcnt = &a.farcnt[(size_t)fs * a.S + s];
-        far = a.far + ((size_t)fs * a.S + s) * a.V;
+        const uint32_t fw = a.fsel[s], fo = a.fsel[a.S + s];
+        wcnt = &a.farcnt[FARCNT(0, fw, s)];
+        win = a.far + FARROW(0, fw, s);
+        ocnt = &a.farcnt[FARCNT(1, fo, s)];
+        ovf = a.far + FARROW(1, fo, s);
         Tb = a.thr[s];
+        WL = a.wlim[s];
         r1 = (uint32_t)(r + 1);
-        fm = ZG_W_UNSET;
+        fmw = fmo = ZG_W_UNSET;
     }
 
     // lowers d(w) to fl(d + weight); reports a first reach, a queueing for
     // the next round (within the bound) or a first far entry
-    __device__ __forceinline__ void arc(const WbcDev &a, double d, uint32_t e, bool &first, bool &queue, bool &fnew,
-                                        uint32_t &w)
+    __device__ __forceinline__ void arc(const WbcDev &a, double d, uint32_t e, bool &first, bool &queue, bool &fwin,
+                                        bool &fovf, uint32_t &w)
     {
         w = a.out_col[e];
         const unsigned long long nb = zg_f64_to_bits(d + a.out_w[e]);
@@ -313,9 +351,13 @@ struct WbcRelax {
                 if (nb <= Tb) {
                     queue = atomicExch(&stamp[w], r1) != r1;
                     farflag[w] = 2;
+                } else if (nb < WL) {
+                    const uint32_t f = farflag[w];
+                    if (f == 0u || f == 3u) fwin = atomicCAS(&farflag[w], f, 1u) == f;
+                    fmw = nb < fmw ? nb : fmw;
                 } else {
-                    fnew = atomicCAS(&farflag[w], 0u, 1u) == 0u;
-                    fm = nb < fm ? nb : fm;
+                    fovf = atomicCAS(&farflag[w], 0u, 3u) == 0u;
+                    fmo = nb < fmo ? nb : fmo;
                 }
             }
         }
@@ -324,7 +366,7 @@ struct WbcRelax {
     // As arc(), with the head, the weight and a read of the head's
     // distance already loaded (a stale read is never below the current one)
     __device__ __forceinline__ void arc_loaded(double d, uint32_t w, double wt, unsigned long long dw, bool &first,
-                                               bool &queue, bool &fnew)
+                                               bool &queue, bool &fwin, bool &fovf)
     {
         const unsigned long long nb = zg_f64_to_bits(d + wt);
         if (nb < dw) {
@@ -334,19 +376,24 @@ struct WbcRelax {
                 if (nb <= Tb) {
                     queue = atomicExch(&stamp[w], r1) != r1;
                     farflag[w] = 2;
+                } else if (nb < WL) {
+                    const uint32_t f = farflag[w];
+                    if (f == 0u || f == 3u) fwin = atomicCAS(&farflag[w], f, 1u) == f;
+                    fmw = nb < fmw ? nb : fmw;
                 } else {
-                    fnew = atomicCAS(&farflag[w], 0u, 1u) == 0u;
-                    fm = nb < fm ? nb : fm;
+                    fovf = atomicCAS(&farflag[w], 0u, 3u) == 0u;
+                    fmo = nb < fmo ? nb : fmo;
                 }
             }
         }
     }
 
     // The appends of one step; every lane of the wavefront calls it.
-    __device__ __forceinline__ void append(bool first, bool queue, bool fnew, uint32_t w, int lane)
+    __device__ __forceinline__ void append(bool first, bool queue, bool fwin, bool fovf, uint32_t w, int lane)
     {
         wbc_append(queue, w, ncnt, nfr, lane);
-        wbc_append(fnew, w, fcnt, far, lane);
+        wbc_append(fwin, w, wcnt, win, lane);
+        wbc_append(fovf, w, ocnt, ovf, lane);
         wbc_append(first, w, rcnt, reach, lane);
     }
 
@@ -354,18 +401,19 @@ struct WbcRelax {
     __device__ __forceinline__ void wave_range(const WbcDev &a, double d, uint32_t e0, uint32_t e1, int lane)
     {
         for (uint32_t k0 = e0; k0 < e1; k0 += 64) {
-            bool first = false, queue = false, fnew = false;
+            bool first = false, queue = false, fwin = false, fovf = false;
             uint32_t w = 0;
-            if (k0 + lane < e1) arc(a, d, k0 + lane, first, queue, fnew, w);
-            append(first, queue, fnew, w, lane);
+            if (k0 + lane < e1) arc(a, d, k0 + lane, first, queue, fwin, fovf, w);
+            append(first, queue, fwin, fovf, w, lane);
         }
     }
 
     // Folds the lanes' least far distance into the source's; all lanes call it.
     __device__ __forceinline__ void flush(const WbcDev &a, uint32_t s, int lane)
     {
-        const unsigned long long m = zg_wave_min_u64(fm);
-        if (lane == 0 && m != ZG_W_UNSET) atomicMin(&a.fmin[s], m);
+        const unsigned long long mw = zg_wave_min_u64(fmw), mo = zg_wave_min_u64(fmo);
+        if (lane == 0 && mw != ZG_W_UNSET) atomicMin(&a.fmin[s], mw);
+        if (lane == 0 && mo != ZG_W_UNSET) atomicMin(&a.fmin[a.S + s], mo);
     }
 };
 
@@ -433,9 +481,9 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
             for (int t = 0; t < 4; t++) dw[t] = k0 + t < mine ? R.dist[w[t]] : 0ull;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                bool first = false, queue = false, fnew = false;
-                if (k0 + t < mine) R.arc_loaded(dv, w[t], wt[t], dw[t], first, queue, fnew);
-                R.append(first, queue, fnew, w[t], lane);
+                bool first = false, queue = false, fwin = false, fovf = false;
+                if (k0 + t < mine) R.arc_loaded(dv, w[t], wt[t], dw[t], first, queue, fwin, fovf);
+                R.append(first, queue, fwin, fovf, w[t], lane);
             }
         }
         R.flush(a, s, lane);
@@ -676,7 +724,7 @@ int zg_wbc_ws_release(zg_graph *g)
     zg_dfree(w.fcnt); zg_dfree(w.reach); zg_dfree(w.rcnt); zg_dfree(w.order); zg_dfree(w.qlen);
     zg_dfree(w.lstart); zg_dfree(w.tot); zg_dfree(w.src); zg_dfree(w.bc); zg_dfree(w.stats); zg_dfree(w.tbits);
     zg_dfree(w.far); zg_dfree(w.farcnt); zg_dfree(w.farflag); zg_dfree(w.fsel); zg_dfree(w.split);
-    zg_dfree(w.thr); zg_dfree(w.fmin); zg_dfree(w.hcnt); zg_dfree(w.hub_v); zg_dfree(w.hub_a);
+    zg_dfree(w.thr); zg_dfree(w.wlim); zg_dfree(w.fmin); zg_dfree(w.hcnt); zg_dfree(w.hub_v); zg_dfree(w.hub_a);
     w.S = 0;
     w.lcap = 0;
     return ZG_OK;
@@ -712,13 +760,14 @@ static int wbc_ws_ensure(zg_graph *g, uint32_t S)
     if (rc == ZG_OK) rc = zg_dmalloc(&w.bc, V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, 3);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.tbits, (size_t)S * zg_wbc_words(g));
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.far, 2 * SV);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.farcnt, 2 * (size_t)S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.far, 4 * SV);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.farcnt, 4 * (size_t)S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.farflag, SV);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.fsel, S);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.split, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fsel, 2 * (size_t)S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.split, 2 * (size_t)S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.thr, S);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.fmin, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.wlim, S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fmin, 2 * (size_t)S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hcnt, 1);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hub_v, ZG_W_HUB_CAP);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.hub_a, ZG_W_HUB_CAP);
@@ -786,7 +835,7 @@ static WbcDev wbc_dev(zg_graph *g)
     a.tbits = w.tbits;
     a.tw = zg_wbc_words(g);
     a.far = w.far; a.farcnt = w.farcnt; a.farflag = w.farflag; a.fsel = w.fsel; a.split = w.split;
-    a.thr = w.thr; a.fmin = w.fmin;
+    a.thr = w.thr; a.wlim = w.wlim; a.fmin = w.fmin;
     a.hcnt = w.hcnt; a.hub_v = w.hub_v; a.hub_a = w.hub_a; a.hcap = ZG_W_HUB_CAP;
     a.delta = zg_wbc_delta(g);
     return a;
@@ -803,9 +852,9 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
     }
     // sources per batch: as many as the call has (rounded up to 64), at most
     // the tuned width (default ZG_W_MAX_SOURCES), and as many as half the
-    // free device memory holds (60 bytes per vertex and 4 per 32 arcs of
+    // free device memory holds (68 bytes per vertex and 4 per 32 arcs of
     // state per source); a larger workspace from an earlier call is kept
-    const uint64_t per_src = 60ull * V + 4ull * zg_wbc_words(g);
+    const uint64_t per_src = 68ull * V + 4ull * zg_wbc_words(g);
     const uint32_t cap = g->wbc_batch ? g->wbc_batch : ZG_W_MAX_SOURCES;
     const uint32_t want = (uint32_t)std::min<uint64_t>(cap, (n + 63) / 64 * 64);
     uint32_t S = g->wbc.S;
@@ -844,7 +893,8 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             a = wbc_dev(g);
             for (uint32_t k = 0; k < ZG_W_ROUNDS_PER_SYNC; k++) {
                 ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_round_kernel, dim3(1), dim3(1024), 0, s, a, (int)(r + k));
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k), 0);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k), 1);
                 ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_relax_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
                 ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_hub_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
             }

@@ -160,11 +160,12 @@ struct zg_wbc_ws {
     uint32_t *lstart = nullptr;          // [lcap][S]
     uint32_t *tot = nullptr;             // [lcap]
     uint32_t *tbits = nullptr;           // [S][(E + 31) / 32 + 1] tight-arc bits
-    uint32_t *far = nullptr;             // [2][S][V] far piles
-    uint32_t *farcnt = nullptr;          // [2][S]
+    uint32_t *far = nullptr;             // [2 piles][2 buffers][S][V] far piles
+    uint32_t *farcnt = nullptr;          // [2][2][S]
     uint32_t *farflag = nullptr;         // [S][V]
-    uint32_t *fsel = nullptr, *split = nullptr;  // [S]
-    unsigned long long *thr = nullptr, *fmin = nullptr;  // [S]
+    uint32_t *fsel = nullptr, *split = nullptr;  // [2][S]
+    unsigned long long *thr = nullptr, *wlim = nullptr;  // [S]
+    unsigned long long *fmin = nullptr;  // [2][S]
     uint32_t *hcnt = nullptr, *hub_v = nullptr, *hub_a = nullptr;  // hub pieces of a relax round
     uint32_t *src = nullptr;             // [S]
     double *bc = nullptr;                // [V]
