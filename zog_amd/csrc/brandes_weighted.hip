@@ -253,14 +253,6 @@ __global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r, int p)
     }
 }
 
-// Tight arc: v != w, w not the source, fl(d(v) + weight) == d(w) bit for bit.
-__device__ __forceinline__ bool wbc_tight(const WbcDev &a, const unsigned long long *dist, uint32_t v, uint32_t src,
-                                          double dv, uint32_t e, uint32_t &w)
-{
-    w = a.out_col[e];
-    return w != v && w != src && zg_f64_to_bits(dv + a.out_w[e]) == dist[w];
-}
-
 __device__ __forceinline__ double zg_readlane_f64(double x, int j)
 {
     return zg_bits_to_f64(zg_readlane64(zg_f64_to_bits(x), j));
@@ -312,6 +304,7 @@ __device__ __forceinline__ WbcTightIt wbc_word_share(uint32_t ej, uint32_t dj, u
 struct WbcRelax {
     unsigned long long *dist;
     uint32_t *stamp, *farflag, *ncnt, *nfr, *rcnt, *reach, *wcnt, *win, *ocnt, *ovf;
+    uint32_t *tb;  // candidate bits: arcs that were tight when relaxed
     unsigned long long Tb, WL;
     uint32_t r1;
     unsigned long long fmw, fmo;  // least window / pile-1 distance this lane recorded
@@ -326,6 +319,7 @@ struct WbcRelax {
         nfr = a.fr + ((size_t)nxt * a.S + s) * a.V;
         rcnt = &a.rcnt[s];
         reach = a.reach + WROW(s);
+        tb = a.tbits + s * a.tw;
         const uint32_t fw = a.fsel[s], fo = a.fsel[a.S + s];
         wcnt = &a.farcnt[FARCNT(0, fw, s)];
         win = a.far + FARROW(0, fw, s);
@@ -337,55 +331,56 @@ struct WbcRelax {
         fmw = fmo = ZG_W_UNSET;
     }
 
-    // lowers d(w) to fl(d + weight); reports a first reach, a queueing for
-    // the next round (within the bound) or a first far entry
-    __device__ __forceinline__ void arc(const WbcDev &a, double d, uint32_t e, bool &first, bool &queue, bool &fwin,
-                                        bool &fovf, uint32_t &w)
-    {
-        w = a.out_col[e];
-        const unsigned long long nb = zg_f64_to_bits(d + a.out_w[e]);
-        if (nb < dist[w]) {
-            const unsigned long long old = atomicMin(&dist[w], nb);
-            if (nb < old) {
-                first = old == ZG_W_UNSET;
-                if (nb <= Tb) {
-                    queue = atomicExch(&stamp[w], r1) != r1;
-                    farflag[w] = 2;
-                } else if (nb < WL) {
-                    const uint32_t f = farflag[w];
-                    if (f == 0u || f == 3u) fwin = atomicCAS(&farflag[w], f, 1u) == f;
-                    fmw = nb < fmw ? nb : fmw;
-                } else {
-                    fovf = atomicCAS(&farflag[w], 0u, 3u) == 0u;
-                    fmo = nb < fmo ? nb : fmo;
-                }
-            }
-        }
-    }
-
-    // As arc(), with the head, the weight and a read of the head's
-    // distance already loaded (a stale read is never below the current one)
-    __device__ __forceinline__ void arc_loaded(double d, uint32_t w, double wt, unsigned long long dw, bool &first,
+    // As arc(), with the head, the weight and a read of the head's distance
+    // already loaded (a stale read is never below the current one).  Returns
+    // whether the arc is a candidate tight arc: fl(d + weight) is at most the
+    // head's distance once lowered.  Every arc that is tight at the end is a
+    // candidate when its tail is relaxed at its final distance.
+    __device__ __forceinline__ bool arc_loaded(double d, uint32_t w, double wt, unsigned long long dw, bool &first,
                                                bool &queue, bool &fwin, bool &fovf)
     {
         const unsigned long long nb = zg_f64_to_bits(d + wt);
-        if (nb < dw) {
-            const unsigned long long old = atomicMin(&dist[w], nb);
-            if (nb < old) {
-                first = old == ZG_W_UNSET;
-                if (nb <= Tb) {
-                    queue = atomicExch(&stamp[w], r1) != r1;
-                    farflag[w] = 2;
-                } else if (nb < WL) {
-                    const uint32_t f = farflag[w];
-                    if (f == 0u || f == 3u) fwin = atomicCAS(&farflag[w], f, 1u) == f;
-                    fmw = nb < fmw ? nb : fmw;
-                } else {
-                    fovf = atomicCAS(&farflag[w], 0u, 3u) == 0u;
-                    fmo = nb < fmo ? nb : fmo;
-                }
+        if (nb > dw) return false;
+        if (nb == dw) return true;
+        const unsigned long long old = atomicMin(&dist[w], nb);
+        if (nb < old) {
+            first = old == ZG_W_UNSET;
+            if (nb <= Tb) {
+                queue = atomicExch(&stamp[w], r1) != r1;
+                farflag[w] = 2;
+            } else if (nb < WL) {
+                const uint32_t f = farflag[w];
+                if (f == 0u || f == 3u) fwin = atomicCAS(&farflag[w], f, 1u) == f;
+                fmw = nb < fmw ? nb : fmw;
+            } else {
+                fovf = atomicCAS(&farflag[w], 0u, 3u) == 0u;
+                fmo = nb < fmo ? nb : fmo;
             }
         }
+        return nb <= old;
+    }
+
+    // lowers d(w) to fl(d + weight) for arc e; reports a first reach, a
+    // queueing for the next round (within the bound), a first entry into a far
+    // pile, and whether the arc is a candidate tight arc
+    __device__ __forceinline__ bool arc(const WbcDev &a, double d, uint32_t e, bool &first, bool &queue, bool &fwin,
+                                        bool &fovf, uint32_t &w)
+    {
+        w = a.out_col[e];
+        const double wt = a.out_w[e];
+        return arc_loaded(d, w, wt, dist[w], first, queue, fwin, fovf);
+    }
+
+    // Marks the candidate arcs of a wavefront step over the arcs [k0, k0+64):
+    // lane i's flag is arc k0+i's.  All lanes call it.
+    __device__ __forceinline__ void mark_wave(bool cand, uint32_t k0, int lane)
+    {
+        const uint64_t M = __ballot(cand);
+        if (!M) return;
+        const uint32_t off = k0 & 31, wb = k0 >> 5;
+        const uint64_t lo = M << off, hi = off ? M >> (64 - off) : 0;
+        const uint32_t word = lane == 0 ? (uint32_t)lo : lane == 1 ? (uint32_t)(lo >> 32) : (uint32_t)hi;
+        if (lane < 3 && word) atomicOr(&tb[wb + lane], word);
     }
 
     // The appends of one step; every lane of the wavefront calls it.
@@ -401,10 +396,11 @@ struct WbcRelax {
     __device__ __forceinline__ void wave_range(const WbcDev &a, double d, uint32_t e0, uint32_t e1, int lane)
     {
         for (uint32_t k0 = e0; k0 < e1; k0 += 64) {
-            bool first = false, queue = false, fwin = false, fovf = false;
+            bool first = false, queue = false, fwin = false, fovf = false, cand = false;
             uint32_t w = 0;
-            if (k0 + lane < e1) arc(a, d, k0 + lane, first, queue, fwin, fovf, w);
+            if (k0 + lane < e1) cand = arc(a, d, k0 + lane, first, queue, fwin, fovf, w);
             append(first, queue, fwin, fovf, w, lane);
+            mark_wave(cand, k0, lane);
         }
     }
 
@@ -479,12 +475,21 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) dw[t] = k0 + t < mine ? R.dist[w[t]] : 0ull;
+            // candidate bits of the four arcs: the word of arc e0+k0 and the next
+            const uint32_t eb = e0 + k0;
+            uint32_t m0 = 0, m1 = 0;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 bool first = false, queue = false, fwin = false, fovf = false;
-                if (k0 + t < mine) R.arc_loaded(dv, w[t], wt[t], dw[t], first, queue, fwin, fovf);
+                if (k0 + t < mine && R.arc_loaded(dv, w[t], wt[t], dw[t], first, queue, fwin, fovf)) {
+                    const uint32_t e = eb + t;
+                    if ((e >> 5) == (eb >> 5)) m0 |= 1u << (e & 31);
+                    else m1 |= 1u << (e & 31);
+                }
                 R.append(first, queue, fwin, fovf, w[t], lane);
             }
+            if (m0) atomicOr(&R.tb[eb >> 5], m0);
+            if (m1) atomicOr(&R.tb[(eb >> 5) + 1], m1);
         }
         R.flush(a, s, lane);
     }
@@ -515,7 +520,10 @@ __global__ __launch_bounds__(256) void wbc_hub_kernel(WbcDev a, int r)
     }
 }
 
-// Tight in-arc counts over every reached vertex, and the sweep statistics.
+// Tight arcs among the candidates the relaxation marked, over every reached
+// vertex: a candidate that is not tight at the final distances (v != w, w not
+// the source, fl(d(v) + weight) == d(w) bit for bit) loses its bit; a tight
+// one counts into its head's tight in-degree.  Also the sweep statistics.
 __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
 {
     __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
@@ -541,39 +549,48 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
         arcs += deg;
         reached += valid ? 1 : 0;
-        for (uint64_t wide = __ballot(deg > ZG_W_WIDE); wide; wide &= wide - 1) {
-            const int j = __builtin_ctzll(wide);
-            const uint32_t vj = zg_readlane32(v, j), ej = zg_readlane32(e0, j), dj = zg_readlane32(deg, j);
-            const double dvj = zg_readlane_f64(dv, j);
-            for (uint32_t k = lane; k < dj; k += 64) {
-                uint32_t w;
-                if (wbc_tight(a, dist, vj, src, dvj, ej + k, w)) {
-                    atomicAdd(&indeg[w], 1u);
-                    atomicOr(&tb[(ej + k) >> 5], 1u << ((ej + k) & 31));
+        // up to four candidates per step, their loads overlapped
+        auto walk = [&](WbcTightIt it, uint32_t vv, double d) {
+            for (;;) {
+                uint32_t e[4] = {0, 0, 0, 0};
+                int n = 0;
+                if (wbc_next_tight(it, tb, e[0])) {
+                    n = 1;
+                    if (wbc_next_tight(it, tb, e[1])) {
+                        n = 2;
+                        if (wbc_next_tight(it, tb, e[2])) n = wbc_next_tight(it, tb, e[3]) ? 4 : 3;
+                    }
                 }
-            }
-        }
-        if (deg <= ZG_W_WIDE)
-            for (uint32_t k0 = 0; k0 < deg; k0 += 4) {
+                if (!n) break;
                 uint32_t w[4];
                 double wt[4];
                 unsigned long long dw[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    w[t] = k0 + t < deg ? a.out_col[e0 + k0 + t] : 0u;
-                    wt[t] = k0 + t < deg ? a.out_w[e0 + k0 + t] : 0.0;
+                    w[t] = t < n ? a.out_col[e[t]] : 0u;
+                    wt[t] = t < n ? a.out_w[e[t]] : 0.0;
                 }
 #pragma unroll
-                for (int t = 0; t < 4; t++) dw[t] = k0 + t < deg ? dist[w[t]] : ZG_W_UNSET;
+                for (int t = 0; t < 4; t++) dw[t] = t < n ? dist[w[t]] : 0ull;
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    const uint32_t e = e0 + k0 + t;
-                    if (k0 + t < deg && w[t] != v && w[t] != src && zg_f64_to_bits(dv + wt[t]) == dw[t]) {
-                        atomicAdd(&indeg[w[t]], 1u);
-                        atomicOr(&tb[e >> 5], 1u << (e & 31));
+                    if (t < n) {
+                        if (w[t] != vv && w[t] != src && zg_f64_to_bits(d + wt[t]) == dw[t])
+                            atomicAdd(&indeg[w[t]], 1u);
+                        else
+                            atomicAnd(&tb[e[t] >> 5], ~(1u << (e[t] & 31)));
                     }
                 }
             }
+        };
+        for (uint64_t wide = __ballot(deg > ZG_W_WIDE); wide; wide &= wide - 1) {
+            const int j = __builtin_ctzll(wide);
+            const uint32_t vj = zg_readlane32(v, j), ej = zg_readlane32(e0, j), dj = zg_readlane32(deg, j);
+            const double dvj = zg_readlane_f64(dv, j);
+            for (uint32_t k0 = 0, nw = wbc_words(ej, dj); k0 < nw; k0 += 64)
+                walk(wbc_word_share(ej, dj, k0 + lane), vj, dvj);
+        }
+        walk(WbcTightIt{e0, deg <= ZG_W_WIDE ? e0 + deg : e0, 0, 0}, v, dv);
     }
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
