@@ -349,8 +349,19 @@ struct WbcRelax {
                 queue = atomicExch(&stamp[w], r1) != r1;
                 farflag[w] = 2;
             } else if (nb < WL) {
-                const uint32_t f = farflag[w];
-                if (f == 0u || f == 3u) fwin = atomicCAS(&farflag[w], f, 1u) == f;
+                // into the window from "never far" or from pile 1; retried
+                // when another lane moves the flag in between (0 -> 3 by a
+                // larger sum that lowered d(w) first), so a vertex below the
+                // window's end never stays in pile 1
+                uint32_t f = farflag[w];
+                while (f == 0u || f == 3u) {
+                    const uint32_t o = atomicCAS(&farflag[w], f, 1u);
+                    if (o == f) {
+                        fwin = true;
+                        break;
+                    }
+                    f = o;
+                }
                 fmw = nb < fmw ? nb : fmw;
             } else {
                 fovf = atomicCAS(&farflag[w], 0u, 3u) == 0u;
