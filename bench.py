@@ -279,7 +279,7 @@ def bench_weighted(args):
     # (8 B) and the head's distance (8 B); a vertex its row bounds (8 B), its
     # distance (8 B) and, in the path-count and dependency passes, its sigma
     # or c value (8 B)
-    kb = {"wbc_relax_kernel": (20 * relaxed_step, prof["bc_forward"]),
+    kb = {"wbc_relax_kernel+wbc_hub_kernel+wbc_split_kernel": (20 * relaxed_step, prof["bc_forward"]),
           "wbc_count_kernel+wbc_peel_kernel": (40 * arcs_step + 48 * nodes_step, prof["bc_dag"]),
           "wbc_backward_kernel": (20 * arcs_step + 24 * nodes_step, prof["bc_backward"])}
     dom = max(kb, key=lambda k: kb[k][1][0])
