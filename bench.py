@@ -663,6 +663,19 @@ def main():
                          f"the {len(sources)} sources of {args.workload}, host cores of the GPU box",
                "pagerank_gbs": round(ref_it * pr_bytes_iter / cpu_pr_s / 1e9, 3),
                "pagerank_sample": f"full FP64 oracle PageRank on {args.workload} ({ref_it} iterations, 1 thread)"}
+        # the parallel mode: the same oracle with the sources split into
+        # contiguous chunks over the host's CPU share (the reference's own
+        # threaded Brandes, centrality.zig:216-256), on 4x the sample
+        pthreads = max(1, min(16, os.cpu_count() or 1))
+        psample = sources if spec["cpu_sources"] is None else sources[: 4 * spec["cpu_sources"]]
+        t0 = time.perf_counter()
+        _, _, parcs = og.betweenness(sources=psample, threads=pthreads, return_arcs=True, mode=mode)
+        par_s = time.perf_counter() - t0
+        cpu["parallel"] = {"value": round(float(parcs.sum()) / par_s / 1e9, 5), "unit": "GTEPS",
+                           "cores": pthreads,
+                           "sample": f"{len(psample)} of the sources, source-chunked threads"}
+        log(f"[bench] oracle BC {len(psample)} sources on {pthreads} threads in {par_s:.1f}s = "
+            f"{cpu['parallel']['value']:.4f} GTEPS")
 
     line = {
         "metric": "Brandes GTEPS (exact BC) + PageRank GB/s",

@@ -3,13 +3,14 @@
 // Reference: priv/zog/src/centrality.zig:694-798.  The reference keeps, per
 // source, a BFS queue whose order is also the stack of the dependency sweep;
 // this engine keeps exactly that per source, for S sources at once:
-//   de[s][v]   {depth (u32), exponent (i32)} in one 8-byte word
-//   sig[s][v]  path-count mantissa (FP64): sigma = sig * 2^exponent, sig in
-//              [1, 2) -- every count has its own exponent, so a level whose
-//              counts run from 1 to 2^4000 (a 2048 x 2048 grid) is still
-//              represented; counts below 2^53 are exact
-//   cc[s][v]   (1 + delta) / sig, the dependency term c = (1+delta)/sigma
-//              scaled by 2^exponent
+//   rec[s][v]  one 16-byte record {depth (u32), exponent (i32), value}: the
+//              value is first the path-count mantissa sig (FP64): sigma =
+//              sig * 2^exponent, sig in [1, 2) -- every count has its own
+//              exponent, so a level whose counts run from 1 to 2^4000 (a
+//              2048 x 2048 grid) is still represented; counts below 2^53 are
+//              exact -- and, once the dependency sweep has passed v, cc =
+//              (1 + delta) / sig, the dependency term c = (1+delta)/sigma
+//              scaled by 2^exponent (sig(v) is read only by v itself then)
 //   queue[s][i], lstart[L][s] (queue offset of level L), qlen[s].
 // One lane handles one (source, vertex) pair; a wavefront always holds pairs
 // of a single source, so discoveries are appended to that source's queue
@@ -44,13 +45,12 @@
 #define ZG_Q_GRID 1024
 #define ZG_Q_LEVELS_PER_SYNC 32
 #define ZG_Q_WIDE 32           // vertices with more predecessors are summed by the whole wavefront
-#define ZG_Q_BYTES_PER_SV 28   // workspace bytes per (source, vertex)
+#define ZG_Q_BYTES_PER_SV 20   // workspace bytes per (source, vertex)
 
 struct BcqDev {
     uint32_t V, S, lcap;
     const uint32_t *out_ptr, *out_col, *in_ptr, *in_col;
-    uint2 *de;                      // [S][V] {depth, exponent}
-    double *sig, *cc;               // [S][V]
+    zg_qrec *rec;                   // [S][V] {depth, exponent, sig or cc}
     uint32_t *queue;                // [S][V]
     uint32_t *lstart;               // [lcap][S]
     uint32_t *qlen;                 // [S]
@@ -81,8 +81,7 @@ __global__ void bcq_init_kernel(BcqDev a)
         a.lstart[a.S + s] = 0;
         return;
     }
-    a.de[QROW(s) + v] = make_uint2(0u, 0u);
-    a.sig[QROW(s) + v] = 1.0;
+    a.rec[QROW(s) + v] = zg_qrec{0u, 0u, 1.0};
     a.queue[QROW(s)] = v;
     a.qlen[s] = 1;
     a.lstart[s] = 0;        // level 0 starts at 0
@@ -110,8 +109,7 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
-        uint2 *de = a.de + QROW(s);
-        double *sg = a.sig + QROW(s);
+        zg_qrec *rec = a.rec + QROW(s);
         const uint32_t v = valid ? a.queue[QROW(s) + idx] : 0;
         if (d > 0) {
             // path count of v from its predecessors at depth d-1
@@ -123,24 +121,21 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
                 // four arcs at a time: their loads overlap instead of chaining
                 for (uint32_t k0 = 0; k0 < ideg; k0 += 4) {
                     uint32_t u[4];
-                    uint2 q[4];
-                    double sv[4];
+                    zg_qrec q[4];
 #pragma unroll
                     for (int t = 0; t < 4; t++) u[t] = k0 + t < ideg ? a.in_col[i0 + k0 + t] : 0u;
 #pragma unroll
-                    for (int t = 0; t < 4; t++) q[t] = k0 + t < ideg ? de[u[t]] : make_uint2(ZG_NONE, 0u);
-#pragma unroll
-                    for (int t = 0; t < 4; t++) sv[t] = q[t].x == dprev ? sg[u[t]] : 0.0;
+                    for (int t = 0; t < 4; t++) q[t] = k0 + t < ideg ? rec[u[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
 #pragma unroll
                     for (int t = 0; t < 4; t++) {
-                        if (q[t].x != dprev) continue;
-                        const int eu = (int)q[t].y;
+                        if (q[t].depth != dprev) continue;
+                        const int eu = (int)q[t].exp;
                         if (x == 0.0) E = eu;
                         else if (eu > E) {
                             x = zg_scale(x, E - eu);
                             E = eu;
                         }
-                        x += zg_scale(sv[t], eu - E);
+                        x += zg_scale(q[t].val, eu - E);
                     }
                 }
             }
@@ -150,16 +145,15 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
                 const uint32_t ij = zg_readlane32(i0, j), dj = zg_readlane32(ideg, j);
                 int emax = INT_MIN;
                 for (uint32_t k = lane; k < dj; k += 64) {
-                    const uint2 q = de[a.in_col[ij + k]];
-                    if (q.x == dprev) emax = max(emax, (int)q.y);
+                    const zg_qrec q = rec[a.in_col[ij + k]];
+                    if (q.depth == dprev) emax = max(emax, (int)q.exp);
                 }
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) emax = max(emax, __shfl_xor(emax, off, 64));
                 double part = 0.0;
                 for (uint32_t k = lane; k < dj; k += 64) {
-                    const uint32_t u = a.in_col[ij + k];
-                    const uint2 q = de[u];
-                    if (q.x == dprev) part += zg_scale(sg[u], (int)q.y - emax);
+                    const zg_qrec q = rec[a.in_col[ij + k]];
+                    if (q.depth == dprev) part += zg_scale(q.val, (int)q.exp - emax);
                 }
                 part = zg_wave_sum(part);
                 if (lane == j) {
@@ -169,8 +163,7 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
             }
             if (valid) {
                 const int k = ilogb(x);  // x >= 1: v has a predecessor at depth d-1
-                sg[v] = ldexp(x, -k);
-                de[v].y = (uint32_t)(E + k);
+                rec[v] = zg_qrec{(uint32_t)d, (uint32_t)(E + k), ldexp(x, -k)};  // depth unchanged
             }
         }
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
@@ -187,12 +180,12 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
 #pragma unroll
             for (int t = 0; t < 4; t++) w[t] = k0 + t < deg ? a.out_col[e0 + k0 + t] : 0u;
 #pragma unroll
-            for (int t = 0; t < 4; t++) dw[t] = k0 + t < deg ? de[w[t]].x : 0u;
+            for (int t = 0; t < 4; t++) dw[t] = k0 + t < deg ? rec[w[t]].depth : 0u;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 bool won = false;
                 if (k0 + t < deg && dw[t] == ZG_NONE)
-                    won = atomicCAS(&de[w[t]].x, ZG_NONE, (uint32_t)(d + 1)) == ZG_NONE;
+                    won = atomicCAS(&rec[w[t]].depth, ZG_NONE, (uint32_t)(d + 1)) == ZG_NONE;
                 const uint64_t bw = __ballot(won);
                 if (bw) {
                     uint32_t base = 0;
@@ -249,29 +242,27 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         if (idx >= s_hi[s]) continue;
-        const uint2 *de = a.de + QROW(s);
-        const double *cc = a.cc + QROW(s);
+        zg_qrec *rec = a.rec + QROW(s);
         const uint32_t v = a.queue[QROW(s) + idx];
-        const int ev = (int)de[v].y;
+        const zg_qrec me = rec[v];
+        const int ev = (int)me.exp;
         double S = 0.0;
         for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e += 4) {
-            // four arcs at a time: heads, then depths and exponents, then terms
+            // four arcs at a time: heads, then the records (depth, exponent
+            // and, at depth d+1, the dependency term)
             uint32_t w[4];
-            uint2 q[4];
-            double cw[4];
+            zg_qrec q[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) w[t] = e + t < e1 ? a.out_col[e + t] : 0u;
 #pragma unroll
-            for (int t = 0; t < 4; t++) q[t] = e + t < e1 ? de[w[t]] : make_uint2(ZG_NONE, 0u);
-#pragma unroll
-            for (int t = 0; t < 4; t++) cw[t] = q[t].x == dnext ? cc[w[t]] : 0.0;
+            for (int t = 0; t < 4; t++) q[t] = e + t < e1 ? rec[w[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                if (q[t].x == dnext) S += zg_scale(cw[t], ev - (int)q[t].y);
+                if (q[t].depth == dnext) S += zg_scale(q[t].val, ev - (int)q[t].exp);
         }
-        const double sv = a.sig[QROW(s) + v];
+        const double sv = me.val;
         const double delta = sv * S;
-        a.cc[QROW(s) + v] = (1.0 + delta) / sv;
+        rec[v].val = (1.0 + delta) / sv;  // from here on the dependency term
         if (v != a.src[s]) atomicAdd(&a.bc[v], a.wts ? delta * a.wts[s] : delta);
     }
 }
@@ -285,7 +276,7 @@ __global__ void bcq_out_kernel(uint32_t V, const double *bc, const uint8_t *live
 int zg_bcq_ws_release(zg_graph *g)
 {
     zg_bcq_ws &w = g->bcq;
-    zg_dfree(w.de); zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.queue); zg_dfree(w.lstart);
+    zg_dfree(w.rec); zg_dfree(w.queue); zg_dfree(w.lstart);
     zg_dfree(w.qlen); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.wts); zg_dfree(w.bc);
     zg_dfree(w.stats);
     w.S = 0;
@@ -301,9 +292,7 @@ static int bcq_ws_ensure(zg_graph *g, uint32_t S)
     const size_t V = g->V;
     w.lcap = 1024;
     int rc = ZG_OK;
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.de, S * V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, S * V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.rec, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.queue, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.lstart, (size_t)w.lcap * S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.qlen, S);
@@ -359,9 +348,7 @@ static BcqDev bcq_dev(zg_graph *g, bool weighted)
     a.out_col = g->out_col;
     a.in_ptr = g->in_ptr;
     a.in_col = g->in_col;
-    a.de = w.de;
-    a.sig = w.sig;
-    a.cc = w.cc;
+    a.rec = w.rec;
     a.queue = w.queue;
     a.lstart = w.lstart;
     a.qlen = w.qlen;
@@ -400,7 +387,7 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
         if (d_weights)
             ZG_TRY(hipMemcpyAsync(w.wts, d_weights + b0, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
-        ZG_TRY(hipMemsetAsync(w.de, 0xFF, sizeof(uint2) * (size_t)S * V, s));  // depth unset
+        ZG_TRY(hipMemsetAsync(w.rec, 0xFF, sizeof(zg_qrec) * (size_t)S * V, s));  // depth unset
         ZG_TRY(hipMemsetAsync(w.newcnt, 0, sizeof(uint32_t) * w.lcap, s));
         ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 2, s));
         BcqDev a = bcq_dev(g, d_weights != nullptr);

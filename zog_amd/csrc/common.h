@@ -127,12 +127,20 @@ struct zg_bc_ws {
     uint32_t last_levels = 0;            // levels used by the previous batch
 };
 
+// Per-(source, vertex) record of the queue engine: depth, path-count
+// exponent and one value, the path-count mantissa until the dependency sweep
+// replaces it with the dependency term; one 16-byte line segment, so a
+// neighbour's depth, exponent and value arrive in one load.
+struct alignas(16) zg_qrec {
+    uint32_t depth;
+    uint32_t exp;
+    double val;
+};
+
 // Per-source queue engine workspace (brandes_queue.hip), S sources per batch.
 struct zg_bcq_ws {
     uint32_t S = 0, lcap = 0;
-    uint2 *de = nullptr;                 // [S][V] {depth, path-count exponent}
-    double *sig = nullptr;               // [S][V] path-count mantissa
-    double *cc = nullptr;                // [S][V]
+    zg_qrec *rec = nullptr;              // [S][V]
     uint32_t *queue = nullptr;           // [S][V]
     uint32_t *lstart = nullptr;          // [lcap][S]
     uint32_t *qlen = nullptr;            // [S]
