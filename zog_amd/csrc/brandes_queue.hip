@@ -102,10 +102,10 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t dprev = (uint32_t)(d - 1);
     unsigned long long arcs = 0, reached = 0;
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + wv; c < total; c += waves) {
+    const ZgQWalk wk = zg_q_walk(total);
+    for (uint32_t c = wk.c; c < wk.end; c += wk.stride) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
@@ -236,9 +236,9 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t dnext = (uint32_t)(d + 1);
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+    const ZgQWalk wk = zg_q_walk(total);
+    for (uint32_t c = wk.c; c < wk.end; c += wk.stride) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         if (idx >= s_hi[s]) continue;

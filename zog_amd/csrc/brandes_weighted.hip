@@ -223,9 +223,9 @@ __global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r, int p)
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t mine = p == 0 ? 1u : 3u;
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+    for (ZgQWalk wk = zg_q_walk(total); wk.c < wk.end; wk.c += wk.stride) {
+        const uint32_t c = wk.c;
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
@@ -442,9 +442,9 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
         s_lo, s_hi, s_pre);
     __shared__ unsigned long long red[4];
     const int lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     unsigned long long scanned = 0;
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+    for (ZgQWalk wk = zg_q_walk(total); wk.c < wk.end; wk.c += wk.stride) {
+        const uint32_t c = wk.c;
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
@@ -547,9 +547,9 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     unsigned long long arcs = 0, reached = 0;
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + wv; c < total; c += waves) {
+    for (ZgQWalk wk = zg_q_walk(total); wk.c < wk.end; wk.c += wk.stride) {
+        const uint32_t c = wk.c;
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
@@ -634,8 +634,8 @@ __global__ __launch_bounds__(256) void wbc_peel_kernel(WbcDev a, int r)
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+    for (ZgQWalk wk = zg_q_walk(total); wk.c < wk.end; wk.c += wk.stride) {
+        const uint32_t c = wk.c;
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
@@ -703,8 +703,8 @@ __global__ __launch_bounds__(256) void wbc_backward_kernel(WbcDev a, int r)
         },
         s_lo, s_hi, s_pre);
     const int lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < total; c += waves) {
+    for (ZgQWalk wk = zg_q_walk(total); wk.c < wk.end; wk.c += wk.stride) {
+        const uint32_t c = wk.c;
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];

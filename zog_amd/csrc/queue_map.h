@@ -51,3 +51,23 @@ __device__ __forceinline__ uint32_t zg_q_owner(const uint32_t *s_pre, uint32_t S
     return lo;
 }
 
+
+// XCD-aware walk over the chunks [0, total): workgroups are dispatched
+// round-robin over the chip's 8 XCDs, so block b runs on XCD b % 8.  The
+// chunk space is cut into 8 contiguous equal slices and slice x is walked
+// only by the wavefronts of XCD x's blocks; a source's chunks are contiguous,
+// so its records stay in one XCD's L2 (a grid not a multiple of 8 walks the
+// chunks plainly).
+struct ZgQWalk {
+    uint32_t c, end, stride;
+};
+
+__device__ __forceinline__ ZgQWalk zg_q_walk(uint32_t total)
+{
+    const uint32_t wpb = blockDim.x >> 6, wv = threadIdx.x >> 6;
+    const uint32_t nx = 8;
+    if (gridDim.x % nx != 0) return ZgQWalk{blockIdx.x * wpb + wv, total, gridDim.x * wpb};
+    const uint32_t x = blockIdx.x % nx, lb = blockIdx.x / nx, nb = gridDim.x / nx;
+    const uint32_t c0 = (uint32_t)((uint64_t)total * x / nx), c1 = (uint32_t)((uint64_t)total * (x + 1) / nx);
+    return ZgQWalk{c0 + lb * wpb + wv, c1, nb * wpb};
+}
