@@ -54,10 +54,9 @@ struct WbcDev {
     uint32_t V, S, lcap;
     const uint32_t *out_ptr, *out_col;
     const double *out_w;
-    unsigned long long *dist;   // [S][V] bits of the distance, ZG_W_UNSET = unreached
-    double *sig, *cc;           // [S][V]
-    uint32_t *indeg;            // [S][V] tight in-arcs not yet processed
-    uint32_t *stamp;            // [S][V] relax round that queued the vertex last
+    zg_wrec *rec;               // [S][V] distance bits (ZG_W_UNSET = unreached), sigma, c, the relax
+                                // round that queued the vertex last (then its tight in-arcs not yet
+                                // processed), far-pile flag
     uint32_t *fr;               // [2][S][V] relax frontiers
     uint32_t *fcnt;             // [2][S]
     uint32_t *reach;            // [S][V] reached vertices
@@ -76,7 +75,7 @@ struct WbcDev {
     // pile 0 (the window) holds distances below wlim, pile 1 the rest
     uint32_t *far;              // [2 piles][2 buffers][S][V] (fsel[p][s] is the live buffer)
     uint32_t *farcnt;           // [2][2][S]
-    uint32_t *farflag;          // [S][V] 0 never far, 1 in the window, 3 in pile 1, 2 within the bound
+    // far-pile flag in rec: 0 never far, 1 in the window, 3 in pile 1, 2 within the bound
     uint32_t *fsel;             // [2][S]
     uint32_t *split;            // [2][S] this round splits the window / scans pile 1
     unsigned long long *thr;    // [S] the bound (distance bits)
@@ -105,6 +104,34 @@ __device__ __forceinline__ void wbc_append(bool want, uint32_t item, uint32_t *c
     if (want) list[base + __popcll(b & ((1ull << lane) - 1))] = item;
 }
 
+// Every record unreached: distance all ones, no paths, never queued, never far.
+__global__ void wbc_fill_kernel(zg_wrec *rec, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        rec[i] = zg_wrec{ZG_W_UNSET, 0.0, 0.0, 0xFFFFFFFFu, 0u};
+}
+
+// After the last relax round: the queue stamps of the reached vertices give
+// way to their tight in-arc counts (the unreached are never a tight head).
+__global__ __launch_bounds__(256) void wbc_aux_reset_kernel(WbcDev a)
+{
+    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
+    const uint32_t total = zg_q_chunks(
+        a.S,
+        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
+            lo = 0;
+            hi = a.rcnt[s];
+        },
+        s_lo, s_hi, s_pre);
+    const int lane = threadIdx.x & 63;
+    for (ZgQWalk wk = zg_q_walk(total); wk.c < wk.end; wk.c += wk.stride) {
+        const uint32_t c = wk.c;
+        const uint32_t s = zg_q_owner(s_pre, a.S, c);
+        const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
+        if (idx < s_hi[s]) a.rec[WROW(s) + a.reach[WROW(s) + idx]].aux = 0u;
+    }
+}
+
 __global__ void wbc_init_kernel(WbcDev a)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -128,10 +155,7 @@ __global__ void wbc_init_kernel(WbcDev a)
         a.lstart[a.S + s] = 0;
         return;
     }
-    a.dist[WROW(s) + v] = 0ull;  // +0.0
-    a.farflag[WROW(s) + v] = 2;
-    a.sig[WROW(s) + v] = 1.0;
-    a.stamp[WROW(s) + v] = 0;
+    a.rec[WROW(s) + v] = zg_wrec{0ull, 1.0, 0.0, 0u, 2u};  // distance +0.0, queued for round 0
     a.fr[WROW(s)] = v;
     a.fcnt[s] = 1;
     a.reach[WROW(s)] = v;
@@ -231,14 +255,14 @@ __global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r, int p)
         const bool valid = idx < s_hi[s];
         const uint32_t fs = a.fsel[(size_t)p * a.S + s], fw = a.fsel[s];
         const unsigned long long Tb = a.thr[s], WL = a.wlim[s];
-        uint32_t *flag = a.farflag + WROW(s);
+        zg_wrec *rec = a.rec + WROW(s);
         const uint32_t v = valid ? a.far[FARROW(p, fs ^ 1, s) + idx] : 0;
         bool move = false, to_window = false, keep = false;
         unsigned long long d = ZG_W_UNSET;
-        if (valid && flag[v] == mine) {
-            d = a.dist[WROW(s) + v];
-            if (d <= Tb) move = atomicExch(&flag[v], 2u) == mine;
-            else if (p == 1 && d < WL) to_window = atomicCAS(&flag[v], 3u, 1u) == 3u;
+        if (valid && rec[v].flag == mine) {
+            d = rec[v].dist;
+            if (d <= Tb) move = atomicExch(&rec[v].flag, 2u) == mine;
+            else if (p == 1 && d < WL) to_window = atomicCAS(&rec[v].flag, 3u, 1u) == 3u;
             else keep = true;
         }
         wbc_append(move, v, &a.fcnt[(size_t)cur * a.S + s], a.fr + ((size_t)cur * a.S + s) * a.V, lane);
@@ -302,8 +326,8 @@ __device__ __forceinline__ WbcTightIt wbc_word_share(uint32_t ej, uint32_t dj, u
 
 // One source's view of relax round r.
 struct WbcRelax {
-    unsigned long long *dist;
-    uint32_t *stamp, *farflag, *ncnt, *nfr, *rcnt, *reach, *wcnt, *win, *ocnt, *ovf;
+    zg_wrec *rec;
+    uint32_t *ncnt, *nfr, *rcnt, *reach, *wcnt, *win, *ocnt, *ovf;
     uint32_t *tb;  // candidate bits: arcs that were tight when relaxed
     unsigned long long Tb, WL;
     uint32_t r1;
@@ -312,9 +336,7 @@ struct WbcRelax {
     __device__ __forceinline__ WbcRelax(const WbcDev &a, uint32_t s, int r)
     {
         const int nxt = (r & 1) ^ 1;
-        dist = a.dist + WROW(s);
-        stamp = a.stamp + WROW(s);
-        farflag = a.farflag + WROW(s);
+        rec = a.rec + WROW(s);
         ncnt = &a.fcnt[(size_t)nxt * a.S + s];
         nfr = a.fr + ((size_t)nxt * a.S + s) * a.V;
         rcnt = &a.rcnt[s];
@@ -342,20 +364,20 @@ struct WbcRelax {
         const unsigned long long nb = zg_f64_to_bits(d + wt);
         if (nb > dw) return false;
         if (nb == dw) return true;
-        const unsigned long long old = atomicMin(&dist[w], nb);
+        const unsigned long long old = atomicMin(&rec[w].dist, nb);
         if (nb < old) {
             first = old == ZG_W_UNSET;
             if (nb <= Tb) {
-                queue = atomicExch(&stamp[w], r1) != r1;
-                farflag[w] = 2;
+                queue = atomicExch(&rec[w].aux, r1) != r1;
+                rec[w].flag = 2;
             } else if (nb < WL) {
                 // into the window from "never far" or from pile 1; retried
                 // when another lane moves the flag in between (0 -> 3 by a
                 // larger sum that lowered d(w) first), so a vertex below the
                 // window's end never stays in pile 1
-                uint32_t f = farflag[w];
+                uint32_t f = rec[w].flag;
                 while (f == 0u || f == 3u) {
-                    const uint32_t o = atomicCAS(&farflag[w], f, 1u);
+                    const uint32_t o = atomicCAS(&rec[w].flag, f, 1u);
                     if (o == f) {
                         fwin = true;
                         break;
@@ -364,7 +386,7 @@ struct WbcRelax {
                 }
                 fmw = nb < fmw ? nb : fmw;
             } else {
-                fovf = atomicCAS(&farflag[w], 0u, 3u) == 0u;
+                fovf = atomicCAS(&rec[w].flag, 0u, 3u) == 0u;
                 fmo = nb < fmo ? nb : fmo;
             }
         }
@@ -379,7 +401,7 @@ struct WbcRelax {
     {
         w = a.out_col[e];
         const double wt = a.out_w[e];
-        return arc_loaded(d, w, wt, dist[w], first, queue, fwin, fovf);
+        return arc_loaded(d, w, wt, rec[w].dist, first, queue, fwin, fovf);
     }
 
     // Marks the candidate arcs of a wavefront step over the arcs [k0, k0+64):
@@ -450,7 +472,7 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
         const bool valid = idx < s_hi[s];
         WbcRelax R(a, s, r);
         const uint32_t v = valid ? a.fr[((size_t)cur * a.S + s) * a.V + idx] : 0;
-        const double dv = valid ? zg_bits_to_f64(R.dist[v]) : 0.0;  // latest value
+        const double dv = valid ? zg_bits_to_f64(R.rec[v].dist) : 0.0;  // latest value
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
         scanned += deg;
         for (uint64_t wide = __ballot(deg > ZG_W_WIDE); wide; wide &= wide - 1) {
@@ -485,7 +507,7 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
                 wt[t] = k0 + t < mine ? a.out_w[e0 + k0 + t] : 0.0;
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) dw[t] = k0 + t < mine ? R.dist[w[t]] : 0ull;
+            for (int t = 0; t < 4; t++) dw[t] = k0 + t < mine ? R.rec[w[t]].dist : 0ull;
             // candidate bits of the four arcs: the word of arc e0+k0 and the next
             const uint32_t eb = e0 + k0;
             uint32_t m0 = 0, m1 = 0;
@@ -526,7 +548,7 @@ __global__ __launch_bounds__(256) void wbc_hub_kernel(WbcDev a, int r)
         WbcRelax R(a, s, r);
         const uint32_t b1 = a.out_ptr[v + 1];
         const uint32_t e0 = a.out_ptr[v] + piece * ZG_W_PIECE, e1 = min(b1, e0 + ZG_W_PIECE);
-        R.wave_range(a, zg_bits_to_f64(R.dist[v]), e0, e1, lane);
+        R.wave_range(a, zg_bits_to_f64(R.rec[v].dist), e0, e1, lane);
         R.flush(a, s, lane);
     }
 }
@@ -553,10 +575,10 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
-        const unsigned long long *dist = a.dist + WROW(s);
-        uint32_t *indeg = a.indeg + WROW(s), *tb = a.tbits + s * a.tw;
+        zg_wrec *rec = a.rec + WROW(s);
+        uint32_t *tb = a.tbits + s * a.tw;
         const uint32_t v = valid ? a.reach[WROW(s) + idx] : 0, src = a.src[s];
-        const double dv = valid ? zg_bits_to_f64(dist[v]) : 0.0;
+        const double dv = valid ? zg_bits_to_f64(rec[v].dist) : 0.0;
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
         arcs += deg;
         reached += valid ? 1 : 0;
@@ -582,12 +604,12 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
                     wt[t] = t < n ? a.out_w[e[t]] : 0.0;
                 }
 #pragma unroll
-                for (int t = 0; t < 4; t++) dw[t] = t < n ? dist[w[t]] : 0ull;
+                for (int t = 0; t < 4; t++) dw[t] = t < n ? rec[w[t]].dist : 0ull;
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     if (t < n) {
                         if (w[t] != vv && w[t] != src && zg_f64_to_bits(d + wt[t]) == dw[t])
-                            atomicAdd(&indeg[w[t]], 1u);
+                            atomicAdd(&rec[w[t]].aux, 1u);
                         else
                             atomicAnd(&tb[e[t] >> 5], ~(1u << (e[t] & 31)));
                     }
@@ -639,11 +661,11 @@ __global__ __launch_bounds__(256) void wbc_peel_kernel(WbcDev a, int r)
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
-        double *sig = a.sig + WROW(s);
-        uint32_t *indeg = a.indeg + WROW(s), *qlen = &a.qlen[s], *order = a.order + WROW(s);
+        zg_wrec *rec = a.rec + WROW(s);
+        uint32_t *qlen = &a.qlen[s], *order = a.order + WROW(s);
         const uint32_t *tb = a.tbits + s * a.tw;
         const uint32_t v = valid ? order[idx] : 0;
-        const double sv = valid ? sig[v] : 0.0;
+        const double sv = valid ? rec[v].sig : 0.0;
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
         // one tight arc per lane and step; every lane takes part in the append
         auto walk = [&](WbcTightIt it, double sg) {
@@ -655,8 +677,8 @@ __global__ __launch_bounds__(256) void wbc_peel_kernel(WbcDev a, int r)
                 uint32_t w = 0;
                 if (has) {
                     w = a.out_col[e];
-                    atomicAdd(&sig[w], sg);
-                    ready = atomicSub(&indeg[w], 1u) == 1u;
+                    atomicAdd(&rec[w].sig, sg);
+                    ready = atomicSub(&rec[w].aux, 1u) == 1u;
                 }
                 wbc_append(ready, w, qlen, order, lane);
             }
@@ -708,7 +730,7 @@ __global__ __launch_bounds__(256) void wbc_backward_kernel(WbcDev a, int r)
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
-        double *cc = a.cc + WROW(s);
+        zg_wrec *rec = a.rec + WROW(s);
         const uint32_t *tb = a.tbits + s * a.tw;
         const uint32_t v = valid ? a.order[WROW(s) + idx] : 0, src = a.src[s];
         const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
@@ -720,7 +742,7 @@ __global__ __launch_bounds__(256) void wbc_backward_kernel(WbcDev a, int r)
             for (uint32_t k0 = 0, nw = wbc_words(ej, dj); k0 < nw; k0 += 64) {
                 WbcTightIt it = wbc_word_share(ej, dj, k0 + lane);
                 uint32_t e;
-                while (wbc_next_tight(it, tb, e)) part += cc[a.out_col[e]];
+                while (wbc_next_tight(it, tb, e)) part += rec[a.out_col[e]].cc;
             }
             part = zg_wave_sum(part);
             if (lane == j) S = part;
@@ -728,12 +750,12 @@ __global__ __launch_bounds__(256) void wbc_backward_kernel(WbcDev a, int r)
         if (deg <= ZG_W_WIDE) {
             WbcTightIt it{e0, e0 + deg, 0, 0};
             uint32_t e;
-            while (wbc_next_tight(it, tb, e)) S += cc[a.out_col[e]];
+            while (wbc_next_tight(it, tb, e)) S += rec[a.out_col[e]].cc;
         }
         if (valid) {
-            const double sv = a.sig[WROW(s) + v];
+            const double sv = rec[v].sig;
             const double delta = sv * S;
-            cc[v] = (1.0 + delta) / sv;
+            rec[v].cc = (1.0 + delta) / sv;
             if (v != src) atomicAdd(&a.bc[v], delta);
         }
     }
@@ -748,10 +770,10 @@ __global__ void wbc_out_kernel(uint32_t V, const double *bc, const uint8_t *live
 int zg_wbc_ws_release(zg_graph *g)
 {
     zg_wbc_ws &w = g->wbc;
-    zg_dfree(w.dist); zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.indeg); zg_dfree(w.stamp); zg_dfree(w.fr);
+    zg_dfree(w.rec); zg_dfree(w.fr);
     zg_dfree(w.fcnt); zg_dfree(w.reach); zg_dfree(w.rcnt); zg_dfree(w.order); zg_dfree(w.qlen);
     zg_dfree(w.lstart); zg_dfree(w.tot); zg_dfree(w.src); zg_dfree(w.bc); zg_dfree(w.stats); zg_dfree(w.tbits);
-    zg_dfree(w.far); zg_dfree(w.farcnt); zg_dfree(w.farflag); zg_dfree(w.fsel); zg_dfree(w.split);
+    zg_dfree(w.far); zg_dfree(w.farcnt); zg_dfree(w.fsel); zg_dfree(w.split);
     zg_dfree(w.thr); zg_dfree(w.wlim); zg_dfree(w.fmin); zg_dfree(w.hcnt); zg_dfree(w.hub_v); zg_dfree(w.hub_a);
     w.S = 0;
     w.lcap = 0;
@@ -771,11 +793,7 @@ static int wbc_ws_ensure(zg_graph *g, uint32_t S)
     zg_wbc_ws_release(g);
     const size_t V = g->V ? g->V : 1, SV = (size_t)S * V;
     w.lcap = 256;
-    int rc = zg_dmalloc(&w.dist, SV);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.sig, SV);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.cc, SV);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.indeg, SV);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.stamp, SV);
+    int rc = zg_dmalloc(&w.rec, SV);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 2 * SV);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fcnt, 2 * (size_t)S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.reach, SV);
@@ -790,7 +808,6 @@ static int wbc_ws_ensure(zg_graph *g, uint32_t S)
     if (rc == ZG_OK) rc = zg_dmalloc(&w.tbits, (size_t)S * zg_wbc_words(g));
     if (rc == ZG_OK) rc = zg_dmalloc(&w.far, 4 * SV);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.farcnt, 4 * (size_t)S);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.farflag, SV);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fsel, 2 * (size_t)S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.split, 2 * (size_t)S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.thr, S);
@@ -857,12 +874,12 @@ static WbcDev wbc_dev(zg_graph *g)
     a.out_ptr = g->out_ptr;
     a.out_col = g->out_col;
     a.out_w = g->out_w;
-    a.dist = w.dist; a.sig = w.sig; a.cc = w.cc; a.indeg = w.indeg; a.stamp = w.stamp; a.fr = w.fr;
+    a.rec = w.rec; a.fr = w.fr;
     a.fcnt = w.fcnt; a.reach = w.reach; a.rcnt = w.rcnt; a.order = w.order; a.qlen = w.qlen;
     a.lstart = w.lstart; a.tot = w.tot; a.src = w.src; a.bc = w.bc; a.stats = w.stats;
     a.tbits = w.tbits;
     a.tw = zg_wbc_words(g);
-    a.far = w.far; a.farcnt = w.farcnt; a.farflag = w.farflag; a.fsel = w.fsel; a.split = w.split;
+    a.far = w.far; a.farcnt = w.farcnt; a.fsel = w.fsel; a.split = w.split;
     a.thr = w.thr; a.wlim = w.wlim; a.fmin = w.fmin;
     a.hcnt = w.hcnt; a.hub_v = w.hub_v; a.hub_a = w.hub_a; a.hcap = ZG_W_HUB_CAP;
     a.delta = zg_wbc_delta(g);
@@ -880,9 +897,9 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
     }
     // sources per batch: as many as the call has (rounded up to 64), at most
     // the tuned width (default ZG_W_MAX_SOURCES), and as many as half the
-    // free device memory holds (68 bytes per vertex and 4 per 32 arcs of
+    // free device memory holds (64 bytes per vertex and 4 per 32 arcs of
     // state per source); a larger workspace from an earlier call is kept
-    const uint64_t per_src = 68ull * V + 4ull * zg_wbc_words(g);
+    const uint64_t per_src = 64ull * V + 4ull * zg_wbc_words(g);
     const uint32_t cap = g->wbc_batch ? g->wbc_batch : ZG_W_MAX_SOURCES;
     const uint32_t want = (uint32_t)std::min<uint64_t>(cap, (n + 63) / 64 * 64);
     uint32_t S = g->wbc.S;
@@ -904,13 +921,8 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
         const uint32_t nb = (uint32_t)std::min<uint64_t>(S, n - b0);
         ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * S, s));
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
-        ZG_TRY(hipMemsetAsync(w.dist, 0xFF, sizeof(unsigned long long) * SV, s));
-        ZG_TRY(hipMemsetAsync(w.sig, 0, sizeof(double) * SV, s));
-        ZG_TRY(hipMemsetAsync(w.cc, 0, sizeof(double) * SV, s));
-        ZG_TRY(hipMemsetAsync(w.indeg, 0, sizeof(uint32_t) * SV, s));
-        ZG_TRY(hipMemsetAsync(w.stamp, 0xFF, sizeof(uint32_t) * SV, s));
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_fill_kernel, dim3(4096), dim3(256), 0, s, w.rec, SV);
         ZG_TRY(hipMemsetAsync(w.tbits, 0, sizeof(uint32_t) * S * zg_wbc_words(g), s));
-        ZG_TRY(hipMemsetAsync(w.farflag, 0, sizeof(uint32_t) * SV, s));
         WbcDev a = wbc_dev(g);
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, a);
         ZG_TRY(hipGetLastError());
@@ -934,6 +946,7 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             r += ZG_W_ROUNDS_PER_SYNC;
         }
         // 2. the tight DAG: in-arc counts, then peel rounds
+        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_aux_reset_kernel, dim3(grid), dim3(256), 0, s, a);
         ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_count_kernel, dim3(grid), dim3(256), 0, s, a);
         ZG_TRY(hipGetLastError());
         int last = -1;

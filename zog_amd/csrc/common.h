@@ -151,14 +151,22 @@ struct zg_bcq_ws {
     unsigned long long *stats = nullptr; // [2]
 };
 
+// Per-(source, vertex) record of the weighted engine: everything a lowering,
+// a tight-arc check, a path-count step or a dependency step touches for one
+// vertex sits in one 32-byte line segment.
+struct alignas(32) zg_wrec {
+    unsigned long long dist;  // distance bits (all ones = unreached)
+    double sig;               // path count
+    double cc;                // dependency term (1 + delta) / sigma
+    uint32_t aux;             // relax round that queued the vertex last; then its tight in-arcs not yet processed
+    uint32_t flag;            // far-pile flag
+};
+
 // Weighted (f64 arc weight) engine workspace (brandes_weighted.hip), S
 // sources per batch.
 struct zg_wbc_ws {
     uint32_t S = 0, lcap = 0;
-    unsigned long long *dist = nullptr;  // [S][V] distance bits (all ones = unreached)
-    double *sig = nullptr, *cc = nullptr;  // [S][V]
-    uint32_t *indeg = nullptr;           // [S][V]
-    uint32_t *stamp = nullptr;           // [S][V]
+    zg_wrec *rec = nullptr;              // [S][V]
     uint32_t *fr = nullptr;              // [2][S][V]
     uint32_t *fcnt = nullptr;            // [2][S]
     uint32_t *reach = nullptr;           // [S][V]
@@ -170,7 +178,6 @@ struct zg_wbc_ws {
     uint32_t *tbits = nullptr;           // [S][(E + 31) / 32 + 1] tight-arc bits
     uint32_t *far = nullptr;             // [2 piles][2 buffers][S][V] far piles
     uint32_t *farcnt = nullptr;          // [2][2][S]
-    uint32_t *farflag = nullptr;         // [S][V]
     uint32_t *fsel = nullptr, *split = nullptr;  // [2][S]
     unsigned long long *thr = nullptr, *wlim = nullptr;  // [S]
     unsigned long long *fmin = nullptr;  // [2][S]
