@@ -55,8 +55,9 @@ struct WbcDev {
     const uint32_t *out_ptr, *out_col;
     const double *out_w;
     zg_wrec *rec;               // [S][V] distance bits (ZG_W_UNSET = unreached), sigma, c, the relax
-                                // round that queued the vertex last (then its tight in-arcs not yet
-                                // processed), far-pile flag
+                                // round that queued the vertex last, far-pile flag (2 for every
+                                // reached vertex once the relax rounds end; then 2 + its tight
+                                // in-arcs not yet processed)
     uint32_t *fr;               // [2][S][V] relax frontiers
     uint32_t *fcnt;             // [2][S]
     uint32_t *reach;            // [S][V] reached vertices
@@ -109,27 +110,6 @@ __global__ void wbc_fill_kernel(zg_wrec *rec, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         rec[i] = zg_wrec{ZG_W_UNSET, 0.0, 0.0, 0xFFFFFFFFu, 0u};
-}
-
-// After the last relax round: the queue stamps of the reached vertices give
-// way to their tight in-arc counts (the unreached are never a tight head).
-__global__ __launch_bounds__(256) void wbc_aux_reset_kernel(WbcDev a)
-{
-    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
-    const uint32_t total = zg_q_chunks(
-        a.S,
-        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
-            lo = 0;
-            hi = a.rcnt[s];
-        },
-        s_lo, s_hi, s_pre);
-    const int lane = threadIdx.x & 63;
-    for (ZgQWalk wk = zg_q_walk(total); wk.c < wk.end; wk.c += wk.stride) {
-        const uint32_t c = wk.c;
-        const uint32_t s = zg_q_owner(s_pre, a.S, c);
-        const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
-        if (idx < s_hi[s]) a.rec[WROW(s) + a.reach[WROW(s) + idx]].aux = 0u;
-    }
 }
 
 __global__ void wbc_init_kernel(WbcDev a)
@@ -609,7 +589,7 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
                 for (int t = 0; t < 4; t++) {
                     if (t < n) {
                         if (w[t] != vv && w[t] != src && zg_f64_to_bits(d + wt[t]) == dw[t])
-                            atomicAdd(&rec[w[t]].aux, 1u);
+                            atomicAdd(&rec[w[t]].flag, 1u);
                         else
                             atomicAnd(&tb[e[t] >> 5], ~(1u << (e[t] & 31)));
                     }
@@ -678,7 +658,7 @@ __global__ __launch_bounds__(256) void wbc_peel_kernel(WbcDev a, int r)
                 if (has) {
                     w = a.out_col[e];
                     atomicAdd(&rec[w].sig, sg);
-                    ready = atomicSub(&rec[w].aux, 1u) == 1u;
+                    ready = atomicSub(&rec[w].flag, 1u) == 3u;  // its last tight in-arc
                 }
                 wbc_append(ready, w, qlen, order, lane);
             }
@@ -946,7 +926,6 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             r += ZG_W_ROUNDS_PER_SYNC;
         }
         // 2. the tight DAG: in-arc counts, then peel rounds
-        ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_aux_reset_kernel, dim3(grid), dim3(256), 0, s, a);
         ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_count_kernel, dim3(grid), dim3(256), 0, s, a);
         ZG_TRY(hipGetLastError());
         int last = -1;

@@ -158,8 +158,9 @@ struct alignas(32) zg_wrec {
     unsigned long long dist;  // distance bits (all ones = unreached)
     double sig;               // path count
     double cc;                // dependency term (1 + delta) / sigma
-    uint32_t aux;             // relax round that queued the vertex last; then its tight in-arcs not yet processed
-    uint32_t flag;            // far-pile flag
+    uint32_t aux;             // relax round that queued the vertex last
+    uint32_t flag;            // far-pile flag; 2 for a reached vertex once the relax rounds end (all piles
+                              // empty), then 2 + its tight in-arcs not yet processed
 };
 
 // Weighted (f64 arc weight) engine workspace (brandes_weighted.hip), S
