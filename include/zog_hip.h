@@ -292,7 +292,8 @@ int zg_set_batch_groups(zg_graph *g, int groups);
 #define ZG_TUNE_PR_CACHE 5
 #define ZG_TUNE_PR_GRID 6
 /*   ZG_TUNE_WEIGHTED_BATCH: sources per batch of zg_bc_f64, a multiple of 64
- *   up to 1024 (0 = default: up to 1024, as device memory allows). */
+ *   up to 2048 (0 = default: up to 2048, as half the free device memory
+ *   allows). */
 #define ZG_TUNE_WEIGHTED_BATCH 7
 /*   ZG_TUNE_WEIGHTED_DELTA: zg_bc_f64 relaxes, per source, only vertices
  *   whose distance is within a bound that rises in steps of this value once

@@ -7,7 +7,7 @@
 // delta(u) += (sigma_u / sigma_v)(1 + delta_v) over the shortest-path DAG in
 // reverse order; bc(v) += delta(v) for v != s, unnormalised.
 //
-// GPU form, per source (up to 1,024 sources per batch, each with its own
+// GPU form, per source (up to 2,048 sources per batch, each with its own
 // rows; one wavefront per 64 entries of one source's list, a vertex with
 // more than ZG_W_WIDE arcs walked by the whole wavefront):
 //   1. distances by frontier relaxation: every vertex whose distance dropped
@@ -36,7 +36,7 @@
 
 #include <algorithm>
 
-#define ZG_W_MAX_SOURCES 1024
+#define ZG_W_MAX_SOURCES 2048  // sources per batch (the range tables are sized to the batch in LDS)
 #define ZG_W_GRID 4096  // default workgroups of the per-round kernels
 #define ZG_W_ROUNDS_PER_SYNC 16
 #define ZG_W_WIDE 32  // vertices with more arcs are walked by a whole wavefront
@@ -84,7 +84,7 @@ struct WbcDev {
     unsigned long long *fmin;   // [2][S] least distance recorded for a pile's entry (bits)
     double delta;               // bound step
     uint32_t *hcnt;             // [1] hub pieces listed this round
-    uint32_t *hub_v, *hub_a;    // [hcap] hub vertex; source << 22 | piece
+    uint32_t *hub_v, *hub_a;    // [hcap] hub vertex; source << 21 | piece
     uint32_t hcap;
 };
 
@@ -217,7 +217,8 @@ __device__ __forceinline__ unsigned long long zg_wave_min_u64(unsigned long long
 // recorded.
 __global__ __launch_bounds__(256) void wbc_split_kernel(WbcDev a, int r, int p)
 {
-    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
+    extern __shared__ uint32_t wbc_lds[];  // per-source range tables, 3 S + 1 words
+    uint32_t *s_lo = wbc_lds, *s_hi = wbc_lds + a.S, *s_pre = wbc_lds + 2 * a.S;
     const int cur = r & 1;
     const uint32_t total = zg_q_chunks(
         a.S,
@@ -433,7 +434,8 @@ struct WbcRelax {
 // their own lane.
 __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
 {
-    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
+    extern __shared__ uint32_t wbc_lds[];  // per-source range tables, 3 S + 1 words
+    uint32_t *s_lo = wbc_lds, *s_hi = wbc_lds + a.S, *s_pre = wbc_lds + 2 * a.S;
     const int cur = r & 1;
     const uint32_t total = zg_q_chunks(
         a.S,
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(256) void wbc_relax_kernel(WbcDev a, int r)
                 const bool fits = (uint64_t)base + pieces <= a.hcap;
                 for (uint32_t k = lane; k < pieces && base + k < a.hcap; k += 64) {
                     a.hub_v[base + k] = fits ? vj : ZG_NONE;  // claimed slots of a list that is full stay empty
-                    a.hub_a[base + k] = (s << 22) | k;
+                    a.hub_a[base + k] = (s << 21) | k;  // s < 2^11, k < E / ZG_W_PIECE <= 2^21
                 }
                 if (fits) continue;
             }
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(256) void wbc_hub_kernel(WbcDev a, int r)
     for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < H; i += waves) {
         const uint32_t v = a.hub_v[i];
         if (v == ZG_NONE) continue;
-        const uint32_t item = a.hub_a[i], s = item >> 22, piece = item & ((1u << 22) - 1);
+        const uint32_t item = a.hub_a[i], s = item >> 21, piece = item & ((1u << 21) - 1);
         WbcRelax R(a, s, r);
         const uint32_t b1 = a.out_ptr[v + 1];
         const uint32_t e0 = a.out_ptr[v] + piece * ZG_W_PIECE, e1 = min(b1, e0 + ZG_W_PIECE);
@@ -539,7 +541,8 @@ __global__ __launch_bounds__(256) void wbc_hub_kernel(WbcDev a, int r)
 // one counts into its head's tight in-degree.  Also the sweep statistics.
 __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
 {
-    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
+    extern __shared__ uint32_t wbc_lds[];  // per-source range tables, 3 S + 1 words
+    uint32_t *s_lo = wbc_lds, *s_hi = wbc_lds + a.S, *s_pre = wbc_lds + 2 * a.S;
     __shared__ unsigned long long red[2][4];
     const uint32_t total = zg_q_chunks(
         a.S,
@@ -627,7 +630,8 @@ __global__ __launch_bounds__(256) void wbc_count_kernel(WbcDev a)
 // last tight in-arc this was joins round r+1.
 __global__ __launch_bounds__(256) void wbc_peel_kernel(WbcDev a, int r)
 {
-    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
+    extern __shared__ uint32_t wbc_lds[];  // per-source range tables, 3 S + 1 words
+    uint32_t *s_lo = wbc_lds, *s_hi = wbc_lds + a.S, *s_pre = wbc_lds + 2 * a.S;
     const uint32_t total = zg_q_chunks(
         a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
@@ -696,7 +700,8 @@ __global__ void wbc_close_kernel(WbcDev a, int L)
 
 __global__ __launch_bounds__(256) void wbc_backward_kernel(WbcDev a, int r)
 {
-    __shared__ uint32_t s_lo[ZG_W_MAX_SOURCES], s_hi[ZG_W_MAX_SOURCES], s_pre[ZG_W_MAX_SOURCES + 1];
+    extern __shared__ uint32_t wbc_lds[];  // per-source range tables, 3 S + 1 words
+    uint32_t *s_lo = wbc_lds, *s_hi = wbc_lds + a.S, *s_pre = wbc_lds + 2 * a.S;
     const uint32_t total = zg_q_chunks(
         a.S,
         [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
@@ -894,6 +899,7 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
     zg_wbc_ws &w = g->wbc;
     const size_t SV = (size_t)S * V;
     const uint32_t grid = g->wbc_grid ? g->wbc_grid : ZG_W_GRID;
+    const size_t lds = sizeof(uint32_t) * (3 * (size_t)S + 1);  // range tables of the per-source kernels
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
     ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 3, s));
     std::vector<uint32_t> tot(ZG_W_ROUNDS_PER_SYNC);
@@ -913,9 +919,9 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             a = wbc_dev(g);
             for (uint32_t k = 0; k < ZG_W_ROUNDS_PER_SYNC; k++) {
                 ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_round_kernel, dim3(1), dim3(1024), 0, s, a, (int)(r + k));
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k), 0);
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k), 1);
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_relax_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), lds, s, a, (int)(r + k), 0);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), lds, s, a, (int)(r + k), 1);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_relax_kernel, dim3(grid), dim3(256), lds, s, a, (int)(r + k));
                 ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_hub_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
             }
             ZG_TRY(hipGetLastError());
@@ -926,14 +932,14 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             r += ZG_W_ROUNDS_PER_SYNC;
         }
         // 2. the tight DAG: in-arc counts, then peel rounds
-        ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_count_kernel, dim3(grid), dim3(256), 0, s, a);
+        ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_count_kernel, dim3(grid), dim3(256), lds, s, a);
         ZG_TRY(hipGetLastError());
         int last = -1;
         for (uint32_t p = 0; last < 0;) {
             ZG_TRY_RC(wbc_grow(g, p + ZG_W_ROUNDS_PER_SYNC + 2, p + 2, s));
             a = wbc_dev(g);
             for (uint32_t k = 0; k < ZG_W_ROUNDS_PER_SYNC; k++) {
-                ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_peel_kernel, dim3(grid), dim3(256), 0, s, a, (int)(p + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_peel_kernel, dim3(grid), dim3(256), lds, s, a, (int)(p + k));
                 ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_close_kernel, dim3(1), dim3(1024), 0, s, a, (int)(p + k + 1));
             }
             ZG_TRY(hipGetLastError());
@@ -950,7 +956,7 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
         // 3. dependencies, rounds in reverse (round 0 is the source)
         a = wbc_dev(g);
         for (int q = last; q >= 1; q--)
-            ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, wbc_backward_kernel, dim3(grid), dim3(256), 0, s, a, q);
+            ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, wbc_backward_kernel, dim3(grid), dim3(256), lds, s, a, q);
         ZG_TRY(hipGetLastError());
         inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)last + 1);
         inf->push_levels += r;

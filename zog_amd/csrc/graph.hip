@@ -524,7 +524,7 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
         g->use_graph = value != 0.0;
         return ZG_OK;
     case ZG_TUNE_WEIGHTED_BATCH:
-        if (!(value >= 0.0 && value <= 1024.0) || value != (double)(64 * (uint32_t)(value / 64)))
+        if (!(value >= 0.0 && value <= 2048.0) || value != (double)(64 * (uint32_t)(value / 64)))
             return ZG_ERR_INVALID_ARG;
         g->wbc_batch = (uint32_t)value;
         return ZG_OK;
