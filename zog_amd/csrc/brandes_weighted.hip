@@ -334,10 +334,12 @@ struct WbcRelax {
         fmw = fmo = ZG_W_UNSET;
     }
 
-    // As arc(), with the head, the weight and a read of the head's distance
-    // already loaded (a stale read is never below the current one).  Returns
-    // whether the arc is a candidate tight arc: fl(d + weight) is at most the
-    // head's distance once lowered.  Every arc that is tight at the end is a
+    // Lowers d(w) to fl(d + weight) for an arc whose head, weight and a read
+    // of the head's distance are already loaded (a stale read is never below
+    // the current one); reports a first reach, a queueing for the next round
+    // (within the bound) and a first entry into a far pile.  Returns whether
+    // the arc is a candidate tight arc: fl(d + weight) is at most the head's
+    // distance once lowered.  Every arc that is tight at the end is a
     // candidate when its tail is relaxed at its final distance.
     __device__ __forceinline__ bool arc_loaded(double d, uint32_t w, double wt, unsigned long long dw, bool &first,
                                                bool &queue, bool &fwin, bool &fovf)
@@ -374,16 +376,6 @@ struct WbcRelax {
         return nb <= old;
     }
 
-    // lowers d(w) to fl(d + weight) for arc e; reports a first reach, a
-    // queueing for the next round (within the bound), a first entry into a far
-    // pile, and whether the arc is a candidate tight arc
-    __device__ __forceinline__ bool arc(const WbcDev &a, double d, uint32_t e, bool &first, bool &queue, bool &fwin,
-                                        bool &fovf, uint32_t &w)
-    {
-        w = a.out_col[e];
-        const double wt = a.out_w[e];
-        return arc_loaded(d, w, wt, rec[w].dist, first, queue, fwin, fovf);
-    }
 
     // Marks the candidate arcs of a wavefront step over the arcs [k0, k0+64):
     // lane i's flag is arc k0+i's.  All lanes call it.
@@ -406,15 +398,30 @@ struct WbcRelax {
         wbc_append(first, w, rcnt, reach, lane);
     }
 
-    // The whole wavefront over the arcs [e0, e1) of a vertex at distance d.
+    // The whole wavefront over the arcs [e0, e1) of a vertex at distance d,
+    // 256 arcs a step: each lane loads four heads and weights, then their
+    // distances, so four loads of each kind are in flight per lane.
     __device__ __forceinline__ void wave_range(const WbcDev &a, double d, uint32_t e0, uint32_t e1, int lane)
     {
-        for (uint32_t k0 = e0; k0 < e1; k0 += 64) {
-            bool first = false, queue = false, fwin = false, fovf = false, cand = false;
-            uint32_t w = 0;
-            if (k0 + lane < e1) cand = arc(a, d, k0 + lane, first, queue, fwin, fovf, w);
-            append(first, queue, fwin, fovf, w, lane);
-            mark_wave(cand, k0, lane);
+        for (uint32_t k0 = e0; k0 < e1; k0 += 256) {
+            uint32_t w[4];
+            double wt[4];
+            unsigned long long dw[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t e = k0 + 64 * t + lane;
+                w[t] = e < e1 ? a.out_col[e] : 0u;
+                wt[t] = e < e1 ? a.out_w[e] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) dw[t] = k0 + 64 * t + lane < e1 ? rec[w[t]].dist : 0ull;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                bool first = false, queue = false, fwin = false, fovf = false, cand = false;
+                if (k0 + 64 * t + lane < e1) cand = arc_loaded(d, w[t], wt[t], dw[t], first, queue, fwin, fovf);
+                append(first, queue, fwin, fovf, w[t], lane);
+                mark_wave(cand, k0 + 64 * t, lane);
+            }
         }
     }
 
