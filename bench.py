@@ -547,10 +547,12 @@ def main():
     w0 = time.perf_counter()
     arcs = 0
     reached = 0
+    step_kernel_ms = []  # device time of each timed sweep (the library's events)
     for i in range(args.steps):
         bc_step()
         arcs += info.arcs_traversed
         reached += info.nodes_reached
+        step_kernel_ms.append(info.kernel_ms)
         log(f"[bench] step {i} done")
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
@@ -685,6 +687,9 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3),
+        "step_kernel_ms": ({"min": round(min(step_kernel_ms), 3),
+                            "mean": round(sum(step_kernel_ms) / len(step_kernel_ms), 3), "n": len(step_kernel_ms)}
+                           if step_kernel_ms else None),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
