@@ -479,7 +479,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by source; see DESIGN notes on sharding)", file=sys.stderr)
+              "(the path would shard by source: a graph replica per GPU, the sources split, one "
+              "all-reduce of the scores)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
