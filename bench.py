@@ -308,7 +308,7 @@ def bench_weighted(args):
         log(f"[bench] oracle weighted BC {len(sample)} sources in {cpu_s:.1f}s = {cpu_gteps:.4f} GTEPS")
         cpu = {"value": round(cpu_gteps, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
                "sample": f"oracle/oracle.c weighted Brandes (restated centrality.zig:529-667, binary heap), "
-                         f"1 thread, {len(sample)} of the {len(sources)} sources, host cores of the GPU box"}
+                         f"1 thread, {len(sample)} of the {len(sources)} sources, host cores beside the GPU"}
     line = {
         "metric": "weighted Brandes GTEPS (exact betweennessF64)",
         "value": round(arcs / wall / 1e9, 3), "unit": "GTEPS", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -398,7 +398,7 @@ def bench_power(args):
         parity["eigenvector_max_abs_err_10_iterations"] = float(np.abs(got - ref).max())
         cpu = {"value": round(bytes_iter["katz"] * k / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"oracle/oracle.c Katz (restated centrality.zig:1212-1267), {k} iterations on the same "
-                         f"graph, 1 thread, host cores of the GPU box"}
+                         f"graph, 1 thread, host cores beside the GPU"}
         log(f"[bench] oracle Katz {k} iterations in {cpu_s:.1f}s")
     line = {
         "metric": "power-iteration GB/s (Katz; eigenvector and alpha beside it)",
@@ -663,7 +663,7 @@ def main():
         parity["pagerank_iterations_gpu_oracle"] = [pr_iters, ref_it]
         cpu = {"value": round(cpu_gteps, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
                "sample": f"oracle/oracle.c Brandes (restated centrality.zig:694-798), 1 thread, {len(sample)} of "
-                         f"the {len(sources)} sources of {args.workload}, host cores of the GPU box",
+                         f"the {len(sources)} sources of {args.workload}, host cores beside the GPU",
                "pagerank_gbs": round(ref_it * pr_bytes_iter / cpu_pr_s / 1e9, 3),
                "pagerank_sample": f"full FP64 oracle PageRank on {args.workload} ({ref_it} iterations, 1 thread)"}
         # the parallel mode: the same oracle with the sources split into
