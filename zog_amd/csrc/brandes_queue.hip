@@ -50,6 +50,7 @@
 struct BcqDev {
     uint32_t V, S, lcap;
     const uint32_t *out_ptr, *out_col, *in_ptr, *in_col;
+    const uint4 *adj;               // [V][4] inline adjacency (ZG_Q_INLINE), or null
     zg_qrec *rec;                   // [S][V] {depth, exponent, sig or cc}
     uint32_t *queue;                // [S][V]
     uint32_t *lstart;               // [lcap][S]
@@ -69,6 +70,58 @@ __device__ __forceinline__ double zg_scale(double x, int k)
 {
     return ldexp(x, k);
 }
+
+// Inline adjacency of a vertex with at most ZG_Q_INLINE predecessors and
+// successors: one 64-byte record {counts, predecessors in predecessor-CSR
+// order, successors in successor-CSR order}, so a level step reads one line
+// for both lists instead of the four CSR arrays' lines.  Word 0 is
+// in-degree | out-degree << 16, or ZG_NONE for a vertex the CSR serves.
+#define ZG_Q_INLINE 7
+
+__global__ void bcq_adj_kernel(uint32_t V, const uint32_t *in_ptr, const uint32_t *in_col, const uint32_t *out_ptr,
+                               const uint32_t *out_col, uint4 *adj)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const uint32_t i0 = in_ptr[v], id = in_ptr[v + 1] - i0, o0 = out_ptr[v], od = out_ptr[v + 1] - o0;
+        uint32_t w[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) w[k] = 0u;
+        if (id <= ZG_Q_INLINE && od <= ZG_Q_INLINE) {
+            w[0] = id | (od << 16);
+#pragma unroll
+            for (int k = 0; k < ZG_Q_INLINE; k++) {
+                if ((uint32_t)k < id) w[1 + k] = in_col[i0 + k];
+                if ((uint32_t)k < od) w[8 + k] = out_col[o0 + k];
+            }
+        } else {
+            w[0] = ZG_NONE;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) adj[(size_t)v * 4 + q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    }
+}
+
+// A vertex's inline record, loaded whole (four 16-byte loads of one line).
+struct ZgQAdj {
+    uint32_t w[16];
+    __device__ __forceinline__ void load(const uint4 *adj, uint32_t v)
+    {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 x = adj[(size_t)v * 4 + q];
+            w[4 * q] = x.x;
+            w[4 * q + 1] = x.y;
+            w[4 * q + 2] = x.z;
+            w[4 * q + 3] = x.w;
+        }
+    }
+    __device__ __forceinline__ bool inl() const { return w[0] != ZG_NONE; }
+    __device__ __forceinline__ uint32_t ideg() const { return w[0] & 0xFFFFu; }
+    __device__ __forceinline__ uint32_t odeg() const { return w[0] >> 16; }
+    // predecessor / successor k of batch b (k = 4 b + t, t static)
+    __device__ __forceinline__ uint32_t pred(int b, int t) const { return b == 0 ? w[1 + t] : (t < 3 ? w[5 + t] : 0u); }
+    __device__ __forceinline__ uint32_t succ(int b, int t) const { return b == 0 ? w[8 + t] : (t < 3 ? w[12 + t] : 0u); }
+};
 
 __global__ void bcq_init_kernel(BcqDev a)
 {
@@ -111,13 +164,44 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
         const bool valid = idx < s_hi[s];
         zg_qrec *rec = a.rec + QROW(s);
         const uint32_t v = valid ? a.queue[QROW(s) + idx] : 0;
+        ZgQAdj J;
+        bool inl = false;
+        if (valid && a.adj) {
+            J.load(a.adj, v);
+            inl = J.inl();
+        }
         if (d > 0) {
             // path count of v from its predecessors at depth d-1
-            const uint32_t i0 = valid ? a.in_ptr[v] : 0, ideg = valid ? a.in_ptr[v + 1] - i0 : 0;
+            const uint32_t i0 = valid && !inl ? a.in_ptr[v] : 0;
+            const uint32_t ideg = !valid ? 0 : inl ? J.ideg() : a.in_ptr[v + 1] - i0;
             // narrow rows: one lane, the sum rescaled whenever a larger exponent appears
             int E = 0;
             double x = 0.0;
-            if (ideg <= ZG_Q_WIDE) {
+            auto fold = [&](const zg_qrec &q) {
+                if (q.depth != dprev) return;
+                const int eu = (int)q.exp;
+                if (x == 0.0) E = eu;
+                else if (eu > E) {
+                    x = zg_scale(x, E - eu);
+                    E = eu;
+                }
+                x += zg_scale(q.val, eu - E);
+            };
+            if (inl) {
+                // the record's predecessors, four at a time, in the same order
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    if ((uint32_t)(4 * b) >= ideg) break;
+                    uint32_t u[4];
+                    zg_qrec q[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) u[t] = (uint32_t)(4 * b + t) < ideg ? J.pred(b, t) : 0u;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) q[t] = (uint32_t)(4 * b + t) < ideg ? rec[u[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
+#pragma unroll
+                    for (int t = 0; t < 4; t++) fold(q[t]);
+                }
+            } else if (ideg <= ZG_Q_WIDE) {
                 // four arcs at a time: their loads overlap instead of chaining
                 for (uint32_t k0 = 0; k0 < ideg; k0 += 4) {
                     uint32_t u[4];
@@ -127,16 +211,7 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
 #pragma unroll
                     for (int t = 0; t < 4; t++) q[t] = k0 + t < ideg ? rec[u[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
 #pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        if (q[t].depth != dprev) continue;
-                        const int eu = (int)q[t].exp;
-                        if (x == 0.0) E = eu;
-                        else if (eu > E) {
-                            x = zg_scale(x, E - eu);
-                            E = eu;
-                        }
-                        x += zg_scale(q[t].val, eu - E);
-                    }
+                    for (int t = 0; t < 4; t++) fold(q[t]);
                 }
             }
             // wide rows: the whole wavefront, largest exponent first, then the sum
@@ -166,7 +241,8 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
                 rec[v] = zg_qrec{(uint32_t)d, (uint32_t)(E + k), ldexp(x, -k)};  // depth unchanged
             }
         }
-        const uint32_t e0 = valid ? a.out_ptr[v] : 0, deg = valid ? a.out_ptr[v + 1] - e0 : 0;
+        const uint32_t e0 = valid && !inl ? a.out_ptr[v] : 0;
+        const uint32_t deg = !valid ? 0 : inl ? J.odeg() : a.out_ptr[v + 1] - e0;
         if (valid && d > 0) {  // level 0 (the source) is counted by bcq_init_kernel
             arcs += deg;
             reached += 1;
@@ -178,7 +254,8 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
             // the heads and their depths of four arcs first, then the claims
             uint32_t w[4], dw[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) w[t] = k0 + t < deg ? a.out_col[e0 + k0 + t] : 0u;
+            for (int t = 0; t < 4; t++)
+                w[t] = k0 + t < deg ? (inl ? J.succ((int)(k0 >> 2), t) : a.out_col[e0 + k0 + t]) : 0u;
 #pragma unroll
             for (int t = 0; t < 4; t++) dw[t] = k0 + t < deg ? rec[w[t]].depth : 0u;
 #pragma unroll
@@ -247,18 +324,42 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
         const zg_qrec me = rec[v];
         const int ev = (int)me.exp;
         double S = 0.0;
-        for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e += 4) {
-            // four arcs at a time: heads, then the records (depth, exponent
-            // and, at depth d+1, the dependency term)
-            uint32_t w[4];
-            zg_qrec q[4];
+        ZgQAdj J;
+        bool inl = false;
+        if (a.adj) {
+            J.load(a.adj, v);
+            inl = J.inl();
+        }
+        if (inl) {
+            // the record's successors, four at a time, in the same order
+            const uint32_t od = J.odeg();
 #pragma unroll
-            for (int t = 0; t < 4; t++) w[t] = e + t < e1 ? a.out_col[e + t] : 0u;
+            for (int b = 0; b < 2; b++) {
+                if ((uint32_t)(4 * b) >= od) break;
+                uint32_t w[4];
+                zg_qrec q[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) q[t] = e + t < e1 ? rec[w[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
+                for (int t = 0; t < 4; t++) w[t] = (uint32_t)(4 * b + t) < od ? J.succ(b, t) : 0u;
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                if (q[t].depth == dnext) S += zg_scale(q[t].val, ev - (int)q[t].exp);
+                for (int t = 0; t < 4; t++) q[t] = (uint32_t)(4 * b + t) < od ? rec[w[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (q[t].depth == dnext) S += zg_scale(q[t].val, ev - (int)q[t].exp);
+            }
+        } else {
+            for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e += 4) {
+                // four arcs at a time: heads, then the records (depth, exponent
+                // and, at depth d+1, the dependency term)
+                uint32_t w[4];
+                zg_qrec q[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) w[t] = e + t < e1 ? a.out_col[e + t] : 0u;
+#pragma unroll
+                for (int t = 0; t < 4; t++) q[t] = e + t < e1 ? rec[w[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (q[t].depth == dnext) S += zg_scale(q[t].val, ev - (int)q[t].exp);
+            }
         }
         const double sv = me.val;
         const double delta = sv * S;
@@ -348,6 +449,7 @@ static BcqDev bcq_dev(zg_graph *g, bool weighted)
     a.out_col = g->out_col;
     a.in_ptr = g->in_ptr;
     a.in_col = g->in_col;
+    a.adj = g->qadj;
     a.rec = w.rec;
     a.queue = w.queue;
     a.lstart = w.lstart;
@@ -379,6 +481,12 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
     }
     ZG_TRY_RC(bcq_ws_ensure(g, S));
     zg_bcq_ws &w = g->bcq;
+    if (!g->qadj && V) {  // the inline adjacency, once per resource
+        ZG_TRY_RC(zg_dmalloc(&g->qadj, (size_t)V * 4));
+        hipLaunchKernelGGL(bcq_adj_kernel, dim3(std::min<uint32_t>((V + 255) / 256, 8192)), dim3(256), 0, s, V,
+                           g->in_ptr, g->in_col, g->out_ptr, g->out_col, g->qadj);
+        ZG_TRY(hipGetLastError());
+    }
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
     std::vector<uint32_t> newcnt;
     for (uint64_t b0 = 0; b0 < n; b0 += S) {

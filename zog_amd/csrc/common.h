@@ -251,6 +251,7 @@ struct zg_graph {
     int groups = 4;
     int engine = 0;                      // ZG_ENGINE_*
     double *out_w = nullptr;             // [E] arc weights in successor-CSR order (weighted resources)
+    uint4 *qadj = nullptr;               // [V][4] inline adjacency of the queue engine (on demand)
     bool weights_ok = true;              // every live arc's weight is finite and non-negative
     bool built_on_gpu = false;           // the CSR came from zg_build_csr_gpu
     uint32_t max_in_deg = 0;             // largest in-degree

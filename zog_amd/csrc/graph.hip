@@ -113,6 +113,7 @@ static void zg_free_device(zg_graph *g)
     zg_wbc_ws_release(g);
     zg_pr_ws_release(g);
     zg_dfree(g->out_ptr); zg_dfree(g->out_col); zg_dfree(g->in_ptr); zg_dfree(g->in_col); zg_dfree(g->out_w);
+    zg_dfree(g->qadj);
     zg_dfree(g->und_ptr); zg_dfree(g->und_col);
     zg_dfree(g->live);
     zg_dfree(g->ich_v); zg_dfree(g->ich_e0); zg_dfree(g->ich_e1); zg_dfree(g->inhub);
