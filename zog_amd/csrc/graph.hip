@@ -206,6 +206,17 @@ static int zg_build_csr_host(zg_graph *g, const uint32_t *from, const uint32_t *
     return ZG_OK;
 }
 
+// One wavefront per row: copies row r's n[r] columns from src[r] to dst[r].
+__global__ void zg_rebin_kernel(uint32_t rows, const uint32_t *src, const uint32_t *dst, const uint32_t *n,
+                                const uint32_t *col, uint32_t *out)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < rows; r += (gridDim.x * blockDim.x) >> 6) {
+        const uint32_t a = src[r], b = dst[r], k = n[r];
+        for (uint32_t i = lane; i < k; i += 64) out[b + i] = col[a + i];
+    }
+}
+
 static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
                     const double *weight, uint64_t m)
 {
@@ -332,6 +343,58 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
                                       out_ptr[(size_t)v + 1] - out_ptr[v]));
     g->n_large = (uint32_t)large.size();
     g->n_seg = (uint32_t)seg_row.size();
+    // PageRank columns in row-bin order (small rows, then the medium bins,
+    // then the long rows): the rows a wavefront sums take neighbouring column
+    // words instead of words scattered over the predecessor CSR; each row
+    // keeps its arc order, so its sum is unchanged
+    if (E) {
+        std::vector<uint32_t> rb_src, rb_dst, rb_len;
+        rb_src.reserve(desc.size() + large.size());
+        rb_dst.reserve(desc.size() + large.size());
+        rb_len.reserve(desc.size() + large.size());
+        uint32_t pos = 0;
+        for (uint4 &d : desc) {
+            rb_src.push_back(d.y);
+            rb_dst.push_back(pos);
+            rb_len.push_back(d.z);
+            d.y = pos;
+            pos += d.z;
+        }
+        for (size_t j = 0; j < large.size(); j++) {
+            const uint32_t k0 = large_seg0[j], k1 = large_seg0[j + 1];
+            const uint32_t a0 = seg_e0[k0], len = seg_e1[k1 - 1] - a0;
+            rb_src.push_back(a0);
+            rb_dst.push_back(pos);
+            rb_len.push_back(len);
+            for (uint32_t k = k0; k < k1; k++) {
+                seg_e0[k] = seg_e0[k] - a0 + pos;
+                seg_e1[k] = seg_e1[k] - a0 + pos;
+            }
+            pos += len;
+        }
+        if ((uint64_t)pos != E) return ZG_ERR_INVALID_ARG;  // every arc has a live row
+        uint32_t *d_src = nullptr, *d_dst = nullptr, *d_len = nullptr, *binned = nullptr;
+        int rc = zg_upload(&d_src, rb_src);
+        if (rc == ZG_OK) rc = zg_upload(&d_dst, rb_dst);
+        if (rc == ZG_OK) rc = zg_upload(&d_len, rb_len);
+        if (rc == ZG_OK) rc = zg_dmalloc(&binned, E);
+        if (rc == ZG_OK) {
+            const uint32_t rows = (uint32_t)rb_src.size();
+            hipLaunchKernelGGL(zg_rebin_kernel, dim3(std::min<uint32_t>((rows + 3) / 4, 16384)), dim3(256), 0, g->stream,
+                               rows, d_src, d_dst, d_len, g->pr_col, binned);
+            rc = zg_hip_code(hipGetLastError());
+        }
+        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(g->stream));
+        zg_dfree(d_src);
+        zg_dfree(d_dst);
+        zg_dfree(d_len);
+        if (rc != ZG_OK) {
+            zg_dfree(binned);
+            return rc;
+        }
+        zg_dfree(g->pr_col);
+        g->pr_col = binned;
+    }
 
     g->h_in_ptr = in_ptr;
     ZG_TRY_RC(zg_upload(&g->ich_v, ich_v));
