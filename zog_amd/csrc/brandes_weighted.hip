@@ -18,19 +18,27 @@
 //      frontier runs dry the bound rises by delta (or to the least far
 //      distance) and the far entries under it form the next frontier (the
 //      near-far form of delta-stepping), which saves relaxing a vertex again
-//      and again on the way down to its distance.  It ends at the least fixed
-//      point d(w) = min over arcs of fl(d(v) + w(v,w)), the distances
-//      Dijkstra computes with the same additions.
+//      and again on the way down to its distance.  The far pile has two
+//      tiers: a window below the bound + ZG_W_WINDOW steps, split at every
+//      raise, and an overflow pile scanned only when the window runs dry or
+//      the bound passes its end.  It ends at the least fixed point d(w) =
+//      min over arcs of fl(d(v) + w(v,w)), the distances Dijkstra computes
+//      with the same additions.  Every arc whose sum was at most its head's
+//      distance when relaxed is marked a candidate tight arc.
 //   2. the tight arcs (fl(d(v) + w) == d(w) bit for bit, v != w, w not the
 //      source; with positive weights neither exception is ever tight) form the
-//      shortest-path DAG: count their in-arcs, then peel it in rounds (a
-//      vertex enters the round after its last tight in-arc was processed),
-//      adding sigma(v) into every tight successor.
+//      shortest-path DAG: the candidates are checked against the final
+//      distances and their in-arcs counted, then the DAG is peeled in rounds
+//      (a vertex enters the round after its last tight in-arc was
+//      processed), adding sigma(v) into every tight successor.
 //   3. rounds in reverse: delta(v) = sigma(v) * sum over tight successors of
 //      c(w), c(v) = (1 + delta(v)) / sigma(v); bc(v) += delta(v) (FP64
 //      atomics across sources).
 // Arc weights must be positive for the reference's semantics; a zero-weight
-// cycle would never be peeled (its vertices get no sigma).
+// cycle would never be peeled (its vertices get no sigma).  Everything one
+// step touches for a (source, vertex) pair sits in one 32-byte record
+// (zg_wrec), and the per-source kernels walk their chunks XCD by XCD
+// (zg_q_walk).
 #include "common.h"
 #include "queue_map.h"
 
