@@ -306,9 +306,12 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
         g->pr_xcache_useful = E > 0 && hot * 10 >= E;
     }
 
-    // PageRank row bins over live rows by predecessor count.
+    // PageRank row bins over live rows by predecessor count, each bin in
+    // PageRank-id order, so the rows a wavefront finishes read and write
+    // neighbouring rank words.
     std::vector<uint32_t> small, m4, m16, m64, large, large_deg, large_seg0, seg_row, seg_e0, seg_e1;
-    for (uint32_t v = 0; v < V; v++) {
+    for (uint32_t i = 0; i < V; i++) {
+        const uint32_t v = pr_order[i];
         if (!g->h_live[v]) continue;
         uint32_t d = in_ptr[(size_t)v + 1] - in_ptr[v];
         if (d <= ZG_PR_SMALL) small.push_back(v);
