@@ -297,7 +297,7 @@ int zg_set_batch_groups(zg_graph *g, int groups);
 #define ZG_TUNE_WEIGHTED_BATCH 7
 /*   ZG_TUNE_WEIGHTED_DELTA: zg_bc_f64 relaxes, per source, only vertices
  *   whose distance is within a bound that rises in steps of this value once
- *   nothing under it is left (0 = default: 0.6 x mean arc weight / mean
+ *   nothing under it is left (0 = default: 0.3 x mean arc weight / mean
  *   out-degree; +inf = no bound).  Only the amount of work changes. */
 #define ZG_TUNE_WEIGHTED_DELTA 8
 /*   ZG_TUNE_WEIGHTED_GRID: workgroups of zg_bc_f64's per-round kernels
