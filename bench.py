@@ -583,12 +583,17 @@ def main():
         dom, dom_bytes, dom_ms, dom_n = "bc_backward_kernel", bwd_bytes, bwd_ms, bwd_n
     achieved = dom_bytes / (dom_ms / 1000.0) / 1e9
     traffic = None
+    pr_traffic = None  # the FP32 PageRank iteration kernel's, from the same counter runs
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            ent = json.load(fh).get(dom)
+            tjson = json.load(fh)
+        ent = tjson.get(dom)
         if ent:
             traffic = round(ent["hbm_bytes_per_dispatch"])
+        ent = tjson.get("pr_iter_kernel<float, 0>")
+        if ent and spec["pr"]["precision"] == 32:
+            pr_traffic = round(ent["hbm_bytes_per_dispatch"])
     measured = {}
     if traffic:
         mgbs = traffic * dom_n / (dom_ms / 1000.0) / 1e9
@@ -706,7 +711,7 @@ def main():
                      "dtype": {32: "f32 (Kahan rows, f64 reductions)", 64: "f64",
                                96: "f32 Kahan then f64"}[prs["precision"]],
                      "roofline": {"bound": "hbm", "achieved": round(pr_ach, 1), "peak": HBM_PEAK_GBS,
-                                  "unit": "GB/s", "frac": round(pr_ach / HBM_PEAK_GBS, 4), "traffic": None,
+                                  "unit": "GB/s", "frac": round(pr_ach / HBM_PEAK_GBS, 4), "traffic": pr_traffic,
                                   "kernel": "pr_iter_kernel+pr_finish_kernel",
                                   "avg_launch_us": round(it_us, 2),
                                   "gather_floor_us": round(floor_us.value, 2),
