@@ -71,6 +71,8 @@ struct BcDev {
     double *sig, *cc, *hubacc, *bcg;   // cc: [2][Gcap][V][64]
     uint32_t *dep;
     uint32_t *maxdep;           // [Gcap][V]
+    uint64_t *lvl;              // [lmcap][Gcap][V] lanes whose level-L frontier held v
+    uint32_t lmcap;
     uint64_t *vis, *fr;         // fr: [3][Gcap][V]
     unsigned long long *stats;  // [Gcap][ZG_FIN_BLOCKS][2]
     uint32_t *touched;          // [Gcap][V]
@@ -98,6 +100,9 @@ struct BcDev {
 
 // Levels queued per host sync: the first chunk, doubling up to the cap.
 #define ZG_LEVEL_CHUNK0 12
+// Lane table of the backward pass: at most this many levels and bytes.
+#define ZG_LVL_LEVELS 32
+#define ZG_LVL_BYTES ((size_t)2 << 30)
 #define ZG_LEVEL_CHUNK_MAX 64
 // Fixed grid of the push kernel (it strides over the level's entries).
 #define ZG_PUSH_GRID 1024
@@ -115,6 +120,7 @@ struct BcLevelArgs {
 #define MASK(g, v) ((size_t)(g) * a.Vloc + ((v) - a.vlo[g]))
 #define ROW(g, v) (MASK(g, v) * 64)
 #define EMAX(L, g, l) ((((size_t)(L)) * a.Gcap + (g)) * 64 + (l))
+#define LVLM(L, g, v) ((size_t)(L) * a.Gcap * a.Vloc + MASK(g, v))
 
 __device__ __forceinline__ uint64_t *fr_buf(const BcDev &a, int k)
 {
@@ -472,6 +478,7 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
         if (!nm) continue;
         a.vis[MASK(g, v)] |= nm;
         a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);  // levels only grow
+        if ((uint32_t)(d + 1) < a.lmcap) a.lvl[LVLM(d + 1, g, v)] = nm;
         if (a.dsum) bc_count_lanes(lcnt, nm);
         const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
         nf += kf;
@@ -607,6 +614,7 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
         const uint64_t nm = next[MASK(g, v)];
         a.vis[MASK(g, v)] |= nm;
         a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);
+        if ((uint32_t)(d + 1) < a.lmcap) a.lvl[LVLM(d + 1, g, v)] = nm;
         if (a.dsum) bc_count_lanes(lcnt, nm);
         const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
         nf += kf;
@@ -734,36 +742,42 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
     const bool active = (mask >> lane) & 1;
     const double *cg = cc_buf(a, (d + 1) & 1);
-    // Only successors that some lane reached at depth >= d+1 can be BFS
-    // children; the others' rows are not read.
+    // A successor w is a BFS child of v in lane l iff l's level-(d+1)
+    // frontier held w; the other lanes' entries of w's row are zero.  With
+    // the level's lane table, only child lanes read; past the table, every
+    // active lane reads the rows of successors that some lane reached at
+    // depth >= d+1.
+    const bool exact = (uint32_t)(d + 1) < a.lmcap;
     double S = 0.0;
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
         uint32_t w = 0;
-        bool keep = false;
+        uint64_t wm = 0;
         if (e < e1) {
             w = a.out_col[e];
-            keep = a.maxdep[MASK(g, w)] >= (uint32_t)(d + 1);
+            if (exact) wm = a.lvl[LVLM(d + 1, g, w)] & mask;
+            else wm = a.maxdep[MASK(g, w)] >= (uint32_t)(d + 1) ? mask : 0;
         }
-        uint64_t act = __ballot(keep);
+        uint64_t act = __ballot(wm != 0);
         // eight row reads in flight per round
         while (act) {
             uint32_t wq[8];
-            bool okq[8];
+            uint64_t mq[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                okq[q] = act != 0;
                 if (act) {
                     const int j = __builtin_ctzll(act);
                     act &= act - 1;
                     wq[q] = zg_readlane32(w, j);
+                    mq[q] = zg_readlane64(wm, j);
                 } else {
                     wq[q] = 0;
+                    mq[q] = 0;
                 }
             }
             double val[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) val[q] = (active && okq[q]) ? cg[ROW(g, wq[q]) + lane] : 0.0;
+            for (int q = 0; q < 8; q++) val[q] = ((mq[q] >> lane) & 1) ? cg[ROW(g, wq[q]) + lane] : 0.0;
 #pragma unroll
             for (int q = 0; q < 8; q++) S += val[q];
         }
@@ -828,7 +842,7 @@ __global__ void bc_export_kernel(uint32_t V, uint32_t Vloc, uint32_t vlo, uint32
 int zg_bc_ws_release(zg_graph *g)
 {
     zg_bc_ws &w = g->bc;
-    zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep); zg_dfree(w.maxdep);
+    zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep); zg_dfree(w.maxdep); zg_dfree(w.lvl);
     zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.touched); zg_dfree(w.flist); zg_dfree(w.fmask);
     zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.foff); zg_dfree(w.hoff); zg_dfree(w.modes); zg_dfree(w.src); zg_dfree(w.cnt);
     zg_dfree(w.wts); zg_dfree(w.dsum);
@@ -844,6 +858,7 @@ int zg_bc_ws_release(zg_graph *g)
     w.h_cnt = nullptr;
     w.groups = 0;
     w.lcap = 0;
+    w.lmcap = 0;
     w.vloc = 0;
     w.fcap = w.hcap = 0;
     w.with_depth = false;
@@ -904,6 +919,11 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
     if (rc == ZG_OK) rc = zg_dmalloc(&w.bcg, G * V);
     if (rc == ZG_OK && with_depth) rc = zg_dmalloc(&w.dep, G * VL * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.maxdep, G * VL);
+    // per-level lane masks for the backward pass, up to ZG_LVL_LEVELS levels
+    // and ZG_LVL_BYTES in all (levels past the table use maxdep alone)
+    w.lmcap = (uint32_t)std::min<size_t>(ZG_LVL_LEVELS, ZG_LVL_BYTES / (G * VL * sizeof(uint64_t)));
+    if (w.lmcap < 3) w.lmcap = 0;
+    if (rc == ZG_OK && w.lmcap) rc = zg_dmalloc(&w.lvl, (size_t)w.lmcap * G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * 2);
@@ -1041,6 +1061,7 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used, const GroupSpan &sp, cons
     a.ohub_idx = g->ohub_idx;
     a.live = g->live;
     a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep; a.maxdep = w.maxdep;
+    a.lvl = w.lvl; a.lmcap = w.lvl ? w.lmcap : 0;
     a.vis = w.vis; a.fr = w.fr; a.stats = w.stats; a.touched = w.touched;
     a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
     a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
@@ -1095,6 +1116,9 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         if (opt.h_totals) ZG_TRY(hipMemsetAsync(w.dsum, 0, sizeof(unsigned long long) * Gcap * 64, s));
         ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
         ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
+        if (w.lvl)
+            ZG_TRY(hipMemsetAsync(w.lvl, 0,
+                                  sizeof(uint64_t) * (size_t)std::min(w.last_levels + 2, w.lmcap) * Gcap * w.vloc, s));
         if (max_v) {
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reset_kernel, dim3(cdiv(max_v, 256), G), dim3(256), 0, s, a, sp);
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_rows_kernel, dim3(cdiv(max_v, 4), G), dim3(256), 0, s, a, sp);

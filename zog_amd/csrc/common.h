@@ -93,6 +93,8 @@ struct zg_bc_ws {
     double *sig = nullptr, *cc = nullptr, *hubacc = nullptr, *bcg = nullptr;
     uint32_t *dep = nullptr;
     uint32_t *maxdep = nullptr;          // [G][V] deepest level of v over the group's lanes
+    uint64_t *lvl = nullptr;             // [lmcap][G][V] lanes that reached v at each level
+    uint32_t lmcap = 0;                  // levels the lane table holds (0: none)
     uint64_t *vis = nullptr, *fr = nullptr;
     unsigned long long *stats = nullptr; // [G][ZG_FIN_BLOCKS][2] arcs, reached
     uint32_t *touched = nullptr;         // [G][V] vertices reached by a push level
