@@ -133,8 +133,9 @@ __device__ __forceinline__ double *cc_buf(const BcDev &a, int k)
 }
 
 // Batch reset over a group's vertex span: the visited mask starts as the
-// group's unused lanes, the frontier buffers as empty.
-__global__ void bc_reset_kernel(BcDev a, GroupSpan sp)
+// group's unused lanes, the frontier buffers and the first nlev levels of
+// the lane table as empty.
+__global__ void bc_reset_kernel(BcDev a, GroupSpan sp, uint32_t nlev)
 {
     const uint32_t g = blockIdx.y;
     const uint32_t v = sp.vlo[g] + blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,6 +146,7 @@ __global__ void bc_reset_kernel(BcDev a, GroupSpan sp)
     fr_buf(a, 0)[MASK(g, v)] = 0;
     fr_buf(a, 1)[MASK(g, v)] = 0;
     fr_buf(a, 2)[MASK(g, v)] = 0;
+    for (uint32_t L = 0; L < nlev; L++) a.lvl[LVLM(L, g, v)] = 0;
 }
 
 // Path-count rows of a group's span start at zero (push levels and
@@ -1116,11 +1118,9 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         if (opt.h_totals) ZG_TRY(hipMemsetAsync(w.dsum, 0, sizeof(unsigned long long) * Gcap * 64, s));
         ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
         ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
-        if (w.lvl)
-            ZG_TRY(hipMemsetAsync(w.lvl, 0,
-                                  sizeof(uint64_t) * (size_t)std::min(w.last_levels + 2, w.lmcap) * Gcap * w.vloc, s));
+        const uint32_t nlev = a.lmcap ? std::min(w.last_levels + 2, a.lmcap) : 0;
         if (max_v) {
-            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reset_kernel, dim3(cdiv(max_v, 256), G), dim3(256), 0, s, a, sp);
+            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reset_kernel, dim3(cdiv(max_v, 256), G), dim3(256), 0, s, a, sp, nlev);
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_rows_kernel, dim3(cdiv(max_v, 4), G), dim3(256), 0, s, a, sp);
         }
         ZG_TRY(hipMemsetAsync(w.foff, 0, sizeof(uint32_t) * Gcap, s));  // level 0 starts at 0
