@@ -452,6 +452,8 @@ def apply_tuning(g, args):
         rg.set_pull_grid(g, args.pull_grid)
     if args.alpha is not None:
         rg.set_push_alpha(g, args.alpha)
+    if args.order is not None:
+        rg.set_source_order(g, args.order)
 
 
 def main():
@@ -470,6 +472,8 @@ def main():
     ap.add_argument("--pull-grid", type=int, default=None, help="pull workgroups per group (0 = one wave per chunk)")
     ap.add_argument("--alpha", type=float, default=None, help="push/pull switch factor")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--order", type=int, default=None, choices=[0, 1, 2],
+                    help="source grouping (zg_set_source_order mode)")
     ap.add_argument("--no-graph", action="store_true", help="launch every level kernel from the host")
     ap.add_argument("--pr-cache", type=int, default=None, help="PageRank values staged in LDS (0 = none)")
     ap.add_argument("--pr-grid", type=int, default=None, help="PageRank iteration blocks")
