@@ -922,10 +922,12 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
     if (rc == ZG_OK && with_depth) rc = zg_dmalloc(&w.dep, G * VL * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.maxdep, G * VL);
     // per-level lane masks for the backward pass, up to ZG_LVL_LEVELS levels
-    // and ZG_LVL_BYTES in all (levels past the table use maxdep alone)
-    w.lmcap = (uint32_t)std::min<size_t>(ZG_LVL_LEVELS, ZG_LVL_BYTES / (G * VL * sizeof(uint64_t)));
+    // and ZG_LVL_BYTES in all (levels past the table use maxdep alone; when
+    // the table cannot be allocated, every level does)
+    const size_t lvl_row = G * VL * sizeof(uint64_t);
+    w.lmcap = lvl_row ? (uint32_t)std::min<size_t>(ZG_LVL_LEVELS, ZG_LVL_BYTES / lvl_row) : 0;
     if (w.lmcap < 3) w.lmcap = 0;
-    if (rc == ZG_OK && w.lmcap) rc = zg_dmalloc(&w.lvl, (size_t)w.lmcap * G * VL);
+    if (rc == ZG_OK && w.lmcap && zg_dmalloc(&w.lvl, (size_t)w.lmcap * G * VL) != ZG_OK) w.lmcap = 0;
     if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * 2);
