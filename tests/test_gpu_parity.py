@@ -360,6 +360,24 @@ def test_grid_overflowing_path_counts():
     rg.destroy(g)
 
 
+def test_lane_engine_deeper_than_level_table():
+    # the 64-lane engine on a graph with ~240 BFS levels: the backward pass
+    # keeps per-level lane masks for the first 32 levels and falls back to
+    # the deepest-level filter past them; path counts pass 2^64 (scaled mode)
+    from oracle import oracle as O
+    n, src, dst = gen.grid_2d(120, 120, seed=7, delete=800)
+    sources = np.array([0, 119, 60 * 120 + 60, 120 * 120 - 1, 5, 777, 9000, 14000], np.uint32)
+    say("lane engine deep grid", n, len(src), len(sources))
+    ref, _ = O.Graph(n, src, dst).betweenness(sources=sources, mode=O.SIGMA_SCALED)
+    g = make(n, src, dst)
+    rg.set_engine(g, "lanes")
+    info = {}
+    got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
+    assert info["max_levels"] > 64
+    assert_bc_close(got, ref)
+    rg.destroy(g)
+
+
 def test_queue_engine_counts_beyond_one_level_fp64_range():
     # 1600x1600 grid: a level's path counts span more than the FP64 range
     # (1 on the edges, ~2^1590 mid-grid from a corner); the queue engine's
