@@ -45,7 +45,9 @@
 #include <algorithm>
 
 #define ZG_W_MAX_SOURCES 2048  // sources per batch (the range tables are sized to the batch in LDS)
-#define ZG_W_GRID 4096  // default workgroups of the per-round kernels
+#define ZG_W_GRID 4096  // default workgroups of the per-round kernels ...
+#define ZG_W_GRID_LARGE 12288  // ... and of batches with sources x vertices >= ZG_W_LARGE_SV
+#define ZG_W_LARGE_SV ((size_t)1 << 28)
 #define ZG_W_ROUNDS_PER_SYNC 16
 #define ZG_W_WIDE 32  // vertices with more arcs are walked by a whole wavefront
 #define ZG_W_HUB 4096  // ... and with more than this, cut into pieces for many wavefronts
@@ -914,7 +916,10 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
     ZG_TRY_RC(wbc_ws_ensure(g, S));
     zg_wbc_ws &w = g->wbc;
     const size_t SV = (size_t)S * V;
-    const uint32_t grid = g->wbc_grid ? g->wbc_grid : ZG_W_GRID;
+    const uint32_t grid = g->wbc_grid ? g->wbc_grid : SV >= ZG_W_LARGE_SV ? ZG_W_GRID_LARGE : ZG_W_GRID;
+    // the far-pile split does little per chunk, so every block's O(S) range
+    // prefix dominates it: it keeps the smaller grid
+    const uint32_t gsplit = std::min<uint32_t>(grid, ZG_W_GRID);
     const size_t lds = sizeof(uint32_t) * (3 * (size_t)S + 1);  // range tables of the per-source kernels
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
     ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 3, s));
@@ -935,8 +940,8 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             a = wbc_dev(g);
             for (uint32_t k = 0; k < ZG_W_ROUNDS_PER_SYNC; k++) {
                 ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_round_kernel, dim3(1), dim3(1024), 0, s, a, (int)(r + k));
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), lds, s, a, (int)(r + k), 0);
-                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(grid), dim3(256), lds, s, a, (int)(r + k), 1);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(gsplit), dim3(256), lds, s, a, (int)(r + k), 0);
+                ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_split_kernel, dim3(gsplit), dim3(256), lds, s, a, (int)(r + k), 1);
                 ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_relax_kernel, dim3(grid), dim3(256), lds, s, a, (int)(r + k));
                 ZG_PROF_LAUNCH(ZG_PROF_BC_FORWARD, wbc_hub_kernel, dim3(grid), dim3(256), 0, s, a, (int)(r + k));
             }
