@@ -301,8 +301,9 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   out-degree; +inf = no bound).  Only the amount of work changes. */
 #define ZG_TUNE_WEIGHTED_DELTA 8
 /*   ZG_TUNE_WEIGHTED_GRID: workgroups of zg_bc_f64's per-round kernels
- *   (0 = default: 12288 when batch sources x node capacity >= 2^28, else
- *   4096; the far-pile split uses at most 4096). */
+ *   (0 = default: when batch sources x node capacity >= 2^28, 12288, with
+ *   32768 for the tight-arc count and sigma peel; else 4096; the far-pile
+ *   split uses at most 4096). */
 #define ZG_TUNE_WEIGHTED_GRID 9
 int zg_set_tuning(zg_graph *g, int param, double value);
 

@@ -48,6 +48,7 @@
 #define ZG_W_GRID 4096  // default workgroups of the per-round kernels ...
 #define ZG_W_GRID_LARGE 12288  // ... and of batches with sources x vertices >= ZG_W_LARGE_SV
 #define ZG_W_LARGE_SV ((size_t)1 << 28)
+#define ZG_W_GRID_DAG 32768  // ... and of their tight-arc count and sigma peel
 #define ZG_W_ROUNDS_PER_SYNC 16
 #define ZG_W_WIDE 32  // vertices with more arcs are walked by a whole wavefront
 #define ZG_W_HUB 4096  // ... and with more than this, cut into pieces for many wavefronts
@@ -920,6 +921,8 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
     // the far-pile split does little per chunk, so every block's O(S) range
     // prefix dominates it: it keeps the smaller grid
     const uint32_t gsplit = std::min<uint32_t>(grid, ZG_W_GRID);
+    // the tight-arc count and the sigma peel gain from more workgroups still
+    const uint32_t gdag = g->wbc_grid || SV < ZG_W_LARGE_SV ? grid : ZG_W_GRID_DAG;
     const size_t lds = sizeof(uint32_t) * (3 * (size_t)S + 1);  // range tables of the per-source kernels
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
     ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 3, s));
@@ -953,14 +956,14 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
             r += ZG_W_ROUNDS_PER_SYNC;
         }
         // 2. the tight DAG: in-arc counts, then peel rounds
-        ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_count_kernel, dim3(grid), dim3(256), lds, s, a);
+        ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_count_kernel, dim3(gdag), dim3(256), lds, s, a);
         ZG_TRY(hipGetLastError());
         int last = -1;
         for (uint32_t p = 0; last < 0;) {
             ZG_TRY_RC(wbc_grow(g, p + ZG_W_ROUNDS_PER_SYNC + 2, p + 2, s));
             a = wbc_dev(g);
             for (uint32_t k = 0; k < ZG_W_ROUNDS_PER_SYNC; k++) {
-                ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_peel_kernel, dim3(grid), dim3(256), lds, s, a, (int)(p + k));
+                ZG_PROF_LAUNCH(ZG_PROF_BC_DAG, wbc_peel_kernel, dim3(gdag), dim3(256), lds, s, a, (int)(p + k));
                 ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_close_kernel, dim3(1), dim3(1024), 0, s, a, (int)(p + k + 1));
             }
             ZG_TRY(hipGetLastError());
