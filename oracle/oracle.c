@@ -513,10 +513,56 @@ int zo_bfs_paths(const zo_graph *g, uint32_t s, uint32_t *depth_out, uint64_t *s
 /* PageRank (centrality.zig:871-965)                                          */
 /* ------------------------------------------------------------------------- */
 
+/* Row range of one thread of the row-parallel mode: every row's sum is the
+ * same loop in the same order as the 1-thread mode, so the new ranks are
+ * bit-identical whatever the thread count. */
+typedef struct {
+    const uint32_t *nodes;
+    uint32_t k0, k1;
+    const uint64_t *in_ptr, *out_deg;
+    const uint32_t *in_col;
+    const double *ranks;
+    double *new_ranks;
+    double damping, n_f, sink_sum;
+} zo_pr_rows;
+
+static void zo_pr_rows_run(const zo_pr_rows *j)
+{
+    for (uint32_t k = j->k0; k < j->k1; k++) {
+        uint32_t v = j->nodes[k];
+        double rank_sum = 0.0;
+        for (uint64_t e = j->in_ptr[v]; e < j->in_ptr[v + 1]; e++) {
+            uint32_t u = j->in_col[e];
+            uint64_t du = j->out_deg[u];
+            if (du > 0) rank_sum += j->ranks[u] / (double)du;
+        }
+        j->new_ranks[v] = (1.0 - j->damping) / j->n_f + j->damping * (j->sink_sum + rank_sum);
+    }
+}
+
+static void *zo_pr_rows_worker(void *arg)
+{
+    zo_pr_rows_run((const zo_pr_rows *)arg);
+    return NULL;
+}
+
 /* Returns 0 and fills out[n_cap] (0.0 for non-live), *iters_out = iterations
- * performed, *l1_out = L1 change of the last iteration (0 if none). */
+ * performed, *l1_out = L1 change of the last iteration (0 if none).
+ * n_threads > 1 computes the rows of an iteration on that many threads
+ * (contiguous row ranges); the dangling sum and the L1 change stay one
+ * serial loop in node order, so the result equals the 1-thread result bit
+ * for bit (an extension: the reference's loop is single-threaded). */
+int zo_pagerank_mt(const zo_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                   int n_threads, double *out, uint64_t *iters_out, double *l1_out);
+
 int zo_pagerank(const zo_graph *g, double damping, uint64_t max_iterations, double tolerance,
                 double *out, uint64_t *iters_out, double *l1_out)
+{
+    return zo_pagerank_mt(g, damping, max_iterations, tolerance, 1, out, iters_out, l1_out);
+}
+
+int zo_pagerank_mt(const zo_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                   int n_threads, double *out, uint64_t *iters_out, double *l1_out)
 {
     uint32_t V = g->n_cap;
     for (uint32_t i = 0; i < V; i++) out[i] = 0.0;
@@ -582,15 +628,24 @@ int zo_pagerank(const zo_graph *g, double damping, uint64_t max_iterations, doub
             uint32_t v = nodes[k];
             if (out_deg[v] == 0) sink_sum += ranks[v] / n_f;
         }
-        for (uint32_t k = 0; k < n; k++) {
-            uint32_t v = nodes[k];
-            double rank_sum = 0.0;
-            for (uint64_t e = in_ptr[v]; e < in_ptr[v + 1]; e++) {
-                uint32_t u = in_col[e];
-                uint64_t du = out_deg[u];
-                if (du > 0) rank_sum += ranks[u] / (double)du;
+        zo_pr_rows job = {nodes, 0, n, in_ptr, out_deg, in_col, ranks, new_ranks, damping, n_f, sink_sum};
+        if (n_threads <= 1) {
+            zo_pr_rows_run(&job);
+        } else {
+            zo_pr_rows jobs[64];
+            pthread_t th[64];
+            int joined[64];
+            int nt = n_threads > 64 ? 64 : n_threads;
+            uint32_t chunk = (n + (uint32_t)nt - 1) / (uint32_t)nt;
+            for (int t = 0; t < nt; t++) {
+                jobs[t] = job;
+                jobs[t].k0 = (uint32_t)t * chunk < n ? (uint32_t)t * chunk : n;
+                jobs[t].k1 = jobs[t].k0 + chunk < n ? jobs[t].k0 + chunk : n;
+                joined[t] = pthread_create(&th[t], NULL, zo_pr_rows_worker, &jobs[t]) == 0;
+                if (!joined[t]) zo_pr_rows_run(&jobs[t]);   /* no thread: run the range here */
             }
-            new_ranks[v] = (1.0 - damping) / n_f + damping * (sink_sum + rank_sum);
+            for (int t = 0; t < nt; t++)
+                if (joined[t]) pthread_join(th[t], NULL);
         }
         l1_norm = 0.0;
         for (uint32_t k = 0; k < n; k++) {
@@ -622,10 +677,38 @@ int zo_pagerank(const zo_graph *g, double damping, uint64_t max_iterations, doub
  * unprocessed neighbour's degree dropping by one into bucket max(deg, i)
  * (:47-97).  A per-node mark replaces the reference's linear scan for a
  * repeated neighbour; the lists and the peeling order are the same. */
-static int cmp_u64(const void *a, const void *b)
+/* Ascending sort of n u64 keys: LSD radix sort, 16-bit digits, a pass
+ * skipped when every key has the same digit (any correct sort gives the same
+ * array).  Returns 0, or -1 when the buffer cannot be allocated. */
+static int zo_sort_u64(uint64_t *keys, uint64_t n)
 {
-    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
-    return x < y ? -1 : x > y;
+    if (n < 2) return 0;
+    uint64_t *tmp = malloc(sizeof(uint64_t) * n);
+    uint64_t *cnt = malloc(sizeof(uint64_t) * 65536);
+    if (!tmp || !cnt) {
+        free(tmp); free(cnt);
+        return -1;
+    }
+    uint64_t *a = keys, *b = tmp;
+    for (int sh = 0; sh < 64; sh += 16) {
+        memset(cnt, 0, sizeof(uint64_t) * 65536);
+        for (uint64_t i = 0; i < n; i++) cnt[(a[i] >> sh) & 0xFFFF]++;
+        if (cnt[(a[0] >> sh) & 0xFFFF] == n) continue;
+        uint64_t sum = 0;
+        for (int d = 0; d < 65536; d++) {
+            uint64_t c = cnt[d];
+            cnt[d] = sum;
+            sum += c;
+        }
+        for (uint64_t i = 0; i < n; i++) b[cnt[(a[i] >> sh) & 0xFFFF]++] = a[i];
+        uint64_t *t = a;
+        a = b;
+        b = t;
+    }
+    if (a != keys) memcpy(keys, a, sizeof(uint64_t) * n);
+    free(tmp);
+    free(cnt);
+    return 0;
 }
 
 int zo_core_numbers(const zo_graph *g, uint32_t *out)
@@ -659,7 +742,10 @@ int zo_core_numbers(const zo_graph *g, uint32_t *out)
             if (t != u) keys[k++] = u < t ? ((uint64_t)u << 32 | t) : ((uint64_t)t << 32 | u);
     }
     /* sort + unique (insertion order of the lists does not change core numbers) */
-    qsort(keys, m2, sizeof(uint64_t), cmp_u64);
+    if (zo_sort_u64(keys, m2) != 0) {
+        free(deg); free(cnt); free(keys);
+        return ZO_ERR_OOM;
+    }
     uint64_t m = 0;
     for (uint64_t i = 0; i < m2; i++)
         if (m == 0 || keys[i] != keys[m - 1]) keys[m++] = keys[i];

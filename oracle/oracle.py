@@ -55,6 +55,8 @@ def lib():
         L.zo_bfs_paths_exp.argtypes = [P, ctypes.c_uint32, P, P, P]
         L.zo_bfs_paths_exp.restype = ctypes.c_int
         L.zo_pagerank.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P, P, P]
+        L.zo_pagerank_mt.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int, P, P, P]
+        L.zo_pagerank_mt.restype = ctypes.c_int
         L.zo_core_numbers.argtypes = [P, P]
         L.zo_core_numbers.restype = ctypes.c_int
         L.zo_graph_set_weights.argtypes = [P, P, ctypes.c_uint64]
@@ -156,13 +158,14 @@ class Graph:
             raise OracleError(f"zo_bfs_paths_exp failed: {rc}")
         return depth, mant, ex
 
-    def pagerank(self, damping=0.85, max_iterations=100, tolerance=1e-4):
-        """FP64 PageRank, f64[n_cap]; returns (ranks, iterations, last_l1)."""
+    def pagerank(self, damping=0.85, max_iterations=100, tolerance=1e-4, threads=1):
+        """FP64 PageRank, f64[n_cap]; returns (ranks, iterations, last_l1).
+        threads > 1 splits each iteration's rows over threads (same bits)."""
         out = np.zeros(self.n_cap, dtype=np.float64)
         it = ctypes.c_uint64()
         l1 = ctypes.c_double()
-        rc = lib().zo_pagerank(self._h, damping, max_iterations, tolerance, _ptr(out),
-                               ctypes.byref(it), ctypes.byref(l1))
+        rc = lib().zo_pagerank_mt(self._h, damping, max_iterations, tolerance, int(threads), _ptr(out),
+                                  ctypes.byref(it), ctypes.byref(l1))
         if rc != 0:
             raise OracleError(f"zo_pagerank failed: {rc}")
         return out, int(it.value), float(l1.value)

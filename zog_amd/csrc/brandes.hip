@@ -879,8 +879,13 @@ static void bc_group_span(const zg_graph *g, const uint32_t *src, uint32_t n, ui
     }
     uint32_t lo = ZG_NONE, hi = 0;
     for (uint32_t l = 0; l < n; l++) {
+        if (src[l] == ZG_NONE) continue;  // an empty slot
         lo = std::min(lo, src[l]);
         hi = std::max(hi, src[l]);
+    }
+    if (lo == ZG_NONE) {
+        *vlo = *vhi = 0;
+        return;
     }
     const size_t c_lo = std::upper_bound(co.begin(), co.end(), lo) - co.begin() - 1;
     const size_t c_hi = std::upper_bound(co.begin(), co.end(), hi) - co.begin() - 1;
@@ -1103,8 +1108,11 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         memset(&sp, 0, sizeof(sp));
         uint32_t max_v = 0, max_c = 0;
         for (uint32_t q = 0; q < G; q++) {
-            sp.used[q] = std::min<uint32_t>(64, nb - 64 * q);
-            bc_group_span(g, h_sources + b0 + 64 * q, sp.used[q], &sp.vlo[q], &sp.vhi[q]);
+            const uint32_t slots = std::min<uint32_t>(64, nb - 64 * q);
+            bc_group_span(g, h_sources + b0 + 64 * q, slots, &sp.vlo[q], &sp.vhi[q]);
+            // a padded group's empty slots are its last ones
+            sp.used[q] = slots;
+            while (sp.used[q] > 0 && h_sources[b0 + 64 * q + sp.used[q] - 1] == ZG_NONE) sp.used[q]--;
             if (sp.vhi[q] - sp.vlo[q] > w.vloc) return ZG_ERR_INVALID_ARG;  // workspace sized for other sources
             sp.clo[q] = g->h_ich_first[sp.vlo[q]];
             sp.chi[q] = g->h_ich_first[sp.vhi[q]];
@@ -1393,6 +1401,46 @@ static void bc_order_perm(const zg_graph *g, const std::vector<uint32_t> &src, s
     });
 }
 
+// Group layout of the lane engine on a resource of several graphs: a group
+// takes the next sources of `list` while its span (the components from its
+// first to its last source, which are contiguous id ranges) stays within the
+// widest component that holds a source; otherwise the group is closed and its
+// remaining slots stay empty (ZG_NONE).  Sources that alternate between far
+// apart graphs thus never make a group's state span the graphs between them.
+// slot[i] = position of list[i] in the padded list.
+static void bc_pad_groups(const zg_graph *g, const std::vector<uint32_t> &list, std::vector<uint32_t> &padded,
+                          std::vector<uint64_t> &slot)
+{
+    const std::vector<uint32_t> &co = g->h_comp_off;
+    auto comp = [&](uint32_t x) { return (size_t)(std::upper_bound(co.begin(), co.end(), x) - co.begin() - 1); };
+    uint32_t widest = 1;
+    for (uint32_t x : list) widest = std::max(widest, co[comp(x) + 1] - co[comp(x)]);
+    padded.clear();
+    padded.reserve(list.size());
+    slot.resize(list.size());
+    size_t c_lo = 0, c_hi = 0, fill = 0;
+    for (size_t i = 0; i < list.size(); i++) {
+        const size_t c = comp(list[i]);
+        if (fill > 0) {
+            const size_t lo = std::min(c_lo, c), hi = std::max(c_hi, c);
+            if (fill == 64 || co[hi + 1] - co[lo] > widest) {
+                while (fill < 64) {
+                    padded.push_back(ZG_NONE);
+                    fill++;
+                }
+                fill = 0;
+            } else {
+                c_lo = lo;
+                c_hi = hi;
+            }
+        }
+        if (fill == 0) c_lo = c_hi = c;
+        slot[i] = padded.size();
+        padded.push_back(list[i]);
+        fill++;
+    }
+}
+
 static void bc_all_live_sources(zg_graph *g, std::vector<uint32_t> &out)
 {
     out.clear();
@@ -1444,21 +1492,31 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
         for (uint64_t i = 0; i < n; i++) sorted[i] = list[perm[i]];
         list.swap(sorted);
     }
+    // the lane engine on several graphs sweeps a padded list (no group spans
+    // far apart graphs); slot maps the sorted list into it
+    std::vector<uint64_t> slot;
+    if (!queue_engine && g->h_comp_off.size() > 2) {
+        std::vector<uint32_t> padded;
+        bc_pad_groups(g, list, padded, slot);
+        list.swap(padded);
+    }
+    const uint64_t np = list.size();
+    auto at = [&](uint64_t i) { return slot.empty() ? i : slot[i]; };
     std::vector<double> wsorted, tsorted;
     if (call.h_weights) {
-        wsorted.resize(n);
-        for (uint64_t i = 0; i < n; i++) wsorted[i] = call.h_weights[order ? perm[i] : i];
+        wsorted.assign(np, 0.0);
+        for (uint64_t i = 0; i < n; i++) wsorted[at(i)] = call.h_weights[order ? perm[i] : i];
     }
-    if (call.h_totals) tsorted.assign(n, 0.0);
+    if (call.h_totals) tsorted.assign(np, 0.0);
     uint32_t *d_list = nullptr;
     double *d_w = nullptr, *d_acc = nullptr;
-    int rc = zg_dmalloc(&d_list, n);
-    if (rc == ZG_OK && n)
-        rc = zg_hip_code(hipMemcpyAsync(d_list, list.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    int rc = zg_dmalloc(&d_list, np);
+    if (rc == ZG_OK && np)
+        rc = zg_hip_code(hipMemcpyAsync(d_list, list.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice, s));
     if (rc == ZG_OK && call.h_weights) {
-        rc = zg_dmalloc(&d_w, n);
-        if (rc == ZG_OK && n)
-            rc = zg_hip_code(hipMemcpyAsync(d_w, wsorted.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        rc = zg_dmalloc(&d_w, np);
+        if (rc == ZG_OK && np)
+            rc = zg_hip_code(hipMemcpyAsync(d_w, wsorted.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
     }
     double *d_out = call.d_out;
     if (rc == ZG_OK && !d_out) {  // totals only: the scores go to scratch
@@ -1474,11 +1532,12 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
         if (rc == ZG_OK)
             rc = zg_bcq_sweep(g, d_list, n, d_w, call.h_totals ? tsorted.data() : nullptr, d_out, s, &inf);
     } else {
-        if (rc == ZG_OK) rc = bc_ws_ensure(g, false, list.data(), n);
+        if (rc == ZG_OK) rc = bc_ws_ensure(g, false, list.data(), np);
         BcSweepOpt opt;
         opt.d_weights = d_w;
         opt.h_totals = call.h_totals ? tsorted.data() : nullptr;
-        if (rc == ZG_OK) rc = bc_sweep(g, d_list, list.data(), n, s, &inf, opt);
+        if (rc == ZG_OK) rc = bc_sweep(g, d_list, list.data(), np, s, &inf, opt);
+        inf.n_sources = n;
         if (rc == ZG_OK && V) {
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reduce_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V,
                            (uint32_t)g->bc.groups, g->bc.bcg, g->live, d_out);
@@ -1500,7 +1559,7 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
     zg_dfree(d_w);
     zg_dfree(d_acc);
     if (rc == ZG_OK && call.h_totals)
-        for (uint64_t i = 0; i < n; i++) call.h_totals[order ? perm[i] : i] = tsorted[i];
+        for (uint64_t i = 0; i < n; i++) call.h_totals[order ? perm[i] : i] = tsorted[at(i)];
     if (rc == ZG_OK && info) *info = inf;
     return rc;
 }
