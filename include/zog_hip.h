@@ -51,6 +51,16 @@ typedef struct {
     uint64_t arcs_relaxed;    /* zg_bc_f64: arcs scanned by the distance rounds,
                                  a vertex's arcs once per round it is queued
                                  in (0 for the unweighted sweeps) */
+    /* Work of the batched (64-source group) sweep, for its byte model; 0 from
+     * the other engines.  A row visit is a (group, level, vertex) with the
+     * vertex on that level for at least one of the group's sources, levels
+     * >= 1; row_arcs sums the out-degrees of the visited rows; tree_arcs is
+     * the number of (source, arc v->w) pairs with depth(w) = depth(v) + 1
+     * and depth(v) >= 1 (the arcs the dependency pass follows), counted only
+     * when ZG_TUNE_COUNT_TREE_ARCS is on. */
+    uint64_t row_visits;
+    uint64_t row_arcs;
+    uint64_t tree_arcs;
 } zg_bc_info;
 
 /* Per-run information for PageRank (and the eigenvector, Katz and alpha
@@ -305,6 +315,10 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   32768 for the tight-arc count and sigma peel; else 4096; the far-pile
  *   split uses at most 4096). */
 #define ZG_TUNE_WEIGHTED_GRID 9
+/*   ZG_TUNE_COUNT_TREE_ARCS: 1 makes the 64-lane engine count per-lane
+ *   tree arcs (zg_bc_info.tree_arcs) in its backward pass (one atomic per
+ *   work-list entry; for measurement runs), 0 (default) does not. */
+#define ZG_TUNE_COUNT_TREE_ARCS 10
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the

@@ -506,3 +506,39 @@ def test_autotuner_keeps_results():
     got = rg.betweenness_unweighted_array(g, sources=sources)
     assert_bc_close(got, ref)
     rg.destroy(g)
+
+
+def test_batched_sweep_work_counters():
+    # row visits, row arcs and per-lane tree arcs of the 64-lane sweep (the
+    # inputs of its byte model) against the same counts taken from the
+    # oracle's BFS depths, sources swept in the given order, 64 per group
+    from oracle import oracle as O
+    from zog_amd import _lib
+    n, src, dst = gen.rmat(11, 1 << 14, 71)
+    src = np.concatenate([src, src[:200], np.array([5, 5], np.uint32)])  # parallel arcs, a self-loop
+    dst = np.concatenate([dst, dst[:200], np.array([5, 9], np.uint32)])
+    sources = gen.seeded_sources(n, src, dst, 150, 3)
+    og = O.Graph(n, src, dst)
+    outdeg = np.bincount(src, minlength=n).astype(np.int64)
+    visits = row_arcs = tree = 0
+    for g0 in range(0, len(sources), 64):
+        depths = np.stack([og.bfs_paths(int(s))[0] for s in sources[g0:g0 + 64]])
+        reached = depths != O.NONE_U32
+        for L in range(1, int(depths[reached].max()) + 1):
+            on = np.any(depths == L, axis=0)
+            visits += int(on.sum())
+            row_arcs += int(outdeg[on].sum())
+        for d in depths:
+            r = (d[src] != O.NONE_U32) & (d[src] >= 1)  # the backward pass starts at level 1
+            tree += int(np.count_nonzero(r & (d[dst] == d[src] + 1)))
+    g = make(n, src, dst, groups=2)
+    rg.set_source_order(g, 0)
+    _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_COUNT_TREE_ARCS, 1.0), "zg_set_tuning")
+    info = {}
+    rg.betweenness_unweighted_array(g, sources=sources, info=info)
+    assert (info["row_visits"], info["row_arcs"], info["tree_arcs"]) == (visits, row_arcs, tree)
+    _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_COUNT_TREE_ARCS, 0.0), "zg_set_tuning")
+    info = {}
+    rg.betweenness_unweighted_array(g, sources=sources, info=info)
+    assert info["tree_arcs"] == 0 and info["row_visits"] == visits
+    rg.destroy(g)

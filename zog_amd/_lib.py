@@ -39,6 +39,7 @@ ZG_TUNE_PR_GRID = 6
 ZG_TUNE_WEIGHTED_BATCH = 7
 ZG_TUNE_WEIGHTED_DELTA = 8
 ZG_TUNE_WEIGHTED_GRID = 9
+ZG_TUNE_COUNT_TREE_ARCS = 10
 ZG_EIG_CONVERGED, ZG_EIG_OSCILLATING, ZG_EIG_ZERO, ZG_EIG_CAP, ZG_EIG_TRIVIAL = range(5)
 
 # Every symbol include/zog_hip.h declares.
@@ -103,6 +104,9 @@ class ZgBcInfo(ctypes.Structure):
         ("pull_levels", ctypes.c_uint32),
         ("kernel_ms", ctypes.c_double),
         ("arcs_relaxed", ctypes.c_uint64),
+        ("row_visits", ctypes.c_uint64),
+        ("row_arcs", ctypes.c_uint64),
+        ("tree_arcs", ctypes.c_uint64),
     ]
 
 

@@ -74,7 +74,8 @@ struct BcDev {
     uint64_t *lvl;              // [lmcap][Gcap][V] lanes whose level-L frontier held v
     uint32_t lmcap;
     uint64_t *vis, *fr;         // fr: [3][Gcap][V]
-    unsigned long long *stats;  // [Gcap][ZG_FIN_BLOCKS][2]
+    unsigned long long *stats;  // [Gcap][ZG_FIN_BLOCKS][ZG_BC_STATS]
+    unsigned long long *tree;   // [Gcap][ZG_TREE_SLOTS] per-lane tree arcs seen by the backward pass, or null
     uint32_t *touched;          // [Gcap][V]
     uint32_t *flist;
     uint64_t *fmask;
@@ -198,8 +199,8 @@ __global__ void bc_source_kernel(BcDev a)
     if (l == 0) {
         a.cnt[g].farcs[0] = arcs;
         // slot 0 belongs to finalize block 0, which runs strictly later
-        a.stats[(size_t)g * ZG_FIN_BLOCKS * 2] += arcs;
-        a.stats[(size_t)g * ZG_FIN_BLOCKS * 2 + 1] += reached;
+        a.stats[(size_t)g * ZG_FIN_BLOCKS * ZG_BC_STATS] += arcs;
+        a.stats[(size_t)g * ZG_FIN_BLOCKS * ZG_BC_STATS + 1] += reached;
     }
 }
 
@@ -455,7 +456,7 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
 {
     __shared__ uint32_t lds[40];
     __shared__ uint32_t base[2];
-    __shared__ unsigned long long sred[3][ZG_FIN_THREADS / 64];
+    __shared__ unsigned long long sred[4][ZG_FIN_THREADS / 64];
     const uint32_t g = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
     if (a.cnt[g].mode != ZG_MODE_PULL) return;
     const int lane = tid & 63;
@@ -472,7 +473,7 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
 
     // pass A: fold masks, count list entries, statistics
     uint32_t nf = 0, nh = 0;
-    unsigned long long arcs = 0, reached = 0, farcs = 0;
+    unsigned long long arcs = 0, reached = 0, farcs = 0, visits = 0;
     for (uint32_t c = c0 + tid; c < c1; c += ZG_FIN_THREADS) {
         const uint32_t v = a.ich_v[c];
         if (a.ich_e0[c] != a.in_ptr[v]) continue;
@@ -490,6 +491,7 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
         arcs += pc * od;
         reached += pc;
         farcs += od;
+        visits++;
         if (pushed || a.in_ptr[v + 1] - a.in_ptr[v] > ZG_CHUNK) {
             // summed by atomics: the level-(d+1) exponent is known only now
             uint64_t mm = nm;
@@ -511,23 +513,28 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
     farcs = zg_wave_sum_u64(farcs);
+    visits = zg_wave_sum_u64(visits);
     if (lane == 0) {
         sred[0][tid >> 6] = arcs;
         sred[1][tid >> 6] = reached;
         sred[2][tid >> 6] = farcs;
+        sred[3][tid >> 6] = visits;
     }
     __syncthreads();
     if (a.dsum) bc_flush_lanes(a, g, d, lcnt);
     if (tid == 0) {
-        unsigned long long A = 0, Rr = 0, F = 0;
+        unsigned long long A = 0, Rr = 0, F = 0, Vi = 0;
         for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
             A += sred[0][w];
             Rr += sred[1][w];
             F += sred[2][w];
+            Vi += sred[3][w];
         }
-        unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * 2;
+        unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * ZG_BC_STATS;
         st[0] += A;
         st[1] += Rr;
+        st[2] += Vi;
+        st[3] += F;
         if (F) atomicAdd(&a.cnt[g].farcs[(d + 1) & 1], F);
     }
     if (tf == 0 && th == 0) return;
@@ -593,7 +600,7 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
 {
     __shared__ uint32_t lds[40];
     __shared__ uint32_t base[2];
-    __shared__ unsigned long long sred[3][ZG_FIN_THREADS / 64];
+    __shared__ unsigned long long sred[4][ZG_FIN_THREADS / 64];
     const uint32_t g = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
     if (a.cnt[g].mode != ZG_MODE_PUSH) return;
     const int lane = tid & 63;
@@ -610,7 +617,7 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
     const uint32_t t0 = min(b * R, nt), t1 = min(t0 + R, nt);
 
     uint32_t nf = 0, nh = 0;
-    unsigned long long arcs = 0, reached = 0, farcs = 0;
+    unsigned long long arcs = 0, reached = 0, farcs = 0, visits = 0;
     for (uint32_t t = t0 + tid; t < t1; t += ZG_FIN_THREADS) {
         const uint32_t v = tl[t];
         const uint64_t nm = next[MASK(g, v)];
@@ -626,6 +633,7 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
         arcs += pc * od;
         reached += pc;
         farcs += od;
+        visits++;
         uint64_t mm = nm;
         while (mm) {
             const int l = __builtin_ctzll(mm);
@@ -644,23 +652,28 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
     arcs = zg_wave_sum_u64(arcs);
     reached = zg_wave_sum_u64(reached);
     farcs = zg_wave_sum_u64(farcs);
+    visits = zg_wave_sum_u64(visits);
     if (lane == 0) {
         sred[0][tid >> 6] = arcs;
         sred[1][tid >> 6] = reached;
         sred[2][tid >> 6] = farcs;
+        sred[3][tid >> 6] = visits;
     }
     __syncthreads();
     if (a.dsum) bc_flush_lanes(a, g, d, lcnt);
     if (tid == 0) {
-        unsigned long long A = 0, Rr = 0, F = 0;
+        unsigned long long A = 0, Rr = 0, F = 0, Vi = 0;
         for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
             A += sred[0][w];
             Rr += sred[1][w];
             F += sred[2][w];
+            Vi += sred[3][w];
         }
-        unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * 2;
+        unsigned long long *st = a.stats + ((size_t)g * ZG_FIN_BLOCKS + b) * ZG_BC_STATS;
         st[0] += A;
         st[1] += Rr;
+        st[2] += Vi;
+        st[3] += F;
         if (F) atomicAdd(&a.cnt[g].farcs[(d + 1) & 1], F);
     }
     if (tf == 0 && th == 0) return;
@@ -751,6 +764,7 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     // depth >= d+1.
     const bool exact = (uint32_t)(d + 1) < a.lmcap;
     double S = 0.0;
+    uint32_t tree = 0;  // this lane's children read (counted only when a.tree is set)
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
         uint32_t w = 0;
@@ -782,7 +796,16 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
             for (int q = 0; q < 8; q++) val[q] = ((mq[q] >> lane) & 1) ? cg[ROW(g, wq[q]) + lane] : 0.0;
 #pragma unroll
             for (int q = 0; q < 8; q++) S += val[q];
+            if (a.tree) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) tree += val[q] != 0.0;  // c > 0 exactly for a child
+            }
         }
+    }
+    if (a.tree) {
+        // spread over slots by block: one counter per group would serialise
+        const unsigned long long t = zg_wave_sum_u64(tree);
+        if (lane == 0 && t) atomicAdd(&a.tree[(size_t)g * ZG_TREE_SLOTS + (blockIdx.x % ZG_TREE_SLOTS)], t);
     }
     const uint32_t hub = a.ohub_idx[v];
     if (hub != ZG_NONE) {
@@ -845,7 +868,7 @@ int zg_bc_ws_release(zg_graph *g)
 {
     zg_bc_ws &w = g->bc;
     zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep); zg_dfree(w.maxdep); zg_dfree(w.lvl);
-    zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.touched); zg_dfree(w.flist); zg_dfree(w.fmask);
+    zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.tree); zg_dfree(w.touched); zg_dfree(w.flist); zg_dfree(w.fmask);
     zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.foff); zg_dfree(w.hoff); zg_dfree(w.modes); zg_dfree(w.src); zg_dfree(w.cnt);
     zg_dfree(w.wts); zg_dfree(w.dsum);
     zg_dfree(w.cur);
@@ -935,7 +958,8 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
     if (rc == ZG_OK && w.lmcap && zg_dmalloc(&w.lvl, (size_t)w.lmcap * G * VL) != ZG_OK) w.lmcap = 0;
     if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * VL);
-    if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * 2);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * ZG_BC_STATS);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.tree, G * ZG_TREE_SLOTS);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.touched, G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.flist, G * w.fcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fmask, G * w.fcap);
@@ -1072,6 +1096,7 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used, const GroupSpan &sp, cons
     a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep; a.maxdep = w.maxdep;
     a.lvl = w.lvl; a.lmcap = w.lvl ? w.lmcap : 0;
     a.vis = w.vis; a.fr = w.fr; a.stats = w.stats; a.touched = w.touched;
+    a.tree = g->count_tree ? w.tree : nullptr;
     a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
     a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
     a.wts = opt.d_weights ? w.wts : nullptr;
@@ -1098,7 +1123,8 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
     const bool use_graph = g->use_graph && !g->profiling;
     ZG_TRY(hipMemsetAsync(w.bcg, 0, sizeof(double) * (size_t)Gcap * V, s));
     // per-block work statistics accumulate over all batches
-    ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * 2, s));
+    ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * ZG_BC_STATS, s));
+    ZG_TRY(hipMemsetAsync(w.tree, 0, sizeof(unsigned long long) * Gcap * ZG_TREE_SLOTS, s));
     std::vector<std::vector<uint32_t>> foff(Gcap), hoff(Gcap);
     for (uint64_t b0 = 0; b0 < n; b0 += (uint64_t)Gcap * 64) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)Gcap * 64, n - b0);
@@ -1257,13 +1283,17 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         if (opt.hook) ZG_TRY_RC(opt.hook(g, s, b0, nb, G, opt.user));
     }
     {
-        std::vector<unsigned long long> st((size_t)Gcap * ZG_FIN_BLOCKS * 2);
+        std::vector<unsigned long long> st((size_t)Gcap * ZG_FIN_BLOCKS * ZG_BC_STATS), tr((size_t)Gcap * ZG_TREE_SLOTS);
         ZG_TRY(hipMemcpyAsync(st.data(), w.stats, sizeof(unsigned long long) * st.size(), hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipMemcpyAsync(tr.data(), w.tree, sizeof(unsigned long long) * tr.size(), hipMemcpyDeviceToHost, s));
         ZG_TRY(hipStreamSynchronize(s));
-        for (size_t i = 0; i < st.size(); i += 2) {
+        for (size_t i = 0; i < st.size(); i += ZG_BC_STATS) {
             inf->arcs_traversed += st[i];
             inf->nodes_reached += st[i + 1];
+            inf->row_visits += st[i + 2];
+            inf->row_arcs += st[i + 3];
         }
+        for (unsigned long long t : tr) inf->tree_arcs += t;
     }
     inf->n_sources = n;
     return ZG_OK;

@@ -29,6 +29,12 @@
 // of predecessor chunks, so it reserves list space with one atomic per level.
 #define ZG_FIN_BLOCKS 256
 #define ZG_FIN_THREADS 1024
+// Per-(group, finalize block) statistics of the lane engine: per-lane arcs
+// scanned, lanes reached, row visits (a vertex on a level of some lane), arcs
+// of the visited rows.
+#define ZG_BC_STATS 4
+// Counters per group of the lane engine's tree-arc count.
+#define ZG_TREE_SLOTS 256
 
 // Error bits a Brandes kernel reports to the host.
 #define ZG_BC_ERR_SOURCE 1u     // a source id is out of range or not live
@@ -96,7 +102,8 @@ struct zg_bc_ws {
     uint64_t *lvl = nullptr;             // [lmcap][G][V] lanes that reached v at each level
     uint32_t lmcap = 0;                  // levels the lane table holds (0: none)
     uint64_t *vis = nullptr, *fr = nullptr;
-    unsigned long long *stats = nullptr; // [G][ZG_FIN_BLOCKS][2] arcs, reached
+    unsigned long long *stats = nullptr; // [G][ZG_FIN_BLOCKS][ZG_BC_STATS] arcs, reached, row visits, row arcs
+    unsigned long long *tree = nullptr;  // [G][ZG_TREE_SLOTS] per-lane tree arcs (counted on request)
     uint32_t *touched = nullptr;         // [G][V] vertices reached by a push level
     uint32_t *flist = nullptr;           // [G][fcap]
     uint64_t *fmask = nullptr;
@@ -208,6 +215,7 @@ struct zg_graph {
     hipStream_t stream = nullptr;
     hipStream_t cap_stream = nullptr;    // stream the level graph is captured on (never launched on)
     bool use_graph = true;               // replay captured level chunks (ZG_TUNE_GRAPH)
+    bool count_tree = false;             // lane engine counts per-lane tree arcs (ZG_TUNE_COUNT_TREE_ARCS)
     uint32_t V = 0, n_live = 0;
     uint64_t E = 0;
     // resident CSR

@@ -611,6 +611,10 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
         if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;
         g->wbc_grid = (uint32_t)value;
         return ZG_OK;
+    case ZG_TUNE_COUNT_TREE_ARCS:
+        if (value != 0.0 && value != 1.0) return ZG_ERR_INVALID_ARG;
+        g->count_tree = value != 0.0;
+        return ZG_OK;
     case ZG_TUNE_PULL_GRID:
         if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;
         g->pull_grid = (uint32_t)value;

@@ -8,8 +8,9 @@
       --copy-fetch CF.csv --copy-write CW.csv --copy-bytes 1073741824 -o out.json
       HBM bytes per dispatch of every kernel from one FETCH_SIZE and one
       WRITE_SIZE counter-collection run (KiB per dispatch), each scaled by
-      what the same counter reads for a plain copy of known size (copy
-      kernels: the largest dispatch of the calibration run); the Brandes
+      what the same counter reads for a plain copy of known size (the copy
+      kernel's mean dispatch in the calibration run, whose streaming sum of
+      the same size checks the read factor); the Brandes
       forward pull and push kernels are also summed under
       "bc_forward_kernel+bc_push_kernel", as bench.py reports them together.
 """
@@ -60,13 +61,24 @@ def _counter_per_kernel(path: str, counter: str):
 def traffic(fetch, write, copy_fetch, copy_write, copy_bytes):
     ft, fc, _ = _counter_per_kernel(fetch, "FETCH_SIZE")
     wt, wc, _ = _counter_per_kernel(write, "WRITE_SIZE")
-    _, _, cf = _counter_per_kernel(copy_fetch, "FETCH_SIZE")
+    cft, cfc, _ = _counter_per_kernel(copy_fetch, "FETCH_SIZE")
     _, _, cw = _counter_per_kernel(copy_write, "WRITE_SIZE")
-    # counters are in KiB; a copy of copy_bytes reads and writes copy_bytes
+    # the calibration run holds a copy kernel (reads and writes copy_bytes)
+    # and a streaming sum (reads copy_bytes, writes a few bytes); counters are
+    # in KiB.  The copy sets the factors, the sum checks the read factor on a
+    # kernel that only reads.
+    copy_k = [k for k in cft if "reduce" not in k.lower()]
+    sum_k = [k for k in cft if "reduce" in k.lower()]
+    cf = max(cft[k] / cfc[k] for k in copy_k)
     f_scale = copy_bytes / (cf * 1024.0)
     w_scale = copy_bytes / (cw * 1024.0)
     out = {"_calibration": {"copy_bytes": copy_bytes, "copy_fetch_kib": cf, "copy_write_kib": cw,
                             "fetch_factor": f_scale, "write_factor": w_scale}}
+    if sum_k:
+        sf = max(cft[k] / cfc[k] for k in sum_k)
+        out["_calibration"]["sum_fetch_kib"] = sf
+        out["_calibration"]["sum_read_bytes_scaled"] = sf * 1024.0 * f_scale
+        out["_calibration"]["sum_check"] = sf * 1024.0 * f_scale / copy_bytes
     for k in sorted(set(ft) | set(wt)):
         n = max(fc.get(k, 0), wc.get(k, 0))
         if not n:
