@@ -1621,9 +1621,9 @@ static int bc_host_call(zg_graph *g, const uint32_t *sources, const double *weig
     hipStream_t s = g->stream;
     uint32_t *d_src = nullptr;
     double *d_out = nullptr;
-    int rc = out_bc ? zg_dmalloc(&d_out, g->V) : ZG_OK;
+    int rc = zg_io_out(g, &d_out);  // totals only: the scores still land here
     if (rc == ZG_OK && sources) {
-        rc = zg_dmalloc(&d_src, n_sources);
+        rc = zg_io_src(g, n_sources, &d_src);
         if (rc == ZG_OK && n_sources)
             rc = zg_hip_code(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, s));
     }
@@ -1636,8 +1636,6 @@ static int bc_host_call(zg_graph *g, const uint32_t *sources, const double *weig
     if (rc == ZG_OK) rc = zg_bc_device_impl(g, call, s, info);
     if (rc == ZG_OK && out_bc && g->V)
         rc = zg_hip_code(hipMemcpy(out_bc, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
-    zg_dfree(d_src);
-    zg_dfree(d_out);
     return rc;
 }
 

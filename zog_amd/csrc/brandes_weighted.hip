@@ -1023,8 +1023,8 @@ extern "C" int zg_bc_f64(zg_graph *g, const uint32_t *sources, uint64_t n_source
     uint32_t *d_src = nullptr;
     double *d_out = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = zg_dmalloc(&d_src, list.size());
-    if (rc == ZG_OK) rc = zg_dmalloc(&d_out, g->V);
+    int rc = zg_io_src(g, list.size(), &d_src);
+    if (rc == ZG_OK) rc = zg_io_out(g, &d_out);
     if (rc == ZG_OK && !list.empty())
         rc = zg_hip_code(hipMemcpyAsync(d_src, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, s));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));
@@ -1042,8 +1042,6 @@ extern "C" int zg_bc_f64(zg_graph *g, const uint32_t *sources, uint64_t n_source
     if (rc == ZG_OK && g->V) rc = zg_hip_code(hipMemcpy(out_bc, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    zg_dfree(d_src);
-    zg_dfree(d_out);
     if (rc == ZG_OK && info) *info = inf;
     return rc;
 }

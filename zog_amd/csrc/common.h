@@ -275,6 +275,11 @@ struct zg_graph {
     bool pr_xcache_useful = false;       // the cached values carry a useful share of the gathers
     uint32_t pr_grid_cap = 0;            // PageRank iteration blocks (0 = default)
     bool high_diameter = false;          // set with h_coredist
+    // device staging of the host-buffer entry points (sources in, scores
+    // out), kept between calls so a call allocates nothing once warm
+    double *io_out = nullptr;            // [V]
+    uint32_t *io_src = nullptr;          // [io_src_cap]
+    uint64_t io_src_cap = 0;
     zg_bc_ws bc;
     zg_bcq_ws bcq;
     zg_wbc_ws wbc;
@@ -304,6 +309,10 @@ void zg_prof_release(zg_graph *g);
 
 int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
                      std::vector<uint32_t> &out_ptr, std::vector<uint32_t> &in_ptr, std::vector<uint32_t> &pr_order);
+// The handle's staging buffers: *out = [V] doubles, *src = [n] u32 (grown
+// as needed); the caller holds the handle's lock.
+int zg_io_out(zg_graph *g, double **out);
+int zg_io_src(zg_graph *g, uint64_t n, uint32_t **src);
 int zg_bc_ws_release(zg_graph *g);
 int zg_bcq_ws_release(zg_graph *g);
 int zg_wbc_ws_release(zg_graph *g);

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <new>
 
+#include "csr_host.h"
 #include "pr_bins.h"
 
 extern "C" const char *zg_strerror(int code)
@@ -105,9 +106,31 @@ extern "C" int zg_set_build_threshold(uint64_t min_arcs)
     return ZG_OK;
 }
 
+int zg_io_out(zg_graph *g, double **out)
+{
+    if (!g->io_out) ZG_TRY_RC(zg_dmalloc(&g->io_out, g->V));
+    *out = g->io_out;
+    return ZG_OK;
+}
+
+int zg_io_src(zg_graph *g, uint64_t n, uint32_t **src)
+{
+    if (n > g->io_src_cap) {
+        zg_dfree(g->io_src);
+        g->io_src_cap = 0;
+        ZG_TRY_RC(zg_dmalloc(&g->io_src, n));
+        g->io_src_cap = n;
+    }
+    *src = g->io_src;
+    return ZG_OK;
+}
+
 static void zg_free_device(zg_graph *g)
 {
     zg_prof_release(g);
+    zg_dfree(g->io_out);
+    zg_dfree(g->io_src);
+    g->io_src_cap = 0;
     zg_bc_ws_release(g);
     zg_bcq_ws_release(g);
     zg_wbc_ws_release(g);
@@ -137,72 +160,28 @@ static int zg_upload(T **d, const std::vector<T> &h)
 
 // Host form of the CSR build (zg_build_csr_gpu in graph_build.hip is the
 // device form, used from ZG_GPU_BUILD_MIN_ARCS arcs on; both produce the
-// same arrays): the successor CSR, newest arc first in every row, arcs with
-// a tombstoned endpoint dropped (the iterator skips them); the predecessor
-// CSR by scanning sources in id order; PageRank ids by decreasing
-// out-degree (stable); uploads them with the weights and the live mask.
+// same arrays): csr_host.cpp builds them, this uploads them with the weights
+// and the live mask.
 static int zg_build_csr_host(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
                              std::vector<uint32_t> &out_ptr, std::vector<uint32_t> &in_ptr,
                              std::vector<uint32_t> &pr_order)
 {
-    const uint32_t V = g->V;
-    out_ptr.assign((size_t)V + 1, 0);
-    for (uint64_t e = 0; e < m; e++) {
-        uint32_t f = from[e], t = to[e];
-        if (f >= V || t >= V) return ZG_ERR_NODE_NOT_FOUND;
-        if (g->h_live[f] && g->h_live[t]) out_ptr[(size_t)f + 1]++;
-    }
-    for (uint32_t i = 0; i < V; i++) out_ptr[i + 1] += out_ptr[i];
-    const uint64_t E = out_ptr[V];
-    g->E = E;
-    std::vector<uint32_t> out_col(E);
-    std::vector<double> out_w(weight ? E : 0);
-    double wsum = 0.0;
-    {
-        // filling each row from its end in insertion order leaves the newest
-        // arc first
-        std::vector<uint32_t> fill(out_ptr.begin() + 1, out_ptr.end());
-        for (uint64_t e = 0; e < m; e++) {
-            uint32_t f = from[e], t = to[e];
-            if (g->h_live[f] && g->h_live[t]) {
-                const uint32_t k = --fill[f];
-                out_col[k] = t;
-                if (weight) {
-                    out_w[k] = weight[e];
-                    wsum += weight[e];
-                    // NaN, negative or infinite: no shortest paths under the reference's semantics
-                    if (!(weight[e] >= 0.0) || weight[e] == HUGE_VAL) g->weights_ok = false;
-                }
-            }
-        }
-    }
-    g->wmean = E ? wsum / (double)E : 0.0;
-    if (weight) ZG_TRY_RC(zg_upload(&g->out_w, out_w));
-    in_ptr.assign((size_t)V + 1, 0);
-    for (uint64_t k = 0; k < E; k++) in_ptr[(size_t)out_col[k] + 1]++;
-    for (uint32_t i = 0; i < V; i++) in_ptr[i + 1] += in_ptr[i];
-    std::vector<uint32_t> in_col(E);
-    {
-        std::vector<uint32_t> fill(in_ptr.begin(), in_ptr.end() - 1);
-        for (uint32_t v = 0; v < V; v++)
-            for (uint32_t k = out_ptr[v]; k < out_ptr[(size_t)v + 1]; k++) in_col[fill[out_col[k]]++] = v;
-    }
-    pr_order.resize(V);
-    for (uint32_t v = 0; v < V; v++) pr_order[v] = v;
-    std::stable_sort(pr_order.begin(), pr_order.end(), [&](uint32_t x, uint32_t y) {
-        return out_ptr[(size_t)x + 1] - out_ptr[x] > out_ptr[(size_t)y + 1] - out_ptr[y];
-    });
-    std::vector<uint32_t> pr_id(V);
-    for (uint32_t i = 0; i < V; i++) pr_id[pr_order[i]] = i;
-    std::vector<uint32_t> pr_col(E);
-    for (size_t e = 0; e < E; e++) pr_col[e] = pr_id[in_col[e]];
-    ZG_TRY_RC(zg_upload(&g->out_ptr, out_ptr));
-    ZG_TRY_RC(zg_upload(&g->out_col, out_col));
-    ZG_TRY_RC(zg_upload(&g->in_ptr, in_ptr));
-    ZG_TRY_RC(zg_upload(&g->in_col, in_col));
+    ZgHostCsr c;
+    ZG_TRY_RC(zg_host_csr(g->V, g->h_live.data(), from, to, weight, m, c));
+    g->E = c.out_ptr[g->V];
+    g->wmean = g->E ? c.wsum / (double)g->E : 0.0;
+    g->weights_ok = c.weights_ok;
+    if (weight) ZG_TRY_RC(zg_upload(&g->out_w, c.out_w));
+    ZG_TRY_RC(zg_upload(&g->out_ptr, c.out_ptr));
+    ZG_TRY_RC(zg_upload(&g->out_col, c.out_col));
+    ZG_TRY_RC(zg_upload(&g->in_ptr, c.in_ptr));
+    ZG_TRY_RC(zg_upload(&g->in_col, c.in_col));
     ZG_TRY_RC(zg_upload(&g->live, g->h_live));
-    ZG_TRY_RC(zg_upload(&g->pr_id, pr_id));
-    ZG_TRY_RC(zg_upload(&g->pr_col, pr_col));
+    ZG_TRY_RC(zg_upload(&g->pr_id, c.pr_id));
+    ZG_TRY_RC(zg_upload(&g->pr_col, c.pr_col));
+    out_ptr.swap(c.out_ptr);
+    in_ptr.swap(c.in_ptr);
+    pr_order.swap(c.pr_order);
     return ZG_OK;
 }
 

@@ -424,7 +424,7 @@ static int pr_ws_ensure(zg_graph *g)
 }
 
 template <typename T>
-static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrState *st)
+static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrState *st, double *part = nullptr)
 {
     PrArgs<T> a;
     a.in_ptr = g->in_ptr;
@@ -454,7 +454,7 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.x0 = x0;
     a.x1 = x1;
     a.rb0 = a.rb1 = a.rb2 = nullptr;
-    a.part = g->pr.part;
+    a.part = part ? part : g->pr.part;
     a.seg = seg;
     a.st = st;
     return a;
@@ -463,7 +463,7 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
 // Runs the power iteration in precision T from the initialised buffers.
 // Returns with *host mirror of the final state filled.
 template <typename T>
-static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st, uint64_t max_iter)
+static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st, uint64_t max_iter, bool prof = true)
 {
     const uint32_t grid = pr_grid(g);
     const uint64_t K = 8;
@@ -471,7 +471,7 @@ static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st,
     while (launched < max_iter) {
         uint64_t k = max_iter - launched < K ? max_iter - launched : K;
         for (uint64_t i = 0; i < k; i++) {
-            const size_t p = zg_prof_begin(g, s);
+            const size_t p = prof ? zg_prof_begin(g, s) : SIZE_MAX;
             hipLaunchKernelGGL((pr_iter_kernel<T, ZG_PW_PR>), dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
             hipLaunchKernelGGL((pr_finish_kernel<T, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, grid);
             zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
@@ -489,8 +489,20 @@ static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st,
     return ZG_OK;
 }
 
+// The buffers one PageRank call works in (the handle's workspace unless
+// given); timed = HIP events for the kernel time and per-kernel profile.
+struct PrMem {
+    double *r64 = nullptr, *x64 = nullptr;  // [2][V] each (FP64 phases, residual check)
+    float *r32 = nullptr, *x32 = nullptr;   // [2][V] each (FP32 phases)
+    double *part = nullptr;                 // [2][grid] block partials
+    float *seg32 = nullptr;                 // [n_seg][2] long-row segment partials
+    double *seg64 = nullptr;                // [n_seg][2]
+    PrState *d_st = nullptr, *h_st = nullptr;  // [2] device state and its host mirror
+    bool timed = true;
+};
+
 static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
-                            int precision, double *d_out, hipStream_t s, zg_pr_info *info)
+                            int precision, double *d_out, hipStream_t s, zg_pr_info *info, const PrMem *mem = nullptr)
 {
     const bool check = !(precision & ZG_PR_SKIP_RESIDUAL);
     precision &= 0xFF;
@@ -504,23 +516,35 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
     inf.precision = precision;
     const dim3 vb(256), vg((V + 255) / 256);
     if (n <= 1) {
-        // centrality.zig:875-879: n = 0 gives no scores, n = 1 gives 1.0
+        // centrality.zig:875-879: n = 0 gives no scores (every id is
+        // non-live: 0.0), n = 1 gives 1.0
         if (V) {
-            if (n == 0) ZG_TRY(hipMemsetAsync(d_out, 0, sizeof(double) * V, s));
-            else hipLaunchKernelGGL(pr_single_kernel, vg, vb, 0, s, V, g->live, d_out);
+            hipLaunchKernelGGL(pr_single_kernel, vg, vb, 0, s, V, g->live, d_out);
             ZG_TRY(hipGetLastError());
         }
         ZG_TRY(hipStreamSynchronize(s));
         if (info) *info = inf;
         return ZG_OK;
     }
-    ZG_TRY_RC(pr_ws_ensure(g));
-    double *r64 = (double *)g->pr.r;
-    double *x64 = r64 + 2 * (size_t)V;
-    float *r32 = (float *)(x64 + 2 * (size_t)V);
-    float *x32 = r32 + 2 * (size_t)V;
-    PrState *d_st = (PrState *)g->pr.state;
-    PrState *h_st = (PrState *)g->pr.h_state;
+    PrMem ws;
+    if (!mem) {
+        ZG_TRY_RC(pr_ws_ensure(g));
+        ws.r64 = (double *)g->pr.r;
+        ws.x64 = ws.r64 + 2 * (size_t)V;
+        ws.r32 = (float *)(ws.x64 + 2 * (size_t)V);
+        ws.x32 = ws.r32 + 2 * (size_t)V;
+        ws.part = g->pr.part;
+        ws.seg32 = (float *)g->pr.seg32;
+        ws.seg64 = g->pr.seg64;
+        ws.d_st = (PrState *)g->pr.state;
+        ws.h_st = (PrState *)g->pr.h_state;
+        mem = &ws;
+    }
+    double *r64 = mem->r64, *x64 = mem->x64, *part = mem->part;
+    float *r32 = mem->r32, *x32 = mem->x32;
+    PrState *d_st = mem->d_st;
+    PrState *h_st = mem->h_st;
+    const bool timed = mem->timed;
 
     const double inv_n = 1.0 / (double)n;
     PrState st0;
@@ -533,9 +557,11 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
     st0.base = (1.0 - damping) * inv_n + damping * ((double)g->n_dangling * inv_n * inv_n);
     st0.done = max_iterations == 0 ? 1u : 0u;
 
-    hipEvent_t e0, e1;
-    ZG_TRY(hipEventCreate(&e0));
-    ZG_TRY(hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+        ZG_TRY(hipEventCreate(&e0));
+        ZG_TRY(hipEventCreate(&e1));
+    }
     int rc = ZG_OK;
     do {
         // phase 1: FP64 for ZG_PR_FP64, FP32 otherwise (the mixed mode stops
@@ -552,19 +578,21 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
                                (const double *)nullptr, false, inv_n, r64, r64 + V, x64, x64 + V);
         }
         ZG_TRY(hipGetLastError());
-        ZG_TRY(hipEventRecord(e0, s));
+        if (timed) ZG_TRY(hipEventRecord(e0, s));
         if (!fp64) {
-            PrArgs<float> a = pr_args<float>(g, r32, r32 + V, x32, x32 + V, (float *)g->pr.seg32, d_st);
-            rc = pr_run<float>(g, s, a, h_st, max_iterations);
+            PrArgs<float> a = pr_args<float>(g, r32, r32 + V, x32, x32 + V, mem->seg32, d_st, part);
+            rc = pr_run<float>(g, s, a, h_st, max_iterations, timed);
         } else {
-            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st);
-            rc = pr_run<double>(g, s, a, h_st, max_iterations);
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, mem->seg64, d_st, part);
+            rc = pr_run<double>(g, s, a, h_st, max_iterations, timed);
         }
         if (rc != ZG_OK) break;
         uint64_t iters = h_st->iter;
         // launches queued past convergence returned at once: not profiled
-        if (g->ev_marks.size() > iters) g->ev_marks.resize(iters);
-        ZG_TRY_RC(zg_prof_collect(g, s));
+        if (timed) {
+            if (g->ev_marks.size() > iters) g->ev_marks.resize(iters);
+            ZG_TRY_RC(zg_prof_collect(g, s));
+        }
         bool final64 = fp64;
         if (mixed && !(h_st->last_l1 < tolerance) && iters < max_iterations) {
             // phase 2: FP64 from the FP32 iterate; the buffers of parity
@@ -581,20 +609,24 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             st2.tol = tolerance;
             st2.done = 0;
             ZG_TRY(hipMemcpyAsync(d_st, &st2, sizeof(PrState), hipMemcpyHostToDevice, s));
-            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st);
-            rc = pr_run<double>(g, s, a, h_st, max_iterations - iters);
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, mem->seg64, d_st, part);
+            rc = pr_run<double>(g, s, a, h_st, max_iterations - iters, timed);
             if (rc != ZG_OK) break;
             const uint64_t more = h_st->iter - iters;
-            if (g->ev_marks.size() > more) g->ev_marks.resize(more);
-            ZG_TRY_RC(zg_prof_collect(g, s));
+            if (timed) {
+                if (g->ev_marks.size() > more) g->ev_marks.resize(more);
+                ZG_TRY_RC(zg_prof_collect(g, s));
+            }
             inf.fp64_iterations = more;
             iters = h_st->iter;
         }
-        ZG_TRY(hipEventRecord(e1, s));
-        ZG_TRY(hipEventSynchronize(e1));
-        float ms = 0.f;
-        ZG_TRY(hipEventElapsedTime(&ms, e0, e1));
-        inf.kernel_ms = ms;
+        if (timed) {
+            ZG_TRY(hipEventRecord(e1, s));
+            ZG_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            ZG_TRY(hipEventElapsedTime(&ms, e0, e1));
+            inf.kernel_ms = ms;
+        }
         const bool final_odd = iters & 1;
         inf.iterations = iters;
         inf.last_l1 = h_st->last_l1;
@@ -616,7 +648,7 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr, g->pr_id,
                                (const double *)d_out, false, 0.0, r64, r64 + V, x64, x64 + V);
             ZG_TRY(hipGetLastError());
-            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, g->pr.seg64, d_st + 1);
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, mem->seg64, d_st + 1, part);
             hipLaunchKernelGGL((pr_iter_kernel<double, ZG_PW_PR>), dim3(pr_grid(g)), dim3(ZG_PR_BLOCK), 0, s, a);
             hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a,
                                pr_grid(g));
@@ -627,8 +659,8 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             inf.error_bound_l1 = damping < 1.0 ? inf.residual_l1 / (1.0 - damping) : INFINITY;
         }
     } while (0);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     if (rc != ZG_OK) return rc;
     ZG_TRY(hipStreamSynchronize(s));
     if (info) *info = inf;
@@ -661,13 +693,12 @@ extern "C" int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations,
     std::lock_guard<std::mutex> lk(g->mu);
     ZG_TRY(hipSetDevice(g->device));
     double *d_out = nullptr;
-    ZG_TRY_RC(zg_dmalloc(&d_out, g->V));
+    ZG_TRY_RC(zg_io_out(g, &d_out));
     int rc = zg_pagerank_impl(g, damping, max_iterations, tolerance, precision, d_out, g->stream, info);
     if (rc == ZG_OK && g->V) {
         hipError_t e = hipMemcpy(out, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = zg_hip_code(e);
     }
-    zg_dfree(d_out);
     return rc;
 }
 
@@ -968,10 +999,9 @@ static int zg_power_host(zg_graph *g, int kind, double alpha, double beta, doubl
     std::lock_guard<std::mutex> lk(g->mu);
     ZG_TRY(hipSetDevice(g->device));
     double *d_out = nullptr;
-    ZG_TRY_RC(zg_dmalloc(&d_out, g->V));
+    ZG_TRY_RC(zg_io_out(g, &d_out));
     int rc = zg_power_impl(g, kind, alpha, beta, start, max_iterations, tolerance, d_out, g->stream, info);
     if (rc == ZG_OK && g->V) rc = zg_hip_code(hipMemcpy(out, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
-    zg_dfree(d_out);
     return rc;
 }
 
