@@ -32,6 +32,7 @@ extern "C" {
 #define ZG_ERR_TOO_LARGE (-7)        /* more than 2^32-1 arcs or nodes */
 #define ZG_ERR_INVALID_HANDLE (-8)   /* NULL or destroyed handle */
 #define ZG_ERR_IO (-9)               /* a file could not be opened or read */
+#define ZG_ERR_SCRATCH (-10)         /* more than the fixed scratch of zg_pagerank_device_noalloc holds */
 
 typedef struct zg_graph zg_graph;
 
@@ -219,6 +220,23 @@ int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations, double tol
  * (FP64) on the caller's stream (NULL = the handle's stream). */
 int zg_pagerank_device(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
                        int precision, double *d_out, void *stream, zg_pr_info *info);
+
+/* Allocation-free form of zg_pagerank_device: no counterpart in the
+ * reference; for callers that cannot allocate device memory (work under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same computation and result, bit for bit; every buffer the iteration needs
+ * beyond the resident graph, the caller's d_out and stream comes from one
+ * fixed device array of 64 MiB that the library holds on each device it runs
+ * on until the process ends: its rank vectors take 16 bytes per node
+ * capacity in FP32 (ZG_PR_FP32 | ZG_PR_SKIP_RESIDUAL) and 48 otherwise
+ * (FP64 vectors for the FP64 phases and the residual check): up to about
+ * 1.39 million nodes, or just under 4.19 million in FP32 without the
+ * residual check.  A larger graph returns ZG_ERR_SCRATCH.  Creates no events either: info->kernel_ms
+ * is 0 and no per-kernel profile is taken.  Calls on one device take turns
+ * and share the array, which keeps the last call's contents (every call
+ * rewrites what it reads); the call returns after the stream has drained. */
+int zg_pagerank_device_noalloc(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                               int precision, double *d_out, void *stream, zg_pr_info *info);
 
 /* How zg_eigenvector's iteration ended (zg_pr_info.outcome). */
 #define ZG_EIG_CONVERGED 0    /* L2 change below tolerance: the new iterate */

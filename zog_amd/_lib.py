@@ -22,6 +22,7 @@ ZG_ERR_NO_DEVICE = -6
 ZG_ERR_TOO_LARGE = -7
 ZG_ERR_INVALID_HANDLE = -8
 ZG_ERR_IO = -9
+ZG_ERR_SCRATCH = -10
 
 ZG_PR_FP32 = 32
 ZG_PR_FP64 = 64
@@ -61,6 +62,7 @@ EXPORTED = (
     "zg_bc_f64",
     "zg_pagerank",
     "zg_pagerank_device",
+    "zg_pagerank_device_noalloc",
     "zg_eigenvector",
     "zg_katz",
     "zg_alpha_centrality",
@@ -206,6 +208,9 @@ def lib():
     L.zg_pagerank_device.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                                      P, P, ctypes.POINTER(ZgPrInfo)]
     L.zg_pagerank_device.restype = ctypes.c_int
+    L.zg_pagerank_device_noalloc.argtypes = [P, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                             P, P, ctypes.POINTER(ZgPrInfo)]
+    L.zg_pagerank_device_noalloc.restype = ctypes.c_int
     L.zg_eigenvector.argtypes = [P, ctypes.c_uint64, ctypes.c_double, P, ctypes.POINTER(ZgPrInfo)]
     L.zg_eigenvector.restype = ctypes.c_int
     L.zg_katz.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, P,

@@ -30,6 +30,7 @@ extern "C" const char *zg_strerror(int code)
     case ZG_ERR_TOO_LARGE: return "graph too large (more than 2^32-1 arcs or nodes)";
     case ZG_ERR_INVALID_HANDLE: return "invalid graph handle";
     case ZG_ERR_IO: return "file could not be read";
+    case ZG_ERR_SCRATCH: return "input larger than the fixed scratch of the allocation-free entry point";
     default: return "unknown error";
     }
 }

@@ -24,6 +24,7 @@
 #include "pr_bins.h"
 
 #include <algorithm>
+#include <mutex>
 
 struct PrState {
     unsigned int done;
@@ -489,8 +490,9 @@ static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st,
     return ZG_OK;
 }
 
-// The buffers one PageRank call works in (the handle's workspace unless
-// given); timed = HIP events for the kernel time and per-kernel profile.
+// The buffers one PageRank call works in: the handle's workspace, or the
+// fixed scratch of zg_pagerank_device_noalloc (then no events are created:
+// no kernel time, no per-kernel profile).
 struct PrMem {
     double *r64 = nullptr, *x64 = nullptr;  // [2][V] each (FP64 phases, residual check)
     float *r32 = nullptr, *x32 = nullptr;   // [2][V] each (FP32 phases)
@@ -700,6 +702,92 @@ extern "C" int zg_pagerank(zg_graph *g, double damping, uint64_t max_iterations,
         if (e != hipSuccess) rc = zg_hip_code(e);
     }
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free PageRank: zg_pagerank_device's computation with every
+// buffer it would take from the handle's workspace (rank and gathered-value
+// vectors, block partials, long-row segment partials, the iteration state)
+// carved from one fixed device array instead.  The array is a module global:
+// each device the library runs on holds its own copy, from the module's load
+// until the process ends.  Its name is a host-side stand-in, not a device
+// address; the address is taken once per device with hipGetSymbolAddress and
+// only that pointer is passed to kernels and copies.  Nothing in it has to
+// start at zero: every call writes the state, the vectors (init kernel, both
+// parities) and the partials before reading them.  Calls on one device take
+// turns (a per-device lock); two processes on one device each hold their own
+// array.
+#define ZG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define ZG_NOALLOC_MAX_DEVICES 64
+
+static __device__ __align__(256) unsigned char scratch[ZG_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu[ZG_NOALLOC_MAX_DEVICES];
+static unsigned char *g_scratch_ptr[ZG_NOALLOC_MAX_DEVICES];
+static PrState g_scratch_state[ZG_NOALLOC_MAX_DEVICES][2];  // host mirror of the iteration state
+
+// Device address of `scratch` on device `dev` (current), taken on the first
+// call there.  The caller holds g_scratch_mu[dev].
+static int scratch_address(int dev, unsigned char **out)
+{
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return e != hipSuccess ? zg_hip_code(e) : ZG_ERR_HIP;
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ZG_OK;
+}
+
+extern "C" int zg_pagerank_device_noalloc(zg_graph *g, double damping, uint64_t max_iterations, double tolerance,
+                                          int precision, double *d_out, void *stream, zg_pr_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!d_out && g->V) return ZG_ERR_INVALID_ARG;
+    const int prec = precision & 0xFF;
+    if (prec != ZG_PR_FP32 && prec != ZG_PR_FP64 && prec != ZG_PR_MIXED) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (g->device < 0 || g->device >= ZG_NOALLOC_MAX_DEVICES) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> sl(g_scratch_mu[g->device]);
+    ZG_TRY(hipSetDevice(g->device));
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    // layout: [state][block partials][segment partials FP32][FP64]
+    //         [FP64 r, x: both parities][FP32 r, x: both parities]
+    const size_t V = g->V;
+    const bool need64 = prec != ZG_PR_FP32 || !(precision & ZG_PR_SKIP_RESIDUAL);
+    const bool need32 = prec != ZG_PR_FP64;
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_st = take(2 * sizeof(PrState));
+    const size_t o_part = take(2 * sizeof(double) * pr_grid(g));
+    const size_t o_s32 = take(2 * sizeof(float) * (size_t)g->n_seg);
+    const size_t o_s64 = take(2 * sizeof(double) * (size_t)g->n_seg);
+    const size_t o_64 = need64 ? take(4 * sizeof(double) * V) : 0;
+    const size_t o_32 = need32 ? take(4 * sizeof(float) * V) : 0;
+    if (off > ZG_NOALLOC_SCRATCH_BYTES) return ZG_ERR_SCRATCH;
+    unsigned char *base = nullptr;
+    ZG_TRY_RC(scratch_address(g->device, &base));
+    PrMem m;
+    m.d_st = (PrState *)(base + o_st);
+    m.h_st = g_scratch_state[g->device];
+    m.part = (double *)(base + o_part);
+    m.seg32 = (float *)(base + o_s32);
+    m.seg64 = (double *)(base + o_s64);
+    if (need64) {
+        m.r64 = (double *)(base + o_64);
+        m.x64 = m.r64 + 2 * V;
+    }
+    if (need32) {
+        m.r32 = (float *)(base + o_32);
+        m.x32 = m.r32 + 2 * V;
+    }
+    m.timed = false;
+    return zg_pagerank_impl(g, damping, max_iterations, tolerance, precision, d_out, s, info, &m);
 }
 
 // ---------------------------------------------------------------------------
