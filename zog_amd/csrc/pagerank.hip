@@ -173,9 +173,8 @@ struct PrBufs {
 template <typename T, int MODE, int L, typename X>
 __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *rows, uint32_t nrows,
                                               uint32_t unit, const X &x, const T *rold, T *rnew, T *xnew,
-                                              double base, double damping, double &l1, double &dang)
+                                              double base, double damping, double &l1, double &dang, int tid)
 {
-    const int tid = threadIdx.x;
     const uint32_t i = unit * (ZG_PR_BLOCK / L) + tid / L;
     const uint32_t sub = tid % L;
     const bool ok = i < nrows;
@@ -193,6 +192,45 @@ __device__ __forceinline__ void pr_rows_lanes(const PrArgs<T> &a, const uint4 *r
         kahan_merge(s, c, s2, c2);
     }
     if (ok && sub == 0) pr_row_finish<T, MODE>(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
+}
+
+// One work unit of the row bins (b < nb_small + nb_m4 + nb_m16 + nb_m64)
+// by 256 threads (tid in [0, 256)).
+template <typename T, int MODE, typename X>
+__device__ __forceinline__ void pr_unit(const PrArgs<T> &a, uint32_t b, int tid, const X &x, const T *rold, T *rnew,
+                                        T *xnew, double base, double damping, double &l1, double &dang)
+{
+    if (b < a.nb_small) {
+        // one lane per row (<= ZG_PR_SMALL predecessors)
+        const uint32_t i = b * ZG_PR_BLOCK + tid;
+        if (i < a.n_small) {
+            const uint4 dsc = a.desc[i];
+            const uint32_t e0 = dsc.y, deg = dsc.z;
+            // all predecessor ids, then all gathers, in flight together
+            uint32_t col[ZG_PR_SMALL];
+            T val[ZG_PR_SMALL];
+#pragma unroll
+            for (int k = 0; k < ZG_PR_SMALL; k++) col[k] = (uint32_t)k < deg ? a.in_col[e0 + k] : 0u;
+            const T r_old = MODE != ZG_PW_RAW ? rold[dsc.x] : (T)0;
+#pragma unroll
+            for (int k = 0; k < ZG_PR_SMALL; k++) val[k] = (uint32_t)k < deg ? x[col[k]] : (T)0;
+            T s = 0, c = 0;
+#pragma unroll
+            for (int k = 0; k < ZG_PR_SMALL; k++)
+                if ((uint32_t)k < deg) kahan_add(s, c, val[k]);
+            pr_row_finish<T, MODE>(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
+        }
+    } else if (b < a.nb_small + a.nb_m4) {
+        pr_rows_lanes<T, MODE, 4>(a, a.desc + a.n_small, a.n_m4, b - a.nb_small, x, rold, rnew, xnew, base, damping,
+                                  l1, dang, tid);
+    } else if (b < a.nb_small + a.nb_m4 + a.nb_m16) {
+        pr_rows_lanes<T, MODE, 16>(a, a.desc + a.n_small + a.n_m4, a.n_m16, b - a.nb_small - a.nb_m4, x, rold, rnew,
+                                   xnew, base, damping, l1, dang, tid);
+    } else {
+        pr_rows_lanes<T, MODE, 64>(a, a.desc + a.n_small + a.n_m4 + a.n_m16, a.n_m64,
+                                   b - a.nb_small - a.nb_m4 - a.nb_m16, x, rold, rnew, xnew, base, damping, l1, dang,
+                                   tid);
+    }
 }
 
 template <typename T, int MODE>
@@ -220,35 +258,8 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
     double l1 = 0.0, dang = 0.0;
 
     for (uint32_t b = blockIdx.x; b < units; b += gridDim.x) {
-        if (b < a.nb_small) {
-            // one lane per row (<= ZG_PR_SMALL predecessors)
-            const uint32_t i = b * ZG_PR_BLOCK + tid;
-            if (i < a.n_small) {
-                const uint4 dsc = a.desc[i];
-                const uint32_t e0 = dsc.y, deg = dsc.z;
-                // all predecessor ids, then all gathers, in flight together
-                uint32_t col[ZG_PR_SMALL];
-                T val[ZG_PR_SMALL];
-#pragma unroll
-                for (int k = 0; k < ZG_PR_SMALL; k++) col[k] = (uint32_t)k < deg ? a.in_col[e0 + k] : 0u;
-                const T r_old = MODE != ZG_PW_RAW ? rold[dsc.x] : (T)0;
-#pragma unroll
-                for (int k = 0; k < ZG_PR_SMALL; k++) val[k] = (uint32_t)k < deg ? x[col[k]] : (T)0;
-                T s = 0, c = 0;
-#pragma unroll
-                for (int k = 0; k < ZG_PR_SMALL; k++)
-                    if ((uint32_t)k < deg) kahan_add(s, c, val[k]);
-                pr_row_finish<T, MODE>(rnew, xnew, dsc.x, r_old, dsc.w, s, c, base, damping, l1, dang);
-            }
-        } else if (b < a.nb_small + a.nb_m4) {
-            pr_rows_lanes<T, MODE, 4>(a, a.desc + a.n_small, a.n_m4, b - a.nb_small, x, rold, rnew, xnew, base, damping, l1,
-                                dang);
-        } else if (b < a.nb_small + a.nb_m4 + a.nb_m16) {
-            pr_rows_lanes<T, MODE, 16>(a, a.desc + a.n_small + a.n_m4, a.n_m16, b - a.nb_small - a.nb_m4, x, rold, rnew,
-                                 xnew, base, damping, l1, dang);
-        } else if (b < a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64) {
-            pr_rows_lanes<T, MODE, 64>(a, a.desc + a.n_small + a.n_m4 + a.n_m16, a.n_m64, b - a.nb_small - a.nb_m4 - a.nb_m16,
-                                 x, rold, rnew, xnew, base, damping, l1, dang);
+        if (b < a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64) {
+            pr_unit<T, MODE>(a, b, tid, x, rold, rnew, xnew, base, damping, l1, dang);
         } else {
             // one block per segment of a long row; pr_finish_kernel finishes the row
             const uint32_t sg = b - a.nb_small - a.nb_m4 - a.nb_m16 - a.nb_m64;
