@@ -396,6 +396,31 @@ int zg_get_profile(zg_graph *g, zg_profile *out);
 int zg_bc_weighted(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources,
                    double *out_bc, double *out_totals, zg_bc_info *info);
 
+/* Folding of single-successor sources for sampled BC (no reference
+ * counterpart).  A live vertex of class 1 has exactly one successor other
+ * than itself (parallel arcs counted once), class 0 none, class 2 several.
+ * A class-1 vertex whose successor has class 2 is folded into that
+ * successor: its own Brandes sweep is the successor's, shifted by one arc.
+ * Out (each may be NULL, [n_cap] each): class, the class-1 successor
+ * (0xFFFFFFFF otherwise), the vertex each is folded into (0xFFFFFFFF when
+ * not folded), and how many vertices are folded into each. */
+int zg_fold_map(zg_graph *g, uint8_t *class_out, uint32_t *succ_out, uint32_t *folded_into_out,
+                uint32_t *count_out);
+
+/* Weighted multi-source Brandes over groups: each source m stands for
+ * itself and the sources folded into it (zg_fold_map), so out_bc[v] =
+ * sum_i weights[i] * (delta_{m_i}(v) + sum over s folded into m_i of
+ * delta_s(v)), m_i itself included (the folded sources' paths all pass
+ * through it) -- computed exactly from m_i's own sweep, with per-target
+ * weights in its dependency pass.  With weights 1 and every live vertex
+ * that is not folded as a source, out_bc equals zg_bc_unweighted over all
+ * live sources.  A folded vertex in the list is ZG_ERR_INVALID_ARG (its
+ * group counts it); weights NULL = all 1.  out_totals[i] is the dependency
+ * total of m_i's own sweep, as zg_bc_weighted; either output may be NULL,
+ * not both.  Runs on the 64-lane engine. */
+int zg_bc_folded(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources, double *out_bc,
+                 double *out_totals, zg_bc_info *info);
+
 /* Replaces `nif_node_degrees` (resource_graph.ex:562-599, Elixir
  * node_degrees/1 :2130-2136): out[v] = live out-arcs of v (arcs to
  * tombstoned ids skipped; 0 for a tombstoned v, whose arcs removeNode

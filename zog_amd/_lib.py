@@ -75,6 +75,8 @@ EXPORTED = (
     "zg_get_profile",
     "zg_strerror",
     "zg_bc_weighted",
+    "zg_fold_map",
+    "zg_bc_folded",
     "zg_core_numbers",
     "zg_node_degrees",
     "zg_node_features",
@@ -229,6 +231,10 @@ def lib():
     L.zg_set_tuning.restype = ctypes.c_int
     L.zg_bc_weighted.argtypes = [P, P, P, ctypes.c_uint64, P, P, ctypes.POINTER(ZgBcInfo)]
     L.zg_bc_weighted.restype = ctypes.c_int
+    L.zg_fold_map.argtypes = [P, P, P, P, P]
+    L.zg_fold_map.restype = ctypes.c_int
+    L.zg_bc_folded.argtypes = [P, P, P, ctypes.c_uint64, P, P, ctypes.POINTER(ZgBcInfo)]
+    L.zg_bc_folded.restype = ctypes.c_int
     L.zg_node_degrees.argtypes = [P, P]
     L.zg_node_degrees.restype = ctypes.c_int
     L.zg_core_numbers.argtypes = [P, P]

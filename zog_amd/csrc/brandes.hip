@@ -87,6 +87,11 @@ struct BcDev {
     const uint32_t *src;        // [Gcap][64]
     const double *wts;          // [Gcap][64] source weights, or null (all 1)
     unsigned long long *dsum;   // [Gcap][64] dependency totals, or null (not wanted)
+    // folded sweep (zg_bc_folded), or null: succ1[t] = the source t is folded
+    // into (ZG_NONE if none), fcount[m] = sources folded into m; per lane,
+    // vertices reached beyond the source (rsum) and folded members reached (fsum)
+    const uint32_t *succ1, *fcount;
+    unsigned long long *rsum, *fsum;    // [Gcap][64]
     double push_alpha;
     unsigned long long Eg[ZG_MAX_GROUPS];  // arcs inside each group's span
     zg_bc_ws::Counters *cnt;
@@ -410,8 +415,29 @@ __device__ __forceinline__ void bc_count_lanes(uint32_t *lcnt, uint64_t nm)
 __device__ __forceinline__ void bc_flush_lanes(const BcDev &a, uint32_t g, int d, const uint32_t *lcnt)
 {
     const uint32_t t = threadIdx.x;
-    if (t < 64 && lcnt[t] && d > 0)
+    if (t < 64 && lcnt[t] && d > 0 && a.dsum)
         atomicAdd(&a.dsum[g * 64 + t], (unsigned long long)lcnt[t] * (unsigned long long)d);
+    if (t < 64 && lcnt[t] && a.rsum) atomicAdd(&a.rsum[g * 64 + t], (unsigned long long)lcnt[t]);
+}
+
+// Folded sweep: lanes of nm whose source is the vertex v is folded into
+// (v reached by its own group's representative) count one more folded
+// member reached.
+__device__ __forceinline__ void bc_count_folded(const BcDev &a, uint32_t g, uint32_t v, uint64_t nm, uint32_t *fcnt)
+{
+    const uint32_t m = a.succ1[v];
+    if (m == ZG_NONE) return;
+    while (nm) {
+        const int l = __builtin_ctzll(nm);
+        nm &= nm - 1;
+        if (a.src[g * 64 + l] == m) atomicAdd(&fcnt[l], 1u);
+    }
+}
+
+__device__ __forceinline__ void bc_flush_folded(const BcDev &a, uint32_t g, const uint32_t *fcnt)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < 64 && fcnt[t]) atomicAdd(&a.fsum[g * 64 + t], (unsigned long long)fcnt[t]);
 }
 
 // Clears the frontier buffer of level d-1 (it becomes level d+2's) for the
@@ -462,9 +488,10 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
     const int lane = tid & 63;
     bc_clear_stale(a, d, g, b * ZG_FIN_THREADS + tid, gridDim.x * ZG_FIN_THREADS);
     const uint64_t *next = fr_buf(a, (d + 1) % 3);
-    __shared__ uint32_t lcnt[64];
-    if (a.dsum) {
-        if (tid < 64) lcnt[tid] = 0;
+    __shared__ uint32_t lcnt[64], fcnt[64];
+    const bool lanes = a.dsum || a.succ1;
+    if (lanes) {
+        if (tid < 64) lcnt[tid] = fcnt[tid] = 0;
         __syncthreads();
     }
     const uint32_t nch = sp.chi[g] - sp.clo[g];
@@ -482,7 +509,8 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
         a.vis[MASK(g, v)] |= nm;
         a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);  // levels only grow
         if ((uint32_t)(d + 1) < a.lmcap) a.lvl[LVLM(d + 1, g, v)] = nm;
-        if (a.dsum) bc_count_lanes(lcnt, nm);
+        if (lanes) bc_count_lanes(lcnt, nm);
+        if (a.succ1) bc_count_folded(a, g, v, nm, fcnt);
         const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
         nf += kf;
         nh += kf > 1 ? 1 : 0;
@@ -521,7 +549,8 @@ __device__ __forceinline__ void bc_finalize_body(const BcDev &a, int d, int push
         sred[3][tid >> 6] = visits;
     }
     __syncthreads();
-    if (a.dsum) bc_flush_lanes(a, g, d, lcnt);
+    if (lanes) bc_flush_lanes(a, g, d, lcnt);
+    if (a.succ1) bc_flush_folded(a, g, fcnt);
     if (tid == 0) {
         unsigned long long A = 0, Rr = 0, F = 0, Vi = 0;
         for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
@@ -608,9 +637,10 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
     const uint64_t *next = fr_buf(a, (d + 1) % 3);
     const uint32_t *tl = a.touched + (size_t)g * a.Vloc;
     const uint32_t nt = min(a.cnt[g].tcount, a.Vloc);
-    __shared__ uint32_t lcnt[64];
-    if (a.dsum) {
-        if (tid < 64) lcnt[tid] = 0;
+    __shared__ uint32_t lcnt[64], fcnt[64];
+    const bool lanes = a.dsum || a.succ1;
+    if (lanes) {
+        if (tid < 64) lcnt[tid] = fcnt[tid] = 0;
         __syncthreads();
     }
     const uint32_t R = (nt + gridDim.x - 1) / gridDim.x;
@@ -624,7 +654,8 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
         a.vis[MASK(g, v)] |= nm;
         a.maxdep[MASK(g, v)] = (uint32_t)(d + 1);
         if ((uint32_t)(d + 1) < a.lmcap) a.lvl[LVLM(d + 1, g, v)] = nm;
-        if (a.dsum) bc_count_lanes(lcnt, nm);
+        if (lanes) bc_count_lanes(lcnt, nm);
+        if (a.succ1) bc_count_folded(a, g, v, nm, fcnt);
         const uint32_t kf = a.och_first[v + 1] - a.och_first[v];
         nf += kf;
         nh += kf > 1 ? 1 : 0;
@@ -660,7 +691,8 @@ __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
         sred[3][tid >> 6] = visits;
     }
     __syncthreads();
-    if (a.dsum) bc_flush_lanes(a, g, d, lcnt);
+    if (lanes) bc_flush_lanes(a, g, d, lcnt);
+    if (a.succ1) bc_flush_folded(a, g, fcnt);
     if (tid == 0) {
         unsigned long long A = 0, Rr = 0, F = 0, Vi = 0;
         for (int w = 0; w < ZG_FIN_THREADS / 64; w++) {
@@ -737,7 +769,14 @@ __device__ __forceinline__ void bc_finish(const BcDev &a, uint32_t g, uint32_t v
         const double sv = a.sig[r];
         const int k = a.emax[EMAX(d, g, lane)];
         const double delta = sv * ldexp(S, -k);
-        cc_buf(a, d & 1)[r] = (1.0 + delta) / sv;
+        // target weight: 1, or in a folded sweep 1 + the folded members of
+        // the lane's source, less one when v is itself one of them
+        double tau = 1.0;
+        if (a.succ1) {
+            const uint32_t m = a.src[g * 64 + lane];
+            tau += (double)a.fcount[m] - (a.succ1[v] == m ? 1.0 : 0.0);
+        }
+        cc_buf(a, d & 1)[r] = (tau + delta) / sv;
         if (a.src[g * 64 + lane] != v) contrib = a.wts ? delta * a.wts[g * 64 + lane] : delta;
     }
     contrib = zg_wave_sum(contrib);
@@ -835,6 +874,23 @@ __global__ __launch_bounds__(256) void bc_hub_kernel(BcDev a, int d, GroupRange 
     bc_finish(a, g, v, lane, active, S, d);
 }
 
+// Folded sweep, after a batch's backward pass: each lane's source m gets
+// its folded members' dependency on it, w * (fcount[m] * (reached - 1) -
+// folded members reached) (each member's paths all pass through m), one
+// thread per group so repeated sources add in lane order.
+__global__ void bc_fold_source_kernel(BcDev a)
+{
+    const uint32_t g = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    for (int l = 0; l < 64; l++) {
+        const uint32_t m = a.src[g * 64 + l];
+        if (m == ZG_NONE) continue;
+        const double w = a.wts ? a.wts[g * 64 + l] : 1.0;
+        const double term = (double)a.fcount[m] * (double)a.rsum[g * 64 + l] - (double)a.fsum[g * 64 + l];
+        a.bcg[(size_t)g * a.V + m] += w * term;
+    }
+}
+
 // out[v] = sum over groups of the group accumulators (fixed order).
 __global__ void bc_reduce_kernel(uint32_t V, uint32_t G, const double *bcg, const uint8_t *live, double *out)
 {
@@ -870,7 +926,7 @@ int zg_bc_ws_release(zg_graph *g)
     zg_dfree(w.sig); zg_dfree(w.cc); zg_dfree(w.hubacc); zg_dfree(w.bcg); zg_dfree(w.dep); zg_dfree(w.maxdep); zg_dfree(w.lvl);
     zg_dfree(w.vis); zg_dfree(w.fr); zg_dfree(w.stats); zg_dfree(w.tree); zg_dfree(w.touched); zg_dfree(w.flist); zg_dfree(w.fmask);
     zg_dfree(w.hlist); zg_dfree(w.hmask); zg_dfree(w.emax); zg_dfree(w.foff); zg_dfree(w.hoff); zg_dfree(w.modes); zg_dfree(w.src); zg_dfree(w.cnt);
-    zg_dfree(w.wts); zg_dfree(w.dsum);
+    zg_dfree(w.wts); zg_dfree(w.dsum); zg_dfree(w.rsum); zg_dfree(w.fsum);
     zg_dfree(w.cur);
     if (w.level_args) (void)hipFree(w.level_args);
     if (w.h_level_args) (void)hipHostFree(w.h_level_args);
@@ -972,6 +1028,8 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
     if (rc == ZG_OK) rc = zg_dmalloc(&w.src, G * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.wts, G * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.dsum, G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.rsum, G * 64);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.fsum, G * 64);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.cur, 1);
     if (rc == ZG_OK) rc = zg_hip_code(hipMalloc(&w.level_args, sizeof(BcLevelArgs)));
     if (rc == ZG_OK) rc = zg_hip_code(hipHostMalloc(&w.h_level_args, sizeof(BcLevelArgs), hipHostMallocDefault));
@@ -1074,6 +1132,7 @@ typedef int (*bc_batch_hook)(zg_graph *g, hipStream_t s, uint64_t first, uint32_
 struct BcSweepOpt {
     const double *d_weights = nullptr;
     double *h_totals = nullptr;
+    bool fold = false;          // folded sweep (zg_bc_folded)
     bc_batch_hook hook = nullptr;
     void *user = nullptr;
 };
@@ -1101,6 +1160,10 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used, const GroupSpan &sp, cons
     a.emax = w.emax; a.src = w.src; a.cnt = w.cnt;
     a.wts = opt.d_weights ? w.wts : nullptr;
     a.dsum = opt.h_totals ? w.dsum : nullptr;
+    a.succ1 = opt.fold ? g->fold_succ1 : nullptr;
+    a.fcount = opt.fold ? g->fold_count : nullptr;
+    a.rsum = opt.fold ? w.rsum : nullptr;
+    a.fsum = opt.fold ? w.fsum : nullptr;
     a.foff = w.foff; a.hoff = w.hoff; a.modes = w.modes;
     a.push_alpha = g->push_alpha;
     for (int q = 0; q < ZG_MAX_GROUPS; q++)
@@ -1152,6 +1215,10 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         if (opt.d_weights)
             ZG_TRY(hipMemcpyAsync(w.wts, opt.d_weights + b0, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
         if (opt.h_totals) ZG_TRY(hipMemsetAsync(w.dsum, 0, sizeof(unsigned long long) * Gcap * 64, s));
+        if (opt.fold) {
+            ZG_TRY(hipMemsetAsync(w.rsum, 0, sizeof(unsigned long long) * Gcap * 64, s));
+            ZG_TRY(hipMemsetAsync(w.fsum, 0, sizeof(unsigned long long) * Gcap * 64, s));
+        }
         ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
         ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
         const uint32_t nlev = a.lmcap ? std::min(w.last_levels + 2, a.lmcap) : 0;
@@ -1272,6 +1339,7 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
             if (mr) ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, bc_backward_kernel, dim3(cdiv(mr, 4), G), dim3(256), 0, s, a, L, R);
             if (mh) ZG_PROF_LAUNCH(ZG_PROF_BC_HUB, bc_hub_kernel, dim3(cdiv(mh, 4), G), dim3(256), 0, s, a, L, H);
         }
+        if (opt.fold) ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_fold_source_kernel, dim3(G), dim3(64), 0, s, a);
         ZG_TRY(hipGetLastError());
         inf->batches++;
         if (opt.h_totals) {
@@ -1364,6 +1432,111 @@ static int bc_core_distances(zg_graph *g, hipStream_t s)
     zg_dfree(d_changed);
     if (rc == ZG_OK) g->h_coredist.swap(dist);
     return rc;
+}
+
+// Folding of single-successor sources.  A source s whose only successor
+// (self-loops aside; parallel arcs counted once) is w reaches everything
+// through w, so its dependencies follow from w's sweep: with p_wt(v) the share
+// of shortest w-t paths through v, delta_s(v) = delta_w(v) - p_ws(v) for
+// v != s, w, and delta_s(w) = (reached from w) - 1 - [w reaches s].  Summed
+// over the c_w sources folded into w, w's group {w} + folded contributes
+// sum_t tau(t) p_wt(v) with tau(t) = 1 + c_w - [t folded into w] -- a Brandes
+// dependency pass with per-target weights -- and, at w itself,
+// c_w ((reached from w) - 1) - (folded members w reaches).  A source is
+// folded only into a vertex with several successors, so groups do not nest.
+
+// class[v]: 0 = no successor but itself, 1 = exactly one other, 2 = more;
+// first[v] = that one successor (class 1).  Stops at the second distinct
+// successor, so a row is read only as far as it must be.
+__global__ void fold_class_kernel(uint32_t V, const uint32_t *out_ptr, const uint32_t *out_col, const uint8_t *live,
+                                  uint8_t *cls, uint32_t *first)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint8_t c = 0;
+    uint32_t w1 = ZG_NONE;
+    if (live[v]) {
+        for (uint32_t e = out_ptr[v]; e < out_ptr[v + 1]; e++) {
+            const uint32_t w = out_col[e];
+            if (w == v || w == w1) continue;
+            if (w1 == ZG_NONE) {
+                w1 = w;
+                c = 1;
+            } else {
+                c = 2;
+                break;
+            }
+        }
+    }
+    cls[v] = c;
+    first[v] = c == 1 ? w1 : ZG_NONE;
+}
+
+__global__ void fold_map_kernel(uint32_t V, const uint8_t *cls, const uint32_t *first, uint32_t *succ1, uint32_t *count)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint32_t m = ZG_NONE;
+    if (cls[v] == 1 && cls[first[v]] == 2) m = first[v];
+    succ1[v] = m;
+    if (m != ZG_NONE) atomicAdd(&count[m], 1u);
+}
+
+static int bc_fold_build(zg_graph *g, hipStream_t s)
+{
+    if (g->fold_built) return ZG_OK;
+    const uint32_t V = g->V;
+    uint8_t *d_cls = nullptr;
+    uint32_t *d_first = nullptr;
+    int rc = zg_dmalloc(&d_cls, V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&d_first, V);
+    if (rc == ZG_OK && !g->fold_succ1) rc = zg_dmalloc(&g->fold_succ1, V);
+    if (rc == ZG_OK && !g->fold_count) rc = zg_dmalloc(&g->fold_count, V);
+    if (rc == ZG_OK && V) {
+        rc = zg_hip_code(hipMemsetAsync(g->fold_count, 0, sizeof(uint32_t) * V, s));
+        if (rc == ZG_OK) {
+            hipLaunchKernelGGL(fold_class_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V, g->out_ptr, g->out_col,
+                               g->live, d_cls, d_first);
+            hipLaunchKernelGGL(fold_map_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V, d_cls, d_first, g->fold_succ1,
+                               g->fold_count);
+            rc = zg_hip_code(hipGetLastError());
+        }
+    }
+    if (rc == ZG_OK) {
+        g->h_fold_class.resize(V);
+        g->h_fold_first.resize(V);
+        g->h_fold_succ1.resize(V);
+        g->h_fold_count.resize(V);
+        if (V) {
+            rc = zg_hip_code(hipMemcpyAsync(g->h_fold_class.data(), d_cls, V, hipMemcpyDeviceToHost, s));
+            if (rc == ZG_OK)
+                rc = zg_hip_code(hipMemcpyAsync(g->h_fold_first.data(), d_first, 4ull * V, hipMemcpyDeviceToHost, s));
+            if (rc == ZG_OK)
+                rc = zg_hip_code(hipMemcpyAsync(g->h_fold_succ1.data(), g->fold_succ1, 4ull * V, hipMemcpyDeviceToHost, s));
+            if (rc == ZG_OK)
+                rc = zg_hip_code(hipMemcpyAsync(g->h_fold_count.data(), g->fold_count, 4ull * V, hipMemcpyDeviceToHost, s));
+        }
+        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+    }
+    zg_dfree(d_cls);
+    zg_dfree(d_first);
+    if (rc == ZG_OK) g->fold_built = true;
+    return rc;
+}
+
+extern "C" int zg_fold_map(zg_graph *g, uint8_t *class_out, uint32_t *succ_out, uint32_t *folded_into_out,
+                           uint32_t *count_out)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    ZG_TRY(hipSetDevice(g->device));
+    ZG_TRY_RC(bc_fold_build(g, g->stream));
+    const size_t V = g->V;
+    if (class_out && V) memcpy(class_out, g->h_fold_class.data(), V);
+    if (succ_out && V) memcpy(succ_out, g->h_fold_first.data(), 4 * V);
+    if (folded_into_out && V) memcpy(folded_into_out, g->h_fold_succ1.data(), 4 * V);
+    if (count_out && V) memcpy(count_out, g->h_fold_count.data(), 4 * V);
+    return ZG_OK;
 }
 
 // Discovery rank of every vertex in a breadth-first search from the hub core
@@ -1488,6 +1661,7 @@ struct BcCall {
     const double *h_weights = nullptr;
     double *h_totals = nullptr;
     double *d_out = nullptr;
+    bool fold = false;          // every source stands for itself and the sources folded into it
 };
 
 static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_bc_info *info)
@@ -1514,7 +1688,14 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
     // engine: the per-source queue engine for high-diameter graphs
     if (g->engine == ZG_ENGINE_AUTO || order) ZG_TRY_RC(bc_core_distances(g, s));
     if (order && g->source_order == 2) ZG_TRY_RC(bc_bfs_rank(g, s));
-    const bool queue_engine = g->engine == ZG_ENGINE_QUEUE || (g->engine == ZG_ENGINE_AUTO && g->high_diameter);
+    // the folded sweep runs on the 64-lane engine
+    const bool queue_engine = !call.fold &&
+                              (g->engine == ZG_ENGINE_QUEUE || (g->engine == ZG_ENGINE_AUTO && g->high_diameter));
+    if (call.fold) {
+        ZG_TRY_RC(bc_fold_build(g, s));
+        for (uint32_t x : list)
+            if (g->h_fold_succ1[x] != ZG_NONE) return ZG_ERR_INVALID_ARG;  // counted with its successor
+    }
     std::vector<uint64_t> perm;
     if (order) {
         bc_order_perm(g, list, perm);
@@ -1566,6 +1747,7 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
         BcSweepOpt opt;
         opt.d_weights = d_w;
         opt.h_totals = call.h_totals ? tsorted.data() : nullptr;
+        opt.fold = call.fold;
         if (rc == ZG_OK) rc = bc_sweep(g, d_list, list.data(), np, s, &inf, opt);
         inf.n_sources = n;
         if (rc == ZG_OK && V) {
@@ -1611,7 +1793,7 @@ extern "C" int zg_bc_unweighted_device(zg_graph *g, const uint32_t *d_sources, u
 
 // Host-buffer form shared by zg_bc_unweighted and zg_bc_weighted.
 static int bc_host_call(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources,
-                        double *out_bc, double *out_totals, zg_bc_info *info)
+                        double *out_bc, double *out_totals, zg_bc_info *info, bool fold = false)
 {
     if (sources)
         for (uint64_t i = 0; i < n_sources; i++)
@@ -1633,6 +1815,7 @@ static int bc_host_call(zg_graph *g, const uint32_t *sources, const double *weig
     call.h_weights = weights;
     call.h_totals = out_totals;
     call.d_out = d_out;
+    call.fold = fold;
     if (rc == ZG_OK) rc = zg_bc_device_impl(g, call, s, info);
     if (rc == ZG_OK && out_bc && g->V)
         rc = zg_hip_code(hipMemcpy(out_bc, d_out, sizeof(double) * g->V, hipMemcpyDeviceToHost));
@@ -1657,6 +1840,24 @@ extern "C" int zg_bc_weighted(zg_graph *g, const uint32_t *sources, const double
         for (uint64_t i = 0; i < n_sources; i++)
             if (!std::isfinite(weights[i])) return ZG_ERR_INVALID_ARG;
     return bc_host_call(g, sources, weights, n_sources, out_bc, out_totals, info);
+}
+
+extern "C" int zg_bc_folded(zg_graph *g, const uint32_t *sources, const double *weights, uint64_t n_sources,
+                            double *out_bc, double *out_totals, zg_bc_info *info)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!out_bc && !out_totals) return ZG_ERR_INVALID_ARG;
+    if (!sources && n_sources) return ZG_ERR_INVALID_ARG;
+    if (weights)
+        for (uint64_t i = 0; i < n_sources; i++)
+            if (!std::isfinite(weights[i])) return ZG_ERR_INVALID_ARG;
+    if (!sources) {
+        // no sources: an empty sum
+        if (out_bc && g->V) memset(out_bc, 0, sizeof(double) * g->V);
+        if (info) memset(info, 0, sizeof(*info));
+        return ZG_OK;
+    }
+    return bc_host_call(g, sources, weights, n_sources, out_bc, out_totals, info, true);
 }
 
 struct PathsCtx {

@@ -116,6 +116,7 @@ struct zg_bc_ws {
     uint32_t *src = nullptr;             // [G][64]
     double *wts = nullptr;               // [G][64] per-source weights of a weighted sweep
     unsigned long long *dsum = nullptr;  // [G][64] per-source dependency totals (integers)
+    unsigned long long *rsum = nullptr, *fsum = nullptr;  // [G][64] folded sweep: reached, folded members reached
     uint32_t cmax = 0;                   // most predecessor chunks in one group's span
     void *level_args = nullptr;          // device arguments of the graph-replayed level kernels
     void *h_level_args = nullptr;        // pinned staging copy of them
@@ -225,6 +226,15 @@ struct zg_graph {
     std::vector<uint8_t> h_live;
     std::vector<uint32_t> h_out_deg, h_in_deg;
     std::vector<uint32_t> h_coredist;    // distance to the hub core (lazy)
+    // folding of single-successor sources (zg_fold_map, zg_bc_folded; lazy):
+    // class of each vertex (0: no successor but itself, 1: one distinct
+    // successor, 2: several), that successor, the vertex it is folded into
+    // (its successor when that has several, else ZG_NONE) and the number of
+    // sources folded into each vertex
+    bool fold_built = false;
+    uint32_t *fold_succ1 = nullptr, *fold_count = nullptr;   // device [V]
+    std::vector<uint8_t> h_fold_class;
+    std::vector<uint32_t> h_fold_first, h_fold_succ1, h_fold_count;
     std::vector<uint32_t> h_bfsrank;     // discovery rank in a search from the hub core (lazy)
     int source_order = 1;
     double push_alpha = 8.0;

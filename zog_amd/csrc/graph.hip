@@ -131,6 +131,8 @@ static void zg_free_device(zg_graph *g)
     zg_prof_release(g);
     zg_dfree(g->io_out);
     zg_dfree(g->io_src);
+    zg_dfree(g->fold_succ1);
+    zg_dfree(g->fold_count);
     g->io_src_cap = 0;
     zg_bc_ws_release(g);
     zg_bcq_ws_release(g);

@@ -74,6 +74,54 @@ def bc_weighted(graph: ResourceGraph, sources=None, weights=None, want_bc=True, 
     return bc, tot
 
 
+NONE = 0xFFFFFFFF
+
+
+def fold_map(graph: ResourceGraph):
+    """(class u8, successor u32, folded_into u32, count u32), each [n_cap]
+    (zg_fold_map): class 0 = no successor but itself, 1 = one other, 2 =
+    several; a class-1 vertex whose successor has class 2 is folded into it
+    (its sweep is the successor's, one arc longer); count[m] = vertices
+    folded into m."""
+    res = graph.resource
+    V = res.n_cap
+    cls = np.zeros(V, dtype=np.uint8)
+    succ = np.zeros(V, dtype=np.uint32)
+    into = np.zeros(V, dtype=np.uint32)
+    cnt = np.zeros(V, dtype=np.uint32)
+    _lib.check(_lib.lib().zg_fold_map(res.handle, _ptr(cls), _ptr(succ), _ptr(into), _ptr(cnt)), "zg_fold_map")
+    return cls, succ, into, cnt
+
+
+def bc_folded(graph: ResourceGraph, sources, weights=None, want_bc=True, want_totals=False,
+              info: Optional[dict] = None):
+    """(bc f64[n_cap] or None, totals or None) of zg_bc_folded: each source
+    stands for itself and the sources folded into it."""
+    res = graph.resource
+    s = np.ascontiguousarray(sources, dtype=np.uint32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    bc = np.zeros(res.n_cap, dtype=np.float64) if want_bc else None
+    tot = np.zeros(len(s), dtype=np.float64) if want_totals else None
+    inf = _lib.ZgBcInfo()
+    rc = _lib.lib().zg_bc_folded(res.handle, _ptr(s), _ptr(w), len(s), _ptr(bc), _ptr(tot), ctypes.byref(inf))
+    _lib.check(rc, "zg_bc_folded")
+    if info is not None:
+        info.update({k: getattr(inf, k) for k, _ in _lib.ZgBcInfo._fields_})
+    return bc, tot
+
+
+def sampling_frame(graph: ResourceGraph):
+    """(frame ids, folded count per frame id): the sources a folded sample
+    draws from -- live vertices that are not folded and whose sweep can
+    reach past one arc (class 0, and class 1 into a class-0 successor, add
+    nothing) -- and the size of each one's folded group minus one."""
+    cls, succ, into, cnt = fold_map(graph)
+    live = graph.resource.live_mask()
+    zero = (cls == 0) | ((cls == 1) & (cls[np.minimum(succ, len(cls) - 1)] == 0))
+    frame = np.nonzero(live & (into == NONE) & ~zero)[0].astype(np.int64)
+    return frame, cnt[frame].astype(np.float64)
+
+
 class Scorer:
     """The node-scoring network (one zg_mlp handle on the current device)."""
 
@@ -291,6 +339,37 @@ def predict_totals(graph: ResourceGraph, scorer: Scorer):
     feats = node_features(graph)
     y = scorer.forward(feats).cpu().numpy()[live_ids]
     return predicted_totals(y, n_live), live_ids
+
+
+def sampled_betweenness_folded(graph: ResourceGraph, k: int, rng, pred=None, frame=None, eps: float = 0.05,
+                               info: Optional[dict] = None, slice_sources: int = 1 << 18, log=None):
+    """Estimated BC from about k group representatives of the folded frame
+    (sampling_frame): learned inclusion probabilities proportional to
+    (1 + folded count) * predicted dependency total when pred (per vertex id)
+    is given, uniform over the frame otherwise; Horvitz-Thompson weights
+    1/pi; exact when k covers the frame."""
+    if frame is None:
+        frame = sampling_frame(graph)
+    ids, folded = frame
+    if k >= len(ids):
+        bc, _ = bc_folded(graph, ids, None, info=info)
+        return bc, len(ids)
+    if pred is None:
+        pos = rng.choice(len(ids), size=k, replace=False)
+        src, w = ids[pos].astype(np.uint32), np.full(k, len(ids) / k)
+    else:
+        score = (1.0 + folded) * np.maximum(np.asarray(pred, np.float64)[ids], 0.0)
+        pi = inclusion_probabilities(score, k, eps)
+        order = np.argsort(-score, kind="stable")
+        pos = systematic_sample(pi, order, rng)
+        src, w = ids[pos].astype(np.uint32), 1.0 / pi[pos]
+    bc = np.zeros(graph.resource.n_cap)
+    for a in range(0, len(src), slice_sources):
+        part, _ = bc_folded(graph, src[a:a + slice_sources], w[a:a + slice_sources], info=info)
+        bc += part
+        if log and len(src) > slice_sources:
+            log(f"[pivots] folded sample: {min(a + slice_sources, len(src))}/{len(src)} sources")
+    return bc, len(src)
 
 
 def sampled_betweenness(graph: ResourceGraph, k: int, rng, pred_totals=None, live_ids=None, eps: float = 0.05,
