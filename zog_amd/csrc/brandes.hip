@@ -848,7 +848,8 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     }
     const uint32_t hub = a.ohub_idx[v];
     if (hub != ZG_NONE) {
-        if (active) atomicAdd(&a.hubacc[((size_t)g * a.n_ohub + hub) * 64 + lane], S);
+        // one slot per chunk, summed in chunk order by bc_hub_kernel
+        if (active) a.hubacc[((size_t)g * a.n_ohub + hub + (oc - a.och_first[v])) * 64 + lane] = S;
         return;
     }
     bc_finish(a, g, v, lane, active, S, d);
@@ -867,9 +868,9 @@ __global__ __launch_bounds__(256) void bc_hub_kernel(BcDev a, int d, GroupRange 
     const bool active = (mask >> lane) & 1;
     double S = 0.0;
     if (active) {
-        double *p = &a.hubacc[((size_t)g * a.n_ohub + a.ohub_idx[v]) * 64 + lane];
-        S = *p;
-        *p = 0.0;
+        const double *p = &a.hubacc[((size_t)g * a.n_ohub + a.ohub_idx[v]) * 64 + lane];
+        const uint32_t k = a.och_first[v + 1] - a.och_first[v];
+        for (uint32_t q = 0; q < k; q++) S += p[(size_t)q * 64];
     }
     bc_finish(a, g, v, lane, active, S, d);
 }

@@ -29,8 +29,9 @@
 // unset, a CAS to d+1 (the winner appends w).  Backward level d: delta(v) =
 // sig(v) * sum over successors w at depth d+1 of cc(w) 2^(e_v - e_w)
 // (e_w >= e_v since sigma_w >= sigma_v, so only negligible terms can
-// underflow), cc(v) = (1 + delta(v)) / sig(v), bc[v] += delta(v) for v != s
-// (FP64 atomics across sources).
+// underflow), cc(v) = (1 + delta(v)) / sig(v); delta(v) is kept per (source,
+// vertex) and added into bc[v] source by source after the batch, so the
+// result repeats bit for bit.
 //
 // The level loop is device-driven: every kernel reads its per-source ranges
 // from lstart/qlen, so the host queues ZG_Q_LEVELS_PER_SYNC levels at a time
@@ -45,7 +46,7 @@
 #define ZG_Q_GRID 1024
 #define ZG_Q_LEVELS_PER_SYNC 32
 #define ZG_Q_WIDE 32           // vertices with more predecessors are summed by the whole wavefront
-#define ZG_Q_BYTES_PER_SV 20   // workspace bytes per (source, vertex)
+#define ZG_Q_BYTES_PER_SV 28   // workspace bytes per (source, vertex)
 
 struct BcqDev {
     uint32_t V, S, lcap;
@@ -58,6 +59,7 @@ struct BcqDev {
     uint32_t *newcnt;               // [lcap] entries added at each level
     const uint32_t *src;            // [S]
     const double *wts;              // [S] source weights, or null (all 1)
+    double *dlt;                    // [S][V] weighted dependencies of the batch (summed by bcq_reduce_kernel)
     double *bc;                     // [V]
     unsigned long long *stats;      // [2] arcs, reached
 };
@@ -364,8 +366,21 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
         const double sv = me.val;
         const double delta = sv * S;
         rec[v].val = (1.0 + delta) / sv;  // from here on the dependency term
-        if (v != a.src[s]) atomicAdd(&a.bc[v], a.wts ? delta * a.wts[s] : delta);
+        // kept per (source, vertex) and summed in source order after the batch
+        // (levels >= 1 only: v is never the source here)
+        a.dlt[QROW(s) + v] = a.wts ? delta * a.wts[s] : delta;
     }
+}
+
+// bc[v] += the batch's dependencies of v, source by source in slot order
+// (one thread per vertex, so the sum has one order whatever the timing).
+__global__ void bcq_reduce_kernel(uint32_t V, uint32_t nb, const double *dlt, double *bc)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double acc = bc[v];
+    for (uint32_t s = 0; s < nb; s++) acc += dlt[(size_t)s * V + v];
+    bc[v] = acc;
 }
 
 __global__ void bcq_out_kernel(uint32_t V, const double *bc, const uint8_t *live, double *out)
@@ -377,7 +392,7 @@ __global__ void bcq_out_kernel(uint32_t V, const double *bc, const uint8_t *live
 int zg_bcq_ws_release(zg_graph *g)
 {
     zg_bcq_ws &w = g->bcq;
-    zg_dfree(w.rec); zg_dfree(w.queue); zg_dfree(w.lstart);
+    zg_dfree(w.rec); zg_dfree(w.queue); zg_dfree(w.dlt); zg_dfree(w.lstart);
     zg_dfree(w.qlen); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.wts); zg_dfree(w.bc);
     zg_dfree(w.stats);
     w.S = 0;
@@ -395,6 +410,7 @@ static int bcq_ws_ensure(zg_graph *g, uint32_t S)
     int rc = ZG_OK;
     if (rc == ZG_OK) rc = zg_dmalloc(&w.rec, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.queue, S * V);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.dlt, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.lstart, (size_t)w.lcap * S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.qlen, S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.newcnt, w.lcap);
@@ -458,6 +474,7 @@ static BcqDev bcq_dev(zg_graph *g, bool weighted)
     a.src = w.src;
     a.wts = weighted ? w.wts : nullptr;
     a.bc = w.bc;
+    a.dlt = w.dlt;
     a.stats = w.stats;
     return a;
 }
@@ -496,6 +513,7 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
         if (d_weights)
             ZG_TRY(hipMemcpyAsync(w.wts, d_weights + b0, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
         ZG_TRY(hipMemsetAsync(w.rec, 0xFF, sizeof(zg_qrec) * (size_t)S * V, s));  // depth unset
+        ZG_TRY(hipMemsetAsync(w.dlt, 0, sizeof(double) * (size_t)nb * V, s));      // unreached: no dependency
         ZG_TRY(hipMemsetAsync(w.newcnt, 0, sizeof(uint32_t) * w.lcap, s));
         ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 2, s));
         BcqDev a = bcq_dev(g, d_weights != nullptr);
@@ -526,6 +544,9 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
         }
         for (int L = maxd; L >= 1; L--)
             ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, bcq_backward_kernel, dim3(ZG_Q_GRID), dim3(256), 0, s, a, L);
+        if (V)
+            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bcq_reduce_kernel, dim3((V + 255) / 256), dim3(256), 0, s, V, nb, w.dlt,
+                           w.bc);
         ZG_TRY(hipGetLastError());
         unsigned long long st[2];
         ZG_TRY(hipMemcpyAsync(st, w.stats, sizeof(st), hipMemcpyDeviceToHost, s));

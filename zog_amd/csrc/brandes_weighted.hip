@@ -55,6 +55,7 @@
 #define ZG_W_PIECE 2048
 #define ZG_W_HUB_CAP (1u << 20)  // hub pieces per relax round
 #define ZG_W_UNSET 0xFFFFFFFFFFFFFFFFull
+#define ZG_W_DONE 0xFFFFFFFEu  // aux of a record whose dependency pass is done (dist holds the dependency)
 #define ZG_W_WINDOW 8  // the window pile spans this many bound steps
 
 // far pile p, buffer b, source s
@@ -759,9 +760,26 @@ __global__ __launch_bounds__(256) void wbc_backward_kernel(WbcDev a, int r)
             const double sv = rec[v].sig;
             const double delta = sv * S;
             rec[v].cc = (1.0 + delta) / sv;
-            if (v != src) atomicAdd(&a.bc[v], delta);
+            // the distance is not read again in this batch: it now holds the
+            // dependency, which wbc_reduce_kernel adds in source order
+            rec[v].dist = zg_f64_to_bits(v != src ? delta : 0.0);
+            rec[v].aux = ZG_W_DONE;
         }
     }
+}
+
+// bc[v] += the batch's dependencies of v, source by source in slot order
+// (records the dependency pass finished only).
+__global__ void wbc_reduce_kernel(uint32_t V, uint32_t nb, const zg_wrec *rec, double *bc)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double acc = bc[v];
+    for (uint32_t s = 0; s < nb; s++) {
+        const zg_wrec &r = rec[(size_t)s * V + v];
+        if (r.aux == ZG_W_DONE) acc += zg_bits_to_f64(r.dist);
+    }
+    bc[v] = acc;
 }
 
 __global__ void wbc_out_kernel(uint32_t V, const double *bc, const uint8_t *live, double *out)
@@ -981,6 +999,9 @@ static int wbc_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, double 
         a = wbc_dev(g);
         for (int q = last; q >= 1; q--)
             ZG_PROF_LAUNCH(ZG_PROF_BC_BACKWARD, wbc_backward_kernel, dim3(grid), dim3(256), lds, s, a, q);
+        if (V)
+            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, wbc_reduce_kernel, dim3((V + 255) / 256), dim3(256), 0, s, V, nb, w.rec,
+                           w.bc);
         ZG_TRY(hipGetLastError());
         inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)last + 1);
         inf->push_levels += r;

@@ -152,6 +152,7 @@ struct zg_bcq_ws {
     uint32_t S = 0, lcap = 0;
     zg_qrec *rec = nullptr;              // [S][V]
     uint32_t *queue = nullptr;           // [S][V]
+    double *dlt = nullptr;               // [S][V] weighted dependencies of the batch
     uint32_t *lstart = nullptr;          // [lcap][S]
     uint32_t *qlen = nullptr;            // [S]
     uint32_t *newcnt = nullptr;          // [lcap]
@@ -252,10 +253,10 @@ struct zg_graph {
     // successor chunks (backward sweep work items)
     uint32_t n_och = 0, max_och_per_v = 0;
     uint32_t *och_first = nullptr, *och_v = nullptr, *och_e0 = nullptr, *och_e1 = nullptr;
-    uint32_t n_ohub = 0;
+    uint32_t n_ohub = 0;                 // partial-sum slots of the multi-chunk vertices' chunks
     std::vector<uint32_t> h_och_first;   // [V+1] first successor chunk of each vertex
     std::vector<uint32_t> h_hub_first;   // [V+1] multi-chunk vertices before each vertex
-    uint32_t *ohub_idx = nullptr;        // [V] dense index of multi-chunk vertices or ZG_NONE
+    uint32_t *ohub_idx = nullptr;        // [V] first partial-sum slot of a multi-chunk vertex, or ZG_NONE
     // PageRank row bins
     uint32_t n_small = 0, n_med = 0, n_large = 0, n_seg = 0;
     uint32_t n_m4 = 0, n_m16 = 0, n_m64 = 0;   // n_med = n_m4 + n_m16 + n_m64

@@ -268,7 +268,11 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
                 k++;
             }
         }
-        if (k > 1) ohub_idx[v] = n_ohub++;
+        if (k > 1) {
+            // a vertex with several chunks: one partial-sum slot per chunk
+            ohub_idx[v] = n_ohub;
+            n_ohub += k;
+        }
         max_k = std::max(max_k, k);
     }
     och_first[V] = (uint32_t)och_v.size();
