@@ -917,6 +917,8 @@ def bench_batch(args, steps, warmup):
         "config": {"workload": spec["desc"], "graphs": len(graphs), "nodes": n_tot, "arcs": arcs_tot,
                    "sources": n_tot, "sources_per_launch": 64 * (tuned["groups"] if tuned else args.groups),
                    "autotuned": tuned, "batches": int(step_info["batches"]),
+                   "span_balance": round(step_info["span_balance"], 4),
+                   "idle_lane_share": round(1.0 - n_tot / max(step_info["group_slots"], 1), 5),
                    "arcs_per_step": int(arcs // steps), "levels": int(step_info["max_levels"]),
                    "push_levels": int(step_info["push_levels"]), "pull_levels": int(step_info["pull_levels"])},
         "roofline": roofline, "cpu_baseline": cpu, "parity": parity,

@@ -62,6 +62,12 @@ typedef struct {
     uint64_t row_visits;
     uint64_t row_arcs;
     uint64_t tree_arcs;
+    /* Load balance of the batched sweep: source slots swept (64 per group,
+     * empty slots included; compare n_sources), and the groups' vertex spans
+     * over the span capacity of their launches (1 = every group of every
+     * batch as large as the batch's largest). */
+    uint64_t group_slots;
+    double span_balance;
 } zg_bc_info;
 
 /* Per-run information for PageRank (and the eigenvector, Katz and alpha

@@ -111,6 +111,8 @@ class ZgBcInfo(ctypes.Structure):
         ("row_visits", ctypes.c_uint64),
         ("row_arcs", ctypes.c_uint64),
         ("tree_arcs", ctypes.c_uint64),
+        ("group_slots", ctypes.c_uint64),
+        ("span_balance", ctypes.c_double),
     ]
 
 

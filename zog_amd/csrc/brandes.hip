@@ -1190,6 +1190,7 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
     ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * Gcap * ZG_FIN_BLOCKS * ZG_BC_STATS, s));
     ZG_TRY(hipMemsetAsync(w.tree, 0, sizeof(unsigned long long) * Gcap * ZG_TREE_SLOTS, s));
     std::vector<std::vector<uint32_t>> foff(Gcap), hoff(Gcap);
+    double span_used = 0.0, span_grid = 0.0;  // group spans swept, and the launch grids' span capacity
     for (uint64_t b0 = 0; b0 < n; b0 += (uint64_t)Gcap * 64) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)Gcap * 64, n - b0);
         const uint32_t G = cdiv(nb, 64);
@@ -1210,6 +1211,13 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
             max_c = std::max(max_c, sp.chi[q] - sp.clo[q]);
         }
         BcDev a = bc_dev(g, G, sp, opt);
+        inf->group_slots += 64ull * G;
+        {
+            uint64_t span_sum = 0;
+            for (uint32_t q = 0; q < G; q++) span_sum += sp.vhi[q] - sp.vlo[q];
+            span_used += (double)span_sum;
+            span_grid += (double)G * (double)max_v;
+        }
         // batch reset
         ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * Gcap * 64, s));
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
@@ -1365,6 +1373,7 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         for (unsigned long long t : tr) inf->tree_arcs += t;
     }
     inf->n_sources = n;
+    inf->span_balance = span_grid > 0.0 ? span_used / span_grid : 1.0;
     return ZG_OK;
 }
 
@@ -1578,12 +1587,11 @@ static int bc_bfs_rank(zg_graph *g, hipStream_t s)
     return ZG_OK;
 }
 
-// Order in which the sources are swept: each 64-source group holds sources
-// of one graph (component) at the same distance from its hub core (then
-// larger out-degree first, then id), or (mode 2) in the order a search from
-// the hub core discovers them.  Components are contiguous id ranges,
-// so comparing ids across components orders by component.  Returns the
-// positions of src in sweep order.
+// Order in which the sources are swept: graphs (components) by decreasing
+// size, and inside one graph the sources at the same distance from its hub
+// core together (then larger out-degree first, then id), or (mode 2) in the
+// order a search from the hub core discovers them.  Returns the positions of
+// src in sweep order.
 static void bc_order_perm(const zg_graph *g, const std::vector<uint32_t> &src, std::vector<uint64_t> &perm)
 {
     const std::vector<uint32_t> &cd = g->h_coredist;
@@ -1595,8 +1603,14 @@ static void bc_order_perm(const zg_graph *g, const std::vector<uint32_t> &src, s
     std::stable_sort(perm.begin(), perm.end(), [&](uint64_t i, uint64_t j) {
         const uint32_t x = src[i], y = src[j];
         if (co.size() > 2) {
+            // largest graph first (longest-processing-time order): the groups
+            // of one batch then come from graphs of about the same size, so
+            // one large graph does not hold a batch of small ones
             const auto cx = comp(x), cy = comp(y);
-            if (cx != cy) return cx < cy;
+            if (cx != cy) {
+                const uint32_t nx = co[cx] - co[cx - 1], ny = co[cy] - co[cy - 1];
+                return nx != ny ? nx > ny : cx < cy;
+            }
         }
         if (by_rank) return g->h_bfsrank[x] < g->h_bfsrank[y];
         if (cd[x] != cd[y]) return cd[x] < cd[y];
