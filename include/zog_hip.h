@@ -130,6 +130,42 @@ int zg_set_build_threshold(uint64_t min_arcs);
 int zg_graph_csr(zg_graph *g, uint32_t *out_ptr, uint32_t *out_col, uint32_t *in_ptr, uint32_t *in_col,
                  uint32_t *pr_id);
 
+/* Work tables derived from the CSR at load (on the device), for
+ * zg_graph_table.  Each is an array of u32 words:
+ *   ZG_TABLE_ICH        predecessor chunks: vertex[n], first arc[n], end[n]
+ *   ZG_TABLE_ICH_FIRST  first predecessor chunk of each vertex [n_cap + 1]
+ *   ZG_TABLE_OCH        successor chunks: vertex[n], first arc[n], end[n]
+ *   ZG_TABLE_OCH_FIRST  first successor chunk of each vertex [n_cap + 1]
+ *   ZG_TABLE_OHUB       first partial-sum slot of each vertex with several
+ *                       successor chunks, else 0xFFFFFFFF [n_cap]
+ *   ZG_TABLE_PR_ROWS    PageRank rows in bin order: short, medium (4-, 16-
+ *                       and 64-lane bins) [n_short + n_medium]
+ *   ZG_TABLE_PR_DESC    {PageRank id, first binned arc, in-degree,
+ *                       out-degree} of those rows [4 (n_short + n_medium)]
+ *   ZG_TABLE_PR_LARGE   long rows: PageRank ids[k], out-degrees[k], first
+ *                       segment[k + 1]
+ *   ZG_TABLE_PR_SEG     long-row segments: row[s], first arc[s], end[s]
+ *   ZG_TABLE_PR_COL     predecessor columns as PageRank ids, rows in bin
+ *                       order [arcs]
+ *   ZG_TABLE_PR_BINS    row counts: short, 4-, 16-, 64-lane, long [5] */
+#define ZG_TABLE_ICH 0
+#define ZG_TABLE_ICH_FIRST 1
+#define ZG_TABLE_OCH 2
+#define ZG_TABLE_OCH_FIRST 3
+#define ZG_TABLE_OHUB 4
+#define ZG_TABLE_PR_ROWS 5
+#define ZG_TABLE_PR_DESC 6
+#define ZG_TABLE_PR_LARGE 7
+#define ZG_TABLE_PR_SEG 8
+#define ZG_TABLE_PR_COL 9
+#define ZG_TABLE_PR_BINS 10
+
+/* Copies work table `which` (ZG_TABLE_*) to out[0..cap) and its length in
+ * words to *n (out may be NULL to ask for the length only; cap < *n is
+ * ZG_ERR_INVALID_ARG).  No reference counterpart: for tests and diagnostics
+ * of the load. */
+int zg_graph_table(zg_graph *g, int which, uint32_t *out, uint64_t cap, uint64_t *n);
+
 /* Replaces the parsing half of `nif_read_edgelist` (resource_graph.ex:
  * 990-1084) with `addParsedEdges` (:953-988): reads an edge list (lines of
  * `src dst [weight]`, separators any of space, tab, comma; `#` comments; a
@@ -343,6 +379,11 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   tree arcs (zg_bc_info.tree_arcs) in its backward pass (one atomic per
  *   work-list entry; for measurement runs), 0 (default) does not. */
 #define ZG_TUNE_COUNT_TREE_ARCS 10
+/*   ZG_TUNE_QUEUE_ORDER: 1 (default) runs the per-source queue engine on a
+ *   copy of the CSR renumbered in the order of one breadth-first search, so
+ *   the neighbour records a level touches share lines; 0 keeps the ids.
+ *   Rows keep their arc order: results are the same bit for bit. */
+#define ZG_TUNE_QUEUE_ORDER 11
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the

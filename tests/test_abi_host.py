@@ -43,6 +43,7 @@ def test_null_handle_is_rejected():
     assert L.zg_bc_unweighted(None, None, 0, None, None) == _lib.ZG_ERR_INVALID_HANDLE
     assert L.zg_pagerank(None, 0.85, 10, 1e-4, 32, None, None) == _lib.ZG_ERR_INVALID_HANDLE
     assert L.zg_set_batch_groups(None, 4) == _lib.ZG_ERR_INVALID_HANDLE
+    assert L.zg_graph_table(None, _lib.ZG_TABLE_ICH, None, 0, None) == _lib.ZG_ERR_INVALID_HANDLE
     L.zg_graph_destroy(None)
     assert L.zg_graph_create(3, None, None, None, 0, None) == _lib.ZG_ERR_INVALID_ARG
     assert L.zg_bc_f64(None, None, 0, None, None) == _lib.ZG_ERR_INVALID_HANDLE

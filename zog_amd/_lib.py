@@ -41,6 +41,10 @@ ZG_TUNE_WEIGHTED_BATCH = 7
 ZG_TUNE_WEIGHTED_DELTA = 8
 ZG_TUNE_WEIGHTED_GRID = 9
 ZG_TUNE_COUNT_TREE_ARCS = 10
+ZG_TUNE_QUEUE_ORDER = 11
+# work tables of zg_graph_table
+(ZG_TABLE_ICH, ZG_TABLE_ICH_FIRST, ZG_TABLE_OCH, ZG_TABLE_OCH_FIRST, ZG_TABLE_OHUB, ZG_TABLE_PR_ROWS,
+ ZG_TABLE_PR_DESC, ZG_TABLE_PR_LARGE, ZG_TABLE_PR_SEG, ZG_TABLE_PR_COL, ZG_TABLE_PR_BINS) = range(11)
 ZG_EIG_CONVERGED, ZG_EIG_OSCILLATING, ZG_EIG_ZERO, ZG_EIG_CAP, ZG_EIG_TRIVIAL = range(5)
 
 # Every symbol include/zog_hip.h declares.
@@ -50,6 +54,7 @@ EXPORTED = (
     "zg_graph_create_batch",
     "zg_set_build_threshold",
     "zg_graph_csr",
+    "zg_graph_table",
     "zg_edgelist_read",
     "zg_edgelist_arcs",
     "zg_edgelist_labels",
@@ -192,6 +197,8 @@ def lib():
     L.zg_edgelist_free.restype = None
     L.zg_graph_csr.argtypes = [P, P, P, P, P, P]
     L.zg_graph_csr.restype = ctypes.c_int
+    L.zg_graph_table.argtypes = [P, ctypes.c_int, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    L.zg_graph_table.restype = ctypes.c_int
     L.zg_graph_create_batch.argtypes = [ctypes.c_uint32, P, P, P, P, ctypes.POINTER(P)]
     L.zg_graph_create_batch.restype = ctypes.c_int
     L.zg_graph_components.argtypes = [P, P]

@@ -39,7 +39,7 @@
 // u64 counts there) with a per-(source, level) power-of-two shift k_L taken
 // only when a level's largest exponent passes ZG_SHIFT_T, so counts on
 // high-diameter graphs never overflow; every k is 0 until then.
-#include "common.h"
+#include "build_util.h"
 
 #include <algorithm>
 #include <cmath>
@@ -984,9 +984,10 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
         uint32_t lo, hi;
         bc_group_span(g, h_sources + b0, (uint32_t)std::min<uint64_t>(64, n - b0), &lo, &hi);
         vloc = std::max(vloc, hi - lo);
-        cmax = std::max(cmax, g->h_ich_first[hi] - g->h_ich_first[lo]);
-        och = std::max(och, g->h_och_first[hi] - g->h_och_first[lo]);
-        hubs = std::max(hubs, g->h_hub_first[hi] - g->h_hub_first[lo]);
+        const size_t clo = zg_comp_at(g, lo), chi = zg_comp_at(g, hi);
+        cmax = std::max(cmax, g->h_cb_ich[chi] - g->h_cb_ich[clo]);
+        och = std::max(och, g->h_cb_och[chi] - g->h_cb_och[clo]);
+        hubs = std::max(hubs, g->h_cb_hub[chi] - g->h_cb_hub[clo]);
     }
     const size_t need_f = 64 * (size_t)och + 64 * (size_t)g->max_och_per_v;
     const size_t need_h = 64 * (size_t)hubs + 64;
@@ -1168,7 +1169,7 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used, const GroupSpan &sp, cons
     a.foff = w.foff; a.hoff = w.hoff; a.modes = w.modes;
     a.push_alpha = g->push_alpha;
     for (int q = 0; q < ZG_MAX_GROUPS; q++)
-        a.Eg[q] = sp.vhi[q] > sp.vlo[q] ? g->h_in_ptr[sp.vhi[q]] - g->h_in_ptr[sp.vlo[q]] : 0;
+        a.Eg[q] = sp.vhi[q] > sp.vlo[q] ? g->h_cb_in[zg_comp_at(g, sp.vhi[q])] - g->h_cb_in[zg_comp_at(g, sp.vlo[q])] : 0;
     a.fcap = w.fcap; a.hcap = w.hcap;
     a.n_ohub = g->n_ohub;
     return a;
@@ -1205,8 +1206,8 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
             sp.used[q] = slots;
             while (sp.used[q] > 0 && h_sources[b0 + 64 * q + sp.used[q] - 1] == ZG_NONE) sp.used[q]--;
             if (sp.vhi[q] - sp.vlo[q] > w.vloc) return ZG_ERR_INVALID_ARG;  // workspace sized for other sources
-            sp.clo[q] = g->h_ich_first[sp.vlo[q]];
-            sp.chi[q] = g->h_ich_first[sp.vhi[q]];
+            sp.clo[q] = g->h_cb_ich[zg_comp_at(g, sp.vlo[q])];
+            sp.chi[q] = g->h_cb_ich[zg_comp_at(g, sp.vhi[q])];
             max_v = std::max(max_v, sp.vhi[q] - sp.vlo[q]);
             max_c = std::max(max_c, sp.chi[q] - sp.clo[q]);
         }
@@ -1396,52 +1397,109 @@ __global__ void bc_coredist_kernel(uint32_t V, const uint32_t *out_ptr, const ui
 #define ZG_CORE_SIZE 16
 #define ZG_CORE_MAX_LEVELS 64
 
-// Distance of every vertex to the ZG_CORE_SIZE highest in-degree vertices
-// (capped at ZG_CORE_MAX_LEVELS levels; unreachable = ZG_NONE), cached.
+// Component c of vertex v: co[c] <= v < co[c + 1].
+__device__ __forceinline__ uint32_t bc_comp_of(const uint32_t *co, uint32_t ncomp, uint32_t v)
+{
+    uint32_t lo = 0, hi = ncomp;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (co[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Sort keys for the choice of the hub core: live vertices by decreasing
+// in-degree, then the vertices that are not live; a second stable sort by
+// component (keys = components of the first sort's output) groups them by
+// graph.
+__global__ void bc_core_key_kernel(uint32_t V, const uint32_t *in_ptr, const uint8_t *live, uint32_t *key,
+                                   uint32_t *val)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const uint32_t indeg = in_ptr[v + 1] - in_ptr[v];
+        key[v] = live[v] ? 0xFFFFFFFEu - min(indeg, 0xFFFFFFFEu) : 0xFFFFFFFFu;
+        val[v] = v;
+    }
+}
+
+__global__ void bc_core_comp_kernel(uint32_t V, uint32_t ncomp, const uint32_t *co, const uint32_t *vtx, uint32_t *key)
+{
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < V; j += gridDim.x * blockDim.x)
+        key[j] = bc_comp_of(co, ncomp, vtx[j]);
+}
+
+// The first `per` live vertices of each component in key order are the core
+// (level 0).
+__global__ void bc_core_mark_kernel(uint32_t ncomp, uint32_t per, const uint32_t *co, const uint32_t *order,
+                                    const uint8_t *live, uint32_t *dist)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncomp; c += gridDim.x * blockDim.x)
+        for (uint32_t j = co[c]; j < co[c + 1] && j - co[c] < per; j++) {
+            const uint32_t v = order[j];
+            if (!live[v]) break;  // the component's not-live vertices come last
+            dist[v] = 0;
+        }
+}
+
+// Distance of every vertex to the hub core -- the ZG_CORE_SIZE highest
+// in-degree live vertices of a single graph, the highest one of each graph
+// of a batch -- capped at ZG_CORE_MAX_LEVELS levels (unreachable = ZG_NONE);
+// computed on the device, kept on the host with the out-degrees for the
+// sweep order of later calls.
 static int bc_core_distances(zg_graph *g, hipStream_t s)
 {
     if (!g->h_coredist.empty() || g->V == 0) return ZG_OK;
     const uint32_t V = g->V;
-    std::vector<uint32_t> dist(V, ZG_NONE);
     const std::vector<uint32_t> &co = g->h_comp_off;
-    const size_t ncomp = co.size() - 1;
-    // the core of each component: its highest in-degree live vertices
-    // (ZG_CORE_SIZE of them in a single graph, one per graph in a batch)
+    const uint32_t ncomp = (uint32_t)(co.size() - 1);
     const uint32_t per = ncomp == 1 ? ZG_CORE_SIZE : 1;
-    std::vector<uint32_t> order;
-    for (size_t c = 0; c < ncomp; c++) {
-        order.clear();
-        for (uint32_t v = co[c]; v < co[c + 1]; v++)
-            if (g->h_live[v]) order.push_back(v);
-        const size_t k = std::min<size_t>(per, order.size());
-        std::partial_sort(order.begin(), order.begin() + k, order.end(), [&](uint32_t x, uint32_t y) {
-            if (g->h_in_deg[x] != g->h_in_deg[y]) return g->h_in_deg[x] > g->h_in_deg[y];
-            return x < y;
-        });
-        for (size_t i = 0; i < k; i++) dist[order[i]] = 0;
+    ZgScratch sc;
+    uint32_t *d_co, *d_dist, *d_changed, *key, *key2, *val, *val2, *order;
+    ZG_TRY_RC(sc.get(&d_co, co.size()));
+    ZG_TRY_RC(sc.get(&d_dist, V));
+    ZG_TRY_RC(sc.get(&d_changed, 1));
+    ZG_TRY_RC(sc.get(&key, V));
+    ZG_TRY_RC(sc.get(&key2, V));
+    ZG_TRY_RC(sc.get(&val, V));
+    ZG_TRY_RC(sc.get(&val2, V));
+    ZG_TRY(hipMemcpyAsync(d_co, co.data(), sizeof(uint32_t) * co.size(), hipMemcpyHostToDevice, s));
+    const dim3 grd(std::min<uint32_t>(cdiv(V, 256), 8192));
+    hipLaunchKernelGGL(bc_core_key_kernel, grd, dim3(256), 0, s, V, g->in_ptr, g->live, key, val);
+    ZG_TRY(hipGetLastError());
+    ZG_TRY_RC(zg_sort_pairs(key, key2, val, val2, V, 32, s, sc));
+    order = val2;
+    if (ncomp > 1) {
+        hipLaunchKernelGGL(bc_core_comp_kernel, grd, dim3(256), 0, s, V, ncomp, d_co, val2, key);
+        ZG_TRY(hipGetLastError());
+        unsigned bits = 1;
+        while (bits < 32 && ((ncomp - 1) >> bits)) bits++;
+        ZG_TRY_RC(zg_sort_pairs(key, key2, val2, val, V, bits, s, sc));
+        order = val;
     }
-    uint32_t *d_dist = nullptr, *d_changed = nullptr;
-    int rc = zg_dmalloc(&d_dist, V);
-    if (rc == ZG_OK) rc = zg_dmalloc(&d_changed, 1);
-    if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(d_dist, dist.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice, s));
-    for (uint32_t d = 0; rc == ZG_OK && d < ZG_CORE_MAX_LEVELS; d++) {
+    ZG_TRY(hipMemsetAsync(d_dist, 0xFF, sizeof(uint32_t) * V, s));
+    hipLaunchKernelGGL(bc_core_mark_kernel, dim3(std::min<uint32_t>(cdiv(ncomp, 256), 8192)), dim3(256), 0, s, ncomp,
+                       per, d_co, order, g->live, d_dist);
+    ZG_TRY(hipGetLastError());
+    for (uint32_t d = 0; d < ZG_CORE_MAX_LEVELS; d++) {
         uint32_t changed = 0;
-        rc = zg_hip_code(hipMemsetAsync(d_changed, 0, sizeof(uint32_t), s));
-        if (rc != ZG_OK) break;
+        ZG_TRY(hipMemsetAsync(d_changed, 0, sizeof(uint32_t), s));
         hipLaunchKernelGGL(bc_coredist_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V, g->out_ptr, g->out_col, d_dist,
                            d, d_changed);
-        rc = zg_hip_code(hipGetLastError());
-        if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(&changed, d_changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+        ZG_TRY(hipGetLastError());
+        ZG_TRY(hipMemcpyAsync(&changed, d_changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
         if (!changed) break;
         if (d + 1 == ZG_CORE_MAX_LEVELS) g->high_diameter = true;  // still growing at the cap
     }
-    if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(dist.data(), d_dist, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, s));
-    if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
-    zg_dfree(d_dist);
-    zg_dfree(d_changed);
-    if (rc == ZG_OK) g->h_coredist.swap(dist);
-    return rc;
+    std::vector<uint32_t> dist(V), ptr((size_t)V + 1);
+    ZG_TRY(hipMemcpyAsync(dist.data(), d_dist, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, s));
+    ZG_TRY(hipMemcpyAsync(ptr.data(), g->out_ptr, sizeof(uint32_t) * ptr.size(), hipMemcpyDeviceToHost, s));
+    ZG_TRY(hipStreamSynchronize(s));
+    g->h_out_deg.resize(V);
+    for (uint32_t v = 0; v < V; v++) g->h_out_deg[v] = ptr[(size_t)v + 1] - ptr[v];
+    g->h_coredist.swap(dist);
+    return ZG_OK;
 }
 
 // Folding of single-successor sources.  A source s whose only successor
@@ -1562,7 +1620,9 @@ static int bc_bfs_rank(zg_graph *g, hipStream_t s)
         ZG_TRY(hipMemcpyAsync(col.data(), g->in_col, sizeof(uint32_t) * g->E, hipMemcpyDeviceToHost, s));
         ZG_TRY(hipStreamSynchronize(s));
     }
-    const std::vector<uint32_t> &ip = g->h_in_ptr;
+    std::vector<uint32_t> ip((size_t)V + 1);
+    ZG_TRY(hipMemcpyAsync(ip.data(), g->in_ptr, sizeof(uint32_t) * ip.size(), hipMemcpyDeviceToHost, s));
+    ZG_TRY(hipStreamSynchronize(s));
     std::vector<uint32_t> rank(V, ZG_NONE), queue;
     queue.reserve(V);
     for (uint32_t v = 0; v < V; v++)

@@ -36,7 +36,7 @@
 // The level loop is device-driven: every kernel reads its per-source ranges
 // from lstart/qlen, so the host queues ZG_Q_LEVELS_PER_SYNC levels at a time
 // and syncs only to find the last level.
-#include "common.h"
+#include "build_util.h"
 #include "queue_map.h"
 
 #include <algorithm>
@@ -383,10 +383,185 @@ __global__ void bcq_reduce_kernel(uint32_t V, uint32_t nb, const double *dlt, do
     bc[v] = acc;
 }
 
-__global__ void bcq_out_kernel(uint32_t V, const double *bc, const uint8_t *live, double *out)
+// perm: internal id of each vertex (locality order), or null
+__global__ void bcq_out_kernel(uint32_t V, const double *bc, const uint8_t *live, const uint32_t *perm, double *out)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) out[v] = live[v] ? bc[v] : 0.0;
+    if (v < V) out[v] = live[v] ? bc[perm ? perm[v] : v] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Locality order (ZG_TUNE_QUEUE_ORDER, on by default).  A level touches, for
+// every queued (source, vertex) pair, the 16-byte records of the vertex's
+// neighbours.  With the vertices numbered in the order of one breadth-first
+// search (over arcs in both directions, from the lowest live vertex with an
+// arc; vertices it does not reach follow in id order), the rings of another
+// source's search run along stretches of consecutive ids, so neighbouring
+// records share lines (2048 x 2048 grid, 512 sources: 15% less time than
+// row-major ids, 12% for a Z-order).  The engine runs on a copy of the CSR
+// under these ids in which every row keeps its arc order, so every sum, and
+// the result, is the same bit for bit.
+
+__global__ void bcq_start_kernel(uint32_t V, const uint32_t *out_ptr, const uint32_t *in_ptr, const uint8_t *live,
+                                 uint32_t *st)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+        if (live[v] && (out_ptr[v + 1] > out_ptr[v] || in_ptr[v + 1] > in_ptr[v])) atomicMin(&st[0], v);
+}
+
+// st = {start, level begin, level end, ids given}
+__global__ void bcq_seed_kernel(uint32_t *perm, uint32_t *inv, uint32_t *st)
+{
+    const uint32_t v = st[0];
+    if (v != ZG_NONE) {
+        perm[v] = 0;
+        inv[0] = v;
+    }
+    st[1] = 0;
+    st[2] = st[3] = v != ZG_NONE ? 1u : 0u;
+}
+
+// One level of the search: every vertex of [st[1], st[2]) gives the next ids
+// to its neighbours not yet numbered.
+__global__ void bcq_bfs_kernel(const uint32_t *out_ptr, const uint32_t *out_col, const uint32_t *in_ptr,
+                               const uint32_t *in_col, uint32_t *perm, uint32_t *inv, uint32_t *st)
+{
+    const uint32_t lo = st[1], hi = st[2];
+    for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+        const uint32_t v = inv[i];
+        for (int dir = 0; dir < 2; dir++) {
+            const uint32_t *ptr = dir ? in_ptr : out_ptr, *col = dir ? in_col : out_col;
+            for (uint32_t e = ptr[v]; e < ptr[v + 1]; e++) {
+                const uint32_t w = col[e];
+                if (perm[w] == ZG_NONE && atomicCAS(&perm[w], ZG_NONE, ZG_NONE - 1) == ZG_NONE) {
+                    const uint32_t id = atomicAdd(&st[3], 1u);
+                    perm[w] = id;
+                    inv[id] = w;
+                }
+            }
+        }
+    }
+}
+
+__global__ void bcq_bfs_close_kernel(uint32_t *st)
+{
+    st[1] = st[2];
+    st[2] = st[3];
+}
+
+__global__ void bcq_rest_flag_kernel(uint32_t V, const uint32_t *perm, uint32_t *flag)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x)
+        flag[v] = v < V && perm[v] == ZG_NONE ? 1u : 0u;
+}
+
+// The vertices the search did not reach, in id order after the others.
+__global__ void bcq_rest_kernel(uint32_t V, const uint32_t *flag, const uint32_t *rank, const uint32_t *st,
+                                uint32_t *perm, uint32_t *inv)
+{
+    const uint32_t base = st[3];
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+        if (flag[v]) {
+            perm[v] = base + rank[v];
+            inv[base + rank[v]] = v;
+        }
+}
+
+__global__ void bcq_pdeg_kernel(uint32_t V, const uint32_t *inv, const uint32_t *ptr, uint32_t *deg)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= V; i += gridDim.x * blockDim.x)
+        deg[i] = i < V ? ptr[inv[i] + 1] - ptr[inv[i]] : 0u;
+}
+
+// One wavefront per row: row i is row inv[i] with its heads renumbered.
+__global__ void bcq_prow_kernel(uint32_t V, const uint32_t *inv, const uint32_t *perm, const uint32_t *ptr,
+                                const uint32_t *col, const uint32_t *nptr, uint32_t *ncol)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < V; i += (gridDim.x * blockDim.x) >> 6) {
+        const uint32_t v = inv[i], a = ptr[v], k = ptr[v + 1] - a, b = nptr[i];
+        for (uint32_t j = lane; j < k; j += 64) ncol[b + j] = perm[col[a + j]];
+    }
+}
+
+__global__ void bcq_map_src_kernel(uint32_t S, const uint32_t *perm, uint32_t *src)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S && src[s] != ZG_NONE) src[s] = perm[src[s]];
+}
+
+void zg_bcq_order_release(zg_graph *g)
+{
+    zg_dfree(g->qperm);
+    zg_dfree(g->q_out_ptr);
+    zg_dfree(g->q_out_col);
+    zg_dfree(g->q_in_ptr);
+    zg_dfree(g->q_in_col);
+    zg_dfree(g->qadj);
+}
+
+static int bcq_build_order_into(zg_graph *g, hipStream_t s)
+{
+    const uint32_t V = g->V;
+    const uint64_t E = g->E;
+    const size_t V1 = (size_t)V + 1;
+    ZgScratch sc;
+    uint32_t *inv, *st, *flag, *rank, *deg;
+    ZG_TRY_RC(sc.get(&inv, V));
+    ZG_TRY_RC(sc.get(&st, 4));
+    ZG_TRY_RC(sc.get(&flag, V1));
+    ZG_TRY_RC(sc.get(&rank, V1));
+    ZG_TRY_RC(sc.get(&deg, V1));
+    ZG_TRY_RC(zg_dmalloc(&g->qperm, V));
+    ZG_TRY(hipMemsetAsync(g->qperm, 0xFF, sizeof(uint32_t) * V, s));
+    ZG_TRY(hipMemsetAsync(st, 0xFF, sizeof(uint32_t), s));
+    const dim3 blk(256), grd(std::min<uint32_t>((V + 255) / 256, 8192));
+    hipLaunchKernelGGL(bcq_start_kernel, grd, blk, 0, s, V, g->out_ptr, g->in_ptr, g->live, st);
+    hipLaunchKernelGGL(bcq_seed_kernel, dim3(1), dim3(1), 0, s, g->qperm, inv, st);
+    ZG_TRY(hipGetLastError());
+    for (;;) {
+        for (int k = 0; k < ZG_Q_LEVELS_PER_SYNC; k++) {
+            hipLaunchKernelGGL(bcq_bfs_kernel, dim3(ZG_Q_GRID), blk, 0, s, g->out_ptr, g->out_col, g->in_ptr, g->in_col,
+                               g->qperm, inv, st);
+            hipLaunchKernelGGL(bcq_bfs_close_kernel, dim3(1), dim3(1), 0, s, st);
+        }
+        ZG_TRY(hipGetLastError());
+        uint32_t h[4];
+        ZG_TRY(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+        if (h[1] == h[2]) break;  // no vertex was numbered in the last level
+    }
+    hipLaunchKernelGGL(bcq_rest_flag_kernel, grd, blk, 0, s, V, g->qperm, flag);
+    ZG_TRY(hipGetLastError());
+    ZG_TRY_RC(zg_exclusive_scan(flag, rank, V1, s, sc));
+    hipLaunchKernelGGL(bcq_rest_kernel, grd, blk, 0, s, V, flag, rank, st, g->qperm, inv);
+    ZG_TRY(hipGetLastError());
+    // the CSR under the new ids, rows in their arc order
+    for (int dir = 0; dir < 2; dir++) {
+        const uint32_t *ptr = dir ? g->in_ptr : g->out_ptr, *col = dir ? g->in_col : g->out_col;
+        uint32_t **nptr = dir ? &g->q_in_ptr : &g->q_out_ptr, **ncol = dir ? &g->q_in_col : &g->q_out_col;
+        ZG_TRY_RC(zg_dmalloc(nptr, V1));
+        ZG_TRY_RC(zg_dmalloc(ncol, E));
+        hipLaunchKernelGGL(bcq_pdeg_kernel, grd, blk, 0, s, V, inv, ptr, deg);
+        ZG_TRY(hipGetLastError());
+        ZG_TRY_RC(zg_exclusive_scan(deg, *nptr, V1, s, sc));
+        hipLaunchKernelGGL(bcq_prow_kernel, dim3(std::min<uint32_t>((V + 3) / 4, 16384)), blk, 0, s, V, inv, g->qperm,
+                           ptr, col, *nptr, *ncol);
+        ZG_TRY(hipGetLastError());
+    }
+    ZG_TRY(hipStreamSynchronize(s));
+    return ZG_OK;
+}
+
+// Builds the order and the renumbered CSR, or leaves none of it behind.
+static int bcq_build_order(zg_graph *g, hipStream_t s)
+{
+    const int rc = bcq_build_order_into(g, s);
+    if (rc != ZG_OK) {
+        (void)hipStreamSynchronize(s);
+        zg_bcq_order_release(g);
+    }
+    return rc;
 }
 
 int zg_bcq_ws_release(zg_graph *g)
@@ -461,10 +636,11 @@ static BcqDev bcq_dev(zg_graph *g, bool weighted)
     a.V = g->V;
     a.S = w.S;
     a.lcap = w.lcap;
-    a.out_ptr = g->out_ptr;
-    a.out_col = g->out_col;
-    a.in_ptr = g->in_ptr;
-    a.in_col = g->in_col;
+    const bool ord = g->qperm != nullptr;  // locality order built (and on)
+    a.out_ptr = ord ? g->q_out_ptr : g->out_ptr;
+    a.out_col = ord ? g->q_out_col : g->out_col;
+    a.in_ptr = ord ? g->q_in_ptr : g->in_ptr;
+    a.in_col = ord ? g->q_in_col : g->in_col;
     a.adj = g->qadj;
     a.rec = w.rec;
     a.queue = w.queue;
@@ -498,10 +674,13 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
     }
     ZG_TRY_RC(bcq_ws_ensure(g, S));
     zg_bcq_ws &w = g->bcq;
-    if (!g->qadj && V) {  // the inline adjacency, once per resource
+    if (g->queue_order && !g->qperm && V) ZG_TRY_RC(bcq_build_order(g, s));  // once per resource
+    const bool ord = g->qperm != nullptr;
+    if (!g->qadj && V) {  // the inline adjacency, once per resource (and order)
         ZG_TRY_RC(zg_dmalloc(&g->qadj, (size_t)V * 4));
         hipLaunchKernelGGL(bcq_adj_kernel, dim3(std::min<uint32_t>((V + 255) / 256, 8192)), dim3(256), 0, s, V,
-                           g->in_ptr, g->in_col, g->out_ptr, g->out_col, g->qadj);
+                           ord ? g->q_in_ptr : g->in_ptr, ord ? g->q_in_col : g->in_col,
+                           ord ? g->q_out_ptr : g->out_ptr, ord ? g->q_out_col : g->out_col, g->qadj);
         ZG_TRY(hipGetLastError());
     }
     ZG_TRY(hipMemsetAsync(w.bc, 0, sizeof(double) * (V ? V : 1), s));
@@ -510,6 +689,10 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
         const uint32_t nb = (uint32_t)std::min<uint64_t>(S, n - b0);
         ZG_TRY(hipMemsetAsync(w.src, 0xFF, sizeof(uint32_t) * S, s));
         ZG_TRY(hipMemcpyAsync(w.src, d_sources + b0, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
+        if (ord) {
+            hipLaunchKernelGGL(bcq_map_src_kernel, dim3((S + 255) / 256), dim3(256), 0, s, S, g->qperm, w.src);
+            ZG_TRY(hipGetLastError());
+        }
         if (d_weights)
             ZG_TRY(hipMemcpyAsync(w.wts, d_weights + b0, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
         ZG_TRY(hipMemsetAsync(w.rec, 0xFF, sizeof(zg_qrec) * (size_t)S * V, s));  // depth unset
@@ -570,7 +753,7 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
     }
     if (V) {
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bcq_out_kernel, dim3((V + 255) / 256), dim3(256), 0, s, V, w.bc, g->live,
-                       d_out);
+                       g->qperm, d_out);
         ZG_TRY(hipGetLastError());
     }
     inf->n_sources = n;

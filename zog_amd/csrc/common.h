@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -225,7 +226,7 @@ struct zg_graph {
     uint32_t *in_ptr = nullptr, *in_col = nullptr;
     uint8_t *live = nullptr;
     std::vector<uint8_t> h_live;
-    std::vector<uint32_t> h_out_deg, h_in_deg;
+    std::vector<uint32_t> h_out_deg;     // out-degrees (lazy, with h_coredist)
     std::vector<uint32_t> h_coredist;    // distance to the hub core (lazy)
     // folding of single-successor sources (zg_fold_map, zg_bc_folded; lazy):
     // class of each vertex (0: no successor but itself, 1: one distinct
@@ -243,20 +244,19 @@ struct zg_graph {
     // disjoint components given at creation (zg_graph_create_batch): vertex
     // offsets, n_comp + 1 entries; a plain graph is one component [0, V)
     std::vector<uint32_t> h_comp_off;
+    // work tables (graph_derive.hip), built on the device at load
     // predecessor chunks (forward sweep work items)
     uint32_t n_ich = 0;
+    uint32_t *ich_first = nullptr;       // [V+1] first predecessor chunk of each vertex
     uint32_t *ich_v = nullptr, *ich_e0 = nullptr, *ich_e1 = nullptr;
-    std::vector<uint32_t> h_in_ptr;      // [V+1] host copy of the predecessor CSR offsets
-    std::vector<uint32_t> h_ich_first;   // [V+1] first predecessor chunk of each vertex
-    uint32_t n_inhub = 0;
-    uint32_t *inhub = nullptr;           // vertices with more than one predecessor chunk
     // successor chunks (backward sweep work items)
     uint32_t n_och = 0, max_och_per_v = 0;
     uint32_t *och_first = nullptr, *och_v = nullptr, *och_e0 = nullptr, *och_e1 = nullptr;
     uint32_t n_ohub = 0;                 // partial-sum slots of the multi-chunk vertices' chunks
-    std::vector<uint32_t> h_och_first;   // [V+1] first successor chunk of each vertex
-    std::vector<uint32_t> h_hub_first;   // [V+1] multi-chunk vertices before each vertex
     uint32_t *ohub_idx = nullptr;        // [V] first partial-sum slot of a multi-chunk vertex, or ZG_NONE
+    // at each component offset h_comp_off[i]: predecessor CSR offset, first
+    // predecessor chunk, first successor chunk, multi-chunk vertices before it
+    std::vector<uint32_t> h_cb_in, h_cb_ich, h_cb_och, h_cb_hub;
     // PageRank row bins
     uint32_t n_small = 0, n_med = 0, n_large = 0, n_seg = 0;
     uint32_t n_m4 = 0, n_m16 = 0, n_m64 = 0;   // n_med = n_m4 + n_m16 + n_m64
@@ -273,9 +273,14 @@ struct zg_graph {
     int engine = 0;                      // ZG_ENGINE_*
     double *out_w = nullptr;             // [E] arc weights in successor-CSR order (weighted resources)
     uint4 *qadj = nullptr;               // [V][4] inline adjacency of the queue engine (on demand)
+    // locality order of the queue engine (ZG_TUNE_QUEUE_ORDER; on demand):
+    // the internal id of each vertex and the CSR under those ids, every row
+    // in its arc order
+    bool queue_order = true;
+    uint32_t *qperm = nullptr;           // [V]
+    uint32_t *q_out_ptr = nullptr, *q_out_col = nullptr, *q_in_ptr = nullptr, *q_in_col = nullptr;
     bool weights_ok = true;              // every live arc's weight is finite and non-negative
     bool built_on_gpu = false;           // the CSR came from zg_build_csr_gpu
-    uint32_t max_in_deg = 0;             // largest in-degree
     uint32_t wbc_batch = 0;              // weighted sweep: sources per batch (0 = default)
     double wbc_delta = 0.0;              // weighted sweep: bound step of the near frontier (0 = default)
     uint32_t wbc_grid = 0;               // weighted sweep: workgroups per round kernel (0 = default)
@@ -318,14 +323,23 @@ void zg_prof_release(zg_graph *g);
         zg_prof_end(g, s, (slot), _zp);              \
     } while (0)
 
-int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
-                     std::vector<uint32_t> &out_ptr, std::vector<uint32_t> &in_ptr, std::vector<uint32_t> &pr_order);
+int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m);
+// Chunk lists, hub slots and PageRank row bins from the device CSR and
+// PageRank ids (graph_derive.hip).
+int zg_derive_tables(zg_graph *g);
+// Index i of the component offset h_comp_off[i] == v (v is 0, V or a
+// component's first vertex).
+static inline size_t zg_comp_at(const zg_graph *g, uint32_t v)
+{
+    return (size_t)(std::lower_bound(g->h_comp_off.begin(), g->h_comp_off.end(), v) - g->h_comp_off.begin());
+}
 // The handle's staging buffers: *out = [V] doubles, *src = [n] u32 (grown
 // as needed); the caller holds the handle's lock.
 int zg_io_out(zg_graph *g, double **out);
 int zg_io_src(zg_graph *g, uint64_t n, uint32_t **src);
 int zg_bc_ws_release(zg_graph *g);
 int zg_bcq_ws_release(zg_graph *g);
+void zg_bcq_order_release(zg_graph *g);  // the queue engine's locality order and inline adjacency
 int zg_wbc_ws_release(zg_graph *g);
 int zg_pr_ws_release(zg_graph *g);
 int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const double *d_weights, double *h_totals,

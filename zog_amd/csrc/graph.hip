@@ -139,10 +139,10 @@ static void zg_free_device(zg_graph *g)
     zg_wbc_ws_release(g);
     zg_pr_ws_release(g);
     zg_dfree(g->out_ptr); zg_dfree(g->out_col); zg_dfree(g->in_ptr); zg_dfree(g->in_col); zg_dfree(g->out_w);
-    zg_dfree(g->qadj);
+    zg_bcq_order_release(g);
     zg_dfree(g->und_ptr); zg_dfree(g->und_col);
     zg_dfree(g->live);
-    zg_dfree(g->ich_v); zg_dfree(g->ich_e0); zg_dfree(g->ich_e1); zg_dfree(g->inhub);
+    zg_dfree(g->ich_first); zg_dfree(g->ich_v); zg_dfree(g->ich_e0); zg_dfree(g->ich_e1);
     zg_dfree(g->och_first); zg_dfree(g->och_v); zg_dfree(g->och_e0); zg_dfree(g->och_e1);
     zg_dfree(g->ohub_idx);
     zg_dfree(g->pr_small); zg_dfree(g->pr_med); zg_dfree(g->pr_desc); zg_dfree(g->pr_large); zg_dfree(g->pr_large_seg0);
@@ -165,9 +165,7 @@ static int zg_upload(T **d, const std::vector<T> &h)
 // device form, used from ZG_GPU_BUILD_MIN_ARCS arcs on; both produce the
 // same arrays): csr_host.cpp builds them, this uploads them with the weights
 // and the live mask.
-static int zg_build_csr_host(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
-                             std::vector<uint32_t> &out_ptr, std::vector<uint32_t> &in_ptr,
-                             std::vector<uint32_t> &pr_order)
+static int zg_build_csr_host(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m)
 {
     ZgHostCsr c;
     ZG_TRY_RC(zg_host_csr(g->V, g->h_live.data(), from, to, weight, m, c));
@@ -182,21 +180,7 @@ static int zg_build_csr_host(zg_graph *g, const uint32_t *from, const uint32_t *
     ZG_TRY_RC(zg_upload(&g->live, g->h_live));
     ZG_TRY_RC(zg_upload(&g->pr_id, c.pr_id));
     ZG_TRY_RC(zg_upload(&g->pr_col, c.pr_col));
-    out_ptr.swap(c.out_ptr);
-    in_ptr.swap(c.in_ptr);
-    pr_order.swap(c.pr_order);
     return ZG_OK;
-}
-
-// One wavefront per row: copies row r's n[r] columns from src[r] to dst[r].
-__global__ void zg_rebin_kernel(uint32_t rows, const uint32_t *src, const uint32_t *dst, const uint32_t *n,
-                                const uint32_t *col, uint32_t *out)
-{
-    const uint32_t lane = threadIdx.x & 63;
-    for (uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < rows; r += (gridDim.x * blockDim.x) >> 6) {
-        const uint32_t a = src[r], b = dst[r], k = n[r];
-        for (uint32_t i = lane; i < k; i += 64) out[b + i] = col[a + i];
-    }
 }
 
 static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
@@ -210,209 +194,12 @@ static int zg_build(zg_graph *g, const uint8_t *live_mask, const uint32_t *from,
     uint32_t n_live = 0;
     for (uint32_t i = 0; i < V; i++) n_live += g->h_live[i];
     g->n_live = n_live;
-
-    std::vector<uint32_t> out_ptr, in_ptr, pr_order;
     const uint64_t gpu_min = g_build_gpu_min_arcs;
-    if (m >= gpu_min) ZG_TRY_RC(zg_build_csr_gpu(g, from, to, weight, m, out_ptr, in_ptr, pr_order));
-    else ZG_TRY_RC(zg_build_csr_host(g, from, to, weight, m, out_ptr, in_ptr, pr_order));
+    if (m >= gpu_min) ZG_TRY_RC(zg_build_csr_gpu(g, from, to, weight, m));
+    else ZG_TRY_RC(zg_build_csr_host(g, from, to, weight, m));
     g->built_on_gpu = m >= gpu_min;
-    const uint64_t E = g->E;
-    std::vector<uint32_t> pr_id(V);
-    for (uint32_t i = 0; i < V; i++) pr_id[pr_order[i]] = i;
-
-    g->h_out_deg.resize(V);
-    g->h_in_deg.resize(V);
-    uint32_t n_dang = 0;
-    for (uint32_t v = 0; v < V; v++) {
-        g->h_in_deg[v] = in_ptr[(size_t)v + 1] - in_ptr[v];
-        g->h_out_deg[v] = out_ptr[(size_t)v + 1] - out_ptr[v];
-        g->max_in_deg = std::max(g->max_in_deg, g->h_in_deg[v]);
-        if (g->h_live[v] && g->h_out_deg[v] == 0) n_dang++;
-    }
-    g->n_dangling = n_dang;
-
-    // Predecessor chunks: forward-sweep work items (vertices with no
-    // predecessor are reached only as sources and get no chunk).
-    std::vector<uint32_t> ich_v, ich_e0, ich_e1, inhub;
-    g->h_ich_first.assign((size_t)V + 1, 0);
-    for (uint32_t v = 0; v < V; v++) {
-        g->h_ich_first[v] = (uint32_t)ich_v.size();
-        uint32_t b = in_ptr[v], e = in_ptr[(size_t)v + 1];
-        if (b == e || !g->h_live[v]) continue;
-        if (e - b > ZG_CHUNK) inhub.push_back(v);
-        for (uint32_t c = b; c < e; c += ZG_CHUNK) {
-            ich_v.push_back(v);
-            ich_e0.push_back(c);
-            ich_e1.push_back(std::min<uint32_t>(c + ZG_CHUNK, e));
-        }
-    }
-    g->n_ich = (uint32_t)ich_v.size();
-    g->h_ich_first[V] = g->n_ich;
-    g->n_inhub = (uint32_t)inhub.size();
-    // Successor chunks: backward-sweep work items; every vertex has at least
-    // one (possibly empty) chunk because a leaf still publishes 1/sigma.
-    std::vector<uint32_t> och_first((size_t)V + 1), och_v, och_e0, och_e1, ohub_idx(V, ZG_NONE);
-    uint32_t max_k = 1, n_ohub = 0;
-    for (uint32_t v = 0; v < V; v++) {
-        och_first[v] = (uint32_t)och_v.size();
-        uint32_t b = out_ptr[v], e = out_ptr[(size_t)v + 1];
-        uint32_t k = 0;
-        if (b == e) {
-            och_v.push_back(v); och_e0.push_back(b); och_e1.push_back(e);
-            k = 1;
-        } else {
-            for (uint32_t c = b; c < e; c += ZG_CHUNK) {
-                och_v.push_back(v);
-                och_e0.push_back(c);
-                och_e1.push_back(std::min<uint32_t>(c + ZG_CHUNK, e));
-                k++;
-            }
-        }
-        if (k > 1) {
-            // a vertex with several chunks: one partial-sum slot per chunk
-            ohub_idx[v] = n_ohub;
-            n_ohub += k;
-        }
-        max_k = std::max(max_k, k);
-    }
-    och_first[V] = (uint32_t)och_v.size();
-    g->n_och = (uint32_t)och_v.size();
-    g->h_och_first = och_first;
-    g->h_hub_first.assign((size_t)V + 1, 0);
-    for (uint32_t v = 0; v < V; v++) g->h_hub_first[v + 1] = g->h_hub_first[v] + (ohub_idx[v] != ZG_NONE ? 1 : 0);
-    g->max_och_per_v = max_k;
-    g->n_ohub = n_ohub;
-
-    {
-        // stage the hottest PageRank values in LDS only if they serve >= 10%
-        // of the gathers (PageRank ids order vertices by decreasing out-degree)
-        const uint32_t K = std::min<uint32_t>(V, ZG_PR_XCACHE_BYTES / sizeof(float));
-        uint64_t hot = 0;
-        for (uint32_t i = 0; i < K; i++) hot += out_ptr[(size_t)pr_order[i] + 1] - out_ptr[pr_order[i]];
-        g->pr_xcache_useful = E > 0 && hot * 10 >= E;
-    }
-
-    // PageRank row bins over live rows by predecessor count.  Each bin lists
-    // its rows in PageRank-id order, so the rows a wavefront finishes read and
-    // write neighbouring rank words -- unless the PageRank ids already follow
-    // the vertex ids for most vertices (a grid: nearly all degrees equal),
-    // where vertex order keeps a wavefront's predecessors in a few lines and
-    // the rank words are neighbours either way (R-MAT 20: 2.19 TB/s in
-    // PageRank-id order against 1.98 in vertex order; the 2048 x 2048 grid:
-    // 4.5 TB/s in vertex order against 3.7).
-    uint64_t follow = 0;
-    for (uint32_t v = 1; v < V; v++) follow += pr_id[v] == pr_id[v - 1] + 1;
-    const bool by_pr = follow * 2 < (uint64_t)V;
-    std::vector<uint32_t> small, m4, m16, m64, large, large_deg, large_seg0, seg_row, seg_e0, seg_e1;
-    for (uint32_t i = 0; i < V; i++) {
-        const uint32_t v = by_pr ? pr_order[i] : i;
-        if (!g->h_live[v]) continue;
-        uint32_t d = in_ptr[(size_t)v + 1] - in_ptr[v];
-        if (d <= ZG_PR_SMALL) small.push_back(v);
-        else if (d <= ZG_PR_M4) m4.push_back(v);
-        else if (d <= ZG_PR_M16) m16.push_back(v);
-        else if (d <= ZG_PR_MED) m64.push_back(v);
-        else {
-            large_seg0.push_back((uint32_t)seg_row.size());
-            for (uint32_t c = in_ptr[v]; c < in_ptr[(size_t)v + 1]; c += ZG_PR_SEG) {
-                seg_row.push_back((uint32_t)large.size());
-                seg_e0.push_back(c);
-                seg_e1.push_back(std::min<uint32_t>(c + ZG_PR_SEG, in_ptr[(size_t)v + 1]));
-            }
-            large.push_back(pr_id[v]);
-            large_deg.push_back(out_ptr[(size_t)v + 1] - out_ptr[v]);
-        }
-    }
-    large_seg0.push_back((uint32_t)seg_row.size());
-    g->n_small = (uint32_t)small.size();
-    g->n_m4 = (uint32_t)m4.size();
-    g->n_m16 = (uint32_t)m16.size();
-    g->n_m64 = (uint32_t)m64.size();
-    std::vector<uint32_t> med(m4);
-    med.insert(med.end(), m16.begin(), m16.end());
-    med.insert(med.end(), m64.begin(), m64.end());
-    g->n_med = (uint32_t)med.size();
-    std::vector<uint4> desc;
-    desc.reserve(small.size() + med.size());
-    for (const std::vector<uint32_t> *bin : {&small, &med})
-        for (uint32_t v : *bin)
-            desc.push_back(make_uint4(pr_id[v], in_ptr[v], in_ptr[(size_t)v + 1] - in_ptr[v],
-                                      out_ptr[(size_t)v + 1] - out_ptr[v]));
-    g->n_large = (uint32_t)large.size();
-    g->n_seg = (uint32_t)seg_row.size();
-    // PageRank columns in row-bin order (small rows, then the medium bins,
-    // then the long rows): the rows a wavefront sums take neighbouring column
-    // words instead of words scattered over the predecessor CSR; each row
-    // keeps its arc order, so its sum is unchanged
-    if (E) {
-        std::vector<uint32_t> rb_src, rb_dst, rb_len;
-        rb_src.reserve(desc.size() + large.size());
-        rb_dst.reserve(desc.size() + large.size());
-        rb_len.reserve(desc.size() + large.size());
-        uint32_t pos = 0;
-        for (uint4 &d : desc) {
-            rb_src.push_back(d.y);
-            rb_dst.push_back(pos);
-            rb_len.push_back(d.z);
-            d.y = pos;
-            pos += d.z;
-        }
-        for (size_t j = 0; j < large.size(); j++) {
-            const uint32_t k0 = large_seg0[j], k1 = large_seg0[j + 1];
-            const uint32_t a0 = seg_e0[k0], len = seg_e1[k1 - 1] - a0;
-            rb_src.push_back(a0);
-            rb_dst.push_back(pos);
-            rb_len.push_back(len);
-            for (uint32_t k = k0; k < k1; k++) {
-                seg_e0[k] = seg_e0[k] - a0 + pos;
-                seg_e1[k] = seg_e1[k] - a0 + pos;
-            }
-            pos += len;
-        }
-        if ((uint64_t)pos != E) return ZG_ERR_INVALID_ARG;  // every arc has a live row
-        uint32_t *d_src = nullptr, *d_dst = nullptr, *d_len = nullptr, *binned = nullptr;
-        int rc = zg_upload(&d_src, rb_src);
-        if (rc == ZG_OK) rc = zg_upload(&d_dst, rb_dst);
-        if (rc == ZG_OK) rc = zg_upload(&d_len, rb_len);
-        if (rc == ZG_OK) rc = zg_dmalloc(&binned, E);
-        if (rc == ZG_OK) {
-            const uint32_t rows = (uint32_t)rb_src.size();
-            hipLaunchKernelGGL(zg_rebin_kernel, dim3(std::min<uint32_t>((rows + 3) / 4, 16384)), dim3(256), 0, g->stream,
-                               rows, d_src, d_dst, d_len, g->pr_col, binned);
-            rc = zg_hip_code(hipGetLastError());
-        }
-        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(g->stream));
-        zg_dfree(d_src);
-        zg_dfree(d_dst);
-        zg_dfree(d_len);
-        if (rc != ZG_OK) {
-            zg_dfree(binned);
-            return rc;
-        }
-        zg_dfree(g->pr_col);
-        g->pr_col = binned;
-    }
-
-    g->h_in_ptr = in_ptr;
-    ZG_TRY_RC(zg_upload(&g->ich_v, ich_v));
-    ZG_TRY_RC(zg_upload(&g->ich_e0, ich_e0));
-    ZG_TRY_RC(zg_upload(&g->ich_e1, ich_e1));
-    ZG_TRY_RC(zg_upload(&g->inhub, inhub));
-    ZG_TRY_RC(zg_upload(&g->och_first, och_first));
-    ZG_TRY_RC(zg_upload(&g->och_v, och_v));
-    ZG_TRY_RC(zg_upload(&g->och_e0, och_e0));
-    ZG_TRY_RC(zg_upload(&g->och_e1, och_e1));
-    ZG_TRY_RC(zg_upload(&g->ohub_idx, ohub_idx));
-    ZG_TRY_RC(zg_upload(&g->pr_small, small));
-    ZG_TRY_RC(zg_upload(&g->pr_med, med));
-    ZG_TRY_RC(zg_upload(&g->pr_desc, desc));
-    ZG_TRY_RC(zg_upload(&g->pr_large, large));
-    ZG_TRY_RC(zg_upload(&g->pr_large_deg, large_deg));
-    ZG_TRY_RC(zg_upload(&g->pr_large_seg0, large_seg0));
-    ZG_TRY_RC(zg_upload(&g->pr_seg_row, seg_row));
-    ZG_TRY_RC(zg_upload(&g->pr_seg_e0, seg_e0));
-    ZG_TRY_RC(zg_upload(&g->pr_seg_e1, seg_e1));
-    return ZG_OK;
+    // chunk lists, hub slots and PageRank row bins, on the device
+    return zg_derive_tables(g);
 }
 
 static int zg_create_common(uint32_t n_cap, const uint8_t *live_mask, const uint32_t *from, const uint32_t *to,
@@ -547,6 +334,49 @@ extern "C" int zg_graph_csr(zg_graph *g, uint32_t *out_ptr, uint32_t *out_col, u
     return g->built_on_gpu ? 1 : 0;
 }
 
+extern "C" int zg_graph_table(zg_graph *g, int which, uint32_t *out, uint64_t cap, uint64_t *n)
+{
+    if (!g) return ZG_ERR_INVALID_HANDLE;
+    if (!n) return ZG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
+    const uint64_t V = g->V, nsm = (uint64_t)g->n_small + g->n_med;
+    struct Part {
+        const void *src;
+        uint64_t words;
+    };
+    std::vector<Part> parts;
+    std::vector<uint32_t> bins;
+    switch (which) {
+    case ZG_TABLE_ICH: parts = {{g->ich_v, g->n_ich}, {g->ich_e0, g->n_ich}, {g->ich_e1, g->n_ich}}; break;
+    case ZG_TABLE_ICH_FIRST: parts = {{g->ich_first, V + 1}}; break;
+    case ZG_TABLE_OCH: parts = {{g->och_v, g->n_och}, {g->och_e0, g->n_och}, {g->och_e1, g->n_och}}; break;
+    case ZG_TABLE_OCH_FIRST: parts = {{g->och_first, V + 1}}; break;
+    case ZG_TABLE_OHUB: parts = {{g->ohub_idx, V}}; break;
+    case ZG_TABLE_PR_ROWS: parts = {{g->pr_small, g->n_small}, {g->pr_med, g->n_med}}; break;
+    case ZG_TABLE_PR_DESC: parts = {{g->pr_desc, 4 * nsm}}; break;
+    case ZG_TABLE_PR_LARGE:
+        parts = {{g->pr_large, g->n_large}, {g->pr_large_deg, g->n_large}, {g->pr_large_seg0, (uint64_t)g->n_large + 1}};
+        break;
+    case ZG_TABLE_PR_SEG: parts = {{g->pr_seg_row, g->n_seg}, {g->pr_seg_e0, g->n_seg}, {g->pr_seg_e1, g->n_seg}}; break;
+    case ZG_TABLE_PR_COL: parts = {{g->pr_col, g->E}}; break;
+    case ZG_TABLE_PR_BINS: bins = {g->n_small, g->n_m4, g->n_m16, g->n_m64, g->n_large}; break;
+    default: return ZG_ERR_INVALID_ARG;
+    }
+    uint64_t total = bins.size();
+    for (const Part &p : parts) total += p.words;
+    *n = total;
+    if (!out) return ZG_OK;
+    if (cap < total) return ZG_ERR_INVALID_ARG;
+    ZG_TRY(hipSetDevice(g->device));
+    for (size_t i = 0; i < bins.size(); i++) out[i] = bins[i];
+    uint64_t at = 0;
+    for (const Part &p : parts) {
+        if (p.words) ZG_TRY(hipMemcpy(out + at, p.src, sizeof(uint32_t) * p.words, hipMemcpyDeviceToHost));
+        at += p.words;
+    }
+    return ZG_OK;
+}
+
 extern "C" int zg_node_degrees(zg_graph *g, uint32_t *out)
 {
     if (!g) return ZG_ERR_INVALID_HANDLE;
@@ -554,7 +384,10 @@ extern "C" int zg_node_degrees(zg_graph *g, uint32_t *out)
     std::lock_guard<std::mutex> lk(g->mu);
     // arcs of tombstoned endpoints are not in the successor CSR, as removeNode
     // deletes a node's outgoing and incoming edges (array_graph.zig:128-153)
-    for (uint32_t v = 0; v < g->V; v++) out[v] = g->h_out_deg[v];
+    ZG_TRY(hipSetDevice(g->device));
+    std::vector<uint32_t> ptr((size_t)g->V + 1);
+    ZG_TRY(hipMemcpy(ptr.data(), g->out_ptr, sizeof(uint32_t) * ptr.size(), hipMemcpyDeviceToHost));
+    for (uint32_t v = 0; v < g->V; v++) out[v] = ptr[(size_t)v + 1] - ptr[v];
     return ZG_OK;
 }
 
@@ -600,6 +433,15 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
     case ZG_TUNE_COUNT_TREE_ARCS:
         if (value != 0.0 && value != 1.0) return ZG_ERR_INVALID_ARG;
         g->count_tree = value != 0.0;
+        return ZG_OK;
+    case ZG_TUNE_QUEUE_ORDER:
+        if (value != 0.0 && value != 1.0) return ZG_ERR_INVALID_ARG;
+        if ((value != 0.0) != g->queue_order) {
+            ZG_TRY(hipSetDevice(g->device));
+            ZG_TRY(hipStreamSynchronize(g->stream));
+            zg_bcq_order_release(g);  // rebuilt, in the other numbering, by the next sweep
+        }
+        g->queue_order = value != 0.0;
         return ZG_OK;
     case ZG_TUNE_PULL_GRID:
         if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;

@@ -11,9 +11,9 @@
 //                  stable radix sort by head of the successor CSR's arcs.
 //   PageRank ids   vertices by decreasing out-degree, ties by id: a stable
 //                  sort by the complemented degree.
-// The host keeps the row offsets and the PageRank order (O(V)) for the
-// per-vertex tables built there.
-#include "common.h"
+// The work tables derived from these (graph_derive.hip) are built on the
+// device too; the host reads back only the arc count and error flags.
+#include "build_util.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -97,35 +97,21 @@ unsigned bits_for(uint64_t x)  // bits that hold every value <= x
     return b;
 }
 
-// Device scratch freed on every exit path.
-struct Scratch {
-    std::vector<void *> p;
-    template <typename T>
-    int get(T **out, size_t n)
-    {
-        int rc = zg_dmalloc(out, n);
-        if (rc == ZG_OK) p.push_back(*out);
-        return rc;
-    }
-    ~Scratch()
-    {
-        for (void *q : p) (void)hipFree(q);
-    }
-};
+}  // namespace
 
-int scan_offsets(const uint32_t *deg, uint32_t *ptr, uint32_t V, hipStream_t s, Scratch &sc)
+int zg_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s, ZgScratch &sc)
 {
-    // ptr[0..V] = exclusive scan of deg[0..V] (deg[V] = 0)
+    if (n == 0) return ZG_OK;
     size_t bytes = 0;
-    ZG_TRY(rocprim::exclusive_scan(nullptr, bytes, deg, ptr, 0u, (size_t)V + 1, rocprim::plus<uint32_t>(), s));
+    ZG_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s));
     void *tmp = nullptr;
     ZG_TRY_RC(sc.get((char **)&tmp, bytes ? bytes : 1));
-    ZG_TRY(rocprim::exclusive_scan(tmp, bytes, deg, ptr, 0u, (size_t)V + 1, rocprim::plus<uint32_t>(), s));
+    ZG_TRY(rocprim::exclusive_scan(tmp, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s));
     return ZG_OK;
 }
 
-int sort_pairs(const uint32_t *kin, uint32_t *kout, const uint32_t *vin, uint32_t *vout, uint64_t n, unsigned bits,
-               hipStream_t s, Scratch &sc)
+int zg_sort_pairs(const uint32_t *kin, uint32_t *kout, const uint32_t *vin, uint32_t *vout, uint64_t n, unsigned bits,
+                  hipStream_t s, ZgScratch &sc)
 {
     if (n == 0) return ZG_OK;
     size_t bytes = 0;
@@ -136,17 +122,13 @@ int sort_pairs(const uint32_t *kin, uint32_t *kout, const uint32_t *vin, uint32_
     return ZG_OK;
 }
 
-}  // namespace
-
 // Fills the device arrays out_ptr, out_col, (out_w), in_ptr, in_col, pr_id,
-// pr_col, live and g->E, g->wmean, g->weights_ok; returns the row offsets and
-// the PageRank order on the host.
-int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
-                     std::vector<uint32_t> &out_ptr, std::vector<uint32_t> &in_ptr, std::vector<uint32_t> &pr_order)
+// pr_col, live and g->E, g->wmean, g->weights_ok.
+int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m)
 {
     const uint32_t V = g->V;
     hipStream_t s = g->stream;
-    Scratch sc;
+    ZgScratch sc;
     const dim3 blk(256), grd(4096);
     uint32_t *d_from, *d_to, *d_bad, *key, *key2, *val, *val2, *deg;
     double *d_w = nullptr;
@@ -179,13 +161,12 @@ int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, cons
     ZG_TRY(hipMemsetAsync(deg, 0, sizeof(uint32_t) * ((size_t)V + 1), s));
     hipLaunchKernelGGL(gb_keys_kernel, grd, blk, 0, s, m, V, d_from, d_to, g->live, key, val, deg);
     ZG_TRY(hipGetLastError());
-    ZG_TRY_RC(sort_pairs(key, key2, val, val2, m, bits_for(V), s, sc));
+    ZG_TRY_RC(zg_sort_pairs(key, key2, val, val2, m, bits_for(V), s, sc));
     ZG_TRY_RC(zg_dmalloc(&g->out_ptr, (size_t)V + 1));
-    ZG_TRY_RC(scan_offsets(deg, g->out_ptr, V, s, sc));
-    out_ptr.resize((size_t)V + 1);
-    ZG_TRY(hipMemcpyAsync(out_ptr.data(), g->out_ptr, sizeof(uint32_t) * ((size_t)V + 1), hipMemcpyDeviceToHost, s));
+    ZG_TRY_RC(zg_exclusive_scan(deg, g->out_ptr, (size_t)V + 1, s, sc));
+    uint32_t E = 0;
+    ZG_TRY(hipMemcpyAsync(&E, g->out_ptr + V, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     ZG_TRY(hipStreamSynchronize(s));
-    const uint32_t E = out_ptr[V];
     g->E = E;
     ZG_TRY_RC(zg_dmalloc(&g->out_col, E));
     if (weight) ZG_TRY_RC(zg_dmalloc(&g->out_w, E));
@@ -195,21 +176,17 @@ int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, cons
 
     // predecessor CSR: heads of the successor arcs, stable, sources as values
     ZG_TRY_RC(zg_dmalloc(&g->in_ptr, (size_t)V + 1));
-    ZG_TRY_RC(scan_offsets(deg, g->in_ptr, V, s, sc));
+    ZG_TRY_RC(zg_exclusive_scan(deg, g->in_ptr, (size_t)V + 1, s, sc));
     ZG_TRY_RC(zg_dmalloc(&g->in_col, E));
-    ZG_TRY_RC(sort_pairs(g->out_col, key, key2, g->in_col, E, bits_for(V ? V - 1 : 0), s, sc));
-    in_ptr.resize((size_t)V + 1);
-    ZG_TRY(hipMemcpyAsync(in_ptr.data(), g->in_ptr, sizeof(uint32_t) * ((size_t)V + 1), hipMemcpyDeviceToHost, s));
+    ZG_TRY_RC(zg_sort_pairs(g->out_col, key, key2, g->in_col, E, bits_for(V ? V - 1 : 0), s, sc));
 
     // PageRank ids
     ZG_TRY_RC(zg_dmalloc(&g->pr_id, V));
     ZG_TRY_RC(zg_dmalloc(&g->pr_col, E));
     if (V) {
         hipLaunchKernelGGL(gb_degree_keys_kernel, grd, blk, 0, s, V, g->out_ptr, key, val);
-        ZG_TRY_RC(sort_pairs(key, key2, val, val2, V, 32, s, sc));
+        ZG_TRY_RC(zg_sort_pairs(key, key2, val, val2, V, 32, s, sc));
         hipLaunchKernelGGL(gb_scatter_id_kernel, grd, blk, 0, s, V, val2, g->pr_id);
-        pr_order.resize(V);
-        ZG_TRY(hipMemcpyAsync(pr_order.data(), val2, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, s));
     }
     if (E) hipLaunchKernelGGL(gb_relabel_kernel, grd, blk, 0, s, E, g->in_col, g->pr_id, g->pr_col);
     ZG_TRY(hipGetLastError());
