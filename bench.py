@@ -722,7 +722,8 @@ def bench_graph(args, name, steps, warmup):
     floor_us = ctypes.c_double()
     _lib.check(L.zg_gather_floor(h, 32, 20, ctypes.byref(floor_us)), "zg_gather_floor")
     pr_roof = roofline_entry(pr_bytes_iter * it_n, it_ms, it_n, "pr_iter_kernel+pr_finish_kernel",
-                             traffic_of(name, "pr_iter_kernel<float, 0>") if prs["precision"] == 32 else None)
+                             traffic_of(name, "pr_iter_kernel<float, 0>" if prs["precision"] == 32
+                                        else "pr_iter_kernel<float, 0>+pr_iter_kernel<double, 0>"))
     it_us = pr_roof["avg_launch_us"]
     pr_roof["gather_floor_us"] = round(floor_us.value, 2)
     pr_roof["frac_of_gather_floor"] = round(floor_us.value / it_us, 3) if it_us > 0 else None

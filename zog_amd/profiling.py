@@ -4,6 +4,11 @@
       markdown table of the kernel statistics (per kernel: calls, total ms,
       average / min / max microseconds, share of device time)
 
+  python -m zog_amd.profiling reduce <run_counter_collection.csv> <out.csv>
+      the counter values summed per (kernel, counter), with the number of
+      dispatches: a run of a few hundred thousand dispatches shrinks to a few
+      rows, and `traffic` reads either form
+
   python -m zog_amd.profiling traffic --fetch F.csv --write W.csv \
       --copy-fetch CF.csv --copy-write CW.csv --copy-bytes 1073741824 -o out.json
       HBM bytes per dispatch of every kernel from one FETCH_SIZE and one
@@ -52,10 +57,25 @@ def _counter_per_kernel(path: str, counter: str):
             continue
         v = float(r["Counter_Value"])
         k = _short(r["Kernel_Name"])
+        n = int(r.get("Dispatches") or 1)  # reduced files: sums over n dispatches
         tot[k] += v
-        cnt[k] += 1
-        biggest = max(biggest, v)
+        cnt[k] += n
+        biggest = max(biggest, v / n)
     return tot, cnt, biggest
+
+
+def reduce_counters(path: str, out: str) -> None:
+    tot = defaultdict(float)
+    cnt = defaultdict(int)
+    for r in csv.DictReader(open(path)):
+        key = (r["Kernel_Name"], r["Counter_Name"])
+        tot[key] += float(r["Counter_Value"])
+        cnt[key] += int(r.get("Dispatches") or 1)
+    with open(out, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(["Kernel_Name", "Counter_Name", "Counter_Value", "Dispatches"])
+        for (k, c), v in sorted(tot.items()):
+            w.writerow([k, c, repr(v), cnt[(k, c)]])
 
 
 def traffic(fetch, write, copy_fetch, copy_write, copy_bytes):
@@ -101,6 +121,9 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("csv")
+    r = sub.add_parser("reduce")
+    r.add_argument("csv")
+    r.add_argument("out")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -111,6 +134,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(kernel_stats_markdown(a.csv))
+    elif a.cmd == "reduce":
+        reduce_counters(a.csv, a.out)
     else:
         res = traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes)
         with open(a.out, "w") as fh:
