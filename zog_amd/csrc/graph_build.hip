@@ -110,6 +110,26 @@ int zg_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s
     return ZG_OK;
 }
 
+size_t zg_scan_bytes(size_t n)
+{
+    size_t bytes = 0;
+    if (n == 0) return 0;
+    if (rocprim::exclusive_scan(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, 0u, n,
+                                rocprim::plus<uint32_t>(), (hipStream_t)0) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+int zg_exclusive_scan_tmp(const uint32_t *in, uint32_t *out, size_t n, void *tmp, size_t cap, hipStream_t s)
+{
+    if (n == 0) return ZG_OK;
+    size_t bytes = 0;
+    ZG_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s));
+    if (bytes > cap) return ZG_ERR_INVALID_ARG;
+    ZG_TRY(rocprim::exclusive_scan(tmp, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s));
+    return ZG_OK;
+}
+
 int zg_sort_pairs(const uint32_t *kin, uint32_t *kout, const uint32_t *vin, uint32_t *vout, uint64_t n, unsigned bits,
                   hipStream_t s, ZgScratch &sc)
 {
