@@ -91,8 +91,17 @@ def traffic_of(workload, kernel):
         return None
     with open(tfile) as fh:
         tj = json.load(fh)
-    # "a+b": the kernels' bytes over their dispatches together
-    parts = [tj.get(k) for k in kernel.split("+")]
+    # "a+b": the kernels' bytes over their dispatches together; a name
+    # without its block-shape argument ("pr_iter_kernel<float, 0>") stands
+    # for the shape that ran most often
+    def entry(k):
+        if k in tj:
+            return tj[k]
+        stem = k[:-1] + ", " if k.endswith(">") else None
+        cands = [v for kk, v in tj.items() if stem and kk.startswith(stem) and isinstance(v, dict)]
+        return max(cands, key=lambda v: v["dispatches"]) if cands else None
+
+    parts = [entry(k) for k in kernel.split("+")]
     if not all(parts):
         return None
     n = sum(p["dispatches"] for p in parts)
@@ -449,6 +458,7 @@ def bench_power(args, steps):
     torch.cuda.set_device(0)
     n, src, dst = W.rmat20()
     g = rg.new(SoA.from_arrays("directed", n, src, dst))
+    apply_tuning(g, args)
     V, n_live, E = g.resource.info()
     iters = 100
     # alpha below 1 / spectral radius keeps Katz and alpha finite: the
@@ -486,10 +496,16 @@ def bench_power(args, steps):
                      "ms": round(ms, 2), "bytes_per_iteration": bytes_iter[name], "roofline": rl}
     floor_us = ctypes.c_double()
     _lib.check(_lib.lib().zg_gather_floor(g.resource.handle, 64, 20, ctypes.byref(floor_us)), "zg_gather_floor")
+    floor_lds_us = ctypes.c_double()
+    _lib.check(_lib.lib().zg_gather_floor(g.resource.handle, 64 | _lib.ZG_FLOOR_LDS_COPY, 20,
+                                          ctypes.byref(floor_lds_us)), "zg_gather_floor")
     for name in res:
         it_us = res[name]["roofline"]["avg_launch_us"]
         res[name]["roofline"]["gather_floor_us"] = round(floor_us.value, 2)
         res[name]["roofline"]["frac_of_gather_floor"] = round(floor_us.value / it_us, 3) if it_us > 0 else None
+        res[name]["roofline"]["gather_floor_lds_us"] = round(floor_lds_us.value, 2)
+        res[name]["roofline"]["frac_of_gather_floor_lds"] = (round(floor_lds_us.value / it_us, 3)
+                                                             if it_us > 0 else None)
     cpu = None
     parity = {}
     if not args.no_cpu:
@@ -554,6 +570,9 @@ def apply_tuning(g, args):
                    "zg_set_tuning")
     if args.pr_grid is not None:
         _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_PR_GRID, float(args.pr_grid)),
+                   "zg_set_tuning")
+    if args.pr_block is not None:
+        _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_PR_BLOCK, float(args.pr_block)),
                    "zg_set_tuning")
     if args.pull_grid is not None:
         rg.set_pull_grid(g, args.pull_grid)
@@ -721,12 +740,16 @@ def bench_graph(args, name, steps, warmup):
     pr_result = d_pr.cpu().numpy()
     floor_us = ctypes.c_double()
     _lib.check(L.zg_gather_floor(h, 32, 20, ctypes.byref(floor_us)), "zg_gather_floor")
+    floor_lds_us = ctypes.c_double()
+    _lib.check(L.zg_gather_floor(h, 32 | _lib.ZG_FLOOR_LDS_COPY, 20, ctypes.byref(floor_lds_us)), "zg_gather_floor")
     pr_roof = roofline_entry(pr_bytes_iter * it_n, it_ms, it_n, "pr_iter_kernel+pr_finish_kernel",
                              traffic_of(name, "pr_iter_kernel<float, 0>" if prs["precision"] == 32
                                         else "pr_iter_kernel<float, 0>+pr_iter_kernel<double, 0>"))
     it_us = pr_roof["avg_launch_us"]
     pr_roof["gather_floor_us"] = round(floor_us.value, 2)
     pr_roof["frac_of_gather_floor"] = round(floor_us.value / it_us, 3) if it_us > 0 else None
+    pr_roof["gather_floor_lds_us"] = round(floor_lds_us.value, 2)
+    pr_roof["frac_of_gather_floor_lds"] = round(floor_lds_us.value / it_us, 3) if it_us > 0 else None
     log(f"[bench] {name} pagerank {pr_iters} iterations ({int(pinfo.fp64_iterations)} FP64), {pr_gbs:.1f} GB/s wall, "
         f"kernel {pr_roof['achieved']:.1f} GB/s, residual {pinfo.residual_l1:.3e}")
 
@@ -941,8 +964,9 @@ def compact(line):
         p = line["pagerank"]
         out["pagerank"] = {"value": p["value"], "unit": p["unit"], "iterations": p["iterations"],
                            "fp64_iterations": p["fp64_iterations"],
-                           "roofline": {k: p["roofline"][k] for k in ("achieved", "frac", "traffic", "avg_launch_us",
-                                                                      "frac_of_gather_floor")}}
+                           "roofline": {k: p["roofline"].get(k) for k in ("achieved", "frac", "traffic", "avg_launch_us",
+                                                                          "frac_of_gather_floor",
+                                                                          "frac_of_gather_floor_lds")}}
         if cb and "pagerank_gbs" in cb:
             out["pagerank"]["cpu_baseline_gbs"] = cb["pagerank_gbs"]
     return out
@@ -977,6 +1001,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch every level kernel from the host")
     ap.add_argument("--pr-cache", type=int, default=None, help="PageRank values staged in LDS (0 = none)")
     ap.add_argument("--pr-grid", type=int, default=None, help="PageRank iteration blocks")
+    ap.add_argument("--pr-block", type=int, default=None, choices=[0, 256, 512, 1024],
+                    help="PageRank iteration block threads (0 = default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--wbatch", type=int, default=None, help="rmat20w: sources per batch (multiple of 64)")
     ap.add_argument("--wdelta", type=float, default=None, help="rmat20w: near-frontier bound step (0 = default)")

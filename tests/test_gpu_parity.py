@@ -542,3 +542,34 @@ def test_batched_sweep_work_counters():
     rg.betweenness_unweighted_array(g, sources=sources, info=info)
     assert info["tree_arcs"] == 0 and info["row_visits"] == visits
     rg.destroy(g)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_pagerank_block_shapes_and_lds_copy(precision):
+    # every iteration-block shape, with and without the LDS copy of the
+    # most-gathered values, meets the oracle, repeats bit for bit, and
+    # agrees with the others (only the per-block FP64 order of the dangling
+    # mass differs); rows longer than one segment are included
+    from oracle import oracle as O
+    from zog_amd import _lib
+    n, src, dst = gen.rmat(15, 1 << 19, gen.SEED_BASE + 21)
+    rng = np.random.default_rng(6)
+    src = np.concatenate([src, rng.integers(0, n, 9000)]).astype(np.uint32)
+    dst = np.concatenate([dst, np.full(9000, 11)]).astype(np.uint32)
+    ref, _, _ = O.Graph(n, src, dst).pagerank(0.85, 40, 0.0)
+    g = make(n, src, dst)
+    h = g.resource.handle
+    outs = []
+    for block, cache in ((256, None), (512, None), (1024, None), (1024, 0), (1024, 1000), (256, 0)):
+        say("pagerank shape", block, cache, precision)
+        _lib.check(_lib.lib().zg_set_tuning(h, _lib.ZG_TUNE_PR_BLOCK, float(block)), "zg_set_tuning")
+        _lib.check(_lib.lib().zg_set_tuning(h, _lib.ZG_TUNE_PR_CACHE, float(1 << 30 if cache is None else cache)),
+                   "zg_set_tuning")
+        a = rg.pagerank_array(g, 0.85, 40, 0.0, precision=precision)
+        b = rg.pagerank_array(g, 0.85, 40, 0.0, precision=precision)
+        assert np.array_equal(a, b)
+        assert np.abs(a - ref).sum() <= (PR_L1 if precision == 32 else 1e-12)
+        outs.append(a)
+    for a in outs[1:]:
+        assert np.abs(a - outs[0]).sum() <= (1e-6 if precision == 32 else 1e-13)
+    rg.destroy(g)

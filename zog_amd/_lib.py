@@ -37,6 +37,8 @@ ZG_TUNE_PULL_GRID = 3
 ZG_TUNE_GRAPH = 4
 ZG_TUNE_PR_CACHE = 5
 ZG_TUNE_PR_GRID = 6
+ZG_TUNE_PR_BLOCK = 12
+ZG_FLOOR_LDS_COPY = 0x100
 ZG_TUNE_WEIGHTED_BATCH = 7
 ZG_TUNE_WEIGHTED_DELTA = 8
 ZG_TUNE_WEIGHTED_GRID = 9

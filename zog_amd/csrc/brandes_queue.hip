@@ -172,6 +172,12 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
             J.load(a.adj, v);
             inl = J.inl();
         }
+        // depths of the inline predecessors, kept so that a successor that is
+        // also a predecessor (every arc of a symmetric graph) needs no second
+        // record load: depths only go from unset to set during the sweep, so
+        // a value read here is as good as one read later (the claim is a CAS)
+        uint32_t unset = 0u;  // bit k: inline predecessor k was unreached
+        uint32_t npred = 0u;  // inline predecessors whose depth was read
         if (d > 0) {
             // path count of v from its predecessors at depth d-1
             const uint32_t i0 = valid && !inl ? a.in_ptr[v] : 0;
@@ -201,8 +207,12 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
 #pragma unroll
                     for (int t = 0; t < 4; t++) q[t] = (uint32_t)(4 * b + t) < ideg ? rec[u[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
 #pragma unroll
-                    for (int t = 0; t < 4; t++) fold(q[t]);
+                    for (int t = 0; t < 4; t++) {
+                        fold(q[t]);
+                        if (q[t].depth == ZG_NONE) unset |= 1u << (4 * b + t);
+                    }
                 }
+                npred = ideg;
             } else if (ideg <= ZG_Q_WIDE) {
                 // four arcs at a time: their loads overlap instead of chaining
                 for (uint32_t k0 = 0; k0 < ideg; k0 += 4) {
@@ -259,7 +269,17 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
             for (int t = 0; t < 4; t++)
                 w[t] = k0 + t < deg ? (inl ? J.succ((int)(k0 >> 2), t) : a.out_col[e0 + k0 + t]) : 0u;
 #pragma unroll
-            for (int t = 0; t < 4; t++) dw[t] = k0 + t < deg ? rec[w[t]].depth : 0u;
+            for (int t = 0; t < 4; t++) {
+                bool known = false;
+                dw[t] = 0u;
+#pragma unroll
+                for (int k = 0; k < ZG_Q_INLINE; k++)
+                    if ((uint32_t)k < npred && J.w[1 + k] == w[t]) {
+                        dw[t] = (unset >> k) & 1u ? ZG_NONE : 0u;
+                        known = true;
+                    }
+                if (k0 + t < deg && !known) dw[t] = rec[w[t]].depth;
+            }
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 bool won = false;

@@ -290,6 +290,9 @@ struct zg_graph {
     uint32_t pr_xcache = 1u << 30;       // PageRank values staged in LDS (capped by the LDS size; 0 = none)
     bool pr_xcache_useful = false;       // the cached values carry a useful share of the gathers
     uint32_t pr_grid_cap = 0;            // PageRank iteration blocks (0 = default)
+    uint32_t pr_block = 0;               // PageRank iteration block threads: 256, 512, 1024 (0 = default)
+    uint32_t dev_cus = 0;                // compute units of the device (lazy)
+    uint32_t dev_lds = 0;                // LDS bytes one block may use (lazy)
     bool high_diameter = false;          // set with h_coredist
     // device staging of the host-buffer entry points (sources in, scores
     // out), kept between calls so a call allocates nothing once warm

@@ -447,6 +447,10 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
         if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;
         g->pull_grid = (uint32_t)value;
         return ZG_OK;
+    case ZG_TUNE_PR_BLOCK:
+        if (value != 0.0 && value != 256.0 && value != 512.0 && value != 1024.0) return ZG_ERR_INVALID_ARG;
+        g->pr_block = (uint32_t)value;
+        return ZG_OK;
     default: return ZG_ERR_INVALID_ARG;
     }
 }

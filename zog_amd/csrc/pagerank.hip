@@ -58,7 +58,7 @@ struct PrArgs {
     T *r0, *r1, *x0, *x1;       // parity 0 / 1 buffers
     T *rb0, *rb1, *rb2;         // ZG_PW_RAW: the rotating (older, current, new) iterates
     double *part;               // [gridDim.x][2]
-    T *seg;                     // [n_seg][2] Kahan pairs of long-row segments
+    T *seg;                     // [n_seg][ZG_PR_SEG_PARTS][2] Kahan pairs of long-row segments
     PrState *st;
 };
 
@@ -233,51 +233,55 @@ __device__ __forceinline__ void pr_unit(const PrArgs<T> &a, uint32_t b, int tid,
     }
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
+// One iteration.  A block of NT threads is NT/256 independent workers of
+// 256 threads (quarters); each takes work units with a grid stride, and all
+// share the block's LDS copy of the a.xcache most-gathered values (the
+// lowest PageRank ids), loaded once per launch: with one 1024-thread block
+// per CU the copy holds up to ~40K FP32 values.  A long-row segment is
+// summed by one quarter; its four wavefronts write their Kahan pairs, which
+// pr_finish_kernel merges in wavefront order.
+template <typename T, int MODE, int NT>
+__global__ __launch_bounds__(NT) void pr_iter_kernel(PrArgs<T> a)
 {
-    __shared__ double red[2][ZG_PR_BLOCK / 64];
-    __shared__ T kred[2][ZG_PR_BLOCK / 64];
+    static_assert(NT % ZG_PR_BLOCK == 0, "a block is a whole number of 256-thread workers");
+    constexpr uint32_t NQ = NT / ZG_PR_BLOCK;
+    extern __shared__ double pr_lds[];  // [a.xcache] values of type T (launch-sized)
+    __shared__ double red[2][NT / 64];
     const PrState *st = a.st;
     if (st->done) return;
     const unsigned long long it = st->iter;
     const PrBufs<T, MODE> B(a, it);
     const T *xg = B.xg;
-    // the ZG_PR_XCACHE_BYTES most-gathered values (lowest PageRank ids) in LDS
-    __shared__ T xs[ZG_PR_XCACHE_BYTES / sizeof(T)];
-    const uint32_t K = min(a.xcache, (uint32_t)(ZG_PR_XCACHE_BYTES / sizeof(T)));
-    for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) xs[i] = xg[i];
+    T *xs = reinterpret_cast<T *>(pr_lds);
+    const uint32_t K = a.xcache;
+    for (uint32_t i = threadIdx.x; i < K; i += NT) xs[i] = xg[i];
     __syncthreads();
     const PrGather<T> x{xg, xs, K};
     const T *rold = B.rold;
     T *rnew = B.rnew;
     T *xnew = B.xnew;
     const double base = st->base, damping = st->damping;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint32_t units = a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64 + a.n_seg;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tid = threadIdx.x & (ZG_PR_BLOCK - 1);
+    const uint32_t q = threadIdx.x / ZG_PR_BLOCK;
+    const uint32_t rowunits = a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64;
+    const uint32_t units = rowunits + a.n_seg;
     double l1 = 0.0, dang = 0.0;
 
-    for (uint32_t b = blockIdx.x; b < units; b += gridDim.x) {
-        if (b < a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64) {
+    for (uint32_t b = blockIdx.x * NQ + q; b < units; b += gridDim.x * NQ) {
+        if (b < rowunits) {
             pr_unit<T, MODE>(a, b, tid, x, rold, rnew, xnew, base, damping, l1, dang);
         } else {
-            // one block per segment of a long row; pr_finish_kernel finishes the row
-            const uint32_t sg = b - a.nb_small - a.nb_m4 - a.nb_m16 - a.nb_m64;
+            // one worker per segment of a long row; pr_finish_kernel finishes the row
+            const uint32_t sg = b - rowunits;
             T s = 0, c = 0;
             pr_strided_kahan(a.in_col, x, a.seg_e0[sg] + tid, a.seg_e1[sg], (uint32_t)ZG_PR_BLOCK, s, c);
             wave_kahan_merge(s, c);
             if (lane == 0) {
-                kred[0][wv] = s;
-                kred[1][wv] = c;
+                const size_t o = 2 * ((size_t)ZG_PR_SEG_PARTS * sg + (tid >> 6));
+                a.seg[o] = s;
+                a.seg[o + 1] = c;
             }
-            __syncthreads();
-            if (tid == 0) {
-                T S = kred[0][0], C = kred[1][0];
-                for (int w = 1; w < ZG_PR_BLOCK / 64; w++) kahan_merge(S, C, kred[0][w], kred[1][w]);
-                a.seg[2 * sg] = S;
-                a.seg[2 * sg + 1] = C;
-            }
-            __syncthreads();
         }
     }
 
@@ -289,9 +293,9 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
         red[1][wv] = dang;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
         double L = 0.0, D = 0.0;
-        for (int w = 0; w < ZG_PR_BLOCK / 64; w++) {
+        for (int w = 0; w < NT / 64; w++) {
             L += red[0][w];
             D += red[1][w];
         }
@@ -301,6 +305,17 @@ __global__ __launch_bounds__(ZG_PR_BLOCK) void pr_iter_kernel(PrArgs<T> a)
 }
 
 #define ZG_PR_FINISH_THREADS 1024
+
+// Kahan pair of one long-row segment: its wavefronts' pairs merged in order.
+template <typename T>
+__device__ __forceinline__ void pr_seg_sum(const T *seg, uint32_t q, T &S, T &C)
+{
+    const T *p = seg + 2 * (size_t)ZG_PR_SEG_PARTS * q;
+    S = p[0];
+    C = p[1];
+#pragma unroll
+    for (int w = 1; w < ZG_PR_SEG_PARTS; w++) kahan_merge(S, C, p[2 * w], p[2 * w + 1]);
+}
 
 // One block: finish the long rows from their segment partials, add all block
 // partials in a fixed order, advance the iteration state.
@@ -320,8 +335,13 @@ __global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<
     double l1 = 0.0, dang = 0.0;
     for (uint32_t i = tid; i < a.n_large; i += ZG_PR_FINISH_THREADS) {
         const uint32_t s0 = a.large_seg0[i], s1 = a.large_seg0[i + 1];
-        T S = a.seg[2 * s0], C = a.seg[2 * s0 + 1];
-        for (uint32_t q = s0 + 1; q < s1; q++) kahan_merge(S, C, a.seg[2 * q], a.seg[2 * q + 1]);
+        T S, C;
+        pr_seg_sum(a.seg, s0, S, C);
+        for (uint32_t q = s0 + 1; q < s1; q++) {
+            T S2, C2;
+            pr_seg_sum(a.seg, q, S2, C2);
+            kahan_merge(S, C, S2, C2);
+        }
         const uint32_t v = a.large[i];
         pr_row_finish<T, MODE>(rnew, xnew, v, MODE != ZG_PW_RAW ? rold[v] : (T)0, a.large_deg[i], S, C, base,
                                damping, l1, dang);
@@ -427,11 +447,83 @@ static int pr_ws_ensure(zg_graph *g)
     ZG_TRY_RC(zg_dmalloc((char **)&w.r, V * (4 * 8 + 4 * 4)));
     w.xh = nullptr;
     ZG_TRY_RC(zg_dmalloc(&w.part, 2 * (size_t)pr_grid(g)));
-    ZG_TRY_RC(zg_dmalloc(&w.seg32, g->n_seg));
-    ZG_TRY_RC(zg_dmalloc(&w.seg64, 2 * (size_t)g->n_seg));
+    ZG_TRY_RC(zg_dmalloc(&w.seg32, ZG_PR_SEG_PARTS * (size_t)g->n_seg));
+    ZG_TRY_RC(zg_dmalloc(&w.seg64, 2 * ZG_PR_SEG_PARTS * (size_t)g->n_seg));
     ZG_TRY_RC(zg_dmalloc((char **)&w.state, 2 * sizeof(PrState)));
     ZG_TRY(hipMemset(w.state, 0, 2 * sizeof(PrState)));
     ZG_TRY(hipHostMalloc(&w.h_state, 2 * sizeof(PrState), hipHostMallocDefault));
+    return ZG_OK;
+}
+
+// Launch shape of the iteration kernel.  Where the most-gathered values
+// carry a useful share of the gathers, one 1024-thread block per CU keeps
+// as many of them in LDS as one block may hold (~40K FP32 values on gfx950:
+// 70% of R-MAT 20's gathers, against 35% for the 4K values of a 16 KB copy
+// per 256-thread block); otherwise 256-thread blocks with no copy.
+struct PrShape {
+    uint32_t grid = 1, threads = ZG_PR_BLOCK, K = 0;
+    size_t lds = 0;  // dynamic LDS bytes
+};
+
+static void pr_device_limits(zg_graph *g)
+{
+    if (g->dev_cus) return;
+    int cus = 0, lds = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device) != hipSuccess ||
+        lds < 65536)
+        lds = 65536;
+    g->dev_cus = (uint32_t)cus;
+    g->dev_lds = (uint32_t)lds;
+}
+
+template <typename T>
+static PrShape pr_shape(zg_graph *g)
+{
+    pr_device_limits(g);
+    PrShape sh;
+    const uint32_t units = pr_grid(g);
+    const uint32_t threads = g->pr_block ? g->pr_block : (g->pr_xcache_useful ? 1024u : (uint32_t)ZG_PR_BLOCK);
+    sh.threads = threads;
+    const uint32_t want = g->pr_xcache_useful ? std::min<uint32_t>(g->pr_xcache, g->V) : 0u;
+    if (threads == ZG_PR_BLOCK) {
+        sh.K = std::min<uint32_t>(want, ZG_PR_XCACHE_BYTES / sizeof(T));
+        sh.grid = units;
+    } else {
+        // leave 1 KB of the block's LDS for the kernel's own reductions
+        sh.K = std::min<uint32_t>(want, (g->dev_lds - 1024u) / (uint32_t)sizeof(T));
+        const uint32_t lds = sh.K * (uint32_t)sizeof(T);
+        const uint32_t by_lds = lds ? std::max<uint32_t>(1u, g->dev_lds / lds) : 8u;
+        const uint32_t by_threads = std::max<uint32_t>(1u, 2048u / threads);
+        const uint32_t per_cu = std::min(by_lds, by_threads);
+        const uint32_t quarters = (units + threads / ZG_PR_BLOCK - 1) / (threads / ZG_PR_BLOCK);
+        sh.grid = std::max<uint32_t>(1u, std::min<uint32_t>(g->dev_cus * per_cu, quarters));
+        if (g->pr_grid_cap) sh.grid = std::min(sh.grid, g->pr_grid_cap);
+    }
+    sh.lds = (size_t)sh.K * sizeof(T);
+    return sh;
+}
+
+template <typename T, int MODE>
+static void pr_launch_iter(const PrShape &sh, hipStream_t s, const PrArgs<T> &a)
+{
+    if (sh.threads == 1024)
+        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, 1024>), dim3(sh.grid), dim3(1024), sh.lds, s, a);
+    else if (sh.threads == 512)
+        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, 512>), dim3(sh.grid), dim3(512), sh.lds, s, a);
+    else
+        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, ZG_PR_BLOCK>), dim3(sh.grid), dim3(ZG_PR_BLOCK), sh.lds, s, a);
+}
+
+// The dynamic LDS above the default 64 KB has to be allowed per kernel.
+template <typename T, int MODE>
+static int pr_allow_lds(const PrShape &sh)
+{
+    if (sh.lds <= 65536) return ZG_OK;
+    const void *f = sh.threads == 1024 ? reinterpret_cast<const void *>(&pr_iter_kernel<T, MODE, 1024>)
+                                       : reinterpret_cast<const void *>(&pr_iter_kernel<T, MODE, 512>);
+    ZG_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
     return ZG_OK;
 }
 
@@ -460,7 +552,7 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.nb_m4 = (g->n_m4 + ZG_PR_BLOCK / 4 - 1) / (ZG_PR_BLOCK / 4);
     a.nb_m16 = (g->n_m16 + ZG_PR_BLOCK / 16 - 1) / (ZG_PR_BLOCK / 16);
     a.nb_m64 = (g->n_m64 + ZG_PR_BLOCK / 64 - 1) / (ZG_PR_BLOCK / 64);
-    a.xcache = g->pr_xcache_useful ? std::min<uint32_t>(g->pr_xcache, g->V) : 0;
+    a.xcache = pr_shape<T>(g).K;
     a.r0 = r0;
     a.r1 = r1;
     a.x0 = x0;
@@ -477,15 +569,17 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
 template <typename T>
 static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st, uint64_t max_iter, bool prof = true)
 {
-    const uint32_t grid = pr_grid(g);
+    const PrShape sh = pr_shape<T>(g);
+    ZG_TRY_RC((pr_allow_lds<T, ZG_PW_PR>(sh)));
     const uint64_t K = 8;
     uint64_t launched = 0;
     while (launched < max_iter) {
         uint64_t k = max_iter - launched < K ? max_iter - launched : K;
         for (uint64_t i = 0; i < k; i++) {
             const size_t p = prof ? zg_prof_begin(g, s) : SIZE_MAX;
-            hipLaunchKernelGGL((pr_iter_kernel<T, ZG_PW_PR>), dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
-            hipLaunchKernelGGL((pr_finish_kernel<T, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, grid);
+            pr_launch_iter<T, ZG_PW_PR>(sh, s, a);
+            hipLaunchKernelGGL((pr_finish_kernel<T, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a,
+                               sh.grid);
             zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
         }
         ZG_TRY(hipGetLastError());
@@ -508,8 +602,8 @@ struct PrMem {
     double *r64 = nullptr, *x64 = nullptr;  // [2][V] each (FP64 phases, residual check)
     float *r32 = nullptr, *x32 = nullptr;   // [2][V] each (FP32 phases)
     double *part = nullptr;                 // [2][grid] block partials
-    float *seg32 = nullptr;                 // [n_seg][2] long-row segment partials
-    double *seg64 = nullptr;                // [n_seg][2]
+    float *seg32 = nullptr;                 // [n_seg][ZG_PR_SEG_PARTS][2] long-row segment partials
+    double *seg64 = nullptr;                // [n_seg][ZG_PR_SEG_PARTS][2]
     PrState *d_st = nullptr, *h_st = nullptr;  // [2] device state and its host mirror
     bool timed = true;
 };
@@ -662,9 +756,12 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
                                (const double *)d_out, false, 0.0, r64, r64 + V, x64, x64 + V);
             ZG_TRY(hipGetLastError());
             PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, mem->seg64, d_st + 1, part);
-            hipLaunchKernelGGL((pr_iter_kernel<double, ZG_PW_PR>), dim3(pr_grid(g)), dim3(ZG_PR_BLOCK), 0, s, a);
+            const PrShape sh = pr_shape<double>(g);
+            rc = pr_allow_lds<double, ZG_PW_PR>(sh);
+            if (rc != ZG_OK) break;
+            pr_launch_iter<double, ZG_PW_PR>(sh, s, a);
             hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a,
-                               pr_grid(g));
+                               sh.grid);
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(h_st + 1, d_st + 1, sizeof(PrState), hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
@@ -776,8 +873,8 @@ extern "C" int zg_pagerank_device_noalloc(zg_graph *g, double damping, uint64_t 
     };
     const size_t o_st = take(2 * sizeof(PrState));
     const size_t o_part = take(2 * sizeof(double) * pr_grid(g));
-    const size_t o_s32 = take(2 * sizeof(float) * (size_t)g->n_seg);
-    const size_t o_s64 = take(2 * sizeof(double) * (size_t)g->n_seg);
+    const size_t o_s32 = take(2 * ZG_PR_SEG_PARTS * sizeof(float) * (size_t)g->n_seg);
+    const size_t o_s64 = take(2 * ZG_PR_SEG_PARTS * sizeof(double) * (size_t)g->n_seg);
     const size_t o_64 = need64 ? take(4 * sizeof(double) * V) : 0;
     const size_t o_32 = need32 ? take(4 * sizeof(float) * V) : 0;
     if (off > ZG_NOALLOC_SCRATCH_BYTES) return ZG_ERR_SCRATCH;
@@ -1012,8 +1109,14 @@ static int zg_power_impl(zg_graph *g, int kind, double alpha, double beta, doubl
     a.rb1 = b1;
     a.rb2 = b2;
     const uint32_t grid = pr_grid(g);
+    const PrShape sh = pr_shape<double>(g);
+    int rc = kind == ZG_PW_RAW ? pr_allow_lds<double, ZG_PW_RAW>(sh) : pr_allow_lds<double, ZG_PW_AFFINE>(sh);
+    if (rc != ZG_OK) {
+        zg_dfree(d_start);
+        return rc;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = zg_hip_code(hipEventCreate(&e0));
+    rc = zg_hip_code(hipEventCreate(&e0));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e0, s));
     uint64_t launched = 0;
@@ -1023,15 +1126,15 @@ static int zg_power_impl(zg_graph *g, int kind, double alpha, double beta, doubl
         for (uint64_t i = 0; i < k; i++) {
             const size_t p = zg_prof_begin(g, s);
             if (kind == ZG_PW_RAW) {
-                hipLaunchKernelGGL((pr_iter_kernel<double, ZG_PW_RAW>), dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+                pr_launch_iter<double, ZG_PW_RAW>(sh, s, a);
                 hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_RAW>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s,
-                                   a, grid);
+                                   a, sh.grid);
                 hipLaunchKernelGGL(eig_norm_kernel, dim3(grid), dim3(256), 0, s, a, V);
                 hipLaunchKernelGGL(eig_decide_kernel, dim3(1), dim3(1024), 0, s, a, grid);
             } else {
-                hipLaunchKernelGGL((pr_iter_kernel<double, ZG_PW_AFFINE>), dim3(grid), dim3(ZG_PR_BLOCK), 0, s, a);
+                pr_launch_iter<double, ZG_PW_AFFINE>(sh, s, a);
                 hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_AFFINE>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0,
-                                   s, a, grid);
+                                   s, a, sh.grid);
             }
             zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
         }
@@ -1135,14 +1238,46 @@ __global__ __launch_bounds__(256) void pw_gather_kernel(const uint32_t *__restri
     if (s == (T)-1) sink[0] = s;  // never true (x is zero): keeps the loads
 }
 
+// The same pass with the iteration kernel's LDS copy of the most-gathered
+// values (ids below K), one 1024-thread block per CU: the floor of an
+// iteration that keeps that copy.
+template <typename T>
+__global__ __launch_bounds__(1024) void pw_gather_lds_kernel(const uint32_t *__restrict__ col,
+                                                             const T *__restrict__ x, uint32_t E, uint32_t K, T *sink)
+{
+    extern __shared__ double pw_lds[];
+    T *xs = reinterpret_cast<T *>(pw_lds);
+    for (uint32_t i = threadIdx.x; i < K; i += 1024) xs[i] = x[i];
+    __syncthreads();
+    T s = 0;
+    for (uint32_t e = blockIdx.x * 1024 + threadIdx.x; e < E; e += gridDim.x * 1024) {
+        const uint32_t u = col[e];
+        s += u < K ? xs[u] : x[u];
+    }
+    if (s == (T)-1) sink[0] = s;  // never true (x is zero): keeps the loads
+}
+
 extern "C" int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_per_pass)
 {
     if (!g) return ZG_ERR_INVALID_HANDLE;
+    const bool with_lds = (precision & ZG_FLOOR_LDS_COPY) != 0;
+    precision &= ~ZG_FLOOR_LDS_COPY;
     if ((precision != 32 && precision != 64) || reps < 1 || !us_per_pass) return ZG_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(g->mu);
     ZG_TRY(hipSetDevice(g->device));
     hipStream_t s = g->stream;
     const uint32_t V = g->V, E = (uint32_t)g->E;
+    // the LDS copy: as many values as the iteration kernel's 1024-thread
+    // blocks hold, one block per CU
+    pr_device_limits(g);
+    const uint32_t want = std::min<uint32_t>(g->pr_xcache, V);
+    const uint32_t K = with_lds ? std::min<uint32_t>(want, (g->dev_lds - 1024u) / (uint32_t)(precision / 8)) : 0u;
+    const size_t lds = (size_t)K * (precision / 8);
+    if (with_lds && lds > 65536) {
+        const void *f = precision == 32 ? reinterpret_cast<const void *>(&pw_gather_lds_kernel<float>)
+                                        : reinterpret_cast<const void *>(&pw_gather_lds_kernel<double>);
+        ZG_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     void *x = nullptr;
     ZG_TRY(hipMalloc(&x, (size_t)(precision / 8) * (V ? V : 1) + 64));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1150,10 +1285,17 @@ extern "C" int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
     const dim3 grid(8192), blk(256);
+    const dim3 lgrid(g->dev_cus), lblk(1024);
     for (int pass = 0; pass < 2 && rc == ZG_OK; pass++) {  // the first pass warms the caches
         if (pass) rc = zg_hip_code(hipEventRecord(e0, s));
         for (int i = 0; i < reps && rc == ZG_OK && E; i++) {
-            if (precision == 32)
+            if (with_lds && precision == 32)
+                hipLaunchKernelGGL(pw_gather_lds_kernel<float>, lgrid, lblk, lds, s, g->pr_col, (const float *)x, E,
+                                   K, (float *)x + (V ? V : 1));
+            else if (with_lds)
+                hipLaunchKernelGGL(pw_gather_lds_kernel<double>, lgrid, lblk, lds, s, g->pr_col, (const double *)x,
+                                   E, K, (double *)x + (V ? V : 1));
+            else if (precision == 32)
                 hipLaunchKernelGGL(pw_gather_kernel<float>, grid, blk, 0, s, g->pr_col, (const float *)x, E,
                                    (float *)x + (V ? V : 1));
             else
