@@ -315,10 +315,9 @@ int zg_alpha_centrality(zg_graph *g, double alpha, double initial, uint64_t max_
  * plain gather pass sum_e x[col[e]] over the resident predecessor columns
  * (one thread per arc; precision 32 or 64; averaged over `reps` passes after
  * a warm-up), the floor for any pull iteration over these columns.  With
- * ZG_FLOOR_LDS_COPY or'ed into precision the pass reads the most-gathered
- * values from an LDS copy the way the PageRank iteration does (one
- * 1024-thread block per CU, as many values as one block's LDS holds,
- * ZG_TUNE_PR_CACHE capping them): the floor of that layout. */
+ * ZG_FLOOR_LDS_COPY or'ed into precision the pass runs in the PageRank
+ * iteration kernel's launch shape and reads the most-gathered values from
+ * the same LDS copy: the floor of that layout. */
 #define ZG_FLOOR_LDS_COPY 0x100
 int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_per_pass);
 
@@ -361,10 +360,11 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   kernel from the host.  Per-kernel profiling always launches directly. */
 #define ZG_TUNE_GRAPH 4
 /*   ZG_TUNE_PR_CACHE: PageRank values of the most-gathered vertices each
- *   iteration block copies into LDS (default: as many as the block's LDS
- *   holds -- 16 KiB for 256-thread blocks, the device's per-block limit less
- *   1 KiB for larger ones -- when they serve at least 10% of the gathers;
- *   0 = none).  ZG_TUNE_PR_GRID: PageRank iteration blocks (0 = default:
+ *   iteration block copies into LDS (default, when they serve at least 10%
+ *   of the gathers: 16 KiB for 256-thread blocks; for larger blocks as many
+ *   as let a full CU's blocks -- 2048 threads -- share its LDS, each keeping
+ *   1 KiB; a larger value, up to the device's per-block limit, gives fewer
+ *   blocks per CU; 0 = none).  ZG_TUNE_PR_GRID: PageRank iteration blocks (0 = default:
  *   2048 256-thread blocks, or as many larger blocks as fit the CUs at once).
  *   ZG_TUNE_PR_BLOCK: threads per PageRank iteration block, 256, 512 or
  *   1024 (0 = default: 1024 when the LDS copy is used, else 256).  Row sums

@@ -43,7 +43,9 @@
 #include <climits>
 
 #define ZG_Q_MAX_SOURCES 1024  // sources per batch (LDS range tables)
+#ifndef ZG_Q_GRID
 #define ZG_Q_GRID 1024
+#endif
 #define ZG_Q_LEVELS_PER_SYNC 32
 #define ZG_Q_WIDE 32           // vertices with more predecessors are summed by the whole wavefront
 #define ZG_Q_BYTES_PER_SV 28   // workspace bytes per (source, vertex)
@@ -101,6 +103,19 @@ __global__ void bcq_adj_kernel(uint32_t V, const uint32_t *in_ptr, const uint32_
 #pragma unroll
         for (int q = 0; q < 4; q++) adj[(size_t)v * 4 + q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
     }
+}
+
+// One 16-byte record as a single vector load, so the compiler does not
+// split it into a depth load and a later load of the rest under the depth
+// test (which would add a dependent round trip).
+__device__ __forceinline__ zg_qrec zg_qload(const zg_qrec *p)
+{
+    const uint4 r = *reinterpret_cast<const uint4 *>(p);
+    zg_qrec q;
+    q.depth = r.x;
+    q.exp = r.y;
+    q.val = __hiloint2double((int)r.w, (int)r.z);
+    return q;
 }
 
 // A vertex's inline record, loaded whole (four 16-byte loads of one line).
@@ -165,12 +180,15 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
         const bool valid = idx < s_hi[s];
         zg_qrec *rec = a.rec + QROW(s);
-        const uint32_t v = valid ? a.queue[QROW(s) + idx] : 0;
-        ZgQAdj J;
+        // a lane past the level's end shadows its last entry, so every load
+        // below is unconditional (in flight together, one wait) and in bounds;
+        // only valid lanes write
+        const uint32_t v = a.queue[QROW(s) + (valid ? idx : s_hi[s] - 1)];
+        ZgQAdj J{};
         bool inl = false;
-        if (valid && a.adj) {
+        if (a.adj) {
             J.load(a.adj, v);
-            inl = J.inl();
+            inl = valid && J.inl();
         }
         // depths of the inline predecessors, kept so that a successor that is
         // also a predecessor (every arc of a symmetric graph) needs no second
@@ -195,19 +213,24 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
                 }
                 x += zg_scale(q.val, eu - E);
             };
+            // a second batch of four only where some lane has more than four
+            const bool more = __ballot(inl && ideg > 4) != 0;
             if (inl) {
-                // the record's predecessors, four at a time, in the same order
+                // the record's predecessors, four at a time, in the same order;
+                // the loads are unconditional (v's own record past ideg, whose
+                // depth is d, so the fold skips it) and masked afterwards
 #pragma unroll
                 for (int b = 0; b < 2; b++) {
-                    if ((uint32_t)(4 * b) >= ideg) break;
+                    if (b == 1 && !more) break;
                     uint32_t u[4];
                     zg_qrec q[4];
 #pragma unroll
-                    for (int t = 0; t < 4; t++) u[t] = (uint32_t)(4 * b + t) < ideg ? J.pred(b, t) : 0u;
+                    for (int t = 0; t < 4; t++) u[t] = (uint32_t)(4 * b + t) < ideg ? J.pred(b, t) : v;
 #pragma unroll
-                    for (int t = 0; t < 4; t++) q[t] = (uint32_t)(4 * b + t) < ideg ? rec[u[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
+                    for (int t = 0; t < 4; t++) q[t] = zg_qload(rec + u[t]);
 #pragma unroll
                     for (int t = 0; t < 4; t++) {
+                        if ((uint32_t)(4 * b + t) >= ideg) continue;
                         fold(q[t]);
                         if (q[t].depth == ZG_NONE) unset |= 1u << (4 * b + t);
                     }
@@ -268,6 +291,7 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
 #pragma unroll
             for (int t = 0; t < 4; t++)
                 w[t] = k0 + t < deg ? (inl ? J.succ((int)(k0 >> 2), t) : a.out_col[e0 + k0 + t]) : 0u;
+            bool need[4], any_need = false;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 bool known = false;
@@ -278,19 +302,46 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
                         dw[t] = (unset >> k) & 1u ? ZG_NONE : 0u;
                         known = true;
                     }
-                if (k0 + t < deg && !known) dw[t] = rec[w[t]].depth;
+                need[t] = k0 + t < deg && !known;
+                any_need |= need[t];
             }
+            if (any_need) {
+                // the depths not known yet, four loads in flight
+                uint32_t ld[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) ld[t] = rec[need[t] ? w[t] : v].depth;
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (need[t]) dw[t] = ld[t];
+            }
+            // the four claims in flight together, then one append for the
+            // wavefront's winners (arc by arc, lanes in order within an arc)
+            uint32_t prev[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                bool won = false;
+                prev[t] = 0u;  // any value but ZG_NONE
                 if (k0 + t < deg && dw[t] == ZG_NONE)
-                    won = atomicCAS(&rec[w[t]].depth, ZG_NONE, (uint32_t)(d + 1)) == ZG_NONE;
-                const uint64_t bw = __ballot(won);
-                if (bw) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&a.qlen[s], (uint32_t)__popcll(bw));
-                    base = zg_readlane32(base, 0);
-                    if (won) a.queue[QROW(s) + base + __popcll(bw & ((1ull << lane) - 1))] = w[t];
+                    prev[t] = atomicCAS(&rec[w[t]].depth, ZG_NONE, (uint32_t)(d + 1));
+            }
+            bool won[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) won[t] = prev[t] == ZG_NONE;
+            uint64_t bw[4];
+            uint32_t nwon = 0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                bw[t] = __ballot(won[t]);
+                nwon += (uint32_t)__popcll(bw[t]);
+            }
+            if (nwon) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&a.qlen[s], nwon);
+                base = zg_readlane32(base, 0);
+                const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (won[t]) a.queue[QROW(s) + base + __popcll(bw[t] & below)] = w[t];
+                    base += (uint32_t)__popcll(bw[t]);
                 }
             }
         }
@@ -340,35 +391,44 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
     for (uint32_t c = wk.c; c < wk.end; c += wk.stride) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
-        if (idx >= s_hi[s]) continue;
+        const bool valid = idx < s_hi[s];
         zg_qrec *rec = a.rec + QROW(s);
-        const uint32_t v = a.queue[QROW(s) + idx];
-        const zg_qrec me = rec[v];
-        const int ev = (int)me.exp;
+        // a lane past the level's end shadows its last entry (unconditional
+        // loads, in flight together) and writes nothing
+        const uint32_t v = a.queue[QROW(s) + (valid ? idx : s_hi[s] - 1)];
+        const double ws = a.wts ? a.wts[s] : 1.0;  // loaded with the rest, used at the end
         double S = 0.0;
-        ZgQAdj J;
+        ZgQAdj J{};
         bool inl = false;
         if (a.adj) {
             J.load(a.adj, v);
             inl = J.inl();
         }
+        zg_qrec me;
+        const bool more = __ballot(inl && J.odeg() > 4) != 0;
         if (inl) {
-            // the record's successors, four at a time, in the same order
+            // the record's successors, four at a time, in the same order;
+            // past the out-degree the loads read v's own record (depth d,
+            // skipped), which is needed anyway and comes in the same round
             const uint32_t od = J.odeg();
 #pragma unroll
             for (int b = 0; b < 2; b++) {
-                if ((uint32_t)(4 * b) >= od) break;
+                if (b == 1 && !more) break;
                 uint32_t w[4];
                 zg_qrec q[4];
 #pragma unroll
-                for (int t = 0; t < 4; t++) w[t] = (uint32_t)(4 * b + t) < od ? J.succ(b, t) : 0u;
+                for (int t = 0; t < 4; t++) w[t] = (uint32_t)(4 * b + t) < od ? J.succ(b, t) : v;
 #pragma unroll
-                for (int t = 0; t < 4; t++) q[t] = (uint32_t)(4 * b + t) < od ? rec[w[t]] : zg_qrec{ZG_NONE, 0u, 0.0};
+                for (int t = 0; t < 4; t++) q[t] = zg_qload(rec + w[t]);
+                if (b == 0) me = zg_qload(rec + v);
+                const int ev = (int)me.exp;
 #pragma unroll
                 for (int t = 0; t < 4; t++)
-                    if (q[t].depth == dnext) S += zg_scale(q[t].val, ev - (int)q[t].exp);
+                    if ((uint32_t)(4 * b + t) < od && q[t].depth == dnext) S += zg_scale(q[t].val, ev - (int)q[t].exp);
             }
         } else {
+            me = rec[v];
+            const int ev = (int)me.exp;
             for (uint32_t e = a.out_ptr[v], e1 = a.out_ptr[v + 1]; e < e1; e += 4) {
                 // four arcs at a time: heads, then the records (depth, exponent
                 // and, at depth d+1, the dependency term)
@@ -383,12 +443,13 @@ __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
                     if (q[t].depth == dnext) S += zg_scale(q[t].val, ev - (int)q[t].exp);
             }
         }
+        if (!valid) continue;
         const double sv = me.val;
         const double delta = sv * S;
         rec[v].val = (1.0 + delta) / sv;  // from here on the dependency term
         // kept per (source, vertex) and summed in source order after the batch
         // (levels >= 1 only: v is never the source here)
-        a.dlt[QROW(s) + v] = a.wts ? delta * a.wts[s] : delta;
+        a.dlt[QROW(s) + v] = a.wts ? delta * ws : delta;
     }
 }
 

@@ -212,6 +212,9 @@ struct zg_pr_ws {
     void *h_state = nullptr;   // pinned host mirror
 };
 
+// ZG_TUNE_PR_CACHE's default: as many values as the default block shape holds
+#define ZG_PR_XCACHE_DEFAULT (1u << 30)
+
 struct zg_graph {
     std::mutex mu;
     int device = 0;
@@ -287,7 +290,7 @@ struct zg_graph {
     double wmean = 0.0;                  // mean weight of the live arcs
     uint32_t *und_ptr = nullptr, *und_col = nullptr;  // simple undirected CSR (core numbers), built on demand
     uint32_t pull_grid = 4096;
-    uint32_t pr_xcache = 1u << 30;       // PageRank values staged in LDS (capped by the LDS size; 0 = none)
+    uint32_t pr_xcache = ZG_PR_XCACHE_DEFAULT;  // PageRank values staged in LDS (0 = none)
     bool pr_xcache_useful = false;       // the cached values carry a useful share of the gathers
     uint32_t pr_grid_cap = 0;            // PageRank iteration blocks (0 = default)
     uint32_t pr_block = 0;               // PageRank iteration block threads: 256, 512, 1024 (0 = default)

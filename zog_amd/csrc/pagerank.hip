@@ -103,19 +103,27 @@ struct PrGather {
     __device__ __forceinline__ T operator[](uint32_t u) const { return u < K ? lds[u] : x[u]; }
 };
 
-// Kahan sum of x[col[e]] for e = e0, e0+step, ... < e1, four gathers in flight.
+#ifndef ZG_PR_INFLIGHT
+#define ZG_PR_INFLIGHT 4  // gathers in flight per lane on strided rows
+#endif
+
+// Kahan sum of x[col[e]] for e = e0, e0+step, ... < e1, ZG_PR_INFLIGHT
+// gathers in flight.
 template <typename T, typename X>
 __device__ __forceinline__ void pr_strided_kahan(const uint32_t *__restrict__ col, const X &x,
                                                  uint32_t e0, uint32_t e1, uint32_t step, T &s, T &c)
 {
+    constexpr int N = ZG_PR_INFLIGHT;
     uint32_t e = e0;
-    for (; e + 3 * step < e1; e += 4 * step) {
-        const uint32_t c0 = col[e], c1 = col[e + step], c2 = col[e + 2 * step], c3 = col[e + 3 * step];
-        const T v0 = x[c0], v1 = x[c1], v2 = x[c2], v3 = x[c3];
-        kahan_add(s, c, v0);
-        kahan_add(s, c, v1);
-        kahan_add(s, c, v2);
-        kahan_add(s, c, v3);
+    for (; e + (N - 1) * step < e1; e += N * step) {
+        uint32_t ci[N];
+        T v[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) ci[k] = col[e + k * step];
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = x[ci[k]];
+#pragma unroll
+        for (int k = 0; k < N; k++) kahan_add(s, c, v[k]);
     }
     for (; e < e1; e += step) kahan_add(s, c, x[col[e]]);
 }
@@ -491,8 +499,14 @@ static PrShape pr_shape(zg_graph *g)
         sh.K = std::min<uint32_t>(want, ZG_PR_XCACHE_BYTES / sizeof(T));
         sh.grid = units;
     } else {
-        // leave 1 KB of the block's LDS for the kernel's own reductions
-        sh.K = std::min<uint32_t>(want, (g->dev_lds - 1024u) / (uint32_t)sizeof(T));
+        // by default as many values as let the blocks of a full CU (2048
+        // threads: 8 wavefronts per SIMD) share its LDS, each keeping 1 KB for
+        // its own reductions; a larger cap set by ZG_TUNE_PR_CACHE gives
+        // fewer, larger copies
+        const uint32_t full = std::max<uint32_t>(1u, 2048u / threads);
+        const uint32_t fit = (g->dev_lds / full - 1024u) / (uint32_t)sizeof(T);
+        const uint32_t most = (g->dev_lds - 1024u) / (uint32_t)sizeof(T);
+        sh.K = std::min<uint32_t>(want, g->pr_xcache == ZG_PR_XCACHE_DEFAULT ? fit : most);
         const uint32_t lds = sh.K * (uint32_t)sizeof(T);
         const uint32_t by_lds = lds ? std::max<uint32_t>(1u, g->dev_lds / lds) : 8u;
         const uint32_t by_threads = std::max<uint32_t>(1u, 2048u / threads);
@@ -1238,23 +1252,51 @@ __global__ __launch_bounds__(256) void pw_gather_kernel(const uint32_t *__restri
     if (s == (T)-1) sink[0] = s;  // never true (x is zero): keeps the loads
 }
 
-// The same pass with the iteration kernel's LDS copy of the most-gathered
-// values (ids below K), one 1024-thread block per CU: the floor of an
+// The same pass in the iteration kernel's launch shape (blocks of NT
+// threads, its grid) with its LDS copy of the most-gathered values (ids
+// below K), ZG_PR_INFLIGHT gathers in flight per thread: the floor of an
 // iteration that keeps that copy.
-template <typename T>
-__global__ __launch_bounds__(1024) void pw_gather_lds_kernel(const uint32_t *__restrict__ col,
-                                                             const T *__restrict__ x, uint32_t E, uint32_t K, T *sink)
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void pw_gather_lds_kernel(const uint32_t *__restrict__ col, const T *__restrict__ x,
+                                                           uint32_t E, uint32_t K, T *sink)
 {
     extern __shared__ double pw_lds[];
     T *xs = reinterpret_cast<T *>(pw_lds);
-    for (uint32_t i = threadIdx.x; i < K; i += 1024) xs[i] = x[i];
+    for (uint32_t i = threadIdx.x; i < K; i += NT) xs[i] = x[i];
     __syncthreads();
+    constexpr int N = ZG_PR_INFLIGHT;
+    const uint32_t step = gridDim.x * NT;
     T s = 0;
-    for (uint32_t e = blockIdx.x * 1024 + threadIdx.x; e < E; e += gridDim.x * 1024) {
+    uint32_t e = blockIdx.x * NT + threadIdx.x;
+    for (; e + (N - 1) * step < E; e += N * step) {
+        uint32_t u[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) u[k] = col[e + k * step];
+#pragma unroll
+        for (int k = 0; k < N; k++) s += u[k] < K ? xs[u[k]] : x[u[k]];
+    }
+    for (; e < E; e += step) {
         const uint32_t u = col[e];
         s += u < K ? xs[u] : x[u];
     }
     if (s == (T)-1) sink[0] = s;  // never true (x is zero): keeps the loads
+}
+
+template <typename T>
+static int pw_gather_lds_launch(const PrShape &sh, hipStream_t s, const uint32_t *col, const T *x, uint32_t E, T *sink,
+                                bool allow)
+{
+    const void *f = sh.threads == 1024  ? reinterpret_cast<const void *>(&pw_gather_lds_kernel<T, 1024>)
+                    : sh.threads == 512 ? reinterpret_cast<const void *>(&pw_gather_lds_kernel<T, 512>)
+                                        : reinterpret_cast<const void *>(&pw_gather_lds_kernel<T, 256>);
+    if (allow && sh.lds > 65536) ZG_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
+    if (sh.threads == 1024)
+        hipLaunchKernelGGL((pw_gather_lds_kernel<T, 1024>), dim3(sh.grid), dim3(1024), sh.lds, s, col, x, E, sh.K, sink);
+    else if (sh.threads == 512)
+        hipLaunchKernelGGL((pw_gather_lds_kernel<T, 512>), dim3(sh.grid), dim3(512), sh.lds, s, col, x, E, sh.K, sink);
+    else
+        hipLaunchKernelGGL((pw_gather_lds_kernel<T, 256>), dim3(sh.grid), dim3(256), sh.lds, s, col, x, E, sh.K, sink);
+    return ZG_OK;
 }
 
 extern "C" int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_per_pass)
@@ -1267,17 +1309,8 @@ extern "C" int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_
     ZG_TRY(hipSetDevice(g->device));
     hipStream_t s = g->stream;
     const uint32_t V = g->V, E = (uint32_t)g->E;
-    // the LDS copy: as many values as the iteration kernel's 1024-thread
-    // blocks hold, one block per CU
-    pr_device_limits(g);
-    const uint32_t want = std::min<uint32_t>(g->pr_xcache, V);
-    const uint32_t K = with_lds ? std::min<uint32_t>(want, (g->dev_lds - 1024u) / (uint32_t)(precision / 8)) : 0u;
-    const size_t lds = (size_t)K * (precision / 8);
-    if (with_lds && lds > 65536) {
-        const void *f = precision == 32 ? reinterpret_cast<const void *>(&pw_gather_lds_kernel<float>)
-                                        : reinterpret_cast<const void *>(&pw_gather_lds_kernel<double>);
-        ZG_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
+    // the iteration kernel's shape and LDS copy in this precision
+    const PrShape sh = precision == 32 ? pr_shape<float>(g) : pr_shape<double>(g);
     void *x = nullptr;
     ZG_TRY(hipMalloc(&x, (size_t)(precision / 8) * (V ? V : 1) + 64));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1285,16 +1318,15 @@ extern "C" int zg_gather_floor(zg_graph *g, int precision, int reps, double *us_
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e0));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
     const dim3 grid(8192), blk(256);
-    const dim3 lgrid(g->dev_cus), lblk(1024);
     for (int pass = 0; pass < 2 && rc == ZG_OK; pass++) {  // the first pass warms the caches
         if (pass) rc = zg_hip_code(hipEventRecord(e0, s));
         for (int i = 0; i < reps && rc == ZG_OK && E; i++) {
             if (with_lds && precision == 32)
-                hipLaunchKernelGGL(pw_gather_lds_kernel<float>, lgrid, lblk, lds, s, g->pr_col, (const float *)x, E,
-                                   K, (float *)x + (V ? V : 1));
+                rc = pw_gather_lds_launch<float>(sh, s, g->pr_col, (const float *)x, E, (float *)x + (V ? V : 1),
+                                                 pass == 0 && i == 0);
             else if (with_lds)
-                hipLaunchKernelGGL(pw_gather_lds_kernel<double>, lgrid, lblk, lds, s, g->pr_col, (const double *)x,
-                                   E, K, (double *)x + (V ? V : 1));
+                rc = pw_gather_lds_launch<double>(sh, s, g->pr_col, (const double *)x, E, (double *)x + (V ? V : 1),
+                                                  pass == 0 && i == 0);
             else if (precision == 32)
                 hipLaunchKernelGGL(pw_gather_kernel<float>, grid, blk, 0, s, g->pr_col, (const float *)x, E,
                                    (float *)x + (V ? V : 1));
