@@ -242,23 +242,32 @@ __global__ void bc_level_kernel_g(const BcLevelArgs *__restrict__ p)
     bc_level_body(p->a, d);
 }
 
-// One predecessor chunk of the pull at level d (discovers level d+1).
-__device__ __forceinline__ void bc_pull_chunk(const BcDev &a, uint32_t g, int lane, int d, uint32_t ch)
+// One predecessor chunk of the pull at level d (discovers level d+1); k is
+// the lane's level-d exponent shift.  Everything that needs only the chunk
+// is loaded in one round -- v's visited lanes, its in-degree (a vertex of
+// several chunks adds atomically) and the chunk's first 64 predecessors (a
+// lane past the end re-reads the last one; predecessor chunks are never
+// empty) -- so a chunk costs four dependent round trips: chunk, vertex,
+// predecessor masks, path-count rows.
+__device__ __forceinline__ void bc_pull_chunk(const BcDev &a, uint32_t g, int lane, int d, uint32_t ch, int k)
 {
     const uint32_t v = a.ich_v[ch], e0 = a.ich_e0[ch], e1 = a.ich_e1[ch];
+    const uint32_t e_first = e0 + lane;
+    const uint32_t u_first = a.in_col[e_first < e1 ? e_first : e1 - 1];
     const uint64_t unvis = ~a.vis[MASK(g, v)];
-    if (unvis == 0) return;
+    const uint32_t ideg = a.in_ptr[v + 1] - a.in_ptr[v];
+    // (no vertex id and no in-degree is ZG_NONE: the test is unvis == 0, and
+    // naming the other two keeps their loads ahead of the branch, in the
+    // same round as v's visited lanes)
+    if (unvis == 0 && u_first != ZG_NONE && ideg != ZG_NONE) return;
     const uint64_t *cur = fr_buf(a, d % 3);
     double acc = 0.0;
     uint64_t m = 0;
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
-        uint32_t u = 0;
-        uint64_t f = 0;
-        if (e < e1) {
-            u = a.in_col[e];
-            f = cur[MASK(g, u)] & unvis;
-        }
+        const uint32_t u = base == e0 ? u_first : a.in_col[e < e1 ? e : e1 - 1];
+        uint64_t f = cur[MASK(g, u)] & unvis;
+        if (e >= e1) f = 0;
         uint64_t act = __ballot(f != 0);
         // contributing predecessors, eight row reads in flight per round
         while (act) {
@@ -286,10 +295,9 @@ __device__ __forceinline__ void bc_pull_chunk(const BcDev &a, uint32_t g, int la
     }
     if (m == 0) return;
     const bool mine = (m >> lane) & 1;
-    const bool multi = (a.in_ptr[v + 1] - a.in_ptr[v]) > ZG_CHUNK;
+    const bool multi = ideg > ZG_CHUNK;
     const size_t r = ROW(g, v) + lane;
     if (mine) {
-        const int k = a.emax[EMAX(d, g, lane)];
         const double val = ldexp(acc, -k);
         if (multi) {
             atomicAdd(&a.sig[r], val);
@@ -318,8 +326,9 @@ __device__ __forceinline__ void bc_forward_body(const BcDev &a, int d, const Gro
     if (a.cnt[g].mode != ZG_MODE_PULL) return;
     const int lane = threadIdx.x & 63;
     const uint32_t nch = sp.chi[g] - sp.clo[g];
+    const int k = a.emax[EMAX(d, g, lane)];  // written for level d+1 only during this level
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < nch; i += gridDim.x * 4)
-        bc_pull_chunk(a, g, lane, d, sp.clo[g] + i);
+        bc_pull_chunk(a, g, lane, d, sp.clo[g] + i, k);
 }
 
 __global__ __launch_bounds__(256) void bc_forward_kernel(BcDev a, int d, GroupSpan sp) { bc_forward_body(a, d, sp); }
@@ -758,32 +767,57 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_sparse_kernel_g(co
     bc_finalize_sparse_body(p->a, *p->cur);
 }
 
+// What the dependency step of vertex v in group g needs besides its
+// successors' sum, per lane: the path count, the level's exponent shift,
+// the lane's source, its weight and (folded sweep) its target weight, and
+// the group accumulator of v (read by lane 0 only).  Loaded in one round.
+struct BcFinishIn {
+    double sv, wl, bc_old;
+    int k;
+    uint32_t srcl;
+    double tau;
+};
+
+__device__ __forceinline__ BcFinishIn bc_finish_load(const BcDev &a, uint32_t g, uint32_t v, int lane, int d)
+{
+    BcFinishIn in;
+    in.sv = a.sig[ROW(g, v) + lane];
+    in.k = a.emax[EMAX(d, g, lane)];
+    in.srcl = a.src[g * 64 + lane];
+    in.wl = a.wts ? a.wts[g * 64 + lane] : 1.0;
+    in.bc_old = a.bcg[(size_t)g * a.V + v];
+    // target weight: 1, or in a folded sweep 1 + the folded members of the
+    // lane's source, less one when v is itself one of them (lanes without a
+    // source read slot 0 and are inactive)
+    in.tau = 1.0;
+    if (a.succ1) {
+        const uint32_t m = in.srcl != ZG_NONE ? in.srcl : 0u;
+        in.tau += (double)a.fcount[m] - (a.succ1[v] == m ? 1.0 : 0.0);
+    }
+    return in;
+}
+
 // delta / c / bc for lanes `active` of vertex v at level d, given S = sum of
 // c over v's successors.
 __device__ __forceinline__ void bc_finish(const BcDev &a, uint32_t g, uint32_t v, int lane, bool active,
-                                          double S, int d)
+                                          double S, int d, const BcFinishIn &in)
 {
     double contrib = 0.0;
     if (active) {
         const size_t r = ROW(g, v) + lane;
-        const double sv = a.sig[r];
-        const int k = a.emax[EMAX(d, g, lane)];
-        const double delta = sv * ldexp(S, -k);
-        // target weight: 1, or in a folded sweep 1 + the folded members of
-        // the lane's source, less one when v is itself one of them
-        double tau = 1.0;
-        if (a.succ1) {
-            const uint32_t m = a.src[g * 64 + lane];
-            tau += (double)a.fcount[m] - (a.succ1[v] == m ? 1.0 : 0.0);
-        }
-        cc_buf(a, d & 1)[r] = (tau + delta) / sv;
-        if (a.src[g * 64 + lane] != v) contrib = a.wts ? delta * a.wts[g * 64 + lane] : delta;
+        const double delta = in.sv * ldexp(S, -in.k);
+        cc_buf(a, d & 1)[r] = (in.tau + delta) / in.sv;
+        if (in.srcl != v) contrib = a.wts ? delta * in.wl : delta;
     }
     contrib = zg_wave_sum(contrib);
-    if (lane == 0) a.bcg[(size_t)g * a.V + v] += contrib;
+    // v's entry of the group accumulator has one writer per level (this
+    // wavefront), and earlier levels finished before this launch
+    if (lane == 0) a.bcg[(size_t)g * a.V + v] = in.bc_old + contrib;
 }
 
-// Backward level d over the level's successor-chunk entries.
+// Backward level d over the level's successor-chunk entries.  Round trips:
+// the entry; its chunk; everything that needs only v (first successors,
+// hub slot, the finish inputs); the successors' level masks; their rows.
 __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupRange R)
 {
     const uint32_t g = blockIdx.y;
@@ -796,6 +830,12 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
     const bool active = (mask >> lane) & 1;
     const double *cg = cc_buf(a, (d + 1) & 1);
+    // (a sink's chunk is empty: no successor is read)
+    uint32_t w_first = 0;
+    if (e1 > e0) w_first = a.out_col[e0 + lane < e1 ? e0 + (uint32_t)lane : e1 - 1];
+    const uint32_t hub = a.ohub_idx[v];
+    const uint32_t first_ch = a.och_first[v];
+    const BcFinishIn fin = bc_finish_load(a, g, v, lane, d);
     // A successor w is a BFS child of v in lane l iff l's level-(d+1)
     // frontier held w; the other lanes' entries of w's row are zero.  With
     // the level's lane table, only child lanes read; past the table, every
@@ -806,13 +846,11 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     uint32_t tree = 0;  // this lane's children read (counted only when a.tree is set)
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + lane;
-        uint32_t w = 0;
-        uint64_t wm = 0;
-        if (e < e1) {
-            w = a.out_col[e];
-            if (exact) wm = a.lvl[LVLM(d + 1, g, w)] & mask;
-            else wm = a.maxdep[MASK(g, w)] >= (uint32_t)(d + 1) ? mask : 0;
-        }
+        const uint32_t w = base == e0 ? w_first : a.out_col[e < e1 ? e : e1 - 1];
+        uint64_t wm;
+        if (exact) wm = a.lvl[LVLM(d + 1, g, w)] & mask;
+        else wm = a.maxdep[MASK(g, w)] >= (uint32_t)(d + 1) ? mask : 0;
+        if (e >= e1) wm = 0;
         uint64_t act = __ballot(wm != 0);
         // eight row reads in flight per round
         while (act) {
@@ -846,13 +884,12 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
         const unsigned long long t = zg_wave_sum_u64(tree);
         if (lane == 0 && t) atomicAdd(&a.tree[(size_t)g * ZG_TREE_SLOTS + (blockIdx.x % ZG_TREE_SLOTS)], t);
     }
-    const uint32_t hub = a.ohub_idx[v];
     if (hub != ZG_NONE) {
         // one slot per chunk, summed in chunk order by bc_hub_kernel
-        if (active) a.hubacc[((size_t)g * a.n_ohub + hub + (oc - a.och_first[v])) * 64 + lane] = S;
+        if (active) a.hubacc[((size_t)g * a.n_ohub + hub + (oc - first_ch)) * 64 + lane] = S;
         return;
     }
-    bc_finish(a, g, v, lane, active, S, d);
+    bc_finish(a, g, v, lane, active, S, d, fin);
 }
 
 // Backward level d for vertices with several successor chunks.
@@ -866,13 +903,14 @@ __global__ __launch_bounds__(256) void bc_hub_kernel(BcDev a, int d, GroupRange 
     const uint32_t v = a.hlist[idx];
     const uint64_t mask = a.hmask[idx];
     const bool active = (mask >> lane) & 1;
+    const BcFinishIn fin = bc_finish_load(a, g, v, lane, d);
     double S = 0.0;
     if (active) {
         const double *p = &a.hubacc[((size_t)g * a.n_ohub + a.ohub_idx[v]) * 64 + lane];
         const uint32_t k = a.och_first[v + 1] - a.och_first[v];
         for (uint32_t q = 0; q < k; q++) S += p[(size_t)q * 64];
     }
-    bc_finish(a, g, v, lane, active, S, d);
+    bc_finish(a, g, v, lane, active, S, d, fin);
 }
 
 // Folded sweep, after a batch's backward pass: each lane's source m gets
