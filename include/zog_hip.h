@@ -367,10 +367,13 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   blocks per CU; 0 = none).  ZG_TUNE_PR_GRID: PageRank iteration blocks (0 = default:
  *   2048 256-thread blocks, or as many larger blocks as fit the CUs at once).
  *   ZG_TUNE_PR_BLOCK: threads per PageRank iteration block, 256, 512 or
- *   1024 (0 = default: 1024 when the LDS copy is used, else 256).  Row sums
- *   are the same under every setting; the FP64 sums of the dangling mass
- *   and of the change are added per block, so another grid can move the
- *   last bit of the ranks (a given setting repeats bit for bit). */
+ *   1024 (0 = default: 1024 when the LDS copy is used or some row has more
+ *   than 2048 predecessors -- a whole block sums such a row -- else 256).
+ *   Sums of rows up to 2048 predecessors are the same under every setting;
+ *   a longer row's sum follows the block size, and the FP64 sums of the
+ *   dangling mass and of the change are added per block, so another shape
+ *   can move the last bits of the ranks (a given setting repeats bit for
+ *   bit). */
 #define ZG_TUNE_PR_CACHE 5
 #define ZG_TUNE_PR_GRID 6
 #define ZG_TUNE_PR_BLOCK 12

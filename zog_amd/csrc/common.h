@@ -272,6 +272,13 @@ struct zg_graph {
     uint32_t *pr_large_seg0 = nullptr;   // [n_large + 1] first segment of each large row
     uint32_t *pr_seg_row = nullptr;      // [n_seg] large-row index of each segment
     uint32_t *pr_seg_e0 = nullptr, *pr_seg_e1 = nullptr;
+    // long rows dealt to pr_nlong iteration blocks by arc count (largest
+    // first, each to the least loaded block): slot b sums the rows
+    // pr_long_plan[pr_long_off[b] .. pr_long_off[b+1]), entries {row, first
+    // arc, end arc, 0}
+    uint32_t pr_nlong = 0;
+    uint4 *pr_long_plan = nullptr;
+    uint32_t *pr_long_off = nullptr;
     int groups = 4;
     int engine = 0;                      // ZG_ENGINE_*
     double *out_w = nullptr;             // [E] arc weights in successor-CSR order (weighted resources)
@@ -333,6 +340,9 @@ int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, cons
 // Chunk lists, hub slots and PageRank row bins from the device CSR and
 // PageRank ids (graph_derive.hip).
 int zg_derive_tables(zg_graph *g);
+// The long-row plan of the PageRank iteration (pagerank.hip) from the long
+// rows' first arcs and lengths, on the host (n_large entries).
+int zg_pr_long_plan(zg_graph *g, const uint32_t *h_e0, const uint32_t *h_len, hipStream_t s);
 // Index i of the component offset h_comp_off[i] == v (v is 0, V or a
 // component's first vertex).
 static inline size_t zg_comp_at(const zg_graph *g, uint32_t v)

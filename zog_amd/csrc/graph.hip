@@ -148,6 +148,7 @@ static void zg_free_device(zg_graph *g)
     zg_dfree(g->pr_small); zg_dfree(g->pr_med); zg_dfree(g->pr_desc); zg_dfree(g->pr_large); zg_dfree(g->pr_large_seg0);
     zg_dfree(g->pr_seg_row); zg_dfree(g->pr_seg_e0); zg_dfree(g->pr_seg_e1);
     zg_dfree(g->pr_large_deg); zg_dfree(g->pr_id); zg_dfree(g->pr_col);
+    zg_dfree(g->pr_long_plan); zg_dfree(g->pr_long_off);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
     g->stream = nullptr;

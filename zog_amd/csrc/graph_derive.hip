@@ -26,6 +26,7 @@
 #include "pr_bins.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace {
 
@@ -379,6 +380,17 @@ int zg_derive_tables(zg_graph *g)
         hipLaunchKernelGGL(dv_seg_kernel, grid_for(g->n_large), blk, 0, s, g->n_large, g->pr_large_seg0, pos + nsm,
                            len + nsm, g->pr_seg_row, g->pr_seg_e0, g->pr_seg_e1);
     ZG_TRY(hipGetLastError());
+    {
+        // the long rows' extents for the iteration's long-row plan (n_large
+        // entries, a few thousand at most on a power-law graph)
+        std::vector<uint32_t> h_e0(g->n_large), h_len(g->n_large);
+        if (g->n_large) {
+            ZG_TRY(hipMemcpyAsync(h_e0.data(), pos + nsm, sizeof(uint32_t) * g->n_large, hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipMemcpyAsync(h_len.data(), len + nsm, sizeof(uint32_t) * g->n_large, hipMemcpyDeviceToHost, s));
+            ZG_TRY(hipStreamSynchronize(s));
+        }
+        ZG_TRY_RC(zg_pr_long_plan(g, h_e0.data(), h_len.data(), s));
+    }
     // PageRank columns in row-bin order (short rows, then the medium bins,
     // then the long rows): the rows a wavefront sums take neighbouring column
     // words instead of words scattered over the predecessor CSR

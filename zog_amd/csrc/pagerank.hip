@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <mutex>
+#include <queue>
+#include <vector>
 
 struct PrState {
     unsigned int done;
@@ -50,15 +52,17 @@ struct PrState {
 template <typename T>
 struct PrArgs {
     const uint32_t *in_ptr, *in_col, *out_ptr;   // in_col: predecessors as PageRank ids
-    const uint32_t *small, *med, *large, *large_deg, *large_seg0, *seg_e0, *seg_e1;
+    const uint32_t *small, *med, *large, *large_deg;
     const uint4 *desc;          // [small | m4 | m16 | m64]: {v, first arc, in-degree, out-degree}
-    uint32_t n_small, n_m4, n_m16, n_m64, n_large, n_seg;
+    const uint4 *long_plan;     // long rows by block slot: {row, first arc, end arc, 0}
+    const uint32_t *long_off;   // [n_long + 1]
+    uint32_t n_small, n_m4, n_m16, n_m64, n_large, n_long;
     uint32_t nb_small, nb_m4, nb_m16, nb_m64;  // work units of each bin
     uint32_t xcache;            // values staged in LDS (0 = none)
+    unsigned long long it0;     // iteration index of this run's first launch
     T *r0, *r1, *x0, *x1;       // parity 0 / 1 buffers
     T *rb0, *rb1, *rb2;         // ZG_PW_RAW: the rotating (older, current, new) iterates
-    double *part;               // [gridDim.x][2]
-    T *seg;                     // [n_seg][ZG_PR_SEG_PARTS][2] Kahan pairs of long-row segments
+    double *part;               // [2 parities][gridDim.x][2] block partials
     PrState *st;
 };
 
@@ -241,56 +245,131 @@ __device__ __forceinline__ void pr_unit(const PrArgs<T> &a, uint32_t b, int tid,
     }
 }
 
-// One iteration.  A block of NT threads is NT/256 independent workers of
-// 256 threads (quarters); each takes work units with a grid stride, and all
-// share the block's LDS copy of the a.xcache most-gathered values (the
-// lowest PageRank ids), loaded once per launch: with one 1024-thread block
-// per CU the copy holds up to ~40K FP32 values.  A long-row segment is
-// summed by one quarter; its four wavefronts write their Kahan pairs, which
-// pr_finish_kernel merges in wavefront order.
+// One iteration.
+//
+// PageRank and Katz / alpha (the fused modes) need no finishing launch: a
+// launch for iteration index `it` first adds the previous launch's block
+// partials of (L1 change, dangling mass) -- every block the same numbers in
+// the same order -- and, when the change is below the tolerance or `it` is
+// the iteration cap, block 0 records the end in the state and every block
+// returns; otherwise the dangling mass gives this iteration's constant term.
+// Partials alternate between two buffers by `it`.  The eigenvector
+// iteration (ZG_PW_RAW) reads its index from the state and is finished by
+// pr_finish_kernel and the eig_* kernels.
+//
+// Work: blocks [0, n_long) each sum the long rows (more than ZG_PR_MED
+// predecessors) of their plan slot, one row at a time with all NT threads
+// (dealt by arc count, so these blocks finish with the others); the other
+// blocks are NT/256 independent 256-thread workers (quarters) over the row
+// bins' work units.  All of a block's threads share its LDS copy of the
+// a.xcache most-gathered values (the lowest PageRank ids), loaded once per
+// launch.
 template <typename T, int MODE, int NT>
-__global__ __launch_bounds__(NT) void pr_iter_kernel(PrArgs<T> a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void pr_iter_kernel(PrArgs<T> a, unsigned long long it_arg)
 {
     static_assert(NT % ZG_PR_BLOCK == 0, "a block is a whole number of 256-thread workers");
+    constexpr bool FUSED = MODE != ZG_PW_RAW;
     constexpr uint32_t NQ = NT / ZG_PR_BLOCK;
     extern __shared__ double pr_lds[];  // [a.xcache] values of type T (launch-sized)
     __shared__ double red[2][NT / 64];
-    const PrState *st = a.st;
+    __shared__ T kred[2][NT / 64];
+    __shared__ double tot[2];
+    PrState *st = a.st;
     if (st->done) return;
-    const unsigned long long it = st->iter;
+    const unsigned long long it = FUSED ? it_arg : st->iter;
+    // the previous launch's block partials (this thread's share), then the
+    // LDS copy: both sets of loads in flight together
+    const bool add_up = FUSED && it > a.it0;
+    double L = 0.0, D = 0.0;
+    if (add_up) {
+        const double *pp = a.part + 2 * (size_t)gridDim.x * ((it - 1) & 1);
+        for (uint32_t q = threadIdx.x; q < gridDim.x; q += NT) {
+            L += pp[2 * q];
+            D += pp[2 * q + 1];
+        }
+    }
     const PrBufs<T, MODE> B(a, it);
     const T *xg = B.xg;
     T *xs = reinterpret_cast<T *>(pr_lds);
     const uint32_t K = a.xcache;
     for (uint32_t i = threadIdx.x; i < K; i += NT) xs[i] = xg[i];
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double damping = st->damping;
+    double base = st->base;
+    if constexpr (FUSED) {
+        if (add_up) {
+            L = zg_wave_sum(L);
+            D = zg_wave_sum(D);
+            if (lane == 0) {
+                red[0][wv] = L;
+                red[1][wv] = D;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double Ls = 0.0, Ds = 0.0;
+                for (int w = 0; w < NT / 64; w++) {
+                    Ls += red[0][w];
+                    Ds += red[1][w];
+                }
+                tot[0] = Ls;
+                tot[1] = Ds;
+            }
+            __syncthreads();
+            L = tot[0];
+            D = tot[1];
+            // PageRank: (1-d)/n + d * sink; Katz / alpha: beta throughout
+            const double nb = MODE == ZG_PW_PR ? (1.0 - damping) * st->inv_n + damping * (D * st->inv_n) : base;
+            if (L < st->tol || it >= st->max_iter) {
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    st->iter = it;
+                    st->last_l1 = L;
+                    st->base = nb;
+                    st->done = 1;
+                }
+                return;
+            }
+            base = nb;
+        }
+    }
+    __syncthreads();  // the LDS copy (loads issued before the partials were added)
     const PrGather<T> x{xg, xs, K};
     const T *rold = B.rold;
     T *rnew = B.rnew;
     T *xnew = B.xnew;
-    const double base = st->base, damping = st->damping;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int tid = threadIdx.x & (ZG_PR_BLOCK - 1);
-    const uint32_t q = threadIdx.x / ZG_PR_BLOCK;
-    const uint32_t rowunits = a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64;
-    const uint32_t units = rowunits + a.n_seg;
     double l1 = 0.0, dang = 0.0;
 
-    for (uint32_t b = blockIdx.x * NQ + q; b < units; b += gridDim.x * NQ) {
-        if (b < rowunits) {
-            pr_unit<T, MODE>(a, b, tid, x, rold, rnew, xnew, base, damping, l1, dang);
-        } else {
-            // one worker per segment of a long row; pr_finish_kernel finishes the row
-            const uint32_t sg = b - rowunits;
+    // long rows: slots b, b + gridDim.x, ... (one slot per block unless the
+    // grid is smaller than the plan)
+    for (uint32_t slot = blockIdx.x; slot < a.n_long; slot += gridDim.x) {
+        for (uint32_t k = a.long_off[slot]; k < a.long_off[slot + 1]; k++) {
+            const uint4 p = a.long_plan[k];
             T s = 0, c = 0;
-            pr_strided_kahan(a.in_col, x, a.seg_e0[sg] + tid, a.seg_e1[sg], (uint32_t)ZG_PR_BLOCK, s, c);
+            pr_strided_kahan(a.in_col, x, p.y + threadIdx.x, p.z, (uint32_t)NT, s, c);
             wave_kahan_merge(s, c);
             if (lane == 0) {
-                const size_t o = 2 * ((size_t)ZG_PR_SEG_PARTS * sg + (tid >> 6));
-                a.seg[o] = s;
-                a.seg[o + 1] = c;
+                kred[0][wv] = s;
+                kred[1][wv] = c;
             }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                T S = kred[0][0], C = kred[1][0];
+                for (int w = 1; w < NT / 64; w++) kahan_merge(S, C, kred[0][w], kred[1][w]);
+                const uint32_t v = a.large[p.x];
+                pr_row_finish<T, MODE>(rnew, xnew, v, MODE != ZG_PW_RAW ? rold[v] : (T)0, a.large_deg[p.x], S, C,
+                                       base, damping, l1, dang);
+            }
+            __syncthreads();
         }
+    }
+    // row bins: quarters of the blocks past the long-row slots (of every
+    // block when the grid has no block to spare)
+    const uint32_t first = a.n_long < gridDim.x ? a.n_long : 0u;
+    if (blockIdx.x >= first) {
+        const int tid = threadIdx.x & (ZG_PR_BLOCK - 1);
+        const uint32_t q = threadIdx.x / ZG_PR_BLOCK;
+        const uint32_t units = a.nb_small + a.nb_m4 + a.nb_m16 + a.nb_m64;
+        for (uint32_t b = (blockIdx.x - first) * NQ + q; b < units; b += (gridDim.x - first) * NQ)
+            pr_unit<T, MODE>(a, b, tid, x, rold, rnew, xnew, base, damping, l1, dang);
     }
 
     // block partials of (l1, dangling mass); added in a fixed order later
@@ -307,79 +386,32 @@ __global__ __launch_bounds__(NT) void pr_iter_kernel(PrArgs<T> a)
             L += red[0][w];
             D += red[1][w];
         }
-        a.part[2 * blockIdx.x] = L;
-        a.part[2 * blockIdx.x + 1] = D;
+        double *pp = a.part + 2 * (size_t)gridDim.x * (FUSED ? (it & 1) : 0);
+        pp[2 * blockIdx.x] = L;
+        pp[2 * blockIdx.x + 1] = D;
     }
 }
 
 #define ZG_PR_FINISH_THREADS 1024
 
-// Kahan pair of one long-row segment: its wavefronts' pairs merged in order.
-template <typename T>
-__device__ __forceinline__ void pr_seg_sum(const T *seg, uint32_t q, T &S, T &C)
+// Eigenvector iteration: one block adds the iteration kernel's block
+// partials (the sum of squares of the raw iterate) in a fixed order for
+// eig_norm_kernel and eig_decide_kernel.
+__global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<double> a, uint32_t nparts)
 {
-    const T *p = seg + 2 * (size_t)ZG_PR_SEG_PARTS * q;
-    S = p[0];
-    C = p[1];
-#pragma unroll
-    for (int w = 1; w < ZG_PR_SEG_PARTS; w++) kahan_merge(S, C, p[2 * w], p[2 * w + 1]);
-}
-
-// One block: finish the long rows from their segment partials, add all block
-// partials in a fixed order, advance the iteration state.
-template <typename T, int MODE>
-__global__ __launch_bounds__(ZG_PR_FINISH_THREADS) void pr_finish_kernel(PrArgs<T> a, uint32_t nparts)
-{
-    __shared__ double red[2][ZG_PR_FINISH_THREADS / 64];
+    __shared__ double red[ZG_PR_FINISH_THREADS / 64];
     PrState *st = a.st;
     if (st->done) return;
-    const unsigned long long it = st->iter;
-    const PrBufs<T, MODE> B(a, it);
-    const T *rold = B.rold;
-    T *rnew = B.rnew;
-    T *xnew = B.xnew;
-    const double base = st->base, damping = st->damping;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    double l1 = 0.0, dang = 0.0;
-    for (uint32_t i = tid; i < a.n_large; i += ZG_PR_FINISH_THREADS) {
-        const uint32_t s0 = a.large_seg0[i], s1 = a.large_seg0[i + 1];
-        T S, C;
-        pr_seg_sum(a.seg, s0, S, C);
-        for (uint32_t q = s0 + 1; q < s1; q++) {
-            T S2, C2;
-            pr_seg_sum(a.seg, q, S2, C2);
-            kahan_merge(S, C, S2, C2);
-        }
-        const uint32_t v = a.large[i];
-        pr_row_finish<T, MODE>(rnew, xnew, v, MODE != ZG_PW_RAW ? rold[v] : (T)0, a.large_deg[i], S, C, base,
-                               damping, l1, dang);
-    }
-    for (uint32_t q = tid; q < nparts; q += ZG_PR_FINISH_THREADS) {
-        l1 += a.part[2 * q];
-        dang += a.part[2 * q + 1];
-    }
+    double l1 = 0.0;
+    for (uint32_t q = tid; q < nparts; q += ZG_PR_FINISH_THREADS) l1 += a.part[2 * q];
     l1 = zg_wave_sum(l1);
-    dang = zg_wave_sum(dang);
-    if (lane == 0) {
-        red[0][wv] = l1;
-        red[1][wv] = dang;
-    }
+    if (lane == 0) red[wv] = l1;
     __syncthreads();
     if (tid == 0) {
-        double L = 0.0, D = 0.0;
-        for (int w = 0; w < ZG_PR_FINISH_THREADS / 64; w++) {
-            L += red[0][w];
-            D += red[1][w];
-        }
-        if constexpr (MODE == ZG_PW_RAW) {
-            st->sumsq = L;  // eig_norm_kernel / eig_decide_kernel finish the iteration
-        } else {
-            const unsigned long long nit = it + 1;
-            st->iter = nit;
-            st->last_l1 = L;
-            if constexpr (MODE == ZG_PW_PR) st->base = (1.0 - damping) * st->inv_n + damping * (D * st->inv_n);
-            if (L < st->tol || nit >= st->max_iter) st->done = 1;
-        }
+        double L = 0.0;
+        for (int w = 0; w < ZG_PR_FINISH_THREADS / 64; w++) L += red[w];
+        st->sumsq = L;
     }
 }
 
@@ -422,8 +454,6 @@ int zg_pr_ws_release(zg_graph *g)
     zg_dfree(w.r);
     zg_dfree(w.xh);
     zg_dfree(w.part);
-    zg_dfree(w.seg32);
-    zg_dfree(w.seg64);
     zg_dfree(w.state);
     if (w.h_state) (void)hipHostFree(w.h_state);
     w.h_state = nullptr;
@@ -440,7 +470,7 @@ int zg_pr_ws_release(zg_graph *g)
 static uint32_t pr_grid(const zg_graph *g)
 {
     uint32_t units = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK + (g->n_m4 + 63) / 64 + (g->n_m16 + 15) / 16 +
-                     (g->n_m64 + 3) / 4 + g->n_seg;
+                     (g->n_m64 + 3) / 4 + g->pr_nlong;
     const uint32_t cap = g->pr_grid_cap ? g->pr_grid_cap : ZG_PR_MAX_GRID;
     return units < cap ? (units ? units : 1) : cap;
 }
@@ -454,9 +484,7 @@ static int pr_ws_ensure(zg_graph *g)
     // share one allocation: [r64 x2][x64 x2][r32 x2][x32 x2]
     ZG_TRY_RC(zg_dmalloc((char **)&w.r, V * (4 * 8 + 4 * 4)));
     w.xh = nullptr;
-    ZG_TRY_RC(zg_dmalloc(&w.part, 2 * (size_t)pr_grid(g)));
-    ZG_TRY_RC(zg_dmalloc(&w.seg32, ZG_PR_SEG_PARTS * (size_t)g->n_seg));
-    ZG_TRY_RC(zg_dmalloc(&w.seg64, 2 * ZG_PR_SEG_PARTS * (size_t)g->n_seg));
+    ZG_TRY_RC(zg_dmalloc(&w.part, 4 * (size_t)pr_grid(g)));  // two parities of (l1, dangling) per block
     ZG_TRY_RC(zg_dmalloc((char **)&w.state, 2 * sizeof(PrState)));
     ZG_TRY(hipMemset(w.state, 0, 2 * sizeof(PrState)));
     ZG_TRY(hipHostMalloc(&w.h_state, 2 * sizeof(PrState), hipHostMallocDefault));
@@ -492,7 +520,10 @@ static PrShape pr_shape(zg_graph *g)
     pr_device_limits(g);
     PrShape sh;
     const uint32_t units = pr_grid(g);
-    const uint32_t threads = g->pr_block ? g->pr_block : (g->pr_xcache_useful ? 1024u : (uint32_t)ZG_PR_BLOCK);
+    // large blocks where the copy is useful or rows are long (a long row is
+    // summed by a whole block)
+    const uint32_t threads =
+        g->pr_block ? g->pr_block : (g->pr_xcache_useful || g->n_large ? 1024u : (uint32_t)ZG_PR_BLOCK);
     sh.threads = threads;
     const uint32_t want = g->pr_xcache_useful ? std::min<uint32_t>(g->pr_xcache, g->V) : 0u;
     if (threads == ZG_PR_BLOCK) {
@@ -519,15 +550,67 @@ static PrShape pr_shape(zg_graph *g)
     return sh;
 }
 
+// Long rows (more than ZG_PR_MED predecessors) to iteration blocks: as many
+// slots as the default grid's share of blocks matches the long rows' share
+// of the arcs (at least one, at most half the grid), rows dealt largest
+// first to the least loaded slot, so a slot's block finishes with the
+// blocks that sum the row bins.  Each slot's rows in dealing order.
+int zg_pr_long_plan(zg_graph *g, const uint32_t *h_e0, const uint32_t *h_len, hipStream_t s)
+{
+    zg_dfree(g->pr_long_plan);
+    zg_dfree(g->pr_long_off);
+    g->pr_nlong = 0;
+    const uint32_t nl = g->n_large;
+    if (nl == 0) return ZG_OK;
+    const PrShape sh = pr_shape<float>(g);
+    uint64_t long_arcs = 0;
+    for (uint32_t i = 0; i < nl; i++) long_arcs += h_len[i];
+    const double share = g->E ? (double)long_arcs / (double)g->E : 1.0;
+    uint32_t P = (uint32_t)std::llround((double)sh.grid * share);
+    P = std::max<uint32_t>(1u, std::min<uint32_t>(P, sh.grid > 1 ? sh.grid / 2 : 1u));
+    P = std::min<uint32_t>(P, nl);
+    std::vector<uint32_t> order(nl);
+    for (uint32_t i = 0; i < nl; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return h_len[x] > h_len[y]; });
+    // least loaded slot first (ties: lowest slot)
+    using Slot = std::pair<uint64_t, uint32_t>;
+    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+    for (uint32_t b = 0; b < P; b++) heap.push({0ull, b});
+    std::vector<std::vector<uint32_t>> lists(P);
+    for (uint32_t i : order) {
+        Slot t = heap.top();
+        heap.pop();
+        lists[t.second].push_back(i);
+        t.first += h_len[i];
+        heap.push(t);
+    }
+    std::vector<uint4> plan;
+    plan.reserve(nl);
+    std::vector<uint32_t> off(P + 1, 0);
+    for (uint32_t b = 0; b < P; b++) {
+        off[b] = (uint32_t)plan.size();
+        for (uint32_t i : lists[b]) plan.push_back(make_uint4(i, h_e0[i], h_e0[i] + h_len[i], 0u));
+    }
+    off[P] = (uint32_t)plan.size();
+    ZG_TRY_RC(zg_dmalloc(&g->pr_long_plan, nl));
+    ZG_TRY_RC(zg_dmalloc(&g->pr_long_off, (size_t)P + 1));
+    ZG_TRY(hipMemcpyAsync(g->pr_long_plan, plan.data(), sizeof(uint4) * nl, hipMemcpyHostToDevice, s));
+    ZG_TRY(hipMemcpyAsync(g->pr_long_off, off.data(), sizeof(uint32_t) * (P + 1), hipMemcpyHostToDevice, s));
+    ZG_TRY(hipStreamSynchronize(s));
+    g->pr_nlong = P;
+    return ZG_OK;
+}
+
 template <typename T, int MODE>
-static void pr_launch_iter(const PrShape &sh, hipStream_t s, const PrArgs<T> &a)
+static void pr_launch_iter(const PrShape &sh, hipStream_t s, const PrArgs<T> &a, unsigned long long it = 0)
 {
     if (sh.threads == 1024)
-        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, 1024>), dim3(sh.grid), dim3(1024), sh.lds, s, a);
+        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, 1024>), dim3(sh.grid), dim3(1024), sh.lds, s, a, it);
     else if (sh.threads == 512)
-        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, 512>), dim3(sh.grid), dim3(512), sh.lds, s, a);
+        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, 512>), dim3(sh.grid), dim3(512), sh.lds, s, a, it);
     else
-        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, ZG_PR_BLOCK>), dim3(sh.grid), dim3(ZG_PR_BLOCK), sh.lds, s, a);
+        hipLaunchKernelGGL((pr_iter_kernel<T, MODE, ZG_PR_BLOCK>), dim3(sh.grid), dim3(ZG_PR_BLOCK), sh.lds, s, a,
+                           it);
 }
 
 // The dynamic LDS above the default 64 KB has to be allowed per kernel.
@@ -542,7 +625,7 @@ static int pr_allow_lds(const PrShape &sh)
 }
 
 template <typename T>
-static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrState *st, double *part = nullptr)
+static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, PrState *st, double *part = nullptr)
 {
     PrArgs<T> a;
     a.in_ptr = g->in_ptr;
@@ -553,15 +636,15 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.desc = g->pr_desc;
     a.large = g->pr_large;
     a.large_deg = g->pr_large_deg;
-    a.large_seg0 = g->pr_large_seg0;
-    a.seg_e0 = g->pr_seg_e0;
-    a.seg_e1 = g->pr_seg_e1;
+    a.long_plan = g->pr_long_plan;
+    a.long_off = g->pr_long_off;
+    a.n_long = g->pr_nlong;
+    a.it0 = 0;
     a.n_small = g->n_small;
     a.n_m4 = g->n_m4;
     a.n_m16 = g->n_m16;
     a.n_m64 = g->n_m64;
     a.n_large = g->n_large;
-    a.n_seg = g->n_seg;
     a.nb_small = (g->n_small + ZG_PR_BLOCK - 1) / ZG_PR_BLOCK;
     a.nb_m4 = (g->n_m4 + ZG_PR_BLOCK / 4 - 1) / (ZG_PR_BLOCK / 4);
     a.nb_m16 = (g->n_m16 + ZG_PR_BLOCK / 16 - 1) / (ZG_PR_BLOCK / 16);
@@ -573,39 +656,55 @@ static PrArgs<T> pr_args(zg_graph *g, T *r0, T *r1, T *x0, T *x1, T *seg, PrStat
     a.x1 = x1;
     a.rb0 = a.rb1 = a.rb2 = nullptr;
     a.part = part ? part : g->pr.part;
-    a.seg = seg;
     a.st = st;
     return a;
 }
 
-// Runs the power iteration in precision T from the initialised buffers.
-// Returns with *host mirror of the final state filled.
-template <typename T>
+// Runs a fused-mode power iteration (PageRank, Katz / alpha) in precision
+// T from the initialised buffers and the device state: launches for the
+// iteration indices a.it0, a.it0 + 1, ... up to st.max_iter, the last one
+// only adding up (it records the end when nothing before did), eight per
+// host sync; launches after the end return at once.  Returns with the host
+// mirror of the final state filled.
+template <typename T, int MODE>
 static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st, uint64_t max_iter, bool prof = true)
 {
     const PrShape sh = pr_shape<T>(g);
-    ZG_TRY_RC((pr_allow_lds<T, ZG_PW_PR>(sh)));
+    ZG_TRY_RC((pr_allow_lds<T, MODE>(sh)));
     const uint64_t K = 8;
-    uint64_t launched = 0;
-    while (launched < max_iter) {
-        uint64_t k = max_iter - launched < K ? max_iter - launched : K;
-        for (uint64_t i = 0; i < k; i++) {
+    unsigned long long it = a.it0;
+    bool synced = false;
+    while (it <= max_iter) {
+        const uint64_t k = std::min<uint64_t>(K, max_iter + 1 - it);
+        for (uint64_t i = 0; i < k; i++, it++) {
             const size_t p = prof ? zg_prof_begin(g, s) : SIZE_MAX;
-            pr_launch_iter<T, ZG_PW_PR>(sh, s, a);
-            hipLaunchKernelGGL((pr_finish_kernel<T, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a,
-                               sh.grid);
+            pr_launch_iter<T, MODE>(sh, s, a, it);
             zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
         }
         ZG_TRY(hipGetLastError());
-        launched += k;
         ZG_TRY(hipMemcpyAsync(h_st, a.st, sizeof(PrState), hipMemcpyDeviceToHost, s));
         ZG_TRY(hipStreamSynchronize(s));
+        synced = true;
         if (h_st->done) break;
     }
-    if (max_iter == 0) {
+    if (!synced) {
         ZG_TRY(hipMemcpyAsync(h_st, a.st, sizeof(PrState), hipMemcpyDeviceToHost, s));
         ZG_TRY(hipStreamSynchronize(s));
     }
+    return ZG_OK;
+}
+
+// Per-kernel profile of a fused run that computed `computed` iterations:
+// its launches past the closing one returned at once and are dropped; the
+// closing launch (it only adds the last partials up) is timed with the
+// iterations, whose count is what the profile reports.
+static int pr_collect_run(zg_graph *g, hipStream_t s, uint64_t computed)
+{
+    if (g->ev_marks.empty()) return ZG_OK;
+    if (g->ev_marks.size() > computed + 1) g->ev_marks.resize(computed + 1);
+    const bool closing = g->ev_marks.size() == computed + 1;
+    ZG_TRY_RC(zg_prof_collect(g, s));
+    if (closing && g->prof.launches[ZG_PROF_PR_ITER] > 0) g->prof.launches[ZG_PROF_PR_ITER] -= 1;
     return ZG_OK;
 }
 
@@ -615,9 +714,7 @@ static int pr_run(zg_graph *g, hipStream_t s, const PrArgs<T> &a, PrState *h_st,
 struct PrMem {
     double *r64 = nullptr, *x64 = nullptr;  // [2][V] each (FP64 phases, residual check)
     float *r32 = nullptr, *x32 = nullptr;   // [2][V] each (FP32 phases)
-    double *part = nullptr;                 // [2][grid] block partials
-    float *seg32 = nullptr;                 // [n_seg][ZG_PR_SEG_PARTS][2] long-row segment partials
-    double *seg64 = nullptr;                // [n_seg][ZG_PR_SEG_PARTS][2]
+    double *part = nullptr;                 // [2 parities][grid][2] block partials
     PrState *d_st = nullptr, *h_st = nullptr;  // [2] device state and its host mirror
     bool timed = true;
 };
@@ -655,8 +752,6 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
         ws.r32 = (float *)(ws.x64 + 2 * (size_t)V);
         ws.x32 = ws.r32 + 2 * (size_t)V;
         ws.part = g->pr.part;
-        ws.seg32 = (float *)g->pr.seg32;
-        ws.seg64 = g->pr.seg64;
         ws.d_st = (PrState *)g->pr.state;
         ws.h_st = (PrState *)g->pr.h_state;
         mem = &ws;
@@ -701,19 +796,15 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
         ZG_TRY(hipGetLastError());
         if (timed) ZG_TRY(hipEventRecord(e0, s));
         if (!fp64) {
-            PrArgs<float> a = pr_args<float>(g, r32, r32 + V, x32, x32 + V, mem->seg32, d_st, part);
-            rc = pr_run<float>(g, s, a, h_st, max_iterations, timed);
+            PrArgs<float> a = pr_args<float>(g, r32, r32 + V, x32, x32 + V, d_st, part);
+            rc = pr_run<float, ZG_PW_PR>(g, s, a, h_st, max_iterations, timed);
         } else {
-            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, mem->seg64, d_st, part);
-            rc = pr_run<double>(g, s, a, h_st, max_iterations, timed);
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, d_st, part);
+            rc = pr_run<double, ZG_PW_PR>(g, s, a, h_st, max_iterations, timed);
         }
         if (rc != ZG_OK) break;
         uint64_t iters = h_st->iter;
-        // launches queued past convergence returned at once: not profiled
-        if (timed) {
-            if (g->ev_marks.size() > iters) g->ev_marks.resize(iters);
-            ZG_TRY_RC(zg_prof_collect(g, s));
-        }
+        if (timed) ZG_TRY_RC(pr_collect_run(g, s, iters));
         bool final64 = fp64;
         if (mixed && !(h_st->last_l1 < tolerance) && iters < max_iterations) {
             // phase 2: FP64 from the FP32 iterate; the buffers of parity
@@ -730,14 +821,12 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             st2.tol = tolerance;
             st2.done = 0;
             ZG_TRY(hipMemcpyAsync(d_st, &st2, sizeof(PrState), hipMemcpyHostToDevice, s));
-            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, mem->seg64, d_st, part);
-            rc = pr_run<double>(g, s, a, h_st, max_iterations - iters, timed);
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, d_st, part);
+            a.it0 = iters;  // the FP32 phase's block partials are not read: st2.base carries its dangling mass
+            rc = pr_run<double, ZG_PW_PR>(g, s, a, h_st, max_iterations, timed);
             if (rc != ZG_OK) break;
             const uint64_t more = h_st->iter - iters;
-            if (timed) {
-                if (g->ev_marks.size() > more) g->ev_marks.resize(more);
-                ZG_TRY_RC(zg_prof_collect(g, s));
-            }
+            if (timed) ZG_TRY_RC(pr_collect_run(g, s, more));
             inf.fp64_iterations = more;
             iters = h_st->iter;
         }
@@ -769,13 +858,12 @@ static int zg_pagerank_impl(zg_graph *g, double damping, uint64_t max_iterations
             hipLaunchKernelGGL((pr_init_kernel<double, double>), vg, vb, 0, s, V, g->live, g->out_ptr, g->pr_id,
                                (const double *)d_out, false, 0.0, r64, r64 + V, x64, x64 + V);
             ZG_TRY(hipGetLastError());
-            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, mem->seg64, d_st + 1, part);
+            PrArgs<double> a = pr_args<double>(g, r64, r64 + V, x64, x64 + V, d_st + 1, part);
             const PrShape sh = pr_shape<double>(g);
             rc = pr_allow_lds<double, ZG_PW_PR>(sh);
             if (rc != ZG_OK) break;
-            pr_launch_iter<double, ZG_PW_PR>(sh, s, a);
-            hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_PR>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a,
-                               sh.grid);
+            pr_launch_iter<double, ZG_PW_PR>(sh, s, a, 0);  // the application
+            pr_launch_iter<double, ZG_PW_PR>(sh, s, a, 1);  // its change, added up
             ZG_TRY(hipGetLastError());
             ZG_TRY(hipMemcpyAsync(h_st + 1, d_st + 1, sizeof(PrState), hipMemcpyDeviceToHost, s));
             ZG_TRY(hipStreamSynchronize(s));
@@ -874,7 +962,7 @@ extern "C" int zg_pagerank_device_noalloc(zg_graph *g, double damping, uint64_t 
     std::lock_guard<std::mutex> sl(g_scratch_mu[g->device]);
     ZG_TRY(hipSetDevice(g->device));
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
-    // layout: [state][block partials][segment partials FP32][FP64]
+    // layout: [state][block partials, two parities]
     //         [FP64 r, x: both parities][FP32 r, x: both parities]
     const size_t V = g->V;
     const bool need64 = prec != ZG_PR_FP32 || !(precision & ZG_PR_SKIP_RESIDUAL);
@@ -886,9 +974,7 @@ extern "C" int zg_pagerank_device_noalloc(zg_graph *g, double damping, uint64_t 
         return o;
     };
     const size_t o_st = take(2 * sizeof(PrState));
-    const size_t o_part = take(2 * sizeof(double) * pr_grid(g));
-    const size_t o_s32 = take(2 * ZG_PR_SEG_PARTS * sizeof(float) * (size_t)g->n_seg);
-    const size_t o_s64 = take(2 * ZG_PR_SEG_PARTS * sizeof(double) * (size_t)g->n_seg);
+    const size_t o_part = take(4 * sizeof(double) * pr_grid(g));
     const size_t o_64 = need64 ? take(4 * sizeof(double) * V) : 0;
     const size_t o_32 = need32 ? take(4 * sizeof(float) * V) : 0;
     if (off > ZG_NOALLOC_SCRATCH_BYTES) return ZG_ERR_SCRATCH;
@@ -898,8 +984,6 @@ extern "C" int zg_pagerank_device_noalloc(zg_graph *g, double damping, uint64_t 
     m.d_st = (PrState *)(base + o_st);
     m.h_st = g_scratch_state[g->device];
     m.part = (double *)(base + o_part);
-    m.seg32 = (float *)(base + o_s32);
-    m.seg64 = (double *)(base + o_s64);
     if (need64) {
         m.r64 = (double *)(base + o_64);
         m.x64 = m.r64 + 2 * V;
@@ -1118,7 +1202,7 @@ static int zg_power_impl(zg_graph *g, int kind, double alpha, double beta, doubl
                        kind == ZG_PW_RAW ? b2 : nullptr);
     ZG_TRY(hipMemcpyAsync(d_st, &st0, sizeof(PrState), hipMemcpyHostToDevice, s));
     ZG_TRY(hipGetLastError());
-    PrArgs<double> a = pr_args<double>(g, b0, b1, b0, b1, g->pr.seg64, d_st);  // Katz / alpha gather x = r
+    PrArgs<double> a = pr_args<double>(g, b0, b1, b0, b1, d_st);  // Katz / alpha gather x = r
     a.rb0 = b0;
     a.rb1 = b1;
     a.rb2 = b2;
@@ -1133,34 +1217,31 @@ static int zg_power_impl(zg_graph *g, int kind, double alpha, double beta, doubl
     rc = zg_hip_code(hipEventCreate(&e0));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventCreate(&e1));
     if (rc == ZG_OK) rc = zg_hip_code(hipEventRecord(e0, s));
-    uint64_t launched = 0;
-    const uint64_t K = 8;
-    while (rc == ZG_OK && launched < max_iterations) {
-        const uint64_t k = std::min<uint64_t>(K, max_iterations - launched);
-        for (uint64_t i = 0; i < k; i++) {
-            const size_t p = zg_prof_begin(g, s);
-            if (kind == ZG_PW_RAW) {
+    if (kind == ZG_PW_RAW) {
+        uint64_t launched = 0;
+        const uint64_t K = 8;
+        while (rc == ZG_OK && launched < max_iterations) {
+            const uint64_t k = std::min<uint64_t>(K, max_iterations - launched);
+            for (uint64_t i = 0; i < k; i++) {
+                const size_t p = zg_prof_begin(g, s);
                 pr_launch_iter<double, ZG_PW_RAW>(sh, s, a);
-                hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_RAW>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s,
-                                   a, sh.grid);
+                hipLaunchKernelGGL(pr_finish_kernel, dim3(1), dim3(ZG_PR_FINISH_THREADS), 0, s, a, sh.grid);
                 hipLaunchKernelGGL(eig_norm_kernel, dim3(grid), dim3(256), 0, s, a, V);
                 hipLaunchKernelGGL(eig_decide_kernel, dim3(1), dim3(1024), 0, s, a, grid);
-            } else {
-                pr_launch_iter<double, ZG_PW_AFFINE>(sh, s, a);
-                hipLaunchKernelGGL((pr_finish_kernel<double, ZG_PW_AFFINE>), dim3(1), dim3(ZG_PR_FINISH_THREADS), 0,
-                                   s, a, sh.grid);
+                zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
             }
-            zg_prof_end(g, s, ZG_PROF_PR_ITER, p);
+            rc = zg_hip_code(hipGetLastError());
+            launched += k;
+            if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(h_st, d_st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+            if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+            if (rc == ZG_OK && h_st->done) break;
         }
-        rc = zg_hip_code(hipGetLastError());
-        launched += k;
-        if (rc == ZG_OK) rc = zg_hip_code(hipMemcpyAsync(h_st, d_st, sizeof(PrState), hipMemcpyDeviceToHost, s));
-        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
-        if (rc == ZG_OK && h_st->done) break;
-    }
-    if (rc == ZG_OK && max_iterations == 0) {
-        rc = zg_hip_code(hipMemcpyAsync(h_st, d_st, sizeof(PrState), hipMemcpyDeviceToHost, s));
-        if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+        if (rc == ZG_OK && max_iterations == 0) {
+            rc = zg_hip_code(hipMemcpyAsync(h_st, d_st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+            if (rc == ZG_OK) rc = zg_hip_code(hipStreamSynchronize(s));
+        }
+    } else if (rc == ZG_OK) {
+        rc = pr_run<double, ZG_PW_AFFINE>(g, s, a, h_st, max_iterations);
     }
     if (rc == ZG_OK) {
         // iterations that ran (the eigenvector counts its last, deciding one)
@@ -1168,8 +1249,12 @@ static int zg_power_impl(zg_graph *g, int kind, double alpha, double beta, doubl
         const int oc = h_st->outcome;
         uint64_t ran = it;
         if (kind == ZG_PW_RAW && max_iterations > 0 && oc != ZG_EIG_CAP) ran = it + 1;
-        if (g->ev_marks.size() > ran) g->ev_marks.resize(ran);
-        rc = zg_prof_collect(g, s);
+        if (kind == ZG_PW_RAW) {
+            if (g->ev_marks.size() > ran) g->ev_marks.resize(ran);
+            rc = zg_prof_collect(g, s);
+        } else {
+            rc = pr_collect_run(g, s, ran);
+        }
         inf.iterations = ran;
         inf.last_l1 = h_st->last_l1;
         inf.outcome = kind == ZG_PW_RAW ? oc : 0;
