@@ -57,6 +57,7 @@ struct BcqDev {
     zg_qrec *rec;                   // [S][V] {depth, exponent, sig or cc}
     uint32_t *queue;                // [S][V]
     uint32_t *lstart;               // [lcap][S]
+    uint32_t *cpre;                 // [lcap][S + 1] chunk prefix of each level (total last)
     uint32_t *qlen;                 // [S]
     uint32_t *newcnt;               // [lcap] entries added at each level
     const uint32_t *src;            // [S]
@@ -160,21 +161,38 @@ __global__ void bcq_init_kernel(BcqDev a)
     atomicAdd(&a.stats[1], 1ull);
 }
 
+// The per-source ranges and chunk prefix of level d into LDS (published by
+// the close step), and the total; false when this workgroup has no chunk
+// of the level (then it touches nothing and leaves).
+__device__ __forceinline__ bool bcq_level_tables(const BcqDev &a, int d, uint32_t *s_lo, uint32_t *s_hi,
+                                                 uint32_t *s_pre, uint32_t &total, ZgQWalk &wk)
+{
+    const uint32_t *pre = a.cpre + (size_t)d * (a.S + 1);
+    total = pre[a.S];
+    wk = zg_q_walk(total);
+    // the block's first chunk is its wavefront 0's
+    const uint32_t first = wk.c - (threadIdx.x >> 6);
+    if (first >= wk.end) return false;
+    for (uint32_t s = threadIdx.x; s < a.S; s += blockDim.x) {
+        s_pre[s] = pre[s];
+        s_lo[s] = a.lstart[(size_t)d * a.S + s];
+        s_hi[s] = a.lstart[(size_t)(d + 1) * a.S + s];
+    }
+    if (threadIdx.x == 0) s_pre[a.S] = total;
+    __syncthreads();
+    return true;
+}
+
 __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
 {
     __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
     __shared__ unsigned long long red[2][4];
-    const uint32_t total = zg_q_chunks(
-        a.S,
-        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
-            lo = a.lstart[(size_t)d * a.S + s];
-            hi = a.lstart[(size_t)(d + 1) * a.S + s];
-        },
-        s_lo, s_hi, s_pre);
+    uint32_t total;
+    ZgQWalk wk;
+    if (!bcq_level_tables(a, d, s_lo, s_hi, s_pre, total, wk)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t dprev = (uint32_t)(d - 1);
     unsigned long long arcs = 0, reached = 0;
-    const ZgQWalk wk = zg_q_walk(total);
     for (uint32_t c = wk.c; c < wk.end; c += wk.stride) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
@@ -361,33 +379,46 @@ __global__ __launch_bounds__(256) void bcq_forward_kernel(BcqDev a, int d)
     }
 }
 
-// Close level L (just appended): the offset of level L+1 and the level's
-// size.  One block.
+// Close level L (just appended): the offset of level L+1, the level's size,
+// and the prefix over sources of the level's 64-entry chunks (the forward
+// and backward launches of level L read it instead of each workgroup
+// scanning the ranges again).  One block of 1024 threads; S <= 1024.
 __global__ __launch_bounds__(1024) void bcq_close_kernel(BcqDev a, int L)
 {
-    uint32_t added = 0;
-    for (uint32_t s = threadIdx.x; s < a.S; s += blockDim.x) {
+    __shared__ uint32_t wtot[16];
+    const uint32_t s = threadIdx.x;
+    const int lane = s & 63, wv = s >> 6;
+    uint32_t added = 0, cnt = 0;
+    if (s < a.S) {
         const uint32_t q = a.qlen[s];
         a.lstart[(size_t)(L + 1) * a.S + s] = q;
-        added += q - a.lstart[(size_t)L * a.S + s];
+        added = q - a.lstart[(size_t)L * a.S + s];
+        cnt = (added + 63) / 64;
     }
-    added = (uint32_t)zg_wave_sum_u64(added);
-    if ((threadIdx.x & 63) == 0 && added) atomicAdd(&a.newcnt[L], added);
+    const uint32_t incl = zg_wave_incl_scan(cnt, lane);
+    if (lane == 63) wtot[wv] = incl;
+    const uint32_t tot_added = (uint32_t)zg_wave_sum_u64(added);
+    if (lane == 0 && tot_added) atomicAdd(&a.newcnt[L], tot_added);
+    __syncthreads();
+    uint32_t off = 0;
+    for (int w = 0; w < wv; w++) off += wtot[w];
+    uint32_t *pre = a.cpre + (size_t)L * (a.S + 1);
+    if (s < a.S) pre[s] = off + incl - cnt;
+    if (s == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < 16; w++) t += wtot[w];
+        pre[a.S] = t;
+    }
 }
 
 __global__ __launch_bounds__(256) void bcq_backward_kernel(BcqDev a, int d)
 {
     __shared__ uint32_t s_lo[ZG_Q_MAX_SOURCES], s_hi[ZG_Q_MAX_SOURCES], s_pre[ZG_Q_MAX_SOURCES + 1];
-    const uint32_t total = zg_q_chunks(
-        a.S,
-        [&](uint32_t s, uint32_t &lo, uint32_t &hi) {
-            lo = a.lstart[(size_t)d * a.S + s];
-            hi = a.lstart[(size_t)(d + 1) * a.S + s];
-        },
-        s_lo, s_hi, s_pre);
+    uint32_t total;
+    ZgQWalk wk;
+    if (!bcq_level_tables(a, d, s_lo, s_hi, s_pre, total, wk)) return;
     const int lane = threadIdx.x & 63;
     const uint32_t dnext = (uint32_t)(d + 1);
-    const ZgQWalk wk = zg_q_walk(total);
     for (uint32_t c = wk.c; c < wk.end; c += wk.stride) {
         const uint32_t s = zg_q_owner(s_pre, a.S, c);
         const uint32_t idx = s_lo[s] + (c - s_pre[s]) * 64 + lane;
@@ -676,7 +707,7 @@ static int bcq_build_order(zg_graph *g, hipStream_t s)
 int zg_bcq_ws_release(zg_graph *g)
 {
     zg_bcq_ws &w = g->bcq;
-    zg_dfree(w.rec); zg_dfree(w.queue); zg_dfree(w.dlt); zg_dfree(w.lstart);
+    zg_dfree(w.rec); zg_dfree(w.queue); zg_dfree(w.dlt); zg_dfree(w.lstart); zg_dfree(w.cpre);
     zg_dfree(w.qlen); zg_dfree(w.newcnt); zg_dfree(w.src); zg_dfree(w.wts); zg_dfree(w.bc);
     zg_dfree(w.stats);
     w.S = 0;
@@ -696,6 +727,7 @@ static int bcq_ws_ensure(zg_graph *g, uint32_t S)
     if (rc == ZG_OK) rc = zg_dmalloc(&w.queue, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.dlt, S * V);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.lstart, (size_t)w.lcap * S);
+    if (rc == ZG_OK) rc = zg_dmalloc(&w.cpre, (size_t)w.lcap * (S + 1));
     if (rc == ZG_OK) rc = zg_dmalloc(&w.qlen, S);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.newcnt, w.lcap);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.src, S);
@@ -718,22 +750,27 @@ static int bcq_grow_levels(zg_graph *g, uint32_t need, uint32_t used, hipStream_
     if (need < w.lcap) return ZG_OK;
     uint32_t nl = w.lcap;
     while (nl <= need) nl *= 2;
-    uint32_t *ls = nullptr, *nc = nullptr;
+    uint32_t *ls = nullptr, *nc = nullptr, *cp = nullptr;
     ZG_TRY_RC(zg_dmalloc(&ls, (size_t)nl * w.S));
     int rc = zg_dmalloc(&nc, nl);
+    if (rc == ZG_OK) rc = zg_dmalloc(&cp, (size_t)nl * (w.S + 1));
     if (rc != ZG_OK) {
         zg_dfree(ls);
+        zg_dfree(nc);
         return rc;
     }
+    const size_t keep = std::min(used, w.lcap);
     ZG_TRY(hipMemsetAsync(nc, 0, sizeof(uint32_t) * nl, s));
-    ZG_TRY(hipMemcpyAsync(ls, w.lstart, sizeof(uint32_t) * (size_t)std::min(used, w.lcap) * w.S,
-                          hipMemcpyDeviceToDevice, s));
-    ZG_TRY(hipMemcpyAsync(nc, w.newcnt, sizeof(uint32_t) * std::min(used, w.lcap), hipMemcpyDeviceToDevice, s));
+    ZG_TRY(hipMemcpyAsync(ls, w.lstart, sizeof(uint32_t) * keep * w.S, hipMemcpyDeviceToDevice, s));
+    ZG_TRY(hipMemcpyAsync(cp, w.cpre, sizeof(uint32_t) * keep * (w.S + 1), hipMemcpyDeviceToDevice, s));
+    ZG_TRY(hipMemcpyAsync(nc, w.newcnt, sizeof(uint32_t) * keep, hipMemcpyDeviceToDevice, s));
     ZG_TRY(hipStreamSynchronize(s));
     zg_dfree(w.lstart);
     zg_dfree(w.newcnt);
+    zg_dfree(w.cpre);
     w.lstart = ls;
     w.newcnt = nc;
+    w.cpre = cp;
     w.lcap = nl;
     return ZG_OK;
 }
@@ -754,6 +791,7 @@ static BcqDev bcq_dev(zg_graph *g, bool weighted)
     a.rec = w.rec;
     a.queue = w.queue;
     a.lstart = w.lstart;
+    a.cpre = w.cpre;
     a.qlen = w.qlen;
     a.newcnt = w.newcnt;
     a.src = w.src;
@@ -810,6 +848,8 @@ int zg_bcq_sweep(zg_graph *g, const uint32_t *d_sources, uint64_t n, const doubl
         ZG_TRY(hipMemsetAsync(w.stats, 0, sizeof(unsigned long long) * 2, s));
         BcqDev a = bcq_dev(g, d_weights != nullptr);
         ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bcq_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, a);
+        // level 0's chunk prefix (its offsets are already set; re-set to the same values)
+        ZG_PROF_LAUNCH(ZG_PROF_BC_FINALIZE, bcq_close_kernel, dim3(1), dim3(1024), 0, s, a, 0);
         ZG_TRY(hipGetLastError());
         // forward: queue levels in chunks, sync to find the last one
         uint32_t d = 0;

@@ -155,6 +155,7 @@ struct zg_bcq_ws {
     uint32_t *queue = nullptr;           // [S][V]
     double *dlt = nullptr;               // [S][V] weighted dependencies of the batch
     uint32_t *lstart = nullptr;          // [lcap][S]
+    uint32_t *cpre = nullptr;            // [lcap][S + 1] per-level prefix of 64-entry chunks, total last
     uint32_t *qlen = nullptr;            // [S]
     uint32_t *newcnt = nullptr;          // [lcap]
     uint32_t *src = nullptr;             // [S]

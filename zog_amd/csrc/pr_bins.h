@@ -12,9 +12,8 @@
 #define ZG_PR_M16 256      // <= 256: 16 lanes per row
 #endif
 #define ZG_PR_MED 2048     // <= 2048: 64 lanes per row
-#define ZG_PR_SEG 4096     // longer rows: 256-lane blocks over 4096-arc segments
+#define ZG_PR_SEG 4096     // longer rows: listed as 4096-arc segments (zg_graph_table), summed by a whole block
 #define ZG_PR_BLOCK 256
-#define ZG_PR_SEG_PARTS (ZG_PR_BLOCK / 64)  // wavefront Kahan pairs per long-row segment
 #ifndef ZG_PR_XCACHE_BYTES
 #define ZG_PR_XCACHE_BYTES 16384  // LDS copy of the most-gathered values per 256-thread iteration block
 #endif
