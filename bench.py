@@ -85,7 +85,8 @@ def roofline_entry(bytes_per_step, ms, launches, kernel, traffic=None):
 
 
 def traffic_of(workload, kernel):
-    """HBM bytes per dispatch of `kernel` from profiles/traffic_<workload>.json."""
+    """HBM bytes per dispatch (or per unit of work, where the file gives units) of `kernel` from
+    profiles/traffic_<workload>.json."""
     tfile = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(tfile):
         return None
@@ -104,7 +105,9 @@ def traffic_of(workload, kernel):
     parts = [entry(k) for k in kernel.split("+")]
     if not all(parts):
         return None
-    n = sum(p["dispatches"] for p in parts)
+    # per unit of work where the file says how many units the dispatches
+    # served (PageRank: iterations), else per dispatch
+    n = sum(p.get("units", p["dispatches"]) for p in parts)
     return round(sum(p["hbm_bytes_per_dispatch"] * p["dispatches"] for p in parts) / max(n, 1))
 
 
@@ -488,8 +491,8 @@ def bench_power(args, steps):
         rg.set_profiling(g, False)
         it_ms, it_n = prof["pr_iter"]
         gbs = bytes_iter[name] * inf["iterations"] / (ms / 1000.0) / 1e9
-        rl = roofline_entry(bytes_iter[name] * it_n, it_ms, it_n, "pr_iter_kernel+pr_finish_kernel" +
-                            ("+eig_norm_kernel+eig_decide_kernel" if name == "eigenvector" else ""),
+        rl = roofline_entry(bytes_iter[name] * it_n, it_ms, it_n, "pr_iter_kernel" +
+                            ("+pr_finish_kernel+eig_norm_kernel+eig_decide_kernel" if name == "eigenvector" else ""),
                             traffic_of("power", "pr_iter_kernel<double, 1>" if name != "eigenvector"
                                        else "pr_iter_kernel<double, 2>"))
         res[name] = {"value": round(gbs, 1), "unit": "GB/s", "iterations": int(inf["iterations"]),
@@ -742,7 +745,7 @@ def bench_graph(args, name, steps, warmup):
     _lib.check(L.zg_gather_floor(h, 32, 20, ctypes.byref(floor_us)), "zg_gather_floor")
     floor_lds_us = ctypes.c_double()
     _lib.check(L.zg_gather_floor(h, 32 | _lib.ZG_FLOOR_LDS_COPY, 20, ctypes.byref(floor_lds_us)), "zg_gather_floor")
-    pr_roof = roofline_entry(pr_bytes_iter * it_n, it_ms, it_n, "pr_iter_kernel+pr_finish_kernel",
+    pr_roof = roofline_entry(pr_bytes_iter * it_n, it_ms, it_n, "pr_iter_kernel",
                              traffic_of(name, "pr_iter_kernel<float, 0>" if prs["precision"] == 32
                                         else "pr_iter_kernel<float, 0>+pr_iter_kernel<double, 0>"))
     it_us = pr_roof["avg_launch_us"]

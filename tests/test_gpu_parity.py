@@ -573,3 +573,33 @@ def test_pagerank_block_shapes_and_lds_copy(precision):
     for a in outs[1:]:
         assert np.abs(a - outs[0]).sum() <= (1e-6 if precision == 32 else 1e-13)
     rg.destroy(g)
+
+
+def test_pagerank_long_rows_with_a_grid_smaller_than_the_plan():
+    # many rows past 2,048 predecessors, dealt to block slots at load; a
+    # tuned grid of 3 blocks (fewer than the slots) makes every block take
+    # slots b, b + 3, ... and then the row bins: still the oracle's ranks
+    from oracle import oracle as O
+    from zog_amd import _lib
+    rng = np.random.default_rng(9)
+    n = 40_000
+    heads = rng.choice(n, 40, replace=False)
+    src = [rng.integers(0, n, 120_000)]
+    dst = [rng.integers(0, n, 120_000)]
+    for i, h in enumerate(heads):
+        k = int(2_100 + 700 * i)
+        src.append(rng.integers(0, n, k))
+        dst.append(np.full(k, h))
+    src = np.concatenate(src).astype(np.uint32)
+    dst = np.concatenate(dst).astype(np.uint32)
+    ref, _, _ = O.Graph(n, src, dst).pagerank(0.85, 30, 0.0)
+    g = make(n, src, dst)
+    h = g.resource.handle
+    for grid, block in ((0, 0), (3, 1024), (3, 256), (1, 512)):
+        say("pagerank long rows", grid, block)
+        _lib.check(_lib.lib().zg_set_tuning(h, _lib.ZG_TUNE_PR_GRID, float(grid)), "zg_set_tuning")
+        _lib.check(_lib.lib().zg_set_tuning(h, _lib.ZG_TUNE_PR_BLOCK, float(block)), "zg_set_tuning")
+        for prec, bar in ((32, PR_L1), (64, 1e-12)):
+            got = rg.pagerank_array(g, 0.85, 30, 0.0, precision=prec)
+            assert np.abs(got - ref).sum() <= bar, (grid, block, prec)
+    rg.destroy(g)

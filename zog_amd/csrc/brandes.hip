@@ -778,10 +778,11 @@ struct BcFinishIn {
     double tau;
 };
 
-__device__ __forceinline__ BcFinishIn bc_finish_load(const BcDev &a, uint32_t g, uint32_t v, int lane, int d)
+__device__ __forceinline__ BcFinishIn bc_finish_load(const BcDev &a, uint32_t g, uint32_t v, int lane, int d,
+                                                     bool active)
 {
     BcFinishIn in;
-    in.sv = a.sig[ROW(g, v) + lane];
+    in.sv = active ? a.sig[ROW(g, v) + lane] : 1.0;  // only the active lanes' lines of the row
     in.k = a.emax[EMAX(d, g, lane)];
     in.srcl = a.src[g * 64 + lane];
     in.wl = a.wts ? a.wts[g * 64 + lane] : 1.0;
@@ -835,7 +836,7 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     if (e1 > e0) w_first = a.out_col[e0 + lane < e1 ? e0 + (uint32_t)lane : e1 - 1];
     const uint32_t hub = a.ohub_idx[v];
     const uint32_t first_ch = a.och_first[v];
-    const BcFinishIn fin = bc_finish_load(a, g, v, lane, d);
+    const BcFinishIn fin = bc_finish_load(a, g, v, lane, d, active);
     // A successor w is a BFS child of v in lane l iff l's level-(d+1)
     // frontier held w; the other lanes' entries of w's row are zero.  With
     // the level's lane table, only child lanes read; past the table, every
@@ -903,7 +904,7 @@ __global__ __launch_bounds__(256) void bc_hub_kernel(BcDev a, int d, GroupRange 
     const uint32_t v = a.hlist[idx];
     const uint64_t mask = a.hmask[idx];
     const bool active = (mask >> lane) & 1;
-    const BcFinishIn fin = bc_finish_load(a, g, v, lane, d);
+    const BcFinishIn fin = bc_finish_load(a, g, v, lane, d, active);
     double S = 0.0;
     if (active) {
         const double *p = &a.hubacc[((size_t)g * a.n_ohub + a.ohub_idx[v]) * 64 + lane];

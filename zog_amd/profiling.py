@@ -13,7 +13,8 @@
       rows, and `traffic` reads either form
 
   python -m zog_amd.profiling traffic --fetch F.csv --write W.csv \
-      --copy-fetch CF.csv --copy-write CW.csv --copy-bytes 1073741824 -o out.json
+      --copy-fetch CF.csv --copy-write CW.csv --copy-bytes 1073741824 \
+      [--units KERNEL=COUNT ...] -o out.json
       HBM bytes per dispatch of every kernel from one FETCH_SIZE and one
       WRITE_SIZE counter-collection run (KiB per dispatch), each scaled by
       what the same counter reads for a plain copy of known size (the copy
@@ -32,7 +33,8 @@ from collections import defaultdict
 
 
 def _short(name: str) -> str:
-    """Kernel name without its argument list or template arguments."""
+    """Kernel name without its argument list (template arguments kept)."""
+    name = name.replace("(anonymous namespace)::", "")
     base = name.split("(")[0].strip()
     if base.startswith("void "):
         base = base[5:]
@@ -81,7 +83,7 @@ def reduce_counters(path: str, out: str) -> None:
             w.writerow([k, c, repr(v), cnt[(k, c)]])
 
 
-def traffic(fetch, write, copy_fetch, copy_write, copy_bytes):
+def traffic(fetch, write, copy_fetch, copy_write, copy_bytes, units=None):
     ft, fc, _ = _counter_per_kernel(fetch, "FETCH_SIZE")
     wt, wc, _ = _counter_per_kernel(write, "WRITE_SIZE")
     cft, cfc, _ = _counter_per_kernel(copy_fetch, "FETCH_SIZE")
@@ -116,6 +118,14 @@ def traffic(fetch, write, copy_fetch, copy_write, copy_bytes):
         tot = out[a]["hbm_bytes_per_dispatch"] * out[a]["dispatches"] + \
             out[b]["hbm_bytes_per_dispatch"] * out[b]["dispatches"]
         out[a + "+" + b] = {"dispatches": n, "hbm_bytes_per_dispatch": tot / n}
+    # units of work a kernel's dispatches served in the profiled run, when
+    # that is not one per dispatch (a PageRank run launches its iterations,
+    # one closing launch and launches that return at once): the bytes per
+    # unit are the kernel's total over them
+    for k, u in (units or {}).items():
+        if k in out and u > 0:
+            out[k]["units"] = u
+            out[k]["hbm_bytes_per_unit"] = out[k]["hbm_bytes_per_dispatch"] * out[k]["dispatches"] / u
     return out
 
 
@@ -157,6 +167,8 @@ def main(argv=None):
     t.add_argument("--copy-fetch", required=True)
     t.add_argument("--copy-write", required=True)
     t.add_argument("--copy-bytes", type=int, default=1 << 30)
+    t.add_argument("--units", action="append", default=[],
+                   help="KERNEL=COUNT: units of work (iterations) the kernel's dispatches served")
     t.add_argument("-o", "--out", required=True)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
@@ -166,7 +178,11 @@ def main(argv=None):
     elif a.cmd == "reduce":
         reduce_counters(a.csv, a.out)
     else:
-        res = traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes)
+        units = {}
+        for spec in a.units:
+            k, _, c = spec.rpartition("=")
+            units[k] = int(c)
+        res = traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes, units)
         with open(a.out, "w") as fh:
             json.dump(res, fh, indent=1)
         json.dump(res, sys.stdout, indent=1)
