@@ -400,6 +400,13 @@ int zg_set_batch_groups(zg_graph *g, int groups);
  *   the neighbour records a level touches share lines; 0 keeps the ids.
  *   Rows keep their arc order: results are the same bit for bit. */
 #define ZG_TUNE_QUEUE_ORDER 11
+/*   ZG_TUNE_LANE_ORDER: 1 (default) runs the 64-lane engine on a copy of the
+ *   resource whose vertices are renumbered by decreasing degree within each
+ *   component (built at the engine's first use, kept with the handle; rows
+ *   keep their arc order), so the rows and lane masks the sweep gathers most
+ *   share lines; 0 runs it on the resource's own ids.  Scores agree to FP64
+ *   summation order (the source groups can differ). */
+#define ZG_TUNE_LANE_ORDER 13
 int zg_set_tuning(zg_graph *g, int param, double value);
 
 /* Source grouping for the batched sweep (no reference counterpart; the

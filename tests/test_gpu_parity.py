@@ -544,6 +544,68 @@ def test_batched_sweep_work_counters():
     rg.destroy(g)
 
 
+def _lane_order(g, on):
+    from zog_amd import _lib
+    _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_LANE_ORDER, 1.0 if on else 0.0),
+               "zg_set_tuning")
+
+
+def test_lane_view_matches_resource_ids():
+    # the 64-lane engine on the degree-ordered view and on the resource's own
+    # ids: BC against the oracle (tombstones, parallel arcs, a self-loop),
+    # per-source dependency totals and source weights, work counters, and
+    # the kernel profile read through the parent handle
+    from oracle import oracle as O
+    from zog_amd import pivots
+    rng = np.random.default_rng(404)
+    n, src, dst = gen.rmat(12, 1 << 15, 404)
+    src = np.concatenate([src, src[:300], np.array([3, 3], np.uint32)])
+    dst = np.concatenate([dst, dst[:300], np.array([3, 8], np.uint32)])
+    live = (rng.random(n) > 0.1).astype(np.uint8)
+    og = O.Graph(n, src, dst, live)
+    sources = np.flatnonzero(live)[::5].astype(np.uint32)
+    ref, _, arcs = og.betweenness(sources=sources, return_arcs=True)
+    w = rng.random(len(sources)) + 0.5
+    out = {}
+    for on in (True, False):
+        g = make(n, src, dst, live, groups=4)
+        _lane_order(g, on)
+        rg.set_profiling(g, True)
+        info = {}
+        got = rg.betweenness_unweighted_array(g, sources=sources, info=info)
+        prof = rg.get_profile(g)
+        assert_bc_close(got, ref)
+        assert info["arcs_traversed"] == int(arcs.sum())
+        assert sum(c for _, c in prof.values()) > 0
+        bc_w, tot = pivots.bc_weighted(g, sources, w, want_bc=True, want_totals=True)
+        out[on] = (got, bc_w, tot)
+        assert np.all(got[live == 0] == 0.0)
+        rg.destroy(g)
+    assert_bc_close(out[True][1], out[False][1])
+    np.testing.assert_allclose(out[True][2], out[False][2], rtol=1e-12)
+
+
+def test_lane_view_on_a_batch_of_graphs():
+    # component ranges survive the renumbering: every graph of a batched
+    # resource keeps the oracle's BC, with groups spanning several graphs
+    from oracle import oracle as O
+    graphs_ = [gen.barabasi_albert(n, 3, 900 + i) for i, n in enumerate([300, 40, 1200, 7, 650])]
+    res = rg.BatchResource(graphs_)
+    off = res.offsets
+    n = int(off[-1])
+    src = np.concatenate([g[1] + np.uint32(o) for g, o in zip(graphs_, off[:-1])])
+    dst = np.concatenate([g[2] + np.uint32(o) for g, o in zip(graphs_, off[:-1])])
+    og = O.Graph(n, src, dst)
+    g = rg.ResourceGraph(resource=res, builder=SoA.from_arrays("directed", n, src, dst))
+    scattered = np.random.default_rng(6).permutation(n)[:500].astype(np.uint32)
+    for sources in (None, scattered):
+        ref, _ = og.betweenness(sources=sources)
+        for on in (True, False):
+            _lane_order(g, on)
+            assert_bc_close(rg.betweenness_unweighted_array(g, sources=sources), ref)
+    res.release()
+
+
 @pytest.mark.parametrize("precision", [32, 64])
 def test_pagerank_block_shapes_and_lds_copy(precision):
     # every iteration-block shape, with and without the LDS copy of the

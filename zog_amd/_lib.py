@@ -44,6 +44,7 @@ ZG_TUNE_WEIGHTED_DELTA = 8
 ZG_TUNE_WEIGHTED_GRID = 9
 ZG_TUNE_COUNT_TREE_ARCS = 10
 ZG_TUNE_QUEUE_ORDER = 11
+ZG_TUNE_LANE_ORDER = 13
 # work tables of zg_graph_table
 (ZG_TABLE_ICH, ZG_TABLE_ICH_FIRST, ZG_TABLE_OCH, ZG_TABLE_OCH_FIRST, ZG_TABLE_OHUB, ZG_TABLE_PR_ROWS,
  ZG_TABLE_PR_DESC, ZG_TABLE_PR_LARGE, ZG_TABLE_PR_SEG, ZG_TABLE_PR_COL, ZG_TABLE_PR_BINS) = range(11)

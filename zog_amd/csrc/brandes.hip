@@ -931,13 +931,17 @@ __global__ void bc_fold_source_kernel(BcDev a)
     }
 }
 
-// out[v] = sum over groups of the group accumulators (fixed order).
-__global__ void bc_reduce_kernel(uint32_t V, uint32_t G, const double *bcg, const uint8_t *live, double *out)
+// out[v] = sum over groups of the group accumulators (fixed order); with a
+// view (perm non-null) the accumulators are indexed by the view's ids and
+// perm[v] is v's.
+__global__ void bc_reduce_kernel(uint32_t V, uint32_t G, const double *bcg, const uint8_t *live, const uint32_t *perm,
+                                 double *out)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
+    const uint32_t x = perm ? perm[v] : v;
     double s = 0.0;
-    for (uint32_t g = 0; g < G; g++) s += bcg[(size_t)g * V + v];
+    for (uint32_t g = 0; g < G; g++) s += bcg[(size_t)g * V + x];
     out[v] = live[v] ? s : 0.0;
 }
 
@@ -1800,8 +1804,7 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
     const uint64_t n = list.size();
     const bool order = g->source_order && n > 64;
     // engine: the per-source queue engine for high-diameter graphs
-    if (g->engine == ZG_ENGINE_AUTO || order) ZG_TRY_RC(bc_core_distances(g, s));
-    if (order && g->source_order == 2) ZG_TRY_RC(bc_bfs_rank(g, s));
+    if (g->engine == ZG_ENGINE_AUTO) ZG_TRY_RC(bc_core_distances(g, s));
     // the folded sweep runs on the 64-lane engine
     const bool queue_engine = !call.fold &&
                               (g->engine == ZG_ENGINE_QUEUE || (g->engine == ZG_ENGINE_AUTO && g->high_diameter));
@@ -1810,9 +1813,28 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
         for (uint32_t x : list)
             if (g->h_fold_succ1[x] != ZG_NONE) return ZG_ERR_INVALID_ARG;  // counted with its successor
     }
+    // the lane engine runs on the degree-ordered view when there is one; the
+    // sources are then ordered and grouped by the view's own searches
+    zg_graph *eg = g;
+    if (!queue_engine && !call.fold && g->lane_order) {
+        ZG_TRY_RC(zg_lane_view(g, s));
+        if (g->lane) {
+            eg = g->lane;
+            eg->groups = g->groups;
+            eg->push_alpha = g->push_alpha;
+            eg->pull_grid = g->pull_grid;
+            eg->use_graph = g->use_graph;
+            eg->count_tree = g->count_tree;
+            eg->profiling = g->profiling;
+            eg->source_order = g->source_order;
+            for (uint32_t &x : list) x = g->h_lane_perm[x];
+        }
+    }
+    if (order) ZG_TRY_RC(bc_core_distances(eg, s));
+    if (order && g->source_order == 2) ZG_TRY_RC(bc_bfs_rank(eg, s));
     std::vector<uint64_t> perm;
     if (order) {
-        bc_order_perm(g, list, perm);
+        bc_order_perm(eg, list, perm);
         std::vector<uint32_t> sorted(n);
         for (uint64_t i = 0; i < n; i++) sorted[i] = list[perm[i]];
         list.swap(sorted);
@@ -1822,7 +1844,7 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
     std::vector<uint64_t> slot;
     if (!queue_engine && g->h_comp_off.size() > 2) {
         std::vector<uint32_t> padded;
-        bc_pad_groups(g, list, padded, slot);
+        bc_pad_groups(eg, list, padded, slot);
         list.swap(padded);
     }
     const uint64_t np = list.size();
@@ -1857,16 +1879,18 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
         if (rc == ZG_OK)
             rc = zg_bcq_sweep(g, d_list, n, d_w, call.h_totals ? tsorted.data() : nullptr, d_out, s, &inf);
     } else {
-        if (rc == ZG_OK) rc = bc_ws_ensure(g, false, list.data(), np);
+        if (rc == ZG_OK) rc = bc_ws_ensure(eg, false, list.data(), np);
         BcSweepOpt opt;
         opt.d_weights = d_w;
         opt.h_totals = call.h_totals ? tsorted.data() : nullptr;
         opt.fold = call.fold;
-        if (rc == ZG_OK) rc = bc_sweep(g, d_list, list.data(), np, s, &inf, opt);
+        if (rc == ZG_OK) rc = bc_sweep(eg, d_list, list.data(), np, s, &inf, opt);
         inf.n_sources = n;
         if (rc == ZG_OK && V) {
-            ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reduce_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V,
-                           (uint32_t)g->bc.groups, g->bc.bcg, g->live, d_out);
+            const size_t zp = zg_prof_begin(eg, s);
+            hipLaunchKernelGGL(bc_reduce_kernel, dim3(cdiv(V, 256)), dim3(256), 0, s, V, (uint32_t)eg->bc.groups,
+                               eg->bc.bcg, g->live, eg != g ? g->lane_perm : nullptr, d_out);
+            zg_prof_end(eg, s, ZG_PROF_BC_SETUP, zp);
             rc = zg_hip_code(hipGetLastError());
         }
     }
@@ -1881,6 +1905,15 @@ static int zg_bc_device_impl(zg_graph *g, const BcCall &call, hipStream_t s, zg_
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(s);
     if (rc == ZG_OK) rc = zg_prof_collect(g, s);
+    if (eg != g) {
+        const int prc = zg_prof_collect(eg, s);
+        if (rc == ZG_OK) rc = prc;
+        for (int k = 0; k < ZG_PROF_SLOTS; k++) {
+            g->prof.ms[k] += eg->prof.ms[k];
+            g->prof.launches[k] += eg->prof.launches[k];
+        }
+        memset(&eg->prof, 0, sizeof(eg->prof));
+    }
     zg_dfree(d_list);
     zg_dfree(d_w);
     zg_dfree(d_acc);

@@ -298,6 +298,14 @@ struct zg_graph {
     double wmean = 0.0;                  // mean weight of the live arcs
     uint32_t *und_ptr = nullptr, *und_col = nullptr;  // simple undirected CSR (core numbers), built on demand
     uint32_t pull_grid = 4096;
+    // the 64-lane engine's view (ZG_TUNE_LANE_ORDER; built on first use): a
+    // copy of this resource with every component's vertices renumbered by
+    // decreasing degree, rows in their arc order; lane_perm[v] = the view's
+    // id of v
+    bool lane_order = true;
+    zg_graph *lane = nullptr;
+    uint32_t *lane_perm = nullptr;       // device [V]
+    std::vector<uint32_t> h_lane_perm;
     uint32_t pr_xcache = ZG_PR_XCACHE_DEFAULT;  // PageRank values staged in LDS (0 = none)
     bool pr_xcache_useful = false;       // the cached values carry a useful share of the gathers
     uint32_t pr_grid_cap = 0;            // PageRank iteration blocks (0 = default)
@@ -337,7 +345,12 @@ void zg_prof_release(zg_graph *g);
         zg_prof_end(g, s, (slot), _zp);              \
     } while (0)
 
-int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m);
+int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
+                     bool device_arcs = false);
+// The 64-lane engine's degree-ordered view of g (g->lane, g->lane_perm,
+// g->h_lane_perm), built once; leaves g->lane null when ZG_TUNE_LANE_ORDER
+// is off or the device has no room for it.
+int zg_lane_view(zg_graph *g, hipStream_t s);
 // Chunk lists, hub slots and PageRank row bins from the device CSR and
 // PageRank ids (graph_derive.hip).
 int zg_derive_tables(zg_graph *g);

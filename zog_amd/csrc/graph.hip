@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <new>
 
+#include "build_util.h"
 #include "csr_host.h"
 #include "pr_bins.h"
 
@@ -128,6 +129,12 @@ int zg_io_src(zg_graph *g, uint64_t n, uint32_t **src)
 
 static void zg_free_device(zg_graph *g)
 {
+    if (g->lane) {
+        zg_free_device(g->lane);
+        delete g->lane;
+        g->lane = nullptr;
+    }
+    zg_dfree(g->lane_perm);
     zg_prof_release(g);
     zg_dfree(g->io_out);
     zg_dfree(g->io_src);
@@ -299,6 +306,128 @@ extern "C" int zg_graph_components(const zg_graph *g, uint32_t *n_components)
     return ZG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The 64-lane engine's view: the same graph with every component's vertices
+// renumbered by decreasing degree (in + out; ties by id).  The sweep gathers
+// path-count and dependency rows, frontier and level lane masks of the
+// vertices its arcs reach, most often those of the highest degrees; with them
+// at the low ids those rows and masks share lines and stay in L2 (R-MAT 20,
+// 4,096 sources: 177 -> 147 ms per step).  Components keep their id ranges,
+// and every successor row keeps its arc order.
+
+__global__ void lv_keys_kernel(uint32_t V, const uint32_t *out_ptr, const uint32_t *in_ptr, uint32_t *key,
+                               uint32_t *val)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        key[v] = ~((out_ptr[v + 1] - out_ptr[v]) + (in_ptr[v + 1] - in_ptr[v]));  // ascending: decreasing degree
+        val[v] = v;
+    }
+}
+
+// component index of the vertex at each position (co: ncomp + 1 offsets)
+__global__ void lv_comp_kernel(uint32_t V, const uint32_t *order, const uint32_t *co, uint32_t ncomp, uint32_t *key)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        const uint32_t v = order[i];
+        uint32_t lo = 0, hi = ncomp;  // co[lo] <= v < co[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (co[mid] <= v) lo = mid;
+            else hi = mid;
+        }
+        key[i] = lo;
+    }
+}
+
+__global__ void lv_perm_kernel(uint32_t V, const uint32_t *order, uint32_t *perm)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) perm[order[i]] = i;
+}
+
+// The view's arcs in the order that makes the build give every row its arc
+// order back: the build takes arcs newest (last) first, so arc i is the
+// successor CSR's arc E-1-i.
+__global__ void lv_arcs_kernel(uint32_t V, uint64_t E, const uint32_t *out_ptr, const uint32_t *out_col,
+                               const uint32_t *perm, uint32_t *from, uint32_t *to)
+{
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < V; u += gridDim.x * blockDim.x) {
+        const uint32_t pu = perm[u];
+        for (uint32_t e = out_ptr[u]; e < out_ptr[u + 1]; e++) {
+            const uint64_t i = E - 1 - e;
+            from[i] = pu;
+            to[i] = perm[out_col[e]];
+        }
+    }
+}
+
+int zg_lane_view(zg_graph *g, hipStream_t s)
+{
+    if (g->lane || !g->lane_order || g->V == 0 || g->E == 0) return ZG_OK;
+    const uint32_t V = g->V;
+    const uint64_t E = g->E;
+    const dim3 blk(256), grd((uint32_t)std::min<uint64_t>((V + 255) / 256, 8192));
+    int rc = ZG_OK;
+    {
+        ZgScratch sc;
+        uint32_t *key, *key2, *val, *order, *from, *to, *co = nullptr;
+        ZG_TRY_RC(sc.get(&key, V));
+        ZG_TRY_RC(sc.get(&key2, V));
+        ZG_TRY_RC(sc.get(&val, V));
+        ZG_TRY_RC(sc.get(&order, V));
+        ZG_TRY_RC(zg_dmalloc(&g->lane_perm, V));
+        hipLaunchKernelGGL(lv_keys_kernel, grd, blk, 0, s, V, g->out_ptr, g->in_ptr, key, val);
+        ZG_TRY(hipGetLastError());
+        ZG_TRY_RC(zg_sort_pairs(key, key2, val, order, V, 32, s, sc));
+        const uint32_t ncomp = (uint32_t)g->h_comp_off.size() - 1;
+        if (ncomp > 1) {
+            // stable by component: degree order within each component's range
+            ZG_TRY_RC(sc.get(&co, (size_t)ncomp + 1));
+            ZG_TRY(hipMemcpyAsync(co, g->h_comp_off.data(), sizeof(uint32_t) * (ncomp + 1), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(lv_comp_kernel, grd, blk, 0, s, V, order, co, ncomp, key);
+            ZG_TRY(hipGetLastError());
+            ZG_TRY_RC(zg_sort_pairs(key, key2, order, val, V, 32, s, sc));
+            std::swap(order, val);
+        }
+        hipLaunchKernelGGL(lv_perm_kernel, grd, blk, 0, s, V, order, g->lane_perm);
+        ZG_TRY_RC(sc.get(&from, E));
+        ZG_TRY_RC(sc.get(&to, E));
+        hipLaunchKernelGGL(lv_arcs_kernel, grd, blk, 0, s, V, E, g->out_ptr, g->out_col, g->lane_perm, from, to);
+        ZG_TRY(hipGetLastError());
+        g->h_lane_perm.resize(V);
+        ZG_TRY(hipMemcpyAsync(g->h_lane_perm.data(), g->lane_perm, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
+
+        zg_graph *c = new (std::nothrow) zg_graph();
+        if (!c) return ZG_ERR_OOM;
+        c->V = V;
+        c->device = g->device;
+        c->h_comp_off = g->h_comp_off;
+        c->h_live.assign(V, 0);
+        for (uint32_t v = 0; v < V; v++) c->h_live[g->h_lane_perm[v]] = g->h_live[v];
+        c->n_live = g->n_live;
+        rc = zg_hip_code(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        if (rc == ZG_OK) rc = zg_build_csr_gpu(c, from, to, nullptr, E, true);
+        c->built_on_gpu = true;
+        if (rc == ZG_OK) rc = zg_derive_tables(c);
+        if (rc == ZG_OK && c->E != E) rc = ZG_ERR_INVALID_ARG;  // every arc of the view is live
+        if (rc != ZG_OK) {
+            (void)hipStreamSynchronize(c->stream);
+            zg_free_device(c);
+            delete c;
+        } else {
+            g->lane = c;
+        }
+    }
+    if (rc == ZG_ERR_GPU_OOM) {
+        // no room for the view: the engine runs on the resource itself
+        zg_dfree(g->lane_perm);
+        g->h_lane_perm.clear();
+        g->lane_order = false;
+        return ZG_OK;
+    }
+    return rc;
+}
+
 extern "C" void zg_graph_destroy(zg_graph *g)
 {
     if (!g) return;
@@ -447,6 +576,10 @@ extern "C" int zg_set_tuning(zg_graph *g, int param, double value)
     case ZG_TUNE_PULL_GRID:
         if (!(value >= 0.0 && value <= 65536.0) || value != (double)(uint32_t)value) return ZG_ERR_INVALID_ARG;
         g->pull_grid = (uint32_t)value;
+        return ZG_OK;
+    case ZG_TUNE_LANE_ORDER:
+        if (value != 0.0 && value != 1.0) return ZG_ERR_INVALID_ARG;
+        g->lane_order = value != 0.0;
         return ZG_OK;
     case ZG_TUNE_PR_BLOCK:
         if (value != 0.0 && value != 256.0 && value != 512.0 && value != 1024.0) return ZG_ERR_INVALID_ARG;

@@ -144,7 +144,8 @@ int zg_sort_pairs(const uint32_t *kin, uint32_t *kout, const uint32_t *vin, uint
 
 // Fills the device arrays out_ptr, out_col, (out_w), in_ptr, in_col, pr_id,
 // pr_col, live and g->E, g->wmean, g->weights_ok.
-int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m)
+int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, const double *weight, uint64_t m,
+                     bool device_arcs)
 {
     const uint32_t V = g->V;
     hipStream_t s = g->stream;
@@ -155,11 +156,14 @@ int zg_build_csr_gpu(zg_graph *g, const uint32_t *from, const uint32_t *to, cons
     ZG_TRY_RC(sc.get(&d_from, m));
     ZG_TRY_RC(sc.get(&d_to, m));
     ZG_TRY_RC(sc.get(&d_bad, 2));
-    ZG_TRY(hipMemcpyAsync(d_from, from, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
-    ZG_TRY(hipMemcpyAsync(d_to, to, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+    const hipMemcpyKind kind = device_arcs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (m) {
+        ZG_TRY(hipMemcpyAsync(d_from, from, sizeof(uint32_t) * m, kind, s));
+        ZG_TRY(hipMemcpyAsync(d_to, to, sizeof(uint32_t) * m, kind, s));
+    }
     if (weight) {
         ZG_TRY_RC(sc.get(&d_w, m));
-        ZG_TRY(hipMemcpyAsync(d_w, weight, sizeof(double) * m, hipMemcpyHostToDevice, s));
+        if (m) ZG_TRY(hipMemcpyAsync(d_w, weight, sizeof(double) * m, kind, s));
     }
     ZG_TRY_RC(zg_dmalloc(&g->live, V));
     if (V) ZG_TRY(hipMemcpyAsync(g->live, g->h_live.data(), V, hipMemcpyHostToDevice, s));
