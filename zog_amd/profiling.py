@@ -4,7 +4,7 @@
       markdown table of the kernel statistics (per kernel: calls, total ms,
       average / min / max microseconds, share of device time)
 
-  python -m zog_amd.profiling dispatches <run_kernel_trace.csv> [--match s]
+  python -m zog_amd.profiling dispatches <run_kernel_trace.csv> [--match s] [--last N]
       per-dispatch duration percentiles of the kernels whose name has s
 
   python -m zog_amd.profiling reduce <run_counter_collection.csv> <out.csv>
@@ -129,24 +129,31 @@ def traffic(fetch, write, copy_fetch, copy_write, copy_bytes, units=None):
     return out
 
 
-def dispatch_markdown(path: str, match: str, small_us: float = 20.0) -> str:
+def dispatch_markdown(path: str, match: str, small_us: float = 20.0, last: int = 0) -> str:
     """Per-dispatch duration distribution of the kernels whose name contains
-    `match`, from a rocprofv3 kernel trace: dispatches, total ms, 10th /
+    `match`, from a rocprofv3 kernel trace: dispatches, total ms, mean, 10th /
     50th / 90th percentile and largest microseconds, and the dispatches
-    shorter than `small_us` with their total."""
+    shorter than `small_us` with their total.  With `last`, only each
+    kernel's last `last` dispatches (in start order) count."""
     durs = defaultdict(list)
     for r in csv.DictReader(open(path)):
         name = _short(r["Kernel_Name"])
         if match in name:
-            durs[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
-    out = ["| Kernel | Dispatches | Total ms | p10 us | p50 us | p90 us | Max us | Under %g us | Their ms |" % small_us,
-           "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
-    for name, v in sorted(durs.items(), key=lambda kv: -sum(kv[1])):
+            durs[name].append((int(r["Start_Timestamp"]),
+                               (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0))
+    out = ["| Kernel | Dispatches | Total ms | Mean us | p10 us | p50 us | p90 us | Max us | Under %g us | Their ms |"
+           % small_us, "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    rows = []
+    for name, tv in durs.items():
+        tv.sort()
+        v = [d for _, d in (tv[-last:] if last else tv)]
+        rows.append((name, v))
+    for name, v in sorted(rows, key=lambda kv: -sum(kv[1])):
         v = sorted(v)
         small = [x for x in v if x < small_us]
-        out.append("| `%s` | %d | %.1f | %.1f | %.1f | %.1f | %.1f | %d | %.1f |" % (
-            name, len(v), sum(v) / 1000.0, v[len(v) // 10], v[len(v) // 2], v[(9 * len(v)) // 10], v[-1],
-            len(small), sum(small) / 1000.0))
+        out.append("| `%s` | %d | %.1f | %.1f | %.1f | %.1f | %.1f | %.1f | %d | %.1f |" % (
+            name, len(v), sum(v) / 1000.0, sum(v) / len(v), v[len(v) // 10], v[len(v) // 2],
+            v[(9 * len(v)) // 10], v[-1], len(small), sum(small) / 1000.0))
     return "\n".join(out)
 
 
@@ -158,6 +165,7 @@ def main(argv=None):
     d = sub.add_parser("dispatches")
     d.add_argument("csv")
     d.add_argument("--match", default="")
+    d.add_argument("--last", type=int, default=0, help="only each kernel's last N dispatches")
     r = sub.add_parser("reduce")
     r.add_argument("csv")
     r.add_argument("out")
@@ -174,7 +182,7 @@ def main(argv=None):
     if a.cmd == "stats":
         print(kernel_stats_markdown(a.csv))
     elif a.cmd == "dispatches":
-        print(dispatch_markdown(a.csv, a.match))
+        print(dispatch_markdown(a.csv, a.match, last=a.last))
     elif a.cmd == "reduce":
         reduce_counters(a.csv, a.out)
     else:
