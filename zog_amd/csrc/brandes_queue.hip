@@ -173,10 +173,25 @@ __device__ __forceinline__ bool bcq_level_tables(const BcqDev &a, int d, uint32_
     // the block's first chunk is its wavefront 0's
     const uint32_t first = wk.c - (threadIdx.x >> 6);
     if (first >= wk.end) return false;
-    for (uint32_t s = threadIdx.x; s < a.S; s += blockDim.x) {
-        s_pre[s] = pre[s];
-        s_lo[s] = a.lstart[(size_t)d * a.S + s];
-        s_hi[s] = a.lstart[(size_t)(d + 1) * a.S + s];
+    // 256-thread blocks, at most ZG_Q_MAX_SOURCES sources: every load of the
+    // tables is issued before the first LDS write (indexes past the last
+    // source are clamped to it, so their writes repeat its values)
+    constexpr int U = (ZG_Q_MAX_SOURCES + 255) / 256;
+    const uint32_t *lo = a.lstart + (size_t)d * a.S, *hi = lo + a.S;
+    uint32_t tp[U], tl[U], th[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t s = min(threadIdx.x + 256u * u, a.S - 1);
+        tp[u] = pre[s];
+        tl[u] = lo[s];
+        th[u] = hi[s];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t s = min(threadIdx.x + 256u * u, a.S - 1);
+        s_pre[s] = tp[u];
+        s_lo[s] = tl[u];
+        s_hi[s] = th[u];
     }
     if (threadIdx.x == 0) s_pre[a.S] = total;
     __syncthreads();

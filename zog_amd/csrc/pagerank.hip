@@ -264,35 +264,77 @@ __device__ __forceinline__ void pr_unit(const PrArgs<T> &a, uint32_t b, int tid,
 // bins' work units.  All of a block's threads share its LDS copy of the
 // a.xcache most-gathered values (the lowest PageRank ids), loaded once per
 // launch.
+// dst[0, K) (LDS, 16-byte aligned) = src[0, K), with ZG_LDS_FILL loads in
+// flight per thread before the first LDS write: 16-byte loads when src is
+// 16-byte aligned (the rank vectors of a V-sized layout need not be), element
+// loads otherwise.  Indexes past the end are clamped to the last entry, for
+// the loads and the writes alike (such a write stores the value the last
+// entry's own write stores), so no load is issued under a branch.
+#ifndef ZG_LDS_FILL
+#define ZG_LDS_FILL 8
+#endif
+template <typename T, int NT>
+__device__ __forceinline__ void zg_lds_fill(T *dst, const T *src, uint32_t K)
+{
+    constexpr int U = ZG_LDS_FILL;
+    if (K == 0) return;
+    constexpr uint32_t per = 16 / sizeof(T);
+    const uint32_t n16 = (reinterpret_cast<uintptr_t>(src) & 15) == 0 ? K / per : 0;
+    if (n16) {
+        const uint4 *s16 = reinterpret_cast<const uint4 *>(src);
+        uint4 *d16 = reinterpret_cast<uint4 *>(dst);
+        for (uint32_t i0 = threadIdx.x; i0 < n16; i0 += U * NT) {
+            uint4 t[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) t[u] = s16[min(i0 + (uint32_t)u * NT, n16 - 1)];
+#pragma unroll
+            for (int u = 0; u < U; u++) d16[min(i0 + (uint32_t)u * NT, n16 - 1)] = t[u];
+        }
+    }
+    for (uint32_t i0 = n16 * per + threadIdx.x; i0 < K; i0 += U * NT) {
+        T t[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) t[u] = src[min(i0 + (uint32_t)u * NT, K - 1)];
+#pragma unroll
+        for (int u = 0; u < U; u++) dst[min(i0 + (uint32_t)u * NT, K - 1)] = t[u];
+    }
+}
+
 template <typename T, int MODE, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void pr_iter_kernel(PrArgs<T> a, unsigned long long it_arg)
 {
     static_assert(NT % ZG_PR_BLOCK == 0, "a block is a whole number of 256-thread workers");
     constexpr bool FUSED = MODE != ZG_PW_RAW;
     constexpr uint32_t NQ = NT / ZG_PR_BLOCK;
-    extern __shared__ double pr_lds[];  // [a.xcache] values of type T (launch-sized)
+    extern __shared__ uint4 pr_lds[];  // [a.xcache] values of type T (launch-sized)
     __shared__ double red[2][NT / 64];
     __shared__ T kred[2][NT / 64];
     __shared__ double tot[2];
     PrState *st = a.st;
     if (st->done) return;
     const unsigned long long it = FUSED ? it_arg : st->iter;
-    // the previous launch's block partials (this thread's share), then the
-    // LDS copy: both sets of loads in flight together
+    // the previous launch's block partials (this thread's first pair: the
+    // part buffer always holds both parities, so the load is unconditional),
+    // then the LDS copy: both sets of loads in flight together
     const bool add_up = FUSED && it > a.it0;
-    double L = 0.0, D = 0.0;
-    if (add_up) {
-        const double *pp = a.part + 2 * (size_t)gridDim.x * ((it - 1) & 1);
-        for (uint32_t q = threadIdx.x; q < gridDim.x; q += NT) {
-            L += pp[2 * q];
-            D += pp[2 * q + 1];
-        }
-    }
+    const double *pp = a.part + 2 * (size_t)gridDim.x * ((it - 1) & 1);
+    const double2 p0 = reinterpret_cast<const double2 *>(pp)[min((uint32_t)threadIdx.x, gridDim.x - 1)];
     const PrBufs<T, MODE> B(a, it);
     const T *xg = B.xg;
     T *xs = reinterpret_cast<T *>(pr_lds);
     const uint32_t K = a.xcache;
-    for (uint32_t i = threadIdx.x; i < K; i += NT) xs[i] = xg[i];
+    zg_lds_fill<T, NT>(xs, xg, K);
+    double L = 0.0, D = 0.0;
+    if (add_up) {
+        if (threadIdx.x < gridDim.x) {
+            L = p0.x;
+            D = p0.y;
+        }
+        for (uint32_t q = threadIdx.x + NT; q < gridDim.x; q += NT) {
+            L += pp[2 * q];
+            D += pp[2 * q + 1];
+        }
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const double damping = st->damping;
     double base = st->base;
@@ -1345,9 +1387,9 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) void pw_gather_lds_kernel(const uint32_t *__restrict__ col, const T *__restrict__ x,
                                                            uint32_t E, uint32_t K, T *sink)
 {
-    extern __shared__ double pw_lds[];
+    extern __shared__ uint4 pw_lds[];
     T *xs = reinterpret_cast<T *>(pw_lds);
-    for (uint32_t i = threadIdx.x; i < K; i += NT) xs[i] = x[i];
+    zg_lds_fill<T, NT>(xs, x, K);
     __syncthreads();
     constexpr int N = ZG_PR_INFLIGHT;
     const uint32_t step = gridDim.x * NT;
