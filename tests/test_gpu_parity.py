@@ -606,6 +606,51 @@ def test_lane_view_on_a_batch_of_graphs():
     res.release()
 
 
+def test_spans_end_at_the_last_vertex_with_an_arc():
+    # vertices without arcs after (and among) the others, some of them sources
+    # (the group's span then reaches past the last vertex with an arc), in
+    # one graph and in a batch of graphs, on the view and on the resource's
+    # own ids
+    from oracle import oracle as O
+    rng = np.random.default_rng(77)
+    n = 3000
+    _, src, dst = graphs.random_multigraph(rng, 1800, 9000)
+    src = (src * 3 // 2).astype(np.uint32)  # ids up to ~2700 with gaps; 2701.. have no arcs
+    dst = (dst * 3 // 2).astype(np.uint32)
+    og = O.Graph(n, src, dst)
+    sources = np.concatenate([np.arange(0, n, 7), [n - 1, n - 2, 1]]).astype(np.uint32)
+    ref, _ = og.betweenness(sources=sources)
+    for on in (True, False):
+        g = make(n, src, dst, groups=2)
+        _lane_order(g, on)
+        assert_bc_close(rg.betweenness_unweighted_array(g, sources=sources), ref)
+        assert_bc_close(rg.betweenness_unweighted_array(g), og.betweenness()[0])
+        rg.destroy(g)
+    graphs_ = []
+    for i, m in enumerate([40, 700, 3, 250]):
+        k, s2, d2 = gen.barabasi_albert(m + 4, 2, 40 + i)
+        graphs_.append((k + 5 + i, s2, d2))  # 5 + i vertices without arcs at each graph's end
+    res = rg.BatchResource(graphs_)
+    off = res.offsets
+    nb = int(off[-1])
+    bs = np.concatenate([gg[1] + np.uint32(o) for gg, o in zip(graphs_, off[:-1])])
+    bd = np.concatenate([gg[2] + np.uint32(o) for gg, o in zip(graphs_, off[:-1])])
+    ob = O.Graph(nb, bs, bd)
+    gb = rg.ResourceGraph(resource=res, builder=SoA.from_arrays("directed", nb, bs, bd))
+    tails = np.array([o - 1 for o in off[1:]], np.uint32)  # every graph's last vertex: no arcs
+    picks = np.concatenate([tails, np.random.default_rng(3).permutation(nb)[:150].astype(np.uint32)])
+    for sources in (None, picks):
+        refb, _ = ob.betweenness(sources=sources)
+        for on in (True, False):
+            _lane_order(gb, on)
+            assert_bc_close(rg.betweenness_unweighted_array(gb, sources=sources), refb)
+    depth, sigma, _, _ = rg.bc_paths(gb, picks[:8], max_levels=64)
+    for i, s0 in enumerate(picks[:8].tolist()):
+        d_ref, su, *_ = ob.bfs_paths(s0)
+        assert np.array_equal(depth[i], d_ref)
+    res.release()
+
+
 @pytest.mark.parametrize("precision", [32, 64])
 def test_pagerank_block_shapes_and_lds_copy(precision):
     # every iteration-block shape, with and without the LDS copy of the

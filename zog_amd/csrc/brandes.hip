@@ -992,19 +992,24 @@ int zg_bc_ws_release(zg_graph *g)
 
 // Vertex span of a group: the components its sources belong to (components
 // are contiguous id ranges; a single graph is one component).
+// The span ends at the last graph's last vertex with an arc, or after the
+// group's last source if that comes later: the vertices past it have no
+// arc, so no search reaches them and no row gathers them (the chunk, hub
+// and arc tables at that end equal those at the graph's end).
 static void bc_group_span(const zg_graph *g, const uint32_t *src, uint32_t n, uint32_t *vlo, uint32_t *vhi)
 {
     const std::vector<uint32_t> &co = g->h_comp_off;
-    if (co.size() <= 2 || n == 0) {
-        *vlo = 0;
-        *vhi = g->V;
-        return;
-    }
+    const std::vector<uint32_t> &act = g->h_comp_act;
     uint32_t lo = ZG_NONE, hi = 0;
     for (uint32_t l = 0; l < n; l++) {
         if (src[l] == ZG_NONE) continue;  // an empty slot
         lo = std::min(lo, src[l]);
         hi = std::max(hi, src[l]);
+    }
+    if (co.size() <= 2) {
+        *vlo = 0;
+        *vhi = lo == ZG_NONE || act.size() < 2 ? g->V : std::max(act[0], hi + 1);
+        return;
     }
     if (lo == ZG_NONE) {
         *vlo = *vhi = 0;
@@ -1013,7 +1018,7 @@ static void bc_group_span(const zg_graph *g, const uint32_t *src, uint32_t n, ui
     const size_t c_lo = std::upper_bound(co.begin(), co.end(), lo) - co.begin() - 1;
     const size_t c_hi = std::upper_bound(co.begin(), co.end(), hi) - co.begin() - 1;
     *vlo = co[c_lo];
-    *vhi = co[c_hi + 1];
+    *vhi = act.size() == co.size() ? std::max(act[c_hi], hi + 1) : co[c_hi + 1];
 }
 
 // Sizes the lane-engine state for the sweep over h_sources[0..n): each group

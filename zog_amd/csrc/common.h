@@ -261,6 +261,8 @@ struct zg_graph {
     // at each component offset h_comp_off[i]: predecessor CSR offset, first
     // predecessor chunk, first successor chunk, multi-chunk vertices before it
     std::vector<uint32_t> h_cb_in, h_cb_ich, h_cb_och, h_cb_hub;
+    // per graph c: 1 + its last vertex with an arc (its first vertex if none)
+    std::vector<uint32_t> h_comp_act;
     // PageRank row bins
     uint32_t n_small = 0, n_med = 0, n_large = 0, n_seg = 0;
     uint32_t n_m4 = 0, n_m16 = 0, n_m64 = 0;   // n_med = n_m4 + n_m16 + n_m64
@@ -357,8 +359,9 @@ int zg_derive_tables(zg_graph *g);
 // The long-row plan of the PageRank iteration (pagerank.hip) from the long
 // rows' first arcs and lengths, on the host (n_large entries).
 int zg_pr_long_plan(zg_graph *g, const uint32_t *h_e0, const uint32_t *h_len, hipStream_t s);
-// Index i of the component offset h_comp_off[i] == v (v is 0, V or a
-// component's first vertex).
+// Index i of the first component offset h_comp_off[i] >= v: v's own index
+// when v is 0, V or a component's first vertex, else the index of the end
+// of the component holding v.
 static inline size_t zg_comp_at(const zg_graph *g, uint32_t v)
 {
     return (size_t)(std::lower_bound(g->h_comp_off.begin(), g->h_comp_off.end(), v) - g->h_comp_off.begin());

@@ -110,6 +110,23 @@ __global__ void dv_bound_kernel(uint32_t nb, const uint32_t *co, const uint32_t 
     }
 }
 
+// act[c] = 1 + the last vertex of component c (offsets co[0..nb)) with an
+// arc in or out; act must start at the components' first vertices.
+__global__ void dv_act_kernel(uint32_t V, uint32_t nb, const uint32_t *co, const uint32_t *in_ptr,
+                              const uint32_t *out_ptr, uint32_t *act)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        if (in_ptr[v + 1] == in_ptr[v] && out_ptr[v + 1] == out_ptr[v]) continue;
+        uint32_t lo = 0, hi = nb - 1;  // co[lo] <= v < co[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (co[mid] <= v) lo = mid;
+            else hi = mid;
+        }
+        atomicMax(&act[lo], v + 1);
+    }
+}
+
 // PageRank order (vertices by PageRank id), how many ids follow their
 // predecessor vertex's, and the arcs out of the first K ids.
 __global__ void dv_order_kernel(uint32_t V, const uint32_t *pr_id, uint32_t *pr_order, unsigned long long *follow)
@@ -295,6 +312,16 @@ int zg_derive_tables(zg_graph *g)
         g->h_cb_ich.assign(b.begin() + nb, b.begin() + 2 * (size_t)nb);
         g->h_cb_och.assign(b.begin() + 2 * (size_t)nb, b.begin() + 3 * (size_t)nb);
         g->h_cb_hub.assign(b.begin() + 3 * (size_t)nb, b.end());
+        // each graph's end past its last vertex with an arc (vertices after it
+        // have none: no search reaches them unless they are its sources)
+        uint32_t *d_act;
+        ZG_TRY_RC(sc.get(&d_act, nb));
+        ZG_TRY(hipMemcpyAsync(d_act, co.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+        if (V && nb > 1) hipLaunchKernelGGL(dv_act_kernel, grid_for(V), blk, 0, s, V, nb, d_co, g->in_ptr, g->out_ptr, d_act);
+        ZG_TRY(hipGetLastError());
+        g->h_comp_act.resize(nb);
+        ZG_TRY(hipMemcpyAsync(g->h_comp_act.data(), d_act, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost, s));
+        ZG_TRY(hipStreamSynchronize(s));
     }
 
     // PageRank row bins
