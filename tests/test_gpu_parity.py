@@ -448,11 +448,11 @@ def test_pagerank_hub_rows_and_wiki_vote():
 
 
 def test_pagerank_every_row_bin():
-    # predecessor counts in every bin: 0, <=16, <=64, <=256, <=2048, one and
-    # several 4096-arc segments; parallel arcs and self-loops included
+    # predecessor counts in every bin and at its bounds: 0, <=4, <=32, <=256,
+    # <=2048, longer rows; parallel arcs and self-loops included
     rng = np.random.default_rng(12)
     n = 6000
-    degs = [0, 3, 16, 17, 64, 65, 200, 256, 257, 1500, 2048, 2049, 4096, 4097, 9000]
+    degs = [0, 3, 4, 5, 16, 17, 32, 33, 64, 65, 200, 256, 257, 1500, 2048, 2049, 4096, 4097, 9000]
     src, dst = [], []
     for i, dgr in enumerate(degs):
         src.append(rng.integers(0, n, dgr))

@@ -21,7 +21,7 @@ from zog_amd import resource_graph as rg
 
 pytestmark = pytest.mark.gpu
 
-CHUNK, SMALL, M4, M16, MED, SEG = 256, 4, 64, 256, 2048, 4096
+CHUNK, SMALL, M4, M16, MED, SEG = 256, 4, 32, 256, 2048, 4096
 NONE = 0xFFFFFFFF
 
 

@@ -6,7 +6,7 @@
 #define ZG_PR_SMALL 4      // <= 4 predecessors: one lane per row
 #endif
 #ifndef ZG_PR_M4
-#define ZG_PR_M4 64        // <= 64: 4 lanes per row
+#define ZG_PR_M4 32        // <= 32: 4 lanes per row
 #endif
 #ifndef ZG_PR_M16
 #define ZG_PR_M16 256      // <= 256: 16 lanes per row
