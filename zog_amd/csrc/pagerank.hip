@@ -129,7 +129,25 @@ __device__ __forceinline__ void pr_strided_kahan(const uint32_t *__restrict__ co
 #pragma unroll
         for (int k = 0; k < N; k++) kahan_add(s, c, v[k]);
     }
-    for (; e < e1; e += step) kahan_add(s, c, x[col[e]]);
+    if (e < e1) {
+        // the last (at most N - 1) terms: loads at clamped indexes, all in
+        // flight together, and each sum step kept only for a real term, so
+        // the terms add in the same order as one at a time
+        uint32_t ci[N - 1];
+        T v[N - 1];
+#pragma unroll
+        for (int k = 0; k < N - 1; k++) ci[k] = col[min(e + k * step, e1 - 1)];
+#pragma unroll
+        for (int k = 0; k < N - 1; k++) v[k] = x[ci[k]];
+#pragma unroll
+        for (int k = 0; k < N - 1; k++) {
+            T s2 = s, c2 = c;
+            kahan_add(s2, c2, v[k]);
+            const bool real = e + k * step < e1;
+            s = real ? s2 : s;
+            c = real ? c2 : c;
+        }
+    }
 }
 
 
