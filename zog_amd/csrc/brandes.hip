@@ -634,6 +634,12 @@ __global__ __launch_bounds__(ZG_FIN_THREADS) void bc_finalize_kernel_g(const BcL
 // Finalize level d after a push level: same as bc_finalize_kernel, over the
 // vertices the push reached first (a fixed grid striding over the list).
 #define ZG_SPARSE_FIN_BLOCKS 64
+#ifndef ZG_FIN_CHUNKS
+#define ZG_FIN_CHUNKS 4096        // predecessor chunks per dense-finalize workgroup (at most ZG_FIN_BLOCKS groups)
+#endif
+#ifndef ZG_SPARSE_FIN_VERTS
+#define ZG_SPARSE_FIN_VERTS 8192  // span vertices per sparse-finalize workgroup
+#endif
 __device__ __forceinline__ void bc_finalize_sparse_body(const BcDev &a, int d)
 {
     __shared__ uint32_t lds[40];
@@ -1298,8 +1304,9 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         const uint32_t span_c = use_graph ? w.cmax : max_c, span_v = use_graph ? w.vloc : max_v;
         uint32_t fwd_blocks = std::max<uint32_t>(1, cdiv(span_c, 4));
         if (g->pull_grid) fwd_blocks = std::min(fwd_blocks, g->pull_grid);
-        const uint32_t fin_blocks = std::min<uint32_t>(ZG_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(span_c, 4096)));
-        const uint32_t sfin_blocks = std::min<uint32_t>(ZG_SPARSE_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(span_v, 8192)));
+        const uint32_t fin_blocks = std::min<uint32_t>(ZG_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(span_c, ZG_FIN_CHUNKS)));
+        const uint32_t sfin_blocks =
+            std::min<uint32_t>(ZG_SPARSE_FIN_BLOCKS, std::max<uint32_t>(1, cdiv(span_v, ZG_SPARSE_FIN_VERTS)));
         const uint32_t push_blocks = std::min<uint32_t>(ZG_PUSH_GRID, std::max<uint32_t>(1, cdiv(span_v, 32)));
         if (use_graph) {
             const uint32_t dims[5] = {push_blocks, fwd_blocks, sfin_blocks, fin_blocks, Gcap};
