@@ -578,7 +578,7 @@ def apply_tuning(g, args):
         _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_PR_BLOCK, float(args.pr_block)),
                    "zg_set_tuning")
     if args.no_lane_order:
-        _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_LANE_ORDER, 0.0), "zg_set_tuning")
+        rg.set_lane_order(g, False)
     if args.pull_grid is not None:
         rg.set_pull_grid(g, args.pull_grid)
     if args.alpha is not None:

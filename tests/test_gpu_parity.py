@@ -545,9 +545,7 @@ def test_batched_sweep_work_counters():
 
 
 def _lane_order(g, on):
-    from zog_amd import _lib
-    _lib.check(_lib.lib().zg_set_tuning(g.resource.handle, _lib.ZG_TUNE_LANE_ORDER, 1.0 if on else 0.0),
-               "zg_set_tuning")
+    rg.set_lane_order(g, on)
 
 
 def test_lane_view_matches_resource_ids():

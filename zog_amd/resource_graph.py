@@ -444,6 +444,15 @@ def set_engine(graph: ResourceGraph, engine: str) -> None:
     _lib.check(_lib.lib().zg_set_tuning(graph.resource.handle, _lib.ZG_TUNE_ENGINE, float(code)), "zg_set_tuning")
 
 
+def set_lane_order(graph: ResourceGraph, on: bool = True) -> None:
+    """True (default): the 64-lane engine runs on a copy of the resource whose
+    vertices are renumbered by decreasing degree within each graph (built at
+    its first use); False: on the resource's own ids.  Scores come back in the
+    resource's ids either way."""
+    _lib.check(_lib.lib().zg_set_tuning(graph.resource.handle, _lib.ZG_TUNE_LANE_ORDER, 1.0 if on else 0.0),
+               "zg_set_tuning")
+
+
 def set_source_order(graph: ResourceGraph, mode: int) -> int:
     """0 = sweep sources in the given order, 1 = group them by distance to the
     hub core (default), 2 = by discovery rank in a search from the hub core.
