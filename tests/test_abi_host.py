@@ -23,6 +23,21 @@ def test_header_matches_binding_list():
     assert declared_symbols() == sorted(_lib.EXPORTED)
 
 
+def test_header_constants_match_the_python_mirror():
+    # every #define of the header that the Python binding names holds the
+    # same value there (error codes, precision flags, tuning knobs, tables)
+    text = open(HEADER).read()
+    defines = {}
+    for name, value in re.findall(r"^#define\s+(ZG_\w+)\s+\(?(-?(?:0x[0-9a-fA-F]+|\d+))\)?\s*(?:/\*.*)?$", text, re.M):
+        defines[name] = int(value, 0)
+    mirrored = {n: v for n, v in defines.items() if hasattr(_lib, n)}
+    assert len(mirrored) >= 30
+    for name, value in mirrored.items():
+        assert getattr(_lib, name) == value, name
+    for name in ("ZG_TUNE_LANE_ORDER", "ZG_TUNE_QUEUE_ORDER", "ZG_TUNE_PR_BLOCK", "ZG_ERR_SCRATCH"):
+        assert name in mirrored, name
+
+
 def test_library_loads_and_exports_every_symbol():
     L = _lib.lib()
     for name in declared_symbols():
