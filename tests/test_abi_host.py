@@ -36,6 +36,11 @@ def test_header_constants_match_the_python_mirror():
         assert getattr(_lib, name) == value, name
     for name in ("ZG_TUNE_LANE_ORDER", "ZG_TUNE_QUEUE_ORDER", "ZG_TUNE_PR_BLOCK", "ZG_ERR_SCRATCH"):
         assert name in mirrored, name
+    # profile slots: the binding's names by slot index
+    slots = {n[len("ZG_PROF_"):].lower(): v for n, v in defines.items() if n.startswith("ZG_PROF_") and n != "ZG_PROF_SLOTS"}
+    assert len(_lib.PROF_SLOTS) == defines["ZG_PROF_SLOTS"]
+    for name, value in slots.items():
+        assert _lib.PROF_SLOTS[value] == name, name
 
 
 def test_library_loads_and_exports_every_symbol():
