@@ -115,15 +115,29 @@ __global__ void dv_bound_kernel(uint32_t nb, const uint32_t *co, const uint32_t 
 __global__ void dv_act_kernel(uint32_t V, uint32_t nb, const uint32_t *co, const uint32_t *in_ptr,
                               const uint32_t *out_ptr, uint32_t *act)
 {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
-        if (in_ptr[v + 1] == in_ptr[v] && out_ptr[v + 1] == out_ptr[v]) continue;
-        uint32_t lo = 0, hi = nb - 1;  // co[lo] <= v < co[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (co[mid] <= v) lo = mid;
-            else hi = mid;
+    // a wavefront's lanes hold consecutive vertices, so they span one or two
+    // graphs: one atomic per graph and wavefront, from its highest lane
+    const int lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); v0 < V; v0 += stride) {
+        const uint32_t v = v0 + lane;
+        const bool has = v < V && (in_ptr[v + 1] != in_ptr[v] || out_ptr[v + 1] != out_ptr[v]);
+        uint32_t c = 0;
+        if (has) {
+            uint32_t lo = 0, hi = nb - 1;  // co[lo] <= v < co[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (co[mid] <= v) lo = mid;
+                else hi = mid;
+            }
+            c = lo;
         }
-        atomicMax(&act[lo], v + 1);
+        for (uint64_t todo = __ballot(has); todo;) {
+            const int j = 63 - __builtin_clzll(todo);
+            const uint32_t cj = zg_readlane32(c, j);
+            if (lane == 0) atomicMax(&act[cj], v0 + (uint32_t)j + 1);
+            todo &= ~__ballot(has && c == cj);
+        }
     }
 }
 
