@@ -73,6 +73,12 @@ struct BcDev {
     uint32_t *maxdep;           // [Gcap][V]
     uint64_t *lvl;              // [lmcap][Gcap][V] lanes whose level-L frontier held v
     uint32_t lmcap;
+    // 1: a lane finished at a level below lmcap keeps its (1 + delta) / sigma
+    // in place of its path count (nothing reads the count after that), and
+    // its parents, which read only child lanes there, find it in sig; 0 (the
+    // path-count export, which reads sig after the backward pass): every
+    // level's terms go to the cc parity buffers
+    uint32_t inplace;
     uint64_t *vis, *fr;         // fr: [3][Gcap][V]
     unsigned long long *stats;  // [Gcap][ZG_FIN_BLOCKS][ZG_BC_STATS]
     unsigned long long *tree;   // [Gcap][ZG_TREE_SLOTS] per-lane tree arcs seen by the backward pass, or null
@@ -155,6 +161,20 @@ __global__ void bc_reset_kernel(BcDev a, GroupSpan sp, uint32_t nlev)
     for (uint32_t L = 0; L < nlev; L++) a.lvl[LVLM(L, g, v)] = 0;
 }
 
+// Lane-table rows [from[g], to[g]) of group g (positions past this batch's
+// span that an earlier batch's larger span wrote) cleared for the first nlev
+// levels.
+struct LvlExtent {
+    uint32_t from[ZG_MAX_GROUPS], to[ZG_MAX_GROUPS];
+};
+__global__ void bc_lvl_clear_kernel(BcDev a, LvlExtent x, uint32_t nlev)
+{
+    const uint32_t g = blockIdx.y;
+    const uint32_t p = x.from[g] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= x.to[g]) return;
+    for (uint32_t L = 0; L < nlev; L++) a.lvl[(size_t)L * a.Gcap * a.Vloc + (size_t)g * a.Vloc + p] = 0;
+}
+
 // Path-count rows of a group's span start at zero (push levels and
 // multi-chunk vertices add into them); exported depths start unreached.
 __global__ void bc_zero_rows_kernel(BcDev a, GroupSpan sp)
@@ -181,8 +201,6 @@ __global__ void bc_source_kernel(BcDev a)
             atomicOr((unsigned long long *)&a.vis[MASK(g, s)], (unsigned long long)bit);
             atomicOr((unsigned long long *)&fr_buf(a, 0)[MASK(g, s)], (unsigned long long)bit);
             a.sig[ROW(g, s) + l] = 1.0;
-            cc_buf(a, 0)[ROW(g, s) + l] = 0.0;
-            cc_buf(a, 1)[ROW(g, s) + l] = 0.0;
             if (a.dep) a.dep[ROW(g, s) + l] = 0;
             a.maxdep[MASK(g, s)] = 0;
             uint32_t c0 = a.och_first[s], k = a.och_first[s + 1] - c0;
@@ -306,8 +324,6 @@ __device__ __forceinline__ void bc_pull_chunk(const BcDev &a, uint32_t g, int la
             const int ex = ilogb(val);
             if (ex > ZG_SHIFT_T) atomicMax(&a.emax[EMAX(d + 1, g, lane)], ex);
         }
-        cc_buf(a, 0)[r] = 0.0;
-        cc_buf(a, 1)[r] = 0.0;
         if (a.dep) a.dep[r] = (uint32_t)(d + 1);
     }
     if (lane == 0) {
@@ -386,8 +402,6 @@ __device__ __forceinline__ void bc_push_body(const BcDev &a, int d)
             if ((mj >> lane) & 1) {
                 const size_t r = ROW(g, wj) + lane;
                 atomicAdd(&a.sig[r], val);
-                cc_buf(a, 0)[r] = 0.0;
-                cc_buf(a, 1)[r] = 0.0;
                 if (a.dep) a.dep[r] = (uint32_t)(d + 1);
             }
             unsigned long long old = 0;
@@ -813,7 +827,9 @@ __device__ __forceinline__ void bc_finish(const BcDev &a, uint32_t g, uint32_t v
     if (active) {
         const size_t r = ROW(g, v) + lane;
         const double delta = in.sv * ldexp(S, -in.k);
-        cc_buf(a, d & 1)[r] = (in.tau + delta) / in.sv;
+        const double cv = (in.tau + delta) / in.sv;
+        if (a.inplace && (uint32_t)d < a.lmcap) a.sig[r] = cv;
+        else cc_buf(a, d & 1)[r] = cv;
         if (in.srcl != v) contrib = a.wts ? delta * in.wl : delta;
     }
     contrib = zg_wave_sum(contrib);
@@ -836,7 +852,11 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     const uint64_t mask = a.fmask[idx];
     const uint32_t v = a.och_v[oc], e0 = a.och_e0[oc], e1 = a.och_e1[oc];
     const bool active = (mask >> lane) & 1;
-    const double *cg = cc_buf(a, (d + 1) & 1);
+    // children finished at level d+1: in place below the table's end, else
+    // in the parity buffer (zero in every lane not written at a level of
+    // that parity: cleared before the backward pass of a batch that uses it)
+    const bool exact_rows = a.inplace && (uint32_t)(d + 1) < a.lmcap;
+    const double *cg = exact_rows ? a.sig : cc_buf(a, (d + 1) & 1);
     // (a sink's chunk is empty: no successor is read)
     uint32_t w_first = 0;
     if (e1 > e0) w_first = a.out_col[e0 + lane < e1 ? e0 + (uint32_t)lane : e1 - 1];
@@ -1068,6 +1088,9 @@ static int bc_ws_ensure(zg_graph *g, bool with_depth, const uint32_t *h_sources,
     w.lmcap = lvl_row ? (uint32_t)std::min<size_t>(ZG_LVL_LEVELS, ZG_LVL_BYTES / lvl_row) : 0;
     if (w.lmcap < 3) w.lmcap = 0;
     if (rc == ZG_OK && w.lmcap && zg_dmalloc(&w.lvl, (size_t)w.lmcap * G * VL) != ZG_OK) w.lmcap = 0;
+    // the table starts empty, so a lane bit read there is always a child lane
+    if (rc == ZG_OK && w.lmcap) rc = zg_hip_code(hipMemset(w.lvl, 0, sizeof(uint64_t) * (size_t)w.lmcap * G * VL));
+    for (int q = 0; q < ZG_MAX_GROUPS; q++) w.lvl_span[q] = 0;
     if (rc == ZG_OK) rc = zg_dmalloc(&w.vis, G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.fr, 3 * G * VL);
     if (rc == ZG_OK) rc = zg_dmalloc(&w.stats, G * ZG_FIN_BLOCKS * ZG_BC_STATS);
@@ -1210,6 +1233,7 @@ static BcDev bc_dev(zg_graph *g, uint32_t groups_used, const GroupSpan &sp, cons
     a.live = g->live;
     a.sig = w.sig; a.cc = w.cc; a.hubacc = w.hubacc; a.bcg = w.bcg; a.dep = w.dep; a.maxdep = w.maxdep;
     a.lvl = w.lvl; a.lmcap = w.lvl ? w.lmcap : 0;
+    a.inplace = opt.hook ? 0u : 1u;
     a.vis = w.vis; a.fr = w.fr; a.stats = w.stats; a.touched = w.touched;
     a.tree = g->count_tree ? w.tree : nullptr;
     a.flist = w.flist; a.fmask = w.fmask; a.hlist = w.hlist; a.hmask = w.hmask;
@@ -1286,6 +1310,22 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         ZG_TRY(hipMemsetAsync(w.cnt, 0, sizeof(zg_bc_ws::Counters) * Gcap, s));
         ZG_TRY(hipMemsetAsync(w.emax, 0, sizeof(int) * (size_t)std::min(w.last_levels + 2, w.lcap) * Gcap * 64, s));
         const uint32_t nlev = a.lmcap ? std::min(w.last_levels + 2, a.lmcap) : 0;
+        if (nlev) {
+            // rows an earlier, larger span left bits in (the reset below covers
+            // this batch's spans)
+            LvlExtent ext;
+            uint32_t most = 0;
+            for (uint32_t q = 0; q < Gcap; q++) {
+                const uint32_t cur = q < G ? sp.vhi[q] - sp.vlo[q] : 0u;
+                ext.from[q] = q < G ? cur : 0u;
+                ext.to[q] = q < G ? std::max(cur, w.lvl_span[q]) : 0u;
+                most = std::max(most, ext.to[q] - ext.from[q]);
+                if (q < G) w.lvl_span[q] = cur;
+            }
+            if (most)
+                ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_lvl_clear_kernel, dim3(cdiv(most, 256), Gcap), dim3(256), 0, s, a,
+                               ext, nlev);
+        }
         if (max_v) {
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_reset_kernel, dim3(cdiv(max_v, 256), G), dim3(256), 0, s, a, sp, nlev);
             ZG_PROF_LAUNCH(ZG_PROF_BC_SETUP, bc_zero_rows_kernel, dim3(cdiv(max_v, 4), G), dim3(256), 0, s, a, sp);
@@ -1388,6 +1428,11 @@ static int bc_sweep(zg_graph *g, const uint32_t *d_sources, const uint32_t *h_so
         }
         inf->max_levels = std::max<uint32_t>(inf->max_levels, (uint32_t)maxd + 1);
         w.last_levels = (uint32_t)maxd + 2;
+        // the parity buffers, read where a level's children lie past the lane
+        // table (or for every level when the terms do not stay in place), start
+        // at zero: a lane is read there only as a child or at zero
+        if (!a.inplace || (uint32_t)maxd + 1 >= a.lmcap)
+            ZG_TRY(hipMemsetAsync(w.cc, 0, sizeof(double) * 2 * (size_t)Gcap * w.vloc * 64, s));
         for (int L = maxd; L >= 1; L--) {
             GroupRange R, H;
             memset(&R, 0, sizeof(R));

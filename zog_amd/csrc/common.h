@@ -136,6 +136,9 @@ struct zg_bc_ws {
     Counters *h_cnt = nullptr;           // pinned host mirror
     size_t fcap = 0, hcap = 0;
     uint32_t last_levels = 0;            // levels used by the previous batch
+    // per group: lane-table rows [0, lvl_span[g]) may hold bits (of levels
+    // below last_levels); every other row of the table is zero
+    uint32_t lvl_span[ZG_MAX_GROUPS] = {};
 };
 
 // Per-(source, vertex) record of the queue engine: depth, path-count
