@@ -25,13 +25,16 @@
 // Backward (level d, from the deepest level up): for an entry (v, lanes at
 // depth d), delta_l(v) = sigma_l(v) * sum_{w in succ(v)} c_l(w), where
 // c_l(w) = (1 + delta_l(w)) / sigma_l(w) is published when w's level is done.
-// c is double-buffered by level parity: level d reads buffer (d+1)&1 and
-// writes buffer d&1, and both buffers are cleared for a lane when the lane
-// discovers w.  An arc v->w out of a reached v has depth(w) <= depth(v) + 1,
-// so in buffer (d+1)&1 the only lanes of w already written are those with
-// depth(w) = d+1: c_l(w) != 0 exactly when w is a BFS child of v for source
-// l, with no depth compare, and a same-level neighbour's fresh value (written
-// to the other buffer) is never seen.  No atomics for ordinary vertices; the
+// Within the levels the lane table covers, c_l(w) is written over sigma_l(w)
+// in place (sigma_l(w) is not read again once w's level is done): a parent
+// at level d reads it only for the lanes the table puts at level d+1, so a
+// same-level neighbour's fresh value is never seen.  Past the table (and
+// when the counts are exported through a hook), c is double-buffered by
+// level parity: level d reads buffer (d+1)&1 and writes buffer d&1, both
+// cleared before the backward pass.  An arc v->w out of a reached v has
+// depth(w) <= depth(v) + 1, so in buffer (d+1)&1 the only lanes of w already
+// written are those with depth(w) = d+1: c_l(w) != 0 exactly when w is a BFS
+// child of v for source l, with no depth compare.  No atomics for ordinary vertices; the
 // per-lane bc contributions are summed across the wavefront before one
 // read-modify-write per (vertex, level).
 //
@@ -864,10 +867,12 @@ __global__ __launch_bounds__(256) void bc_backward_kernel(BcDev a, int d, GroupR
     const uint32_t first_ch = a.och_first[v];
     const BcFinishIn fin = bc_finish_load(a, g, v, lane, d, active);
     // A successor w is a BFS child of v in lane l iff l's level-(d+1)
-    // frontier held w; the other lanes' entries of w's row are zero.  With
-    // the level's lane table, only child lanes read; past the table, every
-    // active lane reads the rows of successors that some lane reached at
-    // depth >= d+1.
+    // frontier held w.  With the level's lane table, only child lanes read,
+    // and they read c from w's sigma row, where it replaced sigma (when the
+    // terms are kept in place).  Past the table, every active lane reads
+    // the parity buffer row of successors that some lane reached at depth
+    // >= d+1; those buffers were cleared before the backward pass, so a
+    // lane for which w is no child reads zero.
     const bool exact = (uint32_t)(d + 1) < a.lmcap;
     double S = 0.0;
     uint32_t tree = 0;  // this lane's children read (counted only when a.tree is set)
